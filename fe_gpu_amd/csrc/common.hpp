@@ -1,0 +1,74 @@
+// Shared plumbing for the gfx950 fixed-effects library: error state, launch shapes,
+// a grow-only device workspace and small wavefront reduction helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+
+namespace fehip {
+
+// ---- error state: one thread-local message, int status (0 = success) ----
+int set_ok();
+int set_error(const char* fmt, ...);
+int set_hip_error(hipError_t err, const char* context);
+const char* last_error();
+
+#define FEHIP_CHECK(expr, ctx)                                   \
+    do {                                                         \
+        hipError_t _e = (expr);                                  \
+        if (_e != hipSuccess) return ::fehip::set_hip_error(_e, ctx); \
+    } while (0)
+
+#define FEHIP_LAUNCH_CHECK(ctx)                                  \
+    do {                                                         \
+        hipError_t _e = hipGetLastError();                       \
+        if (_e != hipSuccess) return ::fehip::set_hip_error(_e, ctx); \
+    } while (0)
+
+// ---- device facts ----
+constexpr int kWave = 64;              // CDNA wavefront width
+int num_cus();                         // compute units of the current device (256 on MI355X)
+
+// Grid for a grid-stride streaming kernel: enough workgroups to fill every CU several
+// times over, never more than the work needs.
+inline unsigned stream_grid(size_t work_items, unsigned block, unsigned waves_per_cu = 8) {
+    size_t need = (work_items + block - 1) / block;
+    size_t fill = (size_t)num_cus() * waves_per_cu * kWave / block;
+    if (fill < 1) fill = 1;
+    size_t g = need < fill ? need : fill;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// ---- grow-only workspace on the current device (library-internal temporaries) ----
+// Returns a device buffer of at least `bytes`; the pointer stays valid until the next call
+// that grows it.  All users run on the default stream, so stream order protects reuse.
+void* workspace(size_t bytes, int slot = 0);
+void release_workspaces();
+
+// Pinned host scalar buffer for D2H of small results.
+double* host_scalars();
+
+// FP64 add into LDS through an explicit LDS-address-space pointer (ds_add_f64); a flat
+// pointer would make the backend emit a shared-aperture test.
+__device__ __forceinline__ void lds_add(double* p, double v) {
+    auto* lp = (__attribute__((address_space(3))) double*)(p);
+    __hip_atomic_fetch_add(lp, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// ---- wavefront helpers ----
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));
+    return v;
+}
+
+}  // namespace fehip

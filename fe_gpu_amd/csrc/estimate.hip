@@ -1,0 +1,491 @@
+// fehip_estimate: the whole estimation core on a device-resident panel.
+//
+// Same conventions as the reference's GPU path, fe_gpu_estimate (src/core/fe_pipeline.f90:516-1896):
+//   demean y, W, Z together (:616)  ->  one Gram pass [y W Z]'[y W Z] (replaces the separate
+//   syrk / gemv / gemm calls of :647, :963-964) -> host selection of endogenous/exogenous/
+//   instrument columns (:664-753), IV normal equations (:947-1032), zero-variance column filter
+//   and Cholesky (:1034-1107) -> residual on W~ (:1168) -> homoskedastic or CGM sandwich over the
+//   2^m-1 cluster subsets with the nested small-sample scalar and PSD clip (:1109-1420) ->
+//   regression statistics (finalize_regression_stats).
+// Cluster scores use the panel's cached row partitions: S_c = sum_{i in c} r_i x_i is a
+// segmented sum over rows grouped by cluster (no atomics), and the meat S'S is a split-K Gram.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "dense.hpp"
+#include "panel.hpp"
+#include "within.hpp"
+
+namespace fehip {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kRedBlocks = 512;
+
+using clk = std::chrono::steady_clock;
+
+__global__ __launch_bounds__(kBlock) void wsum2_kernel(const double* __restrict__ y, const uint8_t* __restrict__ w,
+                                                       int64_t n, double shift, double* part) {
+    // part[b] = (sum w*(y-shift), sum w*(y-shift)^2, sum w)
+    __shared__ double ws[3][kBlock / kWave];
+    double a = 0.0, b = 0.0, c = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (w && !w[i]) continue;
+        const double v = y[i] - shift;
+        a += v;
+        b += v * v;
+        c += 1.0;
+    }
+    a = wave_sum(a);
+    b = wave_sum(b);
+    c = wave_sum(c);
+    const int wv = threadIdx.x / kWave;
+    if ((threadIdx.x & (kWave - 1)) == 0) {
+        ws[0][wv] = a;
+        ws[1][wv] = b;
+        ws[2][wv] = c;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) t += ws[threadIdx.x][k];
+        part[blockIdx.x * 3 + threadIdx.x] = t;
+    }
+}
+
+__global__ void residual_work_kernel(const double* __restrict__ work, int64_t n, int K,
+                                     const double* __restrict__ beta, double* r) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double s = work[i];
+        for (int k = 0; k < K; ++k) s -= work[i + (int64_t)(1 + k) * n] * beta[k];
+        r[i] = s;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void dot_part_kernel(const double* __restrict__ a, int64_t n, double* part) {
+    __shared__ double ws[kBlock / kWave];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        s += a[i] * a[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < kBlock / kWave; ++k) t += ws[k];
+        part[blockIdx.x] = t;
+    }
+}
+
+// T[i][j] = r_i X[i][j] (row-major n x K) from column-major X (ld)
+__global__ void scores_rows_kernel(const double* __restrict__ r, const double* __restrict__ X, int64_t ldX, int64_t n,
+                                   int K, double* T) {
+    const int64_t total = n * K;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = e / K;
+        const int j = (int)(e - i * K);
+        T[e] = r[i] * X[i + j * ldX];
+    }
+}
+
+// S[c][j] = sum of T rows of cluster c (rows order[off[c]..off[c+1]) or identity)
+__global__ void segsum_kernel(const double* __restrict__ T, int K, const int* __restrict__ order,
+                              const int64_t* __restrict__ off, int64_t nS, double* S) {
+    const int64_t total = nS * K;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / K;
+        const int j = (int)(e - c * K);
+        double s = 0.0;
+        for (int64_t k = off[c]; k < off[c + 1]; ++k) s += T[(int64_t)(order ? order[k] : k) * K + j];
+        S[e] = s;
+    }
+}
+
+__global__ void nonempty_kernel(const int* __restrict__ ids, const uint8_t* __restrict__ w, int64_t n, int* flags) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (!w || w[i]) flags[ids[i]] = 1;
+}
+
+__global__ __launch_bounds__(kBlock) void count_kernel(const int* __restrict__ f, int64_t n, unsigned long long* out) {
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        c += f[i] ? 1 : 0;
+    for (int off = kWave / 2; off > 0; off >>= 1) c += __shfl_xor(c, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(out, c);
+}
+
+__global__ void weights_panel_kernel(const uint8_t* __restrict__ w, const int* __restrict__ perm, int64_t n,
+                                     uint8_t* out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = w[perm ? perm[i] : i] ? 1 : 0;
+}
+
+struct DevBuf {
+    std::vector<void*> v;
+    ~DevBuf() {
+        for (void* p : v) (void)hipFree(p);
+    }
+    template <typename T>
+    int get(T** p, size_t count) {
+        int e = dalloc(p, count);
+        if (!e) v.push_back((void*)*p);
+        return e;
+    }
+};
+
+int host_sum_parts(const double* dpart, int nblk, int width, std::vector<double>& out) {
+    std::vector<double> h((size_t)nblk * width);
+    FEHIP_CHECK(hipMemcpy(h.data(), dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost), "partials");
+    out.assign(width, 0.0);
+    for (int b = 0; b < nblk; ++b)
+        for (int k = 0; k < width; ++k) out[k] += h[(size_t)b * width + k];
+    return 0;
+}
+
+int sumsq(const double* a, int64_t n, double* part, double* out) {
+    hipLaunchKernelGGL(dot_part_kernel, dim3(kRedBlocks), dim3(kBlock), 0, 0, a, n, part);
+    FEHIP_LAUNCH_CHECK("sumsq");
+    std::vector<double> s;
+    int e = host_sum_parts(part, kRedBlocks, 1, s);
+    if (e) return e;
+    *out = s[0];
+    return 0;
+}
+
+void finalize_stats(fehip_result* r, int64_t n_obs) {
+    const double EPS = 1e-12;
+    const double n = (double)std::max<int64_t>(1, n_obs);
+    const double dfm = (double)std::max(0, r->dof_model);
+    const double dfr = (double)std::max<int64_t>(1, r->dof_resid);
+    const double dev = r->rss;
+    const double dev0 = std::max(EPS, r->tss_total);
+    r->r2 = dev0 > EPS ? std::max(0.0, std::min(1.0, 1.0 - dev / dev0)) : 0.0;
+    r->r2_within = r->tss_within > EPS ? std::max(0.0, std::min(1.0, 1.0 - dev / r->tss_within)) : r->r2;
+    const double k_total = (double)(r->dof_model + r->dof_fes + 1);
+    const double denom = std::max(n - k_total, 1.0);
+    r->r2_adj = (n > 1.0 && dev0 > EPS) ? std::max(0.0, std::min(1.0, 1.0 - (dev * (n - 1.0)) / (dev0 * denom))) : r->r2;
+    const double ssr = std::max(0.0, dev0 - dev);
+    r->f_stat = (dfm > 0.0 && dfr > 0.0 && dev > EPS) ? (ssr / dfm) / (dev / dfr) : 0.0;
+}
+
+}  // namespace
+}  // namespace fehip
+
+using namespace fehip;
+
+extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const double* d_W, int K, const double* d_Z,
+                              int L, int64_t ld, const uint8_t* d_weights, const fehip_opts* opts,
+                              fehip_result* result, double* beta, double* se, double* vcov) {
+    if (!panel || !opts || !result || !d_y) return set_error("fehip_estimate: null argument");
+    Panel* p = reinterpret_cast<Panel*>(panel);
+    const int64_t n = p->n;
+    if (K < 0 || L < 0 || (K > 0 && !d_W) || (L > 0 && !d_Z) || ld < n) return set_error("fehip_estimate: bad columns");
+    std::memset(result, 0, sizeof(*result));
+    for (int k = 0; k < K; ++k) {
+        beta[k] = 0.0;
+        se[k] = 0.0;
+    }
+    if (vcov) std::fill(vcov, vcov + (size_t)K * K, 0.0);
+    DevBuf db;
+    int e;
+    const auto t0 = clk::now();
+    const unsigned fe_mask = (opts->demean.fe_mask ? opts->demean.fe_mask : ((1u << p->m) - 1u)) & ((1u << p->m) - 1u);
+    // cluster dims: valid, unique, first 3 in request order (initialize_cluster_dims)
+    int ncd = 0;
+    int cdims[3];
+    for (int i = 0; i < opts->n_cluster && i < 4; ++i) {
+        const int d = opts->cluster_dims[i];
+        if (d < 1 || d > p->m) continue;
+        bool dup = false;
+        for (int j = 0; j < ncd; ++j) dup |= cdims[j] == d;
+        if (dup) continue;
+        if (ncd == 3) break;
+        cdims[ncd++] = d;
+    }
+    result->n_cluster_dims = ncd;
+    for (int j = 0; j < ncd; ++j) result->cluster_dims[j] = cdims[j];
+
+    double* part = nullptr;
+    if ((e = db.get(&part, (size_t)kRedBlocks * 3))) return e;
+    // weights in panel order, effective N, TSS of y (compute_tss with intercept)
+    uint8_t* wpanel = nullptr;
+    if (d_weights) {
+        if ((e = db.get(&wpanel, (size_t)n))) return e;
+        hipLaunchKernelGGL(weights_panel_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, d_weights, p->perm, n,
+                           wpanel);
+    }
+    std::vector<double> s3;
+    hipLaunchKernelGGL(wsum2_kernel, dim3(kRedBlocks), dim3(kBlock), 0, 0, d_y, d_weights, n, 0.0, part);
+    if ((e = host_sum_parts(part, kRedBlocks, 3, s3))) return e;
+    const int64_t N = (int64_t)llround(s3[2]);
+    result->n_obs = N;
+    const double ybar = N > 0 ? s3[0] / (double)N : 0.0;
+    hipLaunchKernelGGL(wsum2_kernel, dim3(kRedBlocks), dim3(kBlock), 0, 0, d_y, d_weights, n, ybar, part);
+    if ((e = host_sum_parts(part, kRedBlocks, 3, s3))) return e;
+    result->tss_total = s3[1];
+    // group counts: G_d (max id) without weights, non-empty groups with weights
+    int Geff[FEHIP_MAX_FE];
+    for (int d = 0; d < p->m; ++d) {
+        Geff[d] = p->d[d].G;
+        if (wpanel) {
+            int* flags = nullptr;
+            unsigned long long* cnt = nullptr;
+            if ((e = db.get(&flags, (size_t)p->d[d].G)) || (e = db.get(&cnt, 1))) return e;
+            FEHIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * (size_t)p->d[d].G, 0), "memset");
+            FEHIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), 0), "memset");
+            hipLaunchKernelGGL(nonempty_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, p->d[d].ids, wpanel, n,
+                               flags);
+            hipLaunchKernelGGL(count_kernel, dim3(stream_grid(p->d[d].G, kBlock)), dim3(kBlock), 0, 0, flags,
+                               (int64_t)p->d[d].G, cnt);
+            unsigned long long h = 0;
+            FEHIP_CHECK(hipMemcpy(&h, cnt, sizeof(h), hipMemcpyDeviceToHost), "count");
+            Geff[d] = (int)h;
+        }
+    }
+    int dof_fes = 0, dof_nested = 0;
+    for (int d = 0; d < p->m; ++d)
+        if ((fe_mask >> d) & 1u) dof_fes += std::max(0, Geff[d] - 1);
+    for (int j = 0; j < ncd; ++j) dof_nested += std::max(0, Geff[cdims[j] - 1] - 1);
+    result->dof_fes = dof_fes;
+
+    // ---- within transform of [y W Z] into the work buffer (panel order)
+    const int C = 1 + K + L;
+    double* work = nullptr;
+    if ((e = db.get(&work, (size_t)n * C))) return e;
+    std::vector<const double*> xc(C);
+    std::vector<double*> oc(C);
+    xc[0] = d_y;
+    for (int k = 0; k < K; ++k) xc[1 + k] = d_W + (int64_t)k * ld;
+    for (int l = 0; l < L; ++l) xc[1 + K + l] = d_Z + (int64_t)l * ld;
+    for (int c = 0; c < C; ++c) oc[c] = work + (int64_t)c * n;
+    fehip_demean_opts dopt = opts->demean;
+    dopt.fe_mask = fe_mask;
+    WithinStats ws;
+    if ((e = within_transform(p, xc.data(), C, d_weights, dopt, oc.data(), 1, ws, 0))) return e;
+    result->converged = ws.converged;
+    result->fe_iterations = ws.iterations;
+    result->max_update = ws.max_update;
+    const auto t1 = clk::now();
+    result->t_demean = std::chrono::duration<double>(t1 - t0).count();
+    if (K <= 0) return set_ok();
+
+    // ---- Gram of [y W Z] (upper), host copy
+    double* Gd = nullptr;
+    if ((e = db.get(&Gd, (size_t)C * C))) return e;
+    if ((e = gemm_impl(true, false, C, C, n, 1.0, work, n, work, n, 0.0, Gd, C, true))) return e;
+    std::vector<double> G((size_t)C * C);
+    FEHIP_CHECK(hipMemcpy(G.data(), Gd, G.size() * sizeof(double), hipMemcpyDeviceToHost), "gram copy");
+    for (int j = 0; j < C; ++j)
+        for (int i = j + 1; i < C; ++i) G[(size_t)j * C + i] = G[(size_t)i * C + j];
+    auto g = [&](int i, int j) { return G[(size_t)j * C + i]; };
+    result->tss_within = g(0, 0);
+
+    // endogenous / exogenous / instruments (fe_pipeline.f90:664-753)
+    std::vector<int> is_endog(K, 0);
+    for (int i = 0; i < opts->n_iv_reg; ++i) {
+        const int v = opts->iv_reg[i];
+        if (v >= 1 && v <= K) is_endog[v - 1] = 1;
+    }
+    std::vector<int> exog, inst;
+    int n_endog = 0;
+    for (int k = 0; k < K; ++k) {
+        if (is_endog[k])
+            ++n_endog;
+        else
+            exog.push_back(k);
+    }
+    if (L > 0) {
+        for (int i = 0; i < opts->n_iv_inst; ++i)
+            if (opts->iv_inst[i] >= 1 && opts->iv_inst[i] <= L) inst.push_back(opts->iv_inst[i] - 1);
+        if (inst.empty())
+            for (int l = 0; l < L; ++l) inst.push_back(l);
+    }
+    bool use_iv = n_endog > 0 && (int)inst.size() >= n_endog;
+    const int n_aug = (int)exog.size() + (int)inst.size();
+    result->is_iv = use_iv;
+    result->n_instruments = use_iv ? n_aug : 0;
+    std::vector<double> Q((size_t)K * K), b(K);
+    std::vector<double> proj;  // n_aug x K
+    std::vector<int> aug_col;  // work column of each Z_aug column
+    if (use_iv) {
+        for (int k : exog) aug_col.push_back(1 + k);
+        for (int l : inst) aug_col.push_back(1 + K + l);
+        std::vector<double> Qzz((size_t)n_aug * n_aug), Qzy(n_aug), Qzx((size_t)n_aug * K);
+        for (int a = 0; a < n_aug; ++a) {
+            Qzy[a] = g(aug_col[a], 0);
+            for (int c = 0; c < n_aug; ++c) Qzz[(size_t)c * n_aug + a] = g(aug_col[a], aug_col[c]);
+            for (int k = 0; k < K; ++k) Qzx[(size_t)k * n_aug + a] = g(aug_col[a], 1 + k);
+        }
+        std::vector<double> rhs, Qzinv;
+        int info = dense::chol_solve_invert(Qzz, Qzy, n_aug, rhs, Qzinv);
+        if (info != 0) {
+            result->solver_info = info;
+            return set_ok();
+        }
+        proj.assign((size_t)n_aug * K, 0.0);
+        for (int k = 0; k < K; ++k)
+            for (int a = 0; a < n_aug; ++a) {
+                double s = 0.0;
+                for (int c = 0; c < n_aug; ++c) s += Qzinv[(size_t)c * n_aug + a] * Qzx[(size_t)k * n_aug + c];
+                proj[(size_t)k * n_aug + a] = s;
+            }
+        for (int i = 0; i < K; ++i)
+            for (int j = 0; j < K; ++j) {
+                double s = 0.0;
+                for (int a = 0; a < n_aug; ++a) s += Qzx[(size_t)i * n_aug + a] * proj[(size_t)j * n_aug + a];
+                Q[(size_t)j * K + i] = s;
+            }
+        for (int i = 0; i < K; ++i)
+            for (int j = i + 1; j < K; ++j) {
+                const double s = 0.5 * (Q[(size_t)j * K + i] + Q[(size_t)i * K + j]);
+                Q[(size_t)j * K + i] = Q[(size_t)i * K + j] = s;
+            }
+        for (int i = 0; i < K; ++i) {
+            double s = 0.0;
+            for (int a = 0; a < n_aug; ++a) s += Qzx[(size_t)i * n_aug + a] * rhs[a];
+            b[i] = s;
+        }
+    } else {
+        for (int i = 0; i < K; ++i) {
+            b[i] = g(1 + i, 0);
+            for (int j = 0; j < K; ++j) Q[(size_t)j * K + i] = g(1 + i, 1 + j);
+        }
+    }
+    // column filter on diag(W~'W~) and Cholesky (fe_pipeline.f90:1034-1107)
+    double dmax = 0.0;
+    for (int k = 0; k < K; ++k) dmax = std::max(dmax, std::fabs(g(1 + k, 1 + k)));
+    dmax = std::max(dmax, 1.0);
+    const double dtol = 1e-10 * dmax;
+    std::vector<int> keep;
+    for (int k = 0; k < K; ++k)
+        if (!(std::fabs(g(1 + k, 1 + k)) <= dtol)) keep.push_back(k);
+    const int kept = (int)keep.size();
+    result->dof_model = kept;
+    if (kept == 0) {
+        result->solver_info = -1;
+        return set_ok();
+    }
+    std::vector<double> Qs((size_t)kept * kept), bs(kept), bsol, Qinv;
+    for (int a = 0; a < kept; ++a) {
+        bs[a] = b[keep[a]];
+        for (int c = 0; c < kept; ++c) Qs[(size_t)c * kept + a] = Q[(size_t)keep[c] * K + keep[a]];
+    }
+    int info = dense::chol_solve_invert(Qs, bs, kept, bsol, Qinv);
+    result->solver_info = info;
+    if (info != 0) return set_ok();
+    for (int a = 0; a < kept; ++a) beta[keep[a]] = bsol[a];
+    result->rank = kept;
+    const auto t2 = clk::now();
+    result->t_regression = std::chrono::duration<double>(t2 - t1).count();
+
+    // ---- residual on W~ (also for IV), rss, dof
+    double *dbeta = nullptr, *r = nullptr;
+    if ((e = db.get(&dbeta, (size_t)K)) || (e = db.get(&r, (size_t)n))) return e;
+    FEHIP_CHECK(hipMemcpy(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
+    hipLaunchKernelGGL(residual_work_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, work, n, K, dbeta, r);
+    FEHIP_LAUNCH_CHECK("residual");
+    double rss = 0.0;
+    if ((e = sumsq(r, n, part, &rss))) return e;
+    result->rss = rss;
+    int64_t df = std::max<int64_t>(1, N - (int64_t)(kept + dof_fes + 1));
+    result->dof_resid = df;
+    const double sigma2 = rss / (double)df;
+    std::vector<double> cov((size_t)kept * kept, 0.0);
+    bool clustered = ncd > 0;
+    if (clustered) {
+        // regressors for the scores: W~, or X^ = [W~ Z~] proj_full for IV (fe_pipeline.f90:1181-1186)
+        const double* X = work + n;  // W~ columns
+        int64_t ldX = n;
+        double* Xhat = nullptr;
+        if (use_iv) {
+            std::vector<double> pf((size_t)(K + L) * K, 0.0);
+            for (int k = 0; k < K; ++k)
+                for (int a = 0; a < n_aug; ++a) pf[(size_t)k * (K + L) + (aug_col[a] - 1)] = proj[(size_t)k * n_aug + a];
+            double* dpf = nullptr;
+            if ((e = db.get(&dpf, pf.size())) || (e = db.get(&Xhat, (size_t)n * K))) return e;
+            FEHIP_CHECK(hipMemcpy(dpf, pf.data(), pf.size() * sizeof(double), hipMemcpyHostToDevice), "proj copy");
+            if ((e = gemm_impl(false, false, (int)n, K, K + L, 1.0, work + n, n, dpf, K + L, 0.0, Xhat, n, false)))
+                return e;
+            X = Xhat;
+        }
+        double* T = nullptr;
+        if ((e = db.get(&T, (size_t)n * K))) return e;
+        hipLaunchKernelGGL(scores_rows_kernel, dim3(stream_grid((size_t)n * K, kBlock)), dim3(kBlock), 0, 0, r, X, ldX,
+                           n, K, T);
+        FEHIP_LAUNCH_CHECK("scores rows");
+        double* S = nullptr;
+        int64_t S_cap = 0;
+        double* Md = nullptr;
+        if ((e = db.get(&Md, (size_t)K * K))) return e;
+        std::vector<double> M((size_t)K * K), tmp((size_t)kept * kept);
+        int64_t min_single = 0x7fffffff;
+        for (int mask = 1; mask < (1 << ncd); ++mask) {
+            unsigned dims_mask = 0;
+            int len = 0;
+            for (int j = 0; j < ncd; ++j)
+                if (mask & (1 << j)) {
+                    dims_mask |= 1u << (cdims[j] - 1);
+                    ++len;
+                }
+            const double wgt = (len % 2 == 1) ? 1.0 : -1.0;
+            const Partition* part_s = nullptr;
+            if ((e = panel_partition(p, dims_mask, &part_s, 0))) return e;
+            int64_t ncl = part_s->n_clusters;
+            if (len == 1) {
+                ncl = Geff[__builtin_ctz(dims_mask)];
+                min_single = std::min<int64_t>(min_single, ncl);
+            }
+            result->n_clusters[mask - 1] = ncl;
+            const int64_t nS = part_s->n_clusters;
+            if (nS * K > S_cap) {
+                if (S) (void)hipFree(S), db.v.pop_back();
+                S_cap = nS * K;
+                if ((e = db.get(&S, (size_t)S_cap))) return e;
+            }
+            hipLaunchKernelGGL(segsum_kernel, dim3(stream_grid((size_t)nS * K, kBlock)), dim3(kBlock), 0, 0,
+                               (const double*)T, K, (const int*)part_s->order, (const int64_t*)part_s->off, nS, S);
+            FEHIP_LAUNCH_CHECK("segsum");
+            if ((e = gemm_impl(false, true, K, K, nS, 1.0, S, K, S, K, 0.0, Md, K, true))) return e;
+            FEHIP_CHECK(hipMemcpy(M.data(), Md, M.size() * sizeof(double), hipMemcpyDeviceToHost), "meat copy");
+            for (int j = 0; j < K; ++j)
+                for (int i = j + 1; i < K; ++i) M[(size_t)j * K + i] = M[(size_t)i * K + j];
+            // cov += w Qinv M_kept Qinv
+            for (int a = 0; a < kept; ++a)
+                for (int c = 0; c < kept; ++c) {
+                    double s = 0.0;
+                    for (int k = 0; k < kept; ++k) s += M[(size_t)keep[k] * K + keep[a]] * Qinv[(size_t)c * kept + k];
+                    tmp[(size_t)c * kept + a] = s;
+                }
+            for (int a = 0; a < kept; ++a)
+                for (int c = 0; c < kept; ++c) {
+                    double s = 0.0;
+                    for (int k = 0; k < kept; ++k) s += Qinv[(size_t)k * kept + a] * tmp[(size_t)c * kept + k];
+                    cov[(size_t)c * kept + a] += wgt * s;
+                }
+        }
+        const int nested_adj = dof_nested > 0 ? 1 : 0;
+        const int64_t den = std::max<int64_t>(1, N - (kept + nested_adj));
+        double scalar = (double)std::max<int64_t>(1, N - 1) / (double)den;
+        if (min_single > 1) scalar *= (double)min_single / (double)(min_single - 1);
+        for (double& v : cov) v *= scalar;
+        for (int j = 0; j < kept; ++j)
+            for (int i = j + 1; i < kept; ++i) cov[(size_t)j * kept + i] = cov[(size_t)i * kept + j];
+        result->psd_fix_applied = dense::psd_clip(cov, kept);
+        result->min_single_cluster = min_single;
+        result->dof_resid = std::max<int64_t>(1, min_single - 1);
+    } else {
+        for (int a = 0; a < kept; ++a)
+            for (int c = 0; c < kept; ++c) cov[(size_t)c * kept + a] = sigma2 * Qinv[(size_t)c * kept + a];
+    }
+    for (int a = 0; a < kept; ++a) se[keep[a]] = std::sqrt(std::max(0.0, cov[(size_t)a * kept + a]));
+    if (vcov)
+        for (int a = 0; a < kept; ++a)
+            for (int c = 0; c < kept; ++c) vcov[(size_t)keep[c] * K + keep[a]] = cov[(size_t)c * kept + a];
+    finalize_stats(result, N);
+    FEHIP_CHECK(hipDeviceSynchronize(), "estimate sync");
+    result->t_se = std::chrono::duration<double>(clk::now() - t2).count();
+    return set_ok();
+}

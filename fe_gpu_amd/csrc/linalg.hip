@@ -1,0 +1,302 @@
+// Dense-algebra entry points of the drop-in C-ABI (SYRK, GEMV, GEMM, residual, dot,
+// cluster scores and meat).  They replace the reference's cuBLAS wrappers and its
+// cluster-score kernel (src/gpu/fe_gpu_linalg.cu:29-231) with own HIP kernels:
+//   * every product with a long reduction over observations (W'W, W'y, Z'W, S'S) is a
+//     split-K LDS-tiled kernel whose per-chunk partials are reduced in a fixed order, so
+//     the results are deterministic run to run;
+//   * the cluster scores merge runs of equal cluster ids inside each wavefront before the
+//     FP64 atomic, so cluster-sorted rows pay one atomic per run and column.
+#include "../../include/fe_gpu_runtime.h"
+#include "common.hpp"
+
+namespace fehip {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int TM = 32;  // output tile rows
+constexpr int TN = 32;  // output tile cols
+constexpr int TK = 32;  // reduction slab
+
+// C_tile (TM x TN) = sum_{p in [k0,k1)} op(A)[i][p] * op(B)[p][j]
+// split==1: C = alpha*acc + beta*C directly; else partial[z] (m x n, col-major, ld m).
+__global__ __launch_bounds__(kBlock) void gemm_tile_kernel(bool tA, bool tB, int m, int n, long long k,
+                                                           const double* __restrict__ A, long long ldA,
+                                                           const double* __restrict__ B, long long ldB,
+                                                           long long kchunk, double alpha, double beta, double* C,
+                                                           long long ldC, double* partial, bool upper_only) {
+    __shared__ double As[TK][TM + 1];
+    __shared__ double Bs[TK][TN + 1];
+    const int i0 = blockIdx.x * TM;
+    const int j0 = blockIdx.y * TN;
+    if (upper_only && i0 > j0 + TN - 1) return;  // tile strictly below the diagonal
+    const long long k0 = (long long)blockIdx.z * kchunk;
+    long long k1 = k0 + kchunk;
+    if (k1 > k) k1 = k;
+    const int t = threadIdx.x;
+    const int ti = t % 16, tj = t / 16;  // 2x2 outputs per thread: rows ti, ti+16; cols tj, tj+16
+    double c00 = 0, c01 = 0, c10 = 0, c11 = 0;
+    for (long long p0 = k0; p0 < k1; p0 += TK) {
+        // load op(A)[i0..i0+TM)[p0..p0+TK) and op(B)[p0..p0+TK)[j0..j0+TN)
+        for (int e = t; e < TM * TK; e += kBlock) {
+            int pp, ii;
+            if (tA) {  // A is k x m: consecutive threads walk p (contiguous in A)
+                pp = e % TK;
+                ii = e / TK;
+            } else {   // A is m x k: consecutive threads walk i (contiguous)
+                ii = e % TM;
+                pp = e / TM;
+            }
+            long long p = p0 + pp;
+            int i = i0 + ii;
+            double v = 0.0;
+            if (p < k1 && i < m) v = tA ? A[p + (long long)i * ldA] : A[i + p * ldA];
+            As[pp][ii] = v;
+        }
+        for (int e = t; e < TN * TK; e += kBlock) {
+            int pp, jj;
+            if (!tB) {  // B is k x n: contiguous in p
+                pp = e % TK;
+                jj = e / TK;
+            } else {    // B is n x k: contiguous in j
+                jj = e % TN;
+                pp = e / TN;
+            }
+            long long p = p0 + pp;
+            int j = j0 + jj;
+            double v = 0.0;
+            if (p < k1 && j < n) v = tB ? B[j + p * ldB] : B[p + (long long)j * ldB];
+            Bs[pp][jj] = v;
+        }
+        __syncthreads();
+#pragma unroll 8
+        for (int pp = 0; pp < TK; ++pp) {
+            double a0 = As[pp][ti], a1 = As[pp][ti + 16];
+            double b0 = Bs[pp][tj], b1 = Bs[pp][tj + 16];
+            c00 = fma(a0, b0, c00);
+            c01 = fma(a0, b1, c01);
+            c10 = fma(a1, b0, c10);
+            c11 = fma(a1, b1, c11);
+        }
+        __syncthreads();
+    }
+    const double acc[2][2] = {{c00, c01}, {c10, c11}};
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) {
+            int i = i0 + ti + 16 * a, j = j0 + tj + 16 * b;
+            if (i >= m || j >= n) continue;
+            if (upper_only && i > j) continue;
+            if (partial) {
+                partial[(size_t)blockIdx.z * m * n + (size_t)j * m + i] = acc[a][b];
+            } else {
+                double* cp = C + i + (long long)j * ldC;
+                *cp = beta == 0.0 ? alpha * acc[a][b] : alpha * acc[a][b] + beta * (*cp);
+            }
+        }
+}
+
+__global__ void gemm_finish_kernel(const double* __restrict__ partial, int nsplit, int m, int n, double alpha,
+                                   double beta, double* C, long long ldC, bool upper_only) {
+    const size_t total = (size_t)m * n;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        int i = (int)(e % m), j = (int)(e / m);
+        if (upper_only && i > j) continue;
+        double s = 0.0;
+        for (int z = 0; z < nsplit; ++z) s += partial[(size_t)z * total + e];
+        double* cp = C + i + (long long)j * ldC;
+        *cp = beta == 0.0 ? alpha * s : alpha * s + beta * (*cp);
+    }
+}
+
+}  // namespace
+
+int gemm_impl(bool tA, bool tB, int m, int n, long long k, double alpha, const double* A, long long ldA,
+              const double* B, long long ldB, double beta, double* C, long long ldC, bool upper_only) {
+    if (m <= 0 || n <= 0) return set_ok();
+    const long long tiles = (long long)((m + TM - 1) / TM) * ((n + TN - 1) / TN);
+    const long long target = 4LL * num_cus();
+    long long nsplit = 1;
+    if (k > 0 && tiles < target) {
+        nsplit = target / tiles;
+        long long max_by_k = (k + 255) / 256;
+        if (nsplit > max_by_k) nsplit = max_by_k;
+        long long max_by_mem = (256LL << 20) / ((long long)m * n * 8);
+        if (nsplit > max_by_mem) nsplit = max_by_mem;
+        if (nsplit < 1) nsplit = 1;
+        if (nsplit > 65535) nsplit = 65535;
+    }
+    long long kchunk = k > 0 ? (k + nsplit - 1) / nsplit : 1;
+    kchunk = (kchunk + TK - 1) / TK * TK;
+    if (k > 0) nsplit = (k + kchunk - 1) / kchunk;
+    if (nsplit < 1) nsplit = 1;
+    dim3 grid((m + TM - 1) / TM, (n + TN - 1) / TN, (unsigned)nsplit);
+    if (nsplit == 1) {
+        hipLaunchKernelGGL(gemm_tile_kernel, grid, dim3(kBlock), 0, 0, tA, tB, m, n, k, A, ldA, B, ldB, kchunk,
+                           alpha, beta, C, ldC, (double*)nullptr, upper_only);
+        FEHIP_LAUNCH_CHECK("gemm tile kernel");
+        return set_ok();
+    }
+    double* partial = static_cast<double*>(workspace((size_t)nsplit * m * n * sizeof(double), 2));
+    if (!partial) return set_error("gemm: workspace allocation failed");
+    hipLaunchKernelGGL(gemm_tile_kernel, grid, dim3(kBlock), 0, 0, tA, tB, m, n, k, A, ldA, B, ldB, kchunk, alpha,
+                       beta, C, ldC, partial, upper_only);
+    FEHIP_LAUNCH_CHECK("gemm tile kernel");
+    hipLaunchKernelGGL(gemm_finish_kernel, dim3(stream_grid((size_t)m * n, kBlock)), dim3(kBlock), 0, 0, partial,
+                       (int)nsplit, m, n, alpha, beta, C, ldC, upper_only);
+    FEHIP_LAUNCH_CHECK("gemm finish kernel");
+    return set_ok();
+}
+
+namespace {
+
+__global__ void residual_kernel(int n_rows, int n_cols, const double* __restrict__ W, long long ldW,
+                                const double* __restrict__ beta, const double* __restrict__ y, double* r) {
+    extern __shared__ double sb[];
+    for (int c = threadIdx.x; c < n_cols; c += blockDim.x) sb[c] = beta[c];
+    __syncthreads();
+    const long long stride = (long long)gridDim.x * blockDim.x;
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n_rows; i += stride) {
+        double s = y[i];
+        for (int c = 0; c < n_cols; ++c) s -= W[i + c * ldW] * sb[c];
+        r[i] = s;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void dot_partial_kernel(int n, const double* __restrict__ x,
+                                                             const double* __restrict__ y, double* part) {
+    __shared__ double ws[kBlock / kWave];
+    double s = 0.0;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) s = fma(x[i], y[i], s);
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += ws[w];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void sum_partials_kernel(const double* __restrict__ part, int np, double* out) {
+    __shared__ double ws[kBlock / kWave];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += part[i];
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += ws[w];
+        *out = t;
+    }
+}
+
+__device__ __forceinline__ double run_sum(double v, int lane, int seg_end) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        double t = __shfl_down(v, off, kWave);
+        if (lane + off <= seg_end) v += t;
+    }
+    return v;
+}
+
+__global__ __launch_bounds__(kBlock) void cluster_scores_kernel(const double* __restrict__ r,
+                                                                const double* __restrict__ W,
+                                                                const int* __restrict__ ids, long long n, int K,
+                                                                long long ldW, int G, double* __restrict__ S) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const long long wstride = (long long)gridDim.x * blockDim.x;
+    for (long long base = (long long)blockIdx.x * blockDim.x + (threadIdx.x & ~(kWave - 1)); base < n;
+         base += wstride) {
+        const long long i = base + lane;
+        int c = i < n ? ids[i] - 1 : -1;
+        if (c < 0 || c >= G) c = -1 - lane;
+        int cn = __shfl_down(c, 1, kWave), cp = __shfl_up(c, 1, kWave);
+        bool tail = (lane == kWave - 1) || cn != c;
+        bool head = (lane == 0) || cp != c;
+        unsigned long long above = __ballot(tail) & (~0ull << lane);
+        int seg_end = __ffsll((long long)above) - 1;
+        const double ri = c >= 0 ? r[i] : 0.0;
+        for (int j = 0; j < K; ++j) {
+            double v = run_sum(c >= 0 ? ri * W[i + j * ldW] : 0.0, lane, seg_end);
+            if (head && c >= 0) unsafeAtomicAdd(&S[c + (long long)j * G], v);
+        }
+    }
+}
+
+}  // namespace
+}  // namespace fehip
+
+using namespace fehip;
+
+extern "C" {
+
+int fe_gpu_linalg_init(void) { return set_ok(); }
+
+int fe_gpu_linalg_shutdown(void) { return 0; }
+
+int fe_gpu_syrk(int n_rows, int n_cols, double alpha, const double* W, int ldW, double beta, double* Q) {
+    if (n_cols == 0) return set_ok();
+    if (n_cols < 0 || n_rows < 0) return set_error("syrk: negative dimension");
+    return gemm_impl(true, false, n_cols, n_cols, n_rows, alpha, W, ldW, W, ldW, beta, Q, n_cols, true);
+}
+
+int fe_gpu_gemv(int n_rows, int n_cols, double alpha, const double* W, int ldW, const double* y, double beta,
+                double* b) {
+    if (n_cols == 0) return set_ok();
+    if (n_cols < 0 || n_rows < 0) return set_error("gemv: negative dimension");
+    return gemm_impl(true, false, n_cols, 1, n_rows, alpha, W, ldW, y, n_rows > 0 ? n_rows : 1, beta, b, n_cols,
+                     false);
+}
+
+int fe_gpu_residual(int n_rows, int n_cols, const double* W, int ldW, const double* beta, const double* y,
+                    double* residual) {
+    if (n_cols == 0) return set_ok();
+    if (n_rows <= 0) return set_ok();
+    hipLaunchKernelGGL(residual_kernel, dim3(stream_grid((size_t)n_rows, kBlock)), dim3(kBlock),
+                       sizeof(double) * (size_t)n_cols, 0, n_rows, n_cols, W, (long long)ldW, beta, y, residual);
+    FEHIP_LAUNCH_CHECK("residual kernel");
+    return set_ok();
+}
+
+int fe_gpu_dot(int n_rows, const double* x, const double* y, double* result) {
+    if (!result) return set_error("dot: null result");
+    if (n_rows <= 0) {
+        *result = 0.0;
+        return set_ok();
+    }
+    unsigned grid = stream_grid((size_t)n_rows, kBlock, 4);
+    double* part = static_cast<double*>(workspace(sizeof(double) * (grid + 1), 3));
+    if (!part) return set_error("dot: workspace allocation failed");
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(grid), dim3(kBlock), 0, 0, n_rows, x, y, part);
+    FEHIP_LAUNCH_CHECK("dot kernel");
+    hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kBlock), 0, 0, part, (int)grid, part + grid);
+    FEHIP_LAUNCH_CHECK("dot reduce kernel");
+    FEHIP_CHECK(hipMemcpy(result, part + grid, sizeof(double), hipMemcpyDeviceToHost), "dot copy");
+    return set_ok();
+}
+
+int fe_gpu_gemm(char transA, char transB, int m, int n, int k, double alpha, const double* A, int ldA,
+                const double* B, int ldB, double beta, double* C, int ldC) {
+    bool tA = !(transA == 'N' || transA == 'n');
+    bool tB = !(transB == 'N' || transB == 'n');
+    if (m < 0 || n < 0 || k < 0) return set_error("gemm: negative dimension");
+    return gemm_impl(tA, tB, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, false);
+}
+
+int fe_gpu_cluster_scores(const double* residual, const double* W, const int* cluster_ids, int n_rows, int n_cols,
+                          int ldW, int n_clusters, double* scores) {
+    if (n_cols == 0 || n_rows <= 0 || n_clusters <= 0) return set_ok();
+    hipLaunchKernelGGL(cluster_scores_kernel, dim3(stream_grid((size_t)n_rows, kBlock, 8)), dim3(kBlock), 0, 0,
+                       residual, W, cluster_ids, (long long)n_rows, n_cols, (long long)ldW, n_clusters, scores);
+    FEHIP_LAUNCH_CHECK("cluster scores kernel");
+    FEHIP_CHECK(hipDeviceSynchronize(), "cluster scores sync");
+    return set_ok();
+}
+
+int fe_gpu_cluster_meat(int n_clusters, int n_cols, const double* scores, int ldScores, double* meat) {
+    return fe_gpu_syrk(n_clusters, n_cols, 1.0, scores, ldScores, 0.0, meat);
+}
+
+}  // extern "C"

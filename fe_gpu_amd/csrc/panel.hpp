@@ -1,0 +1,77 @@
+// Device-resident panel: FE ids re-ordered by the primary (largest) FE dim, with the segment,
+// run and cluster-partition structures every coarse kernel reads.
+#pragma once
+
+#include <map>
+#include <vector>
+
+#include "../../include/fehip.h"
+#include "common.hpp"
+
+namespace fehip {
+
+// Small dims keep their whole group table in LDS; larger dims scatter through runs.
+constexpr int kSmallMaxGroups = 1024;
+
+struct DimStruct {
+    int G = 0;                 // groups (max id)
+    int small = 0;             // table fits LDS
+    int* ids = nullptr;        // panel order, 0-based
+    int* counts = nullptr;     // rows per group (unweighted)
+    // runs: maximal stretches of equal id inside one primary segment (non-primary dims only)
+    int64_t n_runs = 0;
+    int64_t* seg_run_off = nullptr;  // [G_P + 1] first run of each primary segment
+    int* run_order = nullptr;        // run indices sorted by group
+    int64_t* grp_run_off = nullptr;  // [G + 1] offsets into run_order
+};
+
+// Row partition of a cluster subset: rows of cluster c are order[off[c] .. off[c+1]).
+struct Partition {
+    int64_t n_clusters = 0;
+    int* order = nullptr;      // nullptr => identity (panel order)
+    int64_t* off = nullptr;    // [n_clusters + 1]
+};
+
+struct Panel {
+    int64_t n = 0;
+    int m = 0;
+    int P = 0;                 // primary dim
+    int sorted_input = 1;
+    int max_seg = 0;
+    int* perm = nullptr;       // panel row -> caller row (nullptr if identity)
+    int64_t* seg_off = nullptr;  // [G_P + 1]
+    DimStruct d[FEHIP_MAX_FE];
+    std::map<unsigned, Partition> partitions;  // key: bit mask of 0-based dims
+    int64_t bytes = 0;
+};
+
+int panel_partition(Panel* p, unsigned dims_mask, const Partition** out, hipStream_t s);
+
+// gemm on the default stream (linalg.hip); C = alpha op(A) op(B) + beta C
+int gemm_impl(bool tA, bool tB, int m, int n, long long k, double alpha, const double* A, long long ldA,
+              const double* B, long long ldB, double beta, double* C, long long ldC, bool upper_only);
+
+int build_intersection_ids(const int* const* dims, const unsigned long long* strides, int nd, long long n, int* out,
+                           int* n_clusters, hipStream_t stream);
+
+// device allocation helper that tracks panel bytes
+template <typename T>
+int dalloc(T** p, size_t count, int64_t* tally = nullptr) {
+    size_t bytes = count * sizeof(T);
+    if (bytes == 0) bytes = sizeof(T);
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return set_hip_error(e, "hipMalloc");
+    }
+    if (tally) *tally += (int64_t)bytes;
+    return 0;
+}
+
+template <typename T>
+void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+}  // namespace fehip

@@ -1,0 +1,124 @@
+/*
+ * fehip.h -- coarse-grained C-ABI of the MI355X estimation core (libfe_gpu_hip.so).
+ *
+ * The fine-grained drop-in ABI (fe_gpu_runtime.h) mirrors the reference's per-kernel calls.
+ * These entry points expose the whole estimation core as a few calls so the work can be laid
+ * out for CDNA4: a device-resident panel (FE ids re-ordered by the largest FE dim, segment and
+ * run structures built once), a within-transform solved in FE-table space, fused normal
+ * equations, cluster-robust sandwich and 2SLS stages.  They replace, as one call, the
+ * reference's fe_gpu_estimate (src/core/fe_pipeline.f90:516-1896) with its within transform
+ * (src/gpu/fe_gpu_demean.f90:63-311), and are what the serving and batching front ends use.
+ *
+ * All pointers named d_* are device pointers; everything else is host memory.  Status
+ * conventions are those of fe_gpu_runtime.h (0 = success, message via
+ * fe_gpu_runtime_get_last_error).
+ */
+#ifndef FEHIP_H
+#define FEHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FEHIP_MAX_FE 8
+
+typedef struct fehip_panel fehip_panel;
+
+typedef struct {
+    int64_t n_obs;
+    int32_t n_fe;
+    int32_t primary_dim;               /* 0-based FE dim the rows are ordered by */
+    int32_t sorted_input;              /* 1 if the caller's rows were already in that order */
+    int32_t max_segment;               /* longest primary segment (rows) */
+    int32_t group_sizes[FEHIP_MAX_FE]; /* G_d = max id (src/core/fe_grouping.f90:11-37) */
+    int64_t n_runs[FEHIP_MAX_FE];      /* runs of equal id inside primary segments (0 for the primary) */
+    int64_t device_bytes;              /* device memory held by the panel */
+} fehip_panel_info;
+
+/* Build a panel from n_fe device arrays of 1-based int32 FE ids (d_fe_ids is a host array of
+ * device pointers).  Ids < 1 are rejected (the reference aborts on them).  The ids are copied;
+ * the caller's arrays may be freed afterwards. */
+int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehip_panel** out);
+int fehip_panel_destroy(fehip_panel* panel);
+int fehip_panel_get_info(const fehip_panel* panel, fehip_panel_info* info);
+
+/* Within-transform options. */
+typedef struct {
+    double tol;           /* stop when max |group-mean update| < tol (the reference's criterion) */
+    int32_t max_iter;     /* iteration cap */
+    int32_t method;       /* 0: PCG in FE space (default), 1: alternating projections in row space */
+    uint32_t fe_mask;     /* bit d set => FE dim d is absorbed; 0 means all dims */
+    int32_t precision;    /* 0: FP64, 1: FP32 tables + Kahan sums with FP64 refinement */
+    int32_t check_every;  /* convergence read-back period in iterations (0 -> 4) */
+    int32_t reserved;
+} fehip_demean_opts;
+
+typedef struct {
+    int32_t iterations;   /* max over columns */
+    int32_t converged;    /* all columns */
+    double max_update;    /* max over columns of the final criterion */
+    double seconds;       /* device time of the transform */
+    int64_t bytes_moved;  /* algorithmic bytes (ids, weights, tables, columns) */
+} fehip_demean_stats;
+
+/* In-place M_F on n_cols device columns (column-major, leading dim ld, caller row order).
+ * d_weights (optional, n_obs uint8 in caller order) restricts the transform to rows with
+ * weight 1; other rows are left untouched. */
+int fehip_demean(fehip_panel* panel, double* d_cols, int n_cols, int64_t ld, const uint8_t* d_weights,
+                 const fehip_demean_opts* opts, fehip_demean_stats* stats);
+
+/* Estimation options (formula-level choices stay with the host, as in the reference). */
+typedef struct {
+    fehip_demean_opts demean;
+    int32_t n_cluster;            /* cluster FE dims (1-based, <= 3 used, order kept) */
+    int32_t cluster_dims[4];
+    int32_t n_iv_reg;             /* endogenous regressors (1-based indices into W) */
+    const int32_t* iv_reg;
+    int32_t n_iv_inst;            /* selected instrument columns (1-based into Z); 0 => all */
+    const int32_t* iv_inst;
+} fehip_opts;
+
+typedef struct {
+    int32_t converged, fe_iterations, solver_info, rank;
+    int32_t dof_model, dof_fes, is_iv, n_instruments;
+    int64_t dof_resid, n_obs;
+    int32_t n_cluster_dims, cluster_dims[3];
+    int64_t n_clusters[7];
+    int64_t min_single_cluster;
+    int32_t psd_fix_applied, reserved;
+    double rss, tss_total, tss_within, r2, r2_adj, r2_within, f_stat, max_update;
+    double t_demean, t_regression, t_se;  /* seconds */
+} fehip_result;
+
+/* Full estimation on device columns y (n), W (n x K), Z (n x L), column-major with leading dim
+ * ld, caller row order; optional row weights (uint8).  beta/se (K) and vcov (K x K, may be NULL)
+ * are host outputs.  Follows the GPU-path conventions of the reference (relaxed-AP fixed point,
+ * zero-variance column filter, Cholesky, IV normal equations, CGM with the nested small-sample
+ * scalar and PSD clip). */
+int fehip_estimate(fehip_panel* panel, const double* d_y, const double* d_W, int K, const double* d_Z, int L,
+                   int64_t ld, const uint8_t* d_weights, const fehip_opts* opts, fehip_result* result, double* beta,
+                   double* se, double* vcov);
+
+/* Deterministic synthetic worker-firm-time panel generated on the device (the 150M-row benchmark
+ * panel does not fit a host round trip).  Rows are ordered by worker then period.  Writes
+ * d_fe (3 x n int32, dim-major: worker, firm, period), d_y (n) and d_W (n x 8, col-major). */
+int fehip_synth_worker_firm(int64_t n_workers, int32_t n_periods, int32_t n_firms, double switch_p, uint64_t seed,
+                            int32_t* d_fe, double* d_y, double* d_W);
+
+/* Opt-in per-kernel timing with HIP events on the stream each kernel is launched on.  Kernel ids:
+ * 0 apply_iter (the CG operator product, the dominant kernel), 1 apply_data (right-hand side),
+ * 2 finalize (M_F columns), 3 reduce_runs, 4 gram, 5 segsum (cluster score sums).  bytes/flops
+ * accumulate the algorithmic work of every timed launch. */
+int fehip_kstats_enable(int on);
+int fehip_kstats_reset(void);
+int fehip_kstats_get(int id, char* name, int name_len, int64_t* launches, double* total_ms, double* bytes,
+                     double* flops);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEHIP_H */

@@ -1,0 +1,202 @@
+"""GPU parity of the coarse path (fehip_* : device-resident panel, FE-space CG within-transform,
+fused normal equations, CGM sandwich, 2SLS) against the oracle.
+
+Tolerances: integer outputs (group sizes, dof, cluster counts, drop decisions) exact; beta and SE
+within 1e-8 relative of the oracle with both sides converged to a tight tolerance (the two AP
+schemes stop at different points at the production tolerance 1e-8).
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def note(msg):
+    print(f"[gpu] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def device():
+    from fe_gpu_amd import device as d
+    d.init(0)
+
+
+def rel(a, b):
+    a, b = np.asarray(a, float), np.asarray(b, float)
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))) if a.size else 0.0
+
+
+def test_panel_structures():
+    from fe_gpu_amd import api, synth
+    note("panel structures")
+    p = synth.random_fe_problem(1, n=5000, K=1, G=(40, 1500, 7))
+    pan = api.Panel.from_host(p.fe)
+    inf = pan.info()
+    assert inf["group_sizes"] == [40, 1500, 7]
+    assert inf["primary_dim"] == 1 and inf["sorted_input"] == 0
+    # runs of dim 0 inside primary segments (rows stably sorted by dim 1)
+    order = np.argsort(p.fe[1], kind="stable")
+    f0, f1 = p.fe[0][order], p.fe[1][order]
+    runs = 1 + int(np.sum((f0[1:] != f0[:-1]) | (f1[1:] != f1[:-1])))
+    assert inf["n_runs"][0] == 0            # 40 groups -> LDS dim, no runs
+    pan.close()
+    q = synth.random_fe_problem(2, n=5000, K=1, G=(3000, 1500))
+    pan = api.Panel.from_host(q.fe)
+    inf = pan.info()
+    order = np.argsort(q.fe[0], kind="stable")
+    f0, f1 = q.fe[0][order], q.fe[1][order]
+    assert inf["primary_dim"] == 0
+    assert inf["n_runs"][1] == 1 + int(np.sum((f0[1:] != f0[:-1]) | (f1[1:] != f1[:-1])))
+    pan.close()
+
+
+@pytest.mark.parametrize("G", [(9,), (25, 8), (3000, 1200, 6), (40, 2000, 1500, 9)])
+def test_demean_matches_dense_projection(G):
+    from fe_gpu_amd import api, synth
+    from fe_gpu_amd.device import DeviceBuffer
+    from tests import bruteforce as bf
+    note(f"demean G={G}")
+    p = synth.random_fe_problem(sum(G), n=6000, K=5, G=G)
+    X = np.asfortranarray(np.column_stack([p.y, p.W]))
+    exact = bf.annihilate(X, p.fe)
+    pan = api.Panel.from_host(p.fe)
+    d = DeviceBuffer.from_array(X)
+    st = api.demean(pan, d.ptr, X.shape[1], X.shape[0], tol=1e-13, max_iter=20000)
+    got = d.to_array(np.float64, X.shape, order="F")
+    assert st["converged"] == 1
+    assert np.max(np.abs(got - exact)) < 1e-9 * max(1.0, np.max(np.abs(X)))
+    pan.close()
+
+
+def test_demean_weights_leave_masked_rows_untouched():
+    from fe_gpu_amd import api, synth
+    from fe_gpu_amd.device import DeviceBuffer
+    from tests import bruteforce as bf
+    note("demean weights")
+    p = synth.random_fe_problem(5, n=4000, K=2, G=(300, 1100, 5))
+    rng = np.random.default_rng(0)
+    w = (rng.random(p.n) < 0.6).astype(np.uint8)
+    X = np.asfortranarray(np.column_stack([p.y, p.W]))
+    pan = api.Panel.from_host(p.fe)
+    d = DeviceBuffer.from_array(X)
+    dw = DeviceBuffer.from_array(w)
+    api.demean(pan, d.ptr, 3, p.n, dw.ptr, tol=1e-13, max_iter=20000)
+    got = d.to_array(np.float64, X.shape, order="F")
+    m = w == 1
+    assert np.array_equal(got[~m], X[~m])
+    exact = bf.annihilate(X[m], p.fe[:, m])
+    assert np.max(np.abs(got[m] - exact)) < 1e-9 * np.max(np.abs(X))
+    pan.close()
+
+
+@pytest.mark.parametrize("case", ["small3", "large2", "unsorted4", "one"])
+@pytest.mark.parametrize("cluster", [(), (1,), (1, 2), (1, 2, 3)])
+def test_estimate_matches_oracle(case, cluster):
+    import oracle
+    from fe_gpu_amd import api, synth
+    G = {"small3": (50, 12, 6), "large2": (2500, 1300), "unsorted4": (700, 1600, 30, 4), "one": (90,)}[case]
+    cl = tuple(c for c in cluster if c <= len(G))
+    note(f"estimate case={case} cluster={cl}")
+    p = synth.random_fe_problem(len(G) * 7 + len(cl), n=8000, K=3, G=G)
+    fit = api.estimate(p.y, p.W, p.fe, cluster=cl, tol=1e-13, max_iter=20000)
+    ref = oracle.estimate(p.y, p.W, p.fe, cluster=cl, tol=1e-13, max_iter=200000)
+    assert fit.res["converged"] == 1
+    assert rel(fit.beta, ref.beta) < 1e-8
+    assert rel(fit.se, ref.se) < 1e-8
+    for k in ("dof_fes", "dof_resid", "dof_model", "rank"):
+        assert fit.res[k] == ref.res[k], k
+    if cl:
+        assert fit.res["n_clusters"] == ref.res["n_clusters"]
+        assert fit.res["psd_fix_applied"] == ref.res["psd_fix_applied"]
+    for k in ("rss", "tss_total", "tss_within", "r2", "r2_within", "r2_adj"):
+        assert abs(fit.res[k] - ref.res[k]) <= 1e-8 * max(1.0, abs(ref.res[k])), k
+
+
+def test_estimate_nlsw_like_homoskedastic_and_3way():
+    import oracle
+    from fe_gpu_amd import api, synth
+    note("estimate nlsw-like")
+    p = synth.nlsw_like()
+    for cl in [(), (1, 2, 3)]:
+        fit = api.estimate(p.y, p.W, p.fe, cluster=cl, tol=1e-12, max_iter=20000)
+        ref = oracle.estimate(p.y, p.W, p.fe, cluster=cl, tol=1e-12, max_iter=20000)
+        assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
+        assert fit.res["dof_resid"] == ref.res["dof_resid"]
+
+
+def test_estimate_iv_matches_oracle():
+    import oracle
+    from fe_gpu_amd import api, synth
+    note("estimate IV")
+    p = synth.random_fe_problem(91, n=6000, K=4, L=3, G=(1500, 1100, 8))
+    for cl in [(), (1,), (2, 3)]:
+        fit = api.estimate(p.y, p.W, p.fe, Z=p.Z, iv_reg=[1, 3], cluster=cl, tol=1e-13, max_iter=20000)
+        ref = oracle.estimate(p.y, p.W, p.fe, Z=p.Z, iv_reg=[1, 3], cluster=cl, tol=1e-13, max_iter=200000)
+        assert fit.res["is_iv"] == 1 and fit.res["n_instruments"] == ref.res["n_instruments"]
+        assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
+
+
+def test_estimate_column_filter():
+    import oracle
+    from fe_gpu_amd import api, synth
+    note("column filter")
+    p = synth.random_fe_problem(93, n=5000, K=3, G=(1500, 20))
+    W = p.W.copy()
+    W[:, 2] = np.arange(1, 1501)[p.fe[0] - 1] * 0.1
+    fit = api.estimate(p.y, W, p.fe, tol=1e-13, max_iter=20000)
+    ref = oracle.estimate(p.y, W, p.fe, tol=1e-13, max_iter=200000)
+    assert fit.res["dof_model"] == ref.res["dof_model"] == 2
+    assert fit.beta[2] == 0.0 and fit.se[2] == 0.0
+    assert rel(fit.beta[:2], ref.beta[:2]) < 1e-8
+
+
+def _redense(ids):
+    u, inv = np.unique(ids, return_inverse=True)
+    return (inv + 1).astype(np.int32)
+
+
+@pytest.mark.parametrize("fe_mask,cluster", [(0b011, (1,)), (0b110, (2, 3)), (0b111, (1, 2, 3))])
+def test_masked_request_equals_oracle_on_subset(fe_mask, cluster):
+    """Serving semantics: a request with a row mask and an FE subset equals the oracle run on
+    the masked rows with re-densified ids (G_d = non-empty groups)."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    note(f"masked request fe_mask={fe_mask:b} cluster={cluster}")
+    p = synth.random_fe_problem(55, n=9000, K=3, G=(2000, 1200, 9))
+    rng = np.random.default_rng(1)
+    w = (rng.random(p.n) < 0.45).astype(np.uint8)
+    dims = [d for d in range(3) if fe_mask >> d & 1]
+    fit = api.estimate(p.y, p.W, p.fe, weights=w, cluster=cluster, fe_mask=fe_mask, tol=1e-13, max_iter=20000)
+    m = w == 1
+    fe_sub = np.vstack([_redense(p.fe[d][m]) for d in dims])
+    cl_sub = tuple(dims.index(c - 1) + 1 for c in cluster)
+    ref = oracle.estimate(p.y[m], p.W[m], fe_sub, cluster=cl_sub, tol=1e-13, max_iter=200000)
+    assert fit.res["n_obs"] == int(m.sum())
+    assert fit.res["dof_fes"] == ref.res["dof_fes"] and fit.res["dof_resid"] == ref.res["dof_resid"]
+    assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
+
+
+def test_synth_generator_is_deterministic_and_shaped():
+    from fe_gpu_amd import _lib
+    from fe_gpu_amd.device import DeviceBuffer
+    note("synth generator")
+    L = _lib.load()
+    nw, T, nf = 20000, 30, 4000
+    n = nw * T
+    outs = []
+    for _ in range(2):
+        fe, y, W = DeviceBuffer(3 * n * 4), DeviceBuffer(n * 8), DeviceBuffer(8 * n * 8)
+        assert L.fehip_synth_worker_firm(nw, T, nf, 0.1, 202, fe.ptr, y.ptr, W.ptr) == 0
+        outs.append((fe.to_array(np.int32, (3, n)), y.to_array(np.float64, (n,)),
+                     W.to_array(np.float64, (n, 8), order="F")))
+    assert all(np.array_equal(a, b) for a, b in zip(outs[0], outs[1]))
+    fe, y, W = outs[0]
+    assert fe[0].min() == 1 and fe[0].max() == nw and np.all(np.diff(fe[0]) >= 0)
+    assert fe[2].min() == 1 and fe[2].max() == T and fe[1].max() <= nf
+    firm = fe[1].reshape(nw, T)
+    switch = np.mean(firm[:, 1:] != firm[:, :-1])
+    assert 0.09 < switch < 0.11
+    assert abs(W[:, 1].mean() - 0.05) < 0.005                 # sick_shock ~ Bernoulli(0.05)
