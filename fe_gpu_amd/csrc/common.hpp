@@ -59,12 +59,39 @@ __device__ __forceinline__ void lds_add(double* p, double v) {
     __hip_atomic_fetch_add(lp, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+__device__ __forceinline__ void lds_addf(float* p, float v) {
+    auto* lp = (__attribute__((address_space(3))) float*)(p);
+    __hip_atomic_fetch_add(lp, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // ---- wavefront helpers ----
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
     return v;
 }
+// Segmented sum over contiguous lanes: returns the sum of v over lanes [lane, seg_end] where
+// seg_end is the last lane of the calling lane's run (after step k a lane holds up to 2^k lanes).
+__device__ __forceinline__ double wave_run_sum(double v, int lane, int seg_end) {
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const double t = __shfl_down(v, off, kWave);
+        if (lane + off <= seg_end) v += t;
+    }
+    return v;
+}
+
+// Run structure of a per-lane key within the wavefront: returns the last lane of this lane's run
+// and whether this lane starts its run.
+__device__ __forceinline__ int wave_run_end(int key, int lane, bool* head) {
+    const int kn = __shfl_down(key, 1, kWave);
+    const int kp = __shfl_up(key, 1, kWave);
+    const bool tail = (lane == kWave - 1) || kn != key;
+    *head = (lane == 0) || kp != key;
+    const unsigned long long above = __ballot(tail) & (~0ull << lane);
+    return __ffsll((long long)above) - 1;
+}
+
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
     for (int off = kWave / 2; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, kWave));

@@ -8,12 +8,15 @@
 //   2^m-1 cluster subsets with the nested small-sample scalar and PSD clip (:1109-1420) ->
 //   regression statistics (finalize_regression_stats).
 // Cluster scores use the panel's cached row partitions: S_c = sum_{i in c} r_i x_i is a
-// segmented sum over rows grouped by cluster (no atomics), and the meat S'S is a split-K Gram.
+// reduce-by-key over rows grouped by cluster (no atomics), and the meat sum_c S_c S_c' is
+// accumulated in registers (narrow designs) or by a split-K Gram (wide designs).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
 #include <vector>
+
+#include <rocprim/rocprim.hpp>
 
 #include "dense.hpp"
 #include "panel.hpp"
@@ -99,7 +102,8 @@ __global__ void segsum_kernel(const double* __restrict__ T, int K, const int* __
         const int64_t c = e / K;
         const int j = (int)(e - c * K);
         double s = 0.0;
-        for (int64_t k = off[c]; k < off[c + 1]; ++k) s += T[(int64_t)(order ? order[k] : k) * K + j];
+        const int64_t b = off ? off[c] : c, f = off ? off[c + 1] : c + 1;
+        for (int64_t k = b; k < f; ++k) s += T[(int64_t)(order ? order[k] : k) * K + j];
         S[e] = s;
     }
 }
@@ -121,6 +125,180 @@ __global__ void weights_panel_kernel(const uint8_t* __restrict__ w, const int* _
                                      uint8_t* out) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         out[i] = w[perm ? perm[i] : i] ? 1 : 0;
+}
+
+// Upper triangle of sum_c S_c S_c' with S_c = sum of the T rows of cluster c (T row-major n x K);
+// clusters are rows order[off[c]..off[c+1]) (order == nullptr: identity; off == nullptr: each row
+// is its own cluster).  One thread per cluster, KT >= K accumulators per thread, block partials.
+template <int KT>
+__global__ __launch_bounds__(kBlock) void meat_kernel(const double* __restrict__ T, int K, int ldT,
+                                                      const int* __restrict__ order, const int64_t* __restrict__ off,
+                                                      int64_t nS, double* part) {
+    constexpr int NP = KT * (KT + 1) / 2;
+    double acc[NP];
+#pragma unroll
+    for (int e = 0; e < NP; ++e) acc[e] = 0.0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nS; c += (int64_t)gridDim.x * blockDim.x) {
+        double sc[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) sc[k] = 0.0;
+        const int64_t b = off ? off[c] : c, e = off ? off[c + 1] : c + 1;
+        for (int64_t q = b; q < e; ++q) {
+            const double* row = T + (int64_t)(order ? order[q] : q) * ldT;
+#pragma unroll
+            for (int k = 0; k < KT; ++k)
+                if (k < K) sc[k] += row[k];
+        }
+        int idx = 0;
+#pragma unroll
+        for (int a = 0; a < KT; ++a)
+#pragma unroll
+            for (int bb = a; bb < KT; ++bb) acc[idx++] += sc[a] * sc[bb];
+    }
+    __shared__ double ws[kBlock / kWave][NP];
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        const double v = wave_sum(acc[e]);
+        if (ln == 0) ws[wv][e] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NP; e += blockDim.x) {
+        double v = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) v += ws[w][e];
+        part[(int64_t)blockIdx.x * NP + e] = v;
+    }
+}
+
+// Upper triangle of X'X for n rows of C <= KT column-major columns (ld), block partials.
+template <int KT>
+__global__ __launch_bounds__(kBlock) void gram_rows_kernel(const double* __restrict__ X, int64_t ld, int C, int64_t n,
+                                                           double* part) {
+    constexpr int NP = KT * (KT + 1) / 2;
+    double acc[NP];
+#pragma unroll
+    for (int e = 0; e < NP; ++e) acc[e] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double x[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) x[k] = k < C ? X[i + k * ld] : 0.0;
+        int idx = 0;
+#pragma unroll
+        for (int a = 0; a < KT; ++a)
+#pragma unroll
+            for (int bb = a; bb < KT; ++bb) acc[idx++] += x[a] * x[bb];
+    }
+    __shared__ double ws[kBlock / kWave][NP];
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        const double v = wave_sum(acc[e]);
+        if (ln == 0) ws[wv][e] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NP; e += blockDim.x) {
+        double v = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) v += ws[w][e];
+        part[(int64_t)blockIdx.x * NP + e] = v;
+    }
+}
+
+// Sum the block partials of an upper triangle (KT packing) into a full symmetric K x K matrix.
+int reduce_upper(const double* dpart, int nblk, int KT, int K, std::vector<double>& M) {
+    const int NP = KT * (KT + 1) / 2;
+    std::vector<double> h((size_t)nblk * NP);
+    FEHIP_CHECK(hipMemcpy(h.data(), dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost), "partials");
+    std::vector<double> tot(NP, 0.0);
+    for (int b = 0; b < nblk; ++b)
+        for (int e = 0; e < NP; ++e) tot[e] += h[(size_t)b * NP + e];
+    M.assign((size_t)K * K, 0.0);
+    int idx = 0;
+    for (int a = 0; a < KT; ++a)
+        for (int b = a; b < KT; ++b, ++idx)
+            if (a < K && b < K) M[(size_t)b * K + a] = M[(size_t)a * K + b] = tot[idx];
+    return 0;
+}
+
+template <int KT>
+struct VecK {
+    double v[KT];
+};
+
+template <int KT>
+struct VecPlus {
+    __device__ VecK<KT> operator()(const VecK<KT>& x, const VecK<KT>& y) const {
+        VecK<KT> r;
+#pragma unroll
+        for (int k = 0; k < KT; ++k) r.v[k] = x.v[k] + y.v[k];
+        return r;
+    }
+};
+
+template <int KT>
+struct GatherRow {
+    const double* T;
+    const int* order;
+    int K;
+    __device__ VecK<KT> operator()(int64_t q) const {
+        VecK<KT> r;
+        const double* row = T + (int64_t)(order ? order[q] : q) * K;
+#pragma unroll
+        for (int k = 0; k < KT; ++k) r.v[k] = k < K ? row[k] : 0.0;
+        return r;
+    }
+};
+
+// S[c] = sum of the T rows of cluster c, for a partition given as sorted positions (order) with
+// grouped keys; rocPRIM reduce-by-key handles any mix of cluster sizes (5M-row time clusters and
+// 1-row intersections alike).  S is nS x KT row-major.
+template <int KT>
+int cluster_sums(const double* T, int K, const int* order, const int* keys, int64_t n, double* S, int64_t* d_count,
+                 hipStream_t s) {
+    auto vals = rocprim::make_transform_iterator(rocprim::make_counting_iterator<int64_t>(0),
+                                                 GatherRow<KT>{T, order, K});
+    size_t bytes = 0;
+    FEHIP_CHECK(rocprim::reduce_by_key(nullptr, bytes, keys, vals, (size_t)n, rocprim::make_discard_iterator(),
+                                       reinterpret_cast<VecK<KT>*>(S), d_count, VecPlus<KT>(),
+                                       rocprim::equal_to<int>(), s),
+                "reduce_by_key size");
+    void* tmp = workspace(bytes, 8);
+    if (!tmp) return set_error("reduce_by_key workspace");
+    FEHIP_CHECK(rocprim::reduce_by_key(tmp, bytes, keys, vals, (size_t)n, rocprim::make_discard_iterator(),
+                                       reinterpret_cast<VecK<KT>*>(S), d_count, VecPlus<KT>(),
+                                       rocprim::equal_to<int>(), s),
+                "reduce_by_key");
+    return 0;
+}
+
+constexpr int kMeatBlocks = 1024;
+
+// meat of one partition (K <= 12 fused path)
+int meat_fused(const double* T, int K, int ldT, const int* order, const int64_t* off, int64_t nS, double* part,
+               std::vector<double>& M) {
+    const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kMeatBlocks, (nS + kBlock - 1) / kBlock));
+    if (KT == 4)
+        hipLaunchKernelGGL(meat_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, T, K, ldT, order, off, nS, part);
+    else if (KT == 8)
+        hipLaunchKernelGGL(meat_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, T, K, ldT, order, off, nS, part);
+    else
+        hipLaunchKernelGGL(meat_kernel<12>, dim3(grid), dim3(kBlock), 0, 0, T, K, ldT, order, off, nS, part);
+    FEHIP_LAUNCH_CHECK("meat kernel");
+    return reduce_upper(part, (int)grid, KT, K, M);
+}
+
+// Gram of C <= 12 columns (fused path)
+int gram_fused(const double* X, int64_t ld, int C, int64_t n, double* part, std::vector<double>& G) {
+    const int KT = C <= 4 ? 4 : (C <= 8 ? 8 : 12);
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kMeatBlocks, (n + kBlock - 1) / kBlock));
+    if (KT == 4)
+        hipLaunchKernelGGL(gram_rows_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, X, ld, C, n, part);
+    else if (KT == 8)
+        hipLaunchKernelGGL(gram_rows_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, X, ld, C, n, part);
+    else
+        hipLaunchKernelGGL(gram_rows_kernel<12>, dim3(grid), dim3(kBlock), 0, 0, X, ld, C, n, part);
+    FEHIP_LAUNCH_CHECK("gram kernel");
+    return reduce_upper(part, (int)grid, KT, C, G);
 }
 
 struct DevBuf {
@@ -179,6 +357,7 @@ using namespace fehip;
 extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const double* d_W, int K, const double* d_Z,
                               int L, int64_t ld, const uint8_t* d_weights, const fehip_opts* opts,
                               fehip_result* result, double* beta, double* se, double* vcov) {
+    (void)hipGetLastError();  // start from a clean error state
     if (!panel || !opts || !result || !d_y) return set_error("fehip_estimate: null argument");
     Panel* p = reinterpret_cast<Panel*>(panel);
     const int64_t n = p->n;
@@ -273,13 +452,19 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
     if (K <= 0) return set_ok();
 
     // ---- Gram of [y W Z] (upper), host copy
-    double* Gd = nullptr;
-    if ((e = db.get(&Gd, (size_t)C * C))) return e;
-    if ((e = gemm_impl(true, false, C, C, n, 1.0, work, n, work, n, 0.0, Gd, C, true))) return e;
+    double* mpart = nullptr;
+    if ((e = db.get(&mpart, (size_t)kMeatBlocks * 78))) return e;
     std::vector<double> G((size_t)C * C);
-    FEHIP_CHECK(hipMemcpy(G.data(), Gd, G.size() * sizeof(double), hipMemcpyDeviceToHost), "gram copy");
-    for (int j = 0; j < C; ++j)
-        for (int i = j + 1; i < C; ++i) G[(size_t)j * C + i] = G[(size_t)i * C + j];
+    if (C <= 12) {
+        if ((e = gram_fused(work, n, C, n, mpart, G))) return e;
+    } else {
+        double* Gd = nullptr;
+        if ((e = db.get(&Gd, (size_t)C * C))) return e;
+        if ((e = gemm_impl(true, false, C, C, n, 1.0, work, n, work, n, 0.0, Gd, C, true))) return e;
+        FEHIP_CHECK(hipMemcpy(G.data(), Gd, G.size() * sizeof(double), hipMemcpyDeviceToHost), "gram copy");
+        for (int j = 0; j < C; ++j)
+            for (int i = j + 1; i < C; ++i) G[(size_t)j * C + i] = G[(size_t)i * C + j];
+    }
     auto g = [&](int i, int j) { return G[(size_t)j * C + i]; };
     result->tss_within = g(0, 0);
 
@@ -418,6 +603,8 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
         FEHIP_LAUNCH_CHECK("scores rows");
         double* S = nullptr;
         int64_t S_cap = 0;
+        int64_t* dcount = nullptr;
+        if ((e = db.get(&dcount, 1))) return e;
         double* Md = nullptr;
         if ((e = db.get(&Md, (size_t)K * K))) return e;
         std::vector<double> M((size_t)K * K), tmp((size_t)kept * kept);
@@ -440,18 +627,41 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
             }
             result->n_clusters[mask - 1] = ncl;
             const int64_t nS = part_s->n_clusters;
-            if (nS * K > S_cap) {
-                if (S) (void)hipFree(S), db.v.pop_back();
-                S_cap = nS * K;
-                if ((e = db.get(&S, (size_t)S_cap))) return e;
+            // a partition of singletons needs no row order: sum_i T_i T_i'
+            const bool singletons = nS == n;
+            const int* ord = singletons ? nullptr : part_s->order;
+            const int64_t* off = singletons ? nullptr : part_s->off;
+            if (K <= 12 && singletons) {
+                if ((e = meat_fused(T, K, K, nullptr, nullptr, n, mpart, M))) return e;
+            } else if (K <= 12) {
+                const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
+                if (nS * KT > S_cap) {
+                    if (S) (void)hipFree(S), db.v.pop_back();
+                    S_cap = nS * KT;
+                    if ((e = db.get(&S, (size_t)S_cap))) return e;
+                }
+                if (KT == 4)
+                    e = cluster_sums<4>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
+                else if (KT == 8)
+                    e = cluster_sums<8>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
+                else
+                    e = cluster_sums<12>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
+                if (e) return e;
+                if ((e = meat_fused(S, K, KT, nullptr, nullptr, nS, mpart, M))) return e;
+            } else {
+                if (nS * K > S_cap) {
+                    if (S) (void)hipFree(S), db.v.pop_back();
+                    S_cap = nS * K;
+                    if ((e = db.get(&S, (size_t)S_cap))) return e;
+                }
+                hipLaunchKernelGGL(segsum_kernel, dim3(stream_grid((size_t)nS * K, kBlock)), dim3(kBlock), 0, 0,
+                                   (const double*)T, K, ord, off, nS, S);
+                FEHIP_LAUNCH_CHECK("segsum");
+                if ((e = gemm_impl(false, true, K, K, nS, 1.0, S, K, S, K, 0.0, Md, K, true))) return e;
+                FEHIP_CHECK(hipMemcpy(M.data(), Md, M.size() * sizeof(double), hipMemcpyDeviceToHost), "meat copy");
+                for (int j = 0; j < K; ++j)
+                    for (int i = j + 1; i < K; ++i) M[(size_t)j * K + i] = M[(size_t)i * K + j];
             }
-            hipLaunchKernelGGL(segsum_kernel, dim3(stream_grid((size_t)nS * K, kBlock)), dim3(kBlock), 0, 0,
-                               (const double*)T, K, (const int*)part_s->order, (const int64_t*)part_s->off, nS, S);
-            FEHIP_LAUNCH_CHECK("segsum");
-            if ((e = gemm_impl(false, true, K, K, nS, 1.0, S, K, S, K, 0.0, Md, K, true))) return e;
-            FEHIP_CHECK(hipMemcpy(M.data(), Md, M.size() * sizeof(double), hipMemcpyDeviceToHost), "meat copy");
-            for (int j = 0; j < K; ++j)
-                for (int i = j + 1; i < K; ++i) M[(size_t)j * K + i] = M[(size_t)i * K + j];
             // cov += w Qinv M_kept Qinv
             for (int a = 0; a < kept; ++a)
                 for (int c = 0; c < kept; ++c) {

@@ -2,6 +2,7 @@
 // src/core/fe_grouping.f90:11-37 requires), orders rows by the primary dim with a stable
 // radix sort, and builds the per-segment run structures and cached cluster partitions.
 #include <cstring>
+#include <vector>
 
 #include <rocprim/rocprim.hpp>
 
@@ -169,8 +170,18 @@ int counts_to_offsets(const int* counts, int64_t G, int64_t total, int64_t* off,
 
 void free_panel(Panel* p) {
     if (!p) return;
+    // partitions first: the primary-dim partition aliases seg_off
+    for (auto& kv : p->partitions) {
+        Partition& q = kv.second;
+        dfree(q.order);
+        if (q.off != p->seg_off) dfree(q.off);
+        if (q.keys != p->d[p->P].ids) dfree(q.keys);
+    }
+    p->partitions.clear();
     dfree(p->perm);
     dfree(p->seg_off);
+    dfree(p->tile_seg);
+    dfree(p->long_segs);
     for (int d = 0; d < p->m; ++d) {
         dfree(p->d[d].ids);
         dfree(p->d[d].counts);
@@ -178,12 +189,6 @@ void free_panel(Panel* p) {
         dfree(p->d[d].run_order);
         dfree(p->d[d].grp_run_off);
     }
-    for (auto& kv : p->partitions) {
-        Partition& q = kv.second;
-        dfree(q.order);
-        if (q.off != p->seg_off) dfree(q.off);
-    }
-    p->partitions.clear();
     delete p;
 }
 
@@ -253,6 +258,40 @@ int build_runs(Panel* p, int d, hipStream_t s) {
     return 0;
 }
 
+// Greedy tiles of whole primary segments with at most kTileRows rows; longer segments are listed
+// separately.
+int build_tiles(Panel* p) {
+    const int64_t GP = p->d[p->P].G;
+    std::vector<int64_t> off((size_t)GP + 1);
+    FEHIP_CHECK(hipMemcpy(off.data(), p->seg_off, sizeof(int64_t) * (GP + 1), hipMemcpyDeviceToHost), "seg_off copy");
+    std::vector<int64_t> tiles, longs;
+    tiles.reserve((size_t)(p->n / kTileRows + 16) * 2);
+    int64_t s = 0;
+    while (s < GP) {
+        if (off[s + 1] - off[s] > kTileRows) {
+            longs.push_back(s);
+            ++s;
+            continue;
+        }
+        int64_t e = s + 1;
+        while (e < GP && off[e + 1] - off[s] <= kTileRows && e - s < kTileRows) ++e;
+        tiles.push_back(s);
+        tiles.push_back(e);
+        s = e;
+    }
+    p->n_tiles = (int64_t)tiles.size() / 2;
+    p->n_long = (int64_t)longs.size();
+    int st;
+    if ((st = dalloc(&p->tile_seg, tiles.size() + 2, &p->bytes)) || (st = dalloc(&p->long_segs, longs.size() + 1, &p->bytes)))
+        return st;
+    if (!tiles.empty())
+        FEHIP_CHECK(hipMemcpy(p->tile_seg, tiles.data(), tiles.size() * sizeof(int64_t), hipMemcpyHostToDevice), "tiles");
+    if (!longs.empty())
+        FEHIP_CHECK(hipMemcpy(p->long_segs, longs.data(), longs.size() * sizeof(int64_t), hipMemcpyHostToDevice),
+                    "long segments");
+    return 0;
+}
+
 }  // namespace
 
 int panel_partition(Panel* p, unsigned mask, const Partition** out, hipStream_t s) {
@@ -266,6 +305,7 @@ int panel_partition(Panel* p, unsigned mask, const Partition** out, hipStream_t 
         part.n_clusters = p->d[p->P].G;
         part.order = nullptr;
         part.off = p->seg_off;
+        part.keys = p->d[p->P].ids;
     } else {
         KeyDims kd{};
         int nd = 0;
@@ -319,6 +359,8 @@ int panel_partition(Panel* p, unsigned mask, const Partition** out, hipStream_t 
         if ((st = dalloc(&part.order, (size_t)n, &p->bytes))) return st;
         FEHIP_CHECK(hipMemcpyAsync(part.order, vb.current(), sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, s),
                     "order copy");
+        if ((st = dalloc(&part.keys, (size_t)n, &p->bytes))) return st;
+        FEHIP_CHECK(hipMemcpyAsync(part.keys, cid, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, s), "keys copy");
         FEHIP_CHECK(hipStreamSynchronize(s), "partition sync");
         dfree(k0);
         dfree(k1);
@@ -337,6 +379,7 @@ using namespace fehip;
 extern "C" {
 
 int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehip_panel** out) {
+    (void)hipGetLastError();  // start from a clean error state
     if (!out) return set_error("fehip_panel_create: null output");
     *out = nullptr;
     if (!d_fe_ids || n_fe <= 0 || n_fe > FEHIP_MAX_FE) return set_error("fehip_panel_create: 1..%d FE dims", FEHIP_MAX_FE);
@@ -427,6 +470,10 @@ int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehi
     }
     FEHIP_CHECK(hipMemcpy(&p->max_seg, maxseg, sizeof(int), hipMemcpyDeviceToHost), "max seg");
     dfree(maxseg);
+    if ((st = build_tiles(p))) {
+        free_panel(p);
+        return st;
+    }
     for (int d = 0; d < n_fe; ++d) {
         if (d == p->P || p->d[d].small) continue;
         if ((st = build_runs(p, d, s))) {

@@ -12,6 +12,9 @@ namespace fehip {
 
 // Small dims keep their whole group table in LDS; larger dims scatter through runs.
 constexpr int kSmallMaxGroups = 1024;
+// Rows per tile: a tile is a run of whole primary segments holding at most kTileRows rows,
+// processed by one 256-thread workgroup with one row per thread.
+constexpr int kTileRows = 256;
 
 struct DimStruct {
     int G = 0;                 // groups (max id)
@@ -30,6 +33,7 @@ struct Partition {
     int64_t n_clusters = 0;
     int* order = nullptr;      // nullptr => identity (panel order)
     int64_t* off = nullptr;    // [n_clusters + 1]
+    int* keys = nullptr;       // cluster of each sorted position (grouped, nondecreasing)
 };
 
 struct Panel {
@@ -40,6 +44,10 @@ struct Panel {
     int max_seg = 0;
     int* perm = nullptr;       // panel row -> caller row (nullptr if identity)
     int64_t* seg_off = nullptr;  // [G_P + 1]
+    int64_t n_tiles = 0;
+    int64_t* tile_seg = nullptr;   // [2 * n_tiles]: first and one-past-last segment of each tile
+    int64_t n_long = 0;
+    int64_t* long_segs = nullptr;  // segments longer than kTileRows (handled one thread per segment)
     DimStruct d[FEHIP_MAX_FE];
     std::map<unsigned, Partition> partitions;  // key: bit mask of 0-based dims
     int64_t bytes = 0;

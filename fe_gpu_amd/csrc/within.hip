@@ -30,7 +30,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxScatter = 6;
 constexpr int kMaxLarge = 3;
 
-enum Mode { kIter = 0, kData = 1, kDiag = 2 };
+enum Mode { kIter = 0, kData = 1, kDiag = 2, kFinal = 3 };
 
 struct ScatterDim {
     const int* ids;
@@ -57,7 +57,14 @@ struct ApplyArgs {
     double* oc[16];         // finalize: output columns
     int ncols;              // valid columns in this tile
     const int* perm;
-    double* small_part;     // [grid][Gs][CT]
+    double* small_part;     // [blocks][Gs][CT]
+    const int64_t* tile_seg;   // [2 * n_tiles] segment range of each tile
+    int64_t n_tiles;
+    const int64_t* long_segs;  // segments longer than a tile
+    int64_t n_long;
+    int part_base;             // first partial block written by this launch
+    const double* alpha;       // finalize: solved tables [S][CT]
+    int to_work;               // finalize: 1 = panel-order work columns, 0 = caller columns in place
 };
 
 template <int CT, int MODE>
@@ -102,7 +109,8 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs a) {
     }
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < a.GP; s += stride) {
+    for (int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; li < a.n_long; li += stride) {
+        const int64_t s = a.long_segs[li];
         const int64_t r0 = a.seg_off[s], r1 = a.seg_off[s + 1];
         double mean[CT];
 #pragma unroll
@@ -203,8 +211,340 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs a) {
         }
     }
     __syncthreads();
-    double* part = a.small_part + (size_t)blockIdx.x * a.Gs * CT;
+    double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
     for (int e = threadIdx.x; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
+}
+
+
+// Row-tiled kernel: one workgroup per tile of whole primary segments (<= kTileRows rows), one row
+// per thread.  Phase 1 loads each row's values (coalesced) into LDS, phase 2 removes the segment
+// means (one thread per segment, LDS only), phase 3 scatters the rows into the other dims' tables
+// (LDS atomics for small dims; per-run partials for large dims, runs ranked with a block scan) or,
+// for kFinal, writes the transformed rows out.
+template <int CT, int MODE>
+__global__ __launch_bounds__(kBlock) void tile_kernel(ApplyArgs a) {
+    __shared__ double sv[kTileRows * CT];
+    __shared__ int gd[kTileRows];            // kDiag: group of the processed dim
+    __shared__ unsigned char wr[kTileRows];  // row weight (0/1), 0 past the tile end
+    __shared__ unsigned char sb[kTileRows];  // row starts a segment
+    __shared__ unsigned char hf[kTileRows];  // row starts a run (large-dim scatter)
+    __shared__ short rs[kTileRows], re[kTileRows];  // kDiag: segment range of each row
+    __shared__ int wcnt[kBlock / kWave];
+    extern __shared__ double dyn[];
+    double* sacc = dyn;                          // [Gs][CT]
+    double* sp = dyn + (size_t)a.Gs * CT;        // [Gs][CT] staged p (kIter) or alpha (kFinal)
+    const int j = threadIdx.x;
+    const int lane = j & (kWave - 1), wv = j / kWave;
+    for (int e = j; e < a.Gs * CT; e += blockDim.x) sacc[e] = 0.0;
+    if constexpr (MODE == kIter || MODE == kFinal) {
+        const double* tab = MODE == kIter ? a.p : a.alpha;
+        for (int o = 0; o < a.nd; ++o) {
+            const ScatterDim& d = a.sd[o];
+            if (!d.small) continue;
+            for (int e = j; e < d.G * CT; e += blockDim.x) sp[(size_t)d.lds_off * CT + e] = tab[d.tab_off * CT + e];
+        }
+    }
+    __syncthreads();
+    for (int64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+        const int64_t s0 = a.tile_seg[2 * t], s1 = a.tile_seg[2 * t + 1];
+        const int64_t R0 = a.seg_off[s0];
+        const int nr = (int)(a.seg_off[s1] - R0);
+        const int64_t r = R0 + j;
+        const bool valid = j < nr;
+        const unsigned char w = valid ? (a.w ? a.w[r] : 1) : 0;
+        wr[j] = w;
+        sb[j] = 0;
+        // ---- phase 1: row values
+        if constexpr (MODE == kDiag) {
+            gd[j] = valid ? a.sd[a.diag_dim].ids[r] : -1;
+        } else {
+            double v[CT];
+#pragma unroll
+            for (int c = 0; c < CT; ++c) v[c] = 0.0;
+            if (valid) {
+                if constexpr (MODE == kData || MODE == kFinal) {
+                    const int64_t src = a.perm ? a.perm[r] : r;
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) v[c] = c < a.ncols ? a.xc[c][src] : 0.0;
+                }
+                if constexpr (MODE == kIter || MODE == kFinal) {
+                    const double sign = MODE == kIter ? 1.0 : -1.0;
+                    const double* tab = MODE == kIter ? a.p : a.alpha;
+                    for (int o = 0; o < a.nd; ++o) {
+                        const ScatterDim& d = a.sd[o];
+                        const int g = d.ids[r];
+                        if (d.small) {
+                            const size_t base = (size_t)(d.lds_off + g) * CT;
+#pragma unroll
+                            for (int c = 0; c < CT; ++c) v[c] += sign * sp[base + c];
+                        } else {
+                            const int64_t base = (d.tab_off + g) * CT;
+#pragma unroll
+                            for (int c = 0; c < CT; ++c) v[c] += sign * tab[base + c];
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CT; ++c) sv[j * CT + c] = w ? v[c] : 0.0;
+        }
+        __syncthreads();
+        // ---- phase 2: segment means (one thread per segment)
+        for (int64_t k = j; k < s1 - s0; k += blockDim.x) {
+            const int b = (int)(a.seg_off[s0 + k] - R0), e = (int)(a.seg_off[s0 + k + 1] - R0);
+            if (b < e) sb[b] = 1;
+            if constexpr (MODE == kDiag) {
+                for (int q = b; q < e; ++q) {
+                    rs[q] = (short)b;
+                    re[q] = (short)e;
+                }
+            } else {
+                if (a.elim && b < e) {
+                    double acc[CT];
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) acc[c] = 0.0;
+                    int nw = 0;
+                    for (int q = b; q < e; ++q) {
+                        nw += wr[q];
+#pragma unroll
+                        for (int c = 0; c < CT; ++c) acc[c] += sv[q * CT + c];
+                    }
+                    if (nw > 0) {
+                        const double inv = 1.0 / (double)nw;
+                        for (int q = b; q < e; ++q)
+                            if (wr[q])
+#pragma unroll
+                                for (int c = 0; c < CT; ++c) sv[q * CT + c] -= acc[c] * inv;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if constexpr (MODE == kDiag) {
+            // u0: this row's contribution to the Schur diagonal of its group, u1: its weight
+            double u0 = 0.0, u1 = 0.0;
+            if (valid && w) {
+                u1 = 1.0;
+                if (a.elim) {
+                    int cr = 0, nw = 0;
+                    for (int q = rs[j]; q < re[j]; ++q) {
+                        nw += wr[q];
+                        cr += (wr[q] && gd[q] == gd[j]) ? 1 : 0;
+                    }
+                    u0 = 1.0 - (double)cr / (double)nw;
+                } else {
+                    u0 = 1.0;
+                }
+            }
+            sv[j * CT + 0] = u0;
+            if (CT > 1) sv[j * CT + (1 % CT)] = u1;
+            __syncthreads();
+        }
+        // ---- phase 3
+        if constexpr (MODE == kFinal) {
+            if (valid) {
+                const int64_t src = a.perm ? a.perm[r] : r;
+                for (int c = 0; c < a.ncols; ++c) {
+                    if (a.to_work)
+                        a.oc[c][r] = w ? sv[j * CT + c] : 0.0;
+                    else if (w)
+                        a.oc[c][src] = sv[j * CT + c];
+                }
+            }
+        } else {
+            for (int o = 0; o < a.nd; ++o) {
+                if (MODE == kDiag && o != a.diag_dim) continue;
+                const ScatterDim& d = a.sd[o];
+                if (d.small) {
+                    if (valid && w) {
+                        const int g = MODE == kDiag ? gd[j] : d.ids[r];
+                        double* tq = sacc + (size_t)(d.lds_off + g) * CT;
+#pragma unroll
+                        for (int c = 0; c < CT; ++c) lds_add(&tq[c], sv[j * CT + c]);
+                    }
+                    continue;
+                }
+                // runs: a new run starts at a segment start or where the id changes
+                const int g = valid ? (MODE == kDiag ? gd[j] : d.ids[r]) : -1;
+                const int gp = (valid && j > 0) ? (MODE == kDiag ? gd[j - 1] : d.ids[r - 1]) : -1;
+                const bool head = valid && (j == 0 || sb[j] || g != gp);
+                hf[j] = head;
+                const unsigned long long bal = __ballot(head);
+                if (lane == 0) wcnt[wv] = __popcll(bal);
+                __syncthreads();
+                if (head) {
+                    int rank = __popcll(bal & ((1ull << lane) - 1ull));
+                    for (int q = 0; q < wv; ++q) rank += wcnt[q];
+                    double acc[CT];
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) acc[c] = sv[j * CT + c];
+                    for (int q = j + 1; q < nr && !hf[q]; ++q)
+#pragma unroll
+                        for (int c = 0; c < CT; ++c) acc[c] += sv[q * CT + c];
+                    double* dst = d.runbuf + (d.seg_run_off[s0] + rank) * CT;
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) dst[c] = acc[c];
+                }
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (MODE != kFinal) {
+        double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
+        for (int e = j; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
+    }
+}
+
+
+// Wavefront-parallel row tile (kIter, kData, kFinal): one row per thread; segment sums and run
+// sums are shuffle-based segmented reductions over contiguous lanes, joined across the four
+// wavefronts through LDS atomics (at most a few contributions per segment or run).  Run partials
+// of a tile are contiguous in the run buffer and leave in one coalesced store.
+template <int CT, int MODE>
+__global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
+    __shared__ double acc[kTileRows * CT];   // segment sums, then run sums
+    __shared__ float scnt[kTileRows];         // weighted rows per local segment
+    __shared__ int soff[kTileRows + 1];       // local row offsets of the tile's segments
+    __shared__ int wcnt[kBlock / kWave];
+    extern __shared__ double dyn[];
+    double* sacc = dyn;                       // [Gs][CT]
+    double* sp = dyn + (size_t)a.Gs * CT;     // [Gs][CT]
+    const int j = threadIdx.x;
+    const int lane = j & (kWave - 1), wv = j / kWave;
+    for (int e = j; e < a.Gs * CT; e += blockDim.x) sacc[e] = 0.0;
+    if constexpr (MODE == kIter || MODE == kFinal) {
+        const double* tab = MODE == kIter ? a.p : a.alpha;
+        for (int o = 0; o < a.nd; ++o) {
+            const ScatterDim& d = a.sd[o];
+            if (!d.small) continue;
+            for (int e = j; e < d.G * CT; e += blockDim.x) sp[(size_t)d.lds_off * CT + e] = tab[d.tab_off * CT + e];
+        }
+    }
+    __syncthreads();
+    for (int64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+        const int64_t s0 = a.tile_seg[2 * t], s1 = a.tile_seg[2 * t + 1];
+        const int nseg = (int)(s1 - s0);
+        const int64_t R0 = a.seg_off[s0];
+        for (int k = j; k <= nseg; k += blockDim.x) soff[k] = (int)(a.seg_off[s0 + k] - R0);
+        for (int e = j; e < nseg * CT; e += blockDim.x) acc[e] = 0.0;
+        for (int k = j; k < nseg; k += blockDim.x) scnt[k] = 0.f;
+        __syncthreads();
+        const int nr = soff[nseg];
+        const bool valid = j < nr;
+        const int64_t r = R0 + j;
+        const bool on = valid && (!a.w || a.w[r]);
+        // local segment of row j: the last k with soff[k] <= j (empty segments are skipped)
+        int lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (soff[mid] <= j)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const int seg = lo;
+        double v[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) v[c] = 0.0;
+        if (on) {
+            if constexpr (MODE == kData || MODE == kFinal) {
+                const int64_t src = a.perm ? a.perm[r] : r;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) v[c] = c < a.ncols ? a.xc[c][src] : 0.0;
+            }
+            if constexpr (MODE == kIter || MODE == kFinal) {
+                const double sign = MODE == kIter ? 1.0 : -1.0;
+                const double* tab = MODE == kIter ? a.p : a.alpha;
+                for (int o = 0; o < a.nd; ++o) {
+                    const ScatterDim& d = a.sd[o];
+                    const int g = d.ids[r];
+                    if (d.small) {
+                        const size_t base = (size_t)(d.lds_off + g) * CT;
+#pragma unroll
+                        for (int c = 0; c < CT; ++c) v[c] += sign * sp[base + c];
+                    } else {
+                        const int64_t base = (d.tab_off + g) * CT;
+#pragma unroll
+                        for (int c = 0; c < CT; ++c) v[c] += sign * tab[base + c];
+                    }
+                }
+            }
+        }
+        if (a.elim) {
+            bool head;
+            const int se = wave_run_end(valid ? seg : -1 - lane, lane, &head);
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                const double sm = wave_run_sum(v[c], lane, se);
+                if (head && valid) lds_add(&acc[seg * CT + c], sm);
+            }
+            const double cn = wave_run_sum(on ? 1.0 : 0.0, lane, se);
+            if (head && valid) lds_addf(&scnt[seg], (float)cn);
+            __syncthreads();
+            const float nw = valid ? scnt[seg] : 0.f;
+#pragma unroll
+            for (int c = 0; c < CT; ++c) v[c] = on ? v[c] - (nw > 0.f ? acc[seg * CT + c] / (double)nw : 0.0) : 0.0;
+            __syncthreads();
+        }
+        if constexpr (MODE == kFinal) {
+            if (valid) {
+                const int64_t src = a.perm ? a.perm[r] : r;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) {
+                    if (c >= a.ncols) break;
+                    if (a.to_work)
+                        a.oc[c][r] = v[c];
+                    else if (on)
+                        a.oc[c][src] = v[c];
+                }
+            }
+        } else {
+            for (int o = 0; o < a.nd; ++o) {
+                const ScatterDim& d = a.sd[o];
+                if (d.small) {
+                    if (on) {
+                        double* tq = sacc + (size_t)(d.lds_off + d.ids[r]) * CT;
+#pragma unroll
+                        for (int c = 0; c < CT; ++c) lds_add(&tq[c], v[c]);
+                    }
+                    continue;
+                }
+                // runs: a new run starts at a segment start or where the id changes
+                const int g = valid ? d.ids[r] : -1;
+                int gp = __shfl_up(g, 1, kWave);
+                if (lane == 0) gp = (valid && j > 0) ? d.ids[r - 1] : -1;
+                const bool rhead = valid && (j == 0 || soff[seg] == j || g != gp);
+                const unsigned long long bal = __ballot(rhead);
+                if (lane == 0) wcnt[wv] = __popcll(bal);
+                for (int e = j; e < nr * CT; e += blockDim.x) acc[e] = 0.0;
+                __syncthreads();
+                int base = 0, total = 0;
+                for (int q = 0; q < kBlock / kWave; ++q) {
+                    base += q < wv ? wcnt[q] : 0;
+                    total += wcnt[q];
+                }
+                const unsigned long long upto = lane == kWave - 1 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
+                const int rank = base + __popcll(bal & upto) - 1;
+                bool khead;
+                const int kse = wave_run_end(valid ? rank : -1 - lane, lane, &khead);
+#pragma unroll
+                for (int c = 0; c < CT; ++c) {
+                    const double sm = wave_run_sum(v[c], lane, kse);
+                    if (khead && valid) lds_add(&acc[rank * CT + c], sm);
+                }
+                __syncthreads();
+                double* dst = d.runbuf + d.seg_run_off[s0] * CT;
+                for (int e = j; e < total * CT; e += blockDim.x) dst[e] = acc[e];
+                __syncthreads();
+            }
+        }
+        __syncthreads();
+    }
+    if constexpr (MODE != kFinal) {
+        double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
+        for (int e = j; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
+    }
 }
 
 // q[tab_off + g][c] = sum of the run partials of group g (runs in group order)
@@ -461,7 +801,8 @@ __global__ __launch_bounds__(kBlock) void finalize_kernel(ApplyArgs a, const dou
     }
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < a.GP; s += stride) {
+    for (int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; li < a.n_long; li += stride) {
+        const int64_t s = a.long_segs[li];
         const int64_t r0 = a.seg_off[s], r1 = a.seg_off[s + 1];
         double mean[CT];
 #pragma unroll
@@ -552,23 +893,51 @@ size_t apply_lds(const Layout& L, int mode) {
     return sizeof(double) * (size_t)L.Gs * CT * (mode == kIter ? 2 : 1);
 }
 
-unsigned apply_grid(Panel* p) { return stream_grid((size_t)p->d[p->P].G, kBlock, 8); }
+// dynamic LDS of the tile kernel: small-dim accumulators and staged tables
+template <int CT>
+size_t tile_lds(const Layout& L) {
+    return sizeof(double) * (size_t)L.Gs * CT * 2;
+}
+
+unsigned tile_grid(Panel* p) {
+    const int64_t cap = (int64_t)num_cus() * 8;
+    const int64_t g = p->n_tiles < cap ? p->n_tiles : cap;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+unsigned long_grid(Panel* p) { return p->n_long > 0 ? stream_grid((size_t)p->n_long, kBlock, 2) : 0; }
+
+unsigned apply_grid(Panel* p) { return tile_grid(p) + long_grid(p); }
 
 // Launch one apply (mode) and reduce it into q (table space [S][CT]).
 template <int CT>
 int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* q, hipStream_t s) {
-    const unsigned grid = apply_grid(p);
-    const size_t lds = apply_lds<CT>(L, mode);
+    const unsigned gt = tile_grid(p), gl = long_grid(p);
+    const unsigned grid = gt + gl;
     const int sid = mode == kIter ? kStatApplyIter : kStatApplyData;
     const bool timed = mode != kDiag && kstats_enabled();
     if (timed) kstat_begin(sid, s);
+    a.part_base = 0;
+    const size_t lt = tile_lds<CT>(L);
     if (mode == kIter)
-        hipLaunchKernelGGL((apply_kernel<CT, kIter>), dim3(grid), dim3(kBlock), lds, s, a);
+        hipLaunchKernelGGL((tile_fast_kernel<CT, kIter>), dim3(gt), dim3(kBlock), lt, s, a);
     else if (mode == kData)
-        hipLaunchKernelGGL((apply_kernel<CT, kData>), dim3(grid), dim3(kBlock), lds, s, a);
+        hipLaunchKernelGGL((tile_fast_kernel<CT, kData>), dim3(gt), dim3(kBlock), lt, s, a);
     else
-        hipLaunchKernelGGL((apply_kernel<CT, kDiag>), dim3(grid), dim3(kBlock), lds, s, a);
-    FEHIP_LAUNCH_CHECK("apply kernel");
+        hipLaunchKernelGGL((tile_kernel<CT, kDiag>), dim3(gt), dim3(kBlock), lt, s, a);
+    FEHIP_LAUNCH_CHECK("tile kernel");
+    if (gl > 0) {
+        ApplyArgs b = a;
+        b.part_base = (int)gt;
+        const size_t lds = apply_lds<CT>(L, mode);
+        if (mode == kIter)
+            hipLaunchKernelGGL((apply_kernel<CT, kIter>), dim3(gl), dim3(kBlock), lds, s, b);
+        else if (mode == kData)
+            hipLaunchKernelGGL((apply_kernel<CT, kData>), dim3(gl), dim3(kBlock), lds, s, b);
+        else
+            hipLaunchKernelGGL((apply_kernel<CT, kDiag>), dim3(gl), dim3(kBlock), lds, s, b);
+        FEHIP_LAUNCH_CHECK("long-segment kernel");
+    }
     if (timed) {
         // algorithmic bytes: id streams, segment offsets, weights, table reads, run-partial and
         // LDS-partial writes (+ the data columns and permutation for the right-hand side)
@@ -652,6 +1021,11 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const double* const* xcol
     a.ncols = ncols;
     a.perm = p->perm;
     a.small_part = sc.small_part;
+    a.tile_seg = p->tile_seg;
+    a.n_tiles = p->n_tiles;
+    a.long_segs = p->long_segs;
+    a.n_long = p->n_long;
+    a.to_work = to_work;
     double* xa = sc.tab;           // alpha
     double* r = xa + S * CT;
     double* z = r + S * CT;
@@ -726,10 +1100,16 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const double* const* xcol
     }
     // final pass over the columns
     {
-        const size_t lds = sizeof(double) * (size_t)L.Gs * CT;
         if (kstats_enabled()) kstat_begin(kStatFinalize, s);
-        hipLaunchKernelGGL((finalize_kernel<CT>), dim3(grid), dim3(kBlock), lds, s, a, (const double*)xa, to_work);
-        FEHIP_LAUNCH_CHECK("finalize kernel");
+        a.alpha = xa;
+        hipLaunchKernelGGL((tile_fast_kernel<CT, kFinal>), dim3(tile_grid(p)), dim3(kBlock), tile_lds<CT>(L), s, a);
+        FEHIP_LAUNCH_CHECK("finalize tile kernel");
+        if (p->n_long > 0) {
+            const size_t lds = sizeof(double) * (size_t)L.Gs * CT;
+            hipLaunchKernelGGL((finalize_kernel<CT>), dim3(long_grid(p)), dim3(kBlock), lds, s, a, (const double*)xa,
+                               to_work);
+            FEHIP_LAUNCH_CHECK("finalize kernel");
+        }
         if (kstats_enabled()) {
             double bytes = 16.0 * (double)p->n * ncols + 4.0 * (double)p->n * L.nd + 8.0 * (double)(a.GP + 1) +
                            (a.w ? (double)p->n : 0.0) + (p->perm ? 4.0 * (double)p->n : 0.0);
@@ -761,10 +1141,19 @@ int within_transform(Panel* p, const double* const* xcols, int ncols, const uint
                            p->perm, n, sc.wpanel);
         FEHIP_LAUNCH_CHECK("weights");
     }
-    // tile width: widest power of two <= 16 whose LDS tables fit 64 KB
+    // column-tile width: widest of {16, 12, 9, 8, 6, 4, 3, 2, 1} whose small-dim LDS tables fit
+    // next to the row tile (<= 28 KB of staged + accumulated tables)
     int ct_max = 16;
-    while (ct_max > 1 && (size_t)L.Gs * ct_max * 16 > 64 * 1024) ct_max >>= 1;
-    if ((size_t)L.Gs * 2 * 16 > 64 * 1024) return set_error("small FE dims exceed the LDS budget");
+    {
+        static const int widths[] = {16, 12, 9, 8, 6, 4, 3, 2, 1};
+        ct_max = 0;
+        for (int wdt : widths)
+            if ((size_t)L.Gs * wdt * 16 <= 28 * 1024) {
+                ct_max = wdt;
+                break;
+            }
+        if (ct_max == 0) return set_error("small FE dims exceed the LDS budget (%d groups)", L.Gs);
+    }
     const int64_t S = L.S;
     if ((e = sc.alloc(&sc.tab, (size_t)(S * 16 * 5 + S * 4 + 16)))) return e;
     for (int i = 0; i < L.nd; ++i)
@@ -820,6 +1209,7 @@ using namespace fehip;
 
 extern "C" int fehip_demean(fehip_panel* panel, double* d_cols, int n_cols, int64_t ld, const uint8_t* d_weights,
                             const fehip_demean_opts* opts, fehip_demean_stats* stats) {
+    (void)hipGetLastError();  // start from a clean error state
     if (!panel || !opts) return set_error("fehip_demean: null argument");
     if (n_cols < 0 || (n_cols > 0 && (!d_cols || ld < ((Panel*)panel)->n))) return set_error("fehip_demean: bad columns");
     Panel* p = reinterpret_cast<Panel*>(panel);
