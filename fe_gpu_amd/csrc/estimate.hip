@@ -121,6 +121,13 @@ __global__ __launch_bounds__(kBlock) void count_kernel(const int* __restrict__ f
     if ((threadIdx.x & (kWave - 1)) == 0) atomicAdd(out, c);
 }
 
+// mark[key] = 1 for clusters (sorted-position keys) holding a weighted row
+__global__ void mark_kernel(const int* __restrict__ keys, const int* __restrict__ order, const uint8_t* __restrict__ w,
+                            int64_t n, int* mark) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        if (w[order ? order[k] : k]) mark[keys[k]] = 1;
+}
+
 __global__ void weights_panel_kernel(const uint8_t* __restrict__ w, const int* __restrict__ perm, int64_t n,
                                      uint8_t* out) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -268,6 +275,216 @@ int cluster_sums(const double* T, int K, const int* order, const int* keys, int6
                                        rocprim::equal_to<int>(), s),
                 "reduce_by_key");
     return 0;
+}
+
+// ---- cluster meat by subset class -------------------------------------------------------------
+// Subsets containing the primary dim: every cluster lies inside one primary segment, so a tile of
+// whole segments (<= 256 rows, staged in LDS) holds each of its clusters entirely.  A row that is
+// the first of its cluster in the segment sums the cluster's rows and adds S S' to its registers.
+struct NestKeys {
+    const int* ids[3];  // other dims of the subset (panel order)
+    int nk;
+};
+
+template <int KT>
+__global__ __launch_bounds__(kBlock) void nested_meat_kernel(const double* __restrict__ T, int K,
+                                                             const int64_t* __restrict__ seg_off,
+                                                             const int64_t* __restrict__ tile_seg, int64_t n_tiles,
+                                                             NestKeys nk, const uint8_t* __restrict__ w, double* part,
+                                                             unsigned long long* n_clusters) {
+    unsigned long long nheads = 0;
+    constexpr int NP = KT * (KT + 1) / 2;
+    __shared__ double ts[kTileRows * KT];
+    __shared__ int key[3][kTileRows];
+    __shared__ int soff[kTileRows + 1];
+    __shared__ unsigned char wt[kTileRows];
+    double acc[NP];
+#pragma unroll
+    for (int e = 0; e < NP; ++e) acc[e] = 0.0;
+    const int j = threadIdx.x;
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int64_t s0 = tile_seg[2 * t], s1 = tile_seg[2 * t + 1];
+        const int nseg = (int)(s1 - s0);
+        const int64_t R0 = seg_off[s0];
+        __syncthreads();
+        for (int k = j; k <= nseg; k += blockDim.x) soff[k] = (int)(seg_off[s0 + k] - R0);
+        const int nr = (int)(seg_off[s1] - R0);
+        for (int e = j; e < nr * K; e += blockDim.x) {  // coalesced copy of the tile's T rows
+            const int q = e / K, c = e - q * K;
+            ts[q * KT + c] = T[R0 * K + e];
+        }
+        if (j < nr) {
+            for (int d = 0; d < nk.nk; ++d) key[d][j] = nk.ids[d][R0 + j];
+            wt[j] = w ? w[R0 + j] : 1;
+        }
+        __syncthreads();
+        if (j < nr && wt[j]) {
+            int lo = 0, hi = nseg - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (soff[mid] <= j)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            const int b = soff[lo], f = soff[lo + 1];
+            auto same = [&](int q) {
+                for (int d = 0; d < nk.nk; ++d)
+                    if (key[d][q] != key[d][j]) return false;
+                return true;
+            };
+            bool first = true;
+            for (int q = b; q < j && first; ++q) first = !(wt[q] && same(q));
+            if (first) {
+                ++nheads;
+                double sc[KT];
+#pragma unroll
+                for (int c = 0; c < KT; ++c) sc[c] = 0.0;
+                for (int q = j; q < f; ++q)
+                    if (same(q))
+#pragma unroll
+                        for (int c = 0; c < KT; ++c) sc[c] += c < K ? ts[q * KT + c] : 0.0;
+                int idx = 0;
+#pragma unroll
+                for (int a = 0; a < KT; ++a)
+#pragma unroll
+                    for (int bb = a; bb < KT; ++bb) acc[idx++] += sc[a] * sc[bb];
+            }
+        }
+    }
+    __shared__ double ws[kBlock / kWave][NP];
+    const int wv = j / kWave, ln = j & (kWave - 1);
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        const double v = wave_sum(acc[e]);
+        if (ln == 0) ws[wv][e] = v;
+    }
+    __syncthreads();
+    for (int e = j; e < NP; e += blockDim.x) {
+        double v = 0.0;
+        for (int q = 0; q < kBlock / kWave; ++q) v += ws[q][e];
+        part[(int64_t)blockIdx.x * NP + e] = v;
+    }
+    for (int off = kWave / 2; off > 0; off >>= 1) nheads += __shfl_xor(nheads, off, kWave);
+    if (ln == 0) atomicAdd(n_clusters, nheads);
+}
+
+// Subsets of small dims only: cluster = mixed-radix index of the small ids (<= kSmallCells cells);
+// per-workgroup LDS tables of T sums, written as block partials.
+constexpr int kSmallCells = 1024;
+struct SmallKeys {
+    const int* ids[3];
+    int stride[3];
+    int nk;
+    const uint8_t* w;
+};
+
+template <int KT>
+__global__ __launch_bounds__(kBlock) void small_scores_kernel(const double* __restrict__ T, int K, int64_t n,
+                                                              SmallKeys sk, int cells, double* part) {
+    extern __shared__ double tab[];  // [cells][KT]
+    for (int e = threadIdx.x; e < cells * KT; e += blockDim.x) tab[e] = 0.0;
+    __syncthreads();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int cell = 0;
+        for (int d = 0; d < sk.nk; ++d) cell += sk.ids[d][i] * sk.stride[d];
+        for (int c = 0; c < K; ++c) lds_add(&tab[cell * KT + c], T[i * K + c]);
+        if (!sk.w || sk.w[i]) lds_add(&tab[cell * KT + (KT - 1)], 1.0);  // occupancy (KT > K)
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < cells * KT; e += blockDim.x) part[(int64_t)blockIdx.x * cells * KT + e] = tab[e];
+}
+
+__global__ __launch_bounds__(kBlock) void sum_blocks_kernel(const double* __restrict__ part, int nblk, int width,
+                                                            double* out) {
+    __shared__ double ws[kBlock / kWave];
+    const int e = blockIdx.x;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(int64_t)b * width + e];
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += ws[w];
+        out[e] = t;
+    }
+}
+
+// One large dim (not the primary): sums of T over the runs of equal id inside each primary tile,
+// written contiguously per tile (the same run numbering the panel built), then reduced in group order.
+template <int KT>
+__global__ __launch_bounds__(kBlock) void run_scores_kernel(const double* __restrict__ T, int K,
+                                                            const int64_t* __restrict__ seg_off,
+                                                            const int64_t* __restrict__ tile_seg, int64_t n_tiles,
+                                                            const int* __restrict__ ids,
+                                                            const int64_t* __restrict__ seg_run_off, double* runbuf) {
+    __shared__ double ts[kTileRows * KT];
+    __shared__ int soff[kTileRows + 1];
+    __shared__ int rstart[kTileRows + 1];
+    __shared__ int wcnt[kBlock / kWave];
+    const int j = threadIdx.x, lane = j & (kWave - 1), wv = j / kWave;
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int64_t s0 = tile_seg[2 * t], s1 = tile_seg[2 * t + 1];
+        const int nseg = (int)(s1 - s0);
+        const int64_t R0 = seg_off[s0];
+        __syncthreads();
+        for (int k = j; k <= nseg; k += blockDim.x) soff[k] = (int)(seg_off[s0 + k] - R0);
+        const int nr = (int)(seg_off[s1] - R0);
+        for (int e = j; e < nr * K; e += blockDim.x) {
+            const int q = e / K, c = e - q * K;
+            ts[q * KT + c] = T[R0 * K + e];
+        }
+        __syncthreads();
+        const bool valid = j < nr;
+        const int g = valid ? ids[R0 + j] : -1;
+        int gp = __shfl_up(g, 1, kWave);
+        if (lane == 0) gp = (valid && j > 0) ? ids[R0 + j - 1] : -1;
+        bool segstart = false;
+        if (valid) {
+            int lo = 0, hi = nseg - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (soff[mid] <= j)
+                    lo = mid;
+                else
+                    hi = mid - 1;
+            }
+            segstart = soff[lo] == j;
+        }
+        const bool head = valid && (j == 0 || segstart || g != gp);
+        const unsigned long long bal = __ballot(head);
+        if (lane == 0) wcnt[wv] = __popcll(bal);
+        __syncthreads();
+        int base = 0, total = 0;
+        for (int q = 0; q < kBlock / kWave; ++q) {
+            base += q < wv ? wcnt[q] : 0;
+            total += wcnt[q];
+        }
+        if (head) rstart[base + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+        if (j == 0) rstart[total] = nr;
+        __syncthreads();
+        double* dst = runbuf + seg_run_off[s0] * KT;
+        for (int e = j; e < total * KT; e += blockDim.x) {
+            const int k = e / KT, c = e - k * KT;
+            double sm = 0.0;
+            if (c < K)
+                for (int q = rstart[k]; q < rstart[k + 1]; ++q) sm += ts[q * KT + c];
+            dst[e] = sm;
+        }
+    }
+}
+
+template <int KT>
+__global__ void group_runs_kernel(const double* __restrict__ runbuf, const int* __restrict__ order,
+                                  const int64_t* __restrict__ goff, int G, double* S) {
+    const int64_t total = (int64_t)G * KT;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int g = (int)(e / KT), c = (int)(e % KT);
+        double sm = 0.0;
+        for (int64_t k = goff[g]; k < goff[g + 1]; ++k) sm += runbuf[(int64_t)order[k] * KT + c];
+        S[e] = sm;
+    }
 }
 
 constexpr int kMeatBlocks = 1024;
@@ -604,7 +821,8 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
         double* S = nullptr;
         int64_t S_cap = 0;
         int64_t* dcount = nullptr;
-        if ((e = db.get(&dcount, 1))) return e;
+        unsigned long long* ucount = nullptr;
+        if ((e = db.get(&dcount, 1)) || (e = db.get(&ucount, 1))) return e;
         double* Md = nullptr;
         if ((e = db.get(&Md, (size_t)K * K))) return e;
         std::vector<double> M((size_t)K * K), tmp((size_t)kept * kept);
@@ -618,50 +836,164 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
                     ++len;
                 }
             const double wgt = (len % 2 == 1) ? 1.0 : -1.0;
-            const Partition* part_s = nullptr;
-            if ((e = panel_partition(p, dims_mask, &part_s, 0))) return e;
-            int64_t ncl = part_s->n_clusters;
+            // route the subset to the cheapest exact method (see the kernels above)
+            bool has_p = (dims_mask >> p->P) & 1u;
+            bool all_small = true;
+            int64_t cells = 1;
+            for (int d = 0; d < p->m; ++d)
+                if ((dims_mask >> d) & 1u) {
+                    all_small &= p->d[d].small != 0;
+                    cells *= p->d[d].G;
+                }
+            const int single = len == 1 ? __builtin_ctz(dims_mask) : -1;
+            int64_t ncl = 0;
+            if (has_p && p->n_long == 0 && K <= 12) {
+                NestKeys nk{};
+                for (int d = 0; d < p->m; ++d)
+                    if (d != p->P && ((dims_mask >> d) & 1u)) nk.ids[nk.nk++] = p->d[d].ids;
+                FEHIP_CHECK(hipMemsetAsync(ucount, 0, sizeof(unsigned long long), 0), "memset");
+                const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(p->n_tiles, kMeatBlocks));
+                const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
+                if (KT == 4)
+                    hipLaunchKernelGGL(nested_meat_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, T, K, p->seg_off,
+                                       p->tile_seg, p->n_tiles, nk, wpanel, mpart, ucount);
+                else if (KT == 8)
+                    hipLaunchKernelGGL(nested_meat_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, T, K, p->seg_off,
+                                       p->tile_seg, p->n_tiles, nk, wpanel, mpart, ucount);
+                else
+                    hipLaunchKernelGGL(nested_meat_kernel<12>, dim3(grid), dim3(kBlock), 0, 0, T, K, p->seg_off,
+                                       p->tile_seg, p->n_tiles, nk, wpanel, mpart, ucount);
+                FEHIP_LAUNCH_CHECK("nested meat");
+                if ((e = reduce_upper(mpart, (int)grid, KT, K, M))) return e;
+                unsigned long long h = 0;
+                FEHIP_CHECK(hipMemcpy(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
+                ncl = (int64_t)h;
+            } else if (all_small && cells <= kSmallCells && K <= 11) {
+                SmallKeys sk{};
+                int stride = 1;
+                for (int d = 0; d < p->m; ++d)
+                    if ((dims_mask >> d) & 1u) {
+                        sk.ids[sk.nk] = p->d[d].ids;
+                        sk.stride[sk.nk++] = stride;
+                        stride *= p->d[d].G;
+                    }
+                sk.w = wpanel;
+                const int KT = K <= 3 ? 4 : (K <= 7 ? 8 : 12);
+                const int c = (int)cells;
+                const unsigned grid = stream_grid((size_t)n, kBlock, 2);
+                double* sp_ = nullptr;
+                if ((e = db.get(&sp_, (size_t)grid * c * KT + (size_t)c * KT))) return e;
+                const size_t lds = sizeof(double) * (size_t)c * KT;
+                if (KT == 4)
+                    hipLaunchKernelGGL(small_scores_kernel<4>, dim3(grid), dim3(kBlock), lds, 0, T, K, n, sk, c, sp_);
+                else if (KT == 8)
+                    hipLaunchKernelGGL(small_scores_kernel<8>, dim3(grid), dim3(kBlock), lds, 0, T, K, n, sk, c, sp_);
+                else
+                    hipLaunchKernelGGL(small_scores_kernel<12>, dim3(grid), dim3(kBlock), lds, 0, T, K, n, sk, c, sp_);
+                double* Sc = sp_ + (size_t)grid * c * KT;
+                hipLaunchKernelGGL(sum_blocks_kernel, dim3((unsigned)(c * KT)), dim3(kBlock), 0, 0, sp_, (int)grid,
+                                   c * KT, Sc);
+                FEHIP_LAUNCH_CHECK("small scores");
+                std::vector<double> hS((size_t)c * KT);
+                FEHIP_CHECK(hipMemcpy(hS.data(), Sc, hS.size() * sizeof(double), hipMemcpyDeviceToHost), "S copy");
+                M.assign((size_t)K * K, 0.0);
+                for (int q = 0; q < c; ++q) {
+                    if (hS[(size_t)q * KT + KT - 1] > 0.0) ++ncl;
+                    for (int a2 = 0; a2 < K; ++a2)
+                        for (int b2 = 0; b2 < K; ++b2)
+                            M[(size_t)b2 * K + a2] += hS[(size_t)q * KT + a2] * hS[(size_t)q * KT + b2];
+                }
+            } else if (single >= 0 && single != p->P && !p->d[single].small && p->n_long == 0 && K <= 12) {
+                const DimStruct& D = p->d[single];
+                const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
+                double *rb = nullptr, *Sg = nullptr;
+                if ((e = db.get(&rb, (size_t)D.n_runs * KT + 1)) || (e = db.get(&Sg, (size_t)D.G * KT))) return e;
+                const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(p->n_tiles, (int64_t)num_cus() * 8));
+                if (KT == 4)
+                    hipLaunchKernelGGL(run_scores_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, T, K, p->seg_off,
+                                       p->tile_seg, p->n_tiles, D.ids, D.seg_run_off, rb);
+                else if (KT == 8)
+                    hipLaunchKernelGGL(run_scores_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, T, K, p->seg_off,
+                                       p->tile_seg, p->n_tiles, D.ids, D.seg_run_off, rb);
+                else
+                    hipLaunchKernelGGL(run_scores_kernel<12>, dim3(grid), dim3(kBlock), 0, 0, T, K, p->seg_off,
+                                       p->tile_seg, p->n_tiles, D.ids, D.seg_run_off, rb);
+                FEHIP_LAUNCH_CHECK("run scores");
+                const size_t tot = (size_t)D.G * KT;
+                if (KT == 4)
+                    hipLaunchKernelGGL(group_runs_kernel<4>, dim3(stream_grid(tot, kBlock)), dim3(kBlock), 0, 0, rb,
+                                       D.run_order, D.grp_run_off, D.G, Sg);
+                else if (KT == 8)
+                    hipLaunchKernelGGL(group_runs_kernel<8>, dim3(stream_grid(tot, kBlock)), dim3(kBlock), 0, 0, rb,
+                                       D.run_order, D.grp_run_off, D.G, Sg);
+                else
+                    hipLaunchKernelGGL(group_runs_kernel<12>, dim3(stream_grid(tot, kBlock)), dim3(kBlock), 0, 0, rb,
+                                       D.run_order, D.grp_run_off, D.G, Sg);
+                FEHIP_LAUNCH_CHECK("group runs");
+                if ((e = meat_fused(Sg, K, KT, nullptr, nullptr, D.G, mpart, M))) return e;
+                ncl = Geff[single];
+            } else {
+                const Partition* part_s = nullptr;
+                if ((e = panel_partition(p, dims_mask, &part_s, 0))) return e;
+                ncl = part_s->n_clusters;
+                const int64_t nS = part_s->n_clusters;
+                // a partition of singletons needs no row order: sum_i T_i T_i'
+                const bool singletons = nS == n;
+                const int* ord = singletons ? nullptr : part_s->order;
+                const int64_t* off = singletons ? nullptr : part_s->off;
+                if (K <= 12 && singletons) {
+                    if ((e = meat_fused(T, K, K, nullptr, nullptr, n, mpart, M))) return e;
+                } else if (K <= 12) {
+                    const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
+                    if (nS * KT > S_cap) {
+                        if (S) (void)hipFree(S), db.v.pop_back();
+                        S_cap = nS * KT;
+                        if ((e = db.get(&S, (size_t)S_cap))) return e;
+                    }
+                    if (KT == 4)
+                        e = cluster_sums<4>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
+                    else if (KT == 8)
+                        e = cluster_sums<8>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
+                    else
+                        e = cluster_sums<12>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
+                    if (e) return e;
+                    // aggregates exist only for keys present (a primary id with no rows has none)
+                    int64_t n_present = 0;
+                    FEHIP_CHECK(hipMemcpy(&n_present, dcount, sizeof(int64_t), hipMemcpyDeviceToHost), "count copy");
+                    if ((e = meat_fused(S, K, KT, nullptr, nullptr, n_present, mpart, M))) return e;
+                } else {
+                    if (nS * K > S_cap) {
+                        if (S) (void)hipFree(S), db.v.pop_back();
+                        S_cap = nS * K;
+                        if ((e = db.get(&S, (size_t)S_cap))) return e;
+                    }
+                    hipLaunchKernelGGL(segsum_kernel, dim3(stream_grid((size_t)nS * K, kBlock)), dim3(kBlock), 0, 0,
+                                       (const double*)T, K, ord, off, nS, S);
+                    FEHIP_LAUNCH_CHECK("segsum");
+                    if ((e = gemm_impl(false, true, K, K, nS, 1.0, S, K, S, K, 0.0, Md, K, true))) return e;
+                    FEHIP_CHECK(hipMemcpy(M.data(), Md, M.size() * sizeof(double), hipMemcpyDeviceToHost), "meat copy");
+                    for (int j = 0; j < K; ++j)
+                        for (int i = j + 1; i < K; ++i) M[(size_t)j * K + i] = M[(size_t)i * K + j];
+                }
+                if (wpanel) {  // count the clusters that hold a weighted row
+                    int* mark = nullptr;
+                    if ((e = db.get(&mark, (size_t)nS + 2))) return e;
+                    FEHIP_CHECK(hipMemsetAsync(mark, 0, sizeof(int) * ((size_t)nS + 2), 0), "memset");
+                    FEHIP_CHECK(hipMemsetAsync(ucount, 0, sizeof(unsigned long long), 0), "memset");
+                    hipLaunchKernelGGL(mark_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, part_s->keys,
+                                       part_s->order, wpanel, n, mark);
+                    hipLaunchKernelGGL(count_kernel, dim3(stream_grid((size_t)nS + 2, kBlock)), dim3(kBlock), 0, 0,
+                                       mark, (int64_t)nS + 2, ucount);
+                    unsigned long long h = 0;
+                    FEHIP_CHECK(hipMemcpy(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
+                    ncl = (int64_t)h;
+                }
+            }
             if (len == 1) {
-                ncl = Geff[__builtin_ctz(dims_mask)];
+                ncl = Geff[single];
                 min_single = std::min<int64_t>(min_single, ncl);
             }
             result->n_clusters[mask - 1] = ncl;
-            const int64_t nS = part_s->n_clusters;
-            // a partition of singletons needs no row order: sum_i T_i T_i'
-            const bool singletons = nS == n;
-            const int* ord = singletons ? nullptr : part_s->order;
-            const int64_t* off = singletons ? nullptr : part_s->off;
-            if (K <= 12 && singletons) {
-                if ((e = meat_fused(T, K, K, nullptr, nullptr, n, mpart, M))) return e;
-            } else if (K <= 12) {
-                const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
-                if (nS * KT > S_cap) {
-                    if (S) (void)hipFree(S), db.v.pop_back();
-                    S_cap = nS * KT;
-                    if ((e = db.get(&S, (size_t)S_cap))) return e;
-                }
-                if (KT == 4)
-                    e = cluster_sums<4>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
-                else if (KT == 8)
-                    e = cluster_sums<8>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
-                else
-                    e = cluster_sums<12>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
-                if (e) return e;
-                if ((e = meat_fused(S, K, KT, nullptr, nullptr, nS, mpart, M))) return e;
-            } else {
-                if (nS * K > S_cap) {
-                    if (S) (void)hipFree(S), db.v.pop_back();
-                    S_cap = nS * K;
-                    if ((e = db.get(&S, (size_t)S_cap))) return e;
-                }
-                hipLaunchKernelGGL(segsum_kernel, dim3(stream_grid((size_t)nS * K, kBlock)), dim3(kBlock), 0, 0,
-                                   (const double*)T, K, ord, off, nS, S);
-                FEHIP_LAUNCH_CHECK("segsum");
-                if ((e = gemm_impl(false, true, K, K, nS, 1.0, S, K, S, K, 0.0, Md, K, true))) return e;
-                FEHIP_CHECK(hipMemcpy(M.data(), Md, M.size() * sizeof(double), hipMemcpyDeviceToHost), "meat copy");
-                for (int j = 0; j < K; ++j)
-                    for (int i = j + 1; i < K; ++i) M[(size_t)j * K + i] = M[(size_t)i * K + j];
-            }
             // cov += w Qinv M_kept Qinv
             for (int a = 0; a < kept; ++a)
                 for (int c = 0; c < kept; ++c) {

@@ -59,6 +59,20 @@ __global__ void histogram_kernel(const int* __restrict__ ids, int64_t n, int* co
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) atomicAdd(&counts[ids[i]], 1);
 }
 
+// few groups: per-workgroup LDS counts, one global add per group and workgroup
+constexpr int kLdsHistMax = 8192;
+__global__ __launch_bounds__(kBlock) void histogram_lds_kernel(const int* __restrict__ ids, int64_t n, int G,
+                                                               int* counts) {
+    __shared__ int h[kLdsHistMax];
+    for (int g = threadIdx.x; g < G; g += blockDim.x) h[g] = 0;
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) atomicAdd(&h[ids[i]], 1);
+    __syncthreads();
+    for (int g = threadIdx.x; g < G; g += blockDim.x)
+        if (h[g]) atomicAdd(&counts[g], h[g]);
+}
+
 __global__ void widen_kernel(const int* __restrict__ c, int64_t n, int64_t* out, int* maxv) {
     int m = 0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -453,8 +467,12 @@ int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehi
         hipLaunchKernelGGL(gather_ids_kernel, dim3(stream_grid(n_obs, kBlock)), dim3(kBlock), 0, s, d_fe_ids[d],
                            p->perm, n_obs, D.ids);
         FEHIP_CHECK(hipMemsetAsync(D.counts, 0, sizeof(int) * (size_t)D.G, s), "memset");
-        hipLaunchKernelGGL(histogram_kernel, dim3(stream_grid(n_obs, kBlock)), dim3(kBlock), 0, s, D.ids, n_obs,
-                           D.counts);
+        if (D.G <= kLdsHistMax)
+            hipLaunchKernelGGL(histogram_lds_kernel, dim3(stream_grid(n_obs, kBlock, 2)), dim3(kBlock), 0, s, D.ids,
+                               n_obs, D.G, D.counts);
+        else
+            hipLaunchKernelGGL(histogram_kernel, dim3(stream_grid(n_obs, kBlock)), dim3(kBlock), 0, s, D.ids, n_obs,
+                               D.counts);
         FEHIP_LAUNCH_CHECK("histogram");
     }
     const int64_t GP = p->d[p->P].G;

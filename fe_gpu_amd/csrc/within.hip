@@ -53,6 +53,7 @@ struct ApplyArgs {
     int Gs;                 // small groups in LDS
     int diag_dim;           // kDiag: the scatter dim processed
     const double* p;        // kIter: [S][CT]
+    const int* active;      // kIter: per-column CG activity (nullptr = all active)
     const double* xc[16];   // kData: caller columns of this tile (caller row order)
     double* oc[16];         // finalize: output columns
     int ncols;              // valid columns in this tile
@@ -397,15 +398,25 @@ __global__ __launch_bounds__(kBlock) void tile_kernel(ApplyArgs a) {
 }
 
 
-// Wavefront-parallel row tile (kIter, kData, kFinal): one row per thread; segment sums and run
-// sums are shuffle-based segmented reductions over contiguous lanes, joined across the four
-// wavefronts through LDS atomics (at most a few contributions per segment or run).  Run partials
-// of a tile are contiguous in the run buffer and leave in one coalesced store.
+// Row tile (kIter, kData, kFinal): one workgroup per tile of whole primary segments, one row per
+// thread.  Rows are staged in LDS; segment means are taken by one thread per (segment, column)
+// and run sums by one thread per (run, column), so no thread walks a long serial chain and no
+// cross-lane shuffles are needed.  Run partials of a tile are contiguous in the run buffer and
+// leave in one coalesced store; small-dim tables take LDS atomics.
 template <int CT, int MODE>
 __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
-    __shared__ double acc[kTileRows * CT];   // segment sums, then run sums
-    __shared__ float scnt[kTileRows];         // weighted rows per local segment
+    if constexpr (MODE == kIter) {
+        if (a.active) {  // every column converged: nothing to apply (the host polls periodically)
+            bool any = false;
+#pragma unroll
+            for (int c = 0; c < CT; ++c) any |= a.active[c] != 0;
+            if (!any) return;
+        }
+    }
+    __shared__ double sv[kTileRows * CT];     // row values
+    __shared__ unsigned char wr[kTileRows];   // row weight
     __shared__ int soff[kTileRows + 1];       // local row offsets of the tile's segments
+    __shared__ int rstart[kTileRows + 1];     // first row of each run
     __shared__ int wcnt[kBlock / kWave];
     extern __shared__ double dyn[];
     double* sacc = dyn;                       // [Gs][CT]
@@ -421,32 +432,21 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
             for (int e = j; e < d.G * CT; e += blockDim.x) sp[(size_t)d.lds_off * CT + e] = tab[d.tab_off * CT + e];
         }
     }
-    __syncthreads();
     for (int64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
         const int64_t s0 = a.tile_seg[2 * t], s1 = a.tile_seg[2 * t + 1];
         const int nseg = (int)(s1 - s0);
         const int64_t R0 = a.seg_off[s0];
+        __syncthreads();  // previous tile done with the LDS arrays (and the staged tables are ready)
         for (int k = j; k <= nseg; k += blockDim.x) soff[k] = (int)(a.seg_off[s0 + k] - R0);
-        for (int e = j; e < nseg * CT; e += blockDim.x) acc[e] = 0.0;
-        for (int k = j; k < nseg; k += blockDim.x) scnt[k] = 0.f;
-        __syncthreads();
-        const int nr = soff[nseg];
+        const int nr = (int)(a.seg_off[s1] - R0);
         const bool valid = j < nr;
         const int64_t r = R0 + j;
         const bool on = valid && (!a.w || a.w[r]);
-        // local segment of row j: the last k with soff[k] <= j (empty segments are skipped)
-        int lo = 0, hi = nseg - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (soff[mid] <= j)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const int seg = lo;
+        int gid[kMaxScatter];
         double v[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) v[c] = 0.0;
+        for (int o = 0; o < a.nd; ++o) gid[o] = valid ? a.sd[o].ids[r] : -1;
         if (on) {
             if constexpr (MODE == kData || MODE == kFinal) {
                 const int64_t src = a.perm ? a.perm[r] : r;
@@ -458,33 +458,39 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
                 const double* tab = MODE == kIter ? a.p : a.alpha;
                 for (int o = 0; o < a.nd; ++o) {
                     const ScatterDim& d = a.sd[o];
-                    const int g = d.ids[r];
                     if (d.small) {
-                        const size_t base = (size_t)(d.lds_off + g) * CT;
+                        const size_t base = (size_t)(d.lds_off + gid[o]) * CT;
 #pragma unroll
                         for (int c = 0; c < CT; ++c) v[c] += sign * sp[base + c];
                     } else {
-                        const int64_t base = (d.tab_off + g) * CT;
+                        const int64_t base = (d.tab_off + gid[o]) * CT;
 #pragma unroll
                         for (int c = 0; c < CT; ++c) v[c] += sign * tab[base + c];
                     }
                 }
             }
         }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) sv[j * CT + c] = v[c];
+        wr[j] = on ? 1 : 0;
+        __syncthreads();
         if (a.elim) {
-            bool head;
-            const int se = wave_run_end(valid ? seg : -1 - lane, lane, &head);
-#pragma unroll
-            for (int c = 0; c < CT; ++c) {
-                const double sm = wave_run_sum(v[c], lane, se);
-                if (head && valid) lds_add(&acc[seg * CT + c], sm);
+            // one thread per (segment, column): subtract the weighted segment mean
+            for (int e = j; e < nseg * CT; e += blockDim.x) {
+                const int k = e / CT, c = e - k * CT;
+                const int b = soff[k], f = soff[k + 1];
+                double sm = 0.0;
+                int nw = 0;
+                for (int q = b; q < f; ++q) {
+                    sm += sv[q * CT + c];
+                    nw += wr[q];
+                }
+                if (nw > 0) {
+                    const double mean = sm / (double)nw;
+                    for (int q = b; q < f; ++q)
+                        if (wr[q]) sv[q * CT + c] -= mean;
+                }
             }
-            const double cn = wave_run_sum(on ? 1.0 : 0.0, lane, se);
-            if (head && valid) lds_addf(&scnt[seg], (float)cn);
-            __syncthreads();
-            const float nw = valid ? scnt[seg] : 0.f;
-#pragma unroll
-            for (int c = 0; c < CT; ++c) v[c] = on ? v[c] - (nw > 0.f ? acc[seg * CT + c] / (double)nw : 0.0) : 0.0;
             __syncthreads();
         }
         if constexpr (MODE == kFinal) {
@@ -494,9 +500,9 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
                 for (int c = 0; c < CT; ++c) {
                     if (c >= a.ncols) break;
                     if (a.to_work)
-                        a.oc[c][r] = v[c];
+                        a.oc[c][r] = sv[j * CT + c];
                     else if (on)
-                        a.oc[c][src] = v[c];
+                        a.oc[c][src] = sv[j * CT + c];
                 }
             }
         } else {
@@ -504,44 +510,52 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
                 const ScatterDim& d = a.sd[o];
                 if (d.small) {
                     if (on) {
-                        double* tq = sacc + (size_t)(d.lds_off + d.ids[r]) * CT;
+                        double* tq = sacc + (size_t)(d.lds_off + gid[o]) * CT;
 #pragma unroll
-                        for (int c = 0; c < CT; ++c) lds_add(&tq[c], v[c]);
+                        for (int c = 0; c < CT; ++c) lds_add(&tq[c], sv[j * CT + c]);
                     }
                     continue;
                 }
                 // runs: a new run starts at a segment start or where the id changes
-                const int g = valid ? d.ids[r] : -1;
-                int gp = __shfl_up(g, 1, kWave);
+                int gp = __shfl_up(gid[o], 1, kWave);
                 if (lane == 0) gp = (valid && j > 0) ? d.ids[r - 1] : -1;
-                const bool rhead = valid && (j == 0 || soff[seg] == j || g != gp);
+                bool segstart = false;
+                if (valid) {
+                    int lo = 0, hi = nseg - 1;  // last segment whose start is <= j
+                    while (lo < hi) {
+                        const int mid = (lo + hi + 1) >> 1;
+                        if (soff[mid] <= j)
+                            lo = mid;
+                        else
+                            hi = mid - 1;
+                    }
+                    segstart = soff[lo] == j;
+                }
+                const bool rhead = valid && (j == 0 || segstart || gid[o] != gp);
                 const unsigned long long bal = __ballot(rhead);
                 if (lane == 0) wcnt[wv] = __popcll(bal);
-                for (int e = j; e < nr * CT; e += blockDim.x) acc[e] = 0.0;
                 __syncthreads();
                 int base = 0, total = 0;
                 for (int q = 0; q < kBlock / kWave; ++q) {
                     base += q < wv ? wcnt[q] : 0;
                     total += wcnt[q];
                 }
-                const unsigned long long upto = lane == kWave - 1 ? ~0ull : ((1ull << (lane + 1)) - 1ull);
-                const int rank = base + __popcll(bal & upto) - 1;
-                bool khead;
-                const int kse = wave_run_end(valid ? rank : -1 - lane, lane, &khead);
-#pragma unroll
-                for (int c = 0; c < CT; ++c) {
-                    const double sm = wave_run_sum(v[c], lane, kse);
-                    if (khead && valid) lds_add(&acc[rank * CT + c], sm);
-                }
+                if (rhead) rstart[base + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+                if (j == 0) rstart[total] = nr;
                 __syncthreads();
                 double* dst = d.runbuf + d.seg_run_off[s0] * CT;
-                for (int e = j; e < total * CT; e += blockDim.x) dst[e] = acc[e];
+                for (int e = j; e < total * CT; e += blockDim.x) {
+                    const int k = e / CT, c = e - k * CT;
+                    double sm = 0.0;
+                    for (int q = rstart[k]; q < rstart[k + 1]; ++q) sm += sv[q * CT + c];
+                    dst[e] = sm;
+                }
                 __syncthreads();
             }
         }
-        __syncthreads();
     }
     if constexpr (MODE != kFinal) {
+        __syncthreads();
         double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
         for (int e = j; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
     }
@@ -563,13 +577,23 @@ __global__ void reduce_runs_kernel(const double* __restrict__ runbuf, const int*
 
 // q[tab_off + g][c] = sum over blocks of the LDS partials
 template <int CT>
-__global__ void reduce_small_kernel(const double* __restrict__ part, int nblk, int Gs, int lds_off, int G, double* q) {
-    const int total = G * CT;
-    for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < total; e += gridDim.x * blockDim.x) {
-        double s = 0.0;
-        const size_t off = (size_t)lds_off * CT + e;
-        for (int b = 0; b < nblk; ++b) s += part[(size_t)b * Gs * CT + off];
-        q[e] = s;
+__global__ __launch_bounds__(kBlock) void reduce_small_kernel(const double* __restrict__ part, int nblk, int Gs,
+                                                               int lds_off, int G, double* q) {
+    // one workgroup per table entry: the threads stride over the per-block partials in a fixed
+    // order and combine them with a fixed-shape tree
+    __shared__ double ws[kBlock / kWave];
+    const int e = blockIdx.x;
+    if (e >= G * CT) return;
+    const size_t off = (size_t)lds_off * CT + e;
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s += part[(size_t)b * Gs * CT + off];
+    s = wave_sum(s);
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) t += ws[w];
+        q[e] = t;
     }
 }
 
@@ -957,8 +981,8 @@ int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* 
         const ScatterDim& d = L.sd[o];
         double* qo = q + d.tab_off * CT;
         if (d.small) {
-            hipLaunchKernelGGL((reduce_small_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock)), dim3(kBlock), 0,
-                               s, a.small_part, (int)grid, L.Gs, d.lds_off, d.G, qo);
+            hipLaunchKernelGGL((reduce_small_kernel<CT>), dim3((unsigned)(d.G * CT)), dim3(kBlock), 0, s,
+                               a.small_part, (int)grid, L.Gs, d.lds_off, d.G, qo);
         } else {
             const DimStruct& D = p->d[L.dims[o]];
             const bool timed_r = mode == kIter && kstats_enabled();
@@ -1066,6 +1090,7 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const double* const* xcol
                 if (!any) break;
             }
             a.p = pp;
+            a.active = sc.st->active;
             e = apply_and_reduce<CT>(p, L, a, kIter, q, s);
             if (e) return e;
             hipLaunchKernelGGL((cg_pq_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, (const double*)pp,
