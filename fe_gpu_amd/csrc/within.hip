@@ -561,6 +561,7 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
     }
 }
 
+
 // q[tab_off + g][c] = sum of the run partials of group g (runs in group order)
 template <int CT>
 __global__ void reduce_runs_kernel(const double* __restrict__ runbuf, const int* __restrict__ order,
@@ -932,6 +933,7 @@ unsigned tile_grid(Panel* p) {
 unsigned long_grid(Panel* p) { return p->n_long > 0 ? stream_grid((size_t)p->n_long, kBlock, 2) : 0; }
 
 unsigned apply_grid(Panel* p) { return tile_grid(p) + long_grid(p); }
+
 
 // Launch one apply (mode) and reduce it into q (table space [S][CT]).
 template <int CT>
