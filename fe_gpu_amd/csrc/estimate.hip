@@ -651,16 +651,20 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
     const int C = 1 + K + L;
     double* work = nullptr;
     if ((e = db.get(&work, (size_t)n * C))) return e;
-    std::vector<const double*> xc(C);
-    std::vector<double*> oc(C);
-    xc[0] = d_y;
-    for (int k = 0; k < K; ++k) xc[1 + k] = d_W + (int64_t)k * ld;
-    for (int l = 0; l < L; ++l) xc[1 + K + l] = d_Z + (int64_t)l * ld;
-    for (int c = 0; c < C; ++c) oc[c] = work + (int64_t)c * n;
+    WithinCols wc(C);
+    wc.x[0] = d_y;
+    for (int k = 0; k < K; ++k) wc.x[1 + k] = d_W + (int64_t)k * ld;
+    for (int l = 0; l < L; ++l) wc.x[1 + K + l] = d_Z + (int64_t)l * ld;
+    for (int c = 0; c < C; ++c) wc.out[c] = work + (int64_t)c * n;
+    uint32_t* wm = nullptr;
+    if (d_weights) {
+        if ((e = db.get(&wm, (size_t)n))) return e;
+        if ((e = build_mask_words(p, &d_weights, 1, wm, 0))) return e;
+    }
     fehip_demean_opts dopt = opts->demean;
     dopt.fe_mask = fe_mask;
-    WithinStats ws;
-    if ((e = within_transform(p, xc.data(), C, d_weights, dopt, oc.data(), 1, ws, 0))) return e;
+    WithinStats ws(C);
+    if ((e = within_transform(p, wc, C, wm, 1, dopt, 1, ws, 0))) return e;
     result->converged = ws.converged;
     result->fe_iterations = ws.iterations;
     result->max_update = ws.max_update;

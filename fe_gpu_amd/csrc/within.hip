@@ -12,8 +12,14 @@
 // segment and the run partials are reduced in group order (no HBM atomics, fixed order).
 // The result M_F x = M_P (x - D_o a) is formed in the final pass over the columns.
 //
+// Batching: a solve carries up to 32 request slots.  Each row has a 32-bit mask word (bit b set =>
+// the row belongs to slot b's subsample) and each column a slot bit, so the columns of many
+// requests with the same FE set share every sweep while each keeps its own weights, Schur
+// diagonal and CG scalars.  A column's result does not depend on the other columns of its tile.
+//
 // Stopping rule: max over groups of |r_g / n_g|, the size of the group-mean update the next
 // alternating projection would make -- the reference's criterion (fe_gpu_demean.f90:196-211,294).
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -29,6 +35,7 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kMaxScatter = 6;
 constexpr int kMaxLarge = 3;
+constexpr int kMaxCT = 16;
 
 enum Mode { kIter = 0, kData = 1, kDiag = 2, kFinal = 3 };
 
@@ -47,391 +54,125 @@ struct ApplyArgs {
     const int64_t* seg_off;
     int64_t GP;
     int elim;
-    const uint8_t* w;       // panel-order weights or null
+    const uint32_t* wm;        // panel-order row mask words (nullptr: every row belongs to every slot)
+    uint32_t cbit[kMaxCT];     // slot bit of each column (0 for padding columns)
+    uint32_t anybit;           // OR of the column bits
+    uint32_t dbit;             // kDiag: slot bit
     int nd;
     ScatterDim sd[kMaxScatter];
-    int Gs;                 // small groups in LDS
-    int diag_dim;           // kDiag: the scatter dim processed
-    const double* p;        // kIter: [S][CT]
-    const int* active;      // kIter: per-column CG activity (nullptr = all active)
-    const double* xc[16];   // kData: caller columns of this tile (caller row order)
-    double* oc[16];         // finalize: output columns
-    int ncols;              // valid columns in this tile
+    int Gs;                    // small groups in LDS
+    int diag_dim;              // kDiag: the scatter dim processed
+    const double* p;           // kIter: [S][CT]
+    const int* active;         // kIter: per-column CG activity (nullptr = all active)
+    const double* xc[kMaxCT];  // kData/kFinal: input columns (caller row order)
+    double* oc[kMaxCT];        // kFinal: output columns
+    int ncols;                 // valid columns in this tile
     const int* perm;
-    double* small_part;     // [blocks][Gs][CT]
+    double* small_part;        // [blocks][Gs][CT]
     const int64_t* tile_seg;   // [2 * n_tiles] segment range of each tile
     int64_t n_tiles;
     const int64_t* long_segs;  // segments longer than a tile
     int64_t n_long;
     int part_base;             // first partial block written by this launch
-    const double* alpha;       // finalize: solved tables [S][CT]
-    int to_work;               // finalize: 1 = panel-order work columns, 0 = caller columns in place
+    const double* alpha;       // kFinal: solved tables [S][CT]
+    int to_work;               // kFinal: 1 = panel-order work columns, 0 = caller columns in place
 };
 
+__device__ __forceinline__ uint32_t row_word(const ApplyArgs& a, int64_t r) { return a.wm ? a.wm[r] : ~0u; }
+
+// v = x (kData/kFinal) and +/- the tables at the row's groups (kIter: +p, kFinal: -alpha)
 template <int CT, int MODE>
-__device__ __forceinline__ void row_values(const ApplyArgs& a, int64_t r, const double* sp, double (&v)[CT]) {
-    if constexpr (MODE == kData) {
+__device__ __forceinline__ void row_value(const ApplyArgs& a, int64_t r, const double* sp, double (&v)[CT]) {
+#pragma unroll
+    for (int c = 0; c < CT; ++c) v[c] = 0.0;
+    if constexpr (MODE == kData || MODE == kFinal) {
         const int64_t src = a.perm ? a.perm[r] : r;
 #pragma unroll
         for (int c = 0; c < CT; ++c) v[c] = c < a.ncols ? a.xc[c][src] : 0.0;
-    } else {
-#pragma unroll
-        for (int c = 0; c < CT; ++c) v[c] = 0.0;
+    }
+    if constexpr (MODE == kIter || MODE == kFinal) {
+        const double sign = MODE == kIter ? 1.0 : -1.0;
+        const double* tab = MODE == kIter ? a.p : a.alpha;
         for (int o = 0; o < a.nd; ++o) {
             const ScatterDim& d = a.sd[o];
             const int g = d.ids[r];
             if (d.small) {
-                const double* t = sp + (int64_t)(d.lds_off + g) * CT;
+                const size_t base = (size_t)(d.lds_off + g) * CT;
 #pragma unroll
-                for (int c = 0; c < CT; ++c) v[c] += t[c];
+                for (int c = 0; c < CT; ++c) v[c] += sign * sp[base + c];
             } else {
-                const double* t = a.p + (d.tab_off + g) * CT;
+                const int64_t base = (d.tab_off + g) * CT;
 #pragma unroll
-                for (int c = 0; c < CT; ++c) v[c] += t[c];
+                for (int c = 0; c < CT; ++c) v[c] += sign * tab[base + c];
             }
         }
     }
 }
 
-// One thread per primary segment; see the file comment for the three modes.
+// zero the small-dim accumulators and stage the small-dim tables (kIter: p, kFinal: alpha) in LDS
 template <int CT, int MODE>
-__global__ __launch_bounds__(kBlock) void apply_kernel(ApplyArgs a) {
-    extern __shared__ double lds[];
-    double* sacc = lds;                                   // [Gs][CT]
-    double* sp = lds + (size_t)a.Gs * CT;                  // [Gs][CT] (kIter)
+__device__ __forceinline__ void stage_tables(const ApplyArgs& a, double* sacc, double* sp) {
     for (int e = threadIdx.x; e < a.Gs * CT; e += blockDim.x) sacc[e] = 0.0;
-    if constexpr (MODE == kIter) {
-        for (int o = 0; o < a.nd; ++o) {
-            const ScatterDim& d = a.sd[o];
-            if (!d.small) continue;
-            for (int e = threadIdx.x; e < d.G * CT; e += blockDim.x)
-                sp[(size_t)d.lds_off * CT + e] = a.p[d.tab_off * CT + e];
-        }
-    }
-    __syncthreads();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; li < a.n_long; li += stride) {
-        const int64_t s = a.long_segs[li];
-        const int64_t r0 = a.seg_off[s], r1 = a.seg_off[s + 1];
-        double mean[CT];
-#pragma unroll
-        for (int c = 0; c < CT; ++c) mean[c] = 0.0;
-        double nw = 0.0;
-        if (a.elim || MODE == kDiag) {
-            double acc[CT];
-#pragma unroll
-            for (int c = 0; c < CT; ++c) acc[c] = 0.0;
-            for (int64_t r = r0; r < r1; ++r) {
-                if (a.w && !a.w[r]) continue;
-                nw += 1.0;
-                if constexpr (MODE != kDiag) {
-                    double v[CT];
-                    row_values<CT, MODE>(a, r, sp, v);
-#pragma unroll
-                    for (int c = 0; c < CT; ++c) acc[c] += v[c];
-                }
-            }
-            if (nw > 0.0)
-#pragma unroll
-                for (int c = 0; c < CT; ++c) mean[c] = acc[c] / nw;
-        }
-        int cur[kMaxLarge];
-        int64_t rk[kMaxLarge];
-        double racc[kMaxLarge][CT];
-#pragma unroll
-        for (int j = 0; j < kMaxLarge; ++j) {
-            cur[j] = -1;
-            rk[j] = 0;
-#pragma unroll
-            for (int c = 0; c < CT; ++c) racc[j][c] = 0.0;
-        }
-        for (int o = 0; o < a.nd; ++o)
-            if (!a.sd[o].small) rk[a.sd[o].large_slot] = a.sd[o].seg_run_off[s];
-        for (int64_t r = r0; r < r1; ++r) {
-            const bool on = !a.w || a.w[r];
-            double u[CT];
-            if constexpr (MODE == kDiag) {
-                // u0: Schur-diagonal contribution of this row to its group of dim diag_dim,
-                // u1: its weight.  c_r = weight of the segment's rows sharing the group.
-                double cr = 0.0;
-                if (on && a.elim) {
-                    const int* ids = a.sd[a.diag_dim].ids;
-                    const int g = ids[r];
-                    for (int64_t q = r0; q < r1; ++q) cr += (ids[q] == g && (!a.w || a.w[q])) ? 1.0 : 0.0;
-                }
-#pragma unroll
-                for (int c = 0; c < CT; ++c) u[c] = 0.0;
-                if (on) {
-                    u[0] = a.elim ? 1.0 - cr / nw : 1.0;
-                    if (CT > 1) u[1 % CT] = 1.0;
-                }
-            } else {
-                double v[CT];
-                row_values<CT, MODE>(a, r, sp, v);
-#pragma unroll
-                for (int c = 0; c < CT; ++c) u[c] = on ? v[c] - mean[c] : 0.0;
-            }
-            for (int o = 0; o < a.nd; ++o) {
-                if (MODE == kDiag && o != a.diag_dim) continue;
-                const ScatterDim& d = a.sd[o];
-                const int g = d.ids[r];
-                if (d.small) {
-                    if (on) {
-                        double* t = sacc + (size_t)(d.lds_off + g) * CT;
-#pragma unroll
-                        for (int c = 0; c < CT; ++c) lds_add(&t[c], u[c]);
-                    }
-                } else {
-                    const int j = d.large_slot;
-                    if (g != cur[j]) {
-                        if (cur[j] >= 0) {
-                            double* dst = d.runbuf + rk[j] * CT;
-#pragma unroll
-                            for (int c = 0; c < CT; ++c) dst[c] = racc[j][c];
-                            ++rk[j];
-                        }
-                        cur[j] = g;
-#pragma unroll
-                        for (int c = 0; c < CT; ++c) racc[j][c] = 0.0;
-                    }
-#pragma unroll
-                    for (int c = 0; c < CT; ++c) racc[j][c] += u[c];
-                }
-            }
-        }
-        for (int o = 0; o < a.nd; ++o) {
-            if (MODE == kDiag && o != a.diag_dim) continue;
-            const ScatterDim& d = a.sd[o];
-            if (d.small) continue;
-            const int j = d.large_slot;
-            if (cur[j] >= 0) {
-                double* dst = d.runbuf + rk[j] * CT;
-#pragma unroll
-                for (int c = 0; c < CT; ++c) dst[c] = racc[j][c];
-            }
-        }
-    }
-    __syncthreads();
-    double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
-    for (int e = threadIdx.x; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
-}
-
-
-// Row-tiled kernel: one workgroup per tile of whole primary segments (<= kTileRows rows), one row
-// per thread.  Phase 1 loads each row's values (coalesced) into LDS, phase 2 removes the segment
-// means (one thread per segment, LDS only), phase 3 scatters the rows into the other dims' tables
-// (LDS atomics for small dims; per-run partials for large dims, runs ranked with a block scan) or,
-// for kFinal, writes the transformed rows out.
-template <int CT, int MODE>
-__global__ __launch_bounds__(kBlock) void tile_kernel(ApplyArgs a) {
-    __shared__ double sv[kTileRows * CT];
-    __shared__ int gd[kTileRows];            // kDiag: group of the processed dim
-    __shared__ unsigned char wr[kTileRows];  // row weight (0/1), 0 past the tile end
-    __shared__ unsigned char sb[kTileRows];  // row starts a segment
-    __shared__ unsigned char hf[kTileRows];  // row starts a run (large-dim scatter)
-    __shared__ short rs[kTileRows], re[kTileRows];  // kDiag: segment range of each row
-    __shared__ int wcnt[kBlock / kWave];
-    extern __shared__ double dyn[];
-    double* sacc = dyn;                          // [Gs][CT]
-    double* sp = dyn + (size_t)a.Gs * CT;        // [Gs][CT] staged p (kIter) or alpha (kFinal)
-    const int j = threadIdx.x;
-    const int lane = j & (kWave - 1), wv = j / kWave;
-    for (int e = j; e < a.Gs * CT; e += blockDim.x) sacc[e] = 0.0;
     if constexpr (MODE == kIter || MODE == kFinal) {
         const double* tab = MODE == kIter ? a.p : a.alpha;
         for (int o = 0; o < a.nd; ++o) {
             const ScatterDim& d = a.sd[o];
             if (!d.small) continue;
-            for (int e = j; e < d.G * CT; e += blockDim.x) sp[(size_t)d.lds_off * CT + e] = tab[d.tab_off * CT + e];
+            for (int e = threadIdx.x; e < d.G * CT; e += blockDim.x) sp[(size_t)d.lds_off * CT + e] = tab[d.tab_off * CT + e];
         }
-    }
-    __syncthreads();
-    for (int64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
-        const int64_t s0 = a.tile_seg[2 * t], s1 = a.tile_seg[2 * t + 1];
-        const int64_t R0 = a.seg_off[s0];
-        const int nr = (int)(a.seg_off[s1] - R0);
-        const int64_t r = R0 + j;
-        const bool valid = j < nr;
-        const unsigned char w = valid ? (a.w ? a.w[r] : 1) : 0;
-        wr[j] = w;
-        sb[j] = 0;
-        // ---- phase 1: row values
-        if constexpr (MODE == kDiag) {
-            gd[j] = valid ? a.sd[a.diag_dim].ids[r] : -1;
-        } else {
-            double v[CT];
-#pragma unroll
-            for (int c = 0; c < CT; ++c) v[c] = 0.0;
-            if (valid) {
-                if constexpr (MODE == kData || MODE == kFinal) {
-                    const int64_t src = a.perm ? a.perm[r] : r;
-#pragma unroll
-                    for (int c = 0; c < CT; ++c) v[c] = c < a.ncols ? a.xc[c][src] : 0.0;
-                }
-                if constexpr (MODE == kIter || MODE == kFinal) {
-                    const double sign = MODE == kIter ? 1.0 : -1.0;
-                    const double* tab = MODE == kIter ? a.p : a.alpha;
-                    for (int o = 0; o < a.nd; ++o) {
-                        const ScatterDim& d = a.sd[o];
-                        const int g = d.ids[r];
-                        if (d.small) {
-                            const size_t base = (size_t)(d.lds_off + g) * CT;
-#pragma unroll
-                            for (int c = 0; c < CT; ++c) v[c] += sign * sp[base + c];
-                        } else {
-                            const int64_t base = (d.tab_off + g) * CT;
-#pragma unroll
-                            for (int c = 0; c < CT; ++c) v[c] += sign * tab[base + c];
-                        }
-                    }
-                }
-            }
-#pragma unroll
-            for (int c = 0; c < CT; ++c) sv[j * CT + c] = w ? v[c] : 0.0;
-        }
-        __syncthreads();
-        // ---- phase 2: segment means (one thread per segment)
-        for (int64_t k = j; k < s1 - s0; k += blockDim.x) {
-            const int b = (int)(a.seg_off[s0 + k] - R0), e = (int)(a.seg_off[s0 + k + 1] - R0);
-            if (b < e) sb[b] = 1;
-            if constexpr (MODE == kDiag) {
-                for (int q = b; q < e; ++q) {
-                    rs[q] = (short)b;
-                    re[q] = (short)e;
-                }
-            } else {
-                if (a.elim && b < e) {
-                    double acc[CT];
-#pragma unroll
-                    for (int c = 0; c < CT; ++c) acc[c] = 0.0;
-                    int nw = 0;
-                    for (int q = b; q < e; ++q) {
-                        nw += wr[q];
-#pragma unroll
-                        for (int c = 0; c < CT; ++c) acc[c] += sv[q * CT + c];
-                    }
-                    if (nw > 0) {
-                        const double inv = 1.0 / (double)nw;
-                        for (int q = b; q < e; ++q)
-                            if (wr[q])
-#pragma unroll
-                                for (int c = 0; c < CT; ++c) sv[q * CT + c] -= acc[c] * inv;
-                    }
-                }
-            }
-        }
-        __syncthreads();
-        if constexpr (MODE == kDiag) {
-            // u0: this row's contribution to the Schur diagonal of its group, u1: its weight
-            double u0 = 0.0, u1 = 0.0;
-            if (valid && w) {
-                u1 = 1.0;
-                if (a.elim) {
-                    int cr = 0, nw = 0;
-                    for (int q = rs[j]; q < re[j]; ++q) {
-                        nw += wr[q];
-                        cr += (wr[q] && gd[q] == gd[j]) ? 1 : 0;
-                    }
-                    u0 = 1.0 - (double)cr / (double)nw;
-                } else {
-                    u0 = 1.0;
-                }
-            }
-            sv[j * CT + 0] = u0;
-            if (CT > 1) sv[j * CT + (1 % CT)] = u1;
-            __syncthreads();
-        }
-        // ---- phase 3
-        if constexpr (MODE == kFinal) {
-            if (valid) {
-                const int64_t src = a.perm ? a.perm[r] : r;
-                for (int c = 0; c < a.ncols; ++c) {
-                    if (a.to_work)
-                        a.oc[c][r] = w ? sv[j * CT + c] : 0.0;
-                    else if (w)
-                        a.oc[c][src] = sv[j * CT + c];
-                }
-            }
-        } else {
-            for (int o = 0; o < a.nd; ++o) {
-                if (MODE == kDiag && o != a.diag_dim) continue;
-                const ScatterDim& d = a.sd[o];
-                if (d.small) {
-                    if (valid && w) {
-                        const int g = MODE == kDiag ? gd[j] : d.ids[r];
-                        double* tq = sacc + (size_t)(d.lds_off + g) * CT;
-#pragma unroll
-                        for (int c = 0; c < CT; ++c) lds_add(&tq[c], sv[j * CT + c]);
-                    }
-                    continue;
-                }
-                // runs: a new run starts at a segment start or where the id changes
-                const int g = valid ? (MODE == kDiag ? gd[j] : d.ids[r]) : -1;
-                const int gp = (valid && j > 0) ? (MODE == kDiag ? gd[j - 1] : d.ids[r - 1]) : -1;
-                const bool head = valid && (j == 0 || sb[j] || g != gp);
-                hf[j] = head;
-                const unsigned long long bal = __ballot(head);
-                if (lane == 0) wcnt[wv] = __popcll(bal);
-                __syncthreads();
-                if (head) {
-                    int rank = __popcll(bal & ((1ull << lane) - 1ull));
-                    for (int q = 0; q < wv; ++q) rank += wcnt[q];
-                    double acc[CT];
-#pragma unroll
-                    for (int c = 0; c < CT; ++c) acc[c] = sv[j * CT + c];
-                    for (int q = j + 1; q < nr && !hf[q]; ++q)
-#pragma unroll
-                        for (int c = 0; c < CT; ++c) acc[c] += sv[q * CT + c];
-                    double* dst = d.runbuf + (d.seg_run_off[s0] + rank) * CT;
-#pragma unroll
-                    for (int c = 0; c < CT; ++c) dst[c] = acc[c];
-                }
-                __syncthreads();
-            }
-        }
-        __syncthreads();
-    }
-    if constexpr (MODE != kFinal) {
-        double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
-        for (int e = j; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
     }
 }
 
+template <int CT>
+__device__ __forceinline__ bool any_active(const int* active) {
+    if (!active) return true;
+    bool any = false;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) any |= active[c] != 0;
+    return any;
+}
 
-// Row tile (kIter, kData, kFinal): one workgroup per tile of whole primary segments, one row per
-// thread.  Rows are staged in LDS; segment means are taken by one thread per (segment, column)
-// and run sums by one thread per (run, column), so no thread walks a long serial chain and no
-// cross-lane shuffles are needed.  Run partials of a tile are contiguous in the run buffer and
-// leave in one coalesced store; small-dim tables take LDS atomics.
+// Run heads of one large dim inside a tile: fills rstart[0..total] (local row of each run start,
+// rstart[total] = nr) and returns the number of runs.  Every thread of the block must call it.
+__device__ __forceinline__ int tile_runs(bool valid, bool rhead, int j, int nr, int* wcnt, int* rstart) {
+    const int lane = j & (kWave - 1), wv = j / kWave;
+    const unsigned long long bal = __ballot(rhead);
+    if (lane == 0) wcnt[wv] = __popcll(bal);
+    __syncthreads();
+    int base = 0, total = 0;
+    for (int q = 0; q < kBlock / kWave; ++q) {
+        base += q < wv ? wcnt[q] : 0;
+        total += wcnt[q];
+    }
+    if (rhead) rstart[base + __popcll(bal & ((1ull << lane) - 1ull))] = j;
+    if (j == 0) rstart[total] = nr;
+    __syncthreads();
+    (void)valid;
+    return total;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Row tile (kIter, kData, kFinal): one workgroup per tile of whole primary segments (<= kTileRows
+// rows), one row per thread.  Rows are staged in LDS; segment means are taken by one thread per
+// (segment, column) and run sums by one thread per (run, column), so no thread walks a long serial
+// chain and no cross-lane shuffles are needed.  Run partials of a tile are contiguous in the run
+// buffer and leave in one coalesced store; small-dim tables take LDS atomics.
 template <int CT, int MODE>
 __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
     if constexpr (MODE == kIter) {
-        if (a.active) {  // every column converged: nothing to apply (the host polls periodically)
-            bool any = false;
-#pragma unroll
-            for (int c = 0; c < CT; ++c) any |= a.active[c] != 0;
-            if (!any) return;
-        }
+        if (!any_active<CT>(a.active)) return;  // every column converged (the host polls periodically)
     }
-    __shared__ double sv[kTileRows * CT];     // row values
-    __shared__ unsigned char wr[kTileRows];   // row weight
-    __shared__ int soff[kTileRows + 1];       // local row offsets of the tile's segments
-    __shared__ int rstart[kTileRows + 1];     // first row of each run
+    __shared__ double sv[kTileRows * CT];   // row values
+    __shared__ uint32_t wrow[kTileRows];    // row mask words (0 past the tile end)
+    __shared__ int soff[kTileRows + 1];     // local row offsets of the tile's segments
+    __shared__ int rstart[kTileRows + 1];   // first row of each run
     __shared__ int wcnt[kBlock / kWave];
     extern __shared__ double dyn[];
-    double* sacc = dyn;                       // [Gs][CT]
-    double* sp = dyn + (size_t)a.Gs * CT;     // [Gs][CT]
+    double* sacc = dyn;                     // [Gs][CT]
+    double* sp = dyn + (size_t)a.Gs * CT;   // [Gs][CT]
     const int j = threadIdx.x;
-    const int lane = j & (kWave - 1), wv = j / kWave;
-    for (int e = j; e < a.Gs * CT; e += blockDim.x) sacc[e] = 0.0;
-    if constexpr (MODE == kIter || MODE == kFinal) {
-        const double* tab = MODE == kIter ? a.p : a.alpha;
-        for (int o = 0; o < a.nd; ++o) {
-            const ScatterDim& d = a.sd[o];
-            if (!d.small) continue;
-            for (int e = j; e < d.G * CT; e += blockDim.x) sp[(size_t)d.lds_off * CT + e] = tab[d.tab_off * CT + e];
-        }
-    }
+    const int lane = j & (kWave - 1);
+    stage_tables<CT, MODE>(a, sacc, sp);
     for (int64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
         const int64_t s0 = a.tile_seg[2 * t], s1 = a.tile_seg[2 * t + 1];
         const int nseg = (int)(s1 - s0);
@@ -441,54 +182,32 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
         const int nr = (int)(a.seg_off[s1] - R0);
         const bool valid = j < nr;
         const int64_t r = R0 + j;
-        const bool on = valid && (!a.w || a.w[r]);
-        int gid[kMaxScatter];
+        const uint32_t word = valid ? row_word(a, r) : 0u;
+        const bool on = (word & a.anybit) != 0;
         double v[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) v[c] = 0.0;
-        for (int o = 0; o < a.nd; ++o) gid[o] = valid ? a.sd[o].ids[r] : -1;
-        if (on) {
-            if constexpr (MODE == kData || MODE == kFinal) {
-                const int64_t src = a.perm ? a.perm[r] : r;
+        if (on) row_value<CT, MODE>(a, r, sp, v);
 #pragma unroll
-                for (int c = 0; c < CT; ++c) v[c] = c < a.ncols ? a.xc[c][src] : 0.0;
-            }
-            if constexpr (MODE == kIter || MODE == kFinal) {
-                const double sign = MODE == kIter ? 1.0 : -1.0;
-                const double* tab = MODE == kIter ? a.p : a.alpha;
-                for (int o = 0; o < a.nd; ++o) {
-                    const ScatterDim& d = a.sd[o];
-                    if (d.small) {
-                        const size_t base = (size_t)(d.lds_off + gid[o]) * CT;
-#pragma unroll
-                        for (int c = 0; c < CT; ++c) v[c] += sign * sp[base + c];
-                    } else {
-                        const int64_t base = (d.tab_off + gid[o]) * CT;
-#pragma unroll
-                        for (int c = 0; c < CT; ++c) v[c] += sign * tab[base + c];
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int c = 0; c < CT; ++c) sv[j * CT + c] = v[c];
-        wr[j] = on ? 1 : 0;
+        for (int c = 0; c < CT; ++c) sv[j * CT + c] = (word & a.cbit[c]) ? v[c] : 0.0;
+        wrow[j] = word;
         __syncthreads();
         if (a.elim) {
-            // one thread per (segment, column): subtract the weighted segment mean
+            // one thread per (segment, column): subtract the column's weighted segment mean
             for (int e = j; e < nseg * CT; e += blockDim.x) {
                 const int k = e / CT, c = e - k * CT;
+                const uint32_t bit = a.cbit[c];
                 const int b = soff[k], f = soff[k + 1];
                 double sm = 0.0;
                 int nw = 0;
                 for (int q = b; q < f; ++q) {
                     sm += sv[q * CT + c];
-                    nw += wr[q];
+                    nw += (wrow[q] & bit) ? 1 : 0;
                 }
                 if (nw > 0) {
                     const double mean = sm / (double)nw;
                     for (int q = b; q < f; ++q)
-                        if (wr[q]) sv[q * CT + c] -= mean;
+                        if (wrow[q] & bit) sv[q * CT + c] -= mean;
                 }
             }
             __syncthreads();
@@ -501,23 +220,25 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
                     if (c >= a.ncols) break;
                     if (a.to_work)
                         a.oc[c][r] = sv[j * CT + c];
-                    else if (on)
+                    else if (word & a.cbit[c])
                         a.oc[c][src] = sv[j * CT + c];
                 }
             }
         } else {
             for (int o = 0; o < a.nd; ++o) {
                 const ScatterDim& d = a.sd[o];
+                const int g = valid ? d.ids[r] : -1;
                 if (d.small) {
                     if (on) {
-                        double* tq = sacc + (size_t)(d.lds_off + gid[o]) * CT;
+                        double* tq = sacc + (size_t)(d.lds_off + g) * CT;
 #pragma unroll
-                        for (int c = 0; c < CT; ++c) lds_add(&tq[c], sv[j * CT + c]);
+                        for (int c = 0; c < CT; ++c)
+                            if (word & a.cbit[c]) lds_add(&tq[c], sv[j * CT + c]);
                     }
                     continue;
                 }
                 // runs: a new run starts at a segment start or where the id changes
-                int gp = __shfl_up(gid[o], 1, kWave);
+                int gp = __shfl_up(g, 1, kWave);
                 if (lane == 0) gp = (valid && j > 0) ? d.ids[r - 1] : -1;
                 bool segstart = false;
                 if (valid) {
@@ -531,18 +252,8 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
                     }
                     segstart = soff[lo] == j;
                 }
-                const bool rhead = valid && (j == 0 || segstart || gid[o] != gp);
-                const unsigned long long bal = __ballot(rhead);
-                if (lane == 0) wcnt[wv] = __popcll(bal);
-                __syncthreads();
-                int base = 0, total = 0;
-                for (int q = 0; q < kBlock / kWave; ++q) {
-                    base += q < wv ? wcnt[q] : 0;
-                    total += wcnt[q];
-                }
-                if (rhead) rstart[base + __popcll(bal & ((1ull << lane) - 1ull))] = j;
-                if (j == 0) rstart[total] = nr;
-                __syncthreads();
+                const bool rhead = valid && (j == 0 || segstart || g != gp);
+                const int total = tile_runs(valid, rhead, j, nr, wcnt, rstart);
                 double* dst = d.runbuf + d.seg_run_off[s0] * CT;
                 for (int e = j; e < total * CT; e += blockDim.x) {
                     const int k = e / CT, c = e - k * CT;
@@ -561,11 +272,238 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
     }
 }
 
+// ------------------------------------------------------------------------------------------------
+// Schur diagonal of one slot and one dim (CT = 2): u0 = w (1 - c/n_seg), where c counts the slot's
+// rows of the segment in the same group of diag_dim and n_seg the slot's rows of the segment
+// (u0 = w without elimination); u1 = w gives the weighted group counts.
+__global__ __launch_bounds__(kBlock) void tile_diag_kernel(ApplyArgs a) {
+    constexpr int CT = 2;
+    __shared__ double sv[kTileRows * CT];
+    __shared__ int gd[kTileRows];
+    __shared__ unsigned char wr[kTileRows];
+    __shared__ short rs[kTileRows], re[kTileRows];
+    __shared__ int soff[kTileRows + 1];
+    __shared__ int rstart[kTileRows + 1];
+    __shared__ int wcnt[kBlock / kWave];
+    extern __shared__ double dyn[];
+    double* sacc = dyn;
+    const int j = threadIdx.x;
+    for (int e = j; e < a.Gs * CT; e += blockDim.x) sacc[e] = 0.0;
+    const ScatterDim& d = a.sd[a.diag_dim];
+    for (int64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+        const int64_t s0 = a.tile_seg[2 * t], s1 = a.tile_seg[2 * t + 1];
+        const int nseg = (int)(s1 - s0);
+        const int64_t R0 = a.seg_off[s0];
+        __syncthreads();
+        for (int k = j; k <= nseg; k += blockDim.x) soff[k] = (int)(a.seg_off[s0 + k] - R0);
+        const int nr = (int)(a.seg_off[s1] - R0);
+        const bool valid = j < nr;
+        const int64_t r = R0 + j;
+        const bool on = valid && (row_word(a, r) & a.dbit);
+        gd[j] = valid ? d.ids[r] : -1;
+        wr[j] = on ? 1 : 0;
+        __syncthreads();
+        for (int k = j; k < nseg; k += blockDim.x) {
+            const int b = soff[k], e = soff[k + 1];
+            for (int q = b; q < e; ++q) {
+                rs[q] = (short)b;
+                re[q] = (short)e;
+            }
+        }
+        __syncthreads();
+        double u0 = 0.0, u1 = 0.0;
+        if (on) {
+            u1 = 1.0;
+            if (a.elim) {
+                int cr = 0, nw = 0;
+                for (int q = rs[j]; q < re[j]; ++q) {
+                    nw += wr[q];
+                    cr += (wr[q] && gd[q] == gd[j]) ? 1 : 0;
+                }
+                u0 = 1.0 - (double)cr / (double)nw;
+            } else {
+                u0 = 1.0;
+            }
+        }
+        sv[j * CT + 0] = u0;
+        sv[j * CT + 1] = u1;
+        if (d.small) {
+            if (on) {
+                lds_add(&sacc[(size_t)(d.lds_off + gd[j]) * CT + 0], u0);
+                lds_add(&sacc[(size_t)(d.lds_off + gd[j]) * CT + 1], u1);
+            }
+            continue;
+        }
+        const int gp = (valid && j > 0) ? gd[j - 1] : -1;
+        const bool rhead = valid && (j == 0 || rs[j] == j || gd[j] != gp);
+        const int total = tile_runs(valid, rhead, j, nr, wcnt, rstart);
+        double* dst = d.runbuf + d.seg_run_off[s0] * CT;
+        for (int e = j; e < total * CT; e += blockDim.x) {
+            const int k = e / CT, c = e - k * CT;
+            double sm = 0.0;
+            for (int q = rstart[k]; q < rstart[k + 1]; ++q) sm += sv[q * CT + c];
+            dst[e] = sm;
+        }
+    }
+    __syncthreads();
+    double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
+    for (int e = j; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
+}
 
-// q[tab_off + g][c] = sum of the run partials of group g (runs in group order)
+// ------------------------------------------------------------------------------------------------
+// Segments longer than a tile: one thread per segment (rare; correctness path).
+template <int CT, int MODE>
+__global__ __launch_bounds__(kBlock) void long_kernel(ApplyArgs a) {
+    if constexpr (MODE == kIter) {
+        if (!any_active<CT>(a.active)) return;
+    }
+    extern __shared__ double dyn[];
+    double* sacc = dyn;
+    double* sp = dyn + (size_t)a.Gs * CT;
+    stage_tables<CT, MODE>(a, sacc, sp);
+    __syncthreads();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; li < a.n_long; li += stride) {
+        const int64_t s = a.long_segs[li];
+        const int64_t r0 = a.seg_off[s], r1 = a.seg_off[s + 1];
+        const uint32_t selbit = MODE == kDiag ? a.dbit : a.anybit;
+        double mean[CT];
+        int nwc[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            mean[c] = 0.0;
+            nwc[c] = 0;
+        }
+        int nwd = 0;
+        if (a.elim) {
+            double acc[CT];
+#pragma unroll
+            for (int c = 0; c < CT; ++c) acc[c] = 0.0;
+            for (int64_t r = r0; r < r1; ++r) {
+                const uint32_t word = row_word(a, r);
+                if (!(word & selbit)) continue;
+                ++nwd;
+                if constexpr (MODE != kDiag) {
+                    double v[CT];
+                    row_value<CT, MODE>(a, r, sp, v);
+#pragma unroll
+                    for (int c = 0; c < CT; ++c)
+                        if (word & a.cbit[c]) {
+                            acc[c] += v[c];
+                            ++nwc[c];
+                        }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < CT; ++c) mean[c] = nwc[c] > 0 ? acc[c] / (double)nwc[c] : 0.0;
+        }
+        if constexpr (MODE == kFinal) {
+            for (int64_t r = r0; r < r1; ++r) {
+                const uint32_t word = row_word(a, r);
+                const int64_t src = a.perm ? a.perm[r] : r;
+                double v[CT];
+                row_value<CT, MODE>(a, r, sp, v);
+                for (int c = 0; c < a.ncols; ++c) {
+                    const bool on = word & a.cbit[c];
+                    if (a.to_work)
+                        a.oc[c][r] = on ? v[c] - mean[c] : 0.0;
+                    else if (on)
+                        a.oc[c][src] = v[c] - mean[c];
+                }
+            }
+        } else {
+            int cur[kMaxLarge];
+            int64_t rk[kMaxLarge];
+            double racc[kMaxLarge][CT];
+#pragma unroll
+            for (int k = 0; k < kMaxLarge; ++k) {
+                cur[k] = -1;
+                rk[k] = 0;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) racc[k][c] = 0.0;
+            }
+            for (int o = 0; o < a.nd; ++o)
+                if (!a.sd[o].small) rk[a.sd[o].large_slot] = a.sd[o].seg_run_off[s];
+            for (int64_t r = r0; r < r1; ++r) {
+                const uint32_t word = row_word(a, r);
+                double u[CT];
+                if constexpr (MODE == kDiag) {
+                    const bool on = word & a.dbit;
+                    double cr = 0.0;
+                    if (on && a.elim) {
+                        const int* ids = a.sd[a.diag_dim].ids;
+                        const int g = ids[r];
+                        for (int64_t q = r0; q < r1; ++q) cr += (ids[q] == g && (row_word(a, q) & a.dbit)) ? 1.0 : 0.0;
+                    }
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) u[c] = 0.0;
+                    if (on) {
+                        u[0] = a.elim ? 1.0 - cr / (double)nwd : 1.0;
+                        if (CT > 1) u[1 % CT] = 1.0;
+                    }
+                } else {
+                    double v[CT];
+                    row_value<CT, MODE>(a, r, sp, v);
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) u[c] = (word & a.cbit[c]) ? v[c] - mean[c] : 0.0;
+                }
+                for (int o = 0; o < a.nd; ++o) {
+                    if (MODE == kDiag && o != a.diag_dim) continue;
+                    const ScatterDim& d = a.sd[o];
+                    const int g = d.ids[r];
+                    if (d.small) {
+                        double* t = sacc + (size_t)(d.lds_off + g) * CT;
+#pragma unroll
+                        for (int c = 0; c < CT; ++c)
+                            if (u[c] != 0.0) lds_add(&t[c], u[c]);
+                    } else {
+#pragma unroll
+                        for (int k = 0; k < kMaxLarge; ++k) {
+                            if (k != d.large_slot) continue;
+                            if (g != cur[k]) {
+                                if (cur[k] >= 0) {
+                                    double* dst = d.runbuf + rk[k] * CT;
+#pragma unroll
+                                    for (int c = 0; c < CT; ++c) dst[c] = racc[k][c];
+                                    ++rk[k];
+                                }
+                                cur[k] = g;
+#pragma unroll
+                                for (int c = 0; c < CT; ++c) racc[k][c] = 0.0;
+                            }
+#pragma unroll
+                            for (int c = 0; c < CT; ++c) racc[k][c] += u[c];
+                        }
+                    }
+                }
+            }
+            for (int o = 0; o < a.nd; ++o) {
+                if (MODE == kDiag && o != a.diag_dim) continue;
+                const ScatterDim& d = a.sd[o];
+                if (d.small) continue;
+#pragma unroll
+                for (int k = 0; k < kMaxLarge; ++k) {
+                    if (k != d.large_slot || cur[k] < 0) continue;
+                    double* dst = d.runbuf + rk[k] * CT;
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) dst[c] = racc[k][c];
+                }
+            }
+        }
+    }
+    if constexpr (MODE != kFinal) {
+        __syncthreads();
+        double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
+        for (int e = threadIdx.x; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// q[g][c] = sum of the run partials of group g (runs in group order)
 template <int CT>
 __global__ void reduce_runs_kernel(const double* __restrict__ runbuf, const int* __restrict__ order,
-                                   const int64_t* __restrict__ goff, int G, double* q) {
+                                   const int64_t* __restrict__ goff, int G, double* q, const int* __restrict__ active) {
+    if (!any_active<CT>(active)) return;
     const int64_t total = (int64_t)G * CT;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
@@ -576,12 +514,11 @@ __global__ void reduce_runs_kernel(const double* __restrict__ runbuf, const int*
     }
 }
 
-// q[tab_off + g][c] = sum over blocks of the LDS partials
+// one workgroup per table entry: the threads stride over the per-block partials in a fixed order and
+// combine them with a fixed-shape tree
 template <int CT>
 __global__ __launch_bounds__(kBlock) void reduce_small_kernel(const double* __restrict__ part, int nblk, int Gs,
                                                                int lds_off, int G, double* q) {
-    // one workgroup per table entry: the threads stride over the per-block partials in a fixed
-    // order and combine them with a fixed-shape tree
     __shared__ double ws[kBlock / kWave];
     const int e = blockIdx.x;
     if (e >= G * CT) return;
@@ -598,34 +535,16 @@ __global__ __launch_bounds__(kBlock) void reduce_small_kernel(const double* __re
     }
 }
 
-// ---- CG in table space ([S][CT], one CG per column) ----
+// ---- CG in table space ([S][CT], one CG per column) ---------------------------------------------
 struct CgState {
-    double rz[16], alpha[16], beta[16], maxn[16];
-    int active[16], iters[16];
+    double rz[kMaxCT], alpha[kMaxCT], beta[kMaxCT], maxn[kMaxCT];
+    int active[kMaxCT], iters[kMaxCT];
 };
 
 template <int CT>
-__global__ __launch_bounds__(kBlock) void cg_init_kernel(int64_t S, const double* __restrict__ r,
-                                                         const double* __restrict__ minv,
-                                                         const double* __restrict__ wcnt, double* z, double* p,
-                                                         double* x, double* part_rz, double* part_max) {
+__device__ __forceinline__ void block_rz_max(const double (&rz)[CT], const double (&mx)[CT], double* part_rz,
+                                             double* part_max) {
     __shared__ double ws[2][kBlock / kWave][CT];
-    double rz[CT], mx[CT];
-#pragma unroll
-    for (int c = 0; c < CT; ++c) rz[c] = mx[c] = 0.0;
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
-        const double mi = minv[g], n = wcnt[g];
-#pragma unroll
-        for (int c = 0; c < CT; ++c) {
-            const double rv = r[g * CT + c];
-            const double zv = mi * rv;
-            z[g * CT + c] = zv;
-            p[g * CT + c] = zv;
-            x[g * CT + c] = 0.0;
-            rz[c] += rv * zv;
-            if (n > 0.0) mx[c] = fmax(mx[c], fabs(rv) / n);
-        }
-    }
     const int wv = threadIdx.x / kWave, ln = threadIdx.x & (kWave - 1);
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
@@ -648,10 +567,39 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(int64_t S, const double
 }
 
 template <int CT>
+__global__ __launch_bounds__(kBlock) void cg_init_kernel(int64_t S, const double* __restrict__ r,
+                                                         const double* __restrict__ minv,
+                                                         const double* __restrict__ wcnt, double* z, double* p,
+                                                         double* x, double* part_rz, double* part_max) {
+    double rz[CT], mx[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) rz[c] = mx[c] = 0.0;
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const int64_t e = g * CT + c;
+            const double rv = r[e];
+            const double zv = minv[e] * rv;
+            z[e] = zv;
+            p[e] = zv;
+            x[e] = 0.0;
+            rz[c] += rv * zv;
+            const double n = wcnt[e];
+            if (n > 0.0) mx[c] = fmax(mx[c], fabs(rv) / n);
+        }
+    }
+    block_rz_max<CT>(rz, mx, part_rz, part_max);
+}
+
+template <int CT>
 __global__ void cg_start_scalars(const double* part_rz, const double* part_max, int nblk, double tol, int ncols,
                                  CgState* st) {
     const int c = threadIdx.x;
-    if (c >= CT) return;
+    if (c >= kMaxCT) return;
+    if (c >= CT) {
+        st->active[c] = 0;
+        return;
+    }
     double rz = 0.0, mx = 0.0;
     for (int b = 0; b < nblk; ++b) {
         rz += part_rz[b * CT + c];
@@ -703,7 +651,6 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t S, const CgSt
                                                            const double* __restrict__ minv,
                                                            const double* __restrict__ wcnt, double* x, double* r,
                                                            double* z, double* part_rz, double* part_max) {
-    __shared__ double ws[2][kBlock / kWave][CT];
     double al[CT], rz[CT], mx[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
@@ -711,7 +658,6 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t S, const CgSt
         rz[c] = mx[c] = 0.0;
     }
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
-        const double mi = minv[g], n = wcnt[g];
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
             const int64_t e = g * CT + c;
@@ -721,31 +667,14 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t S, const CgSt
                 rv -= al[c] * q[e];
                 r[e] = rv;
             }
-            const double zv = mi * rv;
+            const double zv = minv[e] * rv;
             z[e] = zv;
             rz[c] += rv * zv;
+            const double n = wcnt[e];
             if (n > 0.0) mx[c] = fmax(mx[c], fabs(rv) / n);
         }
     }
-    const int wv = threadIdx.x / kWave, ln = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-        double a = wave_sum(rz[c]), b = wave_max(mx[c]);
-        if (ln == 0) {
-            ws[0][wv][c] = a;
-            ws[1][wv][c] = b;
-        }
-    }
-    __syncthreads();
-    if (threadIdx.x < CT) {
-        double a = 0.0, b = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) {
-            a += ws[0][w][threadIdx.x];
-            b = fmax(b, ws[1][w][threadIdx.x]);
-        }
-        part_rz[blockIdx.x * CT + threadIdx.x] = a;
-        part_max[blockIdx.x * CT + threadIdx.x] = b;
-    }
+    block_rz_max<CT>(rz, mx, part_rz, part_max);
 }
 
 template <int CT>
@@ -783,97 +712,34 @@ __global__ void cg_pdir_kernel(int64_t S, const CgState* __restrict__ st, const 
             if (act[c]) p[g * CT + c] = z[g * CT + c] + be[c] * p[g * CT + c];
 }
 
-__global__ void make_minv_kernel(int64_t S, const double* __restrict__ dg, double* minv, double* wcnt_out,
-                                 const double* __restrict__ wc) {
+// Per-column preconditioner and weighted counts from the per-slot Schur diagonals ([slot][S][2]).
+struct ColSlots {
+    int slot[kMaxCT];
+};
+
+template <int CT>
+__global__ void make_minv_kernel(int64_t S, const double* __restrict__ dg, ColSlots cs, double* minv, double* wcnt) {
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
-        const double d = dg[g * 2], n = wc[g * 2 + 1];
-        minv[g] = (n > 0.0 && d > 1e-9 * n) ? 1.0 / d : 0.0;
-        wcnt_out[g] = n;
-    }
-}
-
-// sa = sum_o alpha_o[id_o(r)] (small dims from the LDS copy, large dims from global memory)
-template <int CT>
-__device__ __forceinline__ void alpha_sum(const ApplyArgs& a, int64_t r, const double* sp, const double* alpha,
-                                          double (&sa)[CT]) {
 #pragma unroll
-    for (int c = 0; c < CT; ++c) sa[c] = 0.0;
-    for (int o = 0; o < a.nd; ++o) {
-        const ScatterDim& d = a.sd[o];
-        const int g = d.ids[r];
-        if (d.small) {
-            const size_t base = (size_t)(d.lds_off + g) * CT;
-#pragma unroll
-            for (int c = 0; c < CT; ++c) sa[c] += sp[base + c];
-        } else {
-            const int64_t base = (d.tab_off + g) * CT;
-#pragma unroll
-            for (int c = 0; c < CT; ++c) sa[c] += alpha[base + c];
+        for (int c = 0; c < CT; ++c) {
+            const double* e = dg + ((int64_t)cs.slot[c] * S + g) * 2;
+            const double d = e[0], n = e[1];
+            minv[g * CT + c] = (n > 0.0 && d > 1e-9 * n) ? 1.0 / d : 0.0;
+            wcnt[g * CT + c] = n;
         }
     }
 }
 
-// out = M_P (x - D_o a) on weighted rows; to_work: write panel order into work (zeros for
-// unweighted rows), else in place into the caller's columns.
-template <int CT>
-__global__ __launch_bounds__(kBlock) void finalize_kernel(ApplyArgs a, const double* __restrict__ alpha,
-                                                          int to_work) {
-    extern __shared__ double sp[];
-    for (int o = 0; o < a.nd; ++o) {
-        const ScatterDim& d = a.sd[o];
-        if (!d.small) continue;
-        for (int e = threadIdx.x; e < d.G * CT; e += blockDim.x) sp[(size_t)d.lds_off * CT + e] = alpha[d.tab_off * CT + e];
-    }
-    __syncthreads();
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t li = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; li < a.n_long; li += stride) {
-        const int64_t s = a.long_segs[li];
-        const int64_t r0 = a.seg_off[s], r1 = a.seg_off[s + 1];
-        double mean[CT];
-#pragma unroll
-        for (int c = 0; c < CT; ++c) mean[c] = 0.0;
-        if (a.elim) {
-            double acc[CT];
-#pragma unroll
-            for (int c = 0; c < CT; ++c) acc[c] = 0.0;
-            double nw = 0.0;
-            for (int64_t r = r0; r < r1; ++r) {
-                if (a.w && !a.w[r]) continue;
-                nw += 1.0;
-                const int64_t src = a.perm ? a.perm[r] : r;
-                double sa[CT];
-                alpha_sum<CT>(a, r, sp, alpha, sa);
-#pragma unroll
-                for (int c = 0; c < CT; ++c) acc[c] += (c < a.ncols ? a.xc[c][src] : 0.0) - sa[c];
-            }
-            if (nw > 0.0)
-#pragma unroll
-                for (int c = 0; c < CT; ++c) mean[c] = acc[c] / nw;
-        }
-        for (int64_t r = r0; r < r1; ++r) {
-            const bool on = !a.w || a.w[r];
-            const int64_t src = a.perm ? a.perm[r] : r;
-            if (!on) {
-                if (to_work)
-                    for (int c = 0; c < a.ncols; ++c) a.oc[c][r] = 0.0;
-                continue;
-            }
-            double sa[CT];
-            alpha_sum<CT>(a, r, sp, alpha, sa);
-#pragma unroll
-            for (int c = 0; c < CT; ++c)
-                if (c < a.ncols) a.oc[c][to_work ? r : src] = a.xc[c][src] - sa[c] - mean[c];
-        }
+__global__ void mask_words_kernel(const uint8_t* __restrict__ w, const int* __restrict__ perm, int64_t n, uint32_t bit,
+                                  int first, uint32_t* wm) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool on = w ? w[perm ? perm[i] : i] != 0 : true;
+        const uint32_t prev = first ? 0u : wm[i];
+        wm[i] = prev | (on ? bit : 0u);
     }
 }
 
-__global__ void weights_to_panel_kernel(const uint8_t* __restrict__ w, const int* __restrict__ perm, int64_t n,
-                                        uint8_t* out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = w[perm ? perm[i] : i] ? 1 : 0;
-}
-
-// ------------------------------------------------------------------------------------------
+// ------------------------------------------------------------------------------------------------
 // host side
 
 struct Layout {
@@ -897,6 +763,8 @@ int make_layout(Panel* p, unsigned fe_mask, Layout& L) {
         s.G = p->d[d].G;
         s.small = p->d[d].small;
         s.tab_off = L.S;
+        s.runbuf = nullptr;
+        s.seg_run_off = nullptr;
         L.S += s.G;
         if (s.small) {
             s.lds_off = L.Gs;
@@ -913,14 +781,9 @@ int make_layout(Panel* p, unsigned fe_mask, Layout& L) {
     return 0;
 }
 
+// dynamic LDS of the tile and long-segment kernels: small-dim accumulators and staged tables
 template <int CT>
-size_t apply_lds(const Layout& L, int mode) {
-    return sizeof(double) * (size_t)L.Gs * CT * (mode == kIter ? 2 : 1);
-}
-
-// dynamic LDS of the tile kernel: small-dim accumulators and staged tables
-template <int CT>
-size_t tile_lds(const Layout& L) {
+size_t tables_lds(const Layout& L) {
     return sizeof(double) * (size_t)L.Gs * CT * 2;
 }
 
@@ -934,82 +797,82 @@ unsigned long_grid(Panel* p) { return p->n_long > 0 ? stream_grid((size_t)p->n_l
 
 unsigned apply_grid(Panel* p) { return tile_grid(p) + long_grid(p); }
 
+// table-space results of one apply launch: small dims from the per-block partials, large dims from
+// the run partials (only_dim >= 0 restricts to one scatter dim)
+template <int CT>
+void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, double* q, unsigned grid,
+                   const int* active, hipStream_t s, bool time_runs) {
+    for (int o = 0; o < L.nd; ++o) {
+        if (only_dim >= 0 && o != only_dim) continue;
+        const ScatterDim& d = L.sd[o];
+        double* qo = q + d.tab_off * CT;
+        if (d.small) {
+            hipLaunchKernelGGL((reduce_small_kernel<CT>), dim3((unsigned)(d.G * CT)), dim3(kBlock), 0, s,
+                               (const double*)a.small_part, (int)grid, L.Gs, d.lds_off, d.G, qo);
+        } else {
+            const DimStruct& D = p->d[L.dims[o]];
+            if (time_runs) kstat_begin(kStatReduceRuns, s);
+            hipLaunchKernelGGL((reduce_runs_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock)), dim3(kBlock), 0,
+                               s, (const double*)a.sd[o].runbuf, (const int*)D.run_order,
+                               (const int64_t*)D.grp_run_off, d.G, qo, active);
+            if (time_runs)
+                kstat_end(kStatReduceRuns, s,
+                          8.0 * CT * (double)(D.n_runs + d.G) + 4.0 * (double)D.n_runs + 8.0 * (double)(d.G + 1));
+        }
+    }
+}
 
-// Launch one apply (mode) and reduce it into q (table space [S][CT]).
+// One operator product (kIter) or right-hand side (kData) into q (table space [S][CT]).
 template <int CT>
 int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* q, hipStream_t s) {
     const unsigned gt = tile_grid(p), gl = long_grid(p);
     const unsigned grid = gt + gl;
     const int sid = mode == kIter ? kStatApplyIter : kStatApplyData;
-    const bool timed = mode != kDiag && kstats_enabled();
+    const bool timed = kstats_enabled();
     if (timed) kstat_begin(sid, s);
     a.part_base = 0;
-    const size_t lt = tile_lds<CT>(L);
+    const size_t lt = tables_lds<CT>(L);
     if (mode == kIter)
         hipLaunchKernelGGL((tile_fast_kernel<CT, kIter>), dim3(gt), dim3(kBlock), lt, s, a);
-    else if (mode == kData)
-        hipLaunchKernelGGL((tile_fast_kernel<CT, kData>), dim3(gt), dim3(kBlock), lt, s, a);
     else
-        hipLaunchKernelGGL((tile_kernel<CT, kDiag>), dim3(gt), dim3(kBlock), lt, s, a);
+        hipLaunchKernelGGL((tile_fast_kernel<CT, kData>), dim3(gt), dim3(kBlock), lt, s, a);
     FEHIP_LAUNCH_CHECK("tile kernel");
     if (gl > 0) {
         ApplyArgs b = a;
         b.part_base = (int)gt;
-        const size_t lds = apply_lds<CT>(L, mode);
         if (mode == kIter)
-            hipLaunchKernelGGL((apply_kernel<CT, kIter>), dim3(gl), dim3(kBlock), lds, s, b);
-        else if (mode == kData)
-            hipLaunchKernelGGL((apply_kernel<CT, kData>), dim3(gl), dim3(kBlock), lds, s, b);
+            hipLaunchKernelGGL((long_kernel<CT, kIter>), dim3(gl), dim3(kBlock), lt, s, b);
         else
-            hipLaunchKernelGGL((apply_kernel<CT, kDiag>), dim3(gl), dim3(kBlock), lds, s, b);
+            hipLaunchKernelGGL((long_kernel<CT, kData>), dim3(gl), dim3(kBlock), lt, s, b);
         FEHIP_LAUNCH_CHECK("long-segment kernel");
     }
     if (timed) {
-        // algorithmic bytes: id streams, segment offsets, weights, table reads, run-partial and
+        // algorithmic bytes: id streams, segment offsets, row masks, table reads, run-partial and
         // LDS-partial writes (+ the data columns and permutation for the right-hand side)
         const double n = (double)p->n;
-        double bytes = n * 4.0 * L.nd + 8.0 * (double)(p->d[p->P].G + 1) + (a.w ? n : 0.0) +
+        double bytes = n * 4.0 * L.nd + 8.0 * (double)(p->d[p->P].G + 1) + (a.wm ? 4.0 * n : 0.0) +
                        8.0 * (double)grid * L.Gs * CT;
         for (int o = 0; o < L.nd; ++o) {
-            const double G = (double)L.sd[o].G;
-            bytes += 8.0 * CT * G;
+            bytes += 8.0 * CT * (double)L.sd[o].G;
             if (!L.sd[o].small) bytes += 8.0 * CT * (double)p->d[L.dims[o]].n_runs;
         }
         if (mode == kData) bytes += n * 8.0 * a.ncols + (p->perm ? 4.0 * n : 0.0);
         kstat_end(sid, s, bytes);
     }
-    for (int o = 0; o < L.nd; ++o) {
-        if (mode == kDiag && o != a.diag_dim) continue;
-        const ScatterDim& d = L.sd[o];
-        double* qo = q + d.tab_off * CT;
-        if (d.small) {
-            hipLaunchKernelGGL((reduce_small_kernel<CT>), dim3((unsigned)(d.G * CT)), dim3(kBlock), 0, s,
-                               a.small_part, (int)grid, L.Gs, d.lds_off, d.G, qo);
-        } else {
-            const DimStruct& D = p->d[L.dims[o]];
-            const bool timed_r = mode == kIter && kstats_enabled();
-            if (timed_r) kstat_begin(kStatReduceRuns, s);
-            hipLaunchKernelGGL((reduce_runs_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock)), dim3(kBlock), 0,
-                               s, (const double*)a.sd[o].runbuf, (const int*)D.run_order,
-                               (const int64_t*)D.grp_run_off, d.G, qo);
-            if (timed_r)
-                kstat_end(kStatReduceRuns, s,
-                          8.0 * CT * (double)(D.n_runs + d.G) + 4.0 * (double)D.n_runs + 8.0 * (double)(d.G + 1));
-        }
-        FEHIP_LAUNCH_CHECK("reduce kernel");
-    }
+    reduce_tables<CT>(p, L, a, -1, q, grid, mode == kIter ? a.active : nullptr, s, mode == kIter && timed);
+    FEHIP_LAUNCH_CHECK("reduce kernels");
     return 0;
 }
 
 struct Scratch {
-    double* tab = nullptr;      // 7 table arrays
+    std::vector<void*> owned;
+    double* tab = nullptr;      // CG arrays
+    double* dg = nullptr;       // per-slot Schur diagonals and weighted counts [slot][S][2]
     double* runbuf[kMaxLarge] = {nullptr, nullptr, nullptr};
     double* small_part = nullptr;
     double* part = nullptr;     // reduction partials
     CgState* st = nullptr;
     CgState* st_host = nullptr;
-    uint8_t* wpanel = nullptr;
-    std::vector<void*> owned;
     ~Scratch() {
         for (void* v : owned) (void)hipFree(v);
         if (st_host) (void)hipHostFree(st_host);
@@ -1024,57 +887,86 @@ struct Scratch {
 
 constexpr int kCgBlocks = 512;
 
-template <int CT>
-int solve_tile(Panel* p, const Layout& L, Scratch& sc, const double* const* xcols, double* const* ocols, int ncols,
-               const uint8_t* wpanel, const fehip_demean_opts& o, int to_work, TileStats& ts, hipStream_t s) {
-    const int64_t S = L.S;
-    const unsigned grid = apply_grid(p);
+ApplyArgs base_args(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm) {
     ApplyArgs a{};
     a.seg_off = p->seg_off;
     a.GP = p->d[p->P].G;
     a.elim = L.elim;
-    a.w = wpanel;
+    a.wm = wm;
     a.nd = L.nd;
     for (int i = 0; i < L.nd; ++i) {
         a.sd[i] = L.sd[i];
         if (!L.sd[i].small) a.sd[i].runbuf = sc.runbuf[L.sd[i].large_slot];
     }
     a.Gs = L.Gs;
-    for (int c = 0; c < 16; ++c) {
-        a.xc[c] = c < ncols ? xcols[c] : nullptr;
-        a.oc[c] = c < ncols ? ocols[c] : nullptr;
-    }
-    a.ncols = ncols;
     a.perm = p->perm;
     a.small_part = sc.small_part;
     a.tile_seg = p->tile_seg;
     a.n_tiles = p->n_tiles;
     a.long_segs = p->long_segs;
     a.n_long = p->n_long;
+    return a;
+}
+
+// Schur diagonal and weighted group counts of one slot into dg_slot ([S][2]).
+int slot_diagonal(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm, uint32_t bit, double* dg_slot,
+                  hipStream_t s) {
+    for (int od = 0; od < L.nd; ++od) {
+        ApplyArgs a = base_args(p, L, sc, wm);
+        a.diag_dim = od;
+        a.dbit = bit;
+        a.anybit = bit;
+        for (int c = 0; c < kMaxCT; ++c) a.cbit[c] = bit;
+        const unsigned gt = tile_grid(p), gl = long_grid(p);
+        const size_t lds = sizeof(double) * (size_t)L.Gs * 2 * 2;
+        a.part_base = 0;
+        hipLaunchKernelGGL(tile_diag_kernel, dim3(gt), dim3(kBlock), lds, s, a);
+        FEHIP_LAUNCH_CHECK("diag kernel");
+        if (gl > 0) {
+            ApplyArgs b = a;
+            b.part_base = (int)gt;
+            hipLaunchKernelGGL((long_kernel<2, kDiag>), dim3(gl), dim3(kBlock), lds, s, b);
+            FEHIP_LAUNCH_CHECK("long diag kernel");
+        }
+        reduce_tables<2>(p, L, a, od, dg_slot, gt + gl, nullptr, s, false);
+        FEHIP_LAUNCH_CHECK("diag reduce");
+    }
+    return 0;
+}
+
+template <int CT>
+int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int c0, int ncols, const uint32_t* wm,
+               const fehip_demean_opts& o, int to_work, TileStats& ts, hipStream_t s) {
+    const int64_t S = L.S;
+    ApplyArgs a = base_args(p, L, sc, wm);
+    ColSlots cs{};
+    a.anybit = 0;
+    for (int c = 0; c < kMaxCT; ++c) {
+        const bool in = c < ncols;
+        const int slot = wc.slot[c0 + (in ? c : 0)];  // padding columns borrow the first column's slot
+        a.cbit[c] = in ? (wm ? (1u << slot) : ~0u) : 0u;
+        a.anybit |= a.cbit[c];
+        a.xc[c] = in ? wc.x[c0 + c] : nullptr;
+        a.oc[c] = in ? wc.out[c0 + c] : nullptr;
+        cs.slot[c] = slot;
+    }
+    a.ncols = ncols;
     a.to_work = to_work;
     double* xa = sc.tab;           // alpha
     double* r = xa + S * CT;
     double* z = r + S * CT;
     double* pp = z + S * CT;
     double* q = pp + S * CT;
-    double* minv = q + S * CT;     // [S]
-    double* wcnt = minv + S;       // [S]
-    double* dg = wcnt + S;         // [S][2] diag + weight
+    double* minv = q + S * CT;     // [S][CT]
+    double* wcnt = minv + S * CT;  // [S][CT]
     auto t0 = std::chrono::steady_clock::now();
+    ts = TileStats();
+    for (int c = 0; c < ncols; ++c) ts.col_converged[c] = 1;
     if (S > 0) {
-        // Schur diagonal and weighted counts (CT = 2 pass per scattered dim)
-        for (int od = 0; od < L.nd; ++od) {
-            ApplyArgs ad = a;
-            ad.diag_dim = od;
-            // the diag pass writes 2 values per group: use table stride 2
-            int e = apply_and_reduce<2>(p, L, ad, kDiag, dg, s);
-            if (e) return e;
-        }
-        hipLaunchKernelGGL(make_minv_kernel, dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, s, S, dg, minv, wcnt,
-                           (const double*)dg);
+        hipLaunchKernelGGL((make_minv_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, s, S,
+                           (const double*)sc.dg, cs, minv, wcnt);
         FEHIP_LAUNCH_CHECK("minv");
-        // right-hand side b = D_o' W M_P x
-        int e = apply_and_reduce<CT>(p, L, a, kData, r, s);
+        int e = apply_and_reduce<CT>(p, L, a, kData, r, s);  // b = D_o' W M_P x
         if (e) return e;
         hipLaunchKernelGGL((cg_init_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, (const double*)r,
                            (const double*)minv, (const double*)wcnt, z, pp, xa, sc.part, sc.part + kCgBlocks * CT);
@@ -1110,39 +1002,29 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const double* const* xcol
         }
         FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, s), "cg state");
         FEHIP_CHECK(hipStreamSynchronize(s), "cg sync");
-        ts.iterations = 0;
-        ts.converged = 1;
-        ts.max_update = 0.0;
         for (int c = 0; c < ncols; ++c) {
             ts.iterations = std::max(ts.iterations, sc.st_host->iters[c]);
             if (sc.st_host->active[c]) ts.converged = 0;
             ts.max_update = std::max(ts.max_update, sc.st_host->maxn[c]);
+            ts.col_iters[c] = sc.st_host->iters[c];
+            ts.col_converged[c] = !sc.st_host->active[c];
         }
         ts.apply_launches = it;
-    } else {
-        ts.iterations = 0;
-        ts.converged = 1;
-        ts.max_update = 0.0;
-        ts.apply_launches = 0;
     }
-    // final pass over the columns
-    {
-        if (kstats_enabled()) kstat_begin(kStatFinalize, s);
-        a.alpha = xa;
-        hipLaunchKernelGGL((tile_fast_kernel<CT, kFinal>), dim3(tile_grid(p)), dim3(kBlock), tile_lds<CT>(L), s, a);
-        FEHIP_LAUNCH_CHECK("finalize tile kernel");
-        if (p->n_long > 0) {
-            const size_t lds = sizeof(double) * (size_t)L.Gs * CT;
-            hipLaunchKernelGGL((finalize_kernel<CT>), dim3(long_grid(p)), dim3(kBlock), lds, s, a, (const double*)xa,
-                               to_work);
-            FEHIP_LAUNCH_CHECK("finalize kernel");
-        }
-        if (kstats_enabled()) {
-            double bytes = 16.0 * (double)p->n * ncols + 4.0 * (double)p->n * L.nd + 8.0 * (double)(a.GP + 1) +
-                           (a.w ? (double)p->n : 0.0) + (p->perm ? 4.0 * (double)p->n : 0.0);
-            for (int o = 0; o < L.nd; ++o) bytes += 8.0 * CT * (double)L.sd[o].G;
-            kstat_end(kStatFinalize, s, bytes);
-        }
+    // final pass over the columns: M_P (x - D_o a)
+    a.alpha = xa;
+    if (kstats_enabled()) kstat_begin(kStatFinalize, s);
+    hipLaunchKernelGGL((tile_fast_kernel<CT, kFinal>), dim3(tile_grid(p)), dim3(kBlock), tables_lds<CT>(L), s, a);
+    FEHIP_LAUNCH_CHECK("finalize kernel");
+    if (p->n_long > 0) {
+        hipLaunchKernelGGL((long_kernel<CT, kFinal>), dim3(long_grid(p)), dim3(kBlock), tables_lds<CT>(L), s, a);
+        FEHIP_LAUNCH_CHECK("long finalize kernel");
+    }
+    if (kstats_enabled()) {
+        double bytes = 16.0 * (double)p->n * ncols + 4.0 * (double)p->n * L.nd + 8.0 * (double)(a.GP + 1) +
+                       (a.wm ? 4.0 * (double)p->n : 0.0) + (p->perm ? 4.0 * (double)p->n : 0.0);
+        for (int od = 0; od < L.nd; ++od) bytes += 8.0 * CT * (double)L.sd[od].G;
+        kstat_end(kStatFinalize, s, bytes);
     }
     FEHIP_CHECK(hipStreamSynchronize(s), "within sync");
     ts.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -1151,29 +1033,35 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const double* const* xcol
 
 }  // namespace
 
-int within_transform(Panel* p, const double* const* xcols, int ncols, const uint8_t* w_caller,
-                     const fehip_demean_opts& o, double* const* ocols, int to_work, WithinStats& ws, hipStream_t s) {
+int build_mask_words(Panel* p, const uint8_t* const* w_caller, int nslots, uint32_t* wm, hipStream_t s) {
+    if (nslots < 1 || nslots > 32) return set_error("1..32 request slots per batch (%d)", nslots);
+    for (int b = 0; b < nslots; ++b) {
+        hipLaunchKernelGGL(mask_words_kernel, dim3(stream_grid((size_t)p->n, kBlock)), dim3(kBlock), 0, s, w_caller[b],
+                           (const int*)p->perm, p->n, 1u << b, b == 0 ? 1 : 0, wm);
+        FEHIP_LAUNCH_CHECK("mask words");
+    }
+    return 0;
+}
+
+int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* wm, int nslots,
+                     const fehip_demean_opts& o, int to_work, WithinStats& ws, hipStream_t s) {
     unsigned mask = o.fe_mask ? o.fe_mask : ((1u << p->m) - 1u);
     mask &= (1u << p->m) - 1u;
     Layout L;
     int e = make_layout(p, mask, L);
     if (e) return e;
-    ws = WithinStats();
+    ws = WithinStats(ncols);
     if (ncols <= 0) return 0;
+    if (!wm) nslots = 1;
+    if (nslots < 1 || nslots > 32) return set_error("1..32 request slots per solve (%d)", nslots);
+    for (int c = 0; c < ncols; ++c)
+        if (wc.slot[c] < 0 || wc.slot[c] >= nslots) return set_error("column %d: slot out of range", c);
     Scratch sc;
-    const int64_t n = p->n;
-    if (w_caller) {
-        if ((e = sc.alloc(&sc.wpanel, (size_t)n))) return e;
-        hipLaunchKernelGGL(weights_to_panel_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, s, w_caller,
-                           p->perm, n, sc.wpanel);
-        FEHIP_LAUNCH_CHECK("weights");
-    }
-    // column-tile width: widest of {16, 12, 9, 8, 6, 4, 3, 2, 1} whose small-dim LDS tables fit
-    // next to the row tile (<= 28 KB of staged + accumulated tables)
-    int ct_max = 16;
+    // column-tile width: widest of {16, 12, 9, 8, 6, 4, 3, 2, 1} whose small-dim LDS tables fit next to
+    // the row tile (<= 28 KB of staged + accumulated tables)
+    int ct_max = 0;
     {
         static const int widths[] = {16, 12, 9, 8, 6, 4, 3, 2, 1};
-        ct_max = 0;
         for (int wdt : widths)
             if ((size_t)L.Gs * wdt * 16 <= 28 * 1024) {
                 ct_max = wdt;
@@ -1182,17 +1070,27 @@ int within_transform(Panel* p, const double* const* xcols, int ncols, const uint
         if (ct_max == 0) return set_error("small FE dims exceed the LDS budget (%d groups)", L.Gs);
     }
     const int64_t S = L.S;
-    if ((e = sc.alloc(&sc.tab, (size_t)(S * 16 * 5 + S * 4 + 16)))) return e;
+    const int64_t n = p->n;
+    if ((e = sc.alloc(&sc.tab, (size_t)(S * kMaxCT * 7 + 16)))) return e;
+    if ((e = sc.alloc(&sc.dg, (size_t)(S * 2 * nslots + 2)))) return e;
     for (int i = 0; i < L.nd; ++i)
         if (!L.sd[i].small) {
             const int64_t R = p->d[L.dims[i]].n_runs;
-            if ((e = sc.alloc(&sc.runbuf[L.sd[i].large_slot], (size_t)(R * 16 + 1)))) return e;
+            if ((e = sc.alloc(&sc.runbuf[L.sd[i].large_slot], (size_t)(R * kMaxCT + 1)))) return e;
         }
     const unsigned grid = apply_grid(p);
-    if ((e = sc.alloc(&sc.small_part, (size_t)grid * (L.Gs > 0 ? L.Gs : 1) * 16))) return e;
-    if ((e = sc.alloc(&sc.part, (size_t)kCgBlocks * 16 * 2))) return e;
+    if ((e = sc.alloc(&sc.small_part, (size_t)grid * (L.Gs > 0 ? L.Gs : 1) * kMaxCT))) return e;
+    if ((e = sc.alloc(&sc.part, (size_t)kCgBlocks * kMaxCT * 2))) return e;
     if ((e = sc.alloc(&sc.st, 1))) return e;
     FEHIP_CHECK(hipHostMalloc((void**)&sc.st_host, sizeof(CgState), hipHostMallocDefault), "pinned state");
+    // Schur diagonal of every slot in use
+    if (S > 0) {
+        std::vector<int> used(nslots, 0);
+        for (int c = 0; c < ncols; ++c) used[wc.slot[c]] = 1;
+        for (int b = 0; b < nslots; ++b)
+            if (used[b] && (e = slot_diagonal(p, L, sc, wm, wm ? (1u << b) : ~0u, sc.dg + (int64_t)b * S * 2, s)))
+                return e;
+    }
     for (int c0 = 0; c0 < ncols; c0 += ct_max) {
         const int nc = std::min(ct_max, ncols - c0);
         static const int widths[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
@@ -1203,18 +1101,16 @@ int within_transform(Panel* p, const double* const* xcols, int ncols, const uint
                 break;
             }
         TileStats ts;
-        const double* const* xt = xcols + c0;
-        double* const* ot = ocols + c0;
         switch (ct) {
-            case 1: e = solve_tile<1>(p, L, sc, xt, ot, nc, sc.wpanel, o, to_work, ts, s); break;
-            case 2: e = solve_tile<2>(p, L, sc, xt, ot, nc, sc.wpanel, o, to_work, ts, s); break;
-            case 3: e = solve_tile<3>(p, L, sc, xt, ot, nc, sc.wpanel, o, to_work, ts, s); break;
-            case 4: e = solve_tile<4>(p, L, sc, xt, ot, nc, sc.wpanel, o, to_work, ts, s); break;
-            case 6: e = solve_tile<6>(p, L, sc, xt, ot, nc, sc.wpanel, o, to_work, ts, s); break;
-            case 8: e = solve_tile<8>(p, L, sc, xt, ot, nc, sc.wpanel, o, to_work, ts, s); break;
-            case 9: e = solve_tile<9>(p, L, sc, xt, ot, nc, sc.wpanel, o, to_work, ts, s); break;
-            case 12: e = solve_tile<12>(p, L, sc, xt, ot, nc, sc.wpanel, o, to_work, ts, s); break;
-            default: e = solve_tile<16>(p, L, sc, xt, ot, nc, sc.wpanel, o, to_work, ts, s); break;
+            case 1: e = solve_tile<1>(p, L, sc, wc, c0, nc, wm, o, to_work, ts, s); break;
+            case 2: e = solve_tile<2>(p, L, sc, wc, c0, nc, wm, o, to_work, ts, s); break;
+            case 3: e = solve_tile<3>(p, L, sc, wc, c0, nc, wm, o, to_work, ts, s); break;
+            case 4: e = solve_tile<4>(p, L, sc, wc, c0, nc, wm, o, to_work, ts, s); break;
+            case 6: e = solve_tile<6>(p, L, sc, wc, c0, nc, wm, o, to_work, ts, s); break;
+            case 8: e = solve_tile<8>(p, L, sc, wc, c0, nc, wm, o, to_work, ts, s); break;
+            case 9: e = solve_tile<9>(p, L, sc, wc, c0, nc, wm, o, to_work, ts, s); break;
+            case 12: e = solve_tile<12>(p, L, sc, wc, c0, nc, wm, o, to_work, ts, s); break;
+            default: e = solve_tile<16>(p, L, sc, wc, c0, nc, wm, o, to_work, ts, s); break;
         }
         if (e) return e;
         ws.iterations = std::max(ws.iterations, ts.iterations);
@@ -1222,9 +1118,11 @@ int within_transform(Panel* p, const double* const* xcols, int ncols, const uint
         ws.max_update = std::max(ws.max_update, ts.max_update);
         ws.seconds += ts.seconds;
         ws.apply_launches += ts.apply_launches;
-        // algorithmic bytes: ids (+weights) per apply, columns read twice and written once
-        const int64_t id_bytes = n * 4 * (L.nd + (L.elim ? 0 : 0)) + (int64_t)(p->d[p->P].G + 1) * 8 +
-                                 (sc.wpanel ? n : 0);
+        for (int c = 0; c < nc; ++c) {
+            ws.col_iters[c0 + c] = ts.col_iters[c];
+            ws.col_converged[c0 + c] = ts.col_converged[c];
+        }
+        const int64_t id_bytes = n * 4 * L.nd + (int64_t)(p->d[p->P].G + 1) * 8 + (wm ? 4 * n : 0);
         ws.bytes += ts.apply_launches * id_bytes + (int64_t)3 * n * 8 * nc;
     }
     return 0;
@@ -1238,12 +1136,27 @@ extern "C" int fehip_demean(fehip_panel* panel, double* d_cols, int n_cols, int6
                             const fehip_demean_opts* opts, fehip_demean_stats* stats) {
     (void)hipGetLastError();  // start from a clean error state
     if (!panel || !opts) return set_error("fehip_demean: null argument");
-    if (n_cols < 0 || (n_cols > 0 && (!d_cols || ld < ((Panel*)panel)->n))) return set_error("fehip_demean: bad columns");
     Panel* p = reinterpret_cast<Panel*>(panel);
-    std::vector<double*> cols(n_cols);
-    for (int c = 0; c < n_cols; ++c) cols[c] = d_cols + (int64_t)c * ld;
-    WithinStats ws;
-    int e = within_transform(p, cols.data(), n_cols, d_weights, *opts, cols.data(), 0, ws, 0);
+    if (n_cols < 0 || (n_cols > 0 && (!d_cols || ld < p->n))) return set_error("fehip_demean: bad columns");
+    WithinCols wc(n_cols);
+    for (int c = 0; c < n_cols; ++c) {
+        wc.x[c] = d_cols + (int64_t)c * ld;
+        wc.out[c] = d_cols + (int64_t)c * ld;
+        wc.slot[c] = 0;
+    }
+    uint32_t* wm = nullptr;
+    if (d_weights) {
+        int e = dalloc(&wm, (size_t)p->n);
+        if (e) return e;
+        e = build_mask_words(p, &d_weights, 1, wm, 0);
+        if (e) {
+            dfree(wm);
+            return e;
+        }
+    }
+    WithinStats ws(n_cols);
+    int e = within_transform(p, wc, n_cols, wm, 1, *opts, 0, ws, 0);
+    dfree(wm);
     if (e) return e;
     if (stats) {
         stats->iterations = ws.iterations;
