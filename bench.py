@@ -63,6 +63,124 @@ def cpu_baseline(n_workers_full, n_periods, n_firms_full, sample_workers, seed):
     }
 
 
+def _kstat_summary(lib):
+    out = {}
+    for k in range(6):
+        v = kstat(lib, k)
+        if v["launches"]:
+            out[v["name"]] = {"launches": v["launches"], "total_ms": v["ms"]}
+    return out
+
+
+def _fit_summary(fits):
+    return {"mean_fe_iterations": float(np.mean([f.res["fe_iterations"] for f in fits])),
+            "max_fe_iterations": int(max(f.res["fe_iterations"] for f in fits)),
+            "sum_t_se_s": float(sum(f.res["t_se"] for f in fits)),
+            "sum_t_regression_s": float(sum(f.res["t_regression"] for f in fits))}
+
+
+def serving_leg(args, lib):
+    """Config 5: request stream on a 10M-row, 4-FE device-resident panel (worker, firm, time,
+    region), 20 candidate columns.  Each request: k ~ U{2..12} consecutive candidate columns from a
+    random start, 1-3 of the 4 FE dims, 0..min(3, |FE|) cluster dims drawn from its FE dims, and a
+    row mask from a pool of Bernoulli(p) masks with p ~ U(0.05, 1).  Measured: sequential plain FP64
+    solves one at a time (fehip_estimate), the server with everything queued at once (closed loop),
+    and the server under Poisson arrivals at 1.5x the sequential rate (open loop)."""
+    from fe_gpu_amd import api, synth
+    from fe_gpu_amd.device import DeviceBuffer
+    t0 = time.perf_counter()
+    p = synth.serving_panel(seed=505, n_workers=args.serve_workers)
+    n = p.n
+    pan = api.Panel.from_host(p.fe)
+    dy = DeviceBuffer.from_array(p.y)
+    dX = DeviceBuffer.from_array(np.asfortranarray(p.W))
+    rng = np.random.default_rng(5050)
+    pool_p = rng.uniform(0.05, 1.0, args.serve_masks)
+    masks = [DeviceBuffer.from_array((rng.random(n) < q).astype(np.uint8)) for q in pool_p]
+    del p
+    reqs = []
+    for _ in range(args.serve_requests):
+        k = int(rng.integers(2, 13))
+        j0 = int(rng.integers(0, 20 - k + 1))
+        nfe = int(rng.integers(1, 4))
+        dims = sorted(int(d) for d in rng.choice(4, nfe, replace=False))
+        ncl = int(rng.integers(0, min(3, nfe) + 1))
+        cl = tuple(sorted(int(d) + 1 for d in rng.choice(dims, ncl, replace=False))) if ncl else ()
+        mi = int(rng.integers(0, len(masks)))
+        reqs.append(api.Request(dy.ptr, dX.ptr + 8 * n * j0, k, weights_ptr=masks[mi].ptr,
+                                options=dict(cluster=cl, fe_mask=sum(1 << d for d in dims), tol=args.tol,
+                                             max_iter=2000)))
+    log(f"serving panel {n} rows, {len(reqs)} requests, setup {time.perf_counter() - t0:.1f} s")
+    # sequential plain FP64, one request at a time
+    n_seq = min(args.serve_seq, len(reqs))
+    api.estimate_device(pan, reqs[0].y_ptr, reqs[0].W_ptr, reqs[0].K, weights_ptr=reqs[0].weights_ptr,
+                        **reqs[0].options)  # warm-up
+    seq_fits, seq_lat = [], []
+    lib.fehip_kstats_reset()
+    lib.fehip_kstats_enable(1)
+    ts = time.perf_counter()
+    for q in reqs[:n_seq]:
+        t = time.perf_counter()
+        seq_fits.append(api.estimate_device(pan, q.y_ptr, q.W_ptr, q.K, weights_ptr=q.weights_ptr, **q.options))
+        seq_lat.append(time.perf_counter() - t)
+    seq_s = time.perf_counter() - ts
+    seq_k = _kstat_summary(lib)
+    lib.fehip_kstats_reset()
+    seq_rate = n_seq / seq_s
+    log(f"sequential: {n_seq} requests in {seq_s:.2f} s ({seq_rate:.1f}/s)")
+    # closed loop: every request queued at once
+    srv = api.Server(pan)
+    tc = time.perf_counter()
+    tickets = [srv.submit(q) for q in reqs]
+    outs = [srv.wait(t) for t in tickets]
+    closed_s = time.perf_counter() - tc
+    closed_k = _kstat_summary(lib)
+    lib.fehip_kstats_enable(0)
+    st_closed = srv.stats()
+    log(f"server closed loop: {len(reqs)} requests in {closed_s:.2f} s, {st_closed['batches']} batches")
+    b_err = max(float(np.max(np.abs(o[0].beta - f.beta) / np.maximum(np.abs(f.beta), 1e-300)))
+                for o, f in zip(outs[:n_seq], seq_fits))
+    s_err = max(float(np.max(np.abs(o[0].se - f.se) / np.maximum(np.abs(f.se), 1e-300)))
+                for o, f in zip(outs[:n_seq], seq_fits))
+    # open loop: Poisson arrivals at 1.5x the sequential rate
+    srv.reset_stats()
+    n_open = min(args.serve_open, len(reqs))
+    rate = 1.5 * seq_rate
+    arr = np.cumsum(rng.exponential(1.0 / rate, n_open))
+    to = time.perf_counter()
+    tickets = []
+    for i in range(n_open):
+        dt = arr[i] - (time.perf_counter() - to)
+        if dt > 0:
+            time.sleep(dt)
+        tickets.append(srv.submit(reqs[i]))
+    for t in tickets:
+        srv.wait(t)
+    open_s = time.perf_counter() - to
+    st_open = srv.stats()
+    srv.close()
+    pan.close()
+    sl = np.array(seq_lat)
+    return {
+        "workload": f"serving panel {n} rows ({args.serve_workers} workers x 20 periods, 100000 firms, 50 regions), "
+                    f"4 FE dims, 20 candidate columns; {len(reqs)} requests (k 2-12, 1-3 FEs, 0-3 clusters, "
+                    f"row masks 5-100%), tol {args.tol}",
+        "sequential": {"requests": n_seq, "regressions_per_s": seq_rate, "p50_ms": float(np.percentile(sl, 50)) * 1e3,
+                       "p99_ms": float(np.percentile(sl, 99)) * 1e3, "fits": _fit_summary(seq_fits),
+                       "kernels": seq_k},
+        "closed_loop": {"requests": len(reqs), "regressions_per_s": len(reqs) / closed_s,
+                        "p50_ms": st_closed["p50_s"] * 1e3, "p99_ms": st_closed["p99_s"] * 1e3,
+                        "batches": st_closed["batches"], "mean_batch": st_closed["mean_batch"],
+                        "fits": _fit_summary([o[0] for o in outs]), "kernels": closed_k},
+        "open_loop": {"requests": n_open, "arrival_rate_per_s": rate, "regressions_per_s": n_open / open_s,
+                      "p50_ms": st_open["p50_s"] * 1e3, "p99_ms": st_open["p99_s"] * 1e3,
+                      "batches": st_open["batches"], "mean_batch": st_open["mean_batch"]},
+        "speedup_closed_vs_sequential": (len(reqs) / closed_s) / seq_rate,
+        "beta_max_rel_err_vs_sequential": b_err,
+        "se_max_rel_err_vs_sequential": s_err,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,10 +193,17 @@ def main():
     ap.add_argument("--seed", type=int, default=202)
     ap.add_argument("--cpu-sample-workers", type=int, default=100_000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-only", action="store_true", help="run only the request-stream leg")
+    ap.add_argument("--serve-workers", type=int, default=500_000)
+    ap.add_argument("--serve-requests", type=int, default=400)
+    ap.add_argument("--serve-seq", type=int, default=100)
+    ap.add_argument("--serve-open", type=int, default=200)
+    ap.add_argument("--serve-masks", type=int, default=32)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; multi-GPU sharding is design-only "
-                          "(see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on exactly one GPU; sharding across GPUs is a design, "
+                          "not a measured path", "n_gpus": args.gpus}))
         sys.exit(2)
 
     from fe_gpu_amd import api, device
@@ -87,6 +212,9 @@ def main():
 
     info = device.init(0)
     lib = load()
+    if args.serving_only:
+        print(json.dumps({"serving": serving_leg(args, lib)}), flush=True)
+        return
     nw, T, nf = args.workers, args.periods, args.firms
     n = nw * T
     log(f"device {info.name.decode()}; generating {n} rows ({nw} workers x {T} periods, {nf} firms)")
@@ -130,6 +258,12 @@ def main():
     if not args.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample")
         cpu = cpu_baseline(nw, T, nf, min(args.cpu_sample_workers, nw), args.seed)
+    serving = None
+    if not args.no_serving:
+        del d_fe, d_y, d_W
+        pan.close()
+        log("serving leg")
+        serving = serving_leg(args, lib)
     lat = np.array(step_s) * 1e3
     out = {
         "metric": "FE-regressions/sec (150M-row 3-FE OLS, 3-way clustered SEs, FP64)",
@@ -161,6 +295,7 @@ def main():
         "beta": [float(b) for b in fit.beta],
         "se": [float(s) for s in fit.se],
         "reference_h100_same_shape_s": 57.750,
+        "serving": serving,
     }
     print(json.dumps(out), flush=True)
 

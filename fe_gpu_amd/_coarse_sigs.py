@@ -40,6 +40,21 @@ class Result(C.Structure):
                 ("t_se", C.c_double)]
 
 
+class Request(C.Structure):
+    _fields_ = [("d_y", VP), ("d_W", VP), ("d_Z", VP), ("K", I32), ("L", I32), ("ld", I64), ("d_weights", VP),
+                ("opts", Opts), ("result", C.POINTER(Result)), ("beta", DP), ("se", DP), ("vcov", DP)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget", I64), ("max_batch", I32), ("reserved", I32)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", I64), ("completed", I64), ("failed", I64), ("batches", I64), ("queued", I64),
+                ("budget_bytes", I64), ("mean_batch", C.c_double), ("busy_s", C.c_double), ("span_s", C.c_double),
+                ("p50_s", C.c_double), ("p99_s", C.c_double), ("max_s", C.c_double)]
+
+
 COARSE = [
     ("fehip_panel_create", C.c_int, [C.POINTER(VP), C.c_int, I64, C.POINTER(VP)]),
     ("fehip_panel_destroy", C.c_int, [VP]),
@@ -47,6 +62,13 @@ COARSE = [
     ("fehip_demean", C.c_int, [VP, VP, C.c_int, I64, VP, C.POINTER(DemeanOpts), C.POINTER(DemeanStats)]),
     ("fehip_estimate", C.c_int, [VP, VP, VP, C.c_int, VP, C.c_int, I64, VP, C.POINTER(Opts), C.POINTER(Result), DP,
                                  DP, DP]),
+    ("fehip_estimate_batch", C.c_int, [VP, C.POINTER(Request), C.c_int]),
+    ("fehip_server_create", C.c_int, [VP, C.POINTER(ServerOpts), C.POINTER(VP)]),
+    ("fehip_server_submit", C.c_int, [VP, C.POINTER(Request), C.POINTER(I64)]),
+    ("fehip_server_wait", C.c_int, [VP, I64, DP]),
+    ("fehip_server_get_stats", C.c_int, [VP, C.POINTER(ServerStats)]),
+    ("fehip_server_reset_stats", C.c_int, [VP]),
+    ("fehip_server_destroy", C.c_int, [VP]),
     ("fehip_synth_worker_firm", C.c_int, [I64, I32, I32, C.c_double, C.c_uint64, VP, VP, VP]),
     ("fehip_kstats_enable", C.c_int, [C.c_int]),
     ("fehip_kstats_reset", C.c_int, []),

@@ -73,34 +73,134 @@ class Fit:
     res: dict
 
 
+class _Spec:
+    """Host-side storage of one request's options and outputs (kept alive across the C call)."""
+
+    def __init__(self, K, cluster=(), iv_reg=(), iv_inst=(), tol=1e-8, max_iter=500, fe_mask=0, method=0,
+                 precision=0, check_every=4):
+        self.K = K
+        self.opts = S.Opts()
+        self.opts.demean = demean_opts(tol, max_iter, fe_mask, method, precision, check_every)
+        cl = list(cluster)[:4]
+        self.opts.n_cluster = len(cl)
+        for i, d in enumerate(cl):
+            self.opts.cluster_dims[i] = int(d)
+        self._ivr = (S.I32 * max(1, len(iv_reg)))(*[int(v) for v in iv_reg])
+        self._ivi = (S.I32 * max(1, len(iv_inst)))(*[int(v) for v in iv_inst])
+        self.opts.n_iv_reg = len(iv_reg)
+        self.opts.iv_reg = C.cast(self._ivr, C.POINTER(S.I32))
+        self.opts.n_iv_inst = len(iv_inst)
+        self.opts.iv_inst = C.cast(self._ivi, C.POINTER(S.I32))
+        self.r = S.Result()
+        k1 = max(K, 1)
+        self.beta = np.zeros(k1)
+        self.se = np.zeros(k1)
+        self.vcov = np.zeros((k1, k1), order="F")
+
+    def fit(self) -> "Fit":
+        r, K = self.r, self.K
+        res = {f: getattr(r, f) for f, _ in S.Result._fields_}
+        res["cluster_dims"] = list(r.cluster_dims)[: r.n_cluster_dims]
+        res["n_clusters"] = list(r.n_clusters)
+        return Fit(self.beta[:K].copy(), self.se[:K].copy(), np.array(self.vcov)[:K, :K], res)
+
+
+def _dp(a):
+    return a.ctypes.data_as(S.DP)
+
+
 def estimate_device(panel: Panel, y_ptr: int, W_ptr: int, K: int, Z_ptr: int = 0, L: int = 0, ld: int = None,
-                    weights_ptr: int = 0, cluster=(), iv_reg=(), iv_inst=(), tol=1e-8, max_iter=500, fe_mask=0,
-                    method=0, precision=0, check_every=4) -> Fit:
+                    weights_ptr: int = 0, **kw) -> Fit:
     lib = load()
     ld = panel.n_obs if ld is None else ld
-    opts = S.Opts()
-    opts.demean = demean_opts(tol, max_iter, fe_mask, method, precision, check_every)
-    cl = list(cluster)[:4]
-    opts.n_cluster = len(cl)
-    for i, d in enumerate(cl):
-        opts.cluster_dims[i] = int(d)
-    ivr = (S.I32 * max(1, len(iv_reg)))(*[int(v) for v in iv_reg])
-    ivi = (S.I32 * max(1, len(iv_inst)))(*[int(v) for v in iv_inst])
-    opts.n_iv_reg = len(iv_reg)
-    opts.iv_reg = C.cast(ivr, C.POINTER(S.I32))
-    opts.n_iv_inst = len(iv_inst)
-    opts.iv_inst = C.cast(ivi, C.POINTER(S.I32))
-    r = S.Result()
-    beta = np.zeros(max(K, 1))
-    se = np.zeros(max(K, 1))
-    vcov = np.zeros((max(K, 1), max(K, 1)), order="F")
-    dp = lambda a: a.ctypes.data_as(S.DP)  # noqa: E731
+    sp = _Spec(K, **kw)
     check(lib.fehip_estimate(panel.handle, y_ptr, W_ptr or None, K, Z_ptr or None, L, ld, weights_ptr or None,
-                             C.byref(opts), C.byref(r), dp(beta), dp(se), dp(vcov)), "fehip_estimate")
-    res = {f: getattr(r, f) for f, _ in S.Result._fields_}
-    res["cluster_dims"] = list(r.cluster_dims)[: r.n_cluster_dims]
-    res["n_clusters"] = list(r.n_clusters)
-    return Fit(beta[:K], se[:K], np.array(vcov)[:K, :K], res)
+                             C.byref(sp.opts), C.byref(sp.r), _dp(sp.beta), _dp(sp.se), _dp(sp.vcov)),
+          "fehip_estimate")
+    return sp.fit()
+
+
+@dataclass
+class Request:
+    """One specification of a batch (device pointers; options as for estimate_device)."""
+    y_ptr: int
+    W_ptr: int
+    K: int
+    Z_ptr: int = 0
+    L: int = 0
+    ld: int = None
+    weights_ptr: int = 0
+    options: dict = None
+
+
+def estimate_batch(panel: Panel, requests) -> list:
+    """fehip_estimate_batch: all requests in one call; returns one Fit per request."""
+    lib = load()
+    specs = []
+    arr = (S.Request * max(1, len(requests)))()
+    for i, q in enumerate(requests):
+        sp = _Spec(q.K, **(q.options or {}))
+        specs.append(sp)
+        _fill_request(arr[i], panel, q, sp)
+    check(lib.fehip_estimate_batch(panel.handle, arr, len(requests)), "fehip_estimate_batch")
+    return [sp.fit() for sp in specs]
+
+
+def _fill_request(a, panel, q, sp):
+    a.d_y, a.d_W, a.d_Z = q.y_ptr, q.W_ptr or None, q.Z_ptr or None
+    a.K, a.L = q.K, q.L
+    a.ld = panel.n_obs if q.ld is None else q.ld
+    a.d_weights = q.weights_ptr or None
+    a.opts = sp.opts
+    a.result = C.pointer(sp.r)
+    a.beta, a.se, a.vcov = _dp(sp.beta), _dp(sp.se), _dp(sp.vcov)
+
+
+class Server:
+    """fehip_server: a worker thread that batches queued requests on a device-resident panel."""
+
+    def __init__(self, panel: Panel, hbm_budget: int = 0, max_batch: int = 0):
+        self.lib = load()
+        self.panel = panel
+        o = S.ServerOpts(int(hbm_budget), int(max_batch), 0)
+        h = C.c_void_p()
+        check(self.lib.fehip_server_create(panel.handle, C.byref(o), C.byref(h)), "fehip_server_create")
+        self.handle = h.value
+        self._pending = {}
+
+    def submit(self, q: Request) -> int:
+        sp = _Spec(q.K, **(q.options or {}))
+        req = S.Request()
+        _fill_request(req, self.panel, q, sp)
+        t = S.I64()
+        check(self.lib.fehip_server_submit(self.handle, C.byref(req), C.byref(t)), "fehip_server_submit")
+        self._pending[t.value] = sp   # keeps the host outputs alive until wait()
+        return t.value
+
+    def wait(self, ticket: int):
+        lat = C.c_double()
+        sp = self._pending.pop(ticket)
+        check(self.lib.fehip_server_wait(self.handle, ticket, C.byref(lat)), "fehip_server_wait")
+        return sp.fit(), lat.value
+
+    def stats(self) -> dict:
+        st = S.ServerStats()
+        check(self.lib.fehip_server_get_stats(self.handle, C.byref(st)), "fehip_server_get_stats")
+        return {f: getattr(st, f) for f, _ in S.ServerStats._fields_}
+
+    def reset_stats(self):
+        check(self.lib.fehip_server_reset_stats(self.handle), "fehip_server_reset_stats")
+
+    def close(self):
+        if self.handle:
+            check(self.lib.fehip_server_destroy(self.handle), "fehip_server_destroy")
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def estimate(y, W, fe, Z=None, weights=None, panel: Panel = None, **kw) -> Fit:

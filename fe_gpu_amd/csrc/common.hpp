@@ -52,6 +52,17 @@ void release_workspaces();
 // Pinned host scalar buffer for D2H of small results.
 double* host_scalars();
 
+// ---- caching allocator for per-call device temporaries ----
+// Freed blocks are kept per device and handed out again to requests of a similar size, so a stream
+// of regressions does not map and clear fresh device memory for every call.  Reuse is ordered by the
+// default stream every user runs on.  cache_release returns the cached blocks to the driver.
+void* cache_alloc(size_t bytes);
+void cache_free(void* p);
+void cache_release();
+size_t cache_idle_bytes();  // bytes held by cached (free) blocks on the current device
+// Per-thread pinned host scratch (grow-only) for small device-to-host state copies.
+void* pinned_scratch(size_t bytes);
+
 // FP64 add into LDS through an explicit LDS-address-space pointer (ds_add_f64); a flat
 // pointer would make the backend emit a shared-aperture test.
 __device__ __forceinline__ void lds_add(double* p, double v) {

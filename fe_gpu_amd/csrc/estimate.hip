@@ -20,6 +20,7 @@
 
 #include "dense.hpp"
 #include "panel.hpp"
+#include "estimate.hpp"
 #include "within.hpp"
 
 namespace fehip {
@@ -520,14 +521,26 @@ int gram_fused(const double* X, int64_t ld, int C, int64_t n, double* part, std:
 
 struct DevBuf {
     std::vector<void*> v;
-    ~DevBuf() {
-        for (void* p : v) (void)hipFree(p);
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { clear(); }
+    void clear() {
+        for (void* p : v) cache_free(p);
+        v.clear();
     }
     template <typename T>
     int get(T** p, size_t count) {
-        int e = dalloc(p, count);
+        int e = talloc(p, count);
         if (!e) v.push_back((void*)*p);
         return e;
+    }
+    void release(void* p) {
+        auto it = std::find(v.begin(), v.end(), p);
+        if (it != v.end()) {
+            cache_free(p);
+            v.erase(it);
+        }
     }
 };
 
@@ -566,18 +579,48 @@ void finalize_stats(fehip_result* r, int64_t n_obs) {
     r->f_stat = (dfm > 0.0 && dfr > 0.0 && dev > EPS) ? (ssr / dfm) / (dev / dfr) : 0.0;
 }
 
-}  // namespace
-}  // namespace fehip
 
-using namespace fehip;
+// One request of a batch: its inputs, and what the preparation step leaves for the solve and the
+// normal equations / sandwich step.
+struct ReqState {
+    const double* d_y = nullptr;
+    const double* d_W = nullptr;
+    const double* d_Z = nullptr;
+    int K = 0, L = 0;
+    int64_t ld = 0;
+    const uint8_t* d_weights = nullptr;
+    const fehip_opts* opts = nullptr;
+    fehip_result* result = nullptr;
+    double *beta = nullptr, *se = nullptr, *vcov = nullptr;
+    // prepared
+    unsigned fe_mask = 0;
+    int ncd = 0;
+    int cdims[3] = {0, 0, 0};
+    DevBuf db;
+    double* part = nullptr;
+    uint8_t* wpanel = nullptr;
+    int64_t N = 0;
+    int Geff[FEHIP_MAX_FE] = {0};
+    int dof_fes = 0, dof_nested = 0;
+    int C = 0;
+    double* work = nullptr;  // n x C within-transformed [y W Z], panel order
+    clk::time_point t0, t1;
+};
 
-extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const double* d_W, int K, const double* d_Z,
-                              int L, int64_t ld, const uint8_t* d_weights, const fehip_opts* opts,
-                              fehip_result* result, double* beta, double* se, double* vcov) {
-    (void)hipGetLastError();  // start from a clean error state
-    if (!panel || !opts || !result || !d_y) return set_error("fehip_estimate: null argument");
-    Panel* p = reinterpret_cast<Panel*>(panel);
+// Validation, cluster dims, weights in panel order, N, TSS, effective group counts and DoF, and
+// the work buffer of the request.
+int prepare_request(Panel* p, ReqState& s) {
     const int64_t n = p->n;
+    const int K = s.K, L = s.L;
+    const int64_t ld = s.ld;
+    const double *d_y = s.d_y, *d_W = s.d_W, *d_Z = s.d_Z;
+    const uint8_t* d_weights = s.d_weights;
+    const fehip_opts* opts = s.opts;
+    fehip_result* result = s.result;
+    double *beta = s.beta, *se = s.se, *vcov = s.vcov;
+    (void)d_W;
+    (void)d_Z;
+    if (!opts || !result || !d_y || (K > 0 && (!beta || !se))) return set_error("fehip_estimate: null argument");
     if (K < 0 || L < 0 || (K > 0 && !d_W) || (L > 0 && !d_Z) || ld < n) return set_error("fehip_estimate: bad columns");
     std::memset(result, 0, sizeof(*result));
     for (int k = 0; k < K; ++k) {
@@ -585,7 +628,7 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
         se[k] = 0.0;
     }
     if (vcov) std::fill(vcov, vcov + (size_t)K * K, 0.0);
-    DevBuf db;
+    DevBuf& db = s.db;
     int e;
     const auto t0 = clk::now();
     const unsigned fe_mask = (opts->demean.fe_mask ? opts->demean.fe_mask : ((1u << p->m) - 1u)) & ((1u << p->m) - 1u);
@@ -646,31 +689,52 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
         if ((fe_mask >> d) & 1u) dof_fes += std::max(0, Geff[d] - 1);
     for (int j = 0; j < ncd; ++j) dof_nested += std::max(0, Geff[cdims[j] - 1] - 1);
     result->dof_fes = dof_fes;
+    s.fe_mask = fe_mask;
+    s.ncd = ncd;
+    for (int j = 0; j < ncd; ++j) s.cdims[j] = cdims[j];
+    s.part = part;
+    s.wpanel = wpanel;
+    s.N = N;
+    for (int d = 0; d < p->m; ++d) s.Geff[d] = Geff[d];
+    s.dof_fes = dof_fes;
+    s.dof_nested = dof_nested;
+    s.t0 = t0;
+    s.C = 1 + K + L;
+    if ((e = db.get(&s.work, (size_t)n * s.C))) return e;
+    return 0;
+}
 
-    // ---- within transform of [y W Z] into the work buffer (panel order)
-    const int C = 1 + K + L;
-    double* work = nullptr;
-    if ((e = db.get(&work, (size_t)n * C))) return e;
-    WithinCols wc(C);
-    wc.x[0] = d_y;
-    for (int k = 0; k < K; ++k) wc.x[1 + k] = d_W + (int64_t)k * ld;
-    for (int l = 0; l < L; ++l) wc.x[1 + K + l] = d_Z + (int64_t)l * ld;
-    for (int c = 0; c < C; ++c) wc.out[c] = work + (int64_t)c * n;
-    uint32_t* wm = nullptr;
-    if (d_weights) {
-        if ((e = db.get(&wm, (size_t)n))) return e;
-        if ((e = build_mask_words(p, &d_weights, 1, wm, 0))) return e;
-    }
-    fehip_demean_opts dopt = opts->demean;
-    dopt.fe_mask = fe_mask;
-    WithinStats ws(C);
-    if ((e = within_transform(p, wc, C, wm, 1, dopt, 1, ws, 0))) return e;
-    result->converged = ws.converged;
-    result->fe_iterations = ws.iterations;
-    result->max_update = ws.max_update;
-    const auto t1 = clk::now();
-    result->t_demean = std::chrono::duration<double>(t1 - t0).count();
-    if (K <= 0) return set_ok();
+// Normal equations, IV stages, column filter, Cholesky, residual and the sandwich of a request whose
+// work buffer holds the within-transformed columns.
+int finish_request(Panel* p, ReqState& s) {
+    const int64_t n = p->n;
+    const int K = s.K, L = s.L;
+    const int64_t ld = s.ld;
+    const double *d_y = s.d_y, *d_W = s.d_W, *d_Z = s.d_Z;
+    const uint8_t* d_weights = s.d_weights;
+    const fehip_opts* opts = s.opts;
+    fehip_result* result = s.result;
+    double *beta = s.beta, *se = s.se, *vcov = s.vcov;
+    (void)d_y;
+    (void)d_W;
+    (void)d_Z;
+    (void)d_weights;
+    (void)ld;
+    DevBuf& db = s.db;
+    int e;
+    const unsigned fe_mask = s.fe_mask;
+    (void)fe_mask;
+    const int ncd = s.ncd;
+    const int* cdims = s.cdims;
+    double* part = s.part;
+    uint8_t* wpanel = s.wpanel;
+    const int64_t N = s.N;
+    const int* Geff = s.Geff;
+    const int dof_fes = s.dof_fes, dof_nested = s.dof_nested;
+    const int C = s.C;
+    double* work = s.work;
+    const auto t1 = s.t1;
+    if (K <= 0) return 0;
 
     // ---- Gram of [y W Z] (upper), host copy
     double* mpart = nullptr;
@@ -729,7 +793,7 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
         int info = dense::chol_solve_invert(Qzz, Qzy, n_aug, rhs, Qzinv);
         if (info != 0) {
             result->solver_info = info;
-            return set_ok();
+            return 0;
         }
         proj.assign((size_t)n_aug * K, 0.0);
         for (int k = 0; k < K; ++k)
@@ -772,7 +836,7 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
     result->dof_model = kept;
     if (kept == 0) {
         result->solver_info = -1;
-        return set_ok();
+        return 0;
     }
     std::vector<double> Qs((size_t)kept * kept), bs(kept), bsol, Qinv;
     for (int a = 0; a < kept; ++a) {
@@ -781,7 +845,7 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
     }
     int info = dense::chol_solve_invert(Qs, bs, kept, bsol, Qinv);
     result->solver_info = info;
-    if (info != 0) return set_ok();
+    if (info != 0) return 0;
     for (int a = 0; a < kept; ++a) beta[keep[a]] = bsol[a];
     result->rank = kept;
     const auto t2 = clk::now();
@@ -950,7 +1014,7 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
                 } else if (K <= 12) {
                     const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
                     if (nS * KT > S_cap) {
-                        if (S) (void)hipFree(S), db.v.pop_back();
+                        if (S) db.release(S);
                         S_cap = nS * KT;
                         if ((e = db.get(&S, (size_t)S_cap))) return e;
                     }
@@ -967,7 +1031,7 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
                     if ((e = meat_fused(S, K, KT, nullptr, nullptr, n_present, mpart, M))) return e;
                 } else {
                     if (nS * K > S_cap) {
-                        if (S) (void)hipFree(S), db.v.pop_back();
+                        if (S) db.release(S);
                         S_cap = nS * K;
                         if ((e = db.get(&S, (size_t)S_cap))) return e;
                     }
@@ -1033,5 +1097,148 @@ extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const doubl
     finalize_stats(result, N);
     FEHIP_CHECK(hipDeviceSynchronize(), "estimate sync");
     result->t_se = std::chrono::duration<double>(clk::now() - t2).count();
-    return set_ok();
+    return 0;
+}
+
+constexpr int kMaxSlots = 32;
+
+// Solve a batch: requests that absorb the same FE dims share the within-transform sweeps (up to
+// 32 requests per solve, one mask bit each); every request then takes its own normal-equation
+// and sandwich step.
+int run_batch(Panel* p, std::vector<ReqState>& rs) {
+    int e;
+    const int64_t n = p->n;
+    const unsigned all = (1u << p->m) - 1u;
+    for (auto& s : rs) {
+        if (!s.opts) return set_error("fehip_estimate: null options");
+        s.fe_mask = (s.opts->demean.fe_mask ? s.opts->demean.fe_mask : all) & all;
+    }
+    // device memory a chunk may take for its work buffers
+    size_t free_b = 0, total_b = 0;
+    FEHIP_CHECK(hipMemGetInfo(&free_b, &total_b), "memory info");
+    const double budget = 0.7 * (double)(free_b + cache_idle_bytes());
+    std::vector<int> done(rs.size(), 0);
+    for (size_t i0 = 0; i0 < rs.size(); ++i0) {
+        if (done[i0]) continue;
+        // next chunk: unsolved requests with the same FE set, in submission order, within the slot
+        // count and the memory budget (the first request always goes)
+        std::vector<int> chunk;
+        double bytes = 0.0;
+        for (size_t i = i0; i < rs.size() && (int)chunk.size() < kMaxSlots; ++i) {
+            if (done[i] || rs[i].fe_mask != rs[i0].fe_mask) continue;
+            const double b = 8.0 * (double)n * (1 + std::max(0, rs[i].K) + std::max(0, rs[i].L)) + 2.0 * (double)n;
+            if (!chunk.empty() && bytes + b > budget) break;
+            bytes += b;
+            chunk.push_back((int)i);
+        }
+        for (int i : chunk)
+            if ((e = prepare_request(p, rs[i]))) return e;
+        int ncols = 0;
+        bool weighted = false;
+        for (int i : chunk) {
+            ncols += rs[i].C;
+            weighted |= rs[i].d_weights != nullptr;
+        }
+        DevBuf cb;
+        uint32_t* wm = nullptr;
+        if (weighted) {
+            std::vector<const uint8_t*> w(chunk.size());
+            for (size_t b = 0; b < chunk.size(); ++b) w[b] = rs[chunk[b]].d_weights;
+            if ((e = cb.get(&wm, (size_t)n))) return e;
+            if ((e = build_mask_words(p, w.data(), (int)chunk.size(), wm, 0))) return e;
+        }
+        WithinCols wc(ncols);
+        int c = 0;
+        for (size_t b = 0; b < chunk.size(); ++b) {
+            ReqState& s = rs[chunk[b]];
+            for (int j = 0; j < s.C; ++j, ++c) {
+                const double* x = j == 0 ? s.d_y : (j <= s.K ? s.d_W + (int64_t)(j - 1) * s.ld
+                                                             : s.d_Z + (int64_t)(j - 1 - s.K) * s.ld);
+                wc.x[c] = x;
+                wc.out[c] = s.work + (int64_t)j * n;
+                wc.slot[c] = wm ? (int)b : 0;
+            }
+        }
+        fehip_demean_opts dopt = rs[i0].opts->demean;
+        dopt.fe_mask = rs[i0].fe_mask;
+        WithinStats ws(ncols);
+        if ((e = within_transform(p, wc, ncols, wm, wm ? (int)chunk.size() : 1, dopt, 1, ws, 0))) return e;
+        const auto t1 = clk::now();
+        c = 0;
+        for (int i : chunk) {
+            ReqState& s = rs[i];
+            s.result->converged = 1;
+            for (int j = 0; j < s.C; ++j, ++c) {
+                s.result->fe_iterations = std::max(s.result->fe_iterations, ws.col_iters[c]);
+                s.result->converged = s.result->converged && ws.col_converged[c];
+                s.result->max_update = std::max(s.result->max_update, ws.col_max_update[c]);
+            }
+            s.t1 = t1;
+            s.result->t_demean = std::chrono::duration<double>(t1 - s.t0).count();
+            done[i] = 1;
+        }
+        for (int i : chunk) {
+            if ((e = finish_request(p, rs[i]))) return e;
+            rs[i].db.clear();  // release the request's device buffers before the next chunk
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+}  // namespace fehip
+
+using namespace fehip;
+
+extern "C" int fehip_estimate(fehip_panel* panel, const double* d_y, const double* d_W, int K, const double* d_Z,
+                              int L, int64_t ld, const uint8_t* d_weights, const fehip_opts* opts,
+                              fehip_result* result, double* beta, double* se, double* vcov) {
+    (void)hipGetLastError();  // start from a clean error state
+    if (!panel) return set_error("fehip_estimate: null argument");
+    std::vector<ReqState> rs(1);
+    ReqState& s = rs[0];
+    s.d_y = d_y;
+    s.d_W = d_W;
+    s.K = K;
+    s.d_Z = d_Z;
+    s.L = L;
+    s.ld = ld;
+    s.d_weights = d_weights;
+    s.opts = opts;
+    s.result = result;
+    s.beta = beta;
+    s.se = se;
+    s.vcov = vcov;
+    const int e = run_batch(reinterpret_cast<Panel*>(panel), rs);
+    return e ? e : set_ok();
+}
+
+namespace fehip {
+int estimate_requests(Panel* p, const fehip_request* reqs, int n_req) {
+    std::vector<ReqState> rs(n_req);
+    for (int i = 0; i < n_req; ++i) {
+        const fehip_request& q = reqs[i];
+        ReqState& s = rs[i];
+        s.d_y = q.d_y;
+        s.d_W = q.d_W;
+        s.K = q.K;
+        s.d_Z = q.d_Z;
+        s.L = q.L;
+        s.ld = q.ld;
+        s.d_weights = q.d_weights;
+        s.opts = &q.opts;
+        s.result = q.result;
+        s.beta = q.beta;
+        s.se = q.se;
+        s.vcov = q.vcov;
+    }
+    return run_batch(p, rs);
+}
+}  // namespace fehip
+
+extern "C" int fehip_estimate_batch(fehip_panel* panel, const fehip_request* reqs, int n_req) {
+    (void)hipGetLastError();
+    if (!panel || (n_req > 0 && !reqs) || n_req < 0) return set_error("fehip_estimate_batch: bad arguments");
+    const int e = estimate_requests(reinterpret_cast<Panel*>(panel), reqs, n_req);
+    return e ? e : set_ok();
 }

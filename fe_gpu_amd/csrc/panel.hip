@@ -196,6 +196,8 @@ void free_panel(Panel* p) {
     dfree(p->seg_off);
     dfree(p->tile_seg);
     dfree(p->long_segs);
+    dfree(p->wtile_seg);
+    dfree(p->mtile_seg);
     for (int d = 0; d < p->m; ++d) {
         dfree(p->d[d].ids);
         dfree(p->d[d].counts);
@@ -295,7 +297,39 @@ int build_tiles(Panel* p) {
     }
     p->n_tiles = (int64_t)tiles.size() / 2;
     p->n_long = (int64_t)longs.size();
+    // operator partition: runs of consecutive short segments (<= kWaveRows rows, empty ones included)
+    // packed greedily into wave tiles; runs of consecutive medium segments into block tiles
+    std::vector<int64_t> wt, mt;
+    s = 0;
+    while (s < GP) {
+        const int64_t len = off[s + 1] - off[s];
+        if (len > kTileRows) {
+            ++s;
+            continue;
+        }
+        const bool short_seg = len <= kWaveRows;
+        const int cap = short_seg ? kWaveRows : kTileRows;
+        int64_t e = s + 1;
+        while (e < GP) {
+            const int64_t le = off[e + 1] - off[e];
+            if ((le <= kWaveRows) != short_seg || le > kTileRows) break;
+            if (off[e + 1] - off[s] > cap || (!short_seg && e - s >= kTileRows)) break;
+            ++e;
+        }
+        auto& v = short_seg ? wt : mt;
+        v.push_back(s);
+        v.push_back(e);
+        s = e;
+    }
+    p->n_wtiles = (int64_t)wt.size() / 2;
+    p->n_mtiles = (int64_t)mt.size() / 2;
     int st;
+    if ((st = dalloc(&p->wtile_seg, wt.size() + 2, &p->bytes)) || (st = dalloc(&p->mtile_seg, mt.size() + 2, &p->bytes)))
+        return st;
+    if (!wt.empty())
+        FEHIP_CHECK(hipMemcpy(p->wtile_seg, wt.data(), wt.size() * sizeof(int64_t), hipMemcpyHostToDevice), "wave tiles");
+    if (!mt.empty())
+        FEHIP_CHECK(hipMemcpy(p->mtile_seg, mt.data(), mt.size() * sizeof(int64_t), hipMemcpyHostToDevice), "block tiles");
     if ((st = dalloc(&p->tile_seg, tiles.size() + 2, &p->bytes)) || (st = dalloc(&p->long_segs, longs.size() + 1, &p->bytes)))
         return st;
     if (!tiles.empty())
@@ -402,6 +436,7 @@ int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehi
     Panel* p = new Panel();
     p->n = n_obs;
     p->m = n_fe;
+    (void)hipGetDevice(&p->device);
     int st;
     int* mm = nullptr;
     if ((st = dalloc(&mm, 2))) {

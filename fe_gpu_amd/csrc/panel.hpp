@@ -15,6 +15,8 @@ constexpr int kSmallMaxGroups = 1024;
 // Rows per tile: a tile is a run of whole primary segments holding at most kTileRows rows,
 // processed by one 256-thread workgroup with one row per thread.
 constexpr int kTileRows = 256;
+// Rows per wave tile: one row per lane of a 64-wide wavefront.
+constexpr int kWaveRows = 64;
 
 struct DimStruct {
     int G = 0;                 // groups (max id)
@@ -37,6 +39,7 @@ struct Partition {
 };
 
 struct Panel {
+    int device = 0;            // HIP device holding the panel
     int64_t n = 0;
     int m = 0;
     int P = 0;                 // primary dim
@@ -48,6 +51,13 @@ struct Panel {
     int64_t* tile_seg = nullptr;   // [2 * n_tiles]: first and one-past-last segment of each tile
     int64_t n_long = 0;
     int64_t* long_segs = nullptr;  // segments longer than kTileRows (handled one thread per segment)
+    // the operator kernels' partition of the non-long segments: wave tiles (whole segments of at most
+    // kWaveRows rows each, packed into at most kWaveRows rows, one wavefront per tile) and block tiles
+    // (whole segments longer than kWaveRows, at most kTileRows rows per tile)
+    int64_t n_wtiles = 0;
+    int64_t* wtile_seg = nullptr;  // [2 * n_wtiles]
+    int64_t n_mtiles = 0;
+    int64_t* mtile_seg = nullptr;  // [2 * n_mtiles]
     DimStruct d[FEHIP_MAX_FE];
     std::map<unsigned, Partition> partitions;  // key: bit mask of 0-based dims
     int64_t bytes = 0;
@@ -74,6 +84,13 @@ int dalloc(T** p, size_t count, int64_t* tally = nullptr) {
     }
     if (tally) *tally += (int64_t)bytes;
     return 0;
+}
+
+// per-call temporaries from the caching allocator
+template <typename T>
+int talloc(T** p, size_t count) {
+    *p = (T*)cache_alloc(count * sizeof(T));
+    return *p ? 0 : set_error("device allocation of %zu bytes failed", count * sizeof(T));
 }
 
 template <typename T>

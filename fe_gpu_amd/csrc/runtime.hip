@@ -3,6 +3,9 @@
 // (src/gpu/fe_gpu_runtime.cu:24-70 error state, :449-571 entry points).
 #include <cstdarg>
 #include <cstring>
+#include <map>
+#include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "../../include/fe_gpu_runtime.h"
@@ -103,6 +106,101 @@ void release_workspaces() {
     }
 }
 
+namespace {
+std::mutex g_cache_mutex;
+std::multimap<std::pair<int, size_t>, void*> g_cache_free;          // (device, bytes) -> block
+std::unordered_map<void*, std::pair<int, size_t>> g_cache_live;      // block -> (device, bytes)
+
+size_t cache_round(size_t b) {
+    if (b < 256) b = 256;
+    if (b < (1u << 20)) return (b + 511) & ~(size_t)511;
+    const size_t g = (size_t)2 << 20;
+    return (b + g - 1) & ~(g - 1);
+}
+
+void cache_drop_device(int dev) {  // caller holds g_cache_mutex
+    for (auto it = g_cache_free.begin(); it != g_cache_free.end();) {
+        if (it->first.first == dev) {
+            (void)hipFree(it->second);
+            it = g_cache_free.erase(it);
+        } else {
+            ++it;
+        }
+    }
+}
+}  // namespace
+
+void* cache_alloc(size_t bytes) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    const size_t sz = cache_round(bytes);
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    auto it = g_cache_free.lower_bound({dev, sz});
+    if (it != g_cache_free.end() && it->first.first == dev && it->first.second <= sz + sz / 4 + ((size_t)4 << 20)) {
+        void* p = it->second;
+        g_cache_live[p] = it->first;
+        g_cache_free.erase(it);
+        return p;
+    }
+    void* p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        cache_drop_device(dev);  // give the cached blocks back and try once more
+        if (hipMalloc(&p, sz) != hipSuccess) return nullptr;
+    }
+    g_cache_live[p] = {dev, sz};
+    return p;
+}
+
+void cache_free(void* p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    auto it = g_cache_live.find(p);
+    if (it == g_cache_live.end()) return;
+    g_cache_free.emplace(it->second, p);
+    g_cache_live.erase(it);
+}
+
+void cache_release() {
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    int cur = 0;
+    if (hipGetDevice(&cur) != hipSuccess) cur = 0;
+    for (auto& kv : g_cache_free) {
+        (void)hipSetDevice(kv.first.first);
+        (void)hipFree(kv.second);
+    }
+    g_cache_free.clear();
+    (void)hipSetDevice(cur);
+}
+
+size_t cache_idle_bytes() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::lock_guard<std::mutex> lock(g_cache_mutex);
+    size_t b = 0;
+    for (auto& kv : g_cache_free)
+        if (kv.first.first == dev) b += kv.first.second;
+    return b;
+}
+
+void* pinned_scratch(size_t bytes) {
+    struct Pinned {
+        void* p = nullptr;
+        size_t bytes = 0;
+        ~Pinned() {
+            if (p) (void)hipHostFree(p);
+        }
+    };
+    thread_local Pinned buf;
+    if (buf.bytes >= bytes) return buf.p;
+    if (buf.p) (void)hipHostFree(buf.p);
+    buf.p = nullptr;
+    buf.bytes = 0;
+    if (hipHostMalloc(&buf.p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;
+    buf.bytes = bytes;
+    return buf.p;
+}
+
 double* host_scalars() {
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     if (!g_host_scalars) {
@@ -151,6 +249,7 @@ int fe_gpu_runtime_init(int device_id, fe_gpu_device_info* info) {
 
 int fe_gpu_runtime_shutdown(void) {
     release_workspaces();
+    cache_release();
     return set_hip_error(hipDeviceReset(), "hipDeviceReset");
 }
 

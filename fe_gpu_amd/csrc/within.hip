@@ -69,8 +69,11 @@ struct ApplyArgs {
     int ncols;                 // valid columns in this tile
     const int* perm;
     double* small_part;        // [blocks][Gs][CT]
-    const int64_t* tile_seg;   // [2 * n_tiles] segment range of each tile
+    const int64_t* tile_seg;   // [2 * n_tiles] segment range of each block tile
     int64_t n_tiles;
+    const int* idP;            // primary ids (panel order): segment heads of the wave tiles
+    const int64_t* wtile_seg;  // [2 * n_wtiles] segment range of each wave tile
+    int64_t n_wtiles;
     const int64_t* long_segs;  // segments longer than a tile
     int64_t n_long;
     int part_base;             // first partial block written by this launch
@@ -149,6 +152,172 @@ __device__ __forceinline__ int tile_runs(bool valid, bool rhead, int j, int nr, 
     __syncthreads();
     (void)valid;
     return total;
+}
+
+// ------------------------------------------------------------------------------------------------
+// Wave tiles: one 64-wide wavefront per tile of whole short primary segments (<= 64 rows in all),
+// one row per lane.  Segment sums and run sums are segmented inclusive scans across the lanes in
+// registers (DPP row shifts and row broadcasts, a fixed-shape tree), the segment total reaches
+// every lane of its segment by one lane permute, and the weighted row counts come from ballots.
+// Nothing is staged in LDS except the small-dim tables; the waves of a block run independently.
+
+// x moved across lanes by a DPP control (lanes without a source read 0)
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Lane predicates of a segmented inclusive scan whose segment of this lane starts at lane s.
+struct ScanMask {
+    bool m1, m2, m4, m8, b15, b31;
+    __device__ __forceinline__ ScanMask(int lane, int s) {
+        const int rl = lane & 15;
+        m1 = rl >= 1 && lane - 1 >= s;
+        m2 = rl >= 2 && lane - 2 >= s;
+        m4 = rl >= 4 && lane - 4 >= s;
+        m8 = rl >= 8 && lane - 8 >= s;
+        b15 = (lane & 31) >= 16 && s < (lane & ~15);
+        b31 = lane >= 32 && s < 32;
+    }
+};
+
+// sum of x over lanes [s, lane] of the wave (every lane of the wave must take part)
+__device__ __forceinline__ double seg_scan(double x, const ScanMask& m) {
+    double t;
+    t = dpp_move<0x111>(x);  // row_shr:1
+    x += m.m1 ? t : 0.0;
+    t = dpp_move<0x112>(x);  // row_shr:2
+    x += m.m2 ? t : 0.0;
+    t = dpp_move<0x114>(x);  // row_shr:4
+    x += m.m4 ? t : 0.0;
+    t = dpp_move<0x118>(x);  // row_shr:8
+    x += m.m8 ? t : 0.0;
+    t = dpp_move<0x142>(x);  // row_bcast:15
+    x += m.b15 ? t : 0.0;
+    t = dpp_move<0x143>(x);  // row_bcast:31
+    x += m.b31 ? t : 0.0;
+    return x;
+}
+
+__device__ __forceinline__ unsigned long long mask_le(int lane) { return (2ull << lane) - 1ull; }
+
+template <int CT, int MODE>
+__global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
+    if constexpr (MODE == kIter) {
+        if (!any_active<CT>(a.active)) return;
+    }
+    // slot bit of each column; columns whose CG has stopped are skipped (their table entries are
+    // not read again)
+    uint32_t cb[CT];
+    uint32_t anyb = 0;
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        cb[c] = a.cbit[c];
+        if constexpr (MODE == kIter)
+            if (a.active && !a.active[c]) cb[c] = 0u;
+        anyb |= cb[c];
+    }
+    extern __shared__ double dyn[];
+    double* sacc = dyn;                     // [Gs][CT]
+    double* sp = dyn + (size_t)a.Gs * CT;   // [Gs][CT]
+    stage_tables<CT, MODE>(a, sacc, sp);
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; t < a.n_wtiles; t += wstride) {
+        const int64_t s0 = a.wtile_seg[2 * t], s1 = a.wtile_seg[2 * t + 1];
+        const int64_t R0 = a.seg_off[s0];
+        const int nr = (int)(a.seg_off[s1] - R0);
+        if (nr <= 0) continue;  // a tile of empty segments (uniform across the wave)
+        const bool valid = lane < nr;
+        const int64_t r = R0 + (valid ? lane : 0);
+        const uint32_t word = valid ? row_word(a, r) : 0u;
+        const bool on = (word & anyb) != 0;
+        // segment of this lane: [ss, se]
+        const int ip = valid ? a.idP[r] : -1;
+        const int ipp = __shfl_up(ip, 1, kWave);
+        const bool hs = valid && (lane == 0 || ip != ipp);
+        const unsigned long long H = __ballot(hs);
+        const int ss = 63 - __clzll(H & mask_le(lane));
+        const unsigned long long nh = H & ~mask_le(lane);
+        const int se = nh ? __ffsll((long long)nh) - 2 : nr - 1;
+        double v[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) v[c] = 0.0;
+        if (on) row_value<CT, MODE>(a, r, sp, v);
+#pragma unroll
+        for (int c = 0; c < CT; ++c) v[c] = (word & cb[c]) ? v[c] : 0.0;
+        if (a.elim) {
+            const ScanMask sm(lane, ss);
+            const unsigned long long segmask = mask_le(se) & ~((1ull << ss) - 1ull);
+            double inv = 0.0;
+            uint32_t cnt_bit = 0u;
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                if (cb[c] == 0u) continue;  // uniform across the wave
+                if (cb[c] != cnt_bit) {     // columns of one slot share the row count
+                    const unsigned long long B = __ballot((word & cb[c]) != 0);
+                    const int cnt = __popcll(B & segmask);
+                    inv = cnt > 0 ? 1.0 / (double)cnt : 0.0;
+                    cnt_bit = cb[c];
+                }
+                const double x = seg_scan(v[c], sm);
+                const double tot = __shfl(x, se, kWave);
+                v[c] = (word & cb[c]) ? v[c] - tot * inv : 0.0;
+            }
+        }
+        if constexpr (MODE == kFinal) {
+            if (valid) {
+                const int64_t src = a.perm ? a.perm[r] : r;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) {
+                    if (c >= a.ncols) break;
+                    if (a.to_work)
+                        a.oc[c][r] = v[c];
+                    else if (word & cb[c])
+                        a.oc[c][src] = v[c];
+                }
+            }
+        } else {
+            for (int o = 0; o < a.nd; ++o) {
+                const ScatterDim& d = a.sd[o];
+                const int g = valid ? d.ids[r] : -1;
+                if (d.small) {
+                    if (on) {
+                        double* tq = sacc + (size_t)(d.lds_off + g) * CT;
+#pragma unroll
+                        for (int c = 0; c < CT; ++c)
+                            if (word & cb[c]) lds_add(&tq[c], v[c]);
+                    }
+                    continue;
+                }
+                // runs: a new run starts at a segment head or where the id changes
+                const int gp = __shfl_up(g, 1, kWave);
+                const bool rh = valid && (hs || g != gp);
+                const unsigned long long HR = __ballot(rh);
+                const unsigned long long le = HR & mask_le(lane);
+                const int rs = 63 - __clzll(le);
+                const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
+                const int64_t slot = d.seg_run_off[s0] + (__popcll(le) - 1);
+                const ScanMask rm(lane, rs);
+                double* dst = d.runbuf + slot * CT;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) {
+                    if (cb[c] == 0u) continue;
+                    const double x = seg_scan(v[c], rm);
+                    if (rend) dst[c] = x;
+                }
+            }
+        }
+    }
+    if constexpr (MODE != kFinal) {
+        __syncthreads();
+        double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
+        for (int e = threadIdx.x; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -787,15 +956,31 @@ size_t tables_lds(const Layout& L) {
     return sizeof(double) * (size_t)L.Gs * CT * 2;
 }
 
-unsigned tile_grid(Panel* p) {
+unsigned capped_grid(int64_t work) {
     const int64_t cap = (int64_t)num_cus() * 8;
-    const int64_t g = p->n_tiles < cap ? p->n_tiles : cap;
-    return (unsigned)(g < 1 ? 1 : g);
+    return (unsigned)(work <= 0 ? 0 : (work < cap ? work : cap));
 }
+
+// all non-long segments in block tiles (Schur diagonal)
+unsigned tile_grid(Panel* p) {
+    const unsigned g = capped_grid(p->n_tiles);
+    return g < 1 ? 1 : g;
+}
+
+// operator kernels: wave tiles (4 per block) and block tiles of medium segments
+unsigned wave_grid(Panel* p) { return capped_grid((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave)); }
+unsigned mtile_grid(Panel* p) { return capped_grid(p->n_mtiles); }
 
 unsigned long_grid(Panel* p) { return p->n_long > 0 ? stream_grid((size_t)p->n_long, kBlock, 2) : 0; }
 
-unsigned apply_grid(Panel* p) { return tile_grid(p) + long_grid(p); }
+unsigned apply_grid(Panel* p) {
+    return std::max(tile_grid(p) + long_grid(p), wave_grid(p) + mtile_grid(p) + long_grid(p));
+}
+
+// One pass of the operator kernels (kIter, kData or kFinal) over every row: wave tiles, block tiles,
+// long segments; returns the number of per-block partial tables written.
+template <int CT, int MODE>
+unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s);
 
 // table-space results of one apply launch: small dims from the per-block partials, large dims from
 // the run partials (only_dim >= 0 restricts to one scatter dim)
@@ -823,35 +1008,41 @@ void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, 
 }
 
 // One operator product (kIter) or right-hand side (kData) into q (table space [S][CT]).
+template <int CT, int MODE>
+unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
+    const unsigned gw = wave_grid(p), gm = mtile_grid(p), gl = long_grid(p);
+    const size_t lt = tables_lds<CT>(L);
+    a.part_base = 0;
+    if (gw > 0) hipLaunchKernelGGL((wave_tile_kernel<CT, MODE>), dim3(gw), dim3(kBlock), lt, s, a);
+    if (gm > 0) {
+        ApplyArgs b = a;
+        b.tile_seg = p->mtile_seg;
+        b.n_tiles = p->n_mtiles;
+        b.part_base = (int)gw;
+        hipLaunchKernelGGL((tile_fast_kernel<CT, MODE>), dim3(gm), dim3(kBlock), lt, s, b);
+    }
+    if (gl > 0) {
+        ApplyArgs b = a;
+        b.part_base = (int)(gw + gm);
+        hipLaunchKernelGGL((long_kernel<CT, MODE>), dim3(gl), dim3(kBlock), lt, s, b);
+    }
+    return gw + gm + gl;
+}
+
+// One operator product (kIter) or right-hand side (kData) into q (table space [S][CT]).
 template <int CT>
 int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* q, hipStream_t s) {
-    const unsigned gt = tile_grid(p), gl = long_grid(p);
-    const unsigned grid = gt + gl;
     const int sid = mode == kIter ? kStatApplyIter : kStatApplyData;
     const bool timed = kstats_enabled();
     if (timed) kstat_begin(sid, s);
-    a.part_base = 0;
-    const size_t lt = tables_lds<CT>(L);
-    if (mode == kIter)
-        hipLaunchKernelGGL((tile_fast_kernel<CT, kIter>), dim3(gt), dim3(kBlock), lt, s, a);
-    else
-        hipLaunchKernelGGL((tile_fast_kernel<CT, kData>), dim3(gt), dim3(kBlock), lt, s, a);
-    FEHIP_LAUNCH_CHECK("tile kernel");
-    if (gl > 0) {
-        ApplyArgs b = a;
-        b.part_base = (int)gt;
-        if (mode == kIter)
-            hipLaunchKernelGGL((long_kernel<CT, kIter>), dim3(gl), dim3(kBlock), lt, s, b);
-        else
-            hipLaunchKernelGGL((long_kernel<CT, kData>), dim3(gl), dim3(kBlock), lt, s, b);
-        FEHIP_LAUNCH_CHECK("long-segment kernel");
-    }
+    const unsigned grid = mode == kIter ? launch_pass<CT, kIter>(p, L, a, s) : launch_pass<CT, kData>(p, L, a, s);
+    FEHIP_LAUNCH_CHECK("operator kernels");
     if (timed) {
-        // algorithmic bytes: id streams, segment offsets, row masks, table reads, run-partial and
-        // LDS-partial writes (+ the data columns and permutation for the right-hand side)
+        // algorithmic bytes: id streams (the scattered dims and the primary, which marks the segments),
+        // row masks, table reads, run-partial and per-block partial writes (+ the data columns and the
+        // permutation for the right-hand side)
         const double n = (double)p->n;
-        double bytes = n * 4.0 * L.nd + 8.0 * (double)(p->d[p->P].G + 1) + (a.wm ? 4.0 * n : 0.0) +
-                       8.0 * (double)grid * L.Gs * CT;
+        double bytes = n * 4.0 * (L.nd + 1) + (a.wm ? 4.0 * n : 0.0) + 8.0 * (double)grid * L.Gs * CT;
         for (int o = 0; o < L.nd; ++o) {
             bytes += 8.0 * CT * (double)L.sd[o].G;
             if (!L.sd[o].small) bytes += 8.0 * CT * (double)p->d[L.dims[o]].n_runs;
@@ -872,14 +1063,13 @@ struct Scratch {
     double* small_part = nullptr;
     double* part = nullptr;     // reduction partials
     CgState* st = nullptr;
-    CgState* st_host = nullptr;
+    CgState* st_host = nullptr;  // per-thread pinned scratch
     ~Scratch() {
-        for (void* v : owned) (void)hipFree(v);
-        if (st_host) (void)hipHostFree(st_host);
+        for (void* v : owned) cache_free(v);
     }
     template <typename T>
     int alloc(T** ptr, size_t count) {
-        int e = dalloc(ptr, count);
+        int e = talloc(ptr, count);
         if (!e) owned.push_back((void*)*ptr);
         return e;
     }
@@ -903,6 +1093,9 @@ ApplyArgs base_args(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm) 
     a.small_part = sc.small_part;
     a.tile_seg = p->tile_seg;
     a.n_tiles = p->n_tiles;
+    a.idP = p->d[p->P].ids;
+    a.wtile_seg = p->wtile_seg;
+    a.n_wtiles = p->n_wtiles;
     a.long_segs = p->long_segs;
     a.n_long = p->n_long;
     return a;
@@ -1008,20 +1201,17 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             ts.max_update = std::max(ts.max_update, sc.st_host->maxn[c]);
             ts.col_iters[c] = sc.st_host->iters[c];
             ts.col_converged[c] = !sc.st_host->active[c];
+            ts.col_max_update[c] = sc.st_host->maxn[c];
         }
         ts.apply_launches = it;
     }
     // final pass over the columns: M_P (x - D_o a)
     a.alpha = xa;
     if (kstats_enabled()) kstat_begin(kStatFinalize, s);
-    hipLaunchKernelGGL((tile_fast_kernel<CT, kFinal>), dim3(tile_grid(p)), dim3(kBlock), tables_lds<CT>(L), s, a);
-    FEHIP_LAUNCH_CHECK("finalize kernel");
-    if (p->n_long > 0) {
-        hipLaunchKernelGGL((long_kernel<CT, kFinal>), dim3(long_grid(p)), dim3(kBlock), tables_lds<CT>(L), s, a);
-        FEHIP_LAUNCH_CHECK("long finalize kernel");
-    }
+    launch_pass<CT, kFinal>(p, L, a, s);
+    FEHIP_LAUNCH_CHECK("finalize kernels");
     if (kstats_enabled()) {
-        double bytes = 16.0 * (double)p->n * ncols + 4.0 * (double)p->n * L.nd + 8.0 * (double)(a.GP + 1) +
+        double bytes = 16.0 * (double)p->n * ncols + 4.0 * (double)p->n * (L.nd + 1) +
                        (a.wm ? 4.0 * (double)p->n : 0.0) + (p->perm ? 4.0 * (double)p->n : 0.0);
         for (int od = 0; od < L.nd; ++od) bytes += 8.0 * CT * (double)L.sd[od].G;
         kstat_end(kStatFinalize, s, bytes);
@@ -1082,7 +1272,8 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
     if ((e = sc.alloc(&sc.small_part, (size_t)grid * (L.Gs > 0 ? L.Gs : 1) * kMaxCT))) return e;
     if ((e = sc.alloc(&sc.part, (size_t)kCgBlocks * kMaxCT * 2))) return e;
     if ((e = sc.alloc(&sc.st, 1))) return e;
-    FEHIP_CHECK(hipHostMalloc((void**)&sc.st_host, sizeof(CgState), hipHostMallocDefault), "pinned state");
+    sc.st_host = (CgState*)pinned_scratch(sizeof(CgState));
+    if (!sc.st_host) return set_error("pinned host allocation failed");
     // Schur diagonal of every slot in use
     if (S > 0) {
         std::vector<int> used(nslots, 0);
@@ -1121,6 +1312,7 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
         for (int c = 0; c < nc; ++c) {
             ws.col_iters[c0 + c] = ts.col_iters[c];
             ws.col_converged[c0 + c] = ts.col_converged[c];
+            ws.col_max_update[c0 + c] = ts.col_max_update[c];
         }
         const int64_t id_bytes = n * 4 * L.nd + (int64_t)(p->d[p->P].G + 1) * 8 + (wm ? 4 * n : 0);
         ws.bytes += ts.apply_launches * id_bytes + (int64_t)3 * n * 8 * nc;
@@ -1146,17 +1338,17 @@ extern "C" int fehip_demean(fehip_panel* panel, double* d_cols, int n_cols, int6
     }
     uint32_t* wm = nullptr;
     if (d_weights) {
-        int e = dalloc(&wm, (size_t)p->n);
+        int e = talloc(&wm, (size_t)p->n);
         if (e) return e;
         e = build_mask_words(p, &d_weights, 1, wm, 0);
         if (e) {
-            dfree(wm);
+            cache_free(wm);
             return e;
         }
     }
     WithinStats ws(n_cols);
     int e = within_transform(p, wc, n_cols, wm, 1, *opts, 0, ws, 0);
-    dfree(wm);
+    cache_free(wm);
     if (e) return e;
     if (stats) {
         stats->iterations = ws.iterations;

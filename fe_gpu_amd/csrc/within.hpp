@@ -15,10 +15,11 @@ struct TileStats {
     int apply_launches = 0;
     int col_iters[16] = {0};
     int col_converged[16] = {0};
+    double col_max_update[16] = {0};
 };
 
 struct WithinStats {
-    explicit WithinStats(int ncols = 0) : col_iters(ncols, 0), col_converged(ncols, 1) {}
+    explicit WithinStats(int ncols = 0) : col_iters(ncols, 0), col_converged(ncols, 1), col_max_update(ncols, 0.0) {}
     int iterations = 0;
     int converged = 1;
     double max_update = 0.0;
@@ -27,6 +28,7 @@ struct WithinStats {
     int64_t bytes = 0;
     std::vector<int> col_iters;      // per column: CG iterations
     std::vector<int> col_converged;  // per column: criterion met
+    std::vector<double> col_max_update;  // per column: last max |r_g / n_g|
 };
 
 // Columns of one solve: input and output device pointers and the request slot of each column.

@@ -105,6 +105,39 @@ def worker_firm_panel(seed=202, n_workers=20000, n_periods=30, n_firms=4000, swi
     return Panel(y, X, fe, names, ["worker", "firm", "time"], beta)
 
 
+def serving_panel(seed=505, n_workers=500_000, n_periods=20, n_firms=100_000, n_regions=50, n_cand=20,
+                  switch_p=0.10):
+    """Config-5 serving panel: workers x periods, sticky firms, 20 periods, a 50-level region
+    assigned per firm (nested in firm), y plus n_cand candidate columns.  FE dims: worker, firm,
+    time, region.  Candidate j = 0.3 alpha_w + 0.2 alpha_f + sqrt(0.87) e_j; y loads on the first
+    six candidates."""
+    rng = np.random.default_rng(seed)
+    W_, T_ = n_workers, n_periods
+    sw = rng.random((W_, T_)) < switch_p
+    sw[:, 0] = True
+    draws = rng.integers(1, n_firms + 1, (W_, T_), dtype=np.int32)
+    idx = np.maximum.accumulate(np.where(sw, np.arange(T_)[None, :], 0), axis=1)
+    firm = np.take_along_axis(draws, idx, axis=1).reshape(-1)
+    n = W_ * T_
+    worker = np.repeat(np.arange(1, W_ + 1, dtype=np.int32), T_)
+    period = np.tile(np.arange(1, T_ + 1, dtype=np.int32), W_)
+    region_of_firm = rng.integers(1, n_regions + 1, n_firms, dtype=np.int32)
+    region_of_firm[rng.integers(0, n_firms)] = n_regions
+    region = region_of_firm[firm - 1]
+    a_w = rng.normal(0, 1.0, W_)[worker - 1]
+    a_f = rng.normal(0, 0.5, n_firms)[firm - 1]
+    a_t = rng.normal(0, 0.1, T_)[period - 1]
+    a_r = rng.normal(0, 0.2, n_regions)[region - 1]
+    X = np.empty((n, n_cand), order="F")
+    for j in range(n_cand):
+        X[:, j] = 0.3 * a_w + 0.2 * a_f + np.sqrt(0.87) * rng.normal(0, 1, n)
+    beta = np.zeros(n_cand)
+    beta[:6] = [0.5, -0.25, 0.1, 0.8, -0.6, 0.3]
+    y = X[:, :6] @ beta[:6] + a_w + a_f + a_t + a_r + rng.normal(0, 1.0, n)
+    fe = np.vstack([worker, firm, period, region]).astype(np.int32)
+    return Panel(y, X, fe, [f"cand_{j}" for j in range(1, n_cand + 1)], ["worker", "firm", "time", "region"], beta)
+
+
 def random_fe_problem(seed, n=200, K=3, L=0, G=(7, 5), noise=0.1):
     """Small random problem with arbitrary (unsorted) FE ids for parity tests."""
     rng = np.random.default_rng(seed)

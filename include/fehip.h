@@ -102,6 +102,62 @@ int fehip_estimate(fehip_panel* panel, const double* d_y, const double* d_W, int
                    int64_t ld, const uint8_t* d_weights, const fehip_opts* opts, fehip_result* result, double* beta,
                    double* se, double* vcov);
 
+/* One specification of a batch: the arguments of fehip_estimate in one struct. */
+typedef struct {
+    const double* d_y;
+    const double* d_W;
+    const double* d_Z;
+    int32_t K, L;
+    int64_t ld;
+    const uint8_t* d_weights;  /* optional per-request row subsample (uint8, caller order) */
+    fehip_opts opts;
+    fehip_result* result;      /* host outputs */
+    double* beta;
+    double* se;
+    double* vcov;              /* may be NULL */
+} fehip_request;
+
+/* Estimate n_req specifications on one panel.  Requests that absorb the same FE dims are solved
+ * together: their columns share every within-transform sweep (up to 32 requests per sweep, each
+ * with its own row subsample, Schur preconditioner and CG scalars), then each request takes its
+ * own normal-equation and sandwich step.  Each request's outputs match fehip_estimate on that
+ * request alone (same stopping rule per column; summation order inside the sweeps may differ in
+ * the last bits).  Batching is the multi-specification path that has no counterpart in the
+ * reference, which estimates one specification per process run (src/src_main.f90). */
+int fehip_estimate_batch(fehip_panel* panel, const fehip_request* reqs, int n_req);
+
+/* Request server over a device-resident panel (no counterpart in the reference, which runs one
+ * specification per process).  A worker thread on the panel's device takes queued requests in
+ * arrival order and solves as many as fit max_batch and the HBM budget as one batch
+ * (fehip_estimate_batch semantics); requests that arrive meanwhile join the next batch.  The
+ * request's device inputs and host outputs must stay valid until fehip_server_wait returns for
+ * its ticket; the option arrays (iv_reg, iv_inst) are copied at submit. */
+typedef struct fehip_server fehip_server;
+
+typedef struct {
+    int64_t hbm_budget;  /* device bytes the requests of one batch may hold; 0 -> 80% of free memory */
+    int32_t max_batch;   /* requests per batch; 0 -> 32 */
+    int32_t reserved;
+} fehip_server_opts;
+
+typedef struct {
+    int64_t submitted, completed, failed, batches, queued;
+    int64_t budget_bytes;
+    double mean_batch;   /* requests per batch */
+    double busy_s;       /* worker time spent solving */
+    double span_s;       /* first batch start to last batch end */
+    double p50_s, p99_s, max_s;  /* submit-to-completion latency of completed requests */
+} fehip_server_stats;
+
+int fehip_server_create(fehip_panel* panel, const fehip_server_opts* opts, fehip_server** out);
+int fehip_server_submit(fehip_server* server, const fehip_request* req, int64_t* ticket);
+/* Blocks until the ticket's request is done; returns its status (message via get_last_error). */
+int fehip_server_wait(fehip_server* server, int64_t ticket, double* latency_s);
+int fehip_server_get_stats(fehip_server* server, fehip_server_stats* stats);
+int fehip_server_reset_stats(fehip_server* server);
+/* Finishes every queued request, then stops the worker. */
+int fehip_server_destroy(fehip_server* server);
+
 /* Deterministic synthetic worker-firm-time panel generated on the device (the 150M-row benchmark
  * panel does not fit a host round trip).  Rows are ordered by worker then period.  Writes
  * d_fe (3 x n int32, dim-major: worker, firm, period), d_y (n) and d_W (n x 8, col-major). */

@@ -53,6 +53,35 @@ def test_ctypes_struct_layouts_match_headers():
     assert C.sizeof(S.Opts) == 32 + 4 + 16 + 4 + 8 + 4 + 4 + 8
 
 
+def test_ctypes_structs_match_the_c_compiler(tmp_path):
+    """sizeof and every field offset of the fehip.h structs, as gcc lays them out."""
+    import shutil
+    import subprocess
+    from fe_gpu_amd import _coarse_sigs as S
+    if shutil.which("gcc") is None:
+        pytest.skip("gcc not available")
+    structs = {"fehip_panel_info": S.PanelInfo, "fehip_demean_opts": S.DemeanOpts,
+               "fehip_demean_stats": S.DemeanStats, "fehip_opts": S.Opts, "fehip_result": S.Result,
+               "fehip_request": S.Request, "fehip_server_opts": S.ServerOpts,
+               "fehip_server_stats": S.ServerStats}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fehip.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split("\n")
+    got = {(a, b): int(c) for a, b, c in (ln.split() for ln in out if ln)}
+    for cname, py in structs.items():
+        assert got[(cname, "sizeof")] == C.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
 def test_product_never_imports_the_oracle():
     pkg = os.path.join(ROOT, "fe_gpu_amd")
     for dirpath, _, files in os.walk(pkg):
