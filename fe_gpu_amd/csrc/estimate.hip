@@ -21,6 +21,7 @@
 #include "dense.hpp"
 #include "panel.hpp"
 #include "estimate.hpp"
+#include "wave.hpp"
 #include "within.hpp"
 
 namespace fehip {
@@ -85,13 +86,14 @@ __global__ __launch_bounds__(kBlock) void dot_part_kernel(const double* __restri
 }
 
 // T[i][j] = r_i X[i][j] (row-major n x K) from column-major X (ld)
-__global__ void scores_rows_kernel(const double* __restrict__ r, const double* __restrict__ X, int64_t ldX, int64_t n,
+template <typename TX>
+__global__ void scores_rows_kernel(const double* __restrict__ r, const TX* __restrict__ X, int64_t ldX, int64_t n,
                                    int K, double* T) {
     const int64_t total = n * K;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int64_t i = e / K;
         const int j = (int)(e - i * K);
-        T[e] = r[i] * X[i + j * ldX];
+        T[e] = r[i] * (double)X[i + j * ldX];
     }
 }
 
@@ -179,8 +181,8 @@ __global__ __launch_bounds__(kBlock) void meat_kernel(const double* __restrict__
 }
 
 // Upper triangle of X'X for n rows of C <= KT column-major columns (ld), block partials.
-template <int KT>
-__global__ __launch_bounds__(kBlock) void gram_rows_kernel(const double* __restrict__ X, int64_t ld, int C, int64_t n,
+template <int KT, typename TX>
+__global__ __launch_bounds__(kBlock) void gram_rows_kernel(const TX* __restrict__ X, int64_t ld, int C, int64_t n,
                                                            double* part) {
     constexpr int NP = KT * (KT + 1) / 2;
     double acc[NP];
@@ -189,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void gram_rows_kernel(const double* __restr
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         double x[KT];
 #pragma unroll
-        for (int k = 0; k < KT; ++k) x[k] = k < C ? X[i + k * ld] : 0.0;
+        for (int k = 0; k < KT; ++k) x[k] = k < C ? (double)X[i + k * ld] : 0.0;
         int idx = 0;
 #pragma unroll
         for (int a = 0; a < KT; ++a)
@@ -282,6 +284,154 @@ int cluster_sums(const double* T, int K, const int* order, const int* keys, int6
 // Subsets containing the primary dim: every cluster lies inside one primary segment, so a tile of
 // whole segments (<= 256 rows, staged in LDS) holds each of its clusters entirely.  A row that is
 // the first of its cluster in the segment sums the cluster's rows and adds S S' to its registers.
+// Subsets that contain the primary dim P, on the wave tiles (whole primary segments of <= 64 rows,
+// one row per lane): the scores t = r x are formed on the fly from the residual and the regressor
+// columns, summed per run of equal subset ids by a segmented DPP scan, and runs of one segment with
+// the same ids are merged (a cluster inside a segment need not be contiguous).  Each cluster's
+// leader lane accumulates S S' in registers; block partials of the upper triangle.
+struct WaveNest {
+    const double* r;            // residual (panel order)
+    int64_t ldX;                // regressor columns (panel order)
+    int K;
+    const int* idP;
+    const int64_t* seg_off;
+    const int64_t* wtile_seg;
+    int64_t n_wtiles;
+    const int* kid[2];          // the subset's other dims (panel order ids)
+    int nk;
+    const uint8_t* w;           // panel-order weights (nullptr: all rows)
+};
+
+template <int KT, typename TX>
+__global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, const TX* __restrict__ X, double* part,
+                                                                  unsigned long long* ucount) {
+    constexpr int NP = KT * (KT + 1) / 2;
+    double acc[NP];
+#pragma unroll
+    for (int e = 0; e < NP; ++e) acc[e] = 0.0;
+    const int lane = threadIdx.x & (kWave - 1);
+    unsigned long long nclusters = 0;
+    const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; t < a.n_wtiles; t += wstride) {
+        const int64_t s0 = a.wtile_seg[2 * t], s1 = a.wtile_seg[2 * t + 1];
+        const int64_t R0 = a.seg_off[s0];
+        const int nr = (int)(a.seg_off[s1] - R0);
+        if (nr <= 0) continue;
+        const bool valid = lane < nr;
+        const int64_t ri = R0 + (valid ? lane : 0);
+        const int ip = valid ? a.idP[ri] : -1;
+        const int k0 = (valid && a.nk > 0) ? a.kid[0][ri] : -1;
+        const int k1 = (valid && a.nk > 1) ? a.kid[1][ri] : -1;
+        const int ipp = __shfl_up(ip, 1, kWave), k0p = __shfl_up(k0, 1, kWave), k1p = __shfl_up(k1, 1, kWave);
+        const bool hs = valid && (lane == 0 || ip != ipp);
+        const unsigned long long H = __ballot(hs);
+        const int ss = 63 - __clzll(H & mask_le(lane));  // segment start lane
+        const bool rh = valid && (hs || k0 != k0p || k1 != k1p);
+        const unsigned long long HR = __ballot(rh);
+        const unsigned long long le = HR & mask_le(lane);
+        const int rs = 63 - __clzll(le);
+        const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
+        const bool wrow = valid && (!a.w || a.w[ri]);
+        const unsigned long long BW = __ballot(wrow);
+        bool ne = (BW & mask_le(lane) & ~((1ull << rs) - 1ull)) != 0;  // run holds a weighted row
+        const ScanMask rm(lane, rs);
+        const double rr = valid ? a.r[ri] : 0.0;
+        double sv[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+            const double tv = (k < a.K && valid) ? rr * (double)X[ri + (int64_t)k * a.ldX] : 0.0;
+            sv[k] = seg_scan(tv, rm);
+        }
+        bool leader = rend;
+        if (a.nk > 0) {
+            // merge the runs of one segment that carry the same ids
+            unsigned long long E = __ballot(rend);
+            while (E) {
+                const int j = __ffsll((long long)E) - 1;
+                E &= E - 1ull;
+                const int kj0 = __shfl(k0, j, kWave), kj1 = __shfl(k1, j, kWave), sj = __shfl(ss, j, kWave);
+                const bool match = rend && lane != j && sj == ss && kj0 == k0 && kj1 == k1;
+                if (match && j < lane) leader = false;
+                const bool take = match && j > lane;
+                if (__ballot(take)) {
+#pragma unroll
+                    for (int k = 0; k < KT; ++k) {
+                        const double vj = __shfl(sv[k], j, kWave);
+                        if (take) sv[k] += vj;
+                    }
+                    const bool nej = __shfl((int)ne, j, kWave) != 0;
+                    if (take) ne = ne || nej;
+                }
+            }
+        }
+        const bool use = leader && ne;
+        const unsigned long long BU = __ballot(use);
+        if (lane == 0) nclusters += (unsigned long long)__popcll(BU);
+        if (use) {
+            int idx = 0;
+#pragma unroll
+            for (int i = 0; i < KT; ++i)
+#pragma unroll
+                for (int j = i; j < KT; ++j) acc[idx++] += sv[i] * sv[j];
+        }
+    }
+    if (lane == 0 && nclusters) atomicAdd(ucount, nclusters);
+    __shared__ double ws[kBlock / kWave][NP];
+    const int wv = threadIdx.x / kWave;
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        const double v = wave_sum(acc[e]);
+        if (lane == 0) ws[wv][e] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NP; e += blockDim.x) {
+        double v = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) v += ws[w][e];
+        part[(int64_t)blockIdx.x * NP + e] = v;
+    }
+}
+
+// Any subset through its cached sorted partition: one thread per cluster sums the score rows of its
+// members (order[off[c] .. off[c+1])) and accumulates S S' in registers; block partials.
+template <int KT>
+__global__ __launch_bounds__(kBlock) void sorted_meat_kernel(const double* __restrict__ T, int K,
+                                                             const int* __restrict__ order,
+                                                             const int64_t* __restrict__ off, int64_t nS, double* part) {
+    constexpr int NP = KT * (KT + 1) / 2;
+    double acc[NP];
+#pragma unroll
+    for (int e = 0; e < NP; ++e) acc[e] = 0.0;
+    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nS; c += (int64_t)gridDim.x * blockDim.x) {
+        double sv[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) sv[k] = 0.0;
+        for (int64_t q = off[c]; q < off[c + 1]; ++q) {
+            const double* row = T + (order ? (int64_t)order[q] : q) * K;
+#pragma unroll
+            for (int k = 0; k < KT; ++k)
+                if (k < K) sv[k] += row[k];
+        }
+        int idx = 0;
+#pragma unroll
+        for (int i = 0; i < KT; ++i)
+#pragma unroll
+            for (int j = i; j < KT; ++j) acc[idx++] += sv[i] * sv[j];
+    }
+    __shared__ double ws[kBlock / kWave][NP];
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int e = 0; e < NP; ++e) {
+        const double v = wave_sum(acc[e]);
+        if (ln == 0) ws[wv][e] = v;
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < NP; e += blockDim.x) {
+        double v = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) v += ws[w][e];
+        part[(int64_t)blockIdx.x * NP + e] = v;
+    }
+}
+
 struct NestKeys {
     const int* ids[3];  // other dims of the subset (panel order)
     int nk;
@@ -506,17 +656,83 @@ int meat_fused(const double* T, int K, int ldT, const int* order, const int64_t*
 }
 
 // Gram of C <= 12 columns (fused path)
-int gram_fused(const double* X, int64_t ld, int C, int64_t n, double* part, std::vector<double>& G) {
+template <typename TX>
+int gram_fused(const TX* X, int64_t ld, int C, int64_t n, double* part, std::vector<double>& G) {
     const int KT = C <= 4 ? 4 : (C <= 8 ? 8 : 12);
     const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kMeatBlocks, (n + kBlock - 1) / kBlock));
     if (KT == 4)
-        hipLaunchKernelGGL(gram_rows_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, X, ld, C, n, part);
+        hipLaunchKernelGGL((gram_rows_kernel<4, TX>), dim3(grid), dim3(kBlock), 0, 0, X, ld, C, n, part);
     else if (KT == 8)
-        hipLaunchKernelGGL(gram_rows_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, X, ld, C, n, part);
+        hipLaunchKernelGGL((gram_rows_kernel<8, TX>), dim3(grid), dim3(kBlock), 0, 0, X, ld, C, n, part);
     else
-        hipLaunchKernelGGL(gram_rows_kernel<12>, dim3(grid), dim3(kBlock), 0, 0, X, ld, C, n, part);
+        hipLaunchKernelGGL((gram_rows_kernel<12, TX>), dim3(grid), dim3(kBlock), 0, 0, X, ld, C, n, part);
     FEHIP_LAUNCH_CHECK("gram kernel");
     return reduce_upper(part, (int)grid, KT, C, G);
+}
+
+// ---- FP32 storage with FP64 refinement -------------------------------------------------------
+// e = y - W beta (caller order, FP64 columns)
+__global__ void combo_residual_kernel(const double* __restrict__ y, const double* __restrict__ W, int64_t ld, int K,
+                                      const double* __restrict__ beta, int64_t n, double* e) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v = y[i];
+        for (int k = 0; k < K; ++k) v -= beta[k] * W[i + (int64_t)k * ld];
+        e[i] = v;
+    }
+}
+
+// tables of e: a_e = a_y - sum_k beta_k a_k  (tables [C][S], column 0 = y)
+__global__ void combo_tables_kernel(const double* __restrict__ alpha, int64_t S, int K, const double* __restrict__ beta,
+                                    double* out) {
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
+        double v = alpha[g];
+        for (int k = 0; k < K; ++k) v -= beta[k] * alpha[(int64_t)(1 + k) * S + g];
+        out[g] = v;
+    }
+}
+
+// part[b][j] = sum_i X_j[i] e[i] for the FP32 work columns j = 1..C-1 (chunk of at most 16)
+template <typename TX>
+__global__ __launch_bounds__(kBlock) void xdot_kernel(const TX* __restrict__ X, int64_t n, int j0, int nj,
+                                                      const double* __restrict__ e, double* part) {
+    double acc[16];
+#pragma unroll
+    for (int k = 0; k < 16; ++k) acc[k] = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double ev = e[i];
+#pragma unroll
+        for (int k = 0; k < 16; ++k)
+            if (k < nj) acc[k] += (double)X[i + (int64_t)(j0 + k) * n] * ev;
+    }
+    __shared__ double ws[kBlock / kWave][16];
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const double v = wave_sum(acc[k]);
+        if (ln == 0) ws[wv][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        double v = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) v += ws[w][threadIdx.x];
+        part[(int64_t)blockIdx.x * 16 + threadIdx.x] = v;
+    }
+}
+
+// r = e - sum_k X_{1+k} dbeta_k (FP32 work columns)
+template <typename TX>
+__global__ void resid_update_kernel(const double* __restrict__ e, const TX* __restrict__ X, int64_t n, int K,
+                                    const double* __restrict__ dbeta, double* r) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v = e[i];
+        for (int k = 0; k < K; ++k) v -= (double)X[i + (int64_t)(1 + k) * n] * dbeta[k];
+        r[i] = v;
+    }
+}
+
+__global__ void widen_kernel(const float* __restrict__ x, int64_t count, double* y) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (double)x[i];
 }
 
 struct DevBuf {
@@ -603,7 +819,10 @@ struct ReqState {
     int Geff[FEHIP_MAX_FE] = {0};
     int dof_fes = 0, dof_nested = 0;
     int C = 0;
-    double* work = nullptr;  // n x C within-transformed [y W Z], panel order
+    double* work = nullptr;    // n x C within-transformed [y W Z], panel order (FP64 mode)
+    float* work32 = nullptr;   // the same stored in FP32 (mixed mode)
+    double* alpha = nullptr;   // mixed mode: solved FE tables of every column, [C][S]
+    int64_t S = 0;
     clk::time_point t0, t1;
 };
 
@@ -700,7 +919,19 @@ int prepare_request(Panel* p, ReqState& s) {
     s.dof_nested = dof_nested;
     s.t0 = t0;
     s.C = 1 + K + L;
-    if ((e = db.get(&s.work, (size_t)n * s.C))) return e;
+    // 2SLS keeps FP64 storage in the mixed mode: the first-stage projection (Z'Z, Z'X) must be FP64,
+    // and an over-identified fit does not drive Z'e to zero, so FP32 cross products would bias beta
+    const bool iv_request = opts->n_iv_reg > 0 && L > 0;
+    if (opts->demean.precision == 1 && !iv_request) {
+        if ((e = db.get(&s.work32, (size_t)n * s.C))) return e;
+        s.S = within_table_size(p, fe_mask);
+        if (s.S < 0) return 1;
+        if (s.S > 0 && (e = db.get(&s.alpha, (size_t)s.S * s.C))) return e;
+    } else if (opts->demean.precision == 0 || opts->demean.precision == 1) {
+        if ((e = db.get(&s.work, (size_t)n * s.C))) return e;
+    } else {
+        return set_error("fehip_estimate: precision must be 0 (FP64) or 1 (FP32 storage, FP64 refinement)");
+    }
     return 0;
 }
 
@@ -715,11 +946,7 @@ int finish_request(Panel* p, ReqState& s) {
     const fehip_opts* opts = s.opts;
     fehip_result* result = s.result;
     double *beta = s.beta, *se = s.se, *vcov = s.vcov;
-    (void)d_y;
-    (void)d_W;
     (void)d_Z;
-    (void)d_weights;
-    (void)ld;
     DevBuf& db = s.db;
     int e;
     const unsigned fe_mask = s.fe_mask;
@@ -740,12 +967,26 @@ int finish_request(Panel* p, ReqState& s) {
     double* mpart = nullptr;
     if ((e = db.get(&mpart, (size_t)kMeatBlocks * 78))) return e;
     std::vector<double> G((size_t)C * C);
+    const bool f32 = s.work32 != nullptr;
+    float* work32 = s.work32;
     if (C <= 12) {
-        if ((e = gram_fused(work, n, C, n, mpart, G))) return e;
+        if (f32) {
+            if ((e = gram_fused(work32, n, C, n, mpart, G))) return e;
+        } else if ((e = gram_fused((const double*)work, n, C, n, mpart, G))) {
+            return e;
+        }
     } else {
         double* Gd = nullptr;
         if ((e = db.get(&Gd, (size_t)C * C))) return e;
-        if ((e = gemm_impl(true, false, C, C, n, 1.0, work, n, work, n, 0.0, Gd, C, true))) return e;
+        const double* src = work;
+        if (f32) {  // wide designs: the split-K GEMM reads FP64
+            double* wide = nullptr;
+            if ((e = db.get(&wide, (size_t)n * C))) return e;
+            hipLaunchKernelGGL(widen_kernel, dim3(stream_grid((size_t)n * C, kBlock)), dim3(kBlock), 0, 0,
+                               (const float*)work32, n * C, wide);
+            src = wide;
+        }
+        if ((e = gemm_impl(true, false, C, C, n, 1.0, src, n, src, n, 0.0, Gd, C, true))) return e;
         FEHIP_CHECK(hipMemcpy(G.data(), Gd, G.size() * sizeof(double), hipMemcpyDeviceToHost), "gram copy");
         for (int j = 0; j < C; ++j)
             for (int i = j + 1; i < C; ++i) G[(size_t)j * C + i] = G[(size_t)i * C + j];
@@ -854,9 +1095,62 @@ int finish_request(Panel* p, ReqState& s) {
     // ---- residual on W~ (also for IV), rss, dof
     double *dbeta = nullptr, *r = nullptr;
     if ((e = db.get(&dbeta, (size_t)K)) || (e = db.get(&r, (size_t)n))) return e;
-    FEHIP_CHECK(hipMemcpy(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
-    hipLaunchKernelGGL(residual_work_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, work, n, K, dbeta, r);
-    FEHIP_LAUNCH_CHECK("residual");
+    if (!f32) {
+        FEHIP_CHECK(hipMemcpy(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
+        hipLaunchKernelGGL(residual_work_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, work, n, K, dbeta,
+                           r);
+        FEHIP_LAUNCH_CHECK("residual");
+    } else {
+        // FP64 refinement on the residual: e = M_F(y - W beta) in FP64 from the caller's FP64 columns and
+        // the same combination of the solved tables (M_F is linear, so no further solve), then
+        // beta += Q^-1 X~'e; one sweep (the correction's own error is second order in the FP32
+        // rounding).  The refined residual feeds rss and the sandwich.
+        double *ec = nullptr, *et = nullptr, *ae = nullptr, *xpart = nullptr;
+        uint32_t* wm1 = nullptr;
+        const int kRefineBlocks = 256;
+        if ((e = db.get(&ec, (size_t)n)) || (e = db.get(&et, (size_t)n)) ||
+            (e = db.get(&ae, (size_t)std::max<int64_t>(1, s.S))) || (e = db.get(&xpart, (size_t)kRefineBlocks * 16)))
+            return e;
+        if (d_weights) {
+            if ((e = db.get(&wm1, (size_t)n))) return e;
+            if ((e = build_mask_words(p, &d_weights, 1, wm1, 0))) return e;
+        }
+        std::vector<double> dbh(K, 0.0), gvec(C, 0.0), hx;
+        const int sweeps = 1;
+        for (int sw = 0; sw < sweeps; ++sw) {
+            FEHIP_CHECK(hipMemcpy(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
+            hipLaunchKernelGGL(combo_residual_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, d_y, d_W, ld, K,
+                               (const double*)dbeta, n, ec);
+            if (s.S > 0)
+                hipLaunchKernelGGL(combo_tables_kernel, dim3(stream_grid(s.S, kBlock)), dim3(kBlock), 0, 0,
+                                   (const double*)s.alpha, s.S, K, (const double*)dbeta, ae);
+            FEHIP_LAUNCH_CHECK("refinement residual");
+            if ((e = within_sweep_with_tables(p, s.fe_mask, ec, ae, wm1, 0, et, 0))) return e;
+            for (int j0 = 1; j0 < C; j0 += 16) {
+                const int nj = std::min(16, C - j0);
+                hipLaunchKernelGGL(xdot_kernel<float>, dim3(kRefineBlocks), dim3(kBlock), 0, 0, (const float*)work32,
+                                   n, j0, nj, (const double*)et, xpart);
+                FEHIP_LAUNCH_CHECK("refinement dots");
+                std::vector<double> gs;
+                if ((e = host_sum_parts(xpart, kRefineBlocks, 16, gs))) return e;
+                for (int k = 0; k < nj; ++k) gvec[j0 + k] = gs[k];
+            }
+            std::vector<double> db_full(K, 0.0);  // (2SLS requests are stored in FP64, never here)
+            for (int k = 0; k < K; ++k) db_full[k] = gvec[1 + k];
+            std::fill(dbh.begin(), dbh.end(), 0.0);
+            for (int a2 = 0; a2 < kept; ++a2) {
+                double v = 0.0;
+                for (int c2 = 0; c2 < kept; ++c2) v += Qinv[(size_t)c2 * kept + a2] * db_full[keep[c2]];
+                dbh[keep[a2]] = v;
+            }
+            for (int k = 0; k < K; ++k) beta[k] += dbh[k];
+        }
+        // residual of the refined beta: e~ of the last sweep minus X~ times the last correction
+        FEHIP_CHECK(hipMemcpy(dbeta, dbh.data(), sizeof(double) * K, hipMemcpyHostToDevice), "correction copy");
+        hipLaunchKernelGGL(resid_update_kernel<float>, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0,
+                           (const double*)et, (const float*)work32, n, K, (const double*)dbeta, r);
+        FEHIP_LAUNCH_CHECK("refined residual");
+    }
     double rss = 0.0;
     if ((e = sumsq(r, n, part, &rss))) return e;
     result->rss = rss;
@@ -867,7 +1161,7 @@ int finish_request(Panel* p, ReqState& s) {
     bool clustered = ncd > 0;
     if (clustered) {
         // regressors for the scores: W~, or X^ = [W~ Z~] proj_full for IV (fe_pipeline.f90:1181-1186)
-        const double* X = work + n;  // W~ columns
+        const double* X = f32 ? nullptr : work + n;  // W~ columns
         int64_t ldX = n;
         double* Xhat = nullptr;
         if (use_iv) {
@@ -881,11 +1175,21 @@ int finish_request(Panel* p, ReqState& s) {
                 return e;
             X = Xhat;
         }
+        // scores T = r x (row-major n x K), formed only if a subset's method reads it
         double* T = nullptr;
-        if ((e = db.get(&T, (size_t)n * K))) return e;
-        hipLaunchKernelGGL(scores_rows_kernel, dim3(stream_grid((size_t)n * K, kBlock)), dim3(kBlock), 0, 0, r, X, ldX,
-                           n, K, T);
-        FEHIP_LAUNCH_CHECK("scores rows");
+        auto need_T = [&]() -> int {
+            if (T) return 0;
+            int e2 = db.get(&T, (size_t)n * K);
+            if (e2) return e2;
+            if (X)
+                hipLaunchKernelGGL(scores_rows_kernel<double>, dim3(stream_grid((size_t)n * K, kBlock)), dim3(kBlock),
+                                   0, 0, (const double*)r, X, ldX, n, K, T);
+            else
+                hipLaunchKernelGGL(scores_rows_kernel<float>, dim3(stream_grid((size_t)n * K, kBlock)), dim3(kBlock),
+                                   0, 0, (const double*)r, (const float*)(work32 + n), ldX, n, K, T);
+            FEHIP_LAUNCH_CHECK("scores rows");
+            return 0;
+        };
         double* S = nullptr;
         int64_t S_cap = 0;
         int64_t* dcount = nullptr;
@@ -915,7 +1219,46 @@ int finish_request(Panel* p, ReqState& s) {
                 }
             const int single = len == 1 ? __builtin_ctz(dims_mask) : -1;
             int64_t ncl = 0;
-            if (has_p && p->n_long == 0 && K <= 12) {
+            int nother = 0;
+            for (int d = 0; d < p->m; ++d) nother += (d != p->P && ((dims_mask >> d) & 1u)) ? 1 : 0;
+            if (has_p && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2) {
+                WaveNest wn{};
+                wn.r = r;
+                wn.ldX = n;
+                wn.K = K;
+                wn.idP = p->d[p->P].ids;
+                wn.seg_off = p->seg_off;
+                wn.wtile_seg = p->wtile_seg;
+                wn.n_wtiles = p->n_wtiles;
+                for (int d = 0; d < p->m; ++d)
+                    if (d != p->P && ((dims_mask >> d) & 1u)) wn.kid[wn.nk++] = p->d[d].ids;
+                wn.w = wpanel;
+                FEHIP_CHECK(hipMemsetAsync(ucount, 0, sizeof(unsigned long long), 0), "memset");
+                const unsigned grid = (unsigned)std::max<int64_t>(
+                    1, std::min<int64_t>((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave), kMeatBlocks));
+                const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
+                const float* X32 = X ? nullptr : (const float*)(work32 + n);
+#define FEHIP_WNEST(KTV)                                                                                        \
+    if (X32)                                                                                                    \
+        hipLaunchKernelGGL((wave_nested_meat_kernel<KTV, float>), dim3(grid), dim3(kBlock), 0, 0, wn, X32, mpart, \
+                           ucount);                                                                             \
+    else                                                                                                        \
+        hipLaunchKernelGGL((wave_nested_meat_kernel<KTV, double>), dim3(grid), dim3(kBlock), 0, 0, wn, X, mpart, ucount);
+                if (KT == 4) {
+                    FEHIP_WNEST(4)
+                } else if (KT == 8) {
+                    FEHIP_WNEST(8)
+                } else {
+                    FEHIP_WNEST(12)
+                }
+#undef FEHIP_WNEST
+                FEHIP_LAUNCH_CHECK("wave nested meat");
+                if ((e = reduce_upper(mpart, (int)grid, KT, K, M))) return e;
+                unsigned long long h = 0;
+                FEHIP_CHECK(hipMemcpy(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
+                ncl = (int64_t)h;
+            } else if (has_p && p->n_long == 0 && K <= 12) {
+                if ((e = need_T())) return e;
                 NestKeys nk{};
                 for (int d = 0; d < p->m; ++d)
                     if (d != p->P && ((dims_mask >> d) & 1u)) nk.ids[nk.nk++] = p->d[d].ids;
@@ -937,6 +1280,7 @@ int finish_request(Panel* p, ReqState& s) {
                 FEHIP_CHECK(hipMemcpy(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
                 ncl = (int64_t)h;
             } else if (all_small && cells <= kSmallCells && K <= 11) {
+                if ((e = need_T())) return e;
                 SmallKeys sk{};
                 int stride = 1;
                 for (int d = 0; d < p->m; ++d)
@@ -972,6 +1316,7 @@ int finish_request(Panel* p, ReqState& s) {
                             M[(size_t)b2 * K + a2] += hS[(size_t)q * KT + a2] * hS[(size_t)q * KT + b2];
                 }
             } else if (single >= 0 && single != p->P && !p->d[single].small && p->n_long == 0 && K <= 12) {
+                if ((e = need_T())) return e;
                 const DimStruct& D = p->d[single];
                 const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
                 double *rb = nullptr, *Sg = nullptr;
@@ -1001,6 +1346,7 @@ int finish_request(Panel* p, ReqState& s) {
                 if ((e = meat_fused(Sg, K, KT, nullptr, nullptr, D.G, mpart, M))) return e;
                 ncl = Geff[single];
             } else {
+                if ((e = need_T())) return e;
                 const Partition* part_s = nullptr;
                 if ((e = panel_partition(p, dims_mask, &part_s, 0))) return e;
                 ncl = part_s->n_clusters;
@@ -1013,22 +1359,19 @@ int finish_request(Panel* p, ReqState& s) {
                     if ((e = meat_fused(T, K, K, nullptr, nullptr, n, mpart, M))) return e;
                 } else if (K <= 12) {
                     const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
-                    if (nS * KT > S_cap) {
-                        if (S) db.release(S);
-                        S_cap = nS * KT;
-                        if ((e = db.get(&S, (size_t)S_cap))) return e;
-                    }
+                    const unsigned grid = (unsigned)std::max<int64_t>(
+                        1, std::min<int64_t>((nS + kBlock - 1) / kBlock, (int64_t)kMeatBlocks));
                     if (KT == 4)
-                        e = cluster_sums<4>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
+                        hipLaunchKernelGGL(sorted_meat_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, (const double*)T, K,
+                                           (const int*)ord, (const int64_t*)off, nS, mpart);
                     else if (KT == 8)
-                        e = cluster_sums<8>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
+                        hipLaunchKernelGGL(sorted_meat_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, (const double*)T, K,
+                                           (const int*)ord, (const int64_t*)off, nS, mpart);
                     else
-                        e = cluster_sums<12>(T, K, part_s->order, part_s->keys, n, S, dcount, 0);
-                    if (e) return e;
-                    // aggregates exist only for keys present (a primary id with no rows has none)
-                    int64_t n_present = 0;
-                    FEHIP_CHECK(hipMemcpy(&n_present, dcount, sizeof(int64_t), hipMemcpyDeviceToHost), "count copy");
-                    if ((e = meat_fused(S, K, KT, nullptr, nullptr, n_present, mpart, M))) return e;
+                        hipLaunchKernelGGL(sorted_meat_kernel<12>, dim3(grid), dim3(kBlock), 0, 0, (const double*)T, K,
+                                           (const int*)ord, (const int64_t*)off, nS, mpart);
+                    FEHIP_LAUNCH_CHECK("sorted meat");
+                    if ((e = reduce_upper(mpart, (int)grid, KT, K, M))) return e;
                 } else {
                     if (nS * K > S_cap) {
                         if (S) db.release(S);
@@ -1155,7 +1498,9 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
                 const double* x = j == 0 ? s.d_y : (j <= s.K ? s.d_W + (int64_t)(j - 1) * s.ld
                                                              : s.d_Z + (int64_t)(j - 1 - s.K) * s.ld);
                 wc.x[c] = x;
-                wc.out[c] = s.work + (int64_t)j * n;
+                wc.out[c] = s.work ? s.work + (int64_t)j * n : nullptr;
+                wc.out32[c] = s.work32 ? s.work32 + (int64_t)j * n : nullptr;
+                wc.alpha[c] = s.alpha ? s.alpha + (int64_t)j * s.S : nullptr;
                 wc.slot[c] = wm ? (int)b : 0;
             }
         }

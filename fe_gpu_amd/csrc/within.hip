@@ -27,6 +27,7 @@
 
 #include "kstats.hpp"
 #include "panel.hpp"
+#include "wave.hpp"
 #include "within.hpp"
 
 namespace fehip {
@@ -65,7 +66,8 @@ struct ApplyArgs {
     const double* p;           // kIter: [S][CT]
     const int* active;         // kIter: per-column CG activity (nullptr = all active)
     const double* xc[kMaxCT];  // kData/kFinal: input columns (caller row order)
-    double* oc[kMaxCT];        // kFinal: output columns
+    double* oc[kMaxCT];        // kFinal: output columns (FP64)
+    float* oc32[kMaxCT];       // kFinal: FP32 output columns (panel order), used where non-null
     int ncols;                 // valid columns in this tile
     const int* perm;
     double* small_part;        // [blocks][Gs][CT]
@@ -161,49 +163,6 @@ __device__ __forceinline__ int tile_runs(bool valid, bool rhead, int j, int nr, 
 // every lane of its segment by one lane permute, and the weighted row counts come from ballots.
 // Nothing is staged in LDS except the small-dim tables; the waves of a block run independently.
 
-// x moved across lanes by a DPP control (lanes without a source read 0)
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double x) {
-    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
-    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xf, 0xf, false);
-    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xf, 0xf, false);
-    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
-}
-
-// Lane predicates of a segmented inclusive scan whose segment of this lane starts at lane s.
-struct ScanMask {
-    bool m1, m2, m4, m8, b15, b31;
-    __device__ __forceinline__ ScanMask(int lane, int s) {
-        const int rl = lane & 15;
-        m1 = rl >= 1 && lane - 1 >= s;
-        m2 = rl >= 2 && lane - 2 >= s;
-        m4 = rl >= 4 && lane - 4 >= s;
-        m8 = rl >= 8 && lane - 8 >= s;
-        b15 = (lane & 31) >= 16 && s < (lane & ~15);
-        b31 = lane >= 32 && s < 32;
-    }
-};
-
-// sum of x over lanes [s, lane] of the wave (every lane of the wave must take part)
-__device__ __forceinline__ double seg_scan(double x, const ScanMask& m) {
-    double t;
-    t = dpp_move<0x111>(x);  // row_shr:1
-    x += m.m1 ? t : 0.0;
-    t = dpp_move<0x112>(x);  // row_shr:2
-    x += m.m2 ? t : 0.0;
-    t = dpp_move<0x114>(x);  // row_shr:4
-    x += m.m4 ? t : 0.0;
-    t = dpp_move<0x118>(x);  // row_shr:8
-    x += m.m8 ? t : 0.0;
-    t = dpp_move<0x142>(x);  // row_bcast:15
-    x += m.b15 ? t : 0.0;
-    t = dpp_move<0x143>(x);  // row_bcast:31
-    x += m.b31 ? t : 0.0;
-    return x;
-}
-
-__device__ __forceinline__ unsigned long long mask_le(int lane) { return (2ull << lane) - 1ull; }
-
 template <int CT, int MODE>
 __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     if constexpr (MODE == kIter) {
@@ -275,7 +234,9 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
 #pragma unroll
                 for (int c = 0; c < CT; ++c) {
                     if (c >= a.ncols) break;
-                    if (a.to_work)
+                    if (a.oc32[c])
+                        a.oc32[c][r] = (float)v[c];
+                    else if (a.to_work)
                         a.oc[c][r] = v[c];
                     else if (word & cb[c])
                         a.oc[c][src] = v[c];
@@ -387,7 +348,9 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
 #pragma unroll
                 for (int c = 0; c < CT; ++c) {
                     if (c >= a.ncols) break;
-                    if (a.to_work)
+                    if (a.oc32[c])
+                        a.oc32[c][r] = (float)sv[j * CT + c];
+                    else if (a.to_work)
                         a.oc[c][r] = sv[j * CT + c];
                     else if (word & a.cbit[c])
                         a.oc[c][src] = sv[j * CT + c];
@@ -574,7 +537,9 @@ __global__ __launch_bounds__(kBlock) void long_kernel(ApplyArgs a) {
                 row_value<CT, MODE>(a, r, sp, v);
                 for (int c = 0; c < a.ncols; ++c) {
                     const bool on = word & a.cbit[c];
-                    if (a.to_work)
+                    if (a.oc32[c])
+                        a.oc32[c][r] = on ? (float)(v[c] - mean[c]) : 0.0f;
+                    else if (a.to_work)
                         a.oc[c][r] = on ? v[c] - mean[c] : 0.0;
                     else if (on)
                         a.oc[c][src] = v[c] - mean[c];
@@ -899,6 +864,12 @@ __global__ void make_minv_kernel(int64_t S, const double* __restrict__ dg, ColSl
     }
 }
 
+// out[g] = tab[g][c] of a [S][CT] table
+__global__ void take_column_kernel(const double* __restrict__ tab, int64_t S, int CT, int c, double* out) {
+    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x)
+        out[g] = tab[g * CT + c];
+}
+
 __global__ void mask_words_kernel(const uint8_t* __restrict__ w, const int* __restrict__ perm, int64_t n, uint32_t bit,
                                   int first, uint32_t* wm) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1141,6 +1112,7 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
         a.anybit |= a.cbit[c];
         a.xc[c] = in ? wc.x[c0 + c] : nullptr;
         a.oc[c] = in ? wc.out[c0 + c] : nullptr;
+        a.oc32[c] = in ? wc.out32[c0 + c] : nullptr;
         cs.slot[c] = slot;
     }
     a.ncols = ncols;
@@ -1204,6 +1176,11 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             ts.col_max_update[c] = sc.st_host->maxn[c];
         }
         ts.apply_launches = it;
+        for (int c = 0; c < ncols; ++c)
+            if (wc.alpha[c0 + c])
+                hipLaunchKernelGGL(take_column_kernel, dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, s,
+                                   (const double*)xa, S, CT, c, wc.alpha[c0 + c]);
+        FEHIP_LAUNCH_CHECK("keep tables");
     }
     // final pass over the columns: M_P (x - D_o a)
     a.alpha = xa;
@@ -1230,6 +1207,42 @@ int build_mask_words(Panel* p, const uint8_t* const* w_caller, int nslots, uint3
                            (const int*)p->perm, p->n, 1u << b, b == 0 ? 1 : 0, wm);
         FEHIP_LAUNCH_CHECK("mask words");
     }
+    return 0;
+}
+
+int64_t within_table_size(Panel* p, unsigned fe_mask) {
+    unsigned mask = fe_mask ? fe_mask : ((1u << p->m) - 1u);
+    mask &= (1u << p->m) - 1u;
+    Layout L;
+    if (make_layout(p, mask, L)) return -1;
+    return L.S;
+}
+
+int within_sweep_with_tables(Panel* p, unsigned fe_mask, const double* x, const double* alpha, const uint32_t* wm,
+                             int slot, double* out, hipStream_t s) {
+    unsigned mask = fe_mask ? fe_mask : ((1u << p->m) - 1u);
+    mask &= (1u << p->m) - 1u;
+    Layout L;
+    int e = make_layout(p, mask, L);
+    if (e) return e;
+    if ((size_t)L.Gs * 16 > 28 * 1024) return set_error("small FE dims exceed the LDS budget (%d groups)", L.Gs);
+    Scratch sc;
+    ApplyArgs a = base_args(p, L, sc, wm);
+    for (int c = 0; c < kMaxCT; ++c) {
+        a.cbit[c] = 0u;
+        a.xc[c] = nullptr;
+        a.oc[c] = nullptr;
+        a.oc32[c] = nullptr;
+    }
+    a.cbit[0] = wm ? (1u << slot) : ~0u;
+    a.anybit = a.cbit[0];
+    a.xc[0] = x;
+    a.oc[0] = out;
+    a.ncols = 1;
+    a.to_work = 1;
+    a.alpha = alpha;
+    launch_pass<1, kFinal>(p, L, a, s);
+    FEHIP_LAUNCH_CHECK("sweep with tables");
     return 0;
 }
 

@@ -32,12 +32,26 @@ struct WithinStats {
 };
 
 // Columns of one solve: input and output device pointers and the request slot of each column.
+// out32 (optional, per column) takes the result in FP32 instead of out; alpha (optional, per
+// column) receives the solved FE tables ([S] doubles, S = within_table_size).
 struct WithinCols {
-    explicit WithinCols(int n) : x(n), out(n), slot(n, 0) {}
+    explicit WithinCols(int n) : x(n), out(n), out32(n, nullptr), alpha(n, nullptr), slot(n, 0) {}
     std::vector<const double*> x;
     std::vector<double*> out;
+    std::vector<float*> out32;
+    std::vector<double*> alpha;
     std::vector<int> slot;
 };
+
+// Size of the FE-table space of a solve over fe_mask (groups of the absorbed non-primary dims).
+int64_t within_table_size(Panel* p, unsigned fe_mask);
+
+// One FP64 sweep M_P(x - D_o alpha) of a single column x (caller order) with given tables alpha,
+// into out (panel order, zeros on rows outside the slot).  The tables of a linear combination of
+// solved columns are the same combination of their tables, so this forms M_F of a combination
+// (the regression residual) exactly without another solve.
+int within_sweep_with_tables(Panel* p, unsigned fe_mask, const double* x, const double* alpha, const uint32_t* wm,
+                             int slot, double* out, hipStream_t s);
 
 // Panel-order row mask words from up to 32 caller-order 0/1 weight arrays (nullptr = every row).
 int build_mask_words(Panel* p, const uint8_t* const* w_caller, int nslots, uint32_t* wm, hipStream_t s);
