@@ -196,6 +196,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--serving-only", action="store_true", help="run only the request-stream leg")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run a plain device copy of MIB MiB (torch), the reference kernel for HBM counters")
     ap.add_argument("--serve-workers", type=int, default=500_000)
     ap.add_argument("--serve-requests", type=int, default=400)
     ap.add_argument("--serve-seq", type=int, default=100)
@@ -213,6 +215,14 @@ def main():
 
     info = device.init(0)
     lib = load()
+    if args.calibrate_copy > 0:
+        from fe_gpu_amd.device import DeviceBuffer as _DB
+        nb = args.calibrate_copy << 20
+        src, dst = _DB(nb), _DB(nb)
+        check(lib.fehip_device_copy(src.ptr, dst.ptr, nb), "calibration copy")
+        log(f"calibration copy: {nb} bytes read, {nb} bytes written")
+        src.free()
+        dst.free()
     if args.serving_only:
         print(json.dumps({"serving": serving_leg(args, lib)}), flush=True)
         return

@@ -73,6 +73,7 @@ COARSE = [
     ("fehip_kstats_enable", C.c_int, [C.c_int]),
     ("fehip_kstats_reset", C.c_int, []),
     ("fehip_kstats_get", C.c_int, [C.c_int, C.c_char_p, C.c_int, C.POINTER(I64), DP, DP, DP]),
+    ("fehip_device_copy", C.c_int, [VP, VP, I64]),
 ]
 
 

@@ -295,11 +295,12 @@ struct WaveNest {
     int K;
     const int* idP;
     const int64_t* seg_off;
-    const int64_t* wtile_seg;
+    const int64_t* wtile_rec;   // first row | rows << 32 of each wave tile
     int64_t n_wtiles;
     const int* kid[2];          // the subset's other dims (panel order ids)
     int nk;
     const uint8_t* w;           // panel-order weights (nullptr: all rows)
+    int mode;                   // 2: singletons, 1: clusters are single runs, 0: merge runs of a segment
 };
 
 template <int KT, typename TX>
@@ -312,10 +313,14 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
     const int lane = threadIdx.x & (kWave - 1);
     unsigned long long nclusters = 0;
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
-    for (int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; t < a.n_wtiles; t += wstride) {
-        const int64_t s0 = a.wtile_seg[2 * t], s1 = a.wtile_seg[2 * t + 1];
-        const int64_t R0 = a.seg_off[s0];
-        const int nr = (int)(a.seg_off[s1] - R0);
+    int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    int64_t rec = t < a.n_wtiles ? a.wtile_rec[t] : 0;
+    for (; t < a.n_wtiles; t += wstride) {
+        const int64_t tn = t + wstride;
+        const int64_t rec_n = tn < a.n_wtiles ? a.wtile_rec[tn] : 0;  // next tile, fetched early
+        const int64_t R0 = rec & 0xffffffffll;
+        const int nr = (int)(rec >> 32);
+        rec = rec_n;
         if (nr <= 0) continue;
         const bool valid = lane < nr;
         const int64_t ri = R0 + (valid ? lane : 0);
@@ -340,10 +345,10 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
 #pragma unroll
         for (int k = 0; k < KT; ++k) {
             const double tv = (k < a.K && valid) ? rr * (double)X[ri + (int64_t)k * a.ldX] : 0.0;
-            sv[k] = seg_scan(tv, rm);
+            sv[k] = a.mode == 2 ? tv : seg_scan(tv, rm);
         }
         bool leader = rend;
-        if (a.nk > 0) {
+        if (a.nk > 0 && a.mode == 0) {
             // merge the runs of one segment that carry the same ids
             unsigned long long E = __ballot(rend);
             while (E) {
@@ -735,6 +740,17 @@ __global__ void widen_kernel(const float* __restrict__ x, int64_t count, double*
         y[i] = (double)x[i];
 }
 
+// heads of the runs of a composite key inside primary segments
+__global__ __launch_bounds__(kBlock) void composite_runs_kernel(const int* __restrict__ idP, const int* __restrict__ k0,
+                                                                const int* __restrict__ k1, int64_t n,
+                                                                unsigned long long* count) {
+    unsigned long long c = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        c += (i == 0 || idP[i] != idP[i - 1] || (k0 && k0[i] != k0[i - 1]) || (k1 && k1[i] != k1[i - 1])) ? 1 : 0;
+    for (int o = kWave / 2; o > 0; o >>= 1) c += __shfl_down(c, o, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(count, c);
+}
+
 struct DevBuf {
     std::vector<void*> v;
     DevBuf() = default;
@@ -795,6 +811,32 @@ void finalize_stats(fehip_result* r, int64_t n_obs) {
     r->f_stat = (dfm > 0.0 && dfr > 0.0 && dev > EPS) ? (ssr / dfm) / (dev / dfr) : 0.0;
 }
 
+
+// How clusters of a subset containing the primary lie inside the segments (cached per panel): compares
+// the subset's cluster count with its count of runs inside segments.
+int nest_mode(Panel* p, unsigned dims_mask, const int* const* kid, int nk, unsigned long long* dcount, int* mode) {
+    auto it = p->nest_modes.find(dims_mask);
+    if (it != p->nest_modes.end()) {
+        *mode = it->second;
+        return 0;
+    }
+    int m = 1;
+    if (nk > 0) {
+        const Partition* part_s = nullptr;
+        int e = panel_partition(p, dims_mask, &part_s, 0);
+        if (e) return e;
+        FEHIP_CHECK(hipMemsetAsync(dcount, 0, sizeof(unsigned long long), 0), "memset");
+        hipLaunchKernelGGL(composite_runs_kernel, dim3(stream_grid((size_t)p->n, kBlock)), dim3(kBlock), 0, 0,
+                           (const int*)p->d[p->P].ids, kid[0], nk > 1 ? kid[1] : nullptr, p->n, dcount);
+        FEHIP_LAUNCH_CHECK("composite runs");
+        unsigned long long runs = 0;
+        FEHIP_CHECK(hipMemcpy(&runs, dcount, sizeof(runs), hipMemcpyDeviceToHost), "runs copy");
+        m = part_s->n_clusters == p->n ? 2 : ((unsigned long long)part_s->n_clusters == runs ? 1 : 0);
+    }
+    p->nest_modes[dims_mask] = m;
+    *mode = m;
+    return 0;
+}
 
 // One request of a batch: its inputs, and what the preparation step leaves for the solve and the
 // normal equations / sandwich step.
@@ -1228,11 +1270,12 @@ int finish_request(Panel* p, ReqState& s) {
                 wn.K = K;
                 wn.idP = p->d[p->P].ids;
                 wn.seg_off = p->seg_off;
-                wn.wtile_seg = p->wtile_seg;
+                wn.wtile_rec = p->wtile_rec;
                 wn.n_wtiles = p->n_wtiles;
                 for (int d = 0; d < p->m; ++d)
                     if (d != p->P && ((dims_mask >> d) & 1u)) wn.kid[wn.nk++] = p->d[d].ids;
                 wn.w = wpanel;
+                if ((e = nest_mode(p, dims_mask, wn.kid, wn.nk, ucount, &wn.mode))) return e;
                 FEHIP_CHECK(hipMemsetAsync(ucount, 0, sizeof(unsigned long long), 0), "memset");
                 const unsigned grid = (unsigned)std::max<int64_t>(
                     1, std::min<int64_t>((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave), kMeatBlocks));

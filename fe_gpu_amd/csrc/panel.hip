@@ -125,6 +125,12 @@ __global__ void write_runs_kernel(const int64_t* __restrict__ seg_off, int64_t G
     }
 }
 
+// dest[order[k]] = k
+__global__ void invert_order_kernel(const int* __restrict__ order, int64_t n, int* dest) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        dest[order[k]] = (int)k;
+}
+
 __global__ void run_hist_kernel(const unsigned* __restrict__ grp, int64_t n, int* counts) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) atomicAdd(&counts[grp[i]], 1);
@@ -197,12 +203,14 @@ void free_panel(Panel* p) {
     dfree(p->tile_seg);
     dfree(p->long_segs);
     dfree(p->wtile_seg);
+    dfree(p->wtile_rec);
     dfree(p->mtile_seg);
     for (int d = 0; d < p->m; ++d) {
         dfree(p->d[d].ids);
         dfree(p->d[d].counts);
         dfree(p->d[d].seg_run_off);
         dfree(p->d[d].run_order);
+        dfree(p->d[d].run_dest);
         dfree(p->d[d].grp_run_off);
     }
     delete p;
@@ -265,12 +273,15 @@ int build_runs(Panel* p, int d, hipStream_t s) {
     if (vb.current() != D.run_order)
         FEHIP_CHECK(hipMemcpyAsync(D.run_order, vb.current(), sizeof(int) * (size_t)R, hipMemcpyDeviceToDevice, s),
                     "run order copy");
+    if ((st = dalloc(&D.run_dest, R, &p->bytes))) return st;
+    hipLaunchKernelGGL(invert_order_kernel, dim3(stream_grid((size_t)R, kBlock)), dim3(kBlock), 0, s,
+                       (const int*)D.run_order, R, D.run_dest);
+    FEHIP_LAUNCH_CHECK("run destinations");
     FEHIP_CHECK(hipStreamSynchronize(s), "runs sync");
     dfree(grp);
     dfree(grp2);
     dfree(idx2);
     dfree(cnt);
-    p->bytes += 0;
     return 0;
 }
 
@@ -323,9 +334,16 @@ int build_tiles(Panel* p) {
     }
     p->n_wtiles = (int64_t)wt.size() / 2;
     p->n_mtiles = (int64_t)mt.size() / 2;
+    std::vector<int64_t> rec((size_t)p->n_wtiles + 1, 0);
+    for (int64_t t = 0; t < p->n_wtiles; ++t) {
+        const int64_t r0 = off[wt[2 * t]], nr = off[wt[2 * t + 1]] - r0;
+        rec[t] = r0 | (nr << 32);
+    }
     int st;
-    if ((st = dalloc(&p->wtile_seg, wt.size() + 2, &p->bytes)) || (st = dalloc(&p->mtile_seg, mt.size() + 2, &p->bytes)))
+    if ((st = dalloc(&p->wtile_seg, wt.size() + 2, &p->bytes)) || (st = dalloc(&p->mtile_seg, mt.size() + 2, &p->bytes)) ||
+        (st = dalloc(&p->wtile_rec, rec.size(), &p->bytes)))
         return st;
+    FEHIP_CHECK(hipMemcpy(p->wtile_rec, rec.data(), rec.size() * sizeof(int64_t), hipMemcpyHostToDevice), "wave records");
     if (!wt.empty())
         FEHIP_CHECK(hipMemcpy(p->wtile_seg, wt.data(), wt.size() * sizeof(int64_t), hipMemcpyHostToDevice), "wave tiles");
     if (!mt.empty())

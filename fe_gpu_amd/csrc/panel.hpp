@@ -28,6 +28,7 @@ struct DimStruct {
     int64_t* seg_run_off = nullptr;  // [G_P + 1] first run of each primary segment
     int* run_order = nullptr;        // run indices sorted by group
     int64_t* grp_run_off = nullptr;  // [G + 1] offsets into run_order
+    int* run_dest = nullptr;         // position of each run in group order (inverse of run_order)
 };
 
 // Row partition of a cluster subset: rows of cluster c are order[off[c] .. off[c+1]).
@@ -56,10 +57,14 @@ struct Panel {
     // (whole segments longer than kWaveRows, at most kTileRows rows per tile)
     int64_t n_wtiles = 0;
     int64_t* wtile_seg = nullptr;  // [2 * n_wtiles]
+    int64_t* wtile_rec = nullptr;  // [n_wtiles] first row | rows << 32
     int64_t n_mtiles = 0;
     int64_t* mtile_seg = nullptr;  // [2 * n_mtiles]
     DimStruct d[FEHIP_MAX_FE];
     std::map<unsigned, Partition> partitions;  // key: bit mask of 0-based dims
+    // subsets containing the primary: 2 = every row its own cluster, 1 = each cluster is one run inside
+    // its segment, 0 = runs of one segment may share a cluster
+    std::map<unsigned, int> nest_modes;
     int64_t bytes = 0;
 };
 

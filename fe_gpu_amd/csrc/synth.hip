@@ -86,3 +86,21 @@ extern "C" int fehip_synth_worker_firm(int64_t n_workers, int32_t n_periods, int
     FEHIP_CHECK(hipDeviceSynchronize(), "synth sync");
     return set_ok();
 }
+
+// Plain streaming copy (16-byte vectors), the reference kernel for calibrating HBM byte counters.
+__global__ void device_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n16) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
+extern "C" int fehip_device_copy(const void* d_src, void* d_dst, int64_t bytes) {
+    (void)hipGetLastError();
+    if (!d_src || !d_dst || bytes < 0 || (bytes & 15)) return fehip::set_error("fehip_device_copy: bad arguments");
+    const int64_t n16 = bytes / 16;
+    if (n16 == 0) return fehip::set_ok();
+    hipLaunchKernelGGL(device_copy_kernel, dim3(fehip::stream_grid((size_t)n16, 256)), dim3(256), 0, 0,
+                       (const uint4*)d_src, (uint4*)d_dst, n16);
+    FEHIP_LAUNCH_CHECK("device copy");
+    FEHIP_CHECK(hipDeviceSynchronize(), "device copy");
+    return fehip::set_ok();
+}

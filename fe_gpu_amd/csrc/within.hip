@@ -48,7 +48,8 @@ struct ScatterDim {
     int lds_off;                 // small: group offset in LDS
     int large_slot;              // large: 0..kMaxLarge-1
     const int64_t* seg_run_off;  // large
-    double* runbuf;              // large: [n_runs][CT]
+    const int* run_dest;         // large: group-order position of each run
+    double* runbuf;              // large: [n_runs][CT] in group order
 };
 
 struct ApplyArgs {
@@ -75,6 +76,7 @@ struct ApplyArgs {
     int64_t n_tiles;
     const int* idP;            // primary ids (panel order): segment heads of the wave tiles
     const int64_t* wtile_seg;  // [2 * n_wtiles] segment range of each wave tile
+    const int64_t* wtile_rec;  // [n_wtiles] first row | rows << 32
     int64_t n_wtiles;
     const int64_t* long_segs;  // segments longer than a tile
     int64_t n_long;
@@ -107,6 +109,64 @@ __device__ __forceinline__ void row_value(const ApplyArgs& a, int64_t r, const d
                 for (int c = 0; c < CT; ++c) v[c] += sign * sp[base + c];
             } else {
                 const int64_t base = (d.tab_off + g) * CT;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) v[c] += sign * tab[base + c];
+            }
+        }
+    }
+}
+
+// Row data of one wave tile, loaded one tile ahead of its use.
+struct WaveRows {
+    int64_t rec;               // first row | rows << 32
+    int64_t s0;                // first segment
+    int ip;                    // primary id
+    uint32_t word;             // mask word
+    int g[kMaxScatter];        // ids of the scattered dims
+};
+
+__device__ __forceinline__ void load_wave_rows(const ApplyArgs& a, int64_t t, int lane, WaveRows& w) {
+    if (t >= a.n_wtiles) {
+        w.rec = 0;
+        return;
+    }
+    w.rec = a.wtile_rec[t];
+    w.s0 = a.wtile_seg[2 * t];
+    const int64_t R0 = w.rec & 0xffffffffll;
+    const int nr = (int)(w.rec >> 32);
+    const bool valid = lane < nr;
+    const int64_t r = R0 + (valid ? lane : 0);
+    w.ip = valid ? a.idP[r] : -1;
+    w.word = valid ? row_word(a, r) : 0u;
+#pragma unroll
+    for (int o = 0; o < kMaxScatter; ++o)
+        if (o < a.nd) w.g[o] = valid ? a.sd[o].ids[r] : -1;
+}
+
+// row_value with the group ids already in registers
+template <int CT, int MODE>
+__device__ __forceinline__ void row_value_ids(const ApplyArgs& a, int64_t r, const int (&g)[kMaxScatter],
+                                              const double* sp, double (&v)[CT]) {
+#pragma unroll
+    for (int c = 0; c < CT; ++c) v[c] = 0.0;
+    if constexpr (MODE == kData || MODE == kFinal) {
+        const int64_t src = a.perm ? a.perm[r] : r;
+#pragma unroll
+        for (int c = 0; c < CT; ++c) v[c] = c < a.ncols ? a.xc[c][src] : 0.0;
+    }
+    if constexpr (MODE == kIter || MODE == kFinal) {
+        const double sign = MODE == kIter ? 1.0 : -1.0;
+        const double* tab = MODE == kIter ? a.p : a.alpha;
+#pragma unroll
+        for (int o = 0; o < kMaxScatter; ++o) {
+            if (o >= a.nd) break;
+            const ScatterDim& d = a.sd[o];
+            if (d.small) {
+                const size_t base = (size_t)(d.lds_off + g[o]) * CT;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) v[c] += sign * sp[base + c];
+            } else {
+                const int64_t base = (d.tab_off + g[o]) * CT;
 #pragma unroll
                 for (int c = 0; c < CT; ++c) v[c] += sign * tab[base + c];
             }
@@ -186,17 +246,27 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
-    for (int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; t < a.n_wtiles; t += wstride) {
-        const int64_t s0 = a.wtile_seg[2 * t], s1 = a.wtile_seg[2 * t + 1];
-        const int64_t R0 = a.seg_off[s0];
-        const int nr = (int)(a.seg_off[s1] - R0);
-        if (nr <= 0) continue;  // a tile of empty segments (uniform across the wave)
+    // software pipeline: the next tile's row data (record, ids, mask words) is loaded while the current
+    // tile is processed, so a tile waits only for its table gathers
+    int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    WaveRows cur, nxt;
+    load_wave_rows(a, t, lane, cur);
+    for (; t < a.n_wtiles; t += wstride) {
+        load_wave_rows(a, t + wstride, lane, nxt);
+        const int64_t R0 = cur.rec & 0xffffffffll;
+        const int nr = (int)(cur.rec >> 32);
+        const int64_t s0c = cur.s0;
         const bool valid = lane < nr;
         const int64_t r = R0 + (valid ? lane : 0);
-        const uint32_t word = valid ? row_word(a, r) : 0u;
+        const uint32_t word = cur.word;
         const bool on = (word & anyb) != 0;
+        const int ip = cur.ip;
+        int gid[kMaxScatter];
+#pragma unroll
+        for (int o = 0; o < kMaxScatter; ++o) gid[o] = o < a.nd ? cur.g[o] : -1;
+        cur = nxt;
+        if (nr <= 0) continue;  // a tile of empty segments (uniform across the wave)
         // segment of this lane: [ss, se]
-        const int ip = valid ? a.idP[r] : -1;
         const int ipp = __shfl_up(ip, 1, kWave);
         const bool hs = valid && (lane == 0 || ip != ipp);
         const unsigned long long H = __ballot(hs);
@@ -206,26 +276,41 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         double v[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) v[c] = 0.0;
-        if (on) row_value<CT, MODE>(a, r, sp, v);
+        if (on) row_value_ids<CT, MODE>(a, r, gid, sp, v);
 #pragma unroll
         for (int c = 0; c < CT; ++c) v[c] = (word & cb[c]) ? v[c] : 0.0;
-        if (a.elim) {
-            const ScanMask sm(lane, ss);
+        // One segmented scan per column gives the prefix x = sum of v over [segment start, lane]: its value
+        // at the segment end is the segment total (for the mean), and differences of it are the run sums
+        // of the large dims, so no second scan is needed.
+        bool any_large = false;
+        if constexpr (MODE != kFinal)
+            for (int o = 0; o < a.nd; ++o) any_large |= !a.sd[o].small;
+        double x[CT], mean[CT], u[CT];
+        unsigned long long bw[CT];  // rows of each column's slot
+        {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) x[c] = v[c];
+            if (a.elim || any_large) seg_scan_cols<CT>(x, ScanFactors(ScanMask(lane, ss)));
             const unsigned long long segmask = mask_le(se) & ~((1ull << ss) - 1ull);
-            double inv = 0.0;
+            double inv = 0.0, inslot = 0.0;
             uint32_t cnt_bit = 0u;
+            unsigned long long B = 0ull;
 #pragma unroll
             for (int c = 0; c < CT; ++c) {
+                mean[c] = 0.0;
+                bw[c] = 0ull;
+                u[c] = 0.0;
                 if (cb[c] == 0u) continue;  // uniform across the wave
-                if (cb[c] != cnt_bit) {     // columns of one slot share the row count
-                    const unsigned long long B = __ballot((word & cb[c]) != 0);
+                if (cb[c] != cnt_bit) {     // columns of one slot share the row counts
+                    B = __ballot((word & cb[c]) != 0);
                     const int cnt = __popcll(B & segmask);
                     inv = cnt > 0 ? 1.0 / (double)cnt : 0.0;
+                    inslot = (word & cb[c]) ? 1.0 : 0.0;
                     cnt_bit = cb[c];
                 }
-                const double x = seg_scan(v[c], sm);
-                const double tot = __shfl(x, se, kWave);
-                v[c] = (word & cb[c]) ? v[c] - tot * inv : 0.0;
+                bw[c] = B;
+                if (a.elim) mean[c] = __shfl(x[c], se, kWave) * inv;
+                u[c] = fma(-mean[c], inslot, v[c]);  // v is 0 outside the slot
             }
         }
         if constexpr (MODE == kFinal) {
@@ -235,41 +320,54 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
                 for (int c = 0; c < CT; ++c) {
                     if (c >= a.ncols) break;
                     if (a.oc32[c])
-                        a.oc32[c][r] = (float)v[c];
+                        a.oc32[c][r] = (float)u[c];
                     else if (a.to_work)
-                        a.oc[c][r] = v[c];
+                        a.oc[c][r] = u[c];
                     else if (word & cb[c])
-                        a.oc[c][src] = v[c];
+                        a.oc[c][src] = u[c];
                 }
             }
         } else {
-            for (int o = 0; o < a.nd; ++o) {
+#pragma unroll
+            for (int o = 0; o < kMaxScatter; ++o) {
+                if (o >= a.nd) break;
                 const ScatterDim& d = a.sd[o];
-                const int g = valid ? d.ids[r] : -1;
+                const int g = gid[o];
                 if (d.small) {
                     if (on) {
                         double* tq = sacc + (size_t)(d.lds_off + g) * CT;
 #pragma unroll
                         for (int c = 0; c < CT; ++c)
-                            if (word & cb[c]) lds_add(&tq[c], v[c]);
+                            if (word & cb[c]) lds_add(&tq[c], u[c]);
                     }
                     continue;
                 }
-                // runs: a new run starts at a segment head or where the id changes
+                // runs: a new run starts at a segment head or where the id changes; the run sum of u is
+                // x(run end) - x(run start - 1) - (rows of the slot in the run) * mean
                 const int gp = __shfl_up(g, 1, kWave);
                 const bool rh = valid && (hs || g != gp);
                 const unsigned long long HR = __ballot(rh);
                 const unsigned long long le = HR & mask_le(lane);
                 const int rs = 63 - __clzll(le);
                 const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
-                const int64_t slot = d.seg_run_off[s0] + (__popcll(le) - 1);
-                const ScanMask rm(lane, rs);
-                double* dst = d.runbuf + slot * CT;
+                const int64_t slot = d.seg_run_off[s0c] + (__popcll(le) - 1);
+                const bool rfirst = rs == ss;
+                const int pl = rs > 0 ? rs - 1 : 0;
+                const unsigned long long runmask = mask_le(lane) & ~((1ull << rs) - 1ull);
+                const double keep_pre = rfirst ? 0.0 : 1.0;
+                double* dst = d.runbuf + (int64_t)(rend ? d.run_dest[slot] : 0) * CT;
+                double rcnt = 0.0;
+                uint32_t rc_bit = 0u;
 #pragma unroll
                 for (int c = 0; c < CT; ++c) {
                     if (cb[c] == 0u) continue;
-                    const double x = seg_scan(v[c], rm);
-                    if (rend) dst[c] = x;
+                    if (cb[c] != rc_bit) {  // rows of the slot in the run, shared by the slot's columns
+                        rcnt = (double)__popcll(bw[c] & runmask);
+                        rc_bit = cb[c];
+                    }
+                    const double pre = __shfl(x[c], pl, kWave);
+                    const double ru = fma(-rcnt, mean[c], fma(-pre, keep_pre, x[c]));
+                    if (rend) dst[c] = ru;
                 }
             }
         }
@@ -386,12 +484,12 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
                 }
                 const bool rhead = valid && (j == 0 || segstart || g != gp);
                 const int total = tile_runs(valid, rhead, j, nr, wcnt, rstart);
-                double* dst = d.runbuf + d.seg_run_off[s0] * CT;
+                const int64_t rbase = d.seg_run_off[s0];
                 for (int e = j; e < total * CT; e += blockDim.x) {
                     const int k = e / CT, c = e - k * CT;
                     double sm = 0.0;
                     for (int q = rstart[k]; q < rstart[k + 1]; ++q) sm += sv[q * CT + c];
-                    dst[e] = sm;
+                    d.runbuf[(int64_t)d.run_dest[rbase + k] * CT + c] = sm;
                 }
                 __syncthreads();
             }
@@ -469,12 +567,12 @@ __global__ __launch_bounds__(kBlock) void tile_diag_kernel(ApplyArgs a) {
         const int gp = (valid && j > 0) ? gd[j - 1] : -1;
         const bool rhead = valid && (j == 0 || rs[j] == j || gd[j] != gp);
         const int total = tile_runs(valid, rhead, j, nr, wcnt, rstart);
-        double* dst = d.runbuf + d.seg_run_off[s0] * CT;
+        const int64_t rbase = d.seg_run_off[s0];
         for (int e = j; e < total * CT; e += blockDim.x) {
             const int k = e / CT, c = e - k * CT;
             double sm = 0.0;
             for (int q = rstart[k]; q < rstart[k + 1]; ++q) sm += sv[q * CT + c];
-            dst[e] = sm;
+            d.runbuf[(int64_t)d.run_dest[rbase + k] * CT + c] = sm;
         }
     }
     __syncthreads();
@@ -596,7 +694,7 @@ __global__ __launch_bounds__(kBlock) void long_kernel(ApplyArgs a) {
                             if (k != d.large_slot) continue;
                             if (g != cur[k]) {
                                 if (cur[k] >= 0) {
-                                    double* dst = d.runbuf + rk[k] * CT;
+                                    double* dst = d.runbuf + (int64_t)d.run_dest[rk[k]] * CT;
 #pragma unroll
                                     for (int c = 0; c < CT; ++c) dst[c] = racc[k][c];
                                     ++rk[k];
@@ -618,7 +716,7 @@ __global__ __launch_bounds__(kBlock) void long_kernel(ApplyArgs a) {
 #pragma unroll
                 for (int k = 0; k < kMaxLarge; ++k) {
                     if (k != d.large_slot || cur[k] < 0) continue;
-                    double* dst = d.runbuf + rk[k] * CT;
+                    double* dst = d.runbuf + (int64_t)d.run_dest[rk[k]] * CT;
 #pragma unroll
                     for (int c = 0; c < CT; ++c) dst[c] = racc[k][c];
                 }
@@ -635,15 +733,15 @@ __global__ __launch_bounds__(kBlock) void long_kernel(ApplyArgs a) {
 // ------------------------------------------------------------------------------------------------
 // q[g][c] = sum of the run partials of group g (runs in group order)
 template <int CT>
-__global__ void reduce_runs_kernel(const double* __restrict__ runbuf, const int* __restrict__ order,
-                                   const int64_t* __restrict__ goff, int G, double* q, const int* __restrict__ active) {
+__global__ void reduce_runs_kernel(const double* __restrict__ runbuf, const int64_t* __restrict__ goff, int G,
+                                   double* q, const int* __restrict__ active) {
     if (!any_active<CT>(active)) return;
     const int64_t total = (int64_t)G * CT;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
         const int g = (int)(e / CT), c = (int)(e % CT);
         double s = 0.0;
-        for (int64_t k = goff[g]; k < goff[g + 1]; ++k) s += runbuf[(int64_t)order[k] * CT + c];
+        for (int64_t k = goff[g]; k < goff[g + 1]; ++k) s += runbuf[k * CT + c];
         q[e] = s;
     }
 }
@@ -905,6 +1003,7 @@ int make_layout(Panel* p, unsigned fe_mask, Layout& L) {
         s.tab_off = L.S;
         s.runbuf = nullptr;
         s.seg_run_off = nullptr;
+        s.run_dest = nullptr;
         L.S += s.G;
         if (s.small) {
             s.lds_off = L.Gs;
@@ -914,6 +1013,7 @@ int make_layout(Panel* p, unsigned fe_mask, Layout& L) {
             if (large == kMaxLarge) return set_error("more than %d large non-primary FE dims", kMaxLarge);
             s.large_slot = large++;
             s.seg_run_off = p->d[d].seg_run_off;
+            s.run_dest = p->d[d].run_dest;
             s.lds_off = 0;
         }
         L.dims[L.nd++] = d;
@@ -969,11 +1069,9 @@ void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, 
             const DimStruct& D = p->d[L.dims[o]];
             if (time_runs) kstat_begin(kStatReduceRuns, s);
             hipLaunchKernelGGL((reduce_runs_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock)), dim3(kBlock), 0,
-                               s, (const double*)a.sd[o].runbuf, (const int*)D.run_order,
-                               (const int64_t*)D.grp_run_off, d.G, qo, active);
+                               s, (const double*)a.sd[o].runbuf, (const int64_t*)D.grp_run_off, d.G, qo, active);
             if (time_runs)
-                kstat_end(kStatReduceRuns, s,
-                          8.0 * CT * (double)(D.n_runs + d.G) + 4.0 * (double)D.n_runs + 8.0 * (double)(d.G + 1));
+                kstat_end(kStatReduceRuns, s, 8.0 * CT * (double)(D.n_runs + d.G) + 8.0 * (double)(d.G + 1));
         }
     }
 }
@@ -1016,7 +1114,7 @@ int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* 
         double bytes = n * 4.0 * (L.nd + 1) + (a.wm ? 4.0 * n : 0.0) + 8.0 * (double)grid * L.Gs * CT;
         for (int o = 0; o < L.nd; ++o) {
             bytes += 8.0 * CT * (double)L.sd[o].G;
-            if (!L.sd[o].small) bytes += 8.0 * CT * (double)p->d[L.dims[o]].n_runs;
+            if (!L.sd[o].small) bytes += (8.0 * CT + 4.0) * (double)p->d[L.dims[o]].n_runs;
         }
         if (mode == kData) bytes += n * 8.0 * a.ncols + (p->perm ? 4.0 * n : 0.0);
         kstat_end(sid, s, bytes);
@@ -1066,6 +1164,7 @@ ApplyArgs base_args(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm) 
     a.n_tiles = p->n_tiles;
     a.idP = p->d[p->P].ids;
     a.wtile_seg = p->wtile_seg;
+    a.wtile_rec = p->wtile_rec;
     a.n_wtiles = p->n_wtiles;
     a.long_segs = p->long_segs;
     a.n_long = p->n_long;
@@ -1254,6 +1353,7 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
     int e = make_layout(p, mask, L);
     if (e) return e;
     ws = WithinStats(ncols);
+    if (o.method != 0) return set_error("within transform: method %d is not available (0: PCG in FE space)", o.method);
     if (ncols <= 0) return 0;
     if (!wm) nslots = 1;
     if (nslots < 1 || nslots > 32) return set_error("1..32 request slots per solve (%d)", nslots);

@@ -49,9 +49,11 @@ int fehip_panel_get_info(const fehip_panel* panel, fehip_panel_info* info);
 typedef struct {
     double tol;           /* stop when max |group-mean update| < tol (the reference's criterion) */
     int32_t max_iter;     /* iteration cap */
-    int32_t method;       /* 0: PCG in FE space (default), 1: alternating projections in row space */
+    int32_t method;       /* 0: PCG in FE space (the only method of the coarse path; the reference's
+                             relaxed AP in row space runs through the fine-grained ABI) */
     uint32_t fe_mask;     /* bit d set => FE dim d is absorbed; 0 means all dims */
-    int32_t precision;    /* 0: FP64, 1: FP32 tables + Kahan sums with FP64 refinement */
+    int32_t precision;    /* 0: FP64; 1 (fehip_estimate): within-transformed columns stored in FP32,
+                             beta refined in FP64 on the residual (2SLS requests stay FP64) */
     int32_t check_every;  /* convergence read-back period in iterations (0 -> 4) */
     int32_t reserved;
 } fehip_demean_opts;
@@ -172,6 +174,10 @@ int fehip_kstats_enable(int on);
 int fehip_kstats_reset(void);
 int fehip_kstats_get(int id, char* name, int name_len, int64_t* launches, double* total_ms, double* bytes,
                      double* flops);
+
+/* Plain device-to-device streaming copy of bytes (a multiple of 16) by a kernel: the reference
+ * workload for calibrating hardware byte counters (reads and writes exactly bytes each). */
+int fehip_device_copy(const void* d_src, void* d_dst, int64_t bytes);
 
 #ifdef __cplusplus
 }
