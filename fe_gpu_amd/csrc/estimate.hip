@@ -61,6 +61,31 @@ __global__ __launch_bounds__(kBlock) void wsum2_kernel(const double* __restrict_
     }
 }
 
+// r = y~ - W~ beta on the work buffer, r'r partials per block, and (T != nullptr) the score rows
+// T = r W~ (row-major n x K) in the same pass over the columns.
+__global__ __launch_bounds__(256) void resid_scores_kernel(const double* __restrict__ work, int64_t n, int K,
+                                                           const double* __restrict__ beta, double* r, double* T,
+                                                           double* part) {
+    double ss = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        double v = work[i];
+        for (int k = 0; k < K; ++k) v -= work[i + (int64_t)(1 + k) * n] * beta[k];
+        r[i] = v;
+        ss += v * v;
+        if (T)
+            for (int k = 0; k < K; ++k) T[i * K + k] = v * work[i + (int64_t)(1 + k) * n];
+    }
+    __shared__ double ws[256 / 64];
+    ss = wave_sum(ss);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x / 64] = ss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 256 / 64; ++w) t += ws[w];
+        part[blockIdx.x] = t;
+    }
+}
+
 __global__ void residual_work_kernel(const double* __restrict__ work, int64_t n, int K,
                                      const double* __restrict__ beta, double* r) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -86,14 +111,13 @@ __global__ __launch_bounds__(kBlock) void dot_part_kernel(const double* __restri
 }
 
 // T[i][j] = r_i X[i][j] (row-major n x K) from column-major X (ld)
+// T = r x, one row per thread (the columns read coalesced, each row written as K contiguous values)
 template <typename TX>
 __global__ void scores_rows_kernel(const double* __restrict__ r, const TX* __restrict__ X, int64_t ldX, int64_t n,
                                    int K, double* T) {
-    const int64_t total = n * K;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = e / K;
-        const int j = (int)(e - i * K);
-        T[e] = r[i] * (double)X[i + j * ldX];
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double ri = r[i];
+        for (int k = 0; k < K; ++k) T[i * K + k] = ri * (double)X[i + (int64_t)k * ldX];
     }
 }
 
@@ -303,6 +327,36 @@ struct WaveNest {
     int mode;                   // 2: singletons, 1: clusters are single runs, 0: merge runs of a segment
 };
 
+// Row data of one wave tile of the nested-subset kernel, loaded one tile ahead of its use.
+template <int KT>
+struct NestRows {
+    int64_t rec;
+    int ip, k0, k1;
+    bool w;
+    double t[KT];  // scores r x
+};
+
+template <int KT, typename TX>
+__device__ __forceinline__ void load_nest_rows(const WaveNest& a, const TX* __restrict__ X, int64_t t, int lane,
+                                               NestRows<KT>& q) {
+    if (t >= a.n_wtiles) {
+        q.rec = 0;
+        return;
+    }
+    q.rec = a.wtile_rec[t];
+    const int64_t R0 = q.rec & 0xffffffffll;
+    const int nr = (int)(q.rec >> 32);
+    const bool valid = lane < nr;
+    const int64_t ri = R0 + (valid ? lane : 0);
+    q.ip = valid ? a.idP[ri] : -1;
+    q.k0 = (valid && a.nk > 0) ? a.kid[0][ri] : -1;
+    q.k1 = (valid && a.nk > 1) ? a.kid[1][ri] : -1;
+    q.w = valid && (!a.w || a.w[ri]);
+    const double rr = valid ? a.r[ri] : 0.0;
+#pragma unroll
+    for (int k = 0; k < KT; ++k) q.t[k] = (k < a.K && valid) ? rr * (double)X[ri + (int64_t)k * a.ldX] : 0.0;
+}
+
 template <int KT, typename TX>
 __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, const TX* __restrict__ X, double* part,
                                                                   unsigned long long* ucount) {
@@ -313,20 +367,21 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
     const int lane = threadIdx.x & (kWave - 1);
     unsigned long long nclusters = 0;
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    // software pipeline: the next tile's rows are loaded while the current tile is reduced
     int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    int64_t rec = t < a.n_wtiles ? a.wtile_rec[t] : 0;
+    NestRows<KT> cur, nxt;
+    load_nest_rows<KT, TX>(a, X, t, lane, cur);
     for (; t < a.n_wtiles; t += wstride) {
-        const int64_t tn = t + wstride;
-        const int64_t rec_n = tn < a.n_wtiles ? a.wtile_rec[tn] : 0;  // next tile, fetched early
-        const int64_t R0 = rec & 0xffffffffll;
-        const int nr = (int)(rec >> 32);
-        rec = rec_n;
+        load_nest_rows<KT, TX>(a, X, t + wstride, lane, nxt);
+        const int nr = (int)(cur.rec >> 32);
+        const int ip = cur.ip, k0 = cur.k0, k1 = cur.k1;
+        const bool wrow = cur.w;
+        double sv[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) sv[k] = cur.t[k];
+        cur = nxt;
         if (nr <= 0) continue;
         const bool valid = lane < nr;
-        const int64_t ri = R0 + (valid ? lane : 0);
-        const int ip = valid ? a.idP[ri] : -1;
-        const int k0 = (valid && a.nk > 0) ? a.kid[0][ri] : -1;
-        const int k1 = (valid && a.nk > 1) ? a.kid[1][ri] : -1;
         const int ipp = __shfl_up(ip, 1, kWave), k0p = __shfl_up(k0, 1, kWave), k1p = __shfl_up(k1, 1, kWave);
         const bool hs = valid && (lane == 0 || ip != ipp);
         const unsigned long long H = __ballot(hs);
@@ -336,17 +391,9 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
         const unsigned long long le = HR & mask_le(lane);
         const int rs = 63 - __clzll(le);
         const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
-        const bool wrow = valid && (!a.w || a.w[ri]);
         const unsigned long long BW = __ballot(wrow);
         bool ne = (BW & mask_le(lane) & ~((1ull << rs) - 1ull)) != 0;  // run holds a weighted row
-        const ScanMask rm(lane, rs);
-        const double rr = valid ? a.r[ri] : 0.0;
-        double sv[KT];
-#pragma unroll
-        for (int k = 0; k < KT; ++k) {
-            const double tv = (k < a.K && valid) ? rr * (double)X[ri + (int64_t)k * a.ldX] : 0.0;
-            sv[k] = a.mode == 2 ? tv : seg_scan(tv, rm);
-        }
+        if (a.mode != 2) seg_scan_cols<KT>(sv, ScanFactors(ScanMask(lane, rs)));
         bool leader = rend;
         if (a.nk > 0 && a.mode == 0) {
             // merge the runs of one segment that carry the same ids
@@ -396,43 +443,84 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
     }
 }
 
-// Any subset through its cached sorted partition: one thread per cluster sums the score rows of its
-// members (order[off[c] .. off[c+1])) and accumulates S S' in registers; block partials.
+// Any subset through its cached sorted partition, one wave per 64 consecutive sorted positions: the
+// score rows of the positions are summed per cluster by a segmented DPP scan; a cluster that starts
+// and ends inside the wave adds S S' to its end lane's registers.  A cluster cut by a wave boundary
+// leaves its pieces in two boundary slots per wave (slot 0: the piece before the wave's first cluster
+// head, or the whole wave when it has no head; slot 1: the last cluster when it runs on into the next
+// wave), keyed by cluster id; the pieces are merged by a reduce-by-key afterwards.  An empty slot
+// has zero sums and key -1, except that slot 1 of a headless wave repeats slot 0's key so that a
+// cluster's pieces stay adjacent.
 template <int KT>
-__global__ __launch_bounds__(kBlock) void sorted_meat_kernel(const double* __restrict__ T, int K,
-                                                             const int* __restrict__ order,
-                                                             const int64_t* __restrict__ off, int64_t nS, double* part) {
+__global__ __launch_bounds__(kBlock) void sorted_wave_meat_kernel(const double* __restrict__ T, int K,
+                                                                  const int* __restrict__ order,
+                                                                  const int* __restrict__ keys, int64_t n,
+                                                                  double* part, int* bkey, double* bval) {
     constexpr int NP = KT * (KT + 1) / 2;
     double acc[NP];
 #pragma unroll
     for (int e = 0; e < NP; ++e) acc[e] = 0.0;
-    for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c < nS; c += (int64_t)gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t nwaves = (n + kWave - 1) / kWave;
+    const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t w = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; w < nwaves; w += wstride) {
+        const int64_t q0 = w * kWave;
+        const int nvalid = (int)(n - q0 < kWave ? n - q0 : kWave);
+        const bool valid = lane < nvalid;
+        const int64_t q = q0 + (valid ? lane : 0);
+        const int key = valid ? keys[q] : -1;
+        const int64_t row = order ? (int64_t)order[q] : q;
         double sv[KT];
 #pragma unroll
-        for (int k = 0; k < KT; ++k) sv[k] = 0.0;
-        for (int64_t q = off[c]; q < off[c + 1]; ++q) {
-            const double* row = T + (order ? (int64_t)order[q] : q) * K;
+        for (int k = 0; k < KT; ++k) sv[k] = (valid && k < K) ? T[row * K + k] : 0.0;
+        int kprev = __shfl_up(key, 1, kWave);
+        if (lane == 0) kprev = q0 > 0 ? keys[q0 - 1] : -2;
+        const bool head = valid && key != kprev;
+        const unsigned long long H = __ballot(head);
+        const int last = nvalid - 1;
+        // does the cluster at the last position continue into the next wave?
+        const int klast = __shfl(key, last, kWave);
+        const bool cont = (q0 + nvalid < n) && keys[q0 + nvalid] == klast;
+        const unsigned long long le = H & mask_le(lane);
+        const int ss = le ? 63 - __clzll(le) : 0;
+        seg_scan_cols<KT>(sv, ScanFactors(ScanMask(lane, ss)));
+        const bool nexthead = lane < last ? ((H >> (lane + 1)) & 1ull) != 0 : !cont;
+        const bool cend = valid && (lane <= last) && nexthead;
+        const bool complete = cend && le != 0;  // started at a head of this wave
+        if (complete) {
+            int idx = 0;
 #pragma unroll
-            for (int k = 0; k < KT; ++k)
-                if (k < K) sv[k] += row[k];
+            for (int i = 0; i < KT; ++i)
+#pragma unroll
+                for (int j = i; j < KT; ++j) acc[idx++] += sv[i] * sv[j];
         }
-        int idx = 0;
+        // boundary pieces
+        const bool open_left = (H & 1ull) == 0ull;
+        const int e0 = H ? __ffsll((long long)H) - 2 : last;  // last lane of the leading piece
+        const bool open_right = cont && H != 0ull;             // trailing cluster started in this wave
+        const int64_t b0 = 2 * w, b1 = 2 * w + 1;
+        if (lane == (open_left ? e0 : 0)) {
+            bkey[b0] = open_left ? key : -1;
 #pragma unroll
-        for (int i = 0; i < KT; ++i)
+            for (int k = 0; k < KT; ++k) bval[b0 * KT + k] = open_left ? sv[k] : 0.0;
+        }
+        if (lane == (open_right ? last : 0)) {
+            bkey[b1] = open_right ? key : (H == 0ull ? klast : -1);
 #pragma unroll
-            for (int j = i; j < KT; ++j) acc[idx++] += sv[i] * sv[j];
+            for (int k = 0; k < KT; ++k) bval[b1 * KT + k] = open_right ? sv[k] : 0.0;
+        }
     }
     __shared__ double ws[kBlock / kWave][NP];
-    const int wv = threadIdx.x / kWave, ln = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
 #pragma unroll
     for (int e = 0; e < NP; ++e) {
         const double v = wave_sum(acc[e]);
-        if (ln == 0) ws[wv][e] = v;
+        if (lane == 0) ws[wv][e] = v;
     }
     __syncthreads();
     for (int e = threadIdx.x; e < NP; e += blockDim.x) {
         double v = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) v += ws[w][e];
+        for (int ww = 0; ww < kBlock / kWave; ++ww) v += ws[ww][e];
         part[(int64_t)blockIdx.x * NP + e] = v;
     }
 }
@@ -1137,11 +1225,32 @@ int finish_request(Panel* p, ReqState& s) {
     // ---- residual on W~ (also for IV), rss, dof
     double *dbeta = nullptr, *r = nullptr;
     if ((e = db.get(&dbeta, (size_t)K)) || (e = db.get(&r, (size_t)n))) return e;
+    // score rows T = r W~ are formed in the residual pass when an OLS sandwich has a subset whose method
+    // reads them (every subset but those containing the primary on wave tiles)
+    double* T_early = nullptr;
+    bool rss_done = false;
+    double rss = 0.0;
     if (!f32) {
+        bool any_T = false;
+        if (ncd > 0 && !use_iv)
+            for (int mask = 1; mask < (1 << ncd); ++mask) {
+                unsigned dm = 0;
+                for (int j = 0; j < ncd; ++j)
+                    if (mask & (1 << j)) dm |= 1u << (cdims[j] - 1);
+                int nother = 0;
+                for (int d = 0; d < p->m; ++d) nother += (d != p->P && ((dm >> d) & 1u)) ? 1 : 0;
+                const bool nested = ((dm >> p->P) & 1u) && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2;
+                any_T |= !nested;
+            }
+        if (any_T && (e = db.get(&T_early, (size_t)n * K))) return e;
         FEHIP_CHECK(hipMemcpy(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
-        hipLaunchKernelGGL(residual_work_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, work, n, K, dbeta,
-                           r);
+        hipLaunchKernelGGL(resid_scores_kernel, dim3(kRedBlocks), dim3(256), 0, 0, (const double*)work, n, K,
+                           (const double*)dbeta, r, T_early, part);
         FEHIP_LAUNCH_CHECK("residual");
+        std::vector<double> ps;
+        if ((e = host_sum_parts(part, kRedBlocks, 1, ps))) return e;
+        rss = ps[0];
+        rss_done = true;
     } else {
         // FP64 refinement on the residual: e = M_F(y - W beta) in FP64 from the caller's FP64 columns and
         // the same combination of the solved tables (M_F is linear, so no further solve), then
@@ -1193,8 +1302,7 @@ int finish_request(Panel* p, ReqState& s) {
                            (const double*)et, (const float*)work32, n, K, (const double*)dbeta, r);
         FEHIP_LAUNCH_CHECK("refined residual");
     }
-    double rss = 0.0;
-    if ((e = sumsq(r, n, part, &rss))) return e;
+    if (!rss_done && (e = sumsq(r, n, part, &rss))) return e;
     result->rss = rss;
     int64_t df = std::max<int64_t>(1, N - (int64_t)(kept + dof_fes + 1));
     result->dof_resid = df;
@@ -1218,17 +1326,17 @@ int finish_request(Panel* p, ReqState& s) {
             X = Xhat;
         }
         // scores T = r x (row-major n x K), formed only if a subset's method reads it
-        double* T = nullptr;
+        double* T = T_early;
         auto need_T = [&]() -> int {
             if (T) return 0;
             int e2 = db.get(&T, (size_t)n * K);
             if (e2) return e2;
             if (X)
-                hipLaunchKernelGGL(scores_rows_kernel<double>, dim3(stream_grid((size_t)n * K, kBlock)), dim3(kBlock),
-                                   0, 0, (const double*)r, X, ldX, n, K, T);
+                hipLaunchKernelGGL(scores_rows_kernel<double>, dim3(stream_grid((size_t)n, kBlock)), dim3(kBlock), 0,
+                                   0, (const double*)r, X, ldX, n, K, T);
             else
-                hipLaunchKernelGGL(scores_rows_kernel<float>, dim3(stream_grid((size_t)n * K, kBlock)), dim3(kBlock),
-                                   0, 0, (const double*)r, (const float*)(work32 + n), ldX, n, K, T);
+                hipLaunchKernelGGL(scores_rows_kernel<float>, dim3(stream_grid((size_t)n, kBlock)), dim3(kBlock), 0, 0,
+                                   (const double*)r, (const float*)(work32 + n), ldX, n, K, T);
             FEHIP_LAUNCH_CHECK("scores rows");
             return 0;
         };
@@ -1241,6 +1349,9 @@ int finish_request(Panel* p, ReqState& s) {
         if ((e = db.get(&Md, (size_t)K * K))) return e;
         std::vector<double> M((size_t)K * K), tmp((size_t)kept * kept);
         int64_t min_single = 0x7fffffff;
+        std::vector<double> M_single;  // meat of a subset whose clusters are single rows
+        int64_t ncl_single = 0;
+        bool single_done = false;
         for (int mask = 1; mask < (1 << ncd); ++mask) {
             unsigned dims_mask = 0;
             int len = 0;
@@ -1264,6 +1375,7 @@ int finish_request(Panel* p, ReqState& s) {
             int nother = 0;
             for (int d = 0; d < p->m; ++d) nother += (d != p->P && ((dims_mask >> d) & 1u)) ? 1 : 0;
             if (has_p && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2) {
+                // how the subset's clusters lie in the segments (cached per panel)
                 WaveNest wn{};
                 wn.r = r;
                 wn.ldX = n;
@@ -1276,6 +1388,24 @@ int finish_request(Panel* p, ReqState& s) {
                     if (d != p->P && ((dims_mask >> d) & 1u)) wn.kid[wn.nk++] = p->d[d].ids;
                 wn.w = wpanel;
                 if ((e = nest_mode(p, dims_mask, wn.kid, wn.nk, ucount, &wn.mode))) return e;
+            }
+            if (has_p && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2 && single_done &&
+                p->nest_modes[dims_mask] == 2) {
+                M = M_single;  // every row its own cluster: the same meat as an earlier subset
+                ncl = ncl_single;
+            } else if (has_p && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2) {
+                WaveNest wn{};
+                wn.r = r;
+                wn.ldX = n;
+                wn.K = K;
+                wn.idP = p->d[p->P].ids;
+                wn.seg_off = p->seg_off;
+                wn.wtile_rec = p->wtile_rec;
+                wn.n_wtiles = p->n_wtiles;
+                for (int d = 0; d < p->m; ++d)
+                    if (d != p->P && ((dims_mask >> d) & 1u)) wn.kid[wn.nk++] = p->d[d].ids;
+                wn.w = wpanel;
+                wn.mode = p->nest_modes[dims_mask];
                 FEHIP_CHECK(hipMemsetAsync(ucount, 0, sizeof(unsigned long long), 0), "memset");
                 const unsigned grid = (unsigned)std::max<int64_t>(
                     1, std::min<int64_t>((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave), kMeatBlocks));
@@ -1300,6 +1430,11 @@ int finish_request(Panel* p, ReqState& s) {
                 unsigned long long h = 0;
                 FEHIP_CHECK(hipMemcpy(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
                 ncl = (int64_t)h;
+                if (wn.mode == 2) {
+                    M_single = M;
+                    ncl_single = ncl;
+                    single_done = true;
+                }
             } else if (has_p && p->n_long == 0 && K <= 12) {
                 if ((e = need_T())) return e;
                 NestKeys nk{};
@@ -1402,19 +1537,39 @@ int finish_request(Panel* p, ReqState& s) {
                     if ((e = meat_fused(T, K, K, nullptr, nullptr, n, mpart, M))) return e;
                 } else if (K <= 12) {
                     const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
+                    const int64_t nwaves = (n + kWave - 1) / kWave;
+                    int* bkey = nullptr;
+                    double *bval = nullptr, *Sb = nullptr;
+                    if ((e = db.get(&bkey, (size_t)(2 * nwaves))) || (e = db.get(&bval, (size_t)(2 * nwaves * KT))) ||
+                        (e = db.get(&Sb, (size_t)(2 * nwaves * KT))))
+                        return e;
                     const unsigned grid = (unsigned)std::max<int64_t>(
-                        1, std::min<int64_t>((nS + kBlock - 1) / kBlock, (int64_t)kMeatBlocks));
+                        1, std::min<int64_t>((nwaves + kBlock / kWave - 1) / (kBlock / kWave), (int64_t)kMeatBlocks));
                     if (KT == 4)
-                        hipLaunchKernelGGL(sorted_meat_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, (const double*)T, K,
-                                           (const int*)ord, (const int64_t*)off, nS, mpart);
+                        hipLaunchKernelGGL(sorted_wave_meat_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, (const double*)T,
+                                           K, (const int*)part_s->order, (const int*)part_s->keys, n, mpart, bkey, bval);
                     else if (KT == 8)
-                        hipLaunchKernelGGL(sorted_meat_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, (const double*)T, K,
-                                           (const int*)ord, (const int64_t*)off, nS, mpart);
+                        hipLaunchKernelGGL(sorted_wave_meat_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, (const double*)T,
+                                           K, (const int*)part_s->order, (const int*)part_s->keys, n, mpart, bkey, bval);
                     else
-                        hipLaunchKernelGGL(sorted_meat_kernel<12>, dim3(grid), dim3(kBlock), 0, 0, (const double*)T, K,
-                                           (const int*)ord, (const int64_t*)off, nS, mpart);
-                    FEHIP_LAUNCH_CHECK("sorted meat");
+                        hipLaunchKernelGGL(sorted_wave_meat_kernel<12>, dim3(grid), dim3(kBlock), 0, 0,
+                                           (const double*)T, K, (const int*)part_s->order, (const int*)part_s->keys, n,
+                                           mpart, bkey, bval);
+                    FEHIP_LAUNCH_CHECK("sorted wave meat");
                     if ((e = reduce_upper(mpart, (int)grid, KT, K, M))) return e;
+                    // clusters cut by wave boundaries: merge their pieces, then add their S S'
+                    if (KT == 4)
+                        e = cluster_sums<4>(bval, KT, nullptr, bkey, 2 * nwaves, Sb, dcount, 0);
+                    else if (KT == 8)
+                        e = cluster_sums<8>(bval, KT, nullptr, bkey, 2 * nwaves, Sb, dcount, 0);
+                    else
+                        e = cluster_sums<12>(bval, KT, nullptr, bkey, 2 * nwaves, Sb, dcount, 0);
+                    if (e) return e;
+                    int64_t n_groups = 0;
+                    FEHIP_CHECK(hipMemcpy(&n_groups, dcount, sizeof(int64_t), hipMemcpyDeviceToHost), "count copy");
+                    std::vector<double> Mb;
+                    if ((e = meat_fused(Sb, K, KT, nullptr, nullptr, n_groups, mpart, Mb))) return e;
+                    for (size_t i2 = 0; i2 < M.size(); ++i2) M[i2] += Mb[i2];
                 } else {
                     if (nS * K > S_cap) {
                         if (S) db.release(S);

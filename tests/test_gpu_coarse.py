@@ -200,3 +200,17 @@ def test_synth_generator_is_deterministic_and_shaped():
     switch = np.mean(firm[:, 1:] != firm[:, :-1])
     assert 0.09 < switch < 0.11
     assert abs(W[:, 1].mean() - 0.05) < 0.005                 # sick_shock ~ Bernoulli(0.05)
+
+
+@pytest.mark.parametrize("K,cluster", [(3, (2, 3)), (12, (3,))])
+def test_sorted_subsets_with_clusters_across_waves(K, cluster):
+    """Cluster subsets routed through the sorted partition whose clusters hold hundreds of rows, so
+    they cross many 64-position waves (pieces merged at the boundaries)."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    note(f"sorted subsets K={K} cluster={cluster}")
+    p = synth.random_fe_problem(77 + K, n=40000, K=K, G=(4000, 40, 30))
+    fit = api.estimate(p.y, p.W, p.fe, cluster=cluster, tol=1e-13, max_iter=20000)
+    ref = oracle.estimate(p.y, p.W, p.fe, cluster=cluster, tol=1e-13, max_iter=200000)
+    assert fit.res["n_clusters"] == ref.res["n_clusters"]
+    assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
