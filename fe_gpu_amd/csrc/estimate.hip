@@ -62,18 +62,38 @@ __global__ __launch_bounds__(kBlock) void wsum2_kernel(const double* __restrict_
 }
 
 // r = y~ - W~ beta on the work buffer, r'r partials per block, and (T != nullptr) the score rows
-// T = r W~ (row-major n x K) in the same pass over the columns.
+// T = r W~ (row-major n x K) in the same pass over the columns.  With K <= 16 the block's score rows
+// are staged in LDS and leave as one contiguous stretch.
 __global__ __launch_bounds__(256) void resid_scores_kernel(const double* __restrict__ work, int64_t n, int K,
                                                            const double* __restrict__ beta, double* r, double* T,
                                                            double* part) {
+    extern __shared__ double tstage[];  // [256][K] when staging
+    const bool stage = T && K <= 16;
     double ss = 0.0;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double v = work[i];
-        for (int k = 0; k < K; ++k) v -= work[i + (int64_t)(1 + k) * n] * beta[k];
-        r[i] = v;
-        ss += v * v;
-        if (T)
-            for (int k = 0; k < K; ++k) T[i * K + k] = v * work[i + (int64_t)(1 + k) * n];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
+        const int64_t i = i0 + threadIdx.x;
+        if (i < n) {
+            double v = work[i];
+            for (int k = 0; k < K; ++k) v -= work[i + (int64_t)(1 + k) * n] * beta[k];
+            r[i] = v;
+            ss += v * v;
+            if (T) {
+                for (int k = 0; k < K; ++k) {
+                    const double t = v * work[i + (int64_t)(1 + k) * n];
+                    if (stage)
+                        tstage[threadIdx.x * K + k] = t;
+                    else
+                        T[i * K + k] = t;
+                }
+            }
+        }
+        if (stage) {
+            __syncthreads();
+            const int64_t rows = n - i0 < (int64_t)blockDim.x ? n - i0 : (int64_t)blockDim.x;
+            for (int64_t e = threadIdx.x; e < rows * K; e += blockDim.x) T[i0 * K + e] = tstage[e];
+            __syncthreads();
+        }
     }
     __shared__ double ws[256 / 64];
     ss = wave_sum(ss);
@@ -1244,7 +1264,8 @@ int finish_request(Panel* p, ReqState& s) {
             }
         if (any_T && (e = db.get(&T_early, (size_t)n * K))) return e;
         FEHIP_CHECK(hipMemcpy(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
-        hipLaunchKernelGGL(resid_scores_kernel, dim3(kRedBlocks), dim3(256), 0, 0, (const double*)work, n, K,
+        const size_t tlds = (T_early && K <= 16) ? sizeof(double) * 256 * (size_t)K : 0;
+        hipLaunchKernelGGL(resid_scores_kernel, dim3(kRedBlocks), dim3(256), tlds, 0, (const double*)work, n, K,
                            (const double*)dbeta, r, T_early, part);
         FEHIP_LAUNCH_CHECK("residual");
         std::vector<double> ps;

@@ -202,6 +202,8 @@ void free_panel(Panel* p) {
     dfree(p->seg_off);
     dfree(p->tile_seg);
     dfree(p->long_segs);
+    for (auto& kv : p->diag_cache) dfree(kv.second);
+    p->diag_cache.clear();
     dfree(p->wtile_seg);
     dfree(p->wtile_rec);
     dfree(p->mtile_seg);

@@ -65,6 +65,8 @@ struct Panel {
     // subsets containing the primary: 2 = every row its own cluster, 1 = each cluster is one run inside
     // its segment, 0 = runs of one segment may share a cluster
     std::map<unsigned, int> nest_modes;
+    // Schur diagonal and group counts of unweighted solves, per absorbed-dims mask ([S][2])
+    std::map<unsigned, double*> diag_cache;
     int64_t bytes = 0;
 };
 

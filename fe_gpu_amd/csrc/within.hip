@@ -823,24 +823,58 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(int64_t S, const double
     block_rz_max<CT>(rz, mx, part_rz, part_max);
 }
 
+// Column sums (and maxima) of per-block partials [nblk][CT], by the whole block in a fixed order;
+// the results are in sum[c] / mx[c] (shared) for every thread afterwards.
 template <int CT>
-__global__ void cg_start_scalars(const double* part_rz, const double* part_max, int nblk, double tol, int ncols,
-                                 CgState* st) {
+__device__ __forceinline__ void block_reduce_cols(const double* __restrict__ part, const double* __restrict__ part_max,
+                                                  int nblk, double* sum, double* mx) {
+    __shared__ double ws[2][kBlock / kWave][CT];
+    double a[CT], m[CT];
+#pragma unroll
+    for (int c = 0; c < CT; ++c) a[c] = m[c] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x)
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            a[c] += part[b * CT + c];
+            if (part_max) m[c] = fmax(m[c], part_max[b * CT + c]);
+        }
+    const int wv = threadIdx.x / kWave, ln = threadIdx.x & (kWave - 1);
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        const double sa = wave_sum(a[c]), sm = wave_max(m[c]);
+        if (ln == 0) {
+            ws[0][wv][c] = sa;
+            ws[1][wv][c] = sm;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x < CT) {
+        double ta = 0.0, tm = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) {
+            ta += ws[0][w][threadIdx.x];
+            tm = fmax(tm, ws[1][w][threadIdx.x]);
+        }
+        sum[threadIdx.x] = ta;
+        mx[threadIdx.x] = tm;
+    }
+    __syncthreads();
+}
+
+template <int CT>
+__global__ __launch_bounds__(kBlock) void cg_start_scalars(const double* part_rz, const double* part_max, int nblk,
+                                                           double tol, int ncols, CgState* st) {
+    __shared__ double sum[CT], mx[CT];
+    block_reduce_cols<CT>(part_rz, part_max, nblk, sum, mx);
     const int c = threadIdx.x;
     if (c >= kMaxCT) return;
     if (c >= CT) {
         st->active[c] = 0;
         return;
     }
-    double rz = 0.0, mx = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-        rz += part_rz[b * CT + c];
-        mx = fmax(mx, part_max[b * CT + c]);
-    }
-    st->rz[c] = rz;
-    st->maxn[c] = mx;
+    st->rz[c] = sum[c];
+    st->maxn[c] = mx[c];
     st->iters[c] = 0;
-    st->active[c] = (c < ncols) && (mx >= tol) && rz > 0.0;
+    st->active[c] = (c < ncols) && (mx[c] >= tol) && sum[c] > 0.0;
     st->alpha[c] = st->beta[c] = 0.0;
 }
 
@@ -869,11 +903,12 @@ __global__ __launch_bounds__(kBlock) void cg_pq_kernel(int64_t S, const double* 
 }
 
 template <int CT>
-__global__ void cg_alpha_scalars(const double* part, int nblk, CgState* st) {
+__global__ __launch_bounds__(kBlock) void cg_alpha_scalars(const double* part, int nblk, CgState* st) {
+    __shared__ double sum[CT], mx[CT];
+    block_reduce_cols<CT>(part, nullptr, nblk, sum, mx);
     const int c = threadIdx.x;
     if (c >= CT) return;
-    double pq = 0.0;
-    for (int b = 0; b < nblk; ++b) pq += part[b * CT + c];
+    const double pq = sum[c];
     st->alpha[c] = (st->active[c] && pq > 0.0) ? st->rz[c] / pq : 0.0;
 }
 
@@ -910,18 +945,17 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t S, const CgSt
 }
 
 template <int CT>
-__global__ void cg_beta_scalars(const double* part_rz, const double* part_max, int nblk, double tol, CgState* st) {
+__global__ __launch_bounds__(kBlock) void cg_beta_scalars(const double* part_rz, const double* part_max, int nblk,
+                                                          double tol, CgState* st) {
+    __shared__ double sum[CT], mxs[CT];
+    block_reduce_cols<CT>(part_rz, part_max, nblk, sum, mxs);
     const int c = threadIdx.x;
     if (c >= CT) return;
     if (!st->active[c]) {
         st->beta[c] = 0.0;
         return;
     }
-    double rz = 0.0, mx = 0.0;
-    for (int b = 0; b < nblk; ++b) {
-        rz += part_rz[b * CT + c];
-        mx = fmax(mx, part_max[b * CT + c]);
-    }
+    const double rz = sum[c], mx = mxs[c];
     st->iters[c] += 1;
     st->maxn[c] = mx;
     st->beta[c] = st->rz[c] > 0.0 ? rz / st->rz[c] : 0.0;
@@ -1234,7 +1268,7 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
         if (e) return e;
         hipLaunchKernelGGL((cg_init_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, (const double*)r,
                            (const double*)minv, (const double*)wcnt, z, pp, xa, sc.part, sc.part + kCgBlocks * CT);
-        hipLaunchKernelGGL((cg_start_scalars<CT>), dim3(1), dim3(64), 0, s, (const double*)sc.part,
+        hipLaunchKernelGGL((cg_start_scalars<CT>), dim3(1), dim3(kBlock), 0, s, (const double*)sc.part,
                            (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, ncols, sc.st);
         FEHIP_LAUNCH_CHECK("cg init");
         const int every = o.check_every > 0 ? o.check_every : 4;
@@ -1253,12 +1287,12 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             if (e) return e;
             hipLaunchKernelGGL((cg_pq_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, (const double*)pp,
                                (const double*)q, sc.part);
-            hipLaunchKernelGGL((cg_alpha_scalars<CT>), dim3(1), dim3(64), 0, s, (const double*)sc.part, kCgBlocks,
+            hipLaunchKernelGGL((cg_alpha_scalars<CT>), dim3(1), dim3(kBlock), 0, s, (const double*)sc.part, kCgBlocks,
                                sc.st);
             hipLaunchKernelGGL((cg_update_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, (const CgState*)sc.st,
                                (const double*)pp, (const double*)q, (const double*)minv, (const double*)wcnt, xa, r, z,
                                sc.part, sc.part + kCgBlocks * CT);
-            hipLaunchKernelGGL((cg_beta_scalars<CT>), dim3(1), dim3(64), 0, s, (const double*)sc.part,
+            hipLaunchKernelGGL((cg_beta_scalars<CT>), dim3(1), dim3(kBlock), 0, s, (const double*)sc.part,
                                (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, sc.st);
             hipLaunchKernelGGL((cg_pdir_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, s, S,
                                (const CgState*)sc.st, (const double*)z, pp);
@@ -1387,13 +1421,26 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
     if ((e = sc.alloc(&sc.st, 1))) return e;
     sc.st_host = (CgState*)pinned_scratch(sizeof(CgState));
     if (!sc.st_host) return set_error("pinned host allocation failed");
-    // Schur diagonal of every slot in use
-    if (S > 0) {
+    // Schur diagonal of every slot in use (an unweighted solve's diagonal depends only on the panel and
+    // the absorbed dims, and is kept with the panel)
+    if (S > 0 && !wm) {
+        auto it = p->diag_cache.find(mask);
+        if (it != p->diag_cache.end()) {
+            sc.dg = it->second;
+        } else {
+            if ((e = slot_diagonal(p, L, sc, nullptr, ~0u, sc.dg, s))) return e;
+            double* keep = nullptr;
+            if ((e = dalloc(&keep, (size_t)S * 2, &p->bytes))) return e;
+            FEHIP_CHECK(hipMemcpyAsync(keep, sc.dg, sizeof(double) * (size_t)S * 2, hipMemcpyDeviceToDevice, s),
+                        "diagonal copy");
+            p->diag_cache[mask] = keep;
+            sc.dg = keep;
+        }
+    } else if (S > 0) {
         std::vector<int> used(nslots, 0);
         for (int c = 0; c < ncols; ++c) used[wc.slot[c]] = 1;
         for (int b = 0; b < nslots; ++b)
-            if (used[b] && (e = slot_diagonal(p, L, sc, wm, wm ? (1u << b) : ~0u, sc.dg + (int64_t)b * S * 2, s)))
-                return e;
+            if (used[b] && (e = slot_diagonal(p, L, sc, wm, 1u << b, sc.dg + (int64_t)b * S * 2, s))) return e;
     }
     for (int c0 = 0; c0 < ncols; c0 += ct_max) {
         const int nc = std::min(ct_max, ncols - c0);
