@@ -79,6 +79,51 @@ def _fit_summary(fits):
             "sum_t_regression_s": float(sum(f.res["t_regression"] for f in fits))}
 
 
+def iv_wide_leg(args, lib):
+    """Config 3: 2SLS on a wide expanded design -- 1M individuals x 20 periods (20M rows), 50k firms,
+    20 periods; K = 252 columns (xend and 7 cat8 levels endogenous, x1..x4, i.ind&&i.msp = 239
+    dummies, x1&x2), 21 instruments (z1, z2, 19 zcat20 levels); fe(id firm time), cluster(id firm),
+    generated on the device.  The cross products run on the FP64 matrix cores."""
+    from fe_gpu_amd import api
+    from fe_gpu_amd._lib import check
+    from fe_gpu_amd.device import DeviceBuffer
+    ni, T, nf = args.iv_individuals, 20, 50_000
+    n = ni * T
+    K, L = 252, 21
+    d_fe, d_y = DeviceBuffer(3 * n * 4), DeviceBuffer(n * 8)
+    d_W, d_Z = DeviceBuffer(K * n * 8), DeviceBuffer(L * n * 8)
+    check(lib.fehip_synth_iv_wide(ni, T, nf, 0.10, 303, d_fe.ptr, d_y.ptr, d_W.ptr, d_Z.ptr), "iv wide synth")
+    pan = api.Panel([d_fe.ptr, d_fe.ptr + n * 4, d_fe.ptr + 2 * n * 4], n)
+    kw = dict(cluster=(1, 2), iv_reg=list(range(1, 9)), tol=args.tol, max_iter=2000)
+    t = time.perf_counter()
+    fit = api.estimate_device(pan, d_y.ptr, d_W.ptr, K, d_Z.ptr, L, n, **kw)
+    log(f"iv wide warmup: {time.perf_counter() - t:.2f} s, iterations {fit.res['fe_iterations']}")
+    lib.fehip_kstats_reset()
+    lib.fehip_kstats_enable(1)
+    t = time.perf_counter()
+    fit = api.estimate_device(pan, d_y.ptr, d_W.ptr, K, d_Z.ptr, L, n, **kw)
+    dt = time.perf_counter() - t
+    lib.fehip_kstats_enable(0)
+    g = kstat(lib, 4)
+    gram_tf = g["flops"] / (g["ms"] * 1e-3) / 1e12 if g["ms"] > 0 else 0.0
+    log(f"iv wide: {dt:.2f} s, gram {g['launches']} launches {g['ms']:.1f} ms {gram_tf:.1f} TFLOP/s")
+    out = {
+        "workload": f"2SLS, {n} rows ({ni} individuals x {T} periods, {nf} firms), K={K} (8 endogenous), "
+                    f"{L} excluded instruments, fe(id firm time), cluster(id firm), tol {args.tol}",
+        "seconds_per_regression": dt, "regressions_per_s": 1.0 / dt,
+        "phase_s": {"demean": fit.res["t_demean"], "regression": fit.res["t_regression"], "se": fit.res["t_se"]},
+        "fe_iterations": fit.res["fe_iterations"], "converged": fit.res["converged"],
+        "is_iv": fit.res["is_iv"], "n_instruments": fit.res["n_instruments"], "dof_model": fit.res["dof_model"],
+        "cross_products_mfma": {"launches": g["launches"], "total_ms": g["ms"], "tflops": gram_tf,
+                                "peak_tflops": 78.6, "frac": gram_tf / 78.6},
+        "beta_xend": float(fit.beta[0]), "se_xend": float(fit.se[0]),
+    }
+    pan.close()
+    for b in (d_fe, d_y, d_W, d_Z):
+        b.free()
+    return out
+
+
 def serving_leg(args, lib):
     """Config 5: request stream on a 10M-row, 4-FE device-resident panel (worker, firm, time,
     region), 20 candidate columns.  Each request: k ~ U{2..12} consecutive candidate columns from a
@@ -195,6 +240,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-iv-wide", action="store_true")
+    ap.add_argument("--iv-individuals", type=int, default=1_000_000)
     ap.add_argument("--serving-only", action="store_true", help="run only the request-stream leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run a plain device copy of MIB MiB (torch), the reference kernel for HBM counters")
@@ -304,9 +351,13 @@ def main():
         log("timing the CPU oracle on a bounded sample")
         cpu = cpu_baseline(nw, T, nf, min(args.cpu_sample_workers, nw), args.seed)
     serving = None
+    iv_wide = None
+    del d_fe, d_y, d_W
+    pan.close()
+    if not args.no_iv_wide:
+        log("wide 2SLS leg")
+        iv_wide = iv_wide_leg(args, lib)
     if not args.no_serving:
-        del d_fe, d_y, d_W
-        pan.close()
         log("serving leg")
         serving = serving_leg(args, lib)
     lat = np.array(step_s) * 1e3
@@ -341,6 +392,7 @@ def main():
         "se": [float(s) for s in fit.se],
         "reference_h100_same_shape_s": 57.750,
         "mixed_precision": mixed,
+        "iv_wide": iv_wide,
         "serving": serving,
     }
     print(json.dumps(out), flush=True)

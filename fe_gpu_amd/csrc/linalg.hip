@@ -6,8 +6,11 @@
 //     the results are deterministic run to run;
 //   * the cluster scores merge runs of equal cluster ids inside each wavefront before the
 //     FP64 atomic, so cluster-sorted rows pay one atomic per run and column.
+#include <algorithm>
+
 #include "../../include/fe_gpu_runtime.h"
 #include "common.hpp"
+#include "kstats.hpp"
 
 namespace fehip {
 namespace {
@@ -94,6 +97,72 @@ __global__ __launch_bounds__(kBlock) void gemm_tile_kernel(bool tA, bool tB, int
         }
 }
 
+// C = A^T B for tall A (k x m) and B (k x n), column-major: the normal-equation products X'X, Z'Z, Z'X
+// of wide designs, on the FP64 matrix cores (v_mfma_f64_16x16x4f64).  A workgroup of four waves owns a
+// (32 NB) x (32 NB) output tile, each wave a (16 NB) x (16 NB) block of NB x NB MFMA tiles; slabs of
+// 32 rows of A and B are staged in LDS (loaded along the contiguous row direction) and each lane feeds
+// the matrix core its element (row = lane % 16 of the 16-block, k = lane / 16 of the 4-step).  Each
+// split of the reduction writes its own partial tile; the splits are summed in a fixed order.
+typedef double double4_t __attribute__((ext_vector_type(4)));
+
+template <int NB>
+__global__ __launch_bounds__(256) void gemm_tn_mfma_kernel(int m, int n, long long k, const double* __restrict__ A,
+                                                           long long ldA, const double* __restrict__ B, long long ldB,
+                                                           long long kchunk, double* partial, bool upper_only) {
+    constexpr int BT = 32 * NB;  // block tile
+    constexpr int KS = 32;       // slab rows
+    __shared__ double As[KS][BT + 1];
+    __shared__ double Bs[KS][BT + 1];
+    const int i0 = blockIdx.x * BT, j0 = blockIdx.y * BT;
+    if (upper_only && i0 > j0 + BT - 1) return;
+    const long long k0 = (long long)blockIdx.z * kchunk;
+    long long k1 = k0 + kchunk;
+    if (k1 > k) k1 = k;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int wi = (wave & 1) * 16 * NB, wj = (wave >> 1) * 16 * NB;
+    const int lr = lane & 15, lk = lane >> 4;
+    double4_t acc[NB][NB];
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    for (long long p0 = k0; p0 < k1; p0 += KS) {
+        for (int e = t; e < KS * BT; e += 256) {
+            const int pp = e % KS, ii = e / KS;
+            const long long p = p0 + pp;
+            const int ia = i0 + ii, jb = j0 + ii;
+            As[pp][ii] = (p < k1 && ia < m) ? A[p + (long long)ia * ldA] : 0.0;
+            Bs[pp][ii] = (p < k1 && jb < n) ? B[p + (long long)jb * ldB] : 0.0;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < KS / 4; ++kk) {
+            double av[NB], bv[NB];
+#pragma unroll
+            for (int a = 0; a < NB; ++a) av[a] = As[kk * 4 + lk][wi + a * 16 + lr];
+#pragma unroll
+            for (int b = 0; b < NB; ++b) bv[b] = Bs[kk * 4 + lk][wj + b * 16 + lr];
+#pragma unroll
+            for (int a = 0; a < NB; ++a)
+#pragma unroll
+                for (int b = 0; b < NB; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    // D[i][j] of a 16 x 16 tile: j = lane % 16, i = lane / 16 + 4 v (measured lane layout of the
+    // f64 16x16x4 instruction on gfx950)
+    double* out = partial + (size_t)blockIdx.z * m * n;
+#pragma unroll
+    for (int a = 0; a < NB; ++a)
+#pragma unroll
+        for (int b = 0; b < NB; ++b)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int i = i0 + wi + a * 16 + lk + 4 * v, j = j0 + wj + b * 16 + lr;
+                if (i < m && j < n && !(upper_only && i > j)) out[(size_t)j * m + i] = acc[a][b][v];
+            }
+}
+
 __global__ void gemm_finish_kernel(const double* __restrict__ partial, int nsplit, int m, int n, double alpha,
                                    double beta, double* C, long long ldC, bool upper_only) {
     const size_t total = (size_t)m * n;
@@ -110,9 +179,63 @@ __global__ void gemm_finish_kernel(const double* __restrict__ partial, int nspli
 
 }  // namespace
 
+namespace {
+// A^T B on the matrix cores: tile size with the least padding of m and n (64, 96 or 128), split-K
+// over the rows to fill the device.
+int gemm_tn_mfma(int m, int n, long long k, double alpha, const double* A, long long ldA, const double* B,
+                 long long ldB, double beta, double* C, long long ldC, bool upper_only) {
+    int nb = 2;
+    double best = 0.0;
+    for (int cand : {4, 3, 2}) {
+        const int bt = 32 * cand;
+        const double eff = (double)m * n / ((double)((m + bt - 1) / bt * bt) * ((n + bt - 1) / bt * bt));
+        if (eff > best + 0.02) {
+            best = eff;
+            nb = cand;
+        }
+    }
+    const int bt = 32 * nb;
+    const long long tiles = (long long)((m + bt - 1) / bt) * ((n + bt - 1) / bt);
+    long long nsplit = std::max<long long>(1, (4LL * num_cus()) / std::max<long long>(1, tiles));
+    nsplit = std::min<long long>(nsplit, std::max<long long>(1, k / 1024));
+    nsplit = std::min<long long>(nsplit, std::max<long long>(1, (512LL << 20) / ((long long)m * n * 8)));
+    long long kchunk = (k + nsplit - 1) / nsplit;
+    kchunk = (kchunk + 31) / 32 * 32;
+    nsplit = (k + kchunk - 1) / kchunk;
+    double* partial = static_cast<double*>(workspace((size_t)nsplit * m * n * sizeof(double), 2));
+    if (!partial) return set_error("gemm: workspace allocation failed");
+    dim3 grid((m + bt - 1) / bt, (n + bt - 1) / bt, (unsigned)nsplit);
+    if (nb == 4)
+        hipLaunchKernelGGL(gemm_tn_mfma_kernel<4>, grid, dim3(256), 0, 0, m, n, k, A, ldA, B, ldB, kchunk, partial,
+                           upper_only);
+    else if (nb == 3)
+        hipLaunchKernelGGL(gemm_tn_mfma_kernel<3>, grid, dim3(256), 0, 0, m, n, k, A, ldA, B, ldB, kchunk, partial,
+                           upper_only);
+    else
+        hipLaunchKernelGGL(gemm_tn_mfma_kernel<2>, grid, dim3(256), 0, 0, m, n, k, A, ldA, B, ldB, kchunk, partial,
+                           upper_only);
+    FEHIP_LAUNCH_CHECK("gemm mfma kernel");
+    hipLaunchKernelGGL(gemm_finish_kernel, dim3(stream_grid((size_t)m * n, kBlock)), dim3(kBlock), 0, 0, partial,
+                       (int)nsplit, m, n, alpha, beta, C, ldC, upper_only);
+    FEHIP_LAUNCH_CHECK("gemm finish kernel");
+    return set_ok();
+}
+}  // namespace
+
 int gemm_impl(bool tA, bool tB, int m, int n, long long k, double alpha, const double* A, long long ldA,
               const double* B, long long ldB, double beta, double* C, long long ldC, bool upper_only) {
     if (m <= 0 || n <= 0) return set_ok();
+    // wide cross products over many observations go to the matrix cores
+    if (tA && !tB && m >= 16 && n >= 16 && k >= 4096) {
+        const bool timed = kstats_enabled();
+        if (timed) kstat_begin(kStatGram, 0);
+        const int e = gemm_tn_mfma(m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, upper_only);
+        if (timed) {
+            const double pairs = upper_only ? 0.5 * (double)m * (m + 1) : (double)m * n;
+            kstat_end(kStatGram, 0, 8.0 * (double)k * (upper_only ? m : m + n), 2.0 * pairs * (double)k);
+        }
+        return e;
+    }
     const long long tiles = (long long)((m + TM - 1) / TM) * ((n + TN - 1) / TN);
     const long long target = 4LL * num_cus();
     long long nsplit = 1;

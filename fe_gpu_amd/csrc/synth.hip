@@ -70,6 +70,67 @@ __global__ void synth_kernel(int64_t n_workers, int T, int n_firms, double switc
     }
 }
 
+// Wide 2SLS design (20M-row shape): individuals x periods rows ordered by individual then period,
+// firms moving with probability switch_p per period, time = period.  One thread per row (the firm
+// path of its individual is replayed up to the row, so every row is a pure function of its
+// coordinates and all column writes are coalesced).  Columns of W (K = 252):
+//   0 xend (endogenous), 1-7 cat8 levels 2..8 (endogenous), 8-11 x1..x4,
+//   12-40 ind levels 2..30, 41-47 msp levels 2..8, 48-250 ind x msp (levels 2.. x 2..),
+//   251 x1 * x2;
+// Z (L = 21): 0 z1, 1 z2, 2-20 zcat20 levels 2..20.  xend and the cat8 latent load on the
+// instruments (first-stage correlation about 0.4) and on the structural error u.
+constexpr int kWideK = 252;
+constexpr int kWideL = 21;
+
+__global__ void synth_iv_wide_kernel(int64_t n_ind, int T, int n_firms, double switch_p, uint64_t seed, int32_t* fe,
+                                     double* y, double* W, double* Z) {
+    const int64_t n = n_ind * T;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t w = i / T;
+        const int t = (int)(i - w * T);
+        int firm = 1 + (int)(uni(seed, 2, (uint64_t)w * 64) * n_firms);
+        for (int s = 1; s <= t; ++s)
+            if (uni(seed, 3, (uint64_t)(w * T + s)) < switch_p) firm = 1 + (int)(uni(seed, 4, (uint64_t)(w * T + s)) * n_firms);
+        if (firm > n_firms) firm = n_firms;
+        const double z1 = gauss(seed, 30, (uint64_t)i), z2 = gauss(seed, 31, (uint64_t)i);
+        int zc = 1 + (int)(uni(seed, 32, (uint64_t)i) * 20);
+        zc = zc > 20 ? 20 : zc;
+        const double u = gauss(seed, 33, (uint64_t)i);
+        const double xend = 0.45 * 0.7071067811865476 * (z1 + z2) + 0.6 * u + 0.66 * gauss(seed, 34, (uint64_t)i);
+        const double lat = 0.4 * ((double)zc - 10.5) / 5.766 + 0.5 * u + 0.77 * gauss(seed, 35, (uint64_t)i);
+        int c8 = 1 + (int)floor((lat + 2.5) * 1.6);
+        c8 = c8 < 1 ? 1 : (c8 > 8 ? 8 : c8);
+        double x[4];
+        for (int j = 0; j < 4; ++j) x[j] = gauss(seed, 36 + j, (uint64_t)i);
+        int ind = 1 + (int)(uni(seed, 40, (uint64_t)i) * 30);
+        ind = ind > 30 ? 30 : ind;
+        int msp = 1 + (int)(uni(seed, 41, (uint64_t)i) * 8);
+        msp = msp > 8 ? 8 : msp;
+        // columns
+        for (int c = 0; c < kWideK; ++c) W[i + (int64_t)c * n] = 0.0;
+        W[i] = xend;
+        if (c8 >= 2) W[i + (int64_t)(c8 - 1) * n] = 1.0;
+        for (int j = 0; j < 4; ++j) W[i + (int64_t)(8 + j) * n] = x[j];
+        if (ind >= 2) W[i + (int64_t)(12 + ind - 2) * n] = 1.0;
+        if (msp >= 2) W[i + (int64_t)(41 + msp - 2) * n] = 1.0;
+        if (ind >= 2 && msp >= 2) W[i + (int64_t)(48 + (ind - 2) * 7 + (msp - 2)) * n] = 1.0;
+        W[i + (int64_t)251 * n] = x[0] * x[1];
+        for (int c = 0; c < kWideL; ++c) Z[i + (int64_t)c * n] = 0.0;
+        Z[i] = z1;
+        Z[i + n] = z2;
+        if (zc >= 2) Z[i + (int64_t)(2 + zc - 2) * n] = 1.0;
+        // outcome: effects of xend, the cat8 levels, x1..x4, ind, msp, their interaction and x1 x2
+        const double aw = gauss(seed, 1, (uint64_t)w), af = 0.5 * gauss(seed, 5, (uint64_t)firm);
+        const double at = 0.1 * gauss(seed, 6, (uint64_t)t);
+        double v = 0.8 * xend + 0.15 * (double)(c8 - 1) + 0.5 * x[0] - 0.3 * x[1] + 0.2 * x[2] + 0.1 * x[3];
+        v += 0.02 * (double)ind - 0.05 * (double)msp + 0.01 * (double)((ind * msp) % 7) + 0.25 * x[0] * x[1];
+        y[i] = v + aw + af + at + u + gauss(seed, 42, (uint64_t)i);
+        fe[i] = (int32_t)(w + 1);
+        fe[i + n] = firm;
+        fe[i + 2 * n] = t + 1;
+    }
+}
+
 }  // namespace
 }  // namespace fehip
 
@@ -102,5 +163,19 @@ extern "C" int fehip_device_copy(const void* d_src, void* d_dst, int64_t bytes) 
                        (const uint4*)d_src, (uint4*)d_dst, n16);
     FEHIP_LAUNCH_CHECK("device copy");
     FEHIP_CHECK(hipDeviceSynchronize(), "device copy");
+    return fehip::set_ok();
+}
+
+extern "C" int fehip_synth_iv_wide(int64_t n_ind, int32_t n_periods, int32_t n_firms, double switch_p, uint64_t seed,
+                                   int32_t* d_fe, double* d_y, double* d_W, double* d_Z) {
+    (void)hipGetLastError();
+    if (n_ind <= 0 || n_periods <= 0 || n_firms <= 0 || !d_fe || !d_y || !d_W || !d_Z)
+        return fehip::set_error("fehip_synth_iv_wide: bad arguments");
+    const int64_t n = n_ind * n_periods;
+    if (n > 0x7fffffffLL) return fehip::set_error("fehip_synth_iv_wide: too many rows");
+    hipLaunchKernelGGL(fehip::synth_iv_wide_kernel, dim3(fehip::stream_grid((size_t)n, 256)), dim3(256), 0, 0, n_ind,
+                       (int)n_periods, (int)n_firms, switch_p, seed, d_fe, d_y, d_W, d_Z);
+    FEHIP_LAUNCH_CHECK("iv wide generator");
+    FEHIP_CHECK(hipDeviceSynchronize(), "iv wide generator");
     return fehip::set_ok();
 }

@@ -166,6 +166,14 @@ int fehip_server_destroy(fehip_server* server);
 int fehip_synth_worker_firm(int64_t n_workers, int32_t n_periods, int32_t n_firms, double switch_p, uint64_t seed,
                             int32_t* d_fe, double* d_y, double* d_W);
 
+/* Deterministic wide 2SLS panel generated on the device: n_ind x n_periods rows (ordered by individual
+ * then period), firms moving with probability switch_p, time = period.  Writes d_fe (3 x n int32,
+ * dim-major: individual, firm, period), d_y (n), d_W (n x 252: xend, 7 cat8 levels, x1..x4, 29 ind
+ * levels, 7 msp levels, 203 ind x msp cells, x1*x2; columns 1-8 are endogenous) and d_Z (n x 21: z1,
+ * z2, 19 zcat20 levels), column-major. */
+int fehip_synth_iv_wide(int64_t n_ind, int32_t n_periods, int32_t n_firms, double switch_p, uint64_t seed,
+                        int32_t* d_fe, double* d_y, double* d_W, double* d_Z);
+
 /* Opt-in per-kernel timing with HIP events on the stream each kernel is launched on.  Kernel ids:
  * 0 apply_iter (the CG operator product, the dominant kernel), 1 apply_data (right-hand side),
  * 2 finalize (M_F columns), 3 reduce_runs, 4 gram, 5 segsum (cluster score sums).  bytes/flops

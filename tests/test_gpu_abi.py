@@ -123,6 +123,33 @@ def test_gemm_all_transposes(lib, tA, tB):
     assert rel(dC.to_array(np.float64, (m + 3, n), order="F"), exp) < 1e-12
 
 
+@pytest.mark.parametrize("m,n,k", [(265, 252, 20011), (64, 64, 4096), (100, 17, 9001)])
+def test_gemm_wide_cross_products_on_matrix_cores(lib, m, n, k):
+    """A^T B with many rows runs on the FP64 matrix cores (tile padding, split-K partials)."""
+    note(f"gemm mfma {m}x{n}x{k}")
+    rng = np.random.default_rng(m + n)
+    A = rng.normal(size=(k, m))
+    B = rng.normal(size=(k, n))
+    dA, dB = dev(np.asfortranarray(A)), dev(np.asfortranarray(B))
+    dC = dev(np.zeros((m, n), order="F"))
+    assert lib.fe_gpu_gemm(b"T", b"N", m, n, k, 1.0, dA.ptr, k, dB.ptr, k, 0.0, dC.ptr, m) == 0
+    assert rel(dC.to_array(np.float64, (m, n), order="F"), A.T @ B) < 1e-12
+
+
+def test_syrk_wide_upper_on_matrix_cores(lib):
+    note("syrk mfma 140 cols")
+    rng = np.random.default_rng(11)
+    n, K = 30011, 140
+    W = np.asfortranarray(rng.normal(size=(n, K)))
+    dW = dev(W)
+    dQ = dev(np.full((K, K), 7.0, order="F"))
+    assert lib.fe_gpu_syrk(n, K, 1.0, dW.ptr, n, 0.0, dQ.ptr) == 0
+    Q = dQ.to_array(np.float64, (K, K), order="F")
+    iu = np.triu_indices(K)
+    assert rel(Q[iu], (W.T @ W)[iu]) < 1e-12
+    assert np.all(Q[np.tril_indices(K, -1)] == 7.0)
+
+
 def test_syrk_gemv_residual_dot(lib):
     note("syrk/gemv/residual/dot")
     rng = np.random.default_rng(6)

@@ -214,3 +214,19 @@ def test_sorted_subsets_with_clusters_across_waves(K, cluster):
     ref = oracle.estimate(p.y, p.W, p.fe, cluster=cluster, tol=1e-13, max_iter=200000)
     assert fit.res["n_clusters"] == ref.res["n_clusters"]
     assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
+
+
+def test_wide_iv_design_matches_oracle():
+    """A wide 2SLS design (40 regressors, 8 endogenous, 12 instruments): the normal equations, the
+    first stage and X^ run through the wide GEMM paths."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    note("wide IV design")
+    p = synth.random_fe_problem(808, n=20000, K=40, L=12, G=(1200, 300, 15))
+    Z = p.Z + 0.5 * p.W[:, :12]
+    iv = list(range(1, 9))
+    fit = api.estimate(p.y, p.W, p.fe, Z=Z, iv_reg=iv, cluster=(1, 2), tol=1e-13, max_iter=20000)
+    ref = oracle.estimate(p.y, p.W, p.fe, Z=Z, iv_reg=iv, cluster=(1, 2), tol=1e-13, max_iter=200000)
+    assert fit.res["is_iv"] == 1 and fit.res["n_instruments"] == ref.res["n_instruments"]
+    assert fit.res["dof_model"] == ref.res["dof_model"]
+    assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
