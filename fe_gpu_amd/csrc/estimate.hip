@@ -859,6 +859,17 @@ __global__ __launch_bounds__(kBlock) void composite_runs_kernel(const int* __res
     if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(count, c);
 }
 
+// out[i + a n] = work[i + cols[a] n]
+__global__ void gather_cols_kernel(const double* __restrict__ work, int64_t n, const int* __restrict__ cols, int nc,
+                                   double* out) {
+    const int64_t total = n * nc;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int a = (int)(e / n);
+        const int64_t i = e - (int64_t)a * n;
+        out[e] = work[i + (int64_t)cols[a] * n];
+    }
+}
+
 struct DevBuf {
     std::vector<void*> v;
     DevBuf() = default;
@@ -943,6 +954,45 @@ int nest_mode(Panel* p, unsigned dims_mask, const int* const* kid, int nk, unsig
     }
     p->nest_modes[dims_mask] = m;
     *mode = m;
+    return 0;
+}
+
+// Mx = proj' Mz proj for the 2SLS meat (Mz: m x m, proj: m x k, column-major); small on the host,
+// wide on the device.
+int project_meat(const std::vector<double>& Mz, int m, const std::vector<double>& proj, int k, std::vector<double>& Mx) {
+    Mx.assign((size_t)k * k, 0.0);
+    if ((int64_t)m * m * k <= (int64_t)4 << 20) {
+        std::vector<double> t((size_t)m * k, 0.0);  // Mz proj
+        for (int j = 0; j < k; ++j)
+            for (int b = 0; b < m; ++b) {
+                const double pb = proj[(size_t)j * m + b];
+                if (pb == 0.0) continue;
+                for (int a = 0; a < m; ++a) t[(size_t)j * m + a] += Mz[(size_t)b * m + a] * pb;
+            }
+        for (int j = 0; j < k; ++j)
+            for (int i = 0; i < k; ++i) {
+                double v = 0.0;
+                for (int a = 0; a < m; ++a) v += proj[(size_t)i * m + a] * t[(size_t)j * m + a];
+                Mx[(size_t)j * k + i] = v;
+            }
+        return 0;
+    }
+    double *dM = nullptr, *dP = nullptr, *dT = nullptr, *dX = nullptr;
+    int e;
+    if ((e = talloc(&dM, (size_t)m * m)) || (e = talloc(&dP, (size_t)m * k)) || (e = talloc(&dT, (size_t)m * k)) ||
+        (e = talloc(&dX, (size_t)k * k)))
+        return e;
+    struct Free {
+        double** v[4];
+        ~Free() {
+            for (auto p : v) cache_free(*p);
+        }
+    } fr{{&dM, &dP, &dT, &dX}};
+    FEHIP_CHECK(hipMemcpy(dM, Mz.data(), sizeof(double) * (size_t)m * m, hipMemcpyHostToDevice), "meat copy");
+    FEHIP_CHECK(hipMemcpy(dP, proj.data(), sizeof(double) * (size_t)m * k, hipMemcpyHostToDevice), "proj copy");
+    if ((e = gemm_impl(false, false, m, k, m, 1.0, dM, m, dP, m, 0.0, dT, m, false))) return e;
+    if ((e = gemm_impl(true, false, k, k, m, 1.0, dP, m, dT, m, 0.0, dX, k, false))) return e;
+    FEHIP_CHECK(hipMemcpy(Mx.data(), dX, sizeof(double) * (size_t)k * k, hipMemcpyDeviceToHost), "meat copy");
     return 0;
 }
 
@@ -1331,20 +1381,22 @@ int finish_request(Panel* p, ReqState& s) {
     std::vector<double> cov((size_t)kept * kept, 0.0);
     bool clustered = ncd > 0;
     if (clustered) {
-        // regressors for the scores: W~, or X^ = [W~ Z~] proj_full for IV (fe_pipeline.f90:1181-1186)
+        // Regressors for the scores: W~, or for 2SLS X^ = Z_aug proj (fe_pipeline.f90:1181-1186).  The 2SLS
+        // meat is formed without X^: the cluster sums of r X^ are proj' times those of r Z_aug, so each
+        // subset's meat is proj' M_z proj with M_z the meat of the scores r Z_aug.
+        const int Kx = K;
+        const int K = use_iv ? n_aug : Kx;  // width of the scores (shadows the regressor count below)
         const double* X = f32 ? nullptr : work + n;  // W~ columns
         int64_t ldX = n;
-        double* Xhat = nullptr;
+        double* Zaug = nullptr;
         if (use_iv) {
-            std::vector<double> pf((size_t)(K + L) * K, 0.0);
-            for (int k = 0; k < K; ++k)
-                for (int a = 0; a < n_aug; ++a) pf[(size_t)k * (K + L) + (aug_col[a] - 1)] = proj[(size_t)k * n_aug + a];
-            double* dpf = nullptr;
-            if ((e = db.get(&dpf, pf.size())) || (e = db.get(&Xhat, (size_t)n * K))) return e;
-            FEHIP_CHECK(hipMemcpy(dpf, pf.data(), pf.size() * sizeof(double), hipMemcpyHostToDevice), "proj copy");
-            if ((e = gemm_impl(false, false, (int)n, K, K + L, 1.0, work + n, n, dpf, K + L, 0.0, Xhat, n, false)))
-                return e;
-            X = Xhat;
+            int* dcols = nullptr;
+            if ((e = db.get(&dcols, (size_t)n_aug)) || (e = db.get(&Zaug, (size_t)n * n_aug))) return e;
+            FEHIP_CHECK(hipMemcpy(dcols, aug_col.data(), sizeof(int) * n_aug, hipMemcpyHostToDevice), "columns copy");
+            hipLaunchKernelGGL(gather_cols_kernel, dim3(stream_grid((size_t)n * n_aug, kBlock)), dim3(kBlock), 0, 0,
+                               (const double*)work, n, (const int*)dcols, n_aug, Zaug);
+            FEHIP_LAUNCH_CHECK("instrument columns");
+            X = Zaug;
         }
         // scores T = r x (row-major n x K), formed only if a subset's method reads it
         double* T = T_early;
@@ -1624,11 +1676,17 @@ int finish_request(Panel* p, ReqState& s) {
                 min_single = std::min<int64_t>(min_single, ncl);
             }
             result->n_clusters[mask - 1] = ncl;
+            // 2SLS: the meat of r X^ is proj' M_z proj
+            const std::vector<double>* Mp = &M;
+            std::vector<double> Mx;
+            if (use_iv && (e = project_meat(M, n_aug, proj, Kx, Mx))) return e;
+            if (use_iv) Mp = &Mx;
             // cov += w Qinv M_kept Qinv
             for (int a = 0; a < kept; ++a)
                 for (int c = 0; c < kept; ++c) {
                     double s = 0.0;
-                    for (int k = 0; k < kept; ++k) s += M[(size_t)keep[k] * K + keep[a]] * Qinv[(size_t)c * kept + k];
+                    for (int k = 0; k < kept; ++k)
+                        s += (*Mp)[(size_t)keep[k] * Kx + keep[a]] * Qinv[(size_t)c * kept + k];
                     tmp[(size_t)c * kept + a] = s;
                 }
             for (int a = 0; a < kept; ++a)

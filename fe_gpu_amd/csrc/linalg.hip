@@ -105,17 +105,29 @@ __global__ __launch_bounds__(kBlock) void gemm_tile_kernel(bool tA, bool tB, int
 // split of the reduction writes its own partial tile; the splits are summed in a fixed order.
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-template <int NB>
+// Workgroups are dealt to the 8 XCDs round-robin by linear id; the grid is laid out so that all
+// output tiles of one reduction split land on the same XCD one after another, and the row slabs they
+// all read are shared through that XCD's L2.
+constexpr int kXcds = 8;
+
+// RM: the reduction index runs along the leading dimension's complement, i.e. element (p, i) of the
+// k x m operand is A[p ldA + i] (the A B^T shape of S'S for row-major cluster sums); otherwise A[p + i ldA].
+template <int NB, bool RM>
 __global__ __launch_bounds__(256) void gemm_tn_mfma_kernel(int m, int n, long long k, const double* __restrict__ A,
                                                            long long ldA, const double* __restrict__ B, long long ldB,
-                                                           long long kchunk, double* partial, bool upper_only) {
+                                                           long long kchunk, int nsplit, int tiles_x, int tiles_y,
+                                                           double* partial, bool upper_only) {
     constexpr int BT = 32 * NB;  // block tile
     constexpr int KS = 32;       // slab rows
     __shared__ double As[KS][BT + 1];
     __shared__ double Bs[KS][BT + 1];
-    const int i0 = blockIdx.x * BT, j0 = blockIdx.y * BT;
+    const int ntiles = tiles_x * tiles_y;
+    const int xcd = blockIdx.x % kXcds, local = blockIdx.x / kXcds;
+    const int split = xcd + kXcds * (local / ntiles), tile = local % ntiles;
+    if (split >= nsplit) return;
+    const int i0 = (tile % tiles_x) * BT, j0 = (tile / tiles_x) * BT;
     if (upper_only && i0 > j0 + BT - 1) return;
-    const long long k0 = (long long)blockIdx.z * kchunk;
+    const long long k0 = (long long)split * kchunk;
     long long k1 = k0 + kchunk;
     if (k1 > k) k1 = k;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -128,11 +140,11 @@ __global__ __launch_bounds__(256) void gemm_tn_mfma_kernel(int m, int n, long lo
         for (int b = 0; b < NB; ++b) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
     for (long long p0 = k0; p0 < k1; p0 += KS) {
         for (int e = t; e < KS * BT; e += 256) {
-            const int pp = e % KS, ii = e / KS;
+            const int pp = RM ? e / BT : e % KS, ii = RM ? e % BT : e / KS;  // walk the contiguous direction
             const long long p = p0 + pp;
             const int ia = i0 + ii, jb = j0 + ii;
-            As[pp][ii] = (p < k1 && ia < m) ? A[p + (long long)ia * ldA] : 0.0;
-            Bs[pp][ii] = (p < k1 && jb < n) ? B[p + (long long)jb * ldB] : 0.0;
+            As[pp][ii] = (p < k1 && ia < m) ? (RM ? A[p * ldA + ia] : A[p + (long long)ia * ldA]) : 0.0;
+            Bs[pp][ii] = (p < k1 && jb < n) ? (RM ? B[p * ldB + jb] : B[p + (long long)jb * ldB]) : 0.0;
         }
         __syncthreads();
 #pragma unroll
@@ -151,7 +163,7 @@ __global__ __launch_bounds__(256) void gemm_tn_mfma_kernel(int m, int n, long lo
     }
     // D[i][j] of a 16 x 16 tile: j = lane % 16, i = lane / 16 + 4 v (measured lane layout of the
     // f64 16x16x4 instruction on gfx950)
-    double* out = partial + (size_t)blockIdx.z * m * n;
+    double* out = partial + (size_t)split * m * n;
 #pragma unroll
     for (int a = 0; a < NB; ++a)
 #pragma unroll
@@ -182,7 +194,7 @@ __global__ void gemm_finish_kernel(const double* __restrict__ partial, int nspli
 namespace {
 // A^T B on the matrix cores: tile size with the least padding of m and n (64, 96 or 128), split-K
 // over the rows to fill the device.
-int gemm_tn_mfma(int m, int n, long long k, double alpha, const double* A, long long ldA, const double* B,
+int gemm_tn_mfma(bool rm, int m, int n, long long k, double alpha, const double* A, long long ldA, const double* B,
                  long long ldB, double beta, double* C, long long ldC, bool upper_only) {
     int nb = 2;
     double best = 0.0;
@@ -204,16 +216,20 @@ int gemm_tn_mfma(int m, int n, long long k, double alpha, const double* A, long 
     nsplit = (k + kchunk - 1) / kchunk;
     double* partial = static_cast<double*>(workspace((size_t)nsplit * m * n * sizeof(double), 2));
     if (!partial) return set_error("gemm: workspace allocation failed");
-    dim3 grid((m + bt - 1) / bt, (n + bt - 1) / bt, (unsigned)nsplit);
-    if (nb == 4)
-        hipLaunchKernelGGL(gemm_tn_mfma_kernel<4>, grid, dim3(256), 0, 0, m, n, k, A, ldA, B, ldB, kchunk, partial,
-                           upper_only);
-    else if (nb == 3)
-        hipLaunchKernelGGL(gemm_tn_mfma_kernel<3>, grid, dim3(256), 0, 0, m, n, k, A, ldA, B, ldB, kchunk, partial,
-                           upper_only);
-    else
-        hipLaunchKernelGGL(gemm_tn_mfma_kernel<2>, grid, dim3(256), 0, 0, m, n, k, A, ldA, B, ldB, kchunk, partial,
-                           upper_only);
+    const int tx = (m + bt - 1) / bt, ty = (n + bt - 1) / bt;
+    const long long per_xcd = (nsplit + kXcds - 1) / kXcds;
+    const unsigned grid = (unsigned)(kXcds * per_xcd * tx * ty);
+#define FEHIP_MFMA_LAUNCH(NBV, RMV)                                                                           \
+    hipLaunchKernelGGL((gemm_tn_mfma_kernel<NBV, RMV>), dim3(grid), dim3(256), 0, 0, m, n, k, A, ldA, B, ldB, kchunk, \
+                       (int)nsplit, tx, ty, partial, upper_only)
+    if (nb == 4) {
+        if (rm) FEHIP_MFMA_LAUNCH(4, true); else FEHIP_MFMA_LAUNCH(4, false);
+    } else if (nb == 3) {
+        if (rm) FEHIP_MFMA_LAUNCH(3, true); else FEHIP_MFMA_LAUNCH(3, false);
+    } else {
+        if (rm) FEHIP_MFMA_LAUNCH(2, true); else FEHIP_MFMA_LAUNCH(2, false);
+    }
+#undef FEHIP_MFMA_LAUNCH
     FEHIP_LAUNCH_CHECK("gemm mfma kernel");
     hipLaunchKernelGGL(gemm_finish_kernel, dim3(stream_grid((size_t)m * n, kBlock)), dim3(kBlock), 0, 0, partial,
                        (int)nsplit, m, n, alpha, beta, C, ldC, upper_only);
@@ -226,10 +242,10 @@ int gemm_impl(bool tA, bool tB, int m, int n, long long k, double alpha, const d
               const double* B, long long ldB, double beta, double* C, long long ldC, bool upper_only) {
     if (m <= 0 || n <= 0) return set_ok();
     // wide cross products over many observations go to the matrix cores
-    if (tA && !tB && m >= 16 && n >= 16 && k >= 4096) {
+    if (tA != tB && m >= 16 && n >= 16 && k >= 4096) {  // A^T B (column-major) or A B^T (row-major view)
         const bool timed = kstats_enabled();
         if (timed) kstat_begin(kStatGram, 0);
-        const int e = gemm_tn_mfma(m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, upper_only);
+        const int e = gemm_tn_mfma(!tA, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, upper_only);
         if (timed) {
             const double pairs = upper_only ? 0.5 * (double)m * (m + 1) : (double)m * n;
             kstat_end(kStatGram, 0, 8.0 * (double)k * (upper_only ? m : m + n), 2.0 * pairs * (double)k);

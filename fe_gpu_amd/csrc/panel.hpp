@@ -67,6 +67,9 @@ struct Panel {
     std::map<unsigned, int> nest_modes;
     // Schur diagonal and group counts of unweighted solves, per absorbed-dims mask ([S][2])
     std::map<unsigned, double*> diag_cache;
+    // measured time of one operator product per column-tile width (index into the width list), per
+    // absorbed-dims mask; filled by the within transform's tuner on first use
+    std::map<unsigned, std::vector<float>> width_ms;
     int64_t bytes = 0;
 };
 

@@ -1331,6 +1331,106 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
     return 0;
 }
 
+// ---- column-tile autotuner -------------------------------------------------------------------
+// The operator kernels are instantiated for tile widths 1, 2, 3, 4, 6, 8, 9, 12 and 16 columns.  A
+// wider tile shares the id streams and segment structure among more columns but holds more registers
+// (fewer waves in flight), so the best split of a solve's columns depends on the panel.  The tuner
+// times one operator product (all columns active) per width on first use of a panel and absorbed-dims
+// mask, and the plan then minimises the summed product time of the tiles.
+constexpr int kWidths[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
+constexpr int kNumWidths = 9;
+
+template <int CT>
+int time_width(Panel* p, const Layout& L, Scratch& sc, hipStream_t s, hipEvent_t ev0, hipEvent_t ev1, float* ms) {
+    ApplyArgs a = base_args(p, L, sc, nullptr);
+    for (int c = 0; c < kMaxCT; ++c) {
+        a.cbit[c] = c < CT ? ~0u : 0u;
+        a.xc[c] = nullptr;
+        a.oc[c] = nullptr;
+        a.oc32[c] = nullptr;
+    }
+    a.anybit = ~0u;
+    a.ncols = CT;
+    a.p = sc.tab;
+    a.active = nullptr;
+    double* q = sc.tab + L.S * kMaxCT;
+    unsigned grid = launch_pass<CT, kIter>(p, L, a, s);  // warm-up
+    reduce_tables<CT>(p, L, a, -1, q, grid, nullptr, s, false);
+    FEHIP_CHECK(hipEventRecord(ev0, s), "tuner event");
+    grid = launch_pass<CT, kIter>(p, L, a, s);
+    reduce_tables<CT>(p, L, a, -1, q, grid, nullptr, s, false);
+    FEHIP_CHECK(hipEventRecord(ev1, s), "tuner event");
+    FEHIP_CHECK(hipEventSynchronize(ev1), "tuner sync");
+    FEHIP_CHECK(hipEventElapsedTime(ms, ev0, ev1), "tuner time");
+    return 0;
+}
+
+int plan_tiles(Panel* p, const Layout& L, Scratch& sc, unsigned mask, int ncols, int ct_max, std::vector<int>& tiles,
+               hipStream_t s) {
+    tiles.clear();
+    auto round_up = [&](int c) {
+        for (int w : kWidths)
+            if (w >= c) return w;
+        return 16;
+    };
+    if (ncols <= 0) return 0;
+    if (L.S == 0 || ncols == 1) {  // nothing to iterate on, or nothing to choose
+        for (int c0 = 0; c0 < ncols; c0 += ct_max) tiles.push_back(round_up(std::min(ct_max, ncols - c0)));
+        return 0;
+    }
+    auto it = p->width_ms.find(mask);
+    if (it == p->width_ms.end()) {
+        std::vector<float> t(kNumWidths, -1.0f);
+        FEHIP_CHECK(hipMemsetAsync(sc.tab, 0, sizeof(double) * (size_t)(L.S * kMaxCT * 2), s), "tuner tables");
+        hipEvent_t ev0, ev1;
+        FEHIP_CHECK(hipEventCreate(&ev0), "tuner event");
+        FEHIP_CHECK(hipEventCreate(&ev1), "tuner event");
+        int e = 0;
+        for (int k = 0; k < kNumWidths && !e; ++k) {
+            if (kWidths[k] > ct_max) continue;
+            float ms = 0.0f;
+            switch (kWidths[k]) {
+                case 1: e = time_width<1>(p, L, sc, s, ev0, ev1, &ms); break;
+                case 2: e = time_width<2>(p, L, sc, s, ev0, ev1, &ms); break;
+                case 3: e = time_width<3>(p, L, sc, s, ev0, ev1, &ms); break;
+                case 4: e = time_width<4>(p, L, sc, s, ev0, ev1, &ms); break;
+                case 6: e = time_width<6>(p, L, sc, s, ev0, ev1, &ms); break;
+                case 8: e = time_width<8>(p, L, sc, s, ev0, ev1, &ms); break;
+                case 9: e = time_width<9>(p, L, sc, s, ev0, ev1, &ms); break;
+                case 12: e = time_width<12>(p, L, sc, s, ev0, ev1, &ms); break;
+                default: e = time_width<16>(p, L, sc, s, ev0, ev1, &ms); break;
+            }
+            t[k] = ms;
+        }
+        (void)hipEventDestroy(ev0);
+        (void)hipEventDestroy(ev1);
+        if (e) return e;
+        it = p->width_ms.emplace(mask, t).first;
+    }
+    const std::vector<float>& t = it->second;
+    auto cost_of = [&](int w) {
+        for (int k = 0; k < kNumWidths; ++k)
+            if (kWidths[k] == w) return t[k] >= 0.0f ? (double)t[k] : 1e30;
+        return 1e30;
+    };
+    // full tiles of width w and one tile for the remainder, rounded up to an instantiated width
+    double best = 1e300;
+    int best_w = round_up(std::min(ct_max, ncols));
+    for (int w : kWidths) {
+        if (w > ct_max) continue;
+        const int full = ncols / w, rem = ncols - full * w;
+        const double c = full * cost_of(w) + (rem ? cost_of(round_up(rem)) : 0.0);
+        if (c < best) {
+            best = c;
+            best_w = w;
+        }
+    }
+    const int full = ncols / best_w, rem = ncols - full * best_w;
+    for (int k = 0; k < full; ++k) tiles.push_back(best_w);
+    if (rem) tiles.push_back(round_up(rem));
+    return 0;
+}
+
 }  // namespace
 
 int build_mask_words(Panel* p, const uint8_t* const* w_caller, int nslots, uint32_t* wm, hipStream_t s) {
@@ -1442,15 +1542,12 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
         for (int b = 0; b < nslots; ++b)
             if (used[b] && (e = slot_diagonal(p, L, sc, wm, 1u << b, sc.dg + (int64_t)b * S * 2, s))) return e;
     }
-    for (int c0 = 0; c0 < ncols; c0 += ct_max) {
-        const int nc = std::min(ct_max, ncols - c0);
-        static const int widths[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
-        int ct = 16;
-        for (int wdt : widths)
-            if (wdt >= nc) {
-                ct = wdt;
-                break;
-            }
+    // column tiles: the tuner's measured cost per width picks the split of ncols columns
+    std::vector<int> tiles;
+    if ((e = plan_tiles(p, L, sc, mask, ncols, ct_max, tiles, s))) return e;
+    int c0 = 0;
+    for (int ct : tiles) {
+        const int nc = std::min(ct, ncols - c0);
         TileStats ts;
         switch (ct) {
             case 1: e = solve_tile<1>(p, L, sc, wc, c0, nc, wm, o, to_work, ts, s); break;
@@ -1476,6 +1573,7 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
         }
         const int64_t id_bytes = n * 4 * L.nd + (int64_t)(p->d[p->P].G + 1) * 8 + (wm ? 4 * n : 0);
         ws.bytes += ts.apply_launches * id_bytes + (int64_t)3 * n * 8 * nc;
+        c0 += nc;
     }
     return 0;
 }

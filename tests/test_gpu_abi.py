@@ -124,16 +124,22 @@ def test_gemm_all_transposes(lib, tA, tB):
 
 
 @pytest.mark.parametrize("m,n,k", [(265, 252, 20011), (64, 64, 4096), (100, 17, 9001)])
-def test_gemm_wide_cross_products_on_matrix_cores(lib, m, n, k):
-    """A^T B with many rows runs on the FP64 matrix cores (tile padding, split-K partials)."""
-    note(f"gemm mfma {m}x{n}x{k}")
+@pytest.mark.parametrize("shape", ["TN", "NT"])
+def test_gemm_wide_cross_products_on_matrix_cores(lib, m, n, k, shape):
+    """A^T B and A B^T with a long reduction run on the FP64 matrix cores (tile padding, split-K)."""
+    note(f"gemm mfma {shape} {m}x{n}x{k}")
     rng = np.random.default_rng(m + n)
-    A = rng.normal(size=(k, m))
-    B = rng.normal(size=(k, n))
+    if shape == "TN":
+        A, B = rng.normal(size=(k, m)), rng.normal(size=(k, n))
+        exp = A.T @ B
+    else:
+        A, B = rng.normal(size=(m, k)), rng.normal(size=(n, k))
+        exp = A @ B.T
     dA, dB = dev(np.asfortranarray(A)), dev(np.asfortranarray(B))
     dC = dev(np.zeros((m, n), order="F"))
-    assert lib.fe_gpu_gemm(b"T", b"N", m, n, k, 1.0, dA.ptr, k, dB.ptr, k, 0.0, dC.ptr, m) == 0
-    assert rel(dC.to_array(np.float64, (m, n), order="F"), A.T @ B) < 1e-12
+    assert lib.fe_gpu_gemm(shape[0].encode(), shape[1].encode(), m, n, k, 1.0, dA.ptr, A.shape[0], dB.ptr,
+                           B.shape[0], 0.0, dC.ptr, m) == 0
+    assert rel(dC.to_array(np.float64, (m, n), order="F"), exp) < 1e-12
 
 
 def test_syrk_wide_upper_on_matrix_cores(lib):
