@@ -63,20 +63,58 @@ def cpu_baseline(n_workers_full, n_periods, n_firms_full, sample_workers, seed):
     }
 
 
-def _kstat_summary(lib):
-    out = {}
-    for k in range(6):
-        v = kstat(lib, k)
-        if v["launches"]:
-            out[v["name"]] = {"launches": v["launches"], "total_ms": v["ms"]}
-    return out
-
-
 def _fit_summary(fits):
     return {"mean_fe_iterations": float(np.mean([f.res["fe_iterations"] for f in fits])),
             "max_fe_iterations": int(max(f.res["fe_iterations"] for f in fits)),
             "sum_t_se_s": float(sum(f.res["t_se"] for f in fits)),
             "sum_t_regression_s": float(sum(f.res["t_regression"] for f in fits))}
+
+
+def serving_panel_leg(args, lib, pan, d_y, d_W, n):
+    """North-star form on the 150M-row panel: a stream of full-sample specifications (k ~ U{2..8}
+    consecutive regressors of the 8, a random non-empty subset of the 3 FE dims, 0-3 cluster dims among
+    them) served by the request server in the mixed-precision form (FP32 storage, FP64 refinement),
+    against the same requests solved one at a time in plain FP64."""
+    from fe_gpu_amd import api
+    rng = np.random.default_rng(2020)
+    reqs, plain = [], []
+    for _ in range(args.serve_panel_requests):
+        k = int(rng.integers(2, 9))
+        j0 = int(rng.integers(0, 8 - k + 1))
+        nfe = int(rng.integers(1, 4))
+        dims = sorted(int(d) for d in rng.choice(3, nfe, replace=False))
+        ncl = int(rng.integers(0, nfe + 1))
+        cl = tuple(sorted(int(d) + 1 for d in rng.choice(dims, ncl, replace=False))) if ncl else ()
+        opt = dict(cluster=cl, fe_mask=sum(1 << d for d in dims), tol=args.tol, max_iter=2000)
+        plain.append(api.Request(d_y.ptr, d_W.ptr + 8 * n * j0, k, options=dict(opt, precision=0)))
+        reqs.append(api.Request(d_y.ptr, d_W.ptr + 8 * n * j0, k, options=dict(opt, precision=1)))
+    # sequential plain FP64 (one request at a time)
+    ts = time.perf_counter()
+    seq = [api.estimate_device(pan, q.y_ptr, q.W_ptr, q.K, **q.options) for q in plain]
+    seq_s = time.perf_counter() - ts
+    log(f"150M serving: sequential plain FP64 {len(plain)} requests in {seq_s:.2f} s")
+    srv = api.Server(pan)
+    tc = time.perf_counter()
+    tickets = [srv.submit(q) for q in reqs]
+    outs = [srv.wait(t) for t in tickets]
+    srv_s = time.perf_counter() - tc
+    st = srv.stats()
+    srv.close()
+    log(f"150M serving: batched mixed-precision server {len(reqs)} requests in {srv_s:.2f} s, "
+        f"{st['batches']} batches")
+    berr = max(float(np.max(np.abs(o[0].beta - f.beta) / np.maximum(np.abs(f.beta), 1e-300)))
+               for o, f in zip(outs, seq))
+    serr = max(float(np.max(np.abs(o[0].se - f.se) / np.maximum(np.abs(f.se), 1e-300))) for o, f in zip(outs, seq))
+    return {
+        "workload": f"{len(reqs)} full-sample specifications on the {n}-row panel (k 2-8 of 8 regressors, 1-3 of "
+                    f"the 3 FE dims, 0-3 cluster dims among them), tol {args.tol}",
+        "sequential_plain_fp64": {"regressions_per_s": len(plain) / seq_s, "seconds": seq_s},
+        "server_mixed_precision": {"regressions_per_s": len(reqs) / srv_s, "seconds": srv_s,
+                                   "p50_ms": st["p50_s"] * 1e3, "p99_ms": st["p99_s"] * 1e3,
+                                   "batches": st["batches"], "mean_batch": st["mean_batch"]},
+        "speedup": seq_s / srv_s,
+        "beta_max_rel_err_vs_fp64": berr, "se_max_rel_err_vs_fp64": serr,
+    }
 
 
 def iv_wide_leg(args, lib):
@@ -161,16 +199,12 @@ def serving_leg(args, lib):
     api.estimate_device(pan, reqs[0].y_ptr, reqs[0].W_ptr, reqs[0].K, weights_ptr=reqs[0].weights_ptr,
                         **reqs[0].options)  # warm-up
     seq_fits, seq_lat = [], []
-    lib.fehip_kstats_reset()
-    lib.fehip_kstats_enable(1)
     ts = time.perf_counter()
     for q in reqs[:n_seq]:
         t = time.perf_counter()
         seq_fits.append(api.estimate_device(pan, q.y_ptr, q.W_ptr, q.K, weights_ptr=q.weights_ptr, **q.options))
         seq_lat.append(time.perf_counter() - t)
     seq_s = time.perf_counter() - ts
-    seq_k = _kstat_summary(lib)
-    lib.fehip_kstats_reset()
     seq_rate = n_seq / seq_s
     log(f"sequential: {n_seq} requests in {seq_s:.2f} s ({seq_rate:.1f}/s)")
     # closed loop: every request queued at once
@@ -179,8 +213,6 @@ def serving_leg(args, lib):
     tickets = [srv.submit(q) for q in reqs]
     outs = [srv.wait(t) for t in tickets]
     closed_s = time.perf_counter() - tc
-    closed_k = _kstat_summary(lib)
-    lib.fehip_kstats_enable(0)
     st_closed = srv.stats()
     log(f"server closed loop: {len(reqs)} requests in {closed_s:.2f} s, {st_closed['batches']} batches")
     b_err = max(float(np.max(np.abs(o[0].beta - f.beta) / np.maximum(np.abs(f.beta), 1e-300)))
@@ -211,12 +243,11 @@ def serving_leg(args, lib):
                     f"4 FE dims, 20 candidate columns; {len(reqs)} requests (k 2-12, 1-3 FEs, 0-3 clusters, "
                     f"row masks 5-100%), tol {args.tol}",
         "sequential": {"requests": n_seq, "regressions_per_s": seq_rate, "p50_ms": float(np.percentile(sl, 50)) * 1e3,
-                       "p99_ms": float(np.percentile(sl, 99)) * 1e3, "fits": _fit_summary(seq_fits),
-                       "kernels": seq_k},
+                       "p99_ms": float(np.percentile(sl, 99)) * 1e3, "fits": _fit_summary(seq_fits)},
         "closed_loop": {"requests": len(reqs), "regressions_per_s": len(reqs) / closed_s,
                         "p50_ms": st_closed["p50_s"] * 1e3, "p99_ms": st_closed["p99_s"] * 1e3,
                         "batches": st_closed["batches"], "mean_batch": st_closed["mean_batch"],
-                        "fits": _fit_summary([o[0] for o in outs]), "kernels": closed_k},
+                        "fits": _fit_summary([o[0] for o in outs])},
         "open_loop": {"requests": n_open, "arrival_rate_per_s": rate, "regressions_per_s": n_open / open_s,
                       "p50_ms": st_open["p50_s"] * 1e3, "p99_ms": st_open["p99_s"] * 1e3,
                       "batches": st_open["batches"], "mean_batch": st_open["mean_batch"]},
@@ -241,6 +272,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-iv-wide", action="store_true")
+    ap.add_argument("--no-serve-panel", action="store_true")
+    ap.add_argument("--serve-panel-requests", type=int, default=32)
     ap.add_argument("--iv-individuals", type=int, default=1_000_000)
     ap.add_argument("--serving-only", action="store_true", help="run only the request-stream leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -291,8 +324,6 @@ def main():
         fit = api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kw)
         log(f"warmup {i}: {time.perf_counter() - t:.3f} s, iterations {fit.res['fe_iterations']}, "
             f"converged {fit.res['converged']}")
-    lib.fehip_kstats_reset()
-    lib.fehip_kstats_enable(1)
     step_s = []
     fit = None
     t_all = time.perf_counter()
@@ -303,6 +334,10 @@ def main():
         log(f"step {i}: {step_s[-1]:.3f} s (demean {fit.res['t_demean']:.3f}, regression {fit.res['t_regression']:.3f}, "
             f"se {fit.res['t_se']:.3f}), iterations {fit.res['fe_iterations']}")
     total = time.perf_counter() - t_all
+    # one more step with per-kernel HIP-event timing (kernels launched one by one) for the roofline
+    lib.fehip_kstats_reset()
+    lib.fehip_kstats_enable(1)
+    api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kw)
     lib.fehip_kstats_enable(0)
     ks = {k: kstat(lib, k) for k in range(6)}
     dom = ks[0]
@@ -317,8 +352,6 @@ def main():
         # config 4: the same panel, columns stored in FP32, beta refined in FP64 on the residual
         kwm = dict(kw, precision=1)
         api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kwm)
-        lib.fehip_kstats_reset()
-        lib.fehip_kstats_enable(1)
         ms, mfit = [], None
         tm = time.perf_counter()
         for i in range(args.steps):
@@ -326,6 +359,9 @@ def main():
             mfit = api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kwm)
             ms.append(time.perf_counter() - t)
         mtotal = time.perf_counter() - tm
+        lib.fehip_kstats_reset()
+        lib.fehip_kstats_enable(1)
+        api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kwm)
         lib.fehip_kstats_enable(0)
         mk = {k: kstat(lib, k) for k in range(6)}
         berr = np.abs(mfit.beta - fit.beta) / np.abs(fit.beta)
@@ -346,6 +382,10 @@ def main():
                                     (v["ms"] / max(1, v["launches"]) * 1e-3) / 1e9 if v["ms"] > 0 else 0.0}
                         for v in mk.values() if v["launches"]},
         }
+    serve_panel = None
+    if not args.no_serve_panel:
+        log("serving leg on the large panel")
+        serve_panel = serving_panel_leg(args, lib, pan, d_y, d_W, n)
     cpu = None
     if not args.no_cpu_baseline:
         log("timing the CPU oracle on a bounded sample")
@@ -392,6 +432,7 @@ def main():
         "se": [float(s) for s in fit.se],
         "reference_h100_same_shape_s": 57.750,
         "mixed_precision": mixed,
+        "serving_150M": serve_panel,
         "iv_wide": iv_wide,
         "serving": serving,
     }

@@ -14,6 +14,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <tuple>
 #include <vector>
 
 #include <rocprim/rocprim.hpp>
@@ -1744,8 +1746,12 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
         // count and the memory budget (the first request always goes)
         std::vector<int> chunk;
         double bytes = 0.0;
+        const fehip_demean_opts& lead = rs[i0].opts->demean;
         for (size_t i = i0; i < rs.size() && (int)chunk.size() < kMaxSlots; ++i) {
-            if (done[i] || rs[i].fe_mask != rs[i0].fe_mask) continue;
+            const fehip_demean_opts& o = rs[i].opts->demean;
+            if (done[i] || rs[i].fe_mask != rs[i0].fe_mask || o.tol != lead.tol || o.max_iter != lead.max_iter ||
+                o.check_every != lead.check_every)
+                continue;
             const double b = 8.0 * (double)n * (1 + std::max(0, rs[i].K) + std::max(0, rs[i].L)) + 2.0 * (double)n;
             if (!chunk.empty() && bytes + b > budget) break;
             bytes += b;
@@ -1767,24 +1773,67 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
             if ((e = cb.get(&wm, (size_t)n))) return e;
             if ((e = build_mask_words(p, w.data(), (int)chunk.size(), wm, 0))) return e;
         }
-        WithinCols wc(ncols);
+        // the chunk's columns; a column that several requests share (same input, same row subsample, same
+        // storage precision) is solved once and copied to the others
+        struct ColDst {
+            double* out;
+            float* out32;
+            double* alpha;
+        };
+        std::vector<ColDst> dst(ncols);
+        std::vector<int> uidx(ncols);
+        std::map<std::tuple<const double*, const uint8_t*, bool>, int> seen;
+        WithinCols wc(0);
         int c = 0;
         for (size_t b = 0; b < chunk.size(); ++b) {
             ReqState& s = rs[chunk[b]];
             for (int j = 0; j < s.C; ++j, ++c) {
                 const double* x = j == 0 ? s.d_y : (j <= s.K ? s.d_W + (int64_t)(j - 1) * s.ld
                                                              : s.d_Z + (int64_t)(j - 1 - s.K) * s.ld);
-                wc.x[c] = x;
-                wc.out[c] = s.work ? s.work + (int64_t)j * n : nullptr;
-                wc.out32[c] = s.work32 ? s.work32 + (int64_t)j * n : nullptr;
-                wc.alpha[c] = s.alpha ? s.alpha + (int64_t)j * s.S : nullptr;
-                wc.slot[c] = wm ? (int)b : 0;
+                dst[c] = {s.work ? s.work + (int64_t)j * n : nullptr, s.work32 ? s.work32 + (int64_t)j * n : nullptr,
+                          s.alpha ? s.alpha + (int64_t)j * s.S : nullptr};
+                const auto key = std::make_tuple(x, s.d_weights, s.work32 != nullptr);
+                auto it = seen.find(key);
+                if (it != seen.end() && (dst[it->second].alpha != nullptr) == (dst[c].alpha != nullptr)) {
+                    uidx[c] = it->second;
+                    continue;
+                }
+                seen[key] = c;
+                uidx[c] = c;
+                wc.x.push_back(x);
+                wc.out.push_back(dst[c].out);
+                wc.out32.push_back(dst[c].out32);
+                wc.alpha.push_back(dst[c].alpha);
+                wc.slot.push_back(wm ? (int)b : 0);
             }
         }
+        const int nu = (int)wc.x.size();
+        std::vector<int> upos(ncols, -1);  // position of each solved column in wc
+        for (int c2 = 0, u = 0; c2 < ncols; ++c2)
+            if (uidx[c2] == c2) upos[c2] = u++;
         fehip_demean_opts dopt = rs[i0].opts->demean;
         dopt.fe_mask = rs[i0].fe_mask;
+        WithinStats wsu(nu);
+        if ((e = within_transform(p, wc, nu, wm, wm ? (int)chunk.size() : 1, dopt, 1, wsu, 0))) return e;
         WithinStats ws(ncols);
-        if ((e = within_transform(p, wc, ncols, wm, wm ? (int)chunk.size() : 1, dopt, 1, ws, 0))) return e;
+        for (int c2 = 0; c2 < ncols; ++c2) {
+            const int src = uidx[c2], u = upos[src];
+            ws.col_iters[c2] = wsu.col_iters[u];
+            ws.col_converged[c2] = wsu.col_converged[u];
+            ws.col_max_update[c2] = wsu.col_max_update[u];
+            if (src == c2) continue;
+            if (dst[c2].out)
+                FEHIP_CHECK(hipMemcpyAsync(dst[c2].out, dst[src].out, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, 0),
+                            "shared column");
+            if (dst[c2].out32)
+                FEHIP_CHECK(hipMemcpyAsync(dst[c2].out32, dst[src].out32, sizeof(float) * (size_t)n,
+                                           hipMemcpyDeviceToDevice, 0),
+                            "shared column");
+            if (dst[c2].alpha && dst[src].alpha)
+                FEHIP_CHECK(hipMemcpyAsync(dst[c2].alpha, dst[src].alpha, sizeof(double) * (size_t)within_table_size(p, rs[i0].fe_mask),
+                                           hipMemcpyDeviceToDevice, 0),
+                            "shared tables");
+        }
         const auto t1 = clk::now();
         c = 0;
         for (int i : chunk) {

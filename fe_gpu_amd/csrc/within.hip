@@ -1231,6 +1231,20 @@ int slot_diagonal(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm, ui
     return 0;
 }
 
+// Per-thread stream for capturing CG steps (the legacy default stream cannot be captured).  It is a
+// blocking stream, so its work is ordered with the default-stream work around it.
+hipStream_t graph_stream() {
+    struct Holder {
+        hipStream_t s = nullptr;
+        ~Holder() {
+            if (s) (void)hipStreamDestroy(s);
+        }
+    };
+    thread_local Holder h;
+    if (!h.s && hipStreamCreate(&h.s) != hipSuccess) h.s = nullptr;
+    return h.s;
+}
+
 template <int CT>
 int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int c0, int ncols, const uint32_t* wm,
                const fehip_demean_opts& o, int to_work, TileStats& ts, hipStream_t s) {
@@ -1272,34 +1286,70 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
                            (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, ncols, sc.st);
         FEHIP_LAUNCH_CHECK("cg init");
         const int every = o.check_every > 0 ? o.check_every : 4;
+        a.p = pp;
+        a.active = sc.st->active;
+        // one CG step: operator product, p.q, alpha, update, r.z, beta, new direction
+        auto step = [&](hipStream_t st) -> int {
+            int e2 = apply_and_reduce<CT>(p, L, a, kIter, q, st);
+            if (e2) return e2;
+            hipLaunchKernelGGL((cg_pq_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S, (const double*)pp,
+                               (const double*)q, sc.part);
+            hipLaunchKernelGGL((cg_alpha_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part, kCgBlocks,
+                               sc.st);
+            hipLaunchKernelGGL((cg_update_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S, (const CgState*)sc.st,
+                               (const double*)pp, (const double*)q, (const double*)minv, (const double*)wcnt, xa, r, z,
+                               sc.part, sc.part + kCgBlocks * CT);
+            hipLaunchKernelGGL((cg_beta_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part,
+                               (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, sc.st);
+            hipLaunchKernelGGL((cg_pdir_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, st, S,
+                               (const CgState*)sc.st, (const double*)z, pp);
+            FEHIP_LAUNCH_CHECK("cg step");
+            return 0;
+        };
+        // The step is captured once into a HIP graph and replayed, so an iteration costs one launch (the
+        // step holds no memset and no host work).  With per-kernel timing on, the step is launched
+        // directly so the kernels can be timed one by one.
+        hipStream_t gs = s;
+        hipGraphExec_t gexec = nullptr;
+        struct GraphGuard {
+            hipGraphExec_t* ex;
+            ~GraphGuard() {
+                if (*ex) (void)hipGraphExecDestroy(*ex);
+            }
+        } gguard{&gexec};
+        if (!kstats_enabled() && o.max_iter > every) {
+            gs = graph_stream();
+            if (!gs) return set_error("cg: no capture stream");
+            hipGraph_t graph = nullptr;
+            FEHIP_CHECK(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal), "cg capture");
+            const int ec = step(gs);
+            const hipError_t he = hipStreamEndCapture(gs, &graph);
+            if (ec) {
+                if (graph) (void)hipGraphDestroy(graph);
+                return ec;
+            }
+            FEHIP_CHECK(he, "cg capture end");
+            const hipError_t hi = hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            FEHIP_CHECK(hi, "cg graph");
+        }
         int it = 0;
         for (; it < o.max_iter; ++it) {
             if (it % every == 0) {
-                FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, s), "cg state");
-                FEHIP_CHECK(hipStreamSynchronize(s), "cg sync");
+                FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs), "cg state");
+                FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
                 bool any = false;
                 for (int c = 0; c < CT; ++c) any |= sc.st_host->active[c] != 0;
                 if (!any) break;
             }
-            a.p = pp;
-            a.active = sc.st->active;
-            e = apply_and_reduce<CT>(p, L, a, kIter, q, s);
-            if (e) return e;
-            hipLaunchKernelGGL((cg_pq_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, (const double*)pp,
-                               (const double*)q, sc.part);
-            hipLaunchKernelGGL((cg_alpha_scalars<CT>), dim3(1), dim3(kBlock), 0, s, (const double*)sc.part, kCgBlocks,
-                               sc.st);
-            hipLaunchKernelGGL((cg_update_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, (const CgState*)sc.st,
-                               (const double*)pp, (const double*)q, (const double*)minv, (const double*)wcnt, xa, r, z,
-                               sc.part, sc.part + kCgBlocks * CT);
-            hipLaunchKernelGGL((cg_beta_scalars<CT>), dim3(1), dim3(kBlock), 0, s, (const double*)sc.part,
-                               (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, sc.st);
-            hipLaunchKernelGGL((cg_pdir_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, s, S,
-                               (const CgState*)sc.st, (const double*)z, pp);
-            FEHIP_LAUNCH_CHECK("cg step");
+            if (gexec) {
+                FEHIP_CHECK(hipGraphLaunch(gexec, gs), "cg graph launch");
+            } else if ((e = step(gs))) {
+                return e;
+            }
         }
-        FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, s), "cg state");
-        FEHIP_CHECK(hipStreamSynchronize(s), "cg sync");
+        FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs), "cg state");
+        FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
         for (int c = 0; c < ncols; ++c) {
             ts.iterations = std::max(ts.iterations, sc.st_host->iters[c]);
             if (sc.st_host->active[c]) ts.converged = 0;
