@@ -1752,7 +1752,9 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
             if (done[i] || rs[i].fe_mask != rs[i0].fe_mask || o.tol != lead.tol || o.max_iter != lead.max_iter ||
                 o.check_every != lead.check_every)
                 continue;
-            const double b = 8.0 * (double)n * (1 + std::max(0, rs[i].K) + std::max(0, rs[i].L)) + 2.0 * (double)n;
+            const bool f32 = o.precision == 1 && !(rs[i].opts->n_iv_reg > 0 && rs[i].L > 0);
+            const double b = (f32 ? 4.0 : 8.0) * (double)n * (1 + std::max(0, rs[i].K) + std::max(0, rs[i].L)) +
+                             2.0 * (double)n;
             if (!chunk.empty() && bytes + b > budget) break;
             bytes += b;
             chunk.push_back((int)i);

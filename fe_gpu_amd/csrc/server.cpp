@@ -62,11 +62,23 @@ struct Server {
     bool any_done = false;
 };
 
+// Device bytes a request holds while its batch is solved: the within-transformed columns (FP32 in the
+// mixed form), the panel-order weights and the solved tables of the mixed form.  The temporaries of
+// the normal-equation and sandwich step are taken one request at a time, so they are reserved once
+// per batch (batch_reserve), not per request.
 double request_bytes(const Panel* p, const fehip_request& q) {
     const double n = (double)p->n;
     const double cols = 1.0 + std::max(0, q.K) + std::max(0, q.L);
-    // work columns, panel-order weights and residual/score buffers, the row mask words
-    return 8.0 * n * cols + n + 8.0 * n * (1.0 + std::max(0, q.K)) + 4.0 * n;
+    const bool f32 = q.opts.demean.precision == 1 && !(q.opts.n_iv_reg > 0 && q.L > 0);
+    double groups = 0.0;
+    for (int d = 0; d < p->m; ++d) groups += p->d[d].G;
+    return (f32 ? 4.0 : 8.0) * n * cols + n + (f32 ? 8.0 * groups * cols : 0.0);
+}
+
+double batch_reserve(const Panel* p, const fehip_request& q) {
+    const double n = (double)p->n;
+    // mask words, residual and score rows of the widest request, refinement buffers
+    return 4.0 * n + 8.0 * n * (4.0 + std::max(0, q.K) + std::max(0, q.L));
 }
 
 void run_worker(Server* sv) {
@@ -78,11 +90,13 @@ void run_worker(Server* sv) {
             sv->cv_work.wait(lk, [&] { return sv->stop || !sv->queue.empty(); });
             if (sv->queue.empty()) return;  // stop requested and drained
             // admission: arrival order, up to max_batch requests and the HBM budget (the oldest always goes)
-            double bytes = 0.0;
+            double bytes = 0.0, reserve = 0.0;
             while (!sv->queue.empty() && (int)batch.size() < sv->max_batch) {
                 Ticket* t = sv->tickets[sv->queue.front()].get();
-                if (!batch.empty() && bytes + t->bytes > sv->budget) break;
+                const double res = std::max(reserve, batch_reserve(sv->panel, t->req));
+                if (!batch.empty() && bytes + t->bytes + res > sv->budget) break;
                 bytes += t->bytes;
+                reserve = res;
                 batch.push_back(t);
                 sv->queue.pop_front();
             }
