@@ -93,7 +93,7 @@ def serving_panel_leg(args, lib, pan, d_y, d_W, n):
     seq = [api.estimate_device(pan, q.y_ptr, q.W_ptr, q.K, **q.options) for q in plain]
     seq_s = time.perf_counter() - ts
     log(f"150M serving: sequential plain FP64 {len(plain)} requests in {seq_s:.2f} s")
-    srv = api.Server(pan)
+    srv = api.Server(pan, max_batch=args.server_max_batch, workers=args.server_workers)
     tc = time.perf_counter()
     tickets = [srv.submit(q) for q in reqs]
     outs = [srv.wait(t) for t in tickets]
@@ -111,7 +111,8 @@ def serving_panel_leg(args, lib, pan, d_y, d_W, n):
         "sequential_plain_fp64": {"regressions_per_s": len(plain) / seq_s, "seconds": seq_s},
         "server_mixed_precision": {"regressions_per_s": len(reqs) / srv_s, "seconds": srv_s,
                                    "p50_ms": st["p50_s"] * 1e3, "p99_ms": st["p99_s"] * 1e3,
-                                   "batches": st["batches"], "mean_batch": st["mean_batch"]},
+                                   "batches": st["batches"], "mean_batch": st["mean_batch"],
+                                   "workers": args.server_workers, "max_batch": args.server_max_batch or 32},
         "speedup": seq_s / srv_s,
         "beta_max_rel_err_vs_fp64": berr, "se_max_rel_err_vs_fp64": serr,
     }
@@ -208,7 +209,7 @@ def serving_leg(args, lib):
     seq_rate = n_seq / seq_s
     log(f"sequential: {n_seq} requests in {seq_s:.2f} s ({seq_rate:.1f}/s)")
     # closed loop: every request queued at once
-    srv = api.Server(pan)
+    srv = api.Server(pan, max_batch=args.server_max_batch, workers=args.server_workers)
     tc = time.perf_counter()
     tickets = [srv.submit(q) for q in reqs]
     outs = [srv.wait(t) for t in tickets]
@@ -251,6 +252,7 @@ def serving_leg(args, lib):
         "open_loop": {"requests": n_open, "arrival_rate_per_s": rate, "regressions_per_s": n_open / open_s,
                       "p50_ms": st_open["p50_s"] * 1e3, "p99_ms": st_open["p99_s"] * 1e3,
                       "batches": st_open["batches"], "mean_batch": st_open["mean_batch"]},
+        "server": {"workers": args.server_workers, "max_batch": args.server_max_batch or 32},
         "speedup_closed_vs_sequential": (len(reqs) / closed_s) / seq_rate,
         "beta_max_rel_err_vs_sequential": b_err,
         "se_max_rel_err_vs_sequential": s_err,
@@ -274,6 +276,8 @@ def main():
     ap.add_argument("--no-iv-wide", action="store_true")
     ap.add_argument("--no-serve-panel", action="store_true")
     ap.add_argument("--serve-panel-requests", type=int, default=32)
+    ap.add_argument("--server-workers", type=int, default=2, help="batches in flight at once")
+    ap.add_argument("--server-max-batch", type=int, default=0, help="requests per batch (0: 32)")
     ap.add_argument("--iv-individuals", type=int, default=1_000_000)
     ap.add_argument("--serving-only", action="store_true", help="run only the request-stream leg")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

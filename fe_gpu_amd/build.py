@@ -26,6 +26,8 @@ COMMON = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,
     "-Wall", "-Wno-unused-result",
+    # the default stream is per host thread, so the request server's workers overlap on the device
+    "-fgpu-default-stream=per-thread",
 ]
 
 

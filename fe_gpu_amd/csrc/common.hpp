@@ -43,11 +43,32 @@ inline unsigned stream_grid(size_t work_items, unsigned block, unsigned waves_pe
     return (unsigned)(g < 1 ? 1 : g);
 }
 
+// ---- device-synchronising calls and graph capture ----
+// hipFree, hipHostFree and hipDeviceSynchronize wait on every stream of the device.  The library runs
+// requests from several host threads at once (the request server), and one thread may be capturing a
+// CG step into a graph while another frees memory; the capture and these calls therefore exclude each
+// other through one process-wide mutex.  Code holding it must not allocate through the caching
+// allocator (whose release path takes it).
+std::mutex& capture_mutex();
+// Whether the calling thread replays the CG step from a captured HIP graph (default) or launches it
+// directly; threads that run concurrently with other device work switch capture off.
+void set_thread_capture(bool on);
+hipError_t locked_free(void* p);        // hipFree under capture_mutex
+hipError_t locked_host_free(void* p);   // hipHostFree under capture_mutex
+hipError_t locked_device_sync();        // hipDeviceSynchronize under capture_mutex
+
+// Blocking copy ordered on stream s (the calling thread's default stream unless given).  Used in
+// place of hipMemcpy, which the runtime rejects while any thread of the process captures a graph.
+inline hipError_t copy_sync(void* dst, const void* src, size_t bytes, hipMemcpyKind kind, hipStream_t s = 0) {
+    const hipError_t e = hipMemcpyAsync(dst, src, bytes, kind, s);
+    return e != hipSuccess ? e : hipStreamSynchronize(s);
+}
+
 // ---- grow-only workspace on the current device (library-internal temporaries) ----
 // Returns a device buffer of at least `bytes`; the pointer stays valid until the next call
 // that grows it.  All users run on the default stream, so stream order protects reuse.
 void* workspace(size_t bytes, int slot = 0);
-void release_workspaces();
+void release_workspaces();  // the calling thread's workspaces
 
 // Pinned host scalar buffer for D2H of small results.
 double* host_scalars();

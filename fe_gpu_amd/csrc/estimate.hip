@@ -263,7 +263,7 @@ __global__ __launch_bounds__(kBlock) void gram_rows_kernel(const TX* __restrict_
 int reduce_upper(const double* dpart, int nblk, int KT, int K, std::vector<double>& M) {
     const int NP = KT * (KT + 1) / 2;
     std::vector<double> h((size_t)nblk * NP);
-    FEHIP_CHECK(hipMemcpy(h.data(), dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost), "partials");
+    FEHIP_CHECK(copy_sync(h.data(), dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost), "partials");
     std::vector<double> tot(NP, 0.0);
     for (int b = 0; b < nblk; ++b)
         for (int e = 0; e < NP; ++e) tot[e] += h[(size_t)b * NP + e];
@@ -899,7 +899,7 @@ struct DevBuf {
 
 int host_sum_parts(const double* dpart, int nblk, int width, std::vector<double>& out) {
     std::vector<double> h((size_t)nblk * width);
-    FEHIP_CHECK(hipMemcpy(h.data(), dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost), "partials");
+    FEHIP_CHECK(copy_sync(h.data(), dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost), "partials");
     out.assign(width, 0.0);
     for (int b = 0; b < nblk; ++b)
         for (int k = 0; k < width; ++k) out[k] += h[(size_t)b * width + k];
@@ -936,6 +936,7 @@ void finalize_stats(fehip_result* r, int64_t n_obs) {
 // How clusters of a subset containing the primary lie inside the segments (cached per panel): compares
 // the subset's cluster count with its count of runs inside segments.
 int nest_mode(Panel* p, unsigned dims_mask, const int* const* kid, int nk, unsigned long long* dcount, int* mode) {
+    std::lock_guard<std::recursive_mutex> guard(p->mu);
     auto it = p->nest_modes.find(dims_mask);
     if (it != p->nest_modes.end()) {
         *mode = it->second;
@@ -951,7 +952,7 @@ int nest_mode(Panel* p, unsigned dims_mask, const int* const* kid, int nk, unsig
                            (const int*)p->d[p->P].ids, kid[0], nk > 1 ? kid[1] : nullptr, p->n, dcount);
         FEHIP_LAUNCH_CHECK("composite runs");
         unsigned long long runs = 0;
-        FEHIP_CHECK(hipMemcpy(&runs, dcount, sizeof(runs), hipMemcpyDeviceToHost), "runs copy");
+        FEHIP_CHECK(copy_sync(&runs, dcount, sizeof(runs), hipMemcpyDeviceToHost), "runs copy");
         m = part_s->n_clusters == p->n ? 2 : ((unsigned long long)part_s->n_clusters == runs ? 1 : 0);
     }
     p->nest_modes[dims_mask] = m;
@@ -990,12 +991,18 @@ int project_meat(const std::vector<double>& Mz, int m, const std::vector<double>
             for (auto p : v) cache_free(*p);
         }
     } fr{{&dM, &dP, &dT, &dX}};
-    FEHIP_CHECK(hipMemcpy(dM, Mz.data(), sizeof(double) * (size_t)m * m, hipMemcpyHostToDevice), "meat copy");
-    FEHIP_CHECK(hipMemcpy(dP, proj.data(), sizeof(double) * (size_t)m * k, hipMemcpyHostToDevice), "proj copy");
+    FEHIP_CHECK(copy_sync(dM, Mz.data(), sizeof(double) * (size_t)m * m, hipMemcpyHostToDevice), "meat copy");
+    FEHIP_CHECK(copy_sync(dP, proj.data(), sizeof(double) * (size_t)m * k, hipMemcpyHostToDevice), "proj copy");
     if ((e = gemm_impl(false, false, m, k, m, 1.0, dM, m, dP, m, 0.0, dT, m, false))) return e;
     if ((e = gemm_impl(true, false, k, k, m, 1.0, dP, m, dT, m, 0.0, dX, k, false))) return e;
-    FEHIP_CHECK(hipMemcpy(Mx.data(), dX, sizeof(double) * (size_t)k * k, hipMemcpyDeviceToHost), "meat copy");
+    FEHIP_CHECK(copy_sync(Mx.data(), dX, sizeof(double) * (size_t)k * k, hipMemcpyDeviceToHost), "meat copy");
     return 0;
+}
+
+int cached_nest_mode(Panel* p, unsigned dims_mask) {
+    std::lock_guard<std::recursive_mutex> guard(p->mu);
+    auto it = p->nest_modes.find(dims_mask);
+    return it == p->nest_modes.end() ? -1 : it->second;
 }
 
 // One request of a batch: its inputs, and what the preparation step leaves for the solve and the
@@ -1101,7 +1108,7 @@ int prepare_request(Panel* p, ReqState& s) {
             hipLaunchKernelGGL(count_kernel, dim3(stream_grid(p->d[d].G, kBlock)), dim3(kBlock), 0, 0, flags,
                                (int64_t)p->d[d].G, cnt);
             unsigned long long h = 0;
-            FEHIP_CHECK(hipMemcpy(&h, cnt, sizeof(h), hipMemcpyDeviceToHost), "count");
+            FEHIP_CHECK(copy_sync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost), "count");
             Geff[d] = (int)h;
         }
     }
@@ -1189,7 +1196,7 @@ int finish_request(Panel* p, ReqState& s) {
             src = wide;
         }
         if ((e = gemm_impl(true, false, C, C, n, 1.0, src, n, src, n, 0.0, Gd, C, true))) return e;
-        FEHIP_CHECK(hipMemcpy(G.data(), Gd, G.size() * sizeof(double), hipMemcpyDeviceToHost), "gram copy");
+        FEHIP_CHECK(copy_sync(G.data(), Gd, G.size() * sizeof(double), hipMemcpyDeviceToHost), "gram copy");
         for (int j = 0; j < C; ++j)
             for (int i = j + 1; i < C; ++i) G[(size_t)j * C + i] = G[(size_t)i * C + j];
     }
@@ -1315,7 +1322,7 @@ int finish_request(Panel* p, ReqState& s) {
                 any_T |= !nested;
             }
         if (any_T && (e = db.get(&T_early, (size_t)n * K))) return e;
-        FEHIP_CHECK(hipMemcpy(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
+        FEHIP_CHECK(copy_sync(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
         const size_t tlds = (T_early && K <= 16) ? sizeof(double) * 256 * (size_t)K : 0;
         hipLaunchKernelGGL(resid_scores_kernel, dim3(kRedBlocks), dim3(256), tlds, 0, (const double*)work, n, K,
                            (const double*)dbeta, r, T_early, part);
@@ -1342,7 +1349,7 @@ int finish_request(Panel* p, ReqState& s) {
         std::vector<double> dbh(K, 0.0), gvec(C, 0.0), hx;
         const int sweeps = 1;
         for (int sw = 0; sw < sweeps; ++sw) {
-            FEHIP_CHECK(hipMemcpy(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
+            FEHIP_CHECK(copy_sync(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
             hipLaunchKernelGGL(combo_residual_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, d_y, d_W, ld, K,
                                (const double*)dbeta, n, ec);
             if (s.S > 0)
@@ -1370,7 +1377,7 @@ int finish_request(Panel* p, ReqState& s) {
             for (int k = 0; k < K; ++k) beta[k] += dbh[k];
         }
         // residual of the refined beta: e~ of the last sweep minus X~ times the last correction
-        FEHIP_CHECK(hipMemcpy(dbeta, dbh.data(), sizeof(double) * K, hipMemcpyHostToDevice), "correction copy");
+        FEHIP_CHECK(copy_sync(dbeta, dbh.data(), sizeof(double) * K, hipMemcpyHostToDevice), "correction copy");
         hipLaunchKernelGGL(resid_update_kernel<float>, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0,
                            (const double*)et, (const float*)work32, n, K, (const double*)dbeta, r);
         FEHIP_LAUNCH_CHECK("refined residual");
@@ -1394,7 +1401,7 @@ int finish_request(Panel* p, ReqState& s) {
         if (use_iv) {
             int* dcols = nullptr;
             if ((e = db.get(&dcols, (size_t)n_aug)) || (e = db.get(&Zaug, (size_t)n * n_aug))) return e;
-            FEHIP_CHECK(hipMemcpy(dcols, aug_col.data(), sizeof(int) * n_aug, hipMemcpyHostToDevice), "columns copy");
+            FEHIP_CHECK(copy_sync(dcols, aug_col.data(), sizeof(int) * n_aug, hipMemcpyHostToDevice), "columns copy");
             hipLaunchKernelGGL(gather_cols_kernel, dim3(stream_grid((size_t)n * n_aug, kBlock)), dim3(kBlock), 0, 0,
                                (const double*)work, n, (const int*)dcols, n_aug, Zaug);
             FEHIP_LAUNCH_CHECK("instrument columns");
@@ -1465,7 +1472,7 @@ int finish_request(Panel* p, ReqState& s) {
                 if ((e = nest_mode(p, dims_mask, wn.kid, wn.nk, ucount, &wn.mode))) return e;
             }
             if (has_p && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2 && single_done &&
-                p->nest_modes[dims_mask] == 2) {
+                cached_nest_mode(p, dims_mask) == 2) {
                 M = M_single;  // every row its own cluster: the same meat as an earlier subset
                 ncl = ncl_single;
             } else if (has_p && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2) {
@@ -1480,7 +1487,10 @@ int finish_request(Panel* p, ReqState& s) {
                 for (int d = 0; d < p->m; ++d)
                     if (d != p->P && ((dims_mask >> d) & 1u)) wn.kid[wn.nk++] = p->d[d].ids;
                 wn.w = wpanel;
-                wn.mode = p->nest_modes[dims_mask];
+                {
+                    std::lock_guard<std::recursive_mutex> guard(p->mu);
+                    wn.mode = p->nest_modes[dims_mask];
+                }
                 FEHIP_CHECK(hipMemsetAsync(ucount, 0, sizeof(unsigned long long), 0), "memset");
                 const unsigned grid = (unsigned)std::max<int64_t>(
                     1, std::min<int64_t>((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave), kMeatBlocks));
@@ -1503,7 +1513,7 @@ int finish_request(Panel* p, ReqState& s) {
                 FEHIP_LAUNCH_CHECK("wave nested meat");
                 if ((e = reduce_upper(mpart, (int)grid, KT, K, M))) return e;
                 unsigned long long h = 0;
-                FEHIP_CHECK(hipMemcpy(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
+                FEHIP_CHECK(copy_sync(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
                 ncl = (int64_t)h;
                 if (wn.mode == 2) {
                     M_single = M;
@@ -1530,7 +1540,7 @@ int finish_request(Panel* p, ReqState& s) {
                 FEHIP_LAUNCH_CHECK("nested meat");
                 if ((e = reduce_upper(mpart, (int)grid, KT, K, M))) return e;
                 unsigned long long h = 0;
-                FEHIP_CHECK(hipMemcpy(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
+                FEHIP_CHECK(copy_sync(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
                 ncl = (int64_t)h;
             } else if (all_small && cells <= kSmallCells && K <= 11) {
                 if ((e = need_T())) return e;
@@ -1560,7 +1570,7 @@ int finish_request(Panel* p, ReqState& s) {
                                    c * KT, Sc);
                 FEHIP_LAUNCH_CHECK("small scores");
                 std::vector<double> hS((size_t)c * KT);
-                FEHIP_CHECK(hipMemcpy(hS.data(), Sc, hS.size() * sizeof(double), hipMemcpyDeviceToHost), "S copy");
+                FEHIP_CHECK(copy_sync(hS.data(), Sc, hS.size() * sizeof(double), hipMemcpyDeviceToHost), "S copy");
                 M.assign((size_t)K * K, 0.0);
                 for (int q = 0; q < c; ++q) {
                     if (hS[(size_t)q * KT + KT - 1] > 0.0) ++ncl;
@@ -1641,7 +1651,7 @@ int finish_request(Panel* p, ReqState& s) {
                         e = cluster_sums<12>(bval, KT, nullptr, bkey, 2 * nwaves, Sb, dcount, 0);
                     if (e) return e;
                     int64_t n_groups = 0;
-                    FEHIP_CHECK(hipMemcpy(&n_groups, dcount, sizeof(int64_t), hipMemcpyDeviceToHost), "count copy");
+                    FEHIP_CHECK(copy_sync(&n_groups, dcount, sizeof(int64_t), hipMemcpyDeviceToHost), "count copy");
                     std::vector<double> Mb;
                     if ((e = meat_fused(Sb, K, KT, nullptr, nullptr, n_groups, mpart, Mb))) return e;
                     for (size_t i2 = 0; i2 < M.size(); ++i2) M[i2] += Mb[i2];
@@ -1655,7 +1665,7 @@ int finish_request(Panel* p, ReqState& s) {
                                        (const double*)T, K, ord, off, nS, S);
                     FEHIP_LAUNCH_CHECK("segsum");
                     if ((e = gemm_impl(false, true, K, K, nS, 1.0, S, K, S, K, 0.0, Md, K, true))) return e;
-                    FEHIP_CHECK(hipMemcpy(M.data(), Md, M.size() * sizeof(double), hipMemcpyDeviceToHost), "meat copy");
+                    FEHIP_CHECK(copy_sync(M.data(), Md, M.size() * sizeof(double), hipMemcpyDeviceToHost), "meat copy");
                     for (int j = 0; j < K; ++j)
                         for (int i = j + 1; i < K; ++i) M[(size_t)j * K + i] = M[(size_t)i * K + j];
                 }
@@ -1669,7 +1679,7 @@ int finish_request(Panel* p, ReqState& s) {
                     hipLaunchKernelGGL(count_kernel, dim3(stream_grid((size_t)nS + 2, kBlock)), dim3(kBlock), 0, 0,
                                        mark, (int64_t)nS + 2, ucount);
                     unsigned long long h = 0;
-                    FEHIP_CHECK(hipMemcpy(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
+                    FEHIP_CHECK(copy_sync(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
                     ncl = (int64_t)h;
                 }
             }
@@ -1717,7 +1727,7 @@ int finish_request(Panel* p, ReqState& s) {
         for (int a = 0; a < kept; ++a)
             for (int c = 0; c < kept; ++c) vcov[(size_t)keep[c] * K + keep[a]] = cov[(size_t)c * kept + a];
     finalize_stats(result, N);
-    FEHIP_CHECK(hipDeviceSynchronize(), "estimate sync");
+    FEHIP_CHECK(hipStreamSynchronize(0), "estimate sync");
     result->t_se = std::chrono::duration<double>(clk::now() - t2).count();
     return 0;
 }

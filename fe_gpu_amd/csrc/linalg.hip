@@ -412,7 +412,7 @@ int fe_gpu_dot(int n_rows, const double* x, const double* y, double* result) {
     FEHIP_LAUNCH_CHECK("dot kernel");
     hipLaunchKernelGGL(sum_partials_kernel, dim3(1), dim3(kBlock), 0, 0, part, (int)grid, part + grid);
     FEHIP_LAUNCH_CHECK("dot reduce kernel");
-    FEHIP_CHECK(hipMemcpy(result, part + grid, sizeof(double), hipMemcpyDeviceToHost), "dot copy");
+    FEHIP_CHECK(copy_sync(result, part + grid, sizeof(double), hipMemcpyDeviceToHost), "dot copy");
     return set_ok();
 }
 

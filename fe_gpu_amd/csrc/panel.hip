@@ -292,7 +292,7 @@ int build_runs(Panel* p, int d, hipStream_t s) {
 int build_tiles(Panel* p) {
     const int64_t GP = p->d[p->P].G;
     std::vector<int64_t> off((size_t)GP + 1);
-    FEHIP_CHECK(hipMemcpy(off.data(), p->seg_off, sizeof(int64_t) * (GP + 1), hipMemcpyDeviceToHost), "seg_off copy");
+    FEHIP_CHECK(copy_sync(off.data(), p->seg_off, sizeof(int64_t) * (GP + 1), hipMemcpyDeviceToHost), "seg_off copy");
     std::vector<int64_t> tiles, longs;
     tiles.reserve((size_t)(p->n / kTileRows + 16) * 2);
     int64_t s = 0;
@@ -345,17 +345,17 @@ int build_tiles(Panel* p) {
     if ((st = dalloc(&p->wtile_seg, wt.size() + 2, &p->bytes)) || (st = dalloc(&p->mtile_seg, mt.size() + 2, &p->bytes)) ||
         (st = dalloc(&p->wtile_rec, rec.size(), &p->bytes)))
         return st;
-    FEHIP_CHECK(hipMemcpy(p->wtile_rec, rec.data(), rec.size() * sizeof(int64_t), hipMemcpyHostToDevice), "wave records");
+    FEHIP_CHECK(copy_sync(p->wtile_rec, rec.data(), rec.size() * sizeof(int64_t), hipMemcpyHostToDevice), "wave records");
     if (!wt.empty())
-        FEHIP_CHECK(hipMemcpy(p->wtile_seg, wt.data(), wt.size() * sizeof(int64_t), hipMemcpyHostToDevice), "wave tiles");
+        FEHIP_CHECK(copy_sync(p->wtile_seg, wt.data(), wt.size() * sizeof(int64_t), hipMemcpyHostToDevice), "wave tiles");
     if (!mt.empty())
-        FEHIP_CHECK(hipMemcpy(p->mtile_seg, mt.data(), mt.size() * sizeof(int64_t), hipMemcpyHostToDevice), "block tiles");
+        FEHIP_CHECK(copy_sync(p->mtile_seg, mt.data(), mt.size() * sizeof(int64_t), hipMemcpyHostToDevice), "block tiles");
     if ((st = dalloc(&p->tile_seg, tiles.size() + 2, &p->bytes)) || (st = dalloc(&p->long_segs, longs.size() + 1, &p->bytes)))
         return st;
     if (!tiles.empty())
-        FEHIP_CHECK(hipMemcpy(p->tile_seg, tiles.data(), tiles.size() * sizeof(int64_t), hipMemcpyHostToDevice), "tiles");
+        FEHIP_CHECK(copy_sync(p->tile_seg, tiles.data(), tiles.size() * sizeof(int64_t), hipMemcpyHostToDevice), "tiles");
     if (!longs.empty())
-        FEHIP_CHECK(hipMemcpy(p->long_segs, longs.data(), longs.size() * sizeof(int64_t), hipMemcpyHostToDevice),
+        FEHIP_CHECK(copy_sync(p->long_segs, longs.data(), longs.size() * sizeof(int64_t), hipMemcpyHostToDevice),
                     "long segments");
     return 0;
 }
@@ -363,6 +363,7 @@ int build_tiles(Panel* p) {
 }  // namespace
 
 int panel_partition(Panel* p, unsigned mask, const Partition** out, hipStream_t s) {
+    std::lock_guard<std::recursive_mutex> guard(p->mu);
     auto it = p->partitions.find(mask);
     if (it != p->partitions.end()) {
         *out = &it->second;
@@ -465,10 +466,10 @@ int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehi
     }
     for (int d = 0; d < n_fe; ++d) {
         int init[2] = {0x7fffffff, -0x7fffffff};
-        FEHIP_CHECK(hipMemcpy(mm, init, sizeof(init), hipMemcpyHostToDevice), "init");
+        FEHIP_CHECK(copy_sync(mm, init, sizeof(init), hipMemcpyHostToDevice), "init");
         hipLaunchKernelGGL(minmax_kernel, dim3(stream_grid(n_obs, kBlock)), dim3(kBlock), 0, s, d_fe_ids[d], n_obs, mm);
         int h[2];
-        FEHIP_CHECK(hipMemcpy(h, mm, sizeof(h), hipMemcpyDeviceToHost), "minmax");
+        FEHIP_CHECK(copy_sync(h, mm, sizeof(h), hipMemcpyDeviceToHost), "minmax");
         if (h[0] < 1) {
             dfree(mm);
             free_panel(p);
@@ -482,9 +483,9 @@ int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehi
         if (p->d[d].G > p->d[p->P].G) p->P = d;
     // is the input already ordered by the primary dim?
     int zero = 0, unsorted = 0;
-    FEHIP_CHECK(hipMemcpy(mm, &zero, sizeof(int), hipMemcpyHostToDevice), "init");
+    FEHIP_CHECK(copy_sync(mm, &zero, sizeof(int), hipMemcpyHostToDevice), "init");
     hipLaunchKernelGGL(unsorted_kernel, dim3(stream_grid(n_obs, kBlock)), dim3(kBlock), 0, s, d_fe_ids[p->P], n_obs, mm);
-    FEHIP_CHECK(hipMemcpy(&unsorted, mm, sizeof(int), hipMemcpyDeviceToHost), "sorted flag");
+    FEHIP_CHECK(copy_sync(&unsorted, mm, sizeof(int), hipMemcpyDeviceToHost), "sorted flag");
     dfree(mm);
     p->sorted_input = !unsorted;
     if (unsorted) {
@@ -541,7 +542,7 @@ int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehi
         free_panel(p);
         return st;
     }
-    FEHIP_CHECK(hipMemcpy(&p->max_seg, maxseg, sizeof(int), hipMemcpyDeviceToHost), "max seg");
+    FEHIP_CHECK(copy_sync(&p->max_seg, maxseg, sizeof(int), hipMemcpyDeviceToHost), "max seg");
     dfree(maxseg);
     if ((st = build_tiles(p))) {
         free_panel(p);
@@ -554,14 +555,14 @@ int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehi
             return st;
         }
     }
-    FEHIP_CHECK(hipDeviceSynchronize(), "panel build");
+    FEHIP_CHECK(locked_device_sync(), "panel build");
     *out = reinterpret_cast<fehip_panel*>(p);
     return set_ok();
 }
 
 int fehip_panel_destroy(fehip_panel* panel) {
     if (!panel) return set_ok();
-    FEHIP_CHECK(hipDeviceSynchronize(), "panel destroy");
+    FEHIP_CHECK(locked_device_sync(), "panel destroy");
     free_panel(reinterpret_cast<Panel*>(panel));
     return set_ok();
 }

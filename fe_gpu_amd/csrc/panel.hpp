@@ -3,6 +3,7 @@
 #pragma once
 
 #include <map>
+#include <mutex>
 #include <vector>
 
 #include "../../include/fehip.h"
@@ -40,6 +41,7 @@ struct Partition {
 };
 
 struct Panel {
+    std::recursive_mutex mu;   // guards the structures built lazily on first use (partitions, caches)
     int device = 0;            // HIP device holding the panel
     int64_t n = 0;
     int m = 0;
@@ -105,7 +107,7 @@ int talloc(T** p, size_t count) {
 
 template <typename T>
 void dfree(T*& p) {
-    if (p) (void)hipFree((void*)p);
+    if (p) (void)locked_free((void*)p);
     p = nullptr;
 }
 

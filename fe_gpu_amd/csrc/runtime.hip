@@ -20,8 +20,8 @@ struct Workspace {
     void* ptr = nullptr;
     size_t bytes = 0;
 };
-// [device][slot]
-std::vector<std::vector<Workspace>> g_ws;
+// [device][slot], per host thread (each thread's work runs on its own default stream)
+thread_local std::vector<std::vector<Workspace>> g_ws;
 double* g_host_scalars = nullptr;
 int g_num_cus = 0;
 int g_num_cus_dev = -1;
@@ -60,6 +60,28 @@ int num_cus() {
     return g_num_cus;
 }
 
+std::mutex& capture_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+hipError_t locked_free(void* p) {
+    if (!p) return hipSuccess;
+    std::lock_guard<std::mutex> lock(capture_mutex());
+    return hipFree(p);
+}
+
+hipError_t locked_host_free(void* p) {
+    if (!p) return hipSuccess;
+    std::lock_guard<std::mutex> lock(capture_mutex());
+    return hipHostFree(p);
+}
+
+hipError_t locked_device_sync() {
+    std::lock_guard<std::mutex> lock(capture_mutex());
+    return hipDeviceSynchronize();
+}
+
 void* workspace(size_t bytes, int slot) {
     std::lock_guard<std::mutex> lock(g_ws_mutex);
     int dev = 0;
@@ -71,8 +93,8 @@ void* workspace(size_t bytes, int slot) {
     if (bytes < 256) bytes = 256;
     if (w.bytes >= bytes) return w.ptr;
     if (w.ptr) {
-        if (hipDeviceSynchronize() != hipSuccess) return nullptr;
-        if (hipFree(w.ptr) != hipSuccess) return nullptr;
+        if (hipStreamSynchronize(0) != hipSuccess) return nullptr;
+        if (locked_free(w.ptr) != hipSuccess) return nullptr;
         w.ptr = nullptr;
         w.bytes = 0;
     }
@@ -93,7 +115,7 @@ void release_workspaces() {
         if (hipSetDevice((int)d) != hipSuccess) continue;
         for (auto& w : g_ws[d])
             if (w.ptr) {
-                (void)hipFree(w.ptr);
+                (void)locked_free(w.ptr);
                 w.ptr = nullptr;
                 w.bytes = 0;
             }
@@ -121,7 +143,7 @@ size_t cache_round(size_t b) {
 void cache_drop_device(int dev) {  // caller holds g_cache_mutex
     for (auto it = g_cache_free.begin(); it != g_cache_free.end();) {
         if (it->first.first == dev) {
-            (void)hipFree(it->second);
+            (void)locked_free(it->second);
             it = g_cache_free.erase(it);
         } else {
             ++it;
@@ -167,7 +189,7 @@ void cache_release() {
     if (hipGetDevice(&cur) != hipSuccess) cur = 0;
     for (auto& kv : g_cache_free) {
         (void)hipSetDevice(kv.first.first);
-        (void)hipFree(kv.second);
+        (void)locked_free(kv.second);
     }
     g_cache_free.clear();
     (void)hipSetDevice(cur);
@@ -188,12 +210,12 @@ void* pinned_scratch(size_t bytes) {
         void* p = nullptr;
         size_t bytes = 0;
         ~Pinned() {
-            if (p) (void)hipHostFree(p);
+            if (p) (void)locked_host_free(p);
         }
     };
     thread_local Pinned buf;
     if (buf.bytes >= bytes) return buf.p;
-    if (buf.p) (void)hipHostFree(buf.p);
+    if (buf.p) (void)locked_host_free(buf.p);
     buf.p = nullptr;
     buf.bytes = 0;
     if (hipHostMalloc(&buf.p, bytes, hipHostMallocDefault) != hipSuccess) return nullptr;

@@ -1,11 +1,12 @@
 // Request server over a device-resident panel.
 //
-// Callers submit regression requests (fehip_request) from any thread; a worker thread bound to the
-// panel's device keeps taking the queued requests in arrival order, admits as many as fit the slot
-// limit and the HBM budget (device bytes the requests' work buffers and row masks will hold), and
-// solves them as one batch: their columns share the within-transform sweeps.  Requests that arrive
-// while a batch is on the device wait for the next batch; a request too large for the budget on its
-// own is run alone.  Latency is measured from submit to completion.
+// Callers submit regression requests (fehip_request) from any thread; worker threads bound to the
+// panel's device keep taking the queued requests in arrival order, each admitting as many as fit the
+// slot limit and the HBM budget left by the batches already on the device (device bytes the requests'
+// work buffers, masks and tables will hold), and solve them as one batch: their columns share the
+// within-transform sweeps.  Workers run on their own streams, so a batch that arrives while another
+// is on the device is admitted as soon as memory allows and runs alongside it.  Latency is measured
+// from submit to completion.
 //
 // The reference has no serving layer: it estimates one specification per process run
 // (src/src_main.f90); each request here follows the reference's estimator exactly as
@@ -46,6 +47,7 @@ struct Ticket {
 struct Server {
     Panel* panel = nullptr;
     double budget = 0.0;
+    double inflight = 0.0;  // bytes held by the batches on the device
     int max_batch = 32;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -53,7 +55,7 @@ struct Server {
     std::unordered_map<int64_t, std::unique_ptr<Ticket>> tickets;
     int64_t next_ticket = 1;
     bool stop = false;
-    std::thread worker;
+    std::vector<std::thread> workers;
     // statistics
     int64_t submitted = 0, completed = 0, failed = 0, batches = 0, batched_requests = 0;
     double busy_s = 0.0;
@@ -81,25 +83,43 @@ double batch_reserve(const Panel* p, const fehip_request& q) {
     return 4.0 * n + 8.0 * n * (4.0 + std::max(0, q.K) + std::max(0, q.L));
 }
 
-void run_worker(Server* sv) {
+// Each worker runs its batches on its own default stream, so batches of different workers overlap on
+// the device.  A worker admits the oldest queued requests that fit the budget left by the batches in
+// flight; with nothing in flight the oldest request always goes.
+void run_worker(Server* sv, bool capture) {
     (void)hipSetDevice(sv->panel->device);
+    set_thread_capture(capture);
     for (;;) {
         std::vector<Ticket*> batch;
+        double held = 0.0;
         {
             std::unique_lock<std::mutex> lk(sv->mu);
-            sv->cv_work.wait(lk, [&] { return sv->stop || !sv->queue.empty(); });
-            if (sv->queue.empty()) return;  // stop requested and drained
-            // admission: arrival order, up to max_batch requests and the HBM budget (the oldest always goes)
+            auto first_fits = [&] {
+                if (sv->queue.empty()) return false;
+                if (sv->inflight <= 0.0) return true;
+                const Ticket* t = sv->tickets[sv->queue.front()].get();
+                return sv->inflight + t->bytes + batch_reserve(sv->panel, t->req) <= sv->budget;
+            };
+            sv->cv_work.wait(lk, [&] { return (sv->stop && sv->queue.empty()) || first_fits(); });
+            if (sv->queue.empty()) {  // stop requested and drained
+                lk.unlock();
+                release_workspaces();  // this thread's workspaces
+                return;
+            }
+            // admission: arrival order, up to max_batch requests and the HBM budget
             double bytes = 0.0, reserve = 0.0;
             while (!sv->queue.empty() && (int)batch.size() < sv->max_batch) {
                 Ticket* t = sv->tickets[sv->queue.front()].get();
                 const double res = std::max(reserve, batch_reserve(sv->panel, t->req));
-                if (!batch.empty() && bytes + t->bytes + res > sv->budget) break;
+                if (!(batch.empty() && sv->inflight <= 0.0) && sv->inflight + bytes + t->bytes + res > sv->budget)
+                    break;
                 bytes += t->bytes;
                 reserve = res;
                 batch.push_back(t);
                 sv->queue.pop_front();
             }
+            held = bytes + reserve;
+            sv->inflight += held;
         }
         const auto t0 = clk::now();
         std::vector<fehip_request> reqs(batch.size());
@@ -119,6 +139,7 @@ void run_worker(Server* sv) {
         const auto t1 = clk::now();
         {
             std::lock_guard<std::mutex> lk(sv->mu);
+            sv->inflight -= held;
             sv->busy_s += std::chrono::duration<double>(t1 - t0).count();
             sv->batches += 1;
             sv->batched_requests += (int64_t)batch.size();
@@ -135,11 +156,12 @@ void run_worker(Server* sv) {
                     sv->latencies.push_back(std::chrono::duration<double>(t1 - t->t_submit).count());
                 }
             }
-            if (!sv->any_done) sv->t_first = t0;
+            if (!sv->any_done || t0 < sv->t_first) sv->t_first = t0;
             sv->any_done = true;
-            sv->t_last = t1;
+            if (t1 > sv->t_last) sv->t_last = t1;
         }
         sv->cv_done.notify_all();
+        sv->cv_work.notify_all();  // memory freed: another worker may admit
     }
 }
 
@@ -174,11 +196,19 @@ extern "C" int fehip_server_create(fehip_panel* panel, const fehip_server_opts* 
     sv.budget = (opts && opts->hbm_budget > 0) ? (double)opts->hbm_budget
                                                  : 0.8 * (double)(free_b + cache_idle_bytes());
     sv.max_batch = (opts && opts->max_batch > 0) ? std::min(opts->max_batch, 256) : 32;
+    const int nw = (opts && opts->workers > 0) ? std::min(opts->workers, 8) : 2;
     try {
-        sv.worker = std::thread(run_worker, &sv);
+        for (int w = 0; w < nw; ++w) sv.workers.emplace_back(run_worker, &sv, nw == 1);
     } catch (...) {
+        {
+            std::lock_guard<std::mutex> lk(sv.mu);
+            sv.stop = true;
+        }
+        sv.cv_work.notify_all();
+        for (auto& t : sv.workers)
+            if (t.joinable()) t.join();
         delete srv;
-        return set_error("fehip_server_create: cannot start the worker thread");
+        return set_error("fehip_server_create: cannot start the worker threads");
     }
     *out = srv;
     return set_ok();
@@ -270,7 +300,8 @@ extern "C" int fehip_server_destroy(fehip_server* srv) {
         srv->s.stop = true;
     }
     srv->s.cv_work.notify_all();
-    if (srv->s.worker.joinable()) srv->s.worker.join();
+    for (auto& t : srv->s.workers)
+        if (t.joinable()) t.join();
     delete srv;
     return set_ok();
 }

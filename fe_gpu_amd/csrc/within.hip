@@ -1231,8 +1231,13 @@ int slot_diagonal(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm, ui
     return 0;
 }
 
-// Per-thread stream for capturing CG steps (the legacy default stream cannot be captured).  It is a
-// blocking stream, so its work is ordered with the default-stream work around it.
+// Per-thread stream for capturing CG steps.  Its work is ordered after the thread's default stream by
+// an event before the loop, and the host waits for it after the loop.
+// Graph capture of the CG step is per thread: while a capture is open the runtime rejects blocking
+// calls (hipMemcpy, hipMalloc, ...) from every thread of the process, so threads that run alongside
+// other device work (the request server's workers) solve with direct launches.
+thread_local bool t_capture = true;
+
 hipStream_t graph_stream() {
     struct Holder {
         hipStream_t s = nullptr;
@@ -1241,7 +1246,7 @@ hipStream_t graph_stream() {
         }
     };
     thread_local Holder h;
-    if (!h.s && hipStreamCreate(&h.s) != hipSuccess) h.s = nullptr;
+    if (!h.s && hipStreamCreateWithFlags(&h.s, hipStreamNonBlocking) != hipSuccess) h.s = nullptr;
     return h.s;
 }
 
@@ -1317,10 +1322,17 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
                 if (*ex) (void)hipGraphExecDestroy(*ex);
             }
         } gguard{&gexec};
-        if (!kstats_enabled() && o.max_iter > every) {
+        if (t_capture && !kstats_enabled() && o.max_iter > every) {
             gs = graph_stream();
             if (!gs) return set_error("cg: no capture stream");
+            // the capture stream starts after everything already queued on this thread's stream
+            hipEvent_t ready;
+            FEHIP_CHECK(hipEventCreateWithFlags(&ready, hipEventDisableTiming), "cg event");
+            FEHIP_CHECK(hipEventRecord(ready, s), "cg event");
+            FEHIP_CHECK(hipStreamWaitEvent(gs, ready, 0), "cg event");
+            (void)hipEventDestroy(ready);
             hipGraph_t graph = nullptr;
+            std::lock_guard<std::mutex> capture_lock(capture_mutex());
             FEHIP_CHECK(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal), "cg capture");
             const int ec = step(gs);
             const hipError_t he = hipStreamEndCapture(gs, &graph);
@@ -1428,6 +1440,7 @@ int plan_tiles(Panel* p, const Layout& L, Scratch& sc, unsigned mask, int ncols,
         for (int c0 = 0; c0 < ncols; c0 += ct_max) tiles.push_back(round_up(std::min(ct_max, ncols - c0)));
         return 0;
     }
+    std::lock_guard<std::recursive_mutex> guard(p->mu);
     auto it = p->width_ms.find(mask);
     if (it == p->width_ms.end()) {
         std::vector<float> t(kNumWidths, -1.0f);
@@ -1482,6 +1495,8 @@ int plan_tiles(Panel* p, const Layout& L, Scratch& sc, unsigned mask, int ncols,
 }
 
 }  // namespace
+
+void set_thread_capture(bool on) { t_capture = on; }
 
 int build_mask_words(Panel* p, const uint8_t* const* w_caller, int nslots, uint32_t* wm, hipStream_t s) {
     if (nslots < 1 || nslots > 32) return set_error("1..32 request slots per batch (%d)", nslots);
@@ -1574,6 +1589,7 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
     // Schur diagonal of every slot in use (an unweighted solve's diagonal depends only on the panel and
     // the absorbed dims, and is kept with the panel)
     if (S > 0 && !wm) {
+        std::lock_guard<std::recursive_mutex> guard(p->mu);
         auto it = p->diag_cache.find(mask);
         if (it != p->diag_cache.end()) {
             sc.dg = it->second;

@@ -129,17 +129,18 @@ typedef struct {
 int fehip_estimate_batch(fehip_panel* panel, const fehip_request* reqs, int n_req);
 
 /* Request server over a device-resident panel (no counterpart in the reference, which runs one
- * specification per process).  A worker thread on the panel's device takes queued requests in
- * arrival order and solves as many as fit max_batch and the HBM budget as one batch
- * (fehip_estimate_batch semantics); requests that arrive meanwhile join the next batch.  The
- * request's device inputs and host outputs must stay valid until fehip_server_wait returns for
- * its ticket; the option arrays (iv_reg, iv_inst) are copied at submit. */
+ * specification per process).  Worker threads on the panel's device take queued requests in arrival
+ * order; each solves as many as fit max_batch and the HBM budget left by the batches in flight as one
+ * batch (fehip_estimate_batch semantics), on its own stream, so requests that arrive while a batch is
+ * on the device start as soon as memory allows and overlap with it.  The request's device inputs and
+ * host outputs must stay valid until fehip_server_wait returns for its ticket; the option arrays
+ * (iv_reg, iv_inst) are copied at submit. */
 typedef struct fehip_server fehip_server;
 
 typedef struct {
-    int64_t hbm_budget;  /* device bytes the requests of one batch may hold; 0 -> 80% of free memory */
+    int64_t hbm_budget;  /* device bytes the batches in flight may hold; 0 -> 80% of free memory */
     int32_t max_batch;   /* requests per batch; 0 -> 32 */
-    int32_t reserved;
+    int32_t workers;     /* batches in flight at once (one host thread and stream each); 0 -> 2 */
 } fehip_server_opts;
 
 typedef struct {

@@ -130,13 +130,15 @@ def test_batch_beyond_32_slots_and_duplicates():
     pan.close()
 
 
-def test_server_results_same_alone_and_in_the_mix():
-    """Requests submitted back to back are batched by the server (fewer batches than requests);
-    each result equals the same request solved alone."""
+@pytest.mark.parametrize("workers", [1, 3])
+def test_server_results_same_alone_and_in_the_mix(workers):
+    """Requests submitted back to back are batched by the server (fewer batches than requests),
+    with one worker or several whose batches overlap on the device; each result equals the same
+    request solved alone."""
     import threading
     from fe_gpu_amd import api, synth
     from fe_gpu_amd.device import DeviceBuffer
-    note("server mix")
+    note(f"server mix workers={workers}")
     p = synth.random_fe_problem(406, n=6000, K=6, L=3, G=(1500, 800, 6))
     n = p.n
     rng = np.random.default_rng(11)
@@ -153,7 +155,7 @@ def test_server_results_same_alone_and_in_the_mix():
             opt["iv_reg"] = [1, 3]
         reqs.append(api.Request(dy.ptr, dW.ptr + 8 * n * j0, k, dZ.ptr if iv else 0, 3 if iv else 0, n,
                                 dw.ptr if dw else 0, opt))
-    srv = api.Server(pan, max_batch=8)
+    srv = api.Server(pan, max_batch=8, workers=workers)
     tickets = [None] * len(reqs)
 
     def client(idx):
@@ -169,7 +171,9 @@ def test_server_results_same_alone_and_in_the_mix():
     st = srv.stats()
     srv.close()
     assert st["completed"] == len(reqs) and st["failed"] == 0
-    assert st["batches"] < len(reqs) and st["p99_s"] >= st["p50_s"] > 0.0
+    assert st["batches"] <= len(reqs) and st["p99_s"] >= st["p50_s"] > 0.0
+    if workers == 1:
+        assert st["batches"] < len(reqs)
     for i, (q, (fit, lat)) in enumerate(zip(reqs, outs)):
         alone = api.estimate_device(pan, q.y_ptr, q.W_ptr, q.K, q.Z_ptr, q.L, n, q.weights_ptr, **q.options)
         for key in INT_KEYS:
