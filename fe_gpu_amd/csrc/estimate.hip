@@ -66,7 +66,8 @@ __global__ __launch_bounds__(kBlock) void wsum2_kernel(const double* __restrict_
 // r = y~ - W~ beta on the work buffer, r'r partials per block, and (T != nullptr) the score rows
 // T = r W~ (row-major n x K) in the same pass over the columns.  With K <= 16 the block's score rows
 // are staged in LDS and leave as one contiguous stretch.
-__global__ __launch_bounds__(256) void resid_scores_kernel(const double* __restrict__ work, int64_t n, int K,
+template <typename TX>
+__global__ __launch_bounds__(256) void resid_scores_kernel(const TX* __restrict__ work, int64_t n, int K,
                                                            const double* __restrict__ beta, double* r, double* T,
                                                            double* part) {
     extern __shared__ double tstage[];  // [256][K] when staging
@@ -76,13 +77,13 @@ __global__ __launch_bounds__(256) void resid_scores_kernel(const double* __restr
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
         const int64_t i = i0 + threadIdx.x;
         if (i < n) {
-            double v = work[i];
-            for (int k = 0; k < K; ++k) v -= work[i + (int64_t)(1 + k) * n] * beta[k];
+            double v = (double)work[i];
+            for (int k = 0; k < K; ++k) v -= (double)work[i + (int64_t)(1 + k) * n] * beta[k];
             r[i] = v;
             ss += v * v;
             if (T) {
                 for (int k = 0; k < K; ++k) {
-                    const double t = v * work[i + (int64_t)(1 + k) * n];
+                    const double t = v * (double)work[i + (int64_t)(1 + k) * n];
                     if (stage)
                         tstage[threadIdx.x * K + k] = t;
                     else
@@ -1032,6 +1033,9 @@ struct ReqState {
     float* work32 = nullptr;   // the same stored in FP32 (mixed mode)
     double* alpha = nullptr;   // mixed mode: solved FE tables of every column, [C][S]
     int64_t S = 0;
+    // FP64 cross products of the C transformed columns (C x C, column-major) formed in the final pass
+    // of the within-transform, when all of the request's columns were finalized in one column tile
+    std::vector<double> gram;
     clk::time_point t0, t1;
 };
 
@@ -1178,7 +1182,10 @@ int finish_request(Panel* p, ReqState& s) {
     std::vector<double> G((size_t)C * C);
     const bool f32 = s.work32 != nullptr;
     float* work32 = s.work32;
-    if (C <= 12) {
+    const bool have_gram = s.gram.size() == (size_t)C * C;
+    if (have_gram) {
+        G = s.gram;
+    } else if (C <= 12) {
         if (f32) {
             if ((e = gram_fused(work32, n, C, n, mpart, G))) return e;
         } else if ((e = gram_fused((const double*)work, n, C, n, mpart, G))) {
@@ -1309,7 +1316,11 @@ int finish_request(Panel* p, ReqState& s) {
     double* T_early = nullptr;
     bool rss_done = false;
     double rss = 0.0;
-    if (!f32) {
+    if (!f32 || have_gram) {
+        // With FP32 storage and cross products formed in FP64 before the rounding, beta above is already
+        // the FP64 solution (no refinement); the residual and scores then read the FP32 columns, whose
+        // rounding enters the sandwich only at the level of the storage precision, and rss comes from
+        // the FP64 cross products.
         bool any_T = false;
         if (ncd > 0 && !use_iv)
             for (int mask = 1; mask < (1 << ncd); ++mask) {
@@ -1324,12 +1335,24 @@ int finish_request(Panel* p, ReqState& s) {
         if (any_T && (e = db.get(&T_early, (size_t)n * K))) return e;
         FEHIP_CHECK(copy_sync(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
         const size_t tlds = (T_early && K <= 16) ? sizeof(double) * 256 * (size_t)K : 0;
-        hipLaunchKernelGGL(resid_scores_kernel, dim3(kRedBlocks), dim3(256), tlds, 0, (const double*)work, n, K,
-                           (const double*)dbeta, r, T_early, part);
+        if (f32)
+            hipLaunchKernelGGL(resid_scores_kernel<float>, dim3(kRedBlocks), dim3(256), tlds, 0,
+                               (const float*)work32, n, K, (const double*)dbeta, r, T_early, part);
+        else
+            hipLaunchKernelGGL(resid_scores_kernel<double>, dim3(kRedBlocks), dim3(256), tlds, 0,
+                               (const double*)work, n, K, (const double*)dbeta, r, T_early, part);
         FEHIP_LAUNCH_CHECK("residual");
         std::vector<double> ps;
         if ((e = host_sum_parts(part, kRedBlocks, 1, ps))) return e;
         rss = ps[0];
+        if (f32) {  // y~'y~ - 2 beta' W~'y~ + beta' W~'W~ beta from the FP64 cross products
+            double bq = 0.0, by = 0.0;
+            for (int i = 0; i < K; ++i) {
+                by += beta[i] * g(1 + i, 0);
+                for (int j = 0; j < K; ++j) bq += beta[i] * g(1 + i, 1 + j) * beta[j];
+            }
+            rss = std::max(0.0, g(0, 0) - 2.0 * by + bq);
+        }
         rss_done = true;
     } else {
         // FP64 refinement on the residual: e = M_F(y - W beta) in FP64 from the caller's FP64 columns and
@@ -1737,6 +1760,10 @@ constexpr int kMaxSlots = 32;
 // Solve a batch: requests that absorb the same FE dims share the within-transform sweeps (up to
 // 32 requests per solve, one mask bit each); every request then takes its own normal-equation
 // and sandwich step.
+// a chunk whose distinct columns number at most this many takes its cross products from the final
+// pass of the within-transform (one column tile)
+constexpr int kGramCols = 16;
+
 int run_batch(Panel* p, std::vector<ReqState>& rs) {
     int e;
     const int64_t n = p->n;
@@ -1825,8 +1852,38 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
             if (uidx[c2] == c2) upos[c2] = u++;
         fehip_demean_opts dopt = rs[i0].opts->demean;
         dopt.fe_mask = rs[i0].fe_mask;
+        // cross products of the solved columns from the final pass (one column tile when they fit)
+        double* dgram = nullptr;
+        if (nu <= kGramCols) {
+            if ((e = cb.get(&dgram, (size_t)nu * nu))) return e;
+            wc.gram = dgram;
+            wc.gram_ld = nu;
+        }
         WithinStats wsu(nu);
         if ((e = within_transform(p, wc, nu, wm, wm ? (int)chunk.size() : 1, dopt, 1, wsu, 0))) return e;
+        if (dgram) {
+            std::vector<double> gh((size_t)nu * nu);
+            FEHIP_CHECK(copy_sync(gh.data(), dgram, gh.size() * sizeof(double), hipMemcpyDeviceToHost), "gram copy");
+            int cc = 0;
+            for (int i : chunk) {
+                ReqState& s = rs[i];
+                std::vector<int> u(s.C);
+                bool one_tile = true;
+                for (int j = 0; j < s.C; ++j) {
+                    u[j] = upos[uidx[cc + j]];
+                    one_tile = one_tile && wsu.col_tile[u[j]] == wsu.col_tile[u[0]];
+                }
+                cc += s.C;
+                s.gram.clear();
+                if (!one_tile) continue;
+                s.gram.resize((size_t)s.C * s.C);
+                for (int b2 = 0; b2 < s.C; ++b2)
+                    for (int a2 = 0; a2 < s.C; ++a2) {  // upper triangle of the solve's matrix, mirrored
+                        const int lo = std::min(u[a2], u[b2]), hi = std::max(u[a2], u[b2]);
+                        s.gram[(size_t)b2 * s.C + a2] = gh[(size_t)hi * nu + lo];
+                    }
+            }
+        }
         WithinStats ws(ncols);
         for (int c2 = 0; c2 < ncols; ++c2) {
             const int src = uidx[c2], u = upos[src];

@@ -83,7 +83,49 @@ struct ApplyArgs {
     int part_base;             // first partial block written by this launch
     const double* alpha;       // kFinal: solved tables [S][CT]
     int to_work;               // kFinal: 1 = panel-order work columns, 0 = caller columns in place
+    double* gram;              // kFinal: per-block 16 x 16 cross-product records (nullptr: none)
 };
+
+// ---- cross products of the finalized columns ----------------------------------------------------
+// The final pass holds every row's transformed values of the tile's columns in FP64 registers, so
+// the Gram matrix V'V of the tile is accumulated there, on the FP64 matrix cores, before any
+// rounding to the storage precision.  v_mfma_f64_16x16x4f64 computes D[i][j] += sum_k A[i][k] B[k][j]
+// with A[i][k] taken from lane i + 16k and B[k][j] from lane j + 16k; with A[i][k] = B[k][i] =
+// V[r0 + k][i] both operands are the same value, V[r0 + lane / 16][lane % 16], so each group of four
+// rows is one instruction on values staged (transposed) in LDS.  D[i][j] sits in lane j + 16 (i % 4),
+// element i / 4.  Each workgroup leaves one record of 256 doubles; records are summed in a fixed order.
+typedef double gram4_t __attribute__((ext_vector_type(4)));
+
+template <int CT>
+constexpr int gram_stage() {
+    return 64 * CT > 256 ? 64 * CT : 256;  // doubles of LDS per wave (rows x columns, or one record)
+}
+
+// one wave: the Gram update of 64 rows whose values sit in LDS as st[row * CT + col]
+template <int CT>
+__device__ __forceinline__ void gram_rows64(const double* st, int lane, gram4_t& acc) {
+    const int col = lane & 15, rr = lane >> 4;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const double x = col < CT ? st[(4 * k + rr) * CT + col] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc, 0, 0, 0);
+    }
+}
+
+// the block's waves add their accumulators in a fixed order into the block's record
+__device__ __forceinline__ void gram_store_block(double* stage_all, int stage, const gram4_t& acc, double* rec) {
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    __syncthreads();
+    double* mine = stage_all + (size_t)wv * stage;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) mine[(4 * v + (lane >> 4)) * 16 + (lane & 15)] = acc[v];
+    __syncthreads();
+    for (int e = threadIdx.x; e < 256; e += blockDim.x) {
+        double t = 0.0;
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += stage_all[(size_t)w * stage + e];
+        rec[e] = t;
+    }
+}
 
 __device__ __forceinline__ uint32_t row_word(const ApplyArgs& a, int64_t r) { return a.wm ? a.wm[r] : ~0u; }
 
@@ -246,6 +288,8 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    gram4_t gacc = {0.0, 0.0, 0.0, 0.0};
+    double* gstage = dyn + (size_t)a.Gs * CT * 2;  // kFinal with cross products: per-wave staging
     // software pipeline: the next tile's row data (record, ids, mask words) is loaded while the current
     // tile is processed, so a tile waits only for its table gathers
     int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
@@ -327,6 +371,14 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
                         a.oc[c][src] = u[c];
                 }
             }
+            if (a.gram) {  // uniform; u is 0 on rows past the tile and outside each column's slot
+                double* st = gstage + (size_t)(threadIdx.x / kWave) * gram_stage<CT>();
+#pragma unroll
+                for (int c = 0; c < CT; ++c) st[lane * CT + c] = u[c];
+                __builtin_amdgcn_wave_barrier();
+                gram_rows64<CT>(st, lane, gacc);
+                __builtin_amdgcn_wave_barrier();
+            }
         } else {
 #pragma unroll
             for (int o = 0; o < kMaxScatter; ++o) {
@@ -376,6 +428,8 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         __syncthreads();
         double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
         for (int e = threadIdx.x; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
+    } else {
+        if (a.gram) gram_store_block(gstage, gram_stage<CT>(), gacc, a.gram + (size_t)(a.part_base + blockIdx.x) * 256);
     }
 }
 
@@ -400,6 +454,7 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
     double* sp = dyn + (size_t)a.Gs * CT;   // [Gs][CT]
     const int j = threadIdx.x;
     const int lane = j & (kWave - 1);
+    gram4_t gacc = {0.0, 0.0, 0.0, 0.0};
     stage_tables<CT, MODE>(a, sacc, sp);
     for (int64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
         const int64_t s0 = a.tile_seg[2 * t], s1 = a.tile_seg[2 * t + 1];
@@ -454,6 +509,8 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
                         a.oc[c][src] = sv[j * CT + c];
                 }
             }
+            // rows past the tile end hold 0, and so do values outside each column's slot
+            if (a.gram) gram_rows64<CT>(sv + (size_t)(j / kWave) * kWave * CT, lane, gacc);
         } else {
             for (int o = 0; o < a.nd; ++o) {
                 const ScatterDim& d = a.sd[o];
@@ -499,6 +556,10 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
         __syncthreads();
         double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
         for (int e = j; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
+    } else {
+        if (a.gram)
+            gram_store_block(dyn + (size_t)a.Gs * CT * 2, gram_stage<CT>(), gacc,
+                             a.gram + (size_t)(a.part_base + blockIdx.x) * 256);
     }
 }
 
@@ -590,6 +651,10 @@ __global__ __launch_bounds__(kBlock) void long_kernel(ApplyArgs a) {
     extern __shared__ double dyn[];
     double* sacc = dyn;
     double* sp = dyn + (size_t)a.Gs * CT;
+    double* gl = dyn + (size_t)a.Gs * CT * 2;  // kFinal with cross products: the block's 16 x 16 sums
+    if constexpr (MODE == kFinal)
+        if (a.gram)
+            for (int e = threadIdx.x; e < 256; e += blockDim.x) gl[e] = 0.0;
     stage_tables<CT, MODE>(a, sacc, sp);
     __syncthreads();
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -641,6 +706,16 @@ __global__ __launch_bounds__(kBlock) void long_kernel(ApplyArgs a) {
                         a.oc[c][r] = on ? v[c] - mean[c] : 0.0;
                     else if (on)
                         a.oc[c][src] = v[c] - mean[c];
+                }
+                if (a.gram) {  // rare rows: LDS atomics (summation order not fixed)
+                    double u[CT];
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) u[c] = (c < a.ncols && (word & a.cbit[c])) ? v[c] - mean[c] : 0.0;
+                    for (int c1 = 0; c1 < a.ncols; ++c1)
+                        for (int c2 = 0; c2 < a.ncols; ++c2) {
+                            const double pr = u[c1] * u[c2];
+                            if (pr != 0.0) lds_add(&gl[c1 * 16 + c2], pr);
+                        }
                 }
             }
         } else {
@@ -727,7 +802,33 @@ __global__ __launch_bounds__(kBlock) void long_kernel(ApplyArgs a) {
         __syncthreads();
         double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
         for (int e = threadIdx.x; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
+    } else {
+        if (a.gram) {
+            __syncthreads();
+            double* rec = a.gram + (size_t)(a.part_base + blockIdx.x) * 256;
+            for (int e = threadIdx.x; e < 256; e += blockDim.x) rec[e] = gl[e];
+        }
     }
+}
+
+// Sum of the per-block cross-product records in a fixed order: stage 1 adds runs of kGramRun records,
+// stage 2 (one block) adds the stage-1 sums and writes the nc x nc corner into out (leading dim ld).
+constexpr int kGramRun = 32;
+
+__global__ void gram_sum_kernel(const double* __restrict__ rec, int nrec, double* __restrict__ part) {
+    const int e = threadIdx.x;  // 256 threads, one record element each
+    const int r0 = blockIdx.x * kGramRun, r1 = min(nrec, r0 + kGramRun);
+    double t = 0.0;
+    for (int r = r0; r < r1; ++r) t += rec[(size_t)r * 256 + e];
+    part[(size_t)blockIdx.x * 256 + e] = t;
+}
+
+__global__ void gram_final_kernel(const double* __restrict__ part, int nblk, int nc, double* out, int64_t ld) {
+    const int e = threadIdx.x;
+    double t = 0.0;
+    for (int b = 0; b < nblk; ++b) t += part[(size_t)b * 256 + e];
+    const int i = e >> 4, j = e & 15;
+    if (i < nc && j < nc) out[(int64_t)j * ld + i] = t;
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1114,7 +1215,8 @@ void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, 
 template <int CT, int MODE>
 unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
     const unsigned gw = wave_grid(p), gm = mtile_grid(p), gl = long_grid(p);
-    const size_t lt = tables_lds<CT>(L);
+    size_t lt = tables_lds<CT>(L);
+    if (MODE == kFinal && a.gram) lt += sizeof(double) * (size_t)(kBlock / kWave) * gram_stage<CT>();
     a.part_base = 0;
     if (gw > 0) hipLaunchKernelGGL((wave_tile_kernel<CT, MODE>), dim3(gw), dim3(kBlock), lt, s, a);
     if (gm > 0) {
@@ -1165,6 +1267,7 @@ struct Scratch {
     double* runbuf[kMaxLarge] = {nullptr, nullptr, nullptr};
     double* small_part = nullptr;
     double* part = nullptr;     // reduction partials
+    double* gram_rec = nullptr; // per-block cross-product records of the final pass
     CgState* st = nullptr;
     CgState* st_host = nullptr;  // per-thread pinned scratch
     ~Scratch() {
@@ -1377,11 +1480,20 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
                                    (const double*)xa, S, CT, c, wc.alpha[c0 + c]);
         FEHIP_LAUNCH_CHECK("keep tables");
     }
-    // final pass over the columns: M_P (x - D_o a)
+    // final pass over the columns: M_P (x - D_o a), with the cross products of the tile if asked
     a.alpha = xa;
+    a.gram = wc.gram ? sc.gram_rec : nullptr;
     if (kstats_enabled()) kstat_begin(kStatFinalize, s);
-    launch_pass<CT, kFinal>(p, L, a, s);
+    const unsigned nrec = launch_pass<CT, kFinal>(p, L, a, s);
     FEHIP_LAUNCH_CHECK("finalize kernels");
+    if (wc.gram && nrec > 0) {
+        const unsigned nb = (nrec + kGramRun - 1) / kGramRun;
+        double* gpart = sc.gram_rec + (size_t)nrec * 256;
+        hipLaunchKernelGGL(gram_sum_kernel, dim3(nb), dim3(256), 0, s, (const double*)sc.gram_rec, (int)nrec, gpart);
+        hipLaunchKernelGGL(gram_final_kernel, dim3(1), dim3(256), 0, s, (const double*)gpart, (int)nb, ncols,
+                           wc.gram + (int64_t)c0 * wc.gram_ld + c0, wc.gram_ld);
+        FEHIP_LAUNCH_CHECK("cross products");
+    }
     if (kstats_enabled()) {
         double bytes = 16.0 * (double)p->n * ncols + 4.0 * (double)p->n * (L.nd + 1) +
                        (a.wm ? 4.0 * (double)p->n : 0.0) + (p->perm ? 4.0 * (double)p->n : 0.0);
@@ -1583,6 +1695,12 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
     const unsigned grid = apply_grid(p);
     if ((e = sc.alloc(&sc.small_part, (size_t)grid * (L.Gs > 0 ? L.Gs : 1) * kMaxCT))) return e;
     if ((e = sc.alloc(&sc.part, (size_t)kCgBlocks * kMaxCT * 2))) return e;
+    if (wc.gram) {
+        if (wc.gram_ld < ncols) return set_error("within transform: cross-product leading dim %lld < %d columns",
+                                                  (long long)wc.gram_ld, ncols);
+        const size_t nrec = grid + 1;
+        if ((e = sc.alloc(&sc.gram_rec, (nrec + (nrec + kGramRun - 1) / kGramRun) * 256))) return e;
+    }
     if ((e = sc.alloc(&sc.st, 1))) return e;
     sc.st_host = (CgState*)pinned_scratch(sizeof(CgState));
     if (!sc.st_host) return set_error("pinned host allocation failed");
@@ -1608,10 +1726,20 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
         for (int b = 0; b < nslots; ++b)
             if (used[b] && (e = slot_diagonal(p, L, sc, wm, 1u << b, sc.dg + (int64_t)b * S * 2, s))) return e;
     }
-    // column tiles: the tuner's measured cost per width picks the split of ncols columns
+    // column tiles: the tuner's measured cost per width picks the split of ncols columns; with cross
+    // products asked and at most ct_max columns, one tile holds them all
     std::vector<int> tiles;
-    if ((e = plan_tiles(p, L, sc, mask, ncols, ct_max, tiles, s))) return e;
+    if (wc.gram && ncols <= ct_max) {
+        for (int wdt : kWidths)
+            if (wdt >= ncols) {
+                tiles.push_back(wdt);
+                break;
+            }
+    } else if ((e = plan_tiles(p, L, sc, mask, ncols, ct_max, tiles, s))) {
+        return e;
+    }
     int c0 = 0;
+    int tile_id = 0;
     for (int ct : tiles) {
         const int nc = std::min(ct, ncols - c0);
         TileStats ts;
@@ -1636,7 +1764,9 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
             ws.col_iters[c0 + c] = ts.col_iters[c];
             ws.col_converged[c0 + c] = ts.col_converged[c];
             ws.col_max_update[c0 + c] = ts.col_max_update[c];
+            ws.col_tile[c0 + c] = tile_id;
         }
+        ++tile_id;
         const int64_t id_bytes = n * 4 * L.nd + (int64_t)(p->d[p->P].G + 1) * 8 + (wm ? 4 * n : 0);
         ws.bytes += ts.apply_launches * id_bytes + (int64_t)3 * n * 8 * nc;
         c0 += nc;

@@ -19,7 +19,8 @@ struct TileStats {
 };
 
 struct WithinStats {
-    explicit WithinStats(int ncols = 0) : col_iters(ncols, 0), col_converged(ncols, 1), col_max_update(ncols, 0.0) {}
+    explicit WithinStats(int ncols = 0)
+        : col_iters(ncols, 0), col_converged(ncols, 1), col_max_update(ncols, 0.0), col_tile(ncols, 0) {}
     int iterations = 0;
     int converged = 1;
     double max_update = 0.0;
@@ -29,6 +30,7 @@ struct WithinStats {
     std::vector<int> col_iters;      // per column: CG iterations
     std::vector<int> col_converged;  // per column: criterion met
     std::vector<double> col_max_update;  // per column: last max |r_g / n_g|
+    std::vector<int> col_tile;       // per column: column tile it was solved in
 };
 
 // Columns of one solve: input and output device pointers and the request slot of each column.
@@ -41,6 +43,12 @@ struct WithinCols {
     std::vector<float*> out32;
     std::vector<double*> alpha;
     std::vector<int> slot;
+    // optional device matrix (ncols x ncols, column-major, leading dim gram_ld) that receives the FP64
+    // cross products of the transformed columns (rows outside a column's slot count as 0), formed in
+    // the final pass before any rounding to FP32 storage.  Pairs in different column tiles are left
+    // untouched (WithinStats::col_tile); with at most 16 columns the solve uses one tile.
+    double* gram = nullptr;
+    int64_t gram_ld = 0;
 };
 
 // Size of the FE-table space of a solve over fe_mask (groups of the absorbed non-primary dims).
