@@ -353,7 +353,7 @@ def main():
                            if v["ms"] > 0 else 0.0} for v in ks.values()}
     mixed = None
     if not args.no_mixed:
-        # config 4: the same panel, columns stored in FP32, beta refined in FP64 on the residual
+        # config 4: the same panel, columns stored in FP32, cross products (and beta) in FP64
         kwm = dict(kw, precision=1)
         api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kwm)
         ms, mfit = [], None
@@ -373,8 +373,9 @@ def main():
         log(f"mixed precision: {mtotal / args.steps:.3f} s per regression, max beta err {berr.max():.2e}, "
             f"max se err {serr.max():.2e}")
         mixed = {
-            "workload": "same panel and model; within-transformed columns stored in FP32, beta refined in FP64 on "
-                        "the residual (2 sweeps); Cholesky, 2SLS projection and sandwich in FP64",
+            "workload": "same panel and model; within-transformed columns stored in FP32, their cross products "
+                        "accumulated in FP64 before the rounding (beta without refinement); residual and scores "
+                        "from the FP32 columns, sandwich in FP64",
             "value": args.steps / mtotal, "unit": "regressions/s", "ms_per_step": mtotal / args.steps * 1e3,
             "vs_fp64": (args.steps / mtotal) / (args.steps / total),
             "beta_rel_err_vs_fp64": [float(v) for v in berr], "se_rel_err_vs_fp64": [float(v) for v in serr],

@@ -46,7 +46,7 @@ class Request(C.Structure):
 
 
 class ServerOpts(C.Structure):
-    _fields_ = [("hbm_budget", I64), ("max_batch", I32), ("workers", I32)]
+    _fields_ = [("hbm_budget", I64), ("max_batch", I32), ("workers", I32), ("window_us", I32), ("reserved", I32)]
 
 
 class ServerStats(C.Structure):

@@ -159,10 +159,10 @@ def _fill_request(a, panel, q, sp):
 class Server:
     """fehip_server: a worker thread that batches queued requests on a device-resident panel."""
 
-    def __init__(self, panel: Panel, hbm_budget: int = 0, max_batch: int = 0, workers: int = 0):
+    def __init__(self, panel: Panel, hbm_budget: int = 0, max_batch: int = 0, workers: int = 0, window_us: int = 0):
         self.lib = load()
         self.panel = panel
-        o = S.ServerOpts(int(hbm_budget), int(max_batch), int(workers))
+        o = S.ServerOpts(int(hbm_budget), int(max_batch), int(workers), int(window_us), 0)
         h = C.c_void_p()
         check(self.lib.fehip_server_create(panel.handle, C.byref(o), C.byref(h)), "fehip_server_create")
         self.handle = h.value

@@ -96,6 +96,16 @@ __device__ __forceinline__ void lds_addf(float* p, float v) {
     __hip_atomic_fetch_add(lp, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// Loads through explicit address spaces.  Where one code path reads an LDS table and another a
+// global one, the backend may otherwise merge them into one flat load, which waits on both the LDS and
+// the vector-memory counters and so serialises the gathers behind unrelated LDS traffic.
+__device__ __forceinline__ double ld_global(const double* p) {
+    return *(const __attribute__((address_space(1))) double*)(p);
+}
+__device__ __forceinline__ double ld_lds(const double* p) {
+    return *(const __attribute__((address_space(3))) double*)(p);
+}
+
 // ---- wavefront helpers ----
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll

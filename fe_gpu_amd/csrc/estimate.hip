@@ -23,6 +23,7 @@
 #include "dense.hpp"
 #include "panel.hpp"
 #include "estimate.hpp"
+#include "mfma_gram.hpp"
 #include "wave.hpp"
 #include "within.hpp"
 
@@ -31,6 +32,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kRedBlocks = 512;
+constexpr int kResidBlocks = 2048;  // residual/score pass: eight workgroups per CU keep the loads in flight
 
 using clk = std::chrono::steady_clock;
 
@@ -383,11 +385,10 @@ __device__ __forceinline__ void load_nest_rows(const WaveNest& a, const TX* __re
 template <int KT, typename TX>
 __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, const TX* __restrict__ X, double* part,
                                                                   unsigned long long* ucount) {
-    constexpr int NP = KT * (KT + 1) / 2;
-    double acc[NP];
-#pragma unroll
-    for (int e = 0; e < NP; ++e) acc[e] = 0.0;
+    __shared__ double gst[(kBlock / kWave) * gram_stage<KT>()];  // per-wave staging of the cluster sums
+    gram4_t acc = {0.0, 0.0, 0.0, 0.0};
     const int lane = threadIdx.x & (kWave - 1);
+    double* mystage = gst + (size_t)(threadIdx.x / kWave) * gram_stage<KT>();
     unsigned long long nclusters = 0;
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
     // software pipeline: the next tile's rows are loaded while the current tile is reduced
@@ -442,28 +443,13 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
         const bool use = leader && ne;
         const unsigned long long BU = __ballot(use);
         if (lane == 0) nclusters += (unsigned long long)__popcll(BU);
-        if (use) {
-            int idx = 0;
+        // sum of S S' over the clusters that end here: the cross products of the masked sums
 #pragma unroll
-            for (int i = 0; i < KT; ++i)
-#pragma unroll
-                for (int j = i; j < KT; ++j) acc[idx++] += sv[i] * sv[j];
-        }
+        for (int k = 0; k < KT; ++k) sv[k] = use ? sv[k] : 0.0;
+        gram_wave<KT>(mystage, sv, lane, acc);
     }
     if (lane == 0 && nclusters) atomicAdd(ucount, nclusters);
-    __shared__ double ws[kBlock / kWave][NP];
-    const int wv = threadIdx.x / kWave;
-#pragma unroll
-    for (int e = 0; e < NP; ++e) {
-        const double v = wave_sum(acc[e]);
-        if (lane == 0) ws[wv][e] = v;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < NP; e += blockDim.x) {
-        double v = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) v += ws[w][e];
-        part[(int64_t)blockIdx.x * NP + e] = v;
-    }
+    gram_store_upper<KT>(gst, gram_stage<KT>(), acc, part + (int64_t)blockIdx.x * (KT * (KT + 1) / 2));
 }
 
 // Any subset through its cached sorted partition, one wave per 64 consecutive sorted positions: the
@@ -479,11 +465,10 @@ __global__ __launch_bounds__(kBlock) void sorted_wave_meat_kernel(const double* 
                                                                   const int* __restrict__ order,
                                                                   const int* __restrict__ keys, int64_t n,
                                                                   double* part, int* bkey, double* bval) {
-    constexpr int NP = KT * (KT + 1) / 2;
-    double acc[NP];
-#pragma unroll
-    for (int e = 0; e < NP; ++e) acc[e] = 0.0;
+    __shared__ double gst[(kBlock / kWave) * gram_stage<KT>()];  // per-wave staging of the cluster sums
+    gram4_t acc = {0.0, 0.0, 0.0, 0.0};
     const int lane = threadIdx.x & (kWave - 1);
+    double* mystage = gst + (size_t)(threadIdx.x / kWave) * gram_stage<KT>();
     const int64_t nwaves = (n + kWave - 1) / kWave;
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
     for (int64_t w = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; w < nwaves; w += wstride) {
@@ -510,13 +495,10 @@ __global__ __launch_bounds__(kBlock) void sorted_wave_meat_kernel(const double* 
         const bool nexthead = lane < last ? ((H >> (lane + 1)) & 1ull) != 0 : !cont;
         const bool cend = valid && (lane <= last) && nexthead;
         const bool complete = cend && le != 0;  // started at a head of this wave
-        if (complete) {
-            int idx = 0;
+        double cs[KT];
 #pragma unroll
-            for (int i = 0; i < KT; ++i)
-#pragma unroll
-                for (int j = i; j < KT; ++j) acc[idx++] += sv[i] * sv[j];
-        }
+        for (int k = 0; k < KT; ++k) cs[k] = complete ? sv[k] : 0.0;
+        gram_wave<KT>(mystage, cs, lane, acc);
         // boundary pieces
         const bool open_left = (H & 1ull) == 0ull;
         const int e0 = H ? __ffsll((long long)H) - 2 : last;  // last lane of the leading piece
@@ -533,19 +515,7 @@ __global__ __launch_bounds__(kBlock) void sorted_wave_meat_kernel(const double* 
             for (int k = 0; k < KT; ++k) bval[b1 * KT + k] = open_right ? sv[k] : 0.0;
         }
     }
-    __shared__ double ws[kBlock / kWave][NP];
-    const int wv = threadIdx.x / kWave;
-#pragma unroll
-    for (int e = 0; e < NP; ++e) {
-        const double v = wave_sum(acc[e]);
-        if (lane == 0) ws[wv][e] = v;
-    }
-    __syncthreads();
-    for (int e = threadIdx.x; e < NP; e += blockDim.x) {
-        double v = 0.0;
-        for (int ww = 0; ww < kBlock / kWave; ++ww) v += ws[ww][e];
-        part[(int64_t)blockIdx.x * NP + e] = v;
-    }
+    gram_store_upper<KT>(gst, gram_stage<KT>(), acc, part + (int64_t)blockIdx.x * (KT * (KT + 1) / 2));
 }
 
 struct NestKeys {
@@ -1039,9 +1009,16 @@ struct ReqState {
     clk::time_point t0, t1;
 };
 
+// Per-batch results that depend only on the outcome column and the row subsample: N and the total
+// sum of squares of y (two passes over y), and the non-empty group counts of a subsample.
+struct BatchShared {
+    std::map<std::pair<const double*, const uint8_t*>, std::pair<int64_t, double>> ysums;
+    std::map<const uint8_t*, std::vector<int>> geff;
+};
+
 // Validation, cluster dims, weights in panel order, N, TSS, effective group counts and DoF, and
 // the work buffer of the request.
-int prepare_request(Panel* p, ReqState& s) {
+int prepare_request(Panel* p, ReqState& s, BatchShared& bs) {
     const int64_t n = p->n;
     const int K = s.K, L = s.L;
     const int64_t ld = s.ld;
@@ -1080,7 +1057,7 @@ int prepare_request(Panel* p, ReqState& s) {
     for (int j = 0; j < ncd; ++j) result->cluster_dims[j] = cdims[j];
 
     double* part = nullptr;
-    if ((e = db.get(&part, (size_t)kRedBlocks * 3))) return e;
+    if ((e = db.get(&part, (size_t)std::max(kRedBlocks * 3, kResidBlocks)))) return e;
     // weights in panel order, effective N, TSS of y (compute_tss with intercept)
     uint8_t* wpanel = nullptr;
     if (d_weights) {
@@ -1088,20 +1065,29 @@ int prepare_request(Panel* p, ReqState& s) {
         hipLaunchKernelGGL(weights_panel_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, d_weights, p->perm, n,
                            wpanel);
     }
-    std::vector<double> s3;
-    hipLaunchKernelGGL(wsum2_kernel, dim3(kRedBlocks), dim3(kBlock), 0, 0, d_y, d_weights, n, 0.0, part);
-    if ((e = host_sum_parts(part, kRedBlocks, 3, s3))) return e;
-    const int64_t N = (int64_t)llround(s3[2]);
+    const auto ykey = std::make_pair(d_y, d_weights);
+    auto yit = bs.ysums.find(ykey);
+    if (yit == bs.ysums.end()) {
+        std::vector<double> s3;
+        hipLaunchKernelGGL(wsum2_kernel, dim3(kRedBlocks), dim3(kBlock), 0, 0, d_y, d_weights, n, 0.0, part);
+        if ((e = host_sum_parts(part, kRedBlocks, 3, s3))) return e;
+        const int64_t N0 = (int64_t)llround(s3[2]);
+        const double ybar = N0 > 0 ? s3[0] / (double)N0 : 0.0;
+        hipLaunchKernelGGL(wsum2_kernel, dim3(kRedBlocks), dim3(kBlock), 0, 0, d_y, d_weights, n, ybar, part);
+        if ((e = host_sum_parts(part, kRedBlocks, 3, s3))) return e;
+        yit = bs.ysums.emplace(ykey, std::make_pair(N0, s3[1])).first;
+    }
+    const int64_t N = yit->second.first;
     result->n_obs = N;
-    const double ybar = N > 0 ? s3[0] / (double)N : 0.0;
-    hipLaunchKernelGGL(wsum2_kernel, dim3(kRedBlocks), dim3(kBlock), 0, 0, d_y, d_weights, n, ybar, part);
-    if ((e = host_sum_parts(part, kRedBlocks, 3, s3))) return e;
-    result->tss_total = s3[1];
+    result->tss_total = yit->second.second;
     // group counts: G_d (max id) without weights, non-empty groups with weights
     int Geff[FEHIP_MAX_FE];
+    auto git = wpanel ? bs.geff.find(d_weights) : bs.geff.end();
     for (int d = 0; d < p->m; ++d) {
         Geff[d] = p->d[d].G;
-        if (wpanel) {
+        if (git != bs.geff.end()) {
+            Geff[d] = git->second[d];
+        } else if (wpanel) {
             int* flags = nullptr;
             unsigned long long* cnt = nullptr;
             if ((e = db.get(&flags, (size_t)p->d[d].G)) || (e = db.get(&cnt, 1))) return e;
@@ -1116,6 +1102,7 @@ int prepare_request(Panel* p, ReqState& s) {
             Geff[d] = (int)h;
         }
     }
+    if (wpanel && git == bs.geff.end()) bs.geff[d_weights] = std::vector<int>(Geff, Geff + p->m);
     int dof_fes = 0, dof_nested = 0;
     for (int d = 0; d < p->m; ++d)
         if ((fe_mask >> d) & 1u) dof_fes += std::max(0, Geff[d] - 1);
@@ -1316,7 +1303,16 @@ int finish_request(Panel* p, ReqState& s) {
     double* T_early = nullptr;
     bool rss_done = false;
     double rss = 0.0;
-    if (!f32 || have_gram) {
+    if (have_gram && ncd == 0 && !use_iv) {
+        // no sandwich: rss from the FP64 cross products, no pass over the rows
+        double bq = 0.0, by = 0.0;
+        for (int i = 0; i < K; ++i) {
+            by += beta[i] * g(1 + i, 0);
+            for (int j = 0; j < K; ++j) bq += beta[i] * g(1 + i, 1 + j) * beta[j];
+        }
+        rss = std::max(0.0, g(0, 0) - 2.0 * by + bq);
+        rss_done = true;
+    } else if (!f32 || have_gram) {
         // With FP32 storage and cross products formed in FP64 before the rounding, beta above is already
         // the FP64 solution (no refinement); the residual and scores then read the FP32 columns, whose
         // rounding enters the sandwich only at the level of the storage precision, and rss comes from
@@ -1336,14 +1332,14 @@ int finish_request(Panel* p, ReqState& s) {
         FEHIP_CHECK(copy_sync(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
         const size_t tlds = (T_early && K <= 16) ? sizeof(double) * 256 * (size_t)K : 0;
         if (f32)
-            hipLaunchKernelGGL(resid_scores_kernel<float>, dim3(kRedBlocks), dim3(256), tlds, 0,
+            hipLaunchKernelGGL(resid_scores_kernel<float>, dim3(kResidBlocks), dim3(256), tlds, 0,
                                (const float*)work32, n, K, (const double*)dbeta, r, T_early, part);
         else
-            hipLaunchKernelGGL(resid_scores_kernel<double>, dim3(kRedBlocks), dim3(256), tlds, 0,
+            hipLaunchKernelGGL(resid_scores_kernel<double>, dim3(kResidBlocks), dim3(256), tlds, 0,
                                (const double*)work, n, K, (const double*)dbeta, r, T_early, part);
         FEHIP_LAUNCH_CHECK("residual");
         std::vector<double> ps;
-        if ((e = host_sum_parts(part, kRedBlocks, 1, ps))) return e;
+        if ((e = host_sum_parts(part, kResidBlocks, 1, ps))) return e;
         rss = ps[0];
         if (f32) {  // y~'y~ - 2 beta' W~'y~ + beta' W~'W~ beta from the FP64 cross products
             double bq = 0.0, by = 0.0;
@@ -1578,7 +1574,10 @@ int finish_request(Panel* p, ReqState& s) {
                 sk.w = wpanel;
                 const int KT = K <= 3 ? 4 : (K <= 7 ? 8 : 12);
                 const int c = (int)cells;
-                const unsigned grid = stream_grid((size_t)n, kBlock, 2);
+                // sixteen waves per CU, fewer where the per-block partial tables would pass 4M doubles
+                const unsigned grid = std::max(
+                    1u, std::min(stream_grid((size_t)n, kBlock, 16), (unsigned)std::max<int64_t>(
+                                                                          256, ((int64_t)4 << 20) / ((int64_t)c * KT))));
                 double* sp_ = nullptr;
                 if ((e = db.get(&sp_, (size_t)grid * c * KT + (size_t)c * KT))) return e;
                 const size_t lds = sizeof(double) * (size_t)c * KT;
@@ -1777,6 +1776,7 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
     FEHIP_CHECK(hipMemGetInfo(&free_b, &total_b), "memory info");
     const double budget = 0.7 * (double)(free_b + cache_idle_bytes());
     std::vector<int> done(rs.size(), 0);
+    BatchShared shared;
     for (size_t i0 = 0; i0 < rs.size(); ++i0) {
         if (done[i0]) continue;
         // next chunk: unsolved requests with the same FE set, in submission order, within the slot
@@ -1797,7 +1797,7 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
             chunk.push_back((int)i);
         }
         for (int i : chunk)
-            if ((e = prepare_request(p, rs[i]))) return e;
+            if ((e = prepare_request(p, rs[i], shared))) return e;
         int ncols = 0;
         bool weighted = false;
         for (int i : chunk) {

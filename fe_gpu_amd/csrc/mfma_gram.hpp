@@ -1,0 +1,85 @@
+// Cross products (Gram matrices) of per-row values on the FP64 matrix cores.
+//
+// v_mfma_f64_16x16x4f64 computes D[i][j] += sum_k A[i][k] B[k][j] over four rows k, with A[i][k] taken
+// from lane i + 16k and B[k][j] from lane j + 16k, and D[i][j] held by lane j + 16 (i % 4) as element
+// i / 4 of its four-double accumulator.  For V'V, A[i][k] = B[k][i] = V[r0 + k][i], so both operands are
+// the same value, V[r0 + lane / 16][lane % 16]: with a wave's 64 rows staged (row-major) in LDS, the
+// update of up to 16 columns is sixteen instructions and the accumulator costs 8 registers, where
+// per-lane FMAs would hold one register pair per (i <= j) pair.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace fehip {
+
+typedef double gram4_t __attribute__((ext_vector_type(4)));
+
+// doubles of LDS per wave: 64 staged rows of CT values, or one 16 x 16 record
+template <int CT>
+constexpr int gram_stage() {
+    return 64 * CT > 256 ? 64 * CT : 256;
+}
+
+// one wave: V'V of the 64 rows st[row * CT + col] (CT <= 16) added to acc
+template <int CT>
+__device__ __forceinline__ void gram_rows64(const double* st, int lane, gram4_t& acc) {
+    static_assert(CT <= 16, "at most 16 columns per accumulator");
+    const int col = lane & 15, rr = lane >> 4;
+#pragma unroll
+    for (int k = 0; k < 16; ++k) {
+        const double x = col < CT ? st[(4 * k + rr) * CT + col] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc, 0, 0, 0);
+    }
+}
+
+// one wave: stage this lane's row (CT values) and add the wave's V'V to acc
+template <int CT>
+__device__ __forceinline__ void gram_wave(double* st, const double (&v)[CT], int lane, gram4_t& acc) {
+#pragma unroll
+    for (int c = 0; c < CT; ++c) st[lane * CT + c] = v[c];
+    __builtin_amdgcn_wave_barrier();
+    gram_rows64<CT>(st, lane, acc);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The block's waves add their accumulators in a fixed order; the 16 x 16 sum is left in stage_all[0..255]
+// (row-major) and visible to every thread.  stage_all holds blockDim.x / 64 records of `stage` doubles.
+__device__ __forceinline__ void gram_block_sum(double* stage_all, int stage, const gram4_t& acc) {
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    __syncthreads();
+    double* mine = stage_all + (size_t)wv * stage;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) mine[(4 * v + (lane >> 4)) * 16 + (lane & 15)] = acc[v];
+    __syncthreads();
+    double t = 0.0;
+    const int e = threadIdx.x;
+    if (e < 256)
+        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += stage_all[(size_t)w * stage + e];
+    __syncthreads();
+    if (e < 256) stage_all[e] = t;
+    __syncthreads();
+}
+
+// the block's record of 256 doubles
+__device__ __forceinline__ void gram_store_block(double* stage_all, int stage, const gram4_t& acc, double* rec) {
+    gram_block_sum(stage_all, stage, acc);
+    for (int e = threadIdx.x; e < 256; e += blockDim.x) rec[e] = stage_all[e];
+}
+
+// the block's upper triangle (i <= j < KT, row by row) as KT (KT + 1) / 2 doubles
+template <int KT>
+__device__ __forceinline__ void gram_store_upper(double* stage_all, int stage, const gram4_t& acc, double* out) {
+    gram_block_sum(stage_all, stage, acc);
+    for (int e = threadIdx.x; e < KT * (KT + 1) / 2; e += blockDim.x) {
+        int i = 0, rem = e;
+        while (rem >= KT - i) {
+            rem -= KT - i;
+            ++i;
+        }
+        out[e] = stage_all[i * 16 + i + rem];
+    }
+}
+
+}  // namespace fehip

@@ -49,6 +49,7 @@ struct Server {
     double budget = 0.0;
     double inflight = 0.0;  // bytes held by the batches on the device
     int max_batch = 32;
+    clk::duration window{};  // how long a partial batch waits for more requests
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<int64_t> queue;
@@ -105,6 +106,15 @@ void run_worker(Server* sv, bool capture) {
                 lk.unlock();
                 release_workspaces();  // this thread's workspaces
                 return;
+            }
+            // batching window: a partial batch waits until it is full or its oldest request has waited
+            // the window (requests submitted back to back then share one batch)
+            if (sv->window.count() > 0 && !sv->stop && (int)sv->queue.size() < sv->max_batch) {
+                const auto deadline = sv->tickets[sv->queue.front()]->t_submit + sv->window;
+                sv->cv_work.wait_until(lk, deadline, [&] {
+                    return sv->stop || sv->queue.empty() || (int)sv->queue.size() >= sv->max_batch;
+                });
+                if (!first_fits()) continue;  // taken by another worker, or memory in use meanwhile
             }
             // admission: arrival order, up to max_batch requests and the HBM budget
             double bytes = 0.0, reserve = 0.0;
@@ -196,6 +206,8 @@ extern "C" int fehip_server_create(fehip_panel* panel, const fehip_server_opts* 
     sv.budget = (opts && opts->hbm_budget > 0) ? (double)opts->hbm_budget
                                                  : 0.8 * (double)(free_b + cache_idle_bytes());
     sv.max_batch = (opts && opts->max_batch > 0) ? std::min(opts->max_batch, 256) : 32;
+    const int win_us = !opts ? 2000 : (opts->window_us > 0 ? opts->window_us : (opts->window_us == 0 ? 2000 : 0));
+    sv.window = std::chrono::duration_cast<clk::duration>(std::chrono::microseconds(win_us));
     const int nw = (opts && opts->workers > 0) ? std::min(opts->workers, 8) : 2;
     try {
         for (int w = 0; w < nw; ++w) sv.workers.emplace_back(run_worker, &sv, nw == 1);
@@ -241,7 +253,7 @@ extern "C" int fehip_server_submit(fehip_server* srv, const fehip_request* req, 
         sv.submitted += 1;
         *ticket = id;
     }
-    sv.cv_work.notify_one();
+    sv.cv_work.notify_all();
     return set_ok();
 }
 

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "kstats.hpp"
+#include "mfma_gram.hpp"
 #include "panel.hpp"
 #include "wave.hpp"
 #include "within.hpp"
@@ -86,46 +87,10 @@ struct ApplyArgs {
     double* gram;              // kFinal: per-block 16 x 16 cross-product records (nullptr: none)
 };
 
-// ---- cross products of the finalized columns ----------------------------------------------------
 // The final pass holds every row's transformed values of the tile's columns in FP64 registers, so
-// the Gram matrix V'V of the tile is accumulated there, on the FP64 matrix cores, before any
-// rounding to the storage precision.  v_mfma_f64_16x16x4f64 computes D[i][j] += sum_k A[i][k] B[k][j]
-// with A[i][k] taken from lane i + 16k and B[k][j] from lane j + 16k; with A[i][k] = B[k][i] =
-// V[r0 + k][i] both operands are the same value, V[r0 + lane / 16][lane % 16], so each group of four
-// rows is one instruction on values staged (transposed) in LDS.  D[i][j] sits in lane j + 16 (i % 4),
-// element i / 4.  Each workgroup leaves one record of 256 doubles; records are summed in a fixed order.
-typedef double gram4_t __attribute__((ext_vector_type(4)));
-
-template <int CT>
-constexpr int gram_stage() {
-    return 64 * CT > 256 ? 64 * CT : 256;  // doubles of LDS per wave (rows x columns, or one record)
-}
-
-// one wave: the Gram update of 64 rows whose values sit in LDS as st[row * CT + col]
-template <int CT>
-__device__ __forceinline__ void gram_rows64(const double* st, int lane, gram4_t& acc) {
-    const int col = lane & 15, rr = lane >> 4;
-#pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const double x = col < CT ? st[(4 * k + rr) * CT + col] : 0.0;
-        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc, 0, 0, 0);
-    }
-}
-
-// the block's waves add their accumulators in a fixed order into the block's record
-__device__ __forceinline__ void gram_store_block(double* stage_all, int stage, const gram4_t& acc, double* rec) {
-    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
-    __syncthreads();
-    double* mine = stage_all + (size_t)wv * stage;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) mine[(4 * v + (lane >> 4)) * 16 + (lane & 15)] = acc[v];
-    __syncthreads();
-    for (int e = threadIdx.x; e < 256; e += blockDim.x) {
-        double t = 0.0;
-        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += stage_all[(size_t)w * stage + e];
-        rec[e] = t;
-    }
-}
+// the Gram matrix V'V of the tile is accumulated there (mfma_gram.hpp), before any rounding to the
+// storage precision.  Each workgroup leaves one record of 256 doubles; records are summed in a fixed
+// order.
 
 __device__ __forceinline__ uint32_t row_word(const ApplyArgs& a, int64_t r) { return a.wm ? a.wm[r] : ~0u; }
 
@@ -148,11 +113,11 @@ __device__ __forceinline__ void row_value(const ApplyArgs& a, int64_t r, const d
             if (d.small) {
                 const size_t base = (size_t)(d.lds_off + g) * CT;
 #pragma unroll
-                for (int c = 0; c < CT; ++c) v[c] += sign * sp[base + c];
+                for (int c = 0; c < CT; ++c) v[c] += sign * ld_lds(&sp[base + c]);
             } else {
                 const int64_t base = (d.tab_off + g) * CT;
 #pragma unroll
-                for (int c = 0; c < CT; ++c) v[c] += sign * tab[base + c];
+                for (int c = 0; c < CT; ++c) v[c] += sign * ld_global(&tab[base + c]);
             }
         }
     }
@@ -206,11 +171,11 @@ __device__ __forceinline__ void row_value_ids(const ApplyArgs& a, int64_t r, con
             if (d.small) {
                 const size_t base = (size_t)(d.lds_off + g[o]) * CT;
 #pragma unroll
-                for (int c = 0; c < CT; ++c) v[c] += sign * sp[base + c];
+                for (int c = 0; c < CT; ++c) v[c] += sign * ld_lds(&sp[base + c]);
             } else {
                 const int64_t base = (d.tab_off + g[o]) * CT;
 #pragma unroll
-                for (int c = 0; c < CT; ++c) v[c] += sign * tab[base + c];
+                for (int c = 0; c < CT; ++c) v[c] += sign * ld_global(&tab[base + c]);
             }
         }
     }
@@ -371,14 +336,8 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
                         a.oc[c][src] = u[c];
                 }
             }
-            if (a.gram) {  // uniform; u is 0 on rows past the tile and outside each column's slot
-                double* st = gstage + (size_t)(threadIdx.x / kWave) * gram_stage<CT>();
-#pragma unroll
-                for (int c = 0; c < CT; ++c) st[lane * CT + c] = u[c];
-                __builtin_amdgcn_wave_barrier();
-                gram_rows64<CT>(st, lane, gacc);
-                __builtin_amdgcn_wave_barrier();
-            }
+            if (a.gram)  // uniform; u is 0 on rows past the tile and outside each column's slot
+                gram_wave<CT>(gstage + (size_t)(threadIdx.x / kWave) * gram_stage<CT>(), u, lane, gacc);
         } else {
 #pragma unroll
             for (int o = 0; o < kMaxScatter; ++o) {
@@ -841,9 +800,18 @@ __global__ void reduce_runs_kernel(const double* __restrict__ runbuf, const int6
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
         const int g = (int)(e / CT), c = (int)(e % CT);
-        double s = 0.0;
-        for (int64_t k = goff[g]; k < goff[g + 1]; ++k) s += runbuf[k * CT + c];
-        q[e] = s;
+        // four independent partial sums keep four loads in flight per thread
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        const int64_t k0 = goff[g], k1 = goff[g + 1];
+        int64_t k = k0;
+        for (; k + 4 <= k1; k += 4) {
+            s0 += runbuf[k * CT + c];
+            s1 += runbuf[(k + 1) * CT + c];
+            s2 += runbuf[(k + 2) * CT + c];
+            s3 += runbuf[(k + 3) * CT + c];
+        }
+        for (; k < k1; ++k) s0 += runbuf[k * CT + c];
+        q[e] = (s0 + s1) + (s2 + s3);
     }
 }
 
@@ -902,8 +870,9 @@ __device__ __forceinline__ void block_rz_max(const double (&rz)[CT], const doubl
 template <int CT>
 __global__ __launch_bounds__(kBlock) void cg_init_kernel(int64_t S, const double* __restrict__ r,
                                                          const double* __restrict__ minv,
-                                                         const double* __restrict__ wcnt, double* z, double* p,
-                                                         double* x, double* part_rz, double* part_max) {
+                                                         const double* __restrict__ wcnt, double* __restrict__ z,
+                                                         double* __restrict__ p, double* __restrict__ x,
+                                                         double* part_rz, double* part_max) {
     double rz[CT], mx[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) rz[c] = mx[c] = 0.0;
@@ -1017,29 +986,42 @@ template <int CT>
 __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t S, const CgState* __restrict__ st,
                                                            const double* __restrict__ p, const double* __restrict__ q,
                                                            const double* __restrict__ minv,
-                                                           const double* __restrict__ wcnt, double* x, double* r,
-                                                           double* z, double* part_rz, double* part_max) {
+                                                           const double* __restrict__ wcnt, double* __restrict__ x,
+                                                           double* __restrict__ r, double* __restrict__ z,
+                                                           double* part_rz, double* part_max) {
     double al[CT], rz[CT], mx[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
         al[c] = st->alpha[c];
         rz[c] = mx[c] = 0.0;
     }
+    // the arrays do not overlap, so every load of a table row is issued before its stores
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
+        double rv[CT], xv[CT], pv[CT], qv[CT], mv[CT], nv[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
             const int64_t e = g * CT + c;
-            double rv = r[e];
+            rv[c] = r[e];
+            mv[c] = minv[e];
+            nv[c] = wcnt[e];
             if (al[c] != 0.0) {
-                x[e] += al[c] * p[e];
-                rv -= al[c] * q[e];
-                r[e] = rv;
+                xv[c] = x[e];
+                pv[c] = p[e];
+                qv[c] = q[e];
             }
-            const double zv = minv[e] * rv;
+        }
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            const int64_t e = g * CT + c;
+            if (al[c] != 0.0) {
+                x[e] = fma(al[c], pv[c], xv[c]);
+                rv[c] = fma(-al[c], qv[c], rv[c]);
+                r[e] = rv[c];
+            }
+            const double zv = mv[c] * rv[c];
             z[e] = zv;
-            rz[c] += rv * zv;
-            const double n = wcnt[e];
-            if (n > 0.0) mx[c] = fmax(mx[c], fabs(rv) / n);
+            rz[c] += rv[c] * zv;
+            if (nv[c] > 0.0) mx[c] = fmax(mx[c], fabs(rv[c]) / nv[c]);
         }
     }
     block_rz_max<CT>(rz, mx, part_rz, part_max);
@@ -1203,7 +1185,7 @@ void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, 
         } else {
             const DimStruct& D = p->d[L.dims[o]];
             if (time_runs) kstat_begin(kStatReduceRuns, s);
-            hipLaunchKernelGGL((reduce_runs_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock)), dim3(kBlock), 0,
+            hipLaunchKernelGGL((reduce_runs_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock, 32)), dim3(kBlock), 0,
                                s, (const double*)a.sd[o].runbuf, (const int64_t*)D.grp_run_off, d.G, qo, active);
             if (time_runs)
                 kstat_end(kStatReduceRuns, s, 8.0 * CT * (double)(D.n_runs + d.G) + 8.0 * (double)(d.G + 1));
@@ -1281,7 +1263,7 @@ struct Scratch {
     }
 };
 
-constexpr int kCgBlocks = 512;
+constexpr int kCgBlocks = 2048;  // CG vector kernels: eight waves per SIMD keep the table loads in flight
 
 ApplyArgs base_args(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm) {
     ApplyArgs a{};

@@ -52,8 +52,11 @@ typedef struct {
     int32_t method;       /* 0: PCG in FE space (the only method of the coarse path; the reference's
                              relaxed AP in row space runs through the fine-grained ABI) */
     uint32_t fe_mask;     /* bit d set => FE dim d is absorbed; 0 means all dims */
-    int32_t precision;    /* 0: FP64; 1 (fehip_estimate): within-transformed columns stored in FP32,
-                             beta refined in FP64 on the residual (2SLS requests stay FP64) */
+    int32_t precision;    /* 0: FP64; 1 (fehip_estimate): within-transformed columns stored in FP32.
+                             The FP64 cross products come from the transform's final pass when the
+                             request's columns share a column tile (beta is then the FP64 solution);
+                             otherwise beta is refined once in FP64 on the residual.  2SLS requests
+                             stay FP64 */
     int32_t check_every;  /* convergence read-back period in iterations (0 -> 4) */
     int32_t reserved;
 } fehip_demean_opts;
@@ -141,6 +144,9 @@ typedef struct {
     int64_t hbm_budget;  /* device bytes the batches in flight may hold; 0 -> 80% of free memory */
     int32_t max_batch;   /* requests per batch; 0 -> 32 */
     int32_t workers;     /* batches in flight at once (one host thread and stream each); 0 -> 2 */
+    int32_t window_us;   /* a partial batch waits up to this long after its oldest request's submit for
+                            more requests; 0 -> 2000, < 0 -> no wait */
+    int32_t reserved;
 } fehip_server_opts;
 
 typedef struct {

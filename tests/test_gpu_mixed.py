@@ -1,7 +1,8 @@
 """GPU parity of the mixed-precision form (precision=1): the within-transformed columns are stored in
-FP32, and beta is refined in FP64 on the residual M_F(y - W beta), formed from the caller's FP64
-columns and the solved FE tables.  FP64 is kept for the refinement, the Cholesky solve, the 2SLS
-projection and the sandwich.
+FP32.  Their cross products are accumulated in FP64 by the transform's final pass before the rounding,
+so beta is the FP64 solution; where a request's columns span several column tiles, beta is instead
+refined in FP64 on the residual M_F(y - W beta), formed from the caller's FP64 columns and the solved
+FE tables.  FP64 is kept for the Cholesky solve, the 2SLS projection and the sandwich.
 
 Bar (the stated accuracy target): beta within 1e-6 and SE within 1e-5 relative of the FP64 path
 (and of the oracle); integer outputs identical to the FP64 path.
