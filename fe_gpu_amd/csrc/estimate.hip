@@ -338,18 +338,24 @@ int cluster_sums(const double* T, int K, const int* order, const int* keys, int6
 // columns, summed per run of equal subset ids by a segmented DPP scan, and runs of one segment with
 // the same ids are merged (a cluster inside a segment need not be contiguous).  Each cluster's
 // leader lane accumulates S S' in registers; block partials of the upper triangle.
+// Subsets that contain the primary dim, on the wave tiles: a subset's clusters never cross a primary
+// segment, so one wave per tile forms every cluster sum with segmented scans in registers.  Up to four
+// such subsets share one pass over the residual and the regressors (each reads the same rows); the
+// subset's other dims are among at most two (kd[0], kd[1]).
+constexpr int kMaxNestSubsets = 4;
+
 struct WaveNest {
     const double* r;            // residual (panel order)
     int64_t ldX;                // regressor columns (panel order)
     int K;
     const int* idP;
-    const int64_t* seg_off;
     const int64_t* wtile_rec;   // first row | rows << 32 of each wave tile
     int64_t n_wtiles;
-    const int* kid[2];          // the subset's other dims (panel order ids)
-    int nk;
+    const int* kd[2];           // the other cluster dims of the subsets (panel order ids)
+    int nsub;
+    int use[kMaxNestSubsets];   // per subset: bit j set -> kd[j] belongs to the subset
+    int mode[kMaxNestSubsets];  // 2: singletons, 1: clusters are single runs, 0: merge runs of a segment
     const uint8_t* w;           // panel-order weights (nullptr: all rows)
-    int mode;                   // 2: singletons, 1: clusters are single runs, 0: merge runs of a segment
 };
 
 // Row data of one wave tile of the nested-subset kernel, loaded one tile ahead of its use.
@@ -374,22 +380,26 @@ __device__ __forceinline__ void load_nest_rows(const WaveNest& a, const TX* __re
     const bool valid = lane < nr;
     const int64_t ri = R0 + (valid ? lane : 0);
     q.ip = valid ? a.idP[ri] : -1;
-    q.k0 = (valid && a.nk > 0) ? a.kid[0][ri] : -1;
-    q.k1 = (valid && a.nk > 1) ? a.kid[1][ri] : -1;
+    q.k0 = (valid && a.kd[0]) ? a.kd[0][ri] : -1;
+    q.k1 = (valid && a.kd[1]) ? a.kd[1][ri] : -1;
     q.w = valid && (!a.w || a.w[ri]);
     const double rr = valid ? a.r[ri] : 0.0;
 #pragma unroll
     for (int k = 0; k < KT; ++k) q.t[k] = (k < a.K && valid) ? rr * (double)X[ri + (int64_t)k * a.ldX] : 0.0;
 }
 
+// Per subset s: part[(s * gridDim.x + block) * NP + e] (upper triangle of the block's sum of S S') and
+// ucount[s] (non-empty clusters).
 template <int KT, typename TX>
 __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, const TX* __restrict__ X, double* part,
                                                                   unsigned long long* ucount) {
     __shared__ double gst[(kBlock / kWave) * gram_stage<KT>()];  // per-wave staging of the cluster sums
-    gram4_t acc = {0.0, 0.0, 0.0, 0.0};
+    gram4_t acc[kMaxNestSubsets];
+#pragma unroll
+    for (int q = 0; q < kMaxNestSubsets; ++q) acc[q] = gram4_t{0.0, 0.0, 0.0, 0.0};
+    unsigned long long nclusters[kMaxNestSubsets] = {0, 0, 0, 0};
     const int lane = threadIdx.x & (kWave - 1);
     double* mystage = gst + (size_t)(threadIdx.x / kWave) * gram_stage<KT>();
-    unsigned long long nclusters = 0;
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
     // software pipeline: the next tile's rows are loaded while the current tile is reduced
     int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
@@ -398,58 +408,73 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
     for (; t < a.n_wtiles; t += wstride) {
         load_nest_rows<KT, TX>(a, X, t + wstride, lane, nxt);
         const int nr = (int)(cur.rec >> 32);
-        const int ip = cur.ip, k0 = cur.k0, k1 = cur.k1;
+        const int ip = cur.ip, ka = cur.k0, kb = cur.k1;
         const bool wrow = cur.w;
-        double sv[KT];
+        double tv[KT];
 #pragma unroll
-        for (int k = 0; k < KT; ++k) sv[k] = cur.t[k];
+        for (int k = 0; k < KT; ++k) tv[k] = cur.t[k];
         cur = nxt;
         if (nr <= 0) continue;
         const bool valid = lane < nr;
-        const int ipp = __shfl_up(ip, 1, kWave), k0p = __shfl_up(k0, 1, kWave), k1p = __shfl_up(k1, 1, kWave);
+        const int ipp = __shfl_up(ip, 1, kWave), kap = __shfl_up(ka, 1, kWave), kbp = __shfl_up(kb, 1, kWave);
         const bool hs = valid && (lane == 0 || ip != ipp);
         const unsigned long long H = __ballot(hs);
         const int ss = 63 - __clzll(H & mask_le(lane));  // segment start lane
-        const bool rh = valid && (hs || k0 != k0p || k1 != k1p);
-        const unsigned long long HR = __ballot(rh);
-        const unsigned long long le = HR & mask_le(lane);
-        const int rs = 63 - __clzll(le);
-        const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
         const unsigned long long BW = __ballot(wrow);
-        bool ne = (BW & mask_le(lane) & ~((1ull << rs) - 1ull)) != 0;  // run holds a weighted row
-        if (a.mode != 2) seg_scan_cols<KT>(sv, ScanFactors(ScanMask(lane, rs)));
-        bool leader = rend;
-        if (a.nk > 0 && a.mode == 0) {
-            // merge the runs of one segment that carry the same ids
-            unsigned long long E = __ballot(rend);
-            while (E) {
-                const int j = __ffsll((long long)E) - 1;
-                E &= E - 1ull;
-                const int kj0 = __shfl(k0, j, kWave), kj1 = __shfl(k1, j, kWave), sj = __shfl(ss, j, kWave);
-                const bool match = rend && lane != j && sj == ss && kj0 == k0 && kj1 == k1;
-                if (match && j < lane) leader = false;
-                const bool take = match && j > lane;
-                if (__ballot(take)) {
 #pragma unroll
-                    for (int k = 0; k < KT; ++k) {
-                        const double vj = __shfl(sv[k], j, kWave);
-                        if (take) sv[k] += vj;
+        for (int q = 0; q < kMaxNestSubsets; ++q) {
+            if (q >= a.nsub) break;
+            const int use = a.use[q], mode = a.mode[q];
+            const int k0 = (use & 1) ? ka : -1, k1 = (use & 2) ? kb : -1;
+            const int k0p = (use & 1) ? kap : -1, k1p = (use & 2) ? kbp : -1;
+            const bool rh = valid && (hs || k0 != k0p || k1 != k1p);
+            const unsigned long long HR = __ballot(rh);
+            const unsigned long long le = HR & mask_le(lane);
+            const int rs = 63 - __clzll(le);
+            const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
+            bool ne = (BW & mask_le(lane) & ~((1ull << rs) - 1ull)) != 0;  // run holds a weighted row
+            double sv[KT];
+#pragma unroll
+            for (int k = 0; k < KT; ++k) sv[k] = tv[k];
+            if (mode != 2) seg_scan_cols<KT>(sv, ScanFactors(ScanMask(lane, rs)));
+            bool leader = rend;
+            if (use != 0 && mode == 0) {
+                // merge the runs of one segment that carry the same ids
+                unsigned long long E = __ballot(rend);
+                while (E) {
+                    const int j = __ffsll((long long)E) - 1;
+                    E &= E - 1ull;
+                    const int kj0 = __shfl(k0, j, kWave), kj1 = __shfl(k1, j, kWave), sj = __shfl(ss, j, kWave);
+                    const bool match = rend && lane != j && sj == ss && kj0 == k0 && kj1 == k1;
+                    if (match && j < lane) leader = false;
+                    const bool take = match && j > lane;
+                    if (__ballot(take)) {
+#pragma unroll
+                        for (int k = 0; k < KT; ++k) {
+                            const double vj = __shfl(sv[k], j, kWave);
+                            if (take) sv[k] += vj;
+                        }
+                        const bool nej = __shfl((int)ne, j, kWave) != 0;
+                        if (take) ne = ne || nej;
                     }
-                    const bool nej = __shfl((int)ne, j, kWave) != 0;
-                    if (take) ne = ne || nej;
                 }
             }
-        }
-        const bool use = leader && ne;
-        const unsigned long long BU = __ballot(use);
-        if (lane == 0) nclusters += (unsigned long long)__popcll(BU);
-        // sum of S S' over the clusters that end here: the cross products of the masked sums
+            const bool on = leader && ne;
+            const unsigned long long BU = __ballot(on);
+            if (lane == 0) nclusters[q] += (unsigned long long)__popcll(BU);
+            // sum of S S' over the clusters that end here: the cross products of the masked sums
 #pragma unroll
-        for (int k = 0; k < KT; ++k) sv[k] = use ? sv[k] : 0.0;
-        gram_wave<KT>(mystage, sv, lane, acc);
+            for (int k = 0; k < KT; ++k) sv[k] = on ? sv[k] : 0.0;
+            gram_wave<KT>(mystage, sv, lane, acc[q]);
+        }
     }
-    if (lane == 0 && nclusters) atomicAdd(ucount, nclusters);
-    gram_store_upper<KT>(gst, gram_stage<KT>(), acc, part + (int64_t)blockIdx.x * (KT * (KT + 1) / 2));
+    constexpr int NP = KT * (KT + 1) / 2;
+#pragma unroll
+    for (int q = 0; q < kMaxNestSubsets; ++q) {
+        if (q >= a.nsub) break;
+        if (lane == 0 && nclusters[q]) atomicAdd(ucount + q, nclusters[q]);
+        gram_store_upper<KT>(gst, gram_stage<KT>(), acc[q], part + ((int64_t)q * gridDim.x + blockIdx.x) * NP);
+    }
 }
 
 // Any subset through its cached sorted partition, one wave per 64 consecutive sorted positions: the
@@ -471,16 +496,49 @@ __global__ __launch_bounds__(kBlock) void sorted_wave_meat_kernel(const double* 
     double* mystage = gst + (size_t)(threadIdx.x / kWave) * gram_stage<KT>();
     const int64_t nwaves = (n + kWave - 1) / kWave;
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
-    for (int64_t w = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; w < nwaves; w += wstride) {
+    // Two-stage software pipeline: the keys and row indices of the tile after next and the score rows of
+    // the next tile are requested before the current tile is reduced, so a wave keeps its random row
+    // gathers in flight while it scans.
+    auto load_ids = [&](int64_t wt, int& key, int64_t& row) {
+        const int64_t qq = wt * kWave + lane;
+        const bool v = wt < nwaves && qq < n;
+        key = v ? keys[qq] : -1;
+        row = v ? (order ? (int64_t)order[qq] : qq) : -1;
+    };
+    auto load_rows = [&](int64_t row, double (&t)[KT]) {
+        if (K == KT) {  // full rows: 16-byte loads (KT is even, so every row starts 16-byte aligned)
+            const double2* src = reinterpret_cast<const double2*>(T + (row >= 0 ? row : 0) * KT);
+#pragma unroll
+            for (int k = 0; k < KT / 2; ++k) {
+                const double2 v = row >= 0 ? src[k] : double2{0.0, 0.0};
+                t[2 * k] = v.x;
+                t[2 * k + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < KT; ++k) t[k] = (row >= 0 && k < K) ? T[row * K + k] : 0.0;
+        }
+    };
+    int64_t w = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    int key_c, key_n;
+    int64_t row_c, row_n;
+    double sv_n[KT];
+    load_ids(w, key_c, row_c);
+    load_ids(w + wstride, key_n, row_n);
+    load_rows(row_c, sv_n);
+    for (; w < nwaves; w += wstride) {
+        double sv[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) sv[k] = sv_n[k];
+        const int key0 = key_c;
+        key_c = key_n;
+        row_c = row_n;
+        load_rows(row_c, sv_n);                   // score rows of the next tile
+        load_ids(w + 2 * wstride, key_n, row_n);  // ids of the tile after next
         const int64_t q0 = w * kWave;
         const int nvalid = (int)(n - q0 < kWave ? n - q0 : kWave);
         const bool valid = lane < nvalid;
-        const int64_t q = q0 + (valid ? lane : 0);
-        const int key = valid ? keys[q] : -1;
-        const int64_t row = order ? (int64_t)order[q] : q;
-        double sv[KT];
-#pragma unroll
-        for (int k = 0; k < KT; ++k) sv[k] = (valid && k < K) ? T[row * K + k] : 0.0;
+        const int key = valid ? key0 : -1;
         int kprev = __shfl_up(key, 1, kWave);
         if (lane == 0) kprev = q0 > 0 ? keys[q0 - 1] : -2;
         const bool head = valid && key != kprev;
@@ -1450,9 +1508,89 @@ int finish_request(Panel* p, ReqState& s) {
         if ((e = db.get(&Md, (size_t)K * K))) return e;
         std::vector<double> M((size_t)K * K), tmp((size_t)kept * kept);
         int64_t min_single = 0x7fffffff;
-        std::vector<double> M_single;  // meat of a subset whose clusters are single rows
-        int64_t ncl_single = 0;
-        bool single_done = false;
+        // Subsets with the primary dim on wave tiles: one fused pass forms all of their meats (a subset
+        // whose clusters are single rows repeats the meat of the first such subset).
+        std::map<int, std::pair<std::vector<double>, int64_t>> nest_pre;  // subset mask -> (meat, clusters)
+        if (p->n_long == 0 && p->n_mtiles == 0 && K <= 12) {
+            WaveNest wn{};
+            wn.r = r;
+            wn.ldX = n;
+            wn.K = K;
+            wn.idP = p->d[p->P].ids;
+            wn.wtile_rec = p->wtile_rec;
+            wn.n_wtiles = p->n_wtiles;
+            wn.w = wpanel;
+            std::vector<int> fused, alias;
+            int single_q = -1;
+            for (int mask = 1; mask < (1 << ncd); ++mask) {
+                unsigned dims_mask = 0;
+                for (int j = 0; j < ncd; ++j)
+                    if (mask & (1 << j)) dims_mask |= 1u << (cdims[j] - 1);
+                if (!((dims_mask >> p->P) & 1u)) continue;
+                const int* kid[2] = {nullptr, nullptr};
+                int nk = 0, use = 0;
+                for (int d = 0; d < p->m; ++d) {
+                    if (d == p->P || !((dims_mask >> d) & 1u)) continue;
+                    kid[nk++] = p->d[d].ids;
+                    int slot = -1;
+                    for (int j = 0; j < 2; ++j)
+                        if (wn.kd[j] == p->d[d].ids) slot = j;
+                    for (int j = 0; j < 2 && slot < 0; ++j)
+                        if (!wn.kd[j]) {
+                            wn.kd[j] = p->d[d].ids;
+                            slot = j;
+                        }
+                    use |= 1 << slot;  // at most two other dims among three cluster dims
+                }
+                int mode = 1;
+                if ((e = nest_mode(p, dims_mask, kid, nk, ucount, &mode))) return e;
+                if (mode == 2 && single_q >= 0) {
+                    alias.push_back(mask);
+                    continue;
+                }
+                if (mode == 2) single_q = (int)fused.size();
+                wn.use[fused.size()] = use;
+                wn.mode[fused.size()] = mode;
+                fused.push_back(mask);
+            }
+            wn.nsub = (int)fused.size();
+            if (wn.nsub > 0) {
+                const unsigned grid = (unsigned)std::max<int64_t>(
+                    1, std::min<int64_t>((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave), kMeatBlocks));
+                const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
+                const int NP = KT * (KT + 1) / 2;
+                double* npart = nullptr;
+                unsigned long long* ucounts = nullptr;
+                if ((e = db.get(&npart, (size_t)wn.nsub * grid * NP)) || (e = db.get(&ucounts, kMaxNestSubsets)))
+                    return e;
+                FEHIP_CHECK(hipMemsetAsync(ucounts, 0, sizeof(unsigned long long) * kMaxNestSubsets, 0), "memset");
+                const float* X32 = X ? nullptr : (const float*)(work32 + n);
+#define FEHIP_WNEST(KTV)                                                                                       \
+    if (X32)                                                                                                   \
+        hipLaunchKernelGGL((wave_nested_meat_kernel<KTV, float>), dim3(grid), dim3(kBlock), 0, 0, wn, X32, npart, \
+                           ucounts);                                                                           \
+    else                                                                                                       \
+        hipLaunchKernelGGL((wave_nested_meat_kernel<KTV, double>), dim3(grid), dim3(kBlock), 0, 0, wn, X, npart,  \
+                           ucounts);
+                if (KT == 4) {
+                    FEHIP_WNEST(4)
+                } else if (KT == 8) {
+                    FEHIP_WNEST(8)
+                } else {
+                    FEHIP_WNEST(12)
+                }
+#undef FEHIP_WNEST
+                FEHIP_LAUNCH_CHECK("wave nested meat");
+                unsigned long long hc[kMaxNestSubsets] = {0, 0, 0, 0};
+                FEHIP_CHECK(copy_sync(hc, ucounts, sizeof(hc), hipMemcpyDeviceToHost), "count copy");
+                for (int q = 0; q < wn.nsub; ++q) {
+                    std::vector<double> Mq;
+                    if ((e = reduce_upper(npart + (size_t)q * grid * NP, (int)grid, KT, K, Mq))) return e;
+                    nest_pre[fused[q]] = std::make_pair(Mq, (int64_t)hc[q]);
+                }
+                for (int m2 : alias) nest_pre[m2] = nest_pre[fused[single_q]];
+            }
+        }
         for (int mask = 1; mask < (1 << ncd); ++mask) {
             unsigned dims_mask = 0;
             int len = 0;
@@ -1473,72 +1611,9 @@ int finish_request(Panel* p, ReqState& s) {
                 }
             const int single = len == 1 ? __builtin_ctz(dims_mask) : -1;
             int64_t ncl = 0;
-            int nother = 0;
-            for (int d = 0; d < p->m; ++d) nother += (d != p->P && ((dims_mask >> d) & 1u)) ? 1 : 0;
-            if (has_p && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2) {
-                // how the subset's clusters lie in the segments (cached per panel)
-                WaveNest wn{};
-                wn.r = r;
-                wn.ldX = n;
-                wn.K = K;
-                wn.idP = p->d[p->P].ids;
-                wn.seg_off = p->seg_off;
-                wn.wtile_rec = p->wtile_rec;
-                wn.n_wtiles = p->n_wtiles;
-                for (int d = 0; d < p->m; ++d)
-                    if (d != p->P && ((dims_mask >> d) & 1u)) wn.kid[wn.nk++] = p->d[d].ids;
-                wn.w = wpanel;
-                if ((e = nest_mode(p, dims_mask, wn.kid, wn.nk, ucount, &wn.mode))) return e;
-            }
-            if (has_p && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2 && single_done &&
-                cached_nest_mode(p, dims_mask) == 2) {
-                M = M_single;  // every row its own cluster: the same meat as an earlier subset
-                ncl = ncl_single;
-            } else if (has_p && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2) {
-                WaveNest wn{};
-                wn.r = r;
-                wn.ldX = n;
-                wn.K = K;
-                wn.idP = p->d[p->P].ids;
-                wn.seg_off = p->seg_off;
-                wn.wtile_rec = p->wtile_rec;
-                wn.n_wtiles = p->n_wtiles;
-                for (int d = 0; d < p->m; ++d)
-                    if (d != p->P && ((dims_mask >> d) & 1u)) wn.kid[wn.nk++] = p->d[d].ids;
-                wn.w = wpanel;
-                {
-                    std::lock_guard<std::recursive_mutex> guard(p->mu);
-                    wn.mode = p->nest_modes[dims_mask];
-                }
-                FEHIP_CHECK(hipMemsetAsync(ucount, 0, sizeof(unsigned long long), 0), "memset");
-                const unsigned grid = (unsigned)std::max<int64_t>(
-                    1, std::min<int64_t>((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave), kMeatBlocks));
-                const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
-                const float* X32 = X ? nullptr : (const float*)(work32 + n);
-#define FEHIP_WNEST(KTV)                                                                                        \
-    if (X32)                                                                                                    \
-        hipLaunchKernelGGL((wave_nested_meat_kernel<KTV, float>), dim3(grid), dim3(kBlock), 0, 0, wn, X32, mpart, \
-                           ucount);                                                                             \
-    else                                                                                                        \
-        hipLaunchKernelGGL((wave_nested_meat_kernel<KTV, double>), dim3(grid), dim3(kBlock), 0, 0, wn, X, mpart, ucount);
-                if (KT == 4) {
-                    FEHIP_WNEST(4)
-                } else if (KT == 8) {
-                    FEHIP_WNEST(8)
-                } else {
-                    FEHIP_WNEST(12)
-                }
-#undef FEHIP_WNEST
-                FEHIP_LAUNCH_CHECK("wave nested meat");
-                if ((e = reduce_upper(mpart, (int)grid, KT, K, M))) return e;
-                unsigned long long h = 0;
-                FEHIP_CHECK(copy_sync(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
-                ncl = (int64_t)h;
-                if (wn.mode == 2) {
-                    M_single = M;
-                    ncl_single = ncl;
-                    single_done = true;
-                }
+            if (nest_pre.count(mask)) {
+                M = nest_pre[mask].first;
+                ncl = nest_pre[mask].second;
             } else if (has_p && p->n_long == 0 && K <= 12) {
                 if ((e = need_T())) return e;
                 NestKeys nk{};

@@ -22,14 +22,24 @@ constexpr int gram_stage() {
     return 64 * CT > 256 ? 64 * CT : 256;
 }
 
-// one wave: V'V of the 64 rows st[row * CT + col] (CT <= 16) added to acc
+// one wave: V'V of the 64 rows st[row * CT + col] (CT <= 16) added to acc.  With CT <= 8 (CT <= 4) one
+// instruction serves two (four) groups of four rows, each group in its own diagonal block of D (the
+// off-diagonal blocks are discarded), so 8 (4) instructions cover the 64 rows instead of 16.
+template <int CT>
+constexpr int gram_block() {
+    return CT <= 4 ? 4 : (CT <= 8 ? 8 : 16);
+}
+
 template <int CT>
 __device__ __forceinline__ void gram_rows64(const double* st, int lane, gram4_t& acc) {
     static_assert(CT <= 16, "at most 16 columns per accumulator");
+    constexpr int B = gram_block<CT>(), G = 16 / B;  // block width, row groups per instruction
     const int col = lane & 15, rr = lane >> 4;
+    const int g = col / B, c = col - g * B;
 #pragma unroll
-    for (int k = 0; k < 16; ++k) {
-        const double x = col < CT ? st[(4 * k + rr) * CT + col] : 0.0;
+    for (int k = 0; k < 16 / G; ++k) {
+        const int row = 4 * G * k + 4 * g + rr;
+        const double x = c < CT ? st[row * CT + c] : 0.0;
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc, 0, 0, 0);
     }
 }
@@ -44,34 +54,44 @@ __device__ __forceinline__ void gram_wave(double* st, const double (&v)[CT], int
     __builtin_amdgcn_wave_barrier();
 }
 
-// The block's waves add their accumulators in a fixed order; the 16 x 16 sum is left in stage_all[0..255]
-// (row-major) and visible to every thread.  stage_all holds blockDim.x / 64 records of `stage` doubles.
+// The block's waves add their accumulators in a fixed order and the diagonal blocks are folded; the
+// CT x CT result is left in stage_all[i * 16 + j] (i, j < CT) and visible to every thread.  stage_all
+// holds blockDim.x / 64 records of `stage` doubles.
+template <int CT>
 __device__ __forceinline__ void gram_block_sum(double* stage_all, int stage, const gram4_t& acc) {
+    constexpr int B = gram_block<CT>(), G = 16 / B;
     const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     __syncthreads();
     double* mine = stage_all + (size_t)wv * stage;
 #pragma unroll
     for (int v = 0; v < 4; ++v) mine[(4 * v + (lane >> 4)) * 16 + (lane & 15)] = acc[v];
     __syncthreads();
-    double t = 0.0;
     const int e = threadIdx.x;
-    if (e < 256)
-        for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += stage_all[(size_t)w * stage + e];
+    double t = 0.0;
+    if (e < 256) {
+        const int i = e >> 4, j = e & 15;
+        if (i < B && j < B)
+            for (int gg = 0; gg < G; ++gg) {
+                const int f = (gg * B + i) * 16 + gg * B + j;
+                for (int w = 0; w < (int)(blockDim.x / kWave); ++w) t += stage_all[(size_t)w * stage + f];
+            }
+    }
     __syncthreads();
     if (e < 256) stage_all[e] = t;
     __syncthreads();
 }
 
-// the block's record of 256 doubles
+// the block's record of 256 doubles (the CT x CT corner is meaningful)
+template <int CT>
 __device__ __forceinline__ void gram_store_block(double* stage_all, int stage, const gram4_t& acc, double* rec) {
-    gram_block_sum(stage_all, stage, acc);
+    gram_block_sum<CT>(stage_all, stage, acc);
     for (int e = threadIdx.x; e < 256; e += blockDim.x) rec[e] = stage_all[e];
 }
 
 // the block's upper triangle (i <= j < KT, row by row) as KT (KT + 1) / 2 doubles
 template <int KT>
 __device__ __forceinline__ void gram_store_upper(double* stage_all, int stage, const gram4_t& acc, double* out) {
-    gram_block_sum(stage_all, stage, acc);
+    gram_block_sum<KT>(stage_all, stage, acc);
     for (int e = threadIdx.x; e < KT * (KT + 1) / 2; e += blockDim.x) {
         int i = 0, rem = e;
         while (rem >= KT - i) {

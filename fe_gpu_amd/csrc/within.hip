@@ -388,7 +388,7 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
         for (int e = threadIdx.x; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
     } else {
-        if (a.gram) gram_store_block(gstage, gram_stage<CT>(), gacc, a.gram + (size_t)(a.part_base + blockIdx.x) * 256);
+        if (a.gram) gram_store_block<CT>(gstage, gram_stage<CT>(), gacc, a.gram + (size_t)(a.part_base + blockIdx.x) * 256);
     }
 }
 
@@ -517,7 +517,7 @@ __global__ __launch_bounds__(kBlock) void tile_fast_kernel(ApplyArgs a) {
         for (int e = j; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
     } else {
         if (a.gram)
-            gram_store_block(dyn + (size_t)a.Gs * CT * 2, gram_stage<CT>(), gacc,
+            gram_store_block<CT>(dyn + (size_t)a.Gs * CT * 2, gram_stage<CT>(), gacc,
                              a.gram + (size_t)(a.part_base + blockIdx.x) * 256);
     }
 }
