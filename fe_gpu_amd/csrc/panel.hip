@@ -125,6 +125,23 @@ __global__ void write_runs_kernel(const int64_t* __restrict__ seg_off, int64_t G
     }
 }
 
+// Per row: the group-order position of the run the row ends (-1 if the run goes on), so a kernel that
+// writes run partials finds each destination with the row's own ids instead of two dependent loads.
+__global__ void row_run_dest_kernel(const int64_t* __restrict__ seg_off, int64_t G, const int* __restrict__ ids,
+                                    const int64_t* __restrict__ run_off, const int* __restrict__ run_dest,
+                                    int* row_rdest) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < G; s += stride) {
+        const int64_t r0 = seg_off[s], r1 = seg_off[s + 1];
+        int64_t k = run_off[s];
+        for (int64_t r = r0; r < r1; ++r) {
+            const bool end = r + 1 == r1 || ids[r + 1] != ids[r];
+            row_rdest[r] = end ? run_dest[k] : -1;
+            if (end) ++k;
+        }
+    }
+}
+
 // dest[order[k]] = k
 __global__ void invert_order_kernel(const int* __restrict__ order, int64_t n, int* dest) {
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
@@ -213,6 +230,7 @@ void free_panel(Panel* p) {
         dfree(p->d[d].seg_run_off);
         dfree(p->d[d].run_order);
         dfree(p->d[d].run_dest);
+        dfree(p->d[d].row_rdest);
         dfree(p->d[d].grp_run_off);
     }
     delete p;
@@ -279,6 +297,10 @@ int build_runs(Panel* p, int d, hipStream_t s) {
     hipLaunchKernelGGL(invert_order_kernel, dim3(stream_grid((size_t)R, kBlock)), dim3(kBlock), 0, s,
                        (const int*)D.run_order, R, D.run_dest);
     FEHIP_LAUNCH_CHECK("run destinations");
+    if ((st = dalloc(&D.row_rdest, (size_t)p->n, &p->bytes))) return st;
+    hipLaunchKernelGGL(row_run_dest_kernel, dim3(stream_grid(GP, kBlock)), dim3(kBlock), 0, s, p->seg_off, GP, D.ids,
+                       (const int64_t*)D.seg_run_off, (const int*)D.run_dest, D.row_rdest);
+    FEHIP_LAUNCH_CHECK("row run destinations");
     FEHIP_CHECK(hipStreamSynchronize(s), "runs sync");
     dfree(grp);
     dfree(grp2);

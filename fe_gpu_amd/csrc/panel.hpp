@@ -30,6 +30,7 @@ struct DimStruct {
     int* run_order = nullptr;        // run indices sorted by group
     int64_t* grp_run_off = nullptr;  // [G + 1] offsets into run_order
     int* run_dest = nullptr;         // position of each run in group order (inverse of run_order)
+    int* row_rdest = nullptr;        // [n] per row: run_dest of the run it ends, -1 inside a run
 };
 
 // Row partition of a cluster subset: rows of cluster c are order[off[c] .. off[c+1]).

@@ -50,6 +50,7 @@ struct ScatterDim {
     int large_slot;              // large: 0..kMaxLarge-1
     const int64_t* seg_run_off;  // large
     const int* run_dest;         // large: group-order position of each run
+    const int* row_rdest;        // large: per row, run_dest of the run it ends (-1 inside a run)
     double* runbuf;              // large: [n_runs][CT] in group order
 };
 
@@ -124,21 +125,23 @@ __device__ __forceinline__ void row_value(const ApplyArgs& a, int64_t r, const d
 }
 
 // Row data of one wave tile, loaded one tile ahead of its use.
+template <int ND>
 struct WaveRows {
+    static constexpr int NL = ND < kMaxLarge ? ND : kMaxLarge;
     int64_t rec;               // first row | rows << 32
-    int64_t s0;                // first segment
     int ip;                    // primary id
     uint32_t word;             // mask word
-    int g[kMaxScatter];        // ids of the scattered dims
+    int g[ND];                 // ids of the scattered dims
+    int rd[NL];                // run destination of the row per large dim (-1 inside a run)
 };
 
-__device__ __forceinline__ void load_wave_rows(const ApplyArgs& a, int64_t t, int lane, WaveRows& w) {
+template <int ND>
+__device__ __forceinline__ void load_wave_rows(const ApplyArgs& a, int64_t t, int lane, WaveRows<ND>& w) {
     if (t >= a.n_wtiles) {
         w.rec = 0;
         return;
     }
     w.rec = a.wtile_rec[t];
-    w.s0 = a.wtile_seg[2 * t];
     const int64_t R0 = w.rec & 0xffffffffll;
     const int nr = (int)(w.rec >> 32);
     const bool valid = lane < nr;
@@ -146,13 +149,23 @@ __device__ __forceinline__ void load_wave_rows(const ApplyArgs& a, int64_t t, in
     w.ip = valid ? a.idP[r] : -1;
     w.word = valid ? row_word(a, r) : 0u;
 #pragma unroll
-    for (int o = 0; o < kMaxScatter; ++o)
+    for (int o = 0; o < ND; ++o)
         if (o < a.nd) w.g[o] = valid ? a.sd[o].ids[r] : -1;
+#pragma unroll
+    for (int k = 0; k < WaveRows<ND>::NL; ++k) w.rd[k] = -1;
+#pragma unroll
+    for (int o = 0; o < ND; ++o) {
+        if (o >= a.nd) break;
+        if (a.sd[o].small) continue;
+#pragma unroll
+        for (int k = 0; k < WaveRows<ND>::NL; ++k)
+            if (k == a.sd[o].large_slot) w.rd[k] = valid ? a.sd[o].row_rdest[r] : -1;
+    }
 }
 
 // row_value with the group ids already in registers
-template <int CT, int MODE>
-__device__ __forceinline__ void row_value_ids(const ApplyArgs& a, int64_t r, const int (&g)[kMaxScatter],
+template <int CT, int MODE, int ND>
+__device__ __forceinline__ void row_value_ids(const ApplyArgs& a, int64_t r, const int (&g)[ND],
                                               const double* sp, double (&v)[CT]) {
 #pragma unroll
     for (int c = 0; c < CT; ++c) v[c] = 0.0;
@@ -165,7 +178,7 @@ __device__ __forceinline__ void row_value_ids(const ApplyArgs& a, int64_t r, con
         const double sign = MODE == kIter ? 1.0 : -1.0;
         const double* tab = MODE == kIter ? a.p : a.alpha;
 #pragma unroll
-        for (int o = 0; o < kMaxScatter; ++o) {
+        for (int o = 0; o < ND; ++o) {
             if (o >= a.nd) break;
             const ScatterDim& d = a.sd[o];
             if (d.small) {
@@ -230,8 +243,10 @@ __device__ __forceinline__ int tile_runs(bool valid, bool rhead, int j, int nr, 
 // every lane of its segment by one lane permute, and the weighted row counts come from ballots.
 // Nothing is staged in LDS except the small-dim tables; the waves of a block run independently.
 
-template <int CT, int MODE>
+template <int CT, int MODE, int ND>
 __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
+    // ND: compile-time bound on the scattered dims (a.nd <= ND), so their loops unroll to what is used
+    constexpr int NL = WaveRows<ND>::NL;
     if constexpr (MODE == kIter) {
         if (!any_active<CT>(a.active)) return;
     }
@@ -258,21 +273,22 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     // software pipeline: the next tile's row data (record, ids, mask words) is loaded while the current
     // tile is processed, so a tile waits only for its table gathers
     int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    WaveRows cur, nxt;
+    WaveRows<ND> cur, nxt;
     load_wave_rows(a, t, lane, cur);
     for (; t < a.n_wtiles; t += wstride) {
         load_wave_rows(a, t + wstride, lane, nxt);
         const int64_t R0 = cur.rec & 0xffffffffll;
         const int nr = (int)(cur.rec >> 32);
-        const int64_t s0c = cur.s0;
         const bool valid = lane < nr;
         const int64_t r = R0 + (valid ? lane : 0);
         const uint32_t word = cur.word;
         const bool on = (word & anyb) != 0;
         const int ip = cur.ip;
-        int gid[kMaxScatter];
+        int gid[ND], rdl[NL];
 #pragma unroll
-        for (int o = 0; o < kMaxScatter; ++o) gid[o] = o < a.nd ? cur.g[o] : -1;
+        for (int o = 0; o < ND; ++o) gid[o] = o < a.nd ? cur.g[o] : -1;
+#pragma unroll
+        for (int k = 0; k < NL; ++k) rdl[k] = cur.rd[k];
         cur = nxt;
         if (nr <= 0) continue;  // a tile of empty segments (uniform across the wave)
         // segment of this lane: [ss, se]
@@ -285,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         double v[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) v[c] = 0.0;
-        if (on) row_value_ids<CT, MODE>(a, r, gid, sp, v);
+        if (on) row_value_ids<CT, MODE, ND>(a, r, gid, sp, v);
 #pragma unroll
         for (int c = 0; c < CT; ++c) v[c] = (word & cb[c]) ? v[c] : 0.0;
         // One segmented scan per column gives the prefix x = sum of v over [segment start, lane]: its value
@@ -294,35 +310,31 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         bool any_large = false;
         if constexpr (MODE != kFinal)
             for (int o = 0; o < a.nd; ++o) any_large |= !a.sd[o].small;
-        double x[CT], mean[CT], u[CT];
-        unsigned long long bw[CT];  // rows of each column's slot
-        {
+        double x[CT];
 #pragma unroll
-            for (int c = 0; c < CT; ++c) x[c] = v[c];
-            if (a.elim || any_large) seg_scan_cols<CT>(x, ScanFactors(ScanMask(lane, ss)));
-            const unsigned long long segmask = mask_le(se) & ~((1ull << ss) - 1ull);
-            double inv = 0.0, inslot = 0.0;
-            uint32_t cnt_bit = 0u;
-            unsigned long long B = 0ull;
-#pragma unroll
-            for (int c = 0; c < CT; ++c) {
-                mean[c] = 0.0;
-                bw[c] = 0ull;
-                u[c] = 0.0;
-                if (cb[c] == 0u) continue;  // uniform across the wave
-                if (cb[c] != cnt_bit) {     // columns of one slot share the row counts
-                    B = __ballot((word & cb[c]) != 0);
-                    const int cnt = __popcll(B & segmask);
-                    inv = cnt > 0 ? 1.0 / (double)cnt : 0.0;
-                    inslot = (word & cb[c]) ? 1.0 : 0.0;
-                    cnt_bit = cb[c];
-                }
-                bw[c] = B;
-                if (a.elim) mean[c] = __shfl(x[c], se, kWave) * inv;
-                u[c] = fma(-mean[c], inslot, v[c]);  // v is 0 outside the slot
-            }
-        }
+        for (int c = 0; c < CT; ++c) x[c] = v[c];
+        if (a.elim || any_large) seg_scan_cols<CT>(x, ScanFactors(ScanMask(lane, ss)));
+        const unsigned long long segmask = mask_le(se) & ~((1ull << ss) - 1ull);
         if constexpr (MODE == kFinal) {
+            double u[CT];
+            {
+                double inv = 0.0, inslot = 0.0;
+                uint32_t cnt_bit = 0u;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) {
+                    u[c] = 0.0;
+                    if (cb[c] == 0u) continue;  // uniform across the wave
+                    if (cb[c] != cnt_bit) {     // columns of one slot share the row counts
+                        const unsigned long long B = __ballot((word & cb[c]) != 0);
+                        const int cnt = __popcll(B & segmask);
+                        inv = cnt > 0 ? 1.0 / (double)cnt : 0.0;
+                        inslot = (word & cb[c]) ? 1.0 : 0.0;
+                        cnt_bit = cb[c];
+                    }
+                    const double mean = a.elim ? __shfl(x[c], se, kWave) * inv : 0.0;
+                    u[c] = fma(-mean, inslot, v[c]);  // v is 0 outside the slot
+                }
+            }
             if (valid) {
                 const int64_t src = a.perm ? a.perm[r] : r;
 #pragma unroll
@@ -339,46 +351,76 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
             if (a.gram)  // uniform; u is 0 on rows past the tile and outside each column's slot
                 gram_wave<CT>(gstage + (size_t)(threadIdx.x / kWave) * gram_stage<CT>(), u, lane, gacc);
         } else {
+            // Per scattered dim: the small dim's LDS row, or the large dim's run geometry.  A new run starts
+            // at a segment head or where the id changes; the run sum of u is
+            // x(run end) - x(run start - 1) - (rows of the slot in the run) * mean.
+            double* tq[ND];
+            double* dst[ND];
+            int pl[ND];
+            double keep_pre[ND];
+            unsigned long long runmask[ND];
+            bool rend[ND];
 #pragma unroll
-            for (int o = 0; o < kMaxScatter; ++o) {
-                if (o >= a.nd) break;
+            for (int o = 0; o < ND; ++o) {
+                tq[o] = nullptr;
+                dst[o] = nullptr;
+                pl[o] = 0;
+                keep_pre[o] = 0.0;
+                runmask[o] = 0ull;
+                rend[o] = false;
+                if (o >= a.nd) continue;
                 const ScatterDim& d = a.sd[o];
                 const int g = gid[o];
                 if (d.small) {
-                    if (on) {
-                        double* tq = sacc + (size_t)(d.lds_off + g) * CT;
-#pragma unroll
-                        for (int c = 0; c < CT; ++c)
-                            if (word & cb[c]) lds_add(&tq[c], u[c]);
-                    }
+                    tq[o] = sacc + (size_t)(d.lds_off + (on ? g : 0)) * CT;
                     continue;
                 }
-                // runs: a new run starts at a segment head or where the id changes; the run sum of u is
-                // x(run end) - x(run start - 1) - (rows of the slot in the run) * mean
                 const int gp = __shfl_up(g, 1, kWave);
                 const bool rh = valid && (hs || g != gp);
                 const unsigned long long HR = __ballot(rh);
                 const unsigned long long le = HR & mask_le(lane);
                 const int rs = 63 - __clzll(le);
-                const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
-                const int64_t slot = d.seg_run_off[s0c] + (__popcll(le) - 1);
-                const bool rfirst = rs == ss;
-                const int pl = rs > 0 ? rs - 1 : 0;
-                const unsigned long long runmask = mask_le(lane) & ~((1ull << rs) - 1ull);
-                const double keep_pre = rfirst ? 0.0 : 1.0;
-                double* dst = d.runbuf + (int64_t)(rend ? d.run_dest[slot] : 0) * CT;
-                double rcnt = 0.0;
-                uint32_t rc_bit = 0u;
+                rend[o] = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
+                int rdest = -1;  // the row's run destination, loaded with its ids
 #pragma unroll
-                for (int c = 0; c < CT; ++c) {
-                    if (cb[c] == 0u) continue;
-                    if (cb[c] != rc_bit) {  // rows of the slot in the run, shared by the slot's columns
-                        rcnt = (double)__popcll(bw[c] & runmask);
-                        rc_bit = cb[c];
+                for (int k = 0; k < NL; ++k)
+                    if (k == d.large_slot) rdest = rdl[k];
+                pl[o] = rs > 0 ? rs - 1 : 0;
+                runmask[o] = mask_le(lane) & ~((1ull << rs) - 1ull);
+                keep_pre[o] = rs == ss ? 0.0 : 1.0;
+                dst[o] = d.runbuf + (int64_t)(rend[o] ? rdest : 0) * CT;
+            }
+            // column by column: mean, the row's value u, its small-dim adds and its run sums, so only the
+            // scan prefixes and the row values stay live across the columns
+            double inv = 0.0, inslot = 0.0;
+            double rcnt[ND];
+#pragma unroll
+            for (int o = 0; o < ND; ++o) rcnt[o] = 0.0;
+            uint32_t cnt_bit = 0u;
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                if (cb[c] == 0u) continue;  // uniform across the wave
+                if (cb[c] != cnt_bit) {     // columns of one slot share the row counts
+                    const unsigned long long B = __ballot((word & cb[c]) != 0);
+                    const int cnt = __popcll(B & segmask);
+                    inv = cnt > 0 ? 1.0 / (double)cnt : 0.0;
+                    inslot = (word & cb[c]) ? 1.0 : 0.0;
+                    cnt_bit = cb[c];
+#pragma unroll
+                    for (int o = 0; o < ND; ++o) rcnt[o] = (double)__popcll(B & runmask[o]);
+                }
+                const double mean = a.elim ? __shfl(x[c], se, kWave) * inv : 0.0;
+                const double u = fma(-mean, inslot, v[c]);  // v is 0 outside the slot
+#pragma unroll
+                for (int o = 0; o < ND; ++o) {
+                    if (o >= a.nd) break;
+                    if (tq[o]) {
+                        if (on && (word & cb[c])) lds_add(&tq[o][c], u);
+                    } else {
+                        const double pre = __shfl(x[c], pl[o], kWave);
+                        const double ru = fma(-rcnt[o], mean, fma(-pre, keep_pre[o], x[c]));
+                        if (rend[o]) dst[o][c] = ru;
                     }
-                    const double pre = __shfl(x[c], pl, kWave);
-                    const double ru = fma(-rcnt, mean[c], fma(-pre, keep_pre, x[c]));
-                    if (rend) dst[c] = ru;
                 }
             }
         }
@@ -1121,6 +1163,7 @@ int make_layout(Panel* p, unsigned fe_mask, Layout& L) {
         s.runbuf = nullptr;
         s.seg_run_off = nullptr;
         s.run_dest = nullptr;
+        s.row_rdest = nullptr;
         L.S += s.G;
         if (s.small) {
             s.lds_off = L.Gs;
@@ -1131,6 +1174,7 @@ int make_layout(Panel* p, unsigned fe_mask, Layout& L) {
             s.large_slot = large++;
             s.seg_run_off = p->d[d].seg_run_off;
             s.run_dest = p->d[d].run_dest;
+            s.row_rdest = p->d[d].row_rdest;
             s.lds_off = 0;
         }
         L.dims[L.nd++] = d;
@@ -1200,7 +1244,14 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
     size_t lt = tables_lds<CT>(L);
     if (MODE == kFinal && a.gram) lt += sizeof(double) * (size_t)(kBlock / kWave) * gram_stage<CT>();
     a.part_base = 0;
-    if (gw > 0) hipLaunchKernelGGL((wave_tile_kernel<CT, MODE>), dim3(gw), dim3(kBlock), lt, s, a);
+    if (gw > 0) {
+        if (a.nd <= 2)
+            hipLaunchKernelGGL((wave_tile_kernel<CT, MODE, 2>), dim3(gw), dim3(kBlock), lt, s, a);
+        else if (a.nd == 3)
+            hipLaunchKernelGGL((wave_tile_kernel<CT, MODE, 3>), dim3(gw), dim3(kBlock), lt, s, a);
+        else
+            hipLaunchKernelGGL((wave_tile_kernel<CT, MODE, kMaxScatter>), dim3(gw), dim3(kBlock), lt, s, a);
+    }
     if (gm > 0) {
         ApplyArgs b = a;
         b.tile_seg = p->mtile_seg;
