@@ -63,6 +63,22 @@ def cpu_baseline(n_workers_full, n_periods, n_firms_full, sample_workers, seed):
     }
 
 
+TRAFFIC_PROFILE = "profiles/apply_iter_150M_c2_hbm_bytes.txt"
+
+
+def measured_traffic(n_rows):
+    """HBM bytes per launch of the CG operator product on the 150M-row panel, from the committed
+    FETCH_SIZE / WRITE_SIZE profile (calibrated against a copy kernel); None for other sizes or when the
+    profile is absent."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
+    if n_rows != 150_000_000 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().endswith("wave_tile_kernel<9, 0, 2>"):
+            return float(line.split()[5]) * 1e9
+    return None
+
+
 def _fit_summary(fits):
     return {"mean_fe_iterations": float(np.mean([f.res["fe_iterations"] for f in fits])),
             "max_fe_iterations": int(max(f.res["fe_iterations"] for f in fits)),
@@ -425,8 +441,9 @@ def main():
                                f"switching), K=8, fe(worker firm time), cluster(worker firm time), tol {args.tol}",
                    "rows": n, "regressors": 8, "method": "PCG in FE space (primary dim eliminated)"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom["name"],
-                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": per_launch_bytes},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(n), "kernel": dom["name"],
+                     "avg_launch_ms": avg_ms, "algorithmic_bytes_per_launch": per_launch_bytes,
+                     "traffic_source": TRAFFIC_PROFILE if measured_traffic(n) else None},
         "cpu_baseline": cpu,
         "fe_iterations": fit.res["fe_iterations"],
         "converged": fit.res["converged"],
