@@ -333,7 +333,8 @@ int build_tiles(Panel* p) {
     p->n_tiles = (int64_t)tiles.size() / 2;
     p->n_long = (int64_t)longs.size();
     // operator partition: runs of consecutive short segments (<= kWaveRows rows, empty ones included)
-    // packed greedily into wave tiles; runs of consecutive medium segments into block tiles
+    // packed greedily into wave tiles; runs of consecutive medium segments into block tiles; every tile
+    // holds at least one row
     std::vector<int64_t> wt, mt;
     s = 0;
     while (s < GP) {
@@ -352,8 +353,10 @@ int build_tiles(Panel* p) {
             ++e;
         }
         auto& v = short_seg ? wt : mt;
-        v.push_back(s);
-        v.push_back(e);
+        if (off[e] > off[s]) {  // a tile of empty segments only has nothing to process
+            v.push_back(s);
+            v.push_back(e);
+        }
         s = e;
     }
     p->n_wtiles = (int64_t)wt.size() / 2;

@@ -127,70 +127,82 @@ __device__ __forceinline__ void row_value(const ApplyArgs& a, int64_t r, const d
 // Row data of one wave tile, loaded one tile ahead of its use.
 template <int ND>
 struct WaveRows {
-    static constexpr int NL = ND < kMaxLarge ? ND : kMaxLarge;
     int64_t rec;               // first row | rows << 32
     int ip;                    // primary id
     uint32_t word;             // mask word
     int g[ND];                 // ids of the scattered dims
-    int rd[NL];                // run destination of the row per large dim (-1 inside a run)
+    int rd[ND];                // large dims: run destination of the row (-1 inside a run)
 };
 
+// The loads below are issued unconditionally from clamped, always-valid addresses and their values
+// selected afterwards: a load under a branch leaves the memory counter with a path-dependent count, and
+// the wait for it then drains every load in flight.
+
+// the tile record (first row | rows << 32), 0 past the last tile
+__device__ __forceinline__ int64_t load_wave_rec(const ApplyArgs& a, int64_t t) {
+    const int64_t v = a.wtile_rec[t < a.n_wtiles ? t : a.n_wtiles - 1];
+    return t < a.n_wtiles ? v : 0;
+}
+
+// row data of a tile whose record is already in registers, as loaded: lanes past the tile hold the
+// first row's values, and the consumer masks them (a select right after a load would wait for it)
 template <int ND>
-__device__ __forceinline__ void load_wave_rows(const ApplyArgs& a, int64_t t, int lane, WaveRows<ND>& w) {
-    if (t >= a.n_wtiles) {
-        w.rec = 0;
-        return;
-    }
-    w.rec = a.wtile_rec[t];
-    const int64_t R0 = w.rec & 0xffffffffll;
-    const int nr = (int)(w.rec >> 32);
-    const bool valid = lane < nr;
-    const int64_t r = R0 + (valid ? lane : 0);
-    w.ip = valid ? a.idP[r] : -1;
-    w.word = valid ? row_word(a, r) : 0u;
-#pragma unroll
-    for (int o = 0; o < ND; ++o)
-        if (o < a.nd) w.g[o] = valid ? a.sd[o].ids[r] : -1;
-#pragma unroll
-    for (int k = 0; k < WaveRows<ND>::NL; ++k) w.rd[k] = -1;
+__device__ __forceinline__ void load_wave_rows(const ApplyArgs& a, int64_t rec, int lane, WaveRows<ND>& w) {
+    w.rec = rec;
+    const int64_t R0 = rec & 0xffffffffll;
+    const int nr = (int)(rec >> 32);
+    const int64_t r = R0 + (lane < nr ? lane : 0);
+    w.ip = a.idP[r];
+    w.word = (a.wm ? a.wm : reinterpret_cast<const uint32_t*>(a.idP))[r];
 #pragma unroll
     for (int o = 0; o < ND; ++o) {
-        if (o >= a.nd) break;
-        if (a.sd[o].small) continue;
-#pragma unroll
-        for (int k = 0; k < WaveRows<ND>::NL; ++k)
-            if (k == a.sd[o].large_slot) w.rd[k] = valid ? a.sd[o].row_rdest[r] : -1;
+        // dims past a.nd read the primary ids (valid addresses, values unused)
+        w.g[o] = (o < a.nd ? a.sd[o].ids : a.idP)[r];
+        w.rd[o] = (o < a.nd ? a.sd[o].row_rdest : a.idP)[r];
     }
 }
 
-// row_value with the group ids already in registers
+// Row values of a wave tile in two phases.  row_loads issues every load of the row without a branch:
+// the table entries of all its groups from the global table (small dims too: their few entries stay in
+// the L1 cache) and, for kData/kFinal, the input columns, from clamped addresses.  row_combine adds them
+// up.  Between the two the caller issues the next tile's prefetch, so the wait for these loads does not
+// wait for the prefetch.
 template <int CT, int MODE, int ND>
-__device__ __forceinline__ void row_value_ids(const ApplyArgs& a, int64_t r, const int (&g)[ND],
-                                              const double* sp, double (&v)[CT]) {
-#pragma unroll
-    for (int c = 0; c < CT; ++c) v[c] = 0.0;
+struct RowLoads {
+    double t[(MODE == kIter || MODE == kFinal) ? ND : 1][CT];
+    double x[(MODE == kData || MODE == kFinal) ? CT : 1];
+};
+
+template <int CT, int MODE, int ND>
+__device__ __forceinline__ void row_loads(const ApplyArgs& a, int64_t r, const int (&g)[ND], RowLoads<CT, MODE, ND>& L) {
     if constexpr (MODE == kData || MODE == kFinal) {
         const int64_t src = a.perm ? a.perm[r] : r;
 #pragma unroll
-        for (int c = 0; c < CT; ++c) v[c] = c < a.ncols ? a.xc[c][src] : 0.0;
+        for (int c = 0; c < CT; ++c) L.x[c] = ld_global((c < a.ncols ? a.xc[c] : a.xc[0]) + src);
     }
     if constexpr (MODE == kIter || MODE == kFinal) {
-        const double sign = MODE == kIter ? 1.0 : -1.0;
         const double* tab = MODE == kIter ? a.p : a.alpha;
 #pragma unroll
         for (int o = 0; o < ND; ++o) {
-            if (o >= a.nd) break;
-            const ScatterDim& d = a.sd[o];
-            if (d.small) {
-                const size_t base = (size_t)(d.lds_off + g[o]) * CT;
+            const int64_t base = o < a.nd ? (a.sd[o].tab_off + (g[o] > 0 ? g[o] : 0)) * CT : 0;
 #pragma unroll
-                for (int c = 0; c < CT; ++c) v[c] += sign * ld_lds(&sp[base + c]);
-            } else {
-                const int64_t base = (d.tab_off + g[o]) * CT;
-#pragma unroll
-                for (int c = 0; c < CT; ++c) v[c] += sign * ld_global(&tab[base + c]);
-            }
+            for (int c = 0; c < CT; ++c) L.t[o][c] = ld_global(&tab[base + c]);
         }
+    }
+}
+
+template <int CT, int MODE, int ND>
+__device__ __forceinline__ void row_combine(const ApplyArgs& a, const RowLoads<CT, MODE, ND>& L, double (&v)[CT]) {
+#pragma unroll
+    for (int c = 0; c < CT; ++c) {
+        double s = 0.0;
+        if constexpr (MODE == kData || MODE == kFinal) s = c < a.ncols ? L.x[c] : 0.0;
+        if constexpr (MODE == kIter || MODE == kFinal) {
+            const double sign = MODE == kIter ? 1.0 : -1.0;
+#pragma unroll
+            for (int o = 0; o < ND; ++o) s += (o < a.nd ? sign : 0.0) * L.t[o][c];
+        }
+        v[c] = s;
     }
 }
 
@@ -246,7 +258,6 @@ __device__ __forceinline__ int tile_runs(bool valid, bool rhead, int j, int nr, 
 template <int CT, int MODE, int ND>
 __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     // ND: compile-time bound on the scattered dims (a.nd <= ND), so their loops unroll to what is used
-    constexpr int NL = WaveRows<ND>::NL;
     if constexpr (MODE == kIter) {
         if (!any_active<CT>(a.active)) return;
     }
@@ -273,24 +284,40 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     // software pipeline: the next tile's row data (record, ids, mask words) is loaded while the current
     // tile is processed, so a tile waits only for its table gathers
     int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    // records two tiles ahead, row data one tile ahead: loading the next tile's rows never waits for a
+    // record requested in the same iteration
     WaveRows<ND> cur, nxt;
-    load_wave_rows(a, t, lane, cur);
+    load_wave_rows(a, load_wave_rec(a, t), lane, cur);
+    int64_t rec_n = load_wave_rec(a, t + wstride);
     for (; t < a.n_wtiles; t += wstride) {
-        load_wave_rows(a, t + wstride, lane, nxt);
         const int64_t R0 = cur.rec & 0xffffffffll;
         const int nr = (int)(cur.rec >> 32);
         const bool valid = lane < nr;
         const int64_t r = R0 + (valid ? lane : 0);
-        const uint32_t word = cur.word;
+        const uint32_t word = valid ? (a.wm ? cur.word : ~0u) : 0u;
         const bool on = (word & anyb) != 0;
-        const int ip = cur.ip;
-        int gid[ND], rdl[NL];
+        const int ip = valid ? cur.ip : -1;
+        int gid[ND], rdl[ND];
 #pragma unroll
-        for (int o = 0; o < ND; ++o) gid[o] = o < a.nd ? cur.g[o] : -1;
-#pragma unroll
-        for (int k = 0; k < NL; ++k) rdl[k] = cur.rd[k];
+        for (int o = 0; o < ND; ++o) {
+            gid[o] = (o < a.nd && valid) ? cur.g[o] : -1;
+            rdl[o] = valid ? cur.rd[o] : -1;
+        }
+        // the tile's loads are requested first; the next tiles' record and rows follow, so waiting for the
+        // tile's loads does not wait for the prefetch (one in-order memory counter covers both)
+        RowLoads<CT, MODE, ND> ld;
+        row_loads<CT, MODE, ND>(a, r, gid, ld);
+        const int64_t rec_nn = load_wave_rec(a, t + 2 * wstride);
+        load_wave_rows(a, rec_n, lane, nxt);
+        rec_n = rec_nn;
         cur = nxt;
-        if (nr <= 0) continue;  // a tile of empty segments (uniform across the wave)
+        __builtin_amdgcn_sched_barrier(0);  // keep the loads above ahead of their first use
+        // every tile holds at least one row (the panel drops tiles of empty segments), so there is no
+        // early exit here: a path without this tile's stores would make the wait for the prefetch drain
+        double v[CT];
+        row_combine<CT, MODE, ND>(a, ld, v);
+#pragma unroll
+        for (int c = 0; c < CT; ++c) v[c] = (on && (word & cb[c])) ? v[c] : 0.0;
         // segment of this lane: [ss, se]
         const int ipp = __shfl_up(ip, 1, kWave);
         const bool hs = valid && (lane == 0 || ip != ipp);
@@ -298,12 +325,6 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         const int ss = 63 - __clzll(H & mask_le(lane));
         const unsigned long long nh = H & ~mask_le(lane);
         const int se = nh ? __ffsll((long long)nh) - 2 : nr - 1;
-        double v[CT];
-#pragma unroll
-        for (int c = 0; c < CT; ++c) v[c] = 0.0;
-        if (on) row_value_ids<CT, MODE, ND>(a, r, gid, sp, v);
-#pragma unroll
-        for (int c = 0; c < CT; ++c) v[c] = (word & cb[c]) ? v[c] : 0.0;
         // One segmented scan per column gives the prefix x = sum of v over [segment start, lane]: its value
         // at the segment end is the segment total (for the mean), and differences of it are the run sums
         // of the large dims, so no second scan is needed.
@@ -381,10 +402,7 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
                 const unsigned long long le = HR & mask_le(lane);
                 const int rs = 63 - __clzll(le);
                 rend[o] = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
-                int rdest = -1;  // the row's run destination, loaded with its ids
-#pragma unroll
-                for (int k = 0; k < NL; ++k)
-                    if (k == d.large_slot) rdest = rdl[k];
+                const int rdest = rdl[o];  // the row's run destination, loaded with its ids
                 pl[o] = rs > 0 ? rs - 1 : 0;
                 runmask[o] = mask_le(lane) & ~((1ull << rs) - 1ull);
                 keep_pre[o] = rs == ss ? 0.0 : 1.0;
@@ -1163,7 +1181,7 @@ int make_layout(Panel* p, unsigned fe_mask, Layout& L) {
         s.runbuf = nullptr;
         s.seg_run_off = nullptr;
         s.run_dest = nullptr;
-        s.row_rdest = nullptr;
+        s.row_rdest = p->d[d].ids;  // a readable stand-in for dims without runs (never used as a slot)
         L.S += s.G;
         if (s.small) {
             s.lds_off = L.Gs;
