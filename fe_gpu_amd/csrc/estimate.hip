@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <set>
 #include <tuple>
 #include <vector>
 
@@ -68,8 +69,9 @@ __global__ __launch_bounds__(kBlock) void wsum2_kernel(const double* __restrict_
 // r = y~ - W~ beta on the work buffer, r'r partials per block, and (T != nullptr) the score rows
 // T = r W~ (row-major n x K) in the same pass over the columns.  With K <= 16 the block's score rows
 // are staged in LDS and leave as one contiguous stretch.
+// cols: device array of the 1 + K column pointers [y~ W~] (panel order, any storage)
 template <typename TX>
-__global__ __launch_bounds__(256) void resid_scores_kernel(const TX* __restrict__ work, int64_t n, int K,
+__global__ __launch_bounds__(256) void resid_scores_kernel(const TX* const* __restrict__ cols, int64_t n, int K,
                                                            const double* __restrict__ beta, double* r, double* T,
                                                            double* part) {
     extern __shared__ double tstage[];  // [256][K] when staging
@@ -79,13 +81,13 @@ __global__ __launch_bounds__(256) void resid_scores_kernel(const TX* __restrict_
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
         const int64_t i = i0 + threadIdx.x;
         if (i < n) {
-            double v = (double)work[i];
-            for (int k = 0; k < K; ++k) v -= (double)work[i + (int64_t)(1 + k) * n] * beta[k];
+            double v = (double)cols[0][i];
+            for (int k = 0; k < K; ++k) v -= (double)cols[1 + k][i] * beta[k];
             r[i] = v;
             ss += v * v;
             if (T) {
                 for (int k = 0; k < K; ++k) {
-                    const double t = v * (double)work[i + (int64_t)(1 + k) * n];
+                    const double t = v * (double)cols[1 + k][i];
                     if (stage)
                         tstage[threadIdx.x * K + k] = t;
                     else
@@ -138,11 +140,11 @@ __global__ __launch_bounds__(kBlock) void dot_part_kernel(const double* __restri
 // T[i][j] = r_i X[i][j] (row-major n x K) from column-major X (ld)
 // T = r x, one row per thread (the columns read coalesced, each row written as K contiguous values)
 template <typename TX>
-__global__ void scores_rows_kernel(const double* __restrict__ r, const TX* __restrict__ X, int64_t ldX, int64_t n,
-                                   int K, double* T) {
+__global__ void scores_rows_kernel(const double* __restrict__ r, const TX* const* __restrict__ X, int64_t n, int K,
+                                   double* T) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         const double ri = r[i];
-        for (int k = 0; k < K; ++k) T[i * K + k] = ri * (double)X[i + (int64_t)k * ldX];
+        for (int k = 0; k < K; ++k) T[i * K + k] = ri * (double)X[k][i];
     }
 }
 
@@ -346,7 +348,6 @@ constexpr int kMaxNestSubsets = 4;
 
 struct WaveNest {
     const double* r;            // residual (panel order)
-    int64_t ldX;                // regressor columns (panel order)
     int K;
     const int* idP;
     const int64_t* wtile_rec;   // first row | rows << 32 of each wave tile
@@ -368,7 +369,7 @@ struct NestRows {
 };
 
 template <int KT, typename TX>
-__device__ __forceinline__ void load_nest_rows(const WaveNest& a, const TX* __restrict__ X, int64_t t, int lane,
+__device__ __forceinline__ void load_nest_rows(const WaveNest& a, const TX* const* __restrict__ X, int64_t t, int lane,
                                                NestRows<KT>& q) {
     if (t >= a.n_wtiles) {
         q.rec = 0;
@@ -385,13 +386,13 @@ __device__ __forceinline__ void load_nest_rows(const WaveNest& a, const TX* __re
     q.w = valid && (!a.w || a.w[ri]);
     const double rr = valid ? a.r[ri] : 0.0;
 #pragma unroll
-    for (int k = 0; k < KT; ++k) q.t[k] = (k < a.K && valid) ? rr * (double)X[ri + (int64_t)k * a.ldX] : 0.0;
+    for (int k = 0; k < KT; ++k) q.t[k] = (k < a.K && valid) ? rr * (double)X[k][ri] : 0.0;
 }
 
 // Per subset s: part[(s * gridDim.x + block) * NP + e] (upper triangle of the block's sum of S S') and
 // ucount[s] (non-empty clusters).
 template <int KT, typename TX>
-__global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, const TX* __restrict__ X, double* part,
+__global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, const TX* const* __restrict__ X, double* part,
                                                                   unsigned long long* ucount) {
     __shared__ double gst[(kBlock / kWave) * gram_stage<KT>()];  // per-wave staging of the cluster sums
     gram4_t acc[kMaxNestSubsets];
@@ -926,6 +927,18 @@ struct DevBuf {
     }
 };
 
+// Device array of column pointers (the kernels that read [y W Z] take the columns through it, so a
+// request's columns may live in its own work buffer or in storage shared with other requests).
+template <typename TX>
+int column_array(DevBuf& db, const std::vector<const TX*>& cols, const TX* const** out) {
+    const TX** d = nullptr;
+    int e = db.get(&d, std::max<size_t>(1, cols.size()));
+    if (e) return e;
+    FEHIP_CHECK(copy_sync(d, cols.data(), sizeof(const TX*) * cols.size(), hipMemcpyHostToDevice), "column pointers");
+    *out = d;
+    return 0;
+}
+
 int host_sum_parts(const double* dpart, int nblk, int width, std::vector<double>& out) {
     std::vector<double> h((size_t)nblk * width);
     FEHIP_CHECK(copy_sync(h.data(), dpart, h.size() * sizeof(double), hipMemcpyDeviceToHost), "partials");
@@ -1064,6 +1077,11 @@ struct ReqState {
     // FP64 cross products of the C transformed columns (C x C, column-major) formed in the final pass
     // of the within-transform, when all of the request's columns were finalized in one column tile
     std::vector<double> gram;
+    // columns held in storage shared by the requests of a batch (no work buffer of its own): the
+    // device pointer of each of the C columns (FP32 when work32 would be used, else FP64)
+    std::vector<const void*> shared_cols;
+    bool shared = false;  // set before preparation: no work buffer of its own
+    bool f32 = false;     // transformed columns stored in FP32
     clk::time_point t0, t1;
 };
 
@@ -1180,15 +1198,17 @@ int prepare_request(Panel* p, ReqState& s, BatchShared& bs) {
     // 2SLS keeps FP64 storage in the mixed mode: the first-stage projection (Z'Z, Z'X) must be FP64,
     // and an over-identified fit does not drive Z'e to zero, so FP32 cross products would bias beta
     const bool iv_request = opts->n_iv_reg > 0 && L > 0;
-    if (opts->demean.precision == 1 && !iv_request) {
+    if (opts->demean.precision != 0 && opts->demean.precision != 1)
+        return set_error("fehip_estimate: precision must be 0 (FP64) or 1 (FP32 storage)");
+    s.f32 = opts->demean.precision == 1 && !iv_request;
+    if (s.shared) return 0;  // the batch holds the columns
+    if (s.f32) {
         if ((e = db.get(&s.work32, (size_t)n * s.C))) return e;
         s.S = within_table_size(p, fe_mask);
         if (s.S < 0) return 1;
         if (s.S > 0 && (e = db.get(&s.alpha, (size_t)s.S * s.C))) return e;
-    } else if (opts->demean.precision == 0 || opts->demean.precision == 1) {
-        if ((e = db.get(&s.work, (size_t)n * s.C))) return e;
     } else {
-        return set_error("fehip_estimate: precision must be 0 (FP64) or 1 (FP32 storage, FP64 refinement)");
+        if ((e = db.get(&s.work, (size_t)n * s.C))) return e;
     }
     return 0;
 }
@@ -1220,12 +1240,28 @@ int finish_request(Panel* p, ReqState& s) {
     double* work = s.work;
     const auto t1 = s.t1;
     if (K <= 0) return 0;
+    // the request's columns [y~ W~ Z~] through a device pointer array
+    const double* const* dcols = nullptr;
+    const float* const* dcols32 = nullptr;
+    {
+        if (s.f32) {
+            std::vector<const float*> hp(C);
+            for (int j = 0; j < C; ++j)
+                hp[j] = s.shared_cols.empty() ? s.work32 + (int64_t)j * n : (const float*)s.shared_cols[j];
+            if ((e = column_array(db, hp, &dcols32))) return e;
+        } else {
+            std::vector<const double*> hp(C);
+            for (int j = 0; j < C; ++j)
+                hp[j] = s.shared_cols.empty() ? s.work + (int64_t)j * n : (const double*)s.shared_cols[j];
+            if ((e = column_array(db, hp, &dcols))) return e;
+        }
+    }
 
     // ---- Gram of [y W Z] (upper), host copy
     double* mpart = nullptr;
     if ((e = db.get(&mpart, (size_t)kMeatBlocks * 78))) return e;
     std::vector<double> G((size_t)C * C);
-    const bool f32 = s.work32 != nullptr;
+    const bool f32 = s.f32;
     float* work32 = s.work32;
     const bool have_gram = s.gram.size() == (size_t)C * C;
     if (have_gram) {
@@ -1390,11 +1426,11 @@ int finish_request(Panel* p, ReqState& s) {
         FEHIP_CHECK(copy_sync(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
         const size_t tlds = (T_early && K <= 16) ? sizeof(double) * 256 * (size_t)K : 0;
         if (f32)
-            hipLaunchKernelGGL(resid_scores_kernel<float>, dim3(kResidBlocks), dim3(256), tlds, 0,
-                               (const float*)work32, n, K, (const double*)dbeta, r, T_early, part);
+            hipLaunchKernelGGL(resid_scores_kernel<float>, dim3(kResidBlocks), dim3(256), tlds, 0, dcols32, n, K,
+                               (const double*)dbeta, r, T_early, part);
         else
-            hipLaunchKernelGGL(resid_scores_kernel<double>, dim3(kResidBlocks), dim3(256), tlds, 0,
-                               (const double*)work, n, K, (const double*)dbeta, r, T_early, part);
+            hipLaunchKernelGGL(resid_scores_kernel<double>, dim3(kResidBlocks), dim3(256), tlds, 0, dcols, n, K,
+                               (const double*)dbeta, r, T_early, part);
         FEHIP_LAUNCH_CHECK("residual");
         std::vector<double> ps;
         if ((e = host_sum_parts(part, kResidBlocks, 1, ps))) return e;
@@ -1472,17 +1508,19 @@ int finish_request(Panel* p, ReqState& s) {
         // subset's meat is proj' M_z proj with M_z the meat of the scores r Z_aug.
         const int Kx = K;
         const int K = use_iv ? n_aug : Kx;  // width of the scores (shadows the regressor count below)
-        const double* X = f32 ? nullptr : work + n;  // W~ columns
-        int64_t ldX = n;
+        const double* const* X = f32 ? nullptr : dcols + 1;    // W~ columns
+        const float* const* X32 = f32 ? dcols32 + 1 : nullptr;
         double* Zaug = nullptr;
-        if (use_iv) {
-            int* dcols = nullptr;
-            if ((e = db.get(&dcols, (size_t)n_aug)) || (e = db.get(&Zaug, (size_t)n * n_aug))) return e;
-            FEHIP_CHECK(copy_sync(dcols, aug_col.data(), sizeof(int) * n_aug, hipMemcpyHostToDevice), "columns copy");
+        if (use_iv) {  // 2SLS requests keep their own FP64 work buffer
+            int* dsel = nullptr;
+            if ((e = db.get(&dsel, (size_t)n_aug)) || (e = db.get(&Zaug, (size_t)n * n_aug))) return e;
+            FEHIP_CHECK(copy_sync(dsel, aug_col.data(), sizeof(int) * n_aug, hipMemcpyHostToDevice), "columns copy");
             hipLaunchKernelGGL(gather_cols_kernel, dim3(stream_grid((size_t)n * n_aug, kBlock)), dim3(kBlock), 0, 0,
-                               (const double*)work, n, (const int*)dcols, n_aug, Zaug);
+                               (const double*)work, n, (const int*)dsel, n_aug, Zaug);
             FEHIP_LAUNCH_CHECK("instrument columns");
-            X = Zaug;
+            std::vector<const double*> zp(n_aug);
+            for (int a2 = 0; a2 < n_aug; ++a2) zp[a2] = Zaug + (int64_t)a2 * n;
+            if ((e = column_array(db, zp, &X))) return e;
         }
         // scores T = r x (row-major n x K), formed only if a subset's method reads it
         double* T = T_early;
@@ -1492,10 +1530,10 @@ int finish_request(Panel* p, ReqState& s) {
             if (e2) return e2;
             if (X)
                 hipLaunchKernelGGL(scores_rows_kernel<double>, dim3(stream_grid((size_t)n, kBlock)), dim3(kBlock), 0,
-                                   0, (const double*)r, X, ldX, n, K, T);
+                                   0, (const double*)r, X, n, K, T);
             else
                 hipLaunchKernelGGL(scores_rows_kernel<float>, dim3(stream_grid((size_t)n, kBlock)), dim3(kBlock), 0, 0,
-                                   (const double*)r, (const float*)(work32 + n), ldX, n, K, T);
+                                   (const double*)r, X32, n, K, T);
             FEHIP_LAUNCH_CHECK("scores rows");
             return 0;
         };
@@ -1514,7 +1552,6 @@ int finish_request(Panel* p, ReqState& s) {
         if (p->n_long == 0 && p->n_mtiles == 0 && K <= 12) {
             WaveNest wn{};
             wn.r = r;
-            wn.ldX = n;
             wn.K = K;
             wn.idP = p->d[p->P].ids;
             wn.wtile_rec = p->wtile_rec;
@@ -1564,7 +1601,6 @@ int finish_request(Panel* p, ReqState& s) {
                 if ((e = db.get(&npart, (size_t)wn.nsub * grid * NP)) || (e = db.get(&ucounts, kMaxNestSubsets)))
                     return e;
                 FEHIP_CHECK(hipMemsetAsync(ucounts, 0, sizeof(unsigned long long) * kMaxNestSubsets, 0), "memset");
-                const float* X32 = X ? nullptr : (const float*)(work32 + n);
 #define FEHIP_WNEST(KTV)                                                                                       \
     if (X32)                                                                                                   \
         hipLaunchKernelGGL((wave_nested_meat_kernel<KTV, float>), dim3(grid), dim3(kBlock), 0, 0, wn, X32, npart, \
@@ -1871,8 +1907,28 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
             bytes += b;
             chunk.push_back((int)i);
         }
-        for (int i : chunk)
+        // Shared storage: when the chunk's distinct columns fit one column tile and no request is a 2SLS
+        // one, every request reads its transformed columns (and takes its cross products) from one copy
+        // per distinct column, and no request holds a work buffer of its own.
+        auto col_ptr = [](const ReqState& q, int j) {
+            return j == 0 ? q.d_y : (j <= q.K ? q.d_W + (int64_t)(j - 1) * q.ld : q.d_Z + (int64_t)(j - 1 - q.K) * q.ld);
+        };
+        bool share = true;
+        {
+            std::set<std::tuple<const double*, const uint8_t*, bool>> keys;
+            for (int i : chunk) {
+                const ReqState& q = rs[i];
+                const bool iv = q.opts->n_iv_reg > 0 && q.L > 0;
+                share = share && !iv;
+                const bool f32 = q.opts->demean.precision == 1 && !iv;
+                for (int j = 0; j < 1 + q.K + q.L; ++j) keys.insert(std::make_tuple(col_ptr(q, j), q.d_weights, f32));
+            }
+            share = share && (int)keys.size() <= std::min(kGramCols, within_tile_cap(p, rs[i0].fe_mask));
+        }
+        for (int i : chunk) {
+            rs[i].shared = share;
             if ((e = prepare_request(p, rs[i], shared))) return e;
+        }
         int ncols = 0;
         bool weighted = false;
         for (int i : chunk) {
@@ -1901,14 +1957,25 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
         int c = 0;
         for (size_t b = 0; b < chunk.size(); ++b) {
             ReqState& s = rs[chunk[b]];
+            if (share) s.shared_cols.assign(s.C, nullptr);
             for (int j = 0; j < s.C; ++j, ++c) {
-                const double* x = j == 0 ? s.d_y : (j <= s.K ? s.d_W + (int64_t)(j - 1) * s.ld
-                                                             : s.d_Z + (int64_t)(j - 1 - s.K) * s.ld);
-                dst[c] = {s.work ? s.work + (int64_t)j * n : nullptr, s.work32 ? s.work32 + (int64_t)j * n : nullptr,
-                          s.alpha ? s.alpha + (int64_t)j * s.S : nullptr};
-                const auto key = std::make_tuple(x, s.d_weights, s.work32 != nullptr);
+                const double* x = col_ptr(s, j);
+                const auto key = std::make_tuple(x, s.d_weights, s.f32);
                 auto it = seen.find(key);
-                if (it != seen.end() && (dst[it->second].alpha != nullptr) == (dst[c].alpha != nullptr)) {
+                if (share) {
+                    if (it == seen.end()) {  // first use: the column's one copy
+                        dst[c] = {nullptr, nullptr, nullptr};
+                        if (s.f32 ? (e = cb.get(&dst[c].out32, (size_t)n)) : (e = cb.get(&dst[c].out, (size_t)n)))
+                            return e;
+                    }
+                    const ColDst& d = dst[it == seen.end() ? c : it->second];
+                    s.shared_cols[j] = s.f32 ? (const void*)d.out32 : (const void*)d.out;
+                } else {
+                    dst[c] = {s.work ? s.work + (int64_t)j * n : nullptr,
+                              s.work32 ? s.work32 + (int64_t)j * n : nullptr,
+                              s.alpha ? s.alpha + (int64_t)j * s.S : nullptr};
+                }
+                if (it != seen.end() && (share || (dst[it->second].alpha != nullptr) == (dst[c].alpha != nullptr))) {
                     uidx[c] = it->second;
                     continue;
                 }
@@ -1965,7 +2032,7 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
             ws.col_iters[c2] = wsu.col_iters[u];
             ws.col_converged[c2] = wsu.col_converged[u];
             ws.col_max_update[c2] = wsu.col_max_update[u];
-            if (src == c2) continue;
+            if (src == c2 || share) continue;
             if (dst[c2].out)
                 FEHIP_CHECK(hipMemcpyAsync(dst[c2].out, dst[src].out, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, 0),
                             "shared column");

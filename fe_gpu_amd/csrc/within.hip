@@ -1707,6 +1707,23 @@ int within_sweep_with_tables(Panel* p, unsigned fe_mask, const double* x, const 
     return 0;
 }
 
+// column-tile width cap: widest of {16, 12, 9, 8, 6, 4, 3, 2, 1} whose small-dim LDS tables fit next to the
+// row tile (<= 28 KB of staged + accumulated tables); 0 if none does
+static int tile_cap(const Layout& L) {
+    static const int widths[] = {16, 12, 9, 8, 6, 4, 3, 2, 1};
+    for (int wdt : widths)
+        if ((size_t)L.Gs * wdt * 16 <= 28 * 1024) return wdt;
+    return 0;
+}
+
+int within_tile_cap(Panel* p, unsigned fe_mask) {
+    unsigned mask = fe_mask ? fe_mask : ((1u << p->m) - 1u);
+    mask &= (1u << p->m) - 1u;
+    Layout L;
+    if (make_layout(p, mask, L)) return 0;
+    return tile_cap(L);
+}
+
 int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* wm, int nslots,
                      const fehip_demean_opts& o, int to_work, WithinStats& ws, hipStream_t s) {
     unsigned mask = o.fe_mask ? o.fe_mask : ((1u << p->m) - 1u);
@@ -1722,18 +1739,8 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
     for (int c = 0; c < ncols; ++c)
         if (wc.slot[c] < 0 || wc.slot[c] >= nslots) return set_error("column %d: slot out of range", c);
     Scratch sc;
-    // column-tile width: widest of {16, 12, 9, 8, 6, 4, 3, 2, 1} whose small-dim LDS tables fit next to
-    // the row tile (<= 28 KB of staged + accumulated tables)
-    int ct_max = 0;
-    {
-        static const int widths[] = {16, 12, 9, 8, 6, 4, 3, 2, 1};
-        for (int wdt : widths)
-            if ((size_t)L.Gs * wdt * 16 <= 28 * 1024) {
-                ct_max = wdt;
-                break;
-            }
-        if (ct_max == 0) return set_error("small FE dims exceed the LDS budget (%d groups)", L.Gs);
-    }
+    const int ct_max = tile_cap(L);
+    if (ct_max == 0) return set_error("small FE dims exceed the LDS budget (%d groups)", L.Gs);
     const int64_t S = L.S;
     const int64_t n = p->n;
     if ((e = sc.alloc(&sc.tab, (size_t)(S * kMaxCT * 7 + 16)))) return e;

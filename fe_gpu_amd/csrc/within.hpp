@@ -51,6 +51,10 @@ struct WithinCols {
     int64_t gram_ld = 0;
 };
 
+// Widest column tile of a solve over fe_mask (columns that one tile, hence one set of cross products,
+// can hold); 0 if the small dims do not fit the LDS budget.
+int within_tile_cap(Panel* p, unsigned fe_mask);
+
 // Size of the FE-table space of a solve over fe_mask (groups of the absorbed non-primary dims).
 int64_t within_table_size(Panel* p, unsigned fe_mask);
 
