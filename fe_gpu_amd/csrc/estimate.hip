@@ -675,8 +675,11 @@ struct SmallKeys {
     const uint8_t* w;
 };
 
-template <int KT>
-__global__ __launch_bounds__(kBlock) void small_scores_kernel(const double* __restrict__ T, int K, int64_t n,
+// Scores are read from T (row-major n x K) or, when T is null, formed as r x from the residual and the
+// regressor columns (no score matrix written or read back).
+template <int KT, typename TX>
+__global__ __launch_bounds__(kBlock) void small_scores_kernel(const double* __restrict__ T, const double* __restrict__ r,
+                                                              const TX* const* __restrict__ X, int K, int64_t n,
                                                               SmallKeys sk, int cells, double* part) {
     extern __shared__ double tab[];  // [cells][KT]
     for (int e = threadIdx.x; e < cells * KT; e += blockDim.x) tab[e] = 0.0;
@@ -684,7 +687,12 @@ __global__ __launch_bounds__(kBlock) void small_scores_kernel(const double* __re
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         int cell = 0;
         for (int d = 0; d < sk.nk; ++d) cell += sk.ids[d][i] * sk.stride[d];
-        for (int c = 0; c < K; ++c) lds_add(&tab[cell * KT + c], T[i * K + c]);
+        if (T) {
+            for (int c = 0; c < K; ++c) lds_add(&tab[cell * KT + c], T[i * K + c]);
+        } else {
+            const double ri = r[i];
+            for (int c = 0; c < K; ++c) lds_add(&tab[cell * KT + c], ri * (double)X[c][i]);
+        }
         if (!sk.w || sk.w[i]) lds_add(&tab[cell * KT + (KT - 1)], 1.0);  // occupancy (KT > K)
     }
     __syncthreads();
@@ -709,8 +717,9 @@ __global__ __launch_bounds__(kBlock) void sum_blocks_kernel(const double* __rest
 
 // One large dim (not the primary): sums of T over the runs of equal id inside each primary tile,
 // written contiguously per tile (the same run numbering the panel built), then reduced in group order.
-template <int KT>
-__global__ __launch_bounds__(kBlock) void run_scores_kernel(const double* __restrict__ T, int K,
+template <int KT, typename TX>
+__global__ __launch_bounds__(kBlock) void run_scores_kernel(const double* __restrict__ T, const double* __restrict__ r,
+                                                            const TX* const* __restrict__ X, int K,
                                                             const int64_t* __restrict__ seg_off,
                                                             const int64_t* __restrict__ tile_seg, int64_t n_tiles,
                                                             const int* __restrict__ ids,
@@ -727,9 +736,16 @@ __global__ __launch_bounds__(kBlock) void run_scores_kernel(const double* __rest
         __syncthreads();
         for (int k = j; k <= nseg; k += blockDim.x) soff[k] = (int)(seg_off[s0 + k] - R0);
         const int nr = (int)(seg_off[s1] - R0);
-        for (int e = j; e < nr * K; e += blockDim.x) {
-            const int q = e / K, c = e - q * K;
-            ts[q * KT + c] = T[R0 * K + e];
+        if (T) {
+            for (int e = j; e < nr * K; e += blockDim.x) {
+                const int q = e / K, c = e - q * K;
+                ts[q * KT + c] = T[R0 * K + e];
+            }
+        } else {  // column by column, coalesced
+            for (int e = j; e < nr * K; e += blockDim.x) {
+                const int c = e / nr, q = e - c * nr;
+                ts[q * KT + c] = r[R0 + q] * (double)X[c][R0 + q];
+            }
         }
         __syncthreads();
         const bool valid = j < nr;
@@ -1417,10 +1433,28 @@ int finish_request(Panel* p, ReqState& s) {
                 unsigned dm = 0;
                 for (int j = 0; j < ncd; ++j)
                     if (mask & (1 << j)) dm |= 1u << (cdims[j] - 1);
-                int nother = 0;
-                for (int d = 0; d < p->m; ++d) nother += (d != p->P && ((dm >> d) & 1u)) ? 1 : 0;
-                const bool nested = ((dm >> p->P) & 1u) && p->n_long == 0 && p->n_mtiles == 0 && K <= 12 && nother <= 2;
-                any_T |= !nested;
+                // the subset methods that read the score matrix: block-tile nested, sorted partition, segment
+                // sums; the wave nested, small-table and run methods form r x on the fly
+                const bool has_p = (dm >> p->P) & 1u;
+                bool all_small = true;
+                int64_t cells = 1;
+                int len = 0, single = -1;
+                for (int d = 0; d < p->m; ++d)
+                    if ((dm >> d) & 1u) {
+                        all_small &= p->d[d].small != 0;
+                        cells *= p->d[d].G;
+                        single = d;
+                        ++len;
+                    }
+                if (len != 1) single = -1;
+                bool reads_T = true;
+                if (has_p && p->n_long == 0 && K <= 12)
+                    reads_T = p->n_mtiles != 0;
+                else if (all_small && cells <= kSmallCells && K <= 11)
+                    reads_T = false;
+                else if (single >= 0 && single != p->P && !p->d[single].small && p->n_long == 0 && K <= 12)
+                    reads_T = false;
+                any_T |= reads_T;
             }
         if (any_T && (e = db.get(&T_early, (size_t)n * K))) return e;
         FEHIP_CHECK(copy_sync(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
@@ -1673,7 +1707,6 @@ int finish_request(Panel* p, ReqState& s) {
                 FEHIP_CHECK(copy_sync(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
                 ncl = (int64_t)h;
             } else if (all_small && cells <= kSmallCells && K <= 11) {
-                if ((e = need_T())) return e;
                 SmallKeys sk{};
                 int stride = 1;
                 for (int d = 0; d < p->m; ++d)
@@ -1692,12 +1725,21 @@ int finish_request(Panel* p, ReqState& s) {
                 double* sp_ = nullptr;
                 if ((e = db.get(&sp_, (size_t)grid * c * KT + (size_t)c * KT))) return e;
                 const size_t lds = sizeof(double) * (size_t)c * KT;
-                if (KT == 4)
-                    hipLaunchKernelGGL(small_scores_kernel<4>, dim3(grid), dim3(kBlock), lds, 0, T, K, n, sk, c, sp_);
-                else if (KT == 8)
-                    hipLaunchKernelGGL(small_scores_kernel<8>, dim3(grid), dim3(kBlock), lds, 0, T, K, n, sk, c, sp_);
-                else
-                    hipLaunchKernelGGL(small_scores_kernel<12>, dim3(grid), dim3(kBlock), lds, 0, T, K, n, sk, c, sp_);
+#define FEHIP_SMALL(KTV)                                                                                        \
+    if (X32)                                                                                                    \
+        hipLaunchKernelGGL((small_scores_kernel<KTV, float>), dim3(grid), dim3(kBlock), lds, 0, (const double*)T,  \
+                           (const double*)r, X32, K, n, sk, c, sp_);                                            \
+    else                                                                                                        \
+        hipLaunchKernelGGL((small_scores_kernel<KTV, double>), dim3(grid), dim3(kBlock), lds, 0, (const double*)T, \
+                           (const double*)r, X, K, n, sk, c, sp_);
+                if (KT == 4) {
+                    FEHIP_SMALL(4)
+                } else if (KT == 8) {
+                    FEHIP_SMALL(8)
+                } else {
+                    FEHIP_SMALL(12)
+                }
+#undef FEHIP_SMALL
                 double* Sc = sp_ + (size_t)grid * c * KT;
                 hipLaunchKernelGGL(sum_blocks_kernel, dim3((unsigned)(c * KT)), dim3(kBlock), 0, 0, sp_, (int)grid,
                                    c * KT, Sc);
@@ -1712,21 +1754,26 @@ int finish_request(Panel* p, ReqState& s) {
                             M[(size_t)b2 * K + a2] += hS[(size_t)q * KT + a2] * hS[(size_t)q * KT + b2];
                 }
             } else if (single >= 0 && single != p->P && !p->d[single].small && p->n_long == 0 && K <= 12) {
-                if ((e = need_T())) return e;
                 const DimStruct& D = p->d[single];
                 const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
                 double *rb = nullptr, *Sg = nullptr;
                 if ((e = db.get(&rb, (size_t)D.n_runs * KT + 1)) || (e = db.get(&Sg, (size_t)D.G * KT))) return e;
                 const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(p->n_tiles, (int64_t)num_cus() * 8));
-                if (KT == 4)
-                    hipLaunchKernelGGL(run_scores_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, T, K, p->seg_off,
-                                       p->tile_seg, p->n_tiles, D.ids, D.seg_run_off, rb);
-                else if (KT == 8)
-                    hipLaunchKernelGGL(run_scores_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, T, K, p->seg_off,
-                                       p->tile_seg, p->n_tiles, D.ids, D.seg_run_off, rb);
-                else
-                    hipLaunchKernelGGL(run_scores_kernel<12>, dim3(grid), dim3(kBlock), 0, 0, T, K, p->seg_off,
-                                       p->tile_seg, p->n_tiles, D.ids, D.seg_run_off, rb);
+#define FEHIP_RUNS(KTV)                                                                                          \
+    if (X32)                                                                                                     \
+        hipLaunchKernelGGL((run_scores_kernel<KTV, float>), dim3(grid), dim3(kBlock), 0, 0, (const double*)T,       \
+                           (const double*)r, X32, K, p->seg_off, p->tile_seg, p->n_tiles, D.ids, D.seg_run_off, rb); \
+    else                                                                                                         \
+        hipLaunchKernelGGL((run_scores_kernel<KTV, double>), dim3(grid), dim3(kBlock), 0, 0, (const double*)T,      \
+                           (const double*)r, X, K, p->seg_off, p->tile_seg, p->n_tiles, D.ids, D.seg_run_off, rb);
+                if (KT == 4) {
+                    FEHIP_RUNS(4)
+                } else if (KT == 8) {
+                    FEHIP_RUNS(8)
+                } else {
+                    FEHIP_RUNS(12)
+                }
+#undef FEHIP_RUNS
                 FEHIP_LAUNCH_CHECK("run scores");
                 const size_t tot = (size_t)D.G * KT;
                 if (KT == 4)
