@@ -831,6 +831,27 @@ int gram_fused(const TX* X, int64_t ld, int C, int64_t n, double* part, std::vec
     return reduce_upper(part, (int)grid, KT, C, G);
 }
 
+// Gram of up to 16 columns on the matrix cores (the per-thread accumulators of gram_rows_kernel would
+// need 136 register pairs at 16 columns): one wave per 64 consecutive rows, one block partial (upper
+// triangle) per workgroup.
+template <typename TX>
+__global__ __launch_bounds__(kBlock) void gram_cols_kernel(const TX* const* __restrict__ cols, int C, int64_t n,
+                                                           double* part) {
+    __shared__ double gst[(kBlock / kWave) * gram_stage<16>()];
+    gram4_t acc = {0.0, 0.0, 0.0, 0.0};
+    const int lane = threadIdx.x & (kWave - 1);
+    double* mystage = gst + (size_t)(threadIdx.x / kWave) * gram_stage<16>();
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {  // uniform per block
+        const int64_t i = i0 + threadIdx.x;
+        double v[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) v[c] = (c < C && i < n) ? (double)cols[c][i] : 0.0;
+        gram_wave<16>(mystage, v, lane, acc);
+    }
+    gram_store_upper<16>(gst, gram_stage<16>(), acc, part + (int64_t)blockIdx.x * (16 * 17 / 2));
+}
+
 // ---- FP32 storage with FP64 refinement -------------------------------------------------------
 // e = y - W beta (caller order, FP64 columns)
 __global__ void combo_residual_kernel(const double* __restrict__ y, const double* __restrict__ W, int64_t ld, int K,
@@ -942,6 +963,17 @@ struct DevBuf {
         }
     }
 };
+
+template <typename TX>
+int gram_cols(const TX* const* cols, int C, int64_t n, DevBuf& tmp, std::vector<double>& G) {
+    const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>(kMeatBlocks, (n + kBlock - 1) / kBlock));
+    double* part = nullptr;
+    int e = tmp.get(&part, (size_t)grid * (16 * 17 / 2));
+    if (e) return e;
+    hipLaunchKernelGGL((gram_cols_kernel<TX>), dim3(grid), dim3(kBlock), 0, 0, cols, C, n, part);
+    FEHIP_LAUNCH_CHECK("gram (matrix cores)");
+    return reduce_upper(part, (int)grid, 16, C, G);
+}
 
 // Device array of column pointers (the kernels that read [y W Z] take the columns through it, so a
 // request's columns may live in its own work buffer or in storage shared with other requests).
@@ -1282,6 +1314,8 @@ int finish_request(Panel* p, ReqState& s) {
     const bool have_gram = s.gram.size() == (size_t)C * C;
     if (have_gram) {
         G = s.gram;
+    } else if (C > 12 && C <= 16) {
+        if (f32 ? (e = gram_cols(dcols32, C, n, db, G)) : (e = gram_cols(dcols, C, n, db, G))) return e;
     } else if (C <= 12) {
         if (f32) {
             if ((e = gram_fused(work32, n, C, n, mpart, G))) return e;

@@ -113,3 +113,27 @@ def test_batch_cross_products_with_row_masks_equal_alone():
         assert rel(fb.beta, fa.beta) < 1e-10 and rel(fb.se, fa.se) < 1e-10
         assert abs(fb.res["tss_within"] - fa.res["tss_within"]) <= 1e-12 * fa.res["tss_within"]
     pan.close()
+
+
+def test_wide_requests_in_a_batch_take_the_matrix_core_gram():
+    """Two requests of 13 and 16 columns with their own row subsamples: their 29 distinct columns span
+    two column tiles, so each request's cross products come from the 16-column matrix-core Gram of its
+    own columns; they must equal the same requests alone (cross products from the final pass)."""
+    from fe_gpu_amd import api, synth
+    from fe_gpu_amd.device import DeviceBuffer
+    note("wide requests in a batch")
+    p = synth.random_fe_problem(741, n=9000, K=15, G=(1800, 200, 9))
+    n = p.n
+    rng = np.random.default_rng(8)
+    pan = api.Panel.from_host(p.fe)
+    dy = DeviceBuffer.from_array(p.y)
+    dW = DeviceBuffer.from_array(np.asfortranarray(p.W))
+    ws = [DeviceBuffer.from_array((rng.random(n) < f).astype(np.uint8)) for f in (0.7, 0.9)]
+    reqs = [api.Request(dy.ptr, dW.ptr, 12, weights_ptr=ws[0].ptr, options=dict(cluster=(1,), tol=1e-12)),
+            api.Request(dy.ptr, dW.ptr, 15, weights_ptr=ws[1].ptr, options=dict(cluster=(2,), tol=1e-12))]
+    fits = api.estimate_batch(pan, reqs)
+    for q, fb in zip(reqs, fits):
+        fa = api.estimate_device(pan, q.y_ptr, q.W_ptr, q.K, weights_ptr=q.weights_ptr, **q.options)
+        assert rel(fb.beta, fa.beta) < 1e-10 and rel(fb.se, fa.se) < 1e-10
+        assert abs(fb.res["tss_within"] - fa.res["tss_within"]) <= 1e-12 * fa.res["tss_within"]
+    pan.close()
