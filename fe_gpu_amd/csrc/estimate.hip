@@ -162,9 +162,33 @@ __global__ void segsum_kernel(const double* __restrict__ T, int K, const int* __
     }
 }
 
-__global__ void nonempty_kernel(const int* __restrict__ ids, const uint8_t* __restrict__ w, int64_t n, int* flags) {
+// Non-empty groups of every dim under a row subsample in one pass: flags of dim d at off[d] + id.
+struct DimIds {
+    const int* ids[FEHIP_MAX_FE];
+    int64_t off[FEHIP_MAX_FE + 1];
+    int m;
+};
+
+__global__ void nonempty_dims_kernel(DimIds dk, const uint8_t* __restrict__ w, int64_t n, int* flags) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        if (!w || w[i]) flags[ids[i]] = 1;
+        if (w[i])
+            for (int d = 0; d < dk.m; ++d) flags[dk.off[d] + dk.ids[d][i]] = 1;
+}
+
+// out[d] = set flags of dim d (flags laid out as in nonempty_dims_kernel)
+__global__ __launch_bounds__(kBlock) void count_dims_kernel(const int* __restrict__ f, DimIds dk,
+                                                            unsigned long long* out) {
+    const int64_t total = dk.off[dk.m];
+    unsigned long long c[FEHIP_MAX_FE] = {0};
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x)
+        if (f[i])
+            for (int d = 0; d < dk.m; ++d)
+                if (i >= dk.off[d] && i < dk.off[d + 1]) ++c[d];
+    for (int d = 0; d < dk.m; ++d) {
+        unsigned long long v = c[d];
+        for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+        if ((threadIdx.x & (kWave - 1)) == 0 && v) atomicAdd(out + d, v);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void count_kernel(const int* __restrict__ f, int64_t n, unsigned long long* out) {
@@ -1206,27 +1230,31 @@ int prepare_request(Panel* p, ReqState& s, BatchShared& bs) {
     result->tss_total = yit->second.second;
     // group counts: G_d (max id) without weights, non-empty groups with weights
     int Geff[FEHIP_MAX_FE];
-    auto git = wpanel ? bs.geff.find(d_weights) : bs.geff.end();
-    for (int d = 0; d < p->m; ++d) {
-        Geff[d] = p->d[d].G;
-        if (git != bs.geff.end()) {
-            Geff[d] = git->second[d];
-        } else if (wpanel) {
-            int* flags = nullptr;
-            unsigned long long* cnt = nullptr;
-            if ((e = db.get(&flags, (size_t)p->d[d].G)) || (e = db.get(&cnt, 1))) return e;
-            FEHIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * (size_t)p->d[d].G, 0), "memset");
-            FEHIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long), 0), "memset");
-            hipLaunchKernelGGL(nonempty_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, p->d[d].ids, wpanel, n,
-                               flags);
-            hipLaunchKernelGGL(count_kernel, dim3(stream_grid(p->d[d].G, kBlock)), dim3(kBlock), 0, 0, flags,
-                               (int64_t)p->d[d].G, cnt);
-            unsigned long long h = 0;
-            FEHIP_CHECK(copy_sync(&h, cnt, sizeof(h), hipMemcpyDeviceToHost), "count");
-            Geff[d] = (int)h;
+    auto git = wpanel ? bs.geff.find(d_weights) : bs.geff.end();  // unweighted: G_d = max id
+    if (wpanel && git == bs.geff.end()) {  // every dim in one pass and one read-back
+        DimIds dk{};
+        dk.m = p->m;
+        dk.off[0] = 0;
+        for (int d = 0; d < p->m; ++d) {
+            dk.ids[d] = p->d[d].ids;
+            dk.off[d + 1] = dk.off[d] + p->d[d].G;
         }
+        int* flags = nullptr;
+        unsigned long long* cnt = nullptr;
+        if ((e = db.get(&flags, (size_t)dk.off[p->m])) || (e = db.get(&cnt, FEHIP_MAX_FE))) return e;
+        FEHIP_CHECK(hipMemsetAsync(flags, 0, sizeof(int) * (size_t)dk.off[p->m], 0), "memset");
+        FEHIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * FEHIP_MAX_FE, 0), "memset");
+        hipLaunchKernelGGL(nonempty_dims_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, 0, dk,
+                           (const uint8_t*)wpanel, n, flags);
+        hipLaunchKernelGGL(count_dims_kernel, dim3(stream_grid((size_t)dk.off[p->m], kBlock)), dim3(kBlock), 0, 0,
+                           (const int*)flags, dk, cnt);
+        FEHIP_LAUNCH_CHECK("group counts");
+        unsigned long long h[FEHIP_MAX_FE] = {0};
+        FEHIP_CHECK(copy_sync(h, cnt, sizeof(h), hipMemcpyDeviceToHost), "count");
+        bs.geff[d_weights] = std::vector<int>(h, h + p->m);
+        git = bs.geff.find(d_weights);
     }
-    if (wpanel && git == bs.geff.end()) bs.geff[d_weights] = std::vector<int>(Geff, Geff + p->m);
+    for (int d = 0; d < p->m; ++d) Geff[d] = git != bs.geff.end() ? git->second[d] : p->d[d].G;
     int dof_fes = 0, dof_nested = 0;
     for (int d = 0; d < p->m; ++d)
         if ((fe_mask >> d) & 1u) dof_fes += std::max(0, Geff[d] - 1);
