@@ -69,13 +69,14 @@ __global__ __launch_bounds__(kBlock) void wsum2_kernel(const double* __restrict_
 // r = y~ - W~ beta on the work buffer, r'r partials per block, and (T != nullptr) the score rows
 // T = r W~ (row-major n x K) in the same pass over the columns.  With K <= 16 the block's score rows
 // are staged in LDS and leave as one contiguous stretch.
-// cols: device array of the 1 + K column pointers [y~ W~] (panel order, any storage)
+// cols: device array of the 1 + K column pointers [y~ W~] (panel order, any storage); the scores
+// T = r x (row-major n x Kt) are formed from the Kt columns tcols (W~, or the 2SLS instrument set).
 template <typename TX>
 __global__ __launch_bounds__(256) void resid_scores_kernel(const TX* const* __restrict__ cols, int64_t n, int K,
                                                            const double* __restrict__ beta, double* r, double* T,
-                                                           double* part) {
-    extern __shared__ double tstage[];  // [256][K] when staging
-    const bool stage = T && K <= 16;
+                                                           int Kt, const TX* const* __restrict__ tcols, double* part) {
+    extern __shared__ double tstage[];  // [256][Kt] when staging
+    const bool stage = T && Kt <= 16;
     double ss = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i0 = (int64_t)blockIdx.x * blockDim.x; i0 < n; i0 += stride) {
@@ -86,19 +87,19 @@ __global__ __launch_bounds__(256) void resid_scores_kernel(const TX* const* __re
             r[i] = v;
             ss += v * v;
             if (T) {
-                for (int k = 0; k < K; ++k) {
-                    const double t = v * (double)cols[1 + k][i];
+                for (int k = 0; k < Kt; ++k) {
+                    const double t = v * (double)tcols[k][i];
                     if (stage)
-                        tstage[threadIdx.x * K + k] = t;
+                        tstage[threadIdx.x * Kt + k] = t;
                     else
-                        T[i * K + k] = t;
+                        T[i * Kt + k] = t;
                 }
             }
         }
         if (stage) {
             __syncthreads();
             const int64_t rows = n - i0 < (int64_t)blockDim.x ? n - i0 : (int64_t)blockDim.x;
-            for (int64_t e = threadIdx.x; e < rows * K; e += blockDim.x) T[i0 * K + e] = tstage[e];
+            for (int64_t e = threadIdx.x; e < rows * Kt; e += blockDim.x) T[i0 * Kt + e] = tstage[e];
             __syncthreads();
         }
     }
@@ -952,17 +953,6 @@ __global__ __launch_bounds__(kBlock) void composite_runs_kernel(const int* __res
     if ((threadIdx.x & (kWave - 1)) == 0 && c) atomicAdd(count, c);
 }
 
-// out[i + a n] = work[i + cols[a] n]
-__global__ void gather_cols_kernel(const double* __restrict__ work, int64_t n, const int* __restrict__ cols, int nc,
-                                   double* out) {
-    const int64_t total = n * nc;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
-        const int a = (int)(e / n);
-        const int64_t i = e - (int64_t)a * n;
-        out[e] = work[i + (int64_t)cols[a] * n];
-    }
-}
-
 struct DevBuf {
     std::vector<void*> v;
     DevBuf() = default;
@@ -1473,6 +1463,7 @@ int finish_request(Panel* p, ReqState& s) {
     // score rows T = r W~ are formed in the residual pass when an OLS sandwich has a subset whose method
     // reads them (every subset but those containing the primary on wave tiles)
     double* T_early = nullptr;
+    const double* const* iv_scores = nullptr;  // 2SLS: the instrument-set columns the scores are formed from
     bool rss_done = false;
     double rss = 0.0;
     if (have_gram && ncd == 0 && !use_iv) {
@@ -1489,8 +1480,17 @@ int finish_request(Panel* p, ReqState& s) {
         // the FP64 solution (no refinement); the residual and scores then read the FP32 columns, whose
         // rounding enters the sandwich only at the level of the storage precision, and rss comes from
         // the FP64 cross products.
+        // Scores: r W~, or for 2SLS r Z_aug with Z_aug the exogenous regressors and the instruments, read in
+        // place from the work columns.  The sandwich's subset methods that read the score matrix get it
+        // from this pass.
+        const int Kt = use_iv ? n_aug : K;
+        if (use_iv && ncd > 0) {
+            std::vector<const double*> zp(n_aug);
+            for (int a2 = 0; a2 < n_aug; ++a2) zp[a2] = work + (int64_t)aug_col[a2] * n;
+            if ((e = column_array(db, zp, &iv_scores))) return e;
+        }
         bool any_T = false;
-        if (ncd > 0 && !use_iv)
+        if (ncd > 0)
             for (int mask = 1; mask < (1 << ncd); ++mask) {
                 unsigned dm = 0;
                 for (int j = 0; j < ncd; ++j)
@@ -1510,23 +1510,23 @@ int finish_request(Panel* p, ReqState& s) {
                     }
                 if (len != 1) single = -1;
                 bool reads_T = true;
-                if (has_p && p->n_long == 0 && K <= 12)
+                if (has_p && p->n_long == 0 && Kt <= 12)
                     reads_T = p->n_mtiles != 0;
-                else if (all_small && cells <= kSmallCells && K <= 11)
+                else if (all_small && cells <= kSmallCells && Kt <= 11)
                     reads_T = false;
-                else if (single >= 0 && single != p->P && !p->d[single].small && p->n_long == 0 && K <= 12)
+                else if (single >= 0 && single != p->P && !p->d[single].small && p->n_long == 0 && Kt <= 12)
                     reads_T = false;
                 any_T |= reads_T;
             }
-        if (any_T && (e = db.get(&T_early, (size_t)n * K))) return e;
+        if (any_T && (e = db.get(&T_early, (size_t)n * Kt))) return e;
         FEHIP_CHECK(copy_sync(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
-        const size_t tlds = (T_early && K <= 16) ? sizeof(double) * 256 * (size_t)K : 0;
+        const size_t tlds = (T_early && Kt <= 16) ? sizeof(double) * 256 * (size_t)Kt : 0;
         if (f32)
             hipLaunchKernelGGL(resid_scores_kernel<float>, dim3(kResidBlocks), dim3(256), tlds, 0, dcols32, n, K,
-                               (const double*)dbeta, r, T_early, part);
+                               (const double*)dbeta, r, T_early, Kt, dcols32 + 1, part);
         else
             hipLaunchKernelGGL(resid_scores_kernel<double>, dim3(kResidBlocks), dim3(256), tlds, 0, dcols, n, K,
-                               (const double*)dbeta, r, T_early, part);
+                               (const double*)dbeta, r, T_early, Kt, use_iv ? iv_scores : dcols + 1, part);
         FEHIP_LAUNCH_CHECK("residual");
         std::vector<double> ps;
         if ((e = host_sum_parts(part, kResidBlocks, 1, ps))) return e;
@@ -1606,18 +1606,7 @@ int finish_request(Panel* p, ReqState& s) {
         const int K = use_iv ? n_aug : Kx;  // width of the scores (shadows the regressor count below)
         const double* const* X = f32 ? nullptr : dcols + 1;    // W~ columns
         const float* const* X32 = f32 ? dcols32 + 1 : nullptr;
-        double* Zaug = nullptr;
-        if (use_iv) {  // 2SLS requests keep their own FP64 work buffer
-            int* dsel = nullptr;
-            if ((e = db.get(&dsel, (size_t)n_aug)) || (e = db.get(&Zaug, (size_t)n * n_aug))) return e;
-            FEHIP_CHECK(copy_sync(dsel, aug_col.data(), sizeof(int) * n_aug, hipMemcpyHostToDevice), "columns copy");
-            hipLaunchKernelGGL(gather_cols_kernel, dim3(stream_grid((size_t)n * n_aug, kBlock)), dim3(kBlock), 0, 0,
-                               (const double*)work, n, (const int*)dsel, n_aug, Zaug);
-            FEHIP_LAUNCH_CHECK("instrument columns");
-            std::vector<const double*> zp(n_aug);
-            for (int a2 = 0; a2 < n_aug; ++a2) zp[a2] = Zaug + (int64_t)a2 * n;
-            if ((e = column_array(db, zp, &X))) return e;
-        }
+        if (use_iv) X = iv_scores;  // 2SLS requests keep their own FP64 work buffer; Z_aug read in place
         // scores T = r x (row-major n x K), formed only if a subset's method reads it
         double* T = T_early;
         auto need_T = [&]() -> int {
