@@ -7,7 +7,9 @@
 #include <cstddef>
 #include <cstdint>
 #include <cstdio>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 namespace fehip {
 
@@ -40,6 +42,31 @@ inline unsigned stream_grid(size_t work_items, unsigned block, unsigned waves_pe
     size_t fill = (size_t)num_cus() * waves_per_cu * kWave / block;
     if (fill < 1) fill = 1;
     size_t g = need < fill ? need : fill;
+    return (unsigned)(g < 1 ? 1 : g);
+}
+
+// Workgroups of a grid-stride kernel that the whole device holds at once (its occupancy per CU times
+// the CUs).  A larger grid runs in rounds, the last of them partly empty; `need` caps it for small work.
+// The occupancy query is remembered per (kernel, block size, LDS bytes).
+template <typename F>
+unsigned resident_grid(F kernel, int block, size_t lds, size_t need) {
+    static std::mutex mu;
+    static std::map<std::tuple<const void*, int, size_t>, int> memo;
+    const auto key = std::make_tuple(reinterpret_cast<const void*>(kernel), block, lds);
+    int per_cu = 0;
+    {
+        std::lock_guard<std::mutex> l(mu);
+        auto it = memo.find(key);
+        if (it != memo.end()) per_cu = it->second;
+    }
+    if (per_cu == 0) {
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, lds) != hipSuccess || per_cu < 1)
+            per_cu = 1;
+        std::lock_guard<std::mutex> l(mu);
+        memo[key] = per_cu;
+    }
+    const size_t fill = (size_t)per_cu * (size_t)num_cus();
+    const size_t g = need < fill ? need : fill;
     return (unsigned)(g < 1 ? 1 : g);
 }
 

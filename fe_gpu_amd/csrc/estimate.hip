@@ -384,34 +384,37 @@ struct WaveNest {
     const uint8_t* w;           // panel-order weights (nullptr: all rows)
 };
 
-// Row data of one wave tile of the nested-subset kernel, loaded one tile ahead of its use.
-template <int KT>
+// Row data of one wave tile of the nested-subset kernel as loaded: the loads are issued one tile ahead
+// of their use, without a branch and from clamped addresses (rows past the tile read its first row), and
+// nothing is computed from them until the tile is processed, so the wait for them falls an iteration
+// after their issue.  The tile records are loaded one tile earlier still.
+template <int KT, typename TX>
 struct NestRows {
     int64_t rec;
     int ip, k0, k1;
-    bool w;
-    double t[KT];  // scores r x
+    uint8_t w;
+    double r;
+    TX x[KT];
 };
 
+__device__ __forceinline__ int64_t load_nest_rec(const WaveNest& a, int64_t t) {
+    return a.wtile_rec[t < a.n_wtiles ? t : a.n_wtiles - 1];
+}
+
 template <int KT, typename TX>
-__device__ __forceinline__ void load_nest_rows(const WaveNest& a, const TX* const* __restrict__ X, int64_t t, int lane,
-                                               NestRows<KT>& q) {
-    if (t >= a.n_wtiles) {
-        q.rec = 0;
-        return;
-    }
-    q.rec = a.wtile_rec[t];
-    const int64_t R0 = q.rec & 0xffffffffll;
-    const int nr = (int)(q.rec >> 32);
-    const bool valid = lane < nr;
-    const int64_t ri = R0 + (valid ? lane : 0);
-    q.ip = valid ? a.idP[ri] : -1;
-    q.k0 = (valid && a.kd[0]) ? a.kd[0][ri] : -1;
-    q.k1 = (valid && a.kd[1]) ? a.kd[1][ri] : -1;
-    q.w = valid && (!a.w || a.w[ri]);
-    const double rr = valid ? a.r[ri] : 0.0;
+__device__ __forceinline__ void load_nest_rows(const WaveNest& a, const TX* const (&xk)[KT], int64_t rec, int lane,
+                                               NestRows<KT, TX>& q) {
+    q.rec = rec;
+    const int64_t R0 = rec & 0xffffffffll;
+    const int nr = (int)(rec >> 32);
+    const int64_t ri = R0 + (lane < nr ? lane : 0);
+    q.ip = a.idP[ri];
+    q.k0 = (a.kd[0] ? a.kd[0] : a.idP)[ri];
+    q.k1 = (a.kd[1] ? a.kd[1] : a.idP)[ri];
+    q.w = (a.w ? a.w : reinterpret_cast<const uint8_t*>(a.idP))[ri];
+    q.r = a.r[ri];
 #pragma unroll
-    for (int k = 0; k < KT; ++k) q.t[k] = (k < a.K && valid) ? rr * (double)X[k][ri] : 0.0;
+    for (int k = 0; k < KT; ++k) q.x[k] = xk[k][ri];
 }
 
 // Per subset s: part[(s * gridDim.x + block) * NP + e] (upper triangle of the block's sum of S S') and
@@ -427,21 +430,29 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
     const int lane = threadIdx.x & (kWave - 1);
     double* mystage = gst + (size_t)(threadIdx.x / kWave) * gram_stage<KT>();
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
-    // software pipeline: the next tile's rows are loaded while the current tile is reduced
+    const TX* xk[KT];  // columns past K read the first one (values unused)
+#pragma unroll
+    for (int k = 0; k < KT; ++k) xk[k] = X[k < a.K ? k : 0];
     int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    NestRows<KT> cur, nxt;
-    load_nest_rows<KT, TX>(a, X, t, lane, cur);
+    NestRows<KT, TX> cur, nxt;
+    load_nest_rows<KT, TX>(a, xk, load_nest_rec(a, t), lane, cur);
+    int64_t rec_n = load_nest_rec(a, t + wstride);
     for (; t < a.n_wtiles; t += wstride) {
-        load_nest_rows<KT, TX>(a, X, t + wstride, lane, nxt);
         const int nr = (int)(cur.rec >> 32);
-        const int ip = cur.ip, ka = cur.k0, kb = cur.k1;
-        const bool wrow = cur.w;
+        const bool valid = lane < nr;
+        const int ip = valid ? cur.ip : -1;
+        const int ka = (valid && a.kd[0]) ? cur.k0 : -1, kb = (valid && a.kd[1]) ? cur.k1 : -1;
+        const bool wrow = valid && (!a.w || cur.w);
+        const double rr = valid ? cur.r : 0.0;
         double tv[KT];
 #pragma unroll
-        for (int k = 0; k < KT; ++k) tv[k] = cur.t[k];
+        for (int k = 0; k < KT; ++k) tv[k] = k < a.K ? rr * (double)cur.x[k] : 0.0;
+        // the next tiles' record and rows
+        const int64_t rec_nn = load_nest_rec(a, t + 2 * wstride);
+        load_nest_rows<KT, TX>(a, xk, rec_n, lane, nxt);
+        rec_n = rec_nn;
         cur = nxt;
-        if (nr <= 0) continue;
-        const bool valid = lane < nr;
+        __builtin_amdgcn_sched_barrier(0);
         const int ipp = __shfl_up(ip, 1, kWave), kap = __shfl_up(ka, 1, kWave), kbp = __shfl_up(kb, 1, kWave);
         const bool hs = valid && (lane == 0 || ip != ipp);
         const unsigned long long H = __ballot(hs);
@@ -1634,7 +1645,7 @@ int finish_request(Panel* p, ReqState& s) {
         // Subsets with the primary dim on wave tiles: one fused pass forms all of their meats (a subset
         // whose clusters are single rows repeats the meat of the first such subset).
         std::map<int, std::pair<std::vector<double>, int64_t>> nest_pre;  // subset mask -> (meat, clusters)
-        if (p->n_long == 0 && p->n_mtiles == 0 && K <= 12) {
+        if (p->n_long == 0 && p->n_mtiles == 0 && p->n_wtiles > 0 && K <= 12) {
             WaveNest wn{};
             wn.r = r;
             wn.K = K;
@@ -1677,9 +1688,16 @@ int finish_request(Panel* p, ReqState& s) {
             }
             wn.nsub = (int)fused.size();
             if (wn.nsub > 0) {
-                const unsigned grid = (unsigned)std::max<int64_t>(
-                    1, std::min<int64_t>((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave), kMeatBlocks));
                 const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
+                const size_t need = (size_t)((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave));
+                // one round of resident workgroups
+                const unsigned grid =
+                    KT == 4 ? (X32 ? resident_grid(wave_nested_meat_kernel<4, float>, kBlock, 0, need)
+                                   : resident_grid(wave_nested_meat_kernel<4, double>, kBlock, 0, need))
+                    : KT == 8 ? (X32 ? resident_grid(wave_nested_meat_kernel<8, float>, kBlock, 0, need)
+                                     : resident_grid(wave_nested_meat_kernel<8, double>, kBlock, 0, need))
+                              : (X32 ? resident_grid(wave_nested_meat_kernel<12, float>, kBlock, 0, need)
+                                     : resident_grid(wave_nested_meat_kernel<12, double>, kBlock, 0, need));
                 const int NP = KT * (KT + 1) / 2;
                 double* npart = nullptr;
                 unsigned long long* ucounts = nullptr;

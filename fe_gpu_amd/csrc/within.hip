@@ -285,7 +285,8 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     // tile is processed, so a tile waits only for its table gathers
     int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     // records two tiles ahead, row data one tile ahead: loading the next tile's rows never waits for a
-    // record requested in the same iteration
+    // record requested in the same iteration (gathers a tile ahead as well cost a wave per SIMD of
+    // occupancy and more than they hide)
     WaveRows<ND> cur, nxt;
     load_wave_rows(a, load_wave_rec(a, t), lane, cur);
     int64_t rec_n = load_wave_rec(a, t + wstride);
@@ -1236,7 +1237,7 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s);
 // the run partials (only_dim >= 0 restricts to one scatter dim)
 template <int CT>
 void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, double* q, unsigned grid,
-                   const int* active, hipStream_t s, bool time_runs) {
+                   const int* active, hipStream_t s, bool time_runs, double on = 0.0) {
     for (int o = 0; o < L.nd; ++o) {
         if (only_dim >= 0 && o != only_dim) continue;
         const ScatterDim& d = L.sd[o];
@@ -1250,7 +1251,8 @@ void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, 
             hipLaunchKernelGGL((reduce_runs_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock, 32)), dim3(kBlock), 0,
                                s, (const double*)a.sd[o].runbuf, (const int64_t*)D.grp_run_off, d.G, qo, active);
             if (time_runs)
-                kstat_end(kStatReduceRuns, s, 8.0 * CT * (double)(D.n_runs + d.G) + 8.0 * (double)(d.G + 1));
+                kstat_end(kStatReduceRuns, s,
+                          on > 0.0 ? 8.0 * on * (double)(D.n_runs + d.G) + 8.0 * (double)(d.G + 1) : 0.0);
         }
     }
 }
@@ -1258,17 +1260,24 @@ void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, 
 // One operator product (kIter) or right-hand side (kData) into q (table space [S][CT]).
 template <int CT, int MODE>
 unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
-    const unsigned gw = wave_grid(p), gm = mtile_grid(p), gl = long_grid(p);
+    unsigned gw = wave_grid(p);
+    const unsigned gm = mtile_grid(p), gl = long_grid(p);
     size_t lt = tables_lds<CT>(L);
     if (MODE == kFinal && a.gram) lt += sizeof(double) * (size_t)(kBlock / kWave) * gram_stage<CT>();
     a.part_base = 0;
     if (gw > 0) {
-        if (a.nd <= 2)
+        // two rounds of resident workgroups, so that workgroups of the second round fill the places of
+        // the first round's early finishers (at most the panel's grid, which sizes the partial buffers)
+        if (a.nd <= 2) {
+            gw = std::min(gw, 2u * resident_grid(wave_tile_kernel<CT, MODE, 2>, kBlock, lt, gw));
             hipLaunchKernelGGL((wave_tile_kernel<CT, MODE, 2>), dim3(gw), dim3(kBlock), lt, s, a);
-        else if (a.nd == 3)
+        } else if (a.nd == 3) {
+            gw = std::min(gw, 2u * resident_grid(wave_tile_kernel<CT, MODE, 3>, kBlock, lt, gw));
             hipLaunchKernelGGL((wave_tile_kernel<CT, MODE, 3>), dim3(gw), dim3(kBlock), lt, s, a);
-        else
+        } else {
+            gw = std::min(gw, 2u * resident_grid(wave_tile_kernel<CT, MODE, kMaxScatter>, kBlock, lt, gw));
             hipLaunchKernelGGL((wave_tile_kernel<CT, MODE, kMaxScatter>), dim3(gw), dim3(kBlock), lt, s, a);
+        }
     }
     if (gm > 0) {
         ApplyArgs b = a;
@@ -1285,28 +1294,33 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
     return gw + gm + gl;
 }
 
-// One operator product (kIter) or right-hand side (kData) into q (table space [S][CT]).
+// One operator product (kIter) or right-hand side (kData) into q (table space [S][CT]).  n_on: the
+// columns this launch works on (the right-hand side: every column; an operator product: the columns
+// whose CG is still running, which the caller knows only when it has polled the state just before;
+// -1 = all), used for the algorithmic byte count of per-kernel timing.
 template <int CT>
-int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* q, hipStream_t s) {
+int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* q, hipStream_t s, int n_on = -1) {
     const int sid = mode == kIter ? kStatApplyIter : kStatApplyData;
     const bool timed = kstats_enabled();
     if (timed) kstat_begin(sid, s);
     const unsigned grid = mode == kIter ? launch_pass<CT, kIter>(p, L, a, s) : launch_pass<CT, kData>(p, L, a, s);
     FEHIP_LAUNCH_CHECK("operator kernels");
+    const double on = n_on < 0 ? (double)a.ncols : (double)n_on;
     if (timed) {
         // algorithmic bytes: id streams (the scattered dims and the primary, which marks the segments),
-        // row masks, table reads, run-partial and per-block partial writes (+ the data columns and the
-        // permutation for the right-hand side)
+        // row masks, table reads, run-partial and per-block partial writes of the columns worked on (+ the
+        // data columns and the permutation for the right-hand side); a launch after every column has
+        // converged returns at once and moves nothing
         const double n = (double)p->n;
-        double bytes = n * 4.0 * (L.nd + 1) + (a.wm ? 4.0 * n : 0.0) + 8.0 * (double)grid * L.Gs * CT;
+        double bytes = n * 4.0 * (L.nd + 1) + (a.wm ? 4.0 * n : 0.0) + 8.0 * (double)grid * L.Gs * on;
         for (int o = 0; o < L.nd; ++o) {
-            bytes += 8.0 * CT * (double)L.sd[o].G;
-            if (!L.sd[o].small) bytes += (8.0 * CT + 4.0) * (double)p->d[L.dims[o]].n_runs;
+            bytes += 8.0 * on * (double)L.sd[o].G;
+            if (!L.sd[o].small) bytes += (8.0 * on + 4.0) * (double)p->d[L.dims[o]].n_runs;
         }
         if (mode == kData) bytes += n * 8.0 * a.ncols + (p->perm ? 4.0 * n : 0.0);
-        kstat_end(sid, s, bytes);
+        kstat_end(sid, s, on > 0.0 ? bytes : 0.0);
     }
-    reduce_tables<CT>(p, L, a, -1, q, grid, mode == kIter ? a.active : nullptr, s, mode == kIter && timed);
+    reduce_tables<CT>(p, L, a, -1, q, grid, mode == kIter ? a.active : nullptr, s, mode == kIter && timed, on);
     FEHIP_LAUNCH_CHECK("reduce kernels");
     return 0;
 }
@@ -1448,8 +1462,9 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
         a.p = pp;
         a.active = sc.st->active;
         // one CG step: operator product, p.q, alpha, update, r.z, beta, new direction
+        int n_on = -1;  // columns still running (known under per-kernel timing, which polls every step)
         auto step = [&](hipStream_t st) -> int {
-            int e2 = apply_and_reduce<CT>(p, L, a, kIter, q, st);
+            int e2 = apply_and_reduce<CT>(p, L, a, kIter, q, st, n_on);
             if (e2) return e2;
             hipLaunchKernelGGL((cg_pq_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S, (const double*)pp,
                                (const double*)q, sc.part);
@@ -1508,6 +1523,14 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
                 for (int c = 0; c < CT; ++c) any |= sc.st_host->active[c] != 0;
                 if (!any) break;
             }
+            if (!gexec && kstats_enabled()) {
+                // per-kernel timing: the state is read before every step so the step's byte count covers
+                // the columns it works on; the launches are those of the untimed path
+                FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs), "cg state");
+                FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
+                n_on = 0;
+                for (int c = 0; c < ncols; ++c) n_on += sc.st_host->active[c] != 0;
+            }
             if (gexec) {
                 FEHIP_CHECK(hipGraphLaunch(gexec, gs), "cg graph launch");
             } else if ((e = step(gs))) {
@@ -1548,7 +1571,7 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
     if (kstats_enabled()) {
         double bytes = 16.0 * (double)p->n * ncols + 4.0 * (double)p->n * (L.nd + 1) +
                        (a.wm ? 4.0 * (double)p->n : 0.0) + (p->perm ? 4.0 * (double)p->n : 0.0);
-        for (int od = 0; od < L.nd; ++od) bytes += 8.0 * CT * (double)L.sd[od].G;
+        for (int od = 0; od < L.nd; ++od) bytes += 8.0 * ncols * (double)L.sd[od].G;
         kstat_end(kStatFinalize, s, bytes);
     }
     FEHIP_CHECK(hipStreamSynchronize(s), "within sync");
