@@ -17,6 +17,7 @@
 #include <map>
 #include <set>
 #include <tuple>
+#include <type_traits>
 #include <vector>
 
 #include <rocprim/rocprim.hpp>
@@ -111,15 +112,6 @@ __global__ __launch_bounds__(256) void resid_scores_kernel(const TX* const* __re
         double t = 0.0;
         for (int w = 0; w < 256 / 64; ++w) t += ws[w];
         part[blockIdx.x] = t;
-    }
-}
-
-__global__ void residual_work_kernel(const double* __restrict__ work, int64_t n, int K,
-                                     const double* __restrict__ beta, double* r) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-        double s = work[i];
-        for (int k = 0; k < K; ++k) s -= work[i + (int64_t)(1 + k) * n] * beta[k];
-        r[i] = s;
     }
 }
 
@@ -360,111 +352,209 @@ int cluster_sums(const double* T, int K, const int* order, const int* keys, int6
 // Subsets containing the primary dim: every cluster lies inside one primary segment, so a tile of
 // whole segments (<= 256 rows, staged in LDS) holds each of its clusters entirely.  A row that is
 // the first of its cluster in the segment sums the cluster's rows and adds S S' to its registers.
-// Subsets that contain the primary dim P, on the wave tiles (whole primary segments of <= 64 rows,
-// one row per lane): the scores t = r x are formed on the fly from the residual and the regressor
-// columns, summed per run of equal subset ids by a segmented DPP scan, and runs of one segment with
-// the same ids are merged (a cluster inside a segment need not be contiguous).  Each cluster's
-// leader lane accumulates S S' in registers; block partials of the upper triangle.
-// Subsets that contain the primary dim, on the wave tiles: a subset's clusters never cross a primary
-// segment, so one wave per tile forms every cluster sum with segmented scans in registers.  Up to four
-// such subsets share one pass over the residual and the regressors (each reads the same rows); the
-// subset's other dims are among at most two (kd[0], kd[1]).
+// One pass over the wave tiles (whole primary segments of <= 64 rows, one row per lane) forms the
+// cluster sums of every subset whose clusters a tile's rows determine:
+//  - subsets containing the primary dim P: a cluster never crosses a primary segment, so the scores
+//    t = r x are summed per run of equal subset ids by a segmented DPP scan, runs of one segment with
+//    the same ids are merged (a cluster inside a segment need not be contiguous), and each cluster's
+//    leader lane adds S S' to matrix-core accumulators in registers (up to four such subsets);
+//  - a single large dim other than P: the sums of t over the runs of equal id inside each segment are
+//    stored at the run's place in group order (the panel's run numbering), so the groups' sums are
+//    contiguous segment sums afterwards (up to two such dims);
+//  - subsets of small dims only: t is added into per-block LDS tables indexed by the mixed-radix cell
+//    of the row's ids, with a count of the rows in the subsample (up to three such subsets).
+// The residual is read, or formed in FP64 as y~ - W~ beta from the stored columns, and the block's sum
+// of r^2 is written.  The other cluster dims are loaded per row from up to three slots.
 constexpr int kMaxNestSubsets = 4;
+constexpr int kMaxRunOuts = 2;
+constexpr int kMaxSmallOuts = 3;
+constexpr int kScoreSmallLds = 4096;  // doubles of LDS for the small-cell tables of one block
 
-struct WaveNest {
-    const double* r;            // residual (panel order)
+struct WaveScores {
+    const double* r;            // residual (panel order), or nullptr: formed from y and beta
+    const void* y;              // y~ column (the columns' type) when r is null
+    const double* beta;         // [K] when r is null
     int K;
     const int* idP;
     const int64_t* wtile_rec;   // first row | rows << 32 of each wave tile
     int64_t n_wtiles;
-    const int* kd[2];           // the other cluster dims of the subsets (panel order ids)
+    const int* kd[3];           // ids of the other cluster dims (nullptr: slot unused)
     int nsub;
-    int use[kMaxNestSubsets];   // per subset: bit j set -> kd[j] belongs to the subset
+    int use[kMaxNestSubsets];   // per subset: bit j set -> slot j belongs to the subset
     int mode[kMaxNestSubsets];  // 2: singletons, 1: clusters are single runs, 0: merge runs of a segment
-    const uint8_t* w;           // panel-order weights (nullptr: all rows)
+    int nrun;
+    int rslot[kMaxRunOuts];     // slot of the run dim
+    int rshare[kMaxRunOuts];    // subset whose run scan is this dim's (its use is this slot alone), or -1
+    const int* rdest[kMaxRunOuts];  // per row: the run's place in group order at run ends (-1 inside)
+    double* runbuf[kMaxRunOuts];    // [runs][KT]
+    int nsmall;
+    int sstride[kMaxSmallOuts][3];  // per slot: the radix of its ids in the cell index (0: not in the subset)
+    int soff[kMaxSmallOuts];        // LDS offset (doubles) of the subset's table [cells][KT + 1]
+    int small_lds;                  // doubles of LDS tables
+    double* small_part;             // [grid][small_lds]
+    double* rss_part;               // [grid]
+    const uint8_t* w;               // panel-order weights (nullptr: all rows)
 };
 
-// Row data of one wave tile of the nested-subset kernel as loaded: the loads are issued one tile ahead
-// of their use, without a branch and from clamped addresses (rows past the tile read its first row), and
-// nothing is computed from them until the tile is processed, so the wait for them falls an iteration
-// after their issue.  The tile records are loaded one tile earlier still.
+// Row data of one wave tile as loaded: the loads are issued one tile ahead of their use, without a
+// branch and from clamped addresses (rows past the tile read its first row; unused slots read the
+// primary ids), and nothing is computed from them until the tile is processed, so the wait for them
+// falls an iteration after their issue.  The tile records are loaded one tile earlier still.
 template <int KT, typename TX>
-struct NestRows {
+struct ScoreRows {
     int64_t rec;
-    int ip, k0, k1;
+    int ip, k[3], rd[kMaxRunOuts];
     uint8_t w;
-    double r;
+    double r;  // residual when read; with FP64 columns also y~ when the residual is formed
+    TX y;      // FP32 columns: y~ when the residual is formed
     TX x[KT];
 };
 
-__device__ __forceinline__ int64_t load_nest_rec(const WaveNest& a, int64_t t) {
+__device__ __forceinline__ int64_t load_score_rec(const WaveScores& a, int64_t t) {
     return a.wtile_rec[t < a.n_wtiles ? t : a.n_wtiles - 1];
 }
 
 template <int KT, typename TX>
-__device__ __forceinline__ void load_nest_rows(const WaveNest& a, const TX* const (&xk)[KT], int64_t rec, int lane,
-                                               NestRows<KT, TX>& q) {
+__device__ __forceinline__ void load_score_rows(const WaveScores& a, const TX* const (&xk)[KT], const TX* yc,
+                                                int64_t rec, int lane, ScoreRows<KT, TX>& q) {
     q.rec = rec;
     const int64_t R0 = rec & 0xffffffffll;
     const int nr = (int)(rec >> 32);
     const int64_t ri = R0 + (lane < nr ? lane : 0);
     q.ip = a.idP[ri];
-    q.k0 = (a.kd[0] ? a.kd[0] : a.idP)[ri];
-    q.k1 = (a.kd[1] ? a.kd[1] : a.idP)[ri];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) q.k[j] = (a.kd[j] ? a.kd[j] : a.idP)[ri];
+#pragma unroll
+    for (int j = 0; j < kMaxRunOuts; ++j) q.rd[j] = (j < a.nrun ? a.rdest[j] : a.idP)[ri];
     q.w = (a.w ? a.w : reinterpret_cast<const uint8_t*>(a.idP))[ri];
-    q.r = a.r[ri];
+    if constexpr (std::is_same<TX, double>::value)
+        q.r = (a.r ? a.r : yc)[ri];
+    else if (a.r)
+        q.r = a.r[ri];
+    else
+        q.y = yc[ri];
 #pragma unroll
     for (int k = 0; k < KT; ++k) q.x[k] = xk[k][ri];
 }
 
 // Per subset s: part[(s * gridDim.x + block) * NP + e] (upper triangle of the block's sum of S S') and
-// ucount[s] (non-empty clusters).
-template <int KT, typename TX>
-__global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, const TX* const* __restrict__ X, double* part,
-                                                                  unsigned long long* ucount) {
+// ucount[s] (non-empty clusters); run sums into runbuf; small tables into small_part; r^2 into rss_part.
+// NS: compile-time bound on the nested subsets (a.nsub <= NS), which sizes their accumulators
+template <int KT, typename TX, int NS>
+__global__ __launch_bounds__(kBlock) void wave_scores_kernel(WaveScores a, const TX* const* __restrict__ X, double* part,
+                                                             unsigned long long* ucount) {
     __shared__ double gst[(kBlock / kWave) * gram_stage<KT>()];  // per-wave staging of the cluster sums
-    gram4_t acc[kMaxNestSubsets];
+    extern __shared__ double stab[];                            // small-cell tables [small_lds]
+    __shared__ double rss_w[kBlock / kWave];
+    for (int e = threadIdx.x; e < a.small_lds; e += blockDim.x) stab[e] = 0.0;
+    __syncthreads();
+    gram4_t acc[NS];
 #pragma unroll
-    for (int q = 0; q < kMaxNestSubsets; ++q) acc[q] = gram4_t{0.0, 0.0, 0.0, 0.0};
-    unsigned long long nclusters[kMaxNestSubsets] = {0, 0, 0, 0};
+    for (int q = 0; q < NS; ++q) acc[q] = gram4_t{0.0, 0.0, 0.0, 0.0};
+    unsigned long long nclusters[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) nclusters[q] = 0;
+    double rss = 0.0;
     const int lane = threadIdx.x & (kWave - 1);
     double* mystage = gst + (size_t)(threadIdx.x / kWave) * gram_stage<KT>();
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
     const TX* xk[KT];  // columns past K read the first one (values unused)
 #pragma unroll
     for (int k = 0; k < KT; ++k) xk[k] = X[k < a.K ? k : 0];
+    const TX* yc = a.r ? xk[0] : static_cast<const TX*>(a.y);
     int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-    NestRows<KT, TX> cur, nxt;
-    load_nest_rows<KT, TX>(a, xk, load_nest_rec(a, t), lane, cur);
-    int64_t rec_n = load_nest_rec(a, t + wstride);
+    ScoreRows<KT, TX> cur, nxt;
+    load_score_rows<KT, TX>(a, xk, yc, load_score_rec(a, t), lane, cur);
+    int64_t rec_n = load_score_rec(a, t + wstride);
     for (; t < a.n_wtiles; t += wstride) {
         const int nr = (int)(cur.rec >> 32);
         const bool valid = lane < nr;
         const int ip = valid ? cur.ip : -1;
-        const int ka = (valid && a.kd[0]) ? cur.k0 : -1, kb = (valid && a.kd[1]) ? cur.k1 : -1;
+        int kk[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) kk[j] = (valid && a.kd[j]) ? cur.k[j] : -1;
+        int rdl[kMaxRunOuts];
+#pragma unroll
+        for (int j = 0; j < kMaxRunOuts; ++j) rdl[j] = cur.rd[j];
         const bool wrow = valid && (!a.w || cur.w);
-        const double rr = valid ? cur.r : 0.0;
+        double rr;
+        if (a.r) {
+            rr = cur.r;
+        } else {
+            if constexpr (std::is_same<TX, double>::value)
+                rr = cur.r;
+            else
+                rr = (double)cur.y;
+#pragma unroll
+            for (int k = 0; k < KT; ++k)
+                if (k < a.K) rr = fma(-a.beta[k], (double)cur.x[k], rr);
+        }
+        rr = valid ? rr : 0.0;
+        rss = fma(rr, rr, rss);
         double tv[KT];
 #pragma unroll
         for (int k = 0; k < KT; ++k) tv[k] = k < a.K ? rr * (double)cur.x[k] : 0.0;
         // the next tiles' record and rows
-        const int64_t rec_nn = load_nest_rec(a, t + 2 * wstride);
-        load_nest_rows<KT, TX>(a, xk, rec_n, lane, nxt);
+        const int64_t rec_nn = load_score_rec(a, t + 2 * wstride);
+        load_score_rows<KT, TX>(a, xk, yc, rec_n, lane, nxt);
         rec_n = rec_nn;
         cur = nxt;
         __builtin_amdgcn_sched_barrier(0);
-        const int ipp = __shfl_up(ip, 1, kWave), kap = __shfl_up(ka, 1, kWave), kbp = __shfl_up(kb, 1, kWave);
+        const int ipp = __shfl_up(ip, 1, kWave);
+        int kp[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) kp[j] = __shfl_up(kk[j], 1, kWave);
         const bool hs = valid && (lane == 0 || ip != ipp);
         const unsigned long long H = __ballot(hs);
         const int ss = 63 - __clzll(H & mask_le(lane));  // segment start lane
         const unsigned long long BW = __ballot(wrow);
+        // small-dim subsets: the row's scores into its cell
+        for (int q = 0; q < a.nsmall; ++q) {
+            int cell = 0;
 #pragma unroll
-        for (int q = 0; q < kMaxNestSubsets; ++q) {
+            for (int j = 0; j < 3; ++j) cell += kk[j] * a.sstride[q][j];
+            if (valid) {
+                double* row = stab + a.soff[q] + cell * (KT + 1);
+#pragma unroll
+                for (int k = 0; k < KT; ++k)
+                    if (k < a.K) lds_add(&row[k], tv[k]);
+                if (wrow) lds_add(&row[KT], 1.0);
+            }
+        }
+        // run sums of a single large dim: the runs of equal id inside the segment
+        auto store_runs = [&](int j, const double (&sv)[KT], bool rend) {
+            if (rend) {
+                double* dst = a.runbuf[j] + (int64_t)rdl[j] * KT;
+#pragma unroll
+                for (int k = 0; k < KT; ++k) dst[k] = sv[k];
+            }
+        };
+        for (int j = 0; j < a.nrun; ++j) {
+            if (a.rshare[j] >= 0) continue;  // stored from the subset that shares its scan
+            const int s0 = a.rslot[j];
+            const int kj = s0 == 0 ? kk[0] : (s0 == 1 ? kk[1] : kk[2]);
+            const int kjp = s0 == 0 ? kp[0] : (s0 == 1 ? kp[1] : kp[2]);
+            const bool rh = valid && (hs || kj != kjp);
+            const unsigned long long HR = __ballot(rh);
+            const int rs = 63 - __clzll(HR & mask_le(lane));
+            const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
+            double sv[KT];
+#pragma unroll
+            for (int k = 0; k < KT; ++k) sv[k] = tv[k];
+            seg_scan_cols<KT>(sv, ScanFactors(ScanMask(lane, rs)));
+            store_runs(j, sv, rend);
+        }
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
             if (q >= a.nsub) break;
             const int use = a.use[q], mode = a.mode[q];
-            const int k0 = (use & 1) ? ka : -1, k1 = (use & 2) ? kb : -1;
-            const int k0p = (use & 1) ? kap : -1, k1p = (use & 2) ? kbp : -1;
-            const bool rh = valid && (hs || k0 != k0p || k1 != k1p);
+            int ks[3], ksp[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                ks[j] = ((use >> j) & 1) ? kk[j] : -1;
+                ksp[j] = ((use >> j) & 1) ? kp[j] : -1;
+            }
+            const bool rh = valid && (hs || ks[0] != ksp[0] || ks[1] != ksp[1] || ks[2] != ksp[2]);
             const unsigned long long HR = __ballot(rh);
             const unsigned long long le = HR & mask_le(lane);
             const int rs = 63 - __clzll(le);
@@ -474,6 +564,8 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
 #pragma unroll
             for (int k = 0; k < KT; ++k) sv[k] = tv[k];
             if (mode != 2) seg_scan_cols<KT>(sv, ScanFactors(ScanMask(lane, rs)));
+            for (int j = 0; j < a.nrun; ++j)
+                if (a.rshare[j] == q) store_runs(j, sv, rend);
             bool leader = rend;
             if (use != 0 && mode == 0) {
                 // merge the runs of one segment that carry the same ids
@@ -481,8 +573,10 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
                 while (E) {
                     const int j = __ffsll((long long)E) - 1;
                     E &= E - 1ull;
-                    const int kj0 = __shfl(k0, j, kWave), kj1 = __shfl(k1, j, kWave), sj = __shfl(ss, j, kWave);
-                    const bool match = rend && lane != j && sj == ss && kj0 == k0 && kj1 == k1;
+                    const int kj0 = __shfl(ks[0], j, kWave), kj1 = __shfl(ks[1], j, kWave),
+                              kj2 = __shfl(ks[2], j, kWave), sj = __shfl(ss, j, kWave);
+                    const bool match =
+                        rend && lane != j && sj == ss && kj0 == ks[0] && kj1 == ks[1] && kj2 == ks[2];
                     if (match && j < lane) leader = false;
                     const bool take = match && j > lane;
                     if (__ballot(take)) {
@@ -507,10 +601,33 @@ __global__ __launch_bounds__(kBlock) void wave_nested_meat_kernel(WaveNest a, co
     }
     constexpr int NP = KT * (KT + 1) / 2;
 #pragma unroll
-    for (int q = 0; q < kMaxNestSubsets; ++q) {
+    for (int q = 0; q < NS; ++q) {
         if (q >= a.nsub) break;
         if (lane == 0 && nclusters[q]) atomicAdd(ucount + q, nclusters[q]);
         gram_store_upper<KT>(gst, gram_stage<KT>(), acc[q], part + ((int64_t)q * gridDim.x + blockIdx.x) * NP);
+    }
+    rss = wave_sum(rss);
+    if (lane == 0) rss_w[threadIdx.x / kWave] = rss;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t2 = 0.0;
+        for (int v = 0; v < kBlock / kWave; ++v) t2 += rss_w[v];
+        a.rss_part[blockIdx.x] = t2;
+    }
+    for (int e = threadIdx.x; e < a.small_lds; e += blockDim.x)
+        a.small_part[(int64_t)blockIdx.x * a.small_lds + e] = stab[e];
+}
+
+// sums of the run partials of each group (contiguous in group order): S[g][c] for c < KT
+template <int KT>
+__global__ void group_sum_kernel(const double* __restrict__ runbuf, const int64_t* __restrict__ goff, int G,
+                                 double* S) {
+    const int64_t total = (int64_t)G * KT;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int g = (int)(e / KT), c = (int)(e % KT);
+        double sm = 0.0;
+        for (int64_t k = goff[g]; k < goff[g + 1]; ++k) sm += runbuf[k * KT + c];
+        S[e] = sm;
     }
 }
 
@@ -1114,10 +1231,123 @@ int project_meat(const std::vector<double>& Mz, int m, const std::vector<double>
     return 0;
 }
 
-int cached_nest_mode(Panel* p, unsigned dims_mask) {
-    std::lock_guard<std::recursive_mutex> guard(p->mu);
-    auto it = p->nest_modes.find(dims_mask);
-    return it == p->nest_modes.end() ? -1 : it->second;
+// Which subsets of the sandwich the fused pass over the wave tiles forms, and how: the slot layout of
+// the other cluster dims and the pass's output tables.  The rest go to the methods that read the
+// residual or the score matrix.
+enum { kRouteOther = 0, kRouteNest = 1, kRouteRun = 2, kRouteSmall = 3, kRouteAlias = 4 };
+
+struct ScorePlan {
+    bool fused = false;   // a fused pass runs
+    int KT = 0;
+    WaveScores ws{};      // slots and output tables (buffers are set at the launch)
+    int route[8] = {};    // per subset mask
+    int index[8] = {};    // nested subset, run output or small table of the subset (alias: the subset it repeats)
+    int run_dim[kMaxRunOuts] = {};
+    int small_cells[kMaxSmallOuts] = {};
+    bool needs_r = false;  // a subset left to the other methods reads the residual or the score matrix
+    bool reads_T = false;  // ... the score matrix
+};
+
+int plan_scores(Panel* p, const int* cdims, int ncd, int Kt, unsigned long long* dcount, ScorePlan& pl) {
+    pl = ScorePlan();
+    const bool fused_ok = Kt <= 12 && p->n_long == 0 && p->n_mtiles == 0 && p->n_wtiles > 0;
+    pl.KT = Kt <= 4 ? 4 : (Kt <= 8 ? 8 : 12);
+    WaveScores& w = pl.ws;
+    auto slot_of = [&](int d) {
+        for (int j = 0; j < 3; ++j)
+            if (w.kd[j] == p->d[d].ids) return j;
+        for (int j = 0; j < 3; ++j)
+            if (!w.kd[j]) {
+                w.kd[j] = p->d[d].ids;
+                return j;
+            }
+        return -1;  // not reached: at most three cluster dims
+    };
+    int single_q = -1;
+    for (int mask = 1; mask < (1 << ncd); ++mask) {
+        unsigned dm = 0;
+        int len = 0;
+        for (int j = 0; j < ncd; ++j)
+            if (mask & (1 << j)) {
+                dm |= 1u << (cdims[j] - 1);
+                ++len;
+            }
+        const bool has_p = (dm >> p->P) & 1u;
+        bool all_small = true;
+        int64_t cells = 1;
+        for (int d = 0; d < p->m; ++d)
+            if ((dm >> d) & 1u) {
+                all_small &= p->d[d].small != 0;
+                cells *= p->d[d].G;
+            }
+        const int single = len == 1 ? __builtin_ctz(dm) : -1;
+        int route = kRouteOther;
+        if (fused_ok && has_p) {
+            const int* kid[2] = {nullptr, nullptr};
+            int nk = 0;
+            for (int d = 0; d < p->m; ++d)
+                if (d != p->P && ((dm >> d) & 1u)) kid[nk++] = p->d[d].ids;
+            int mode = 1;
+            int e = nest_mode(p, dm, kid, nk, dcount, &mode);
+            if (e) return e;
+            if (mode == 2 && single_q >= 0) {  // singletons: the meat of the first such subset
+                route = kRouteAlias;
+                pl.index[mask] = single_q;
+            } else if (w.nsub < kMaxNestSubsets) {
+                int use = 0;
+                for (int d = 0; d < p->m; ++d)
+                    if (d != p->P && ((dm >> d) & 1u)) use |= 1 << slot_of(d);
+                if (mode == 2) single_q = mask;
+                w.use[w.nsub] = use;
+                w.mode[w.nsub] = mode;
+                pl.index[mask] = w.nsub++;
+                route = kRouteNest;
+            }
+        } else if (fused_ok && single >= 0 && single != p->P && !p->d[single].small && w.nrun < kMaxRunOuts) {
+            const int j = w.nrun++;
+            w.rslot[j] = slot_of(single);
+            w.rshare[j] = -1;
+            w.rdest[j] = p->d[single].row_rdest;
+            pl.run_dim[j] = single;
+            pl.index[mask] = j;
+            route = kRouteRun;
+        } else if (fused_ok && all_small && w.nsmall < kMaxSmallOuts &&
+                   (int64_t)w.small_lds + cells * (pl.KT + 1) <= kScoreSmallLds) {
+            const int q = w.nsmall++;
+            int stride = 1;
+            for (int j = 0; j < 3; ++j) w.sstride[q][j] = 0;
+            for (int d = 0; d < p->m; ++d)
+                if ((dm >> d) & 1u) {
+                    w.sstride[q][slot_of(d)] = stride;
+                    stride *= p->d[d].G;
+                }
+            w.soff[q] = w.small_lds;
+            w.small_lds += (int)cells * (pl.KT + 1);
+            pl.small_cells[q] = (int)cells;
+            pl.index[mask] = q;
+            route = kRouteSmall;
+        }
+        if (route == kRouteOther) {
+            // the other methods: block-tile nested, small tables and run sums read the residual; the
+            // block-tile nested, sorted-partition and segment-sum methods read the score matrix
+            bool reads_T = true;
+            if (has_p && p->n_long == 0 && Kt <= 12)
+                reads_T = p->n_mtiles != 0;
+            else if (all_small && cells <= kSmallCells && Kt <= 11)
+                reads_T = false;
+            else if (single >= 0 && single != p->P && !p->d[single].small && p->n_long == 0 && Kt <= 12)
+                reads_T = false;
+            pl.reads_T |= reads_T;
+            pl.needs_r = true;
+        }
+        pl.route[mask] = route;
+    }
+    // a run dim whose runs are those of a nested subset of the primary and that dim alone shares its scan
+    for (int j = 0; j < w.nrun; ++j)
+        for (int q = 0; q < w.nsub; ++q)
+            if (w.use[q] == (1 << w.rslot[j])) w.rshare[j] = q;
+    pl.fused = w.nsub > 0 || w.nrun > 0 || w.nsmall > 0;
+    return 0;
 }
 
 // One request of a batch: its inputs, and what the preparation step leaves for the solve and the
@@ -1320,17 +1550,20 @@ int finish_request(Panel* p, ReqState& s) {
     // the request's columns [y~ W~ Z~] through a device pointer array
     const double* const* dcols = nullptr;
     const float* const* dcols32 = nullptr;
+    const void* ycol = nullptr;  // y~
     {
         if (s.f32) {
             std::vector<const float*> hp(C);
             for (int j = 0; j < C; ++j)
                 hp[j] = s.shared_cols.empty() ? s.work32 + (int64_t)j * n : (const float*)s.shared_cols[j];
             if ((e = column_array(db, hp, &dcols32))) return e;
+            ycol = hp[0];
         } else {
             std::vector<const double*> hp(C);
             for (int j = 0; j < C; ++j)
                 hp[j] = s.shared_cols.empty() ? s.work + (int64_t)j * n : (const double*)s.shared_cols[j];
             if ((e = column_array(db, hp, &dcols))) return e;
+            ycol = hp[0];
         }
     }
 
@@ -1471,6 +1704,14 @@ int finish_request(Panel* p, ReqState& s) {
     // ---- residual on W~ (also for IV), rss, dof
     double *dbeta = nullptr, *r = nullptr;
     if ((e = db.get(&dbeta, (size_t)K)) || (e = db.get(&r, (size_t)n))) return e;
+    ScorePlan plan;
+    if (ncd > 0) {
+        unsigned long long* pc = nullptr;
+        if ((e = db.get(&pc, 1)) || (e = plan_scores(p, cdims, ncd, use_iv ? n_aug : K, pc, plan))) return e;
+    }
+    // the fused pass forms the residual itself when no other method reads it (OLS only: the 2SLS scores
+    // are formed from other columns than the residual's)
+    bool resid_in_pass = false;
     // score rows T = r W~ are formed in the residual pass when an OLS sandwich has a subset whose method
     // reads them (every subset but those containing the primary on wave tiles)
     double* T_early = nullptr;
@@ -1493,55 +1734,31 @@ int finish_request(Panel* p, ReqState& s) {
         // the FP64 cross products.
         // Scores: r W~, or for 2SLS r Z_aug with Z_aug the exogenous regressors and the instruments, read in
         // place from the work columns.  The sandwich's subset methods that read the score matrix get it
-        // from this pass.
+        // from this pass; when the fused pass over the wave tiles forms every subset of an OLS sandwich,
+        // that pass forms the residual itself and this one is skipped.
         const int Kt = use_iv ? n_aug : K;
         if (use_iv && ncd > 0) {
             std::vector<const double*> zp(n_aug);
             for (int a2 = 0; a2 < n_aug; ++a2) zp[a2] = work + (int64_t)aug_col[a2] * n;
             if ((e = column_array(db, zp, &iv_scores))) return e;
         }
-        bool any_T = false;
-        if (ncd > 0)
-            for (int mask = 1; mask < (1 << ncd); ++mask) {
-                unsigned dm = 0;
-                for (int j = 0; j < ncd; ++j)
-                    if (mask & (1 << j)) dm |= 1u << (cdims[j] - 1);
-                // the subset methods that read the score matrix: block-tile nested, sorted partition, segment
-                // sums; the wave nested, small-table and run methods form r x on the fly
-                const bool has_p = (dm >> p->P) & 1u;
-                bool all_small = true;
-                int64_t cells = 1;
-                int len = 0, single = -1;
-                for (int d = 0; d < p->m; ++d)
-                    if ((dm >> d) & 1u) {
-                        all_small &= p->d[d].small != 0;
-                        cells *= p->d[d].G;
-                        single = d;
-                        ++len;
-                    }
-                if (len != 1) single = -1;
-                bool reads_T = true;
-                if (has_p && p->n_long == 0 && Kt <= 12)
-                    reads_T = p->n_mtiles != 0;
-                else if (all_small && cells <= kSmallCells && Kt <= 11)
-                    reads_T = false;
-                else if (single >= 0 && single != p->P && !p->d[single].small && p->n_long == 0 && Kt <= 12)
-                    reads_T = false;
-                any_T |= reads_T;
-            }
-        if (any_T && (e = db.get(&T_early, (size_t)n * Kt))) return e;
         FEHIP_CHECK(copy_sync(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
-        const size_t tlds = (T_early && Kt <= 16) ? sizeof(double) * 256 * (size_t)Kt : 0;
-        if (f32)
-            hipLaunchKernelGGL(resid_scores_kernel<float>, dim3(kResidBlocks), dim3(256), tlds, 0, dcols32, n, K,
-                               (const double*)dbeta, r, T_early, Kt, dcols32 + 1, part);
-        else
-            hipLaunchKernelGGL(resid_scores_kernel<double>, dim3(kResidBlocks), dim3(256), tlds, 0, dcols, n, K,
-                               (const double*)dbeta, r, T_early, Kt, use_iv ? iv_scores : dcols + 1, part);
-        FEHIP_LAUNCH_CHECK("residual");
-        std::vector<double> ps;
-        if ((e = host_sum_parts(part, kResidBlocks, 1, ps))) return e;
-        rss = ps[0];
+        if (ncd > 0 && !use_iv && plan.fused && !plan.needs_r) {
+            resid_in_pass = true;  // rss: below from the cross products (FP32 storage) or from the pass
+        } else {
+            if (ncd > 0 && plan.reads_T && (e = db.get(&T_early, (size_t)n * Kt))) return e;
+            const size_t tlds = (T_early && Kt <= 16) ? sizeof(double) * 256 * (size_t)Kt : 0;
+            if (f32)
+                hipLaunchKernelGGL(resid_scores_kernel<float>, dim3(kResidBlocks), dim3(256), tlds, 0, dcols32, n, K,
+                                   (const double*)dbeta, r, T_early, Kt, dcols32 + 1, part);
+            else
+                hipLaunchKernelGGL(resid_scores_kernel<double>, dim3(kResidBlocks), dim3(256), tlds, 0, dcols, n, K,
+                                   (const double*)dbeta, r, T_early, Kt, use_iv ? iv_scores : dcols + 1, part);
+            FEHIP_LAUNCH_CHECK("residual");
+            std::vector<double> ps;
+            if ((e = host_sum_parts(part, kResidBlocks, 1, ps))) return e;
+            rss = ps[0];
+        }
         if (f32) {  // y~'y~ - 2 beta' W~'y~ + beta' W~'W~ beta from the FP64 cross products
             double bq = 0.0, by = 0.0;
             for (int i = 0; i < K; ++i) {
@@ -1642,93 +1859,132 @@ int finish_request(Panel* p, ReqState& s) {
         if ((e = db.get(&Md, (size_t)K * K))) return e;
         std::vector<double> M((size_t)K * K), tmp((size_t)kept * kept);
         int64_t min_single = 0x7fffffff;
-        // Subsets with the primary dim on wave tiles: one fused pass forms all of their meats (a subset
-        // whose clusters are single rows repeats the meat of the first such subset).
-        std::map<int, std::pair<std::vector<double>, int64_t>> nest_pre;  // subset mask -> (meat, clusters)
-        if (p->n_long == 0 && p->n_mtiles == 0 && p->n_wtiles > 0 && K <= 12) {
-            WaveNest wn{};
-            wn.r = r;
-            wn.K = K;
-            wn.idP = p->d[p->P].ids;
-            wn.wtile_rec = p->wtile_rec;
-            wn.n_wtiles = p->n_wtiles;
-            wn.w = wpanel;
-            std::vector<int> fused, alias;
-            int single_q = -1;
-            for (int mask = 1; mask < (1 << ncd); ++mask) {
-                unsigned dims_mask = 0;
-                for (int j = 0; j < ncd; ++j)
-                    if (mask & (1 << j)) dims_mask |= 1u << (cdims[j] - 1);
-                if (!((dims_mask >> p->P) & 1u)) continue;
-                const int* kid[2] = {nullptr, nullptr};
-                int nk = 0, use = 0;
-                for (int d = 0; d < p->m; ++d) {
-                    if (d == p->P || !((dims_mask >> d) & 1u)) continue;
-                    kid[nk++] = p->d[d].ids;
-                    int slot = -1;
-                    for (int j = 0; j < 2; ++j)
-                        if (wn.kd[j] == p->d[d].ids) slot = j;
-                    for (int j = 0; j < 2 && slot < 0; ++j)
-                        if (!wn.kd[j]) {
-                            wn.kd[j] = p->d[d].ids;
-                            slot = j;
-                        }
-                    use |= 1 << slot;  // at most two other dims among three cluster dims
-                }
-                int mode = 1;
-                if ((e = nest_mode(p, dims_mask, kid, nk, ucount, &mode))) return e;
-                if (mode == 2 && single_q >= 0) {
-                    alias.push_back(mask);
-                    continue;
-                }
-                if (mode == 2) single_q = (int)fused.size();
-                wn.use[fused.size()] = use;
-                wn.mode[fused.size()] = mode;
-                fused.push_back(mask);
-            }
-            wn.nsub = (int)fused.size();
-            if (wn.nsub > 0) {
-                const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
-                const size_t need = (size_t)((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave));
-                // one round of resident workgroups
-                const unsigned grid =
-                    KT == 4 ? (X32 ? resident_grid(wave_nested_meat_kernel<4, float>, kBlock, 0, need)
-                                   : resident_grid(wave_nested_meat_kernel<4, double>, kBlock, 0, need))
-                    : KT == 8 ? (X32 ? resident_grid(wave_nested_meat_kernel<8, float>, kBlock, 0, need)
-                                     : resident_grid(wave_nested_meat_kernel<8, double>, kBlock, 0, need))
-                              : (X32 ? resident_grid(wave_nested_meat_kernel<12, float>, kBlock, 0, need)
-                                     : resident_grid(wave_nested_meat_kernel<12, double>, kBlock, 0, need));
-                const int NP = KT * (KT + 1) / 2;
-                double* npart = nullptr;
-                unsigned long long* ucounts = nullptr;
-                if ((e = db.get(&npart, (size_t)wn.nsub * grid * NP)) || (e = db.get(&ucounts, kMaxNestSubsets)))
-                    return e;
-                FEHIP_CHECK(hipMemsetAsync(ucounts, 0, sizeof(unsigned long long) * kMaxNestSubsets, 0), "memset");
-#define FEHIP_WNEST(KTV)                                                                                       \
-    if (X32)                                                                                                   \
-        hipLaunchKernelGGL((wave_nested_meat_kernel<KTV, float>), dim3(grid), dim3(kBlock), 0, 0, wn, X32, npart, \
-                           ucounts);                                                                           \
-    else                                                                                                       \
-        hipLaunchKernelGGL((wave_nested_meat_kernel<KTV, double>), dim3(grid), dim3(kBlock), 0, 0, wn, X, npart,  \
-                           ucounts);
-                if (KT == 4) {
-                    FEHIP_WNEST(4)
-                } else if (KT == 8) {
-                    FEHIP_WNEST(8)
+        // The fused pass over the wave tiles: the meats of the subsets it forms (a subset whose clusters are
+        // single rows repeats the meat of the first such subset), and the rss of a residual formed in it.
+        std::map<int, std::pair<std::vector<double>, int64_t>> pre;  // subset mask -> (meat, clusters)
+        if (plan.fused) {
+            const int KT = plan.KT;
+            const int NP = KT * (KT + 1) / 2;
+            WaveScores ws = plan.ws;
+            ws.r = resid_in_pass ? nullptr : r;
+            ws.y = resid_in_pass ? ycol : nullptr;
+            ws.beta = dbeta;
+            ws.K = K;
+            ws.idP = p->d[p->P].ids;
+            ws.wtile_rec = p->wtile_rec;
+            ws.n_wtiles = p->n_wtiles;
+            ws.w = wpanel;
+            const size_t lds = sizeof(double) * (size_t)ws.small_lds;
+            const size_t need = (size_t)((p->n_wtiles + kBlock / kWave - 1) / (kBlock / kWave));
+            unsigned grid = 1;
+            const int NS = ws.nsub <= 1 ? 1 : (ws.nsub <= 3 ? ws.nsub : 4);
+            double *npart = nullptr, *rssp = nullptr, *spart = nullptr;
+            unsigned long long* ucounts = nullptr;
+            // the kernel for the tile width, the column type and the nested-subset bound; `launch` false
+            // only sizes the grid (one round of resident workgroups)
+            auto scores_kernel = [&](bool launch) {
+#define FEHIP_SCORES_K(KTV, TXV, NSV, XV)                                                                   \
+    do {                                                                                                    \
+        if (!launch) {                                                                                      \
+            grid = resident_grid(wave_scores_kernel<KTV, TXV, NSV>, kBlock, lds, need);                    \
+        } else {                                                                                            \
+            hipLaunchKernelGGL((wave_scores_kernel<KTV, TXV, NSV>), dim3(grid), dim3(kBlock), lds, 0, ws, XV, \
+                               npart, ucounts);                                                             \
+        }                                                                                                   \
+    } while (0)
+#define FEHIP_SCORES_NS(KTV, TXV, XV)        \
+    if (NS == 1)                             \
+        FEHIP_SCORES_K(KTV, TXV, 1, XV);     \
+    else if (NS == 2)                        \
+        FEHIP_SCORES_K(KTV, TXV, 2, XV);     \
+    else if (NS == 3)                        \
+        FEHIP_SCORES_K(KTV, TXV, 3, XV);     \
+    else                                     \
+        FEHIP_SCORES_K(KTV, TXV, 4, XV);
+#define FEHIP_SCORES_KT(TXV, XV)             \
+    if (KT == 4) {                           \
+        FEHIP_SCORES_NS(4, TXV, XV)          \
+    } else if (KT == 8) {                    \
+        FEHIP_SCORES_NS(8, TXV, XV)          \
+    } else {                                 \
+        FEHIP_SCORES_NS(12, TXV, XV)         \
+    }
+                if (X32) {
+                    FEHIP_SCORES_KT(float, X32)
                 } else {
-                    FEHIP_WNEST(12)
+                    FEHIP_SCORES_KT(double, X)
                 }
-#undef FEHIP_WNEST
-                FEHIP_LAUNCH_CHECK("wave nested meat");
-                unsigned long long hc[kMaxNestSubsets] = {0, 0, 0, 0};
-                FEHIP_CHECK(copy_sync(hc, ucounts, sizeof(hc), hipMemcpyDeviceToHost), "count copy");
-                for (int q = 0; q < wn.nsub; ++q) {
-                    std::vector<double> Mq;
-                    if ((e = reduce_upper(npart + (size_t)q * grid * NP, (int)grid, KT, K, Mq))) return e;
-                    nest_pre[fused[q]] = std::make_pair(Mq, (int64_t)hc[q]);
-                }
-                for (int m2 : alias) nest_pre[m2] = nest_pre[fused[single_q]];
+#undef FEHIP_SCORES_KT
+#undef FEHIP_SCORES_NS
+#undef FEHIP_SCORES_K
+            };
+            scores_kernel(false);
+            if ((e = db.get(&npart, (size_t)std::max(1, ws.nsub) * grid * NP)) || (e = db.get(&ucounts, kMaxNestSubsets)) ||
+                (e = db.get(&rssp, grid)) || (e = db.get(&spart, (size_t)grid * ws.small_lds + ws.small_lds + 1)))
+                return e;
+            for (int j = 0; j < ws.nrun; ++j)
+                if ((e = db.get(&ws.runbuf[j], (size_t)p->d[plan.run_dim[j]].n_runs * KT + 1))) return e;
+            ws.small_part = spart;
+            ws.rss_part = rssp;
+            FEHIP_CHECK(hipMemsetAsync(ucounts, 0, sizeof(unsigned long long) * kMaxNestSubsets, 0), "memset");
+            scores_kernel(true);
+            FEHIP_LAUNCH_CHECK("wave scores");
+            if (resid_in_pass && !f32) {  // FP64 columns: rss of the residual formed in the pass
+                std::vector<double> ps;
+                if ((e = host_sum_parts(rssp, (int)grid, 1, ps))) return e;
+                result->rss = ps[0];
             }
+            unsigned long long hc[kMaxNestSubsets] = {0, 0, 0, 0};
+            if (ws.nsub > 0)
+                FEHIP_CHECK(copy_sync(hc, ucounts, sizeof(hc), hipMemcpyDeviceToHost), "count copy");
+            std::vector<double> hS;
+            if (ws.nsmall > 0) {
+                double* Sc = spart + (size_t)grid * ws.small_lds;
+                hipLaunchKernelGGL(sum_blocks_kernel, dim3((unsigned)ws.small_lds), dim3(kBlock), 0, 0,
+                                   (const double*)spart, (int)grid, ws.small_lds, Sc);
+                FEHIP_LAUNCH_CHECK("small scores");
+                hS.resize((size_t)ws.small_lds);
+                FEHIP_CHECK(copy_sync(hS.data(), Sc, hS.size() * sizeof(double), hipMemcpyDeviceToHost), "S copy");
+            }
+            for (int mask = 1; mask < (1 << ncd); ++mask) {
+                const int rt = plan.route[mask], ix = plan.index[mask];
+                if (rt == kRouteNest) {
+                    std::vector<double> Mq;
+                    if ((e = reduce_upper(npart + (size_t)ix * grid * NP, (int)grid, KT, K, Mq))) return e;
+                    pre[mask] = std::make_pair(Mq, (int64_t)hc[ix]);
+                } else if (rt == kRouteRun) {
+                    const DimStruct& D = p->d[plan.run_dim[ix]];
+                    double* Sg = nullptr;
+                    if ((e = db.get(&Sg, (size_t)D.G * KT))) return e;
+                    const size_t tot = (size_t)D.G * KT;
+                    if (KT == 4)
+                        hipLaunchKernelGGL(group_sum_kernel<4>, dim3(stream_grid(tot, kBlock)), dim3(kBlock), 0, 0,
+                                           (const double*)ws.runbuf[ix], (const int64_t*)D.grp_run_off, D.G, Sg);
+                    else if (KT == 8)
+                        hipLaunchKernelGGL(group_sum_kernel<8>, dim3(stream_grid(tot, kBlock)), dim3(kBlock), 0, 0,
+                                           (const double*)ws.runbuf[ix], (const int64_t*)D.grp_run_off, D.G, Sg);
+                    else
+                        hipLaunchKernelGGL(group_sum_kernel<12>, dim3(stream_grid(tot, kBlock)), dim3(kBlock), 0, 0,
+                                           (const double*)ws.runbuf[ix], (const int64_t*)D.grp_run_off, D.G, Sg);
+                    FEHIP_LAUNCH_CHECK("group sums");
+                    std::vector<double> Mr;
+                    if ((e = meat_fused(Sg, K, KT, nullptr, nullptr, D.G, mpart, Mr))) return e;
+                    pre[mask] = std::make_pair(Mr, (int64_t)Geff[plan.run_dim[ix]]);
+                } else if (rt == kRouteSmall) {
+                    std::vector<double> Ms((size_t)K * K, 0.0);
+                    int64_t nc = 0;
+                    const double* tab = hS.data() + ws.soff[ix];
+                    for (int q = 0; q < plan.small_cells[ix]; ++q) {
+                        const double* row = tab + (size_t)q * (KT + 1);
+                        if (row[KT] > 0.0) ++nc;
+                        for (int a2 = 0; a2 < K; ++a2)
+                            for (int b2 = 0; b2 < K; ++b2) Ms[(size_t)b2 * K + a2] += row[a2] * row[b2];
+                    }
+                    pre[mask] = std::make_pair(Ms, nc);
+                }
+            }
+            for (int mask = 1; mask < (1 << ncd); ++mask)
+                if (plan.route[mask] == kRouteAlias) pre[mask] = pre[plan.index[mask]];
         }
         for (int mask = 1; mask < (1 << ncd); ++mask) {
             unsigned dims_mask = 0;
@@ -1750,9 +2006,9 @@ int finish_request(Panel* p, ReqState& s) {
                 }
             const int single = len == 1 ? __builtin_ctz(dims_mask) : -1;
             int64_t ncl = 0;
-            if (nest_pre.count(mask)) {
-                M = nest_pre[mask].first;
-                ncl = nest_pre[mask].second;
+            if (pre.count(mask)) {
+                M = pre[mask].first;
+                ncl = pre[mask].second;
             } else if (has_p && p->n_long == 0 && K <= 12) {
                 if ((e = need_T())) return e;
                 NestKeys nk{};
