@@ -162,6 +162,18 @@ __device__ __forceinline__ void load_wave_rows(const ApplyArgs& a, int64_t rec, 
     }
 }
 
+// A table row of zeros: a row past the tile and a scattered dim past a.nd gather from it, so their
+// values need no masking afterwards.
+__device__ double g_zero_row[kMaxCT];
+
+// x of lane (addr / 4), with the byte address computed once per tile rather than per column
+__device__ __forceinline__ double lane_value(double x, int addr) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)b);
+    const int hi = __builtin_amdgcn_ds_bpermute(addr, (int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
 // Row values of a wave tile in two phases.  row_loads issues every load of the row without a branch:
 // the table entries of all its groups from the global table (small dims too: their few entries stay in
 // the L1 cache) and, for kData/kFinal, the input columns, from clamped addresses.  row_combine adds them
@@ -184,9 +196,10 @@ __device__ __forceinline__ void row_loads(const ApplyArgs& a, int64_t r, const i
         const double* tab = MODE == kIter ? a.p : a.alpha;
 #pragma unroll
         for (int o = 0; o < ND; ++o) {
-            const int64_t base = o < a.nd ? (a.sd[o].tab_off + (g[o] > 0 ? g[o] : 0)) * CT : 0;
+            // g < 0: a row past the tile or a dim past a.nd
+            const double* src = g[o] >= 0 ? tab + (a.sd[o].tab_off + g[o]) * CT : g_zero_row;
 #pragma unroll
-            for (int c = 0; c < CT; ++c) L.t[o][c] = ld_global(&tab[base + c]);
+            for (int c = 0; c < CT; ++c) L.t[o][c] = ld_global(src + c);
         }
     }
 }
@@ -197,10 +210,14 @@ __device__ __forceinline__ void row_combine(const ApplyArgs& a, const RowLoads<C
     for (int c = 0; c < CT; ++c) {
         double s = 0.0;
         if constexpr (MODE == kData || MODE == kFinal) s = c < a.ncols ? L.x[c] : 0.0;
-        if constexpr (MODE == kIter || MODE == kFinal) {
-            const double sign = MODE == kIter ? 1.0 : -1.0;
+        if constexpr (MODE == kIter) {
+            s = L.t[0][c];
 #pragma unroll
-            for (int o = 0; o < ND; ++o) s += (o < a.nd ? sign : 0.0) * L.t[o][c];
+            for (int o = 1; o < ND; ++o) s += L.t[o][c];  // dims past a.nd read the zero row
+        }
+        if constexpr (MODE == kFinal) {
+#pragma unroll
+            for (int o = 0; o < ND; ++o) s -= L.t[o][c];
         }
         v[c] = s;
     }
@@ -317,8 +334,12 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         // early exit here: a path without this tile's stores would make the wait for the prefetch drain
         double v[CT];
         row_combine<CT, MODE, ND>(a, ld, v);
+        // rows outside a column's subsample (and, for the data columns, past the tile) count as zero; an
+        // operator product without row masks has nothing to clear (its rows past the tile gathered zeros)
+        if (MODE != kIter || a.wm) {
 #pragma unroll
-        for (int c = 0; c < CT; ++c) v[c] = (on && (word & cb[c])) ? v[c] : 0.0;
+            for (int c = 0; c < CT; ++c) v[c] = (on && (word & cb[c])) ? v[c] : 0.0;
+        }
         // segment of this lane: [ss, se]
         const int ipp = __shfl_up(ip, 1, kWave);
         const bool hs = valid && (lane == 0 || ip != ipp);
@@ -353,7 +374,7 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
                         inslot = (word & cb[c]) ? 1.0 : 0.0;
                         cnt_bit = cb[c];
                     }
-                    const double mean = a.elim ? __shfl(x[c], se, kWave) * inv : 0.0;
+                    const double mean = a.elim ? lane_value(x[c], se << 2) * inv : 0.0;
                     u[c] = fma(-mean, inslot, v[c]);  // v is 0 outside the slot
                 }
             }
@@ -378,15 +399,16 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
             // x(run end) - x(run start - 1) - (rows of the slot in the run) * mean.
             double* tq[ND];
             double* dst[ND];
-            int pl[ND];
+            int pl_addr[ND];  // byte address of the lane before the run start (its prefix is subtracted)
+            const int se_addr = se << 2;
             double keep_pre[ND];
             unsigned long long runmask[ND];
             bool rend[ND];
 #pragma unroll
             for (int o = 0; o < ND; ++o) {
-                tq[o] = nullptr;
+                tq[o] = sacc;
                 dst[o] = nullptr;
-                pl[o] = 0;
+                pl_addr[o] = 0;
                 keep_pre[o] = 0.0;
                 runmask[o] = 0ull;
                 rend[o] = false;
@@ -404,7 +426,7 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
                 const int rs = 63 - __clzll(le);
                 rend[o] = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
                 const int rdest = rdl[o];  // the row's run destination, loaded with its ids
-                pl[o] = rs > 0 ? rs - 1 : 0;
+                pl_addr[o] = (rs > 0 ? rs - 1 : 0) << 2;
                 runmask[o] = mask_le(lane) & ~((1ull << rs) - 1ull);
                 keep_pre[o] = rs == ss ? 0.0 : 1.0;
                 dst[o] = d.runbuf + (int64_t)(rend[o] ? rdest : 0) * CT;
@@ -428,15 +450,15 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
 #pragma unroll
                     for (int o = 0; o < ND; ++o) rcnt[o] = (double)__popcll(B & runmask[o]);
                 }
-                const double mean = a.elim ? __shfl(x[c], se, kWave) * inv : 0.0;
-                const double u = fma(-mean, inslot, v[c]);  // v is 0 outside the slot
+                const double mean = a.elim ? lane_value(x[c], se_addr) * inv : 0.0;
+                const double u = fma(-mean, inslot, v[c]);  // 0 outside the slot and past the tile
 #pragma unroll
                 for (int o = 0; o < ND; ++o) {
                     if (o >= a.nd) break;
-                    if (tq[o]) {
-                        if (on && (word & cb[c])) lds_add(&tq[o][c], u);
+                    if (a.sd[o].small) {
+                        lds_add(&tq[o][c], u);  // lanes off the slot add zero
                     } else {
-                        const double pre = __shfl(x[c], pl[o], kWave);
+                        const double pre = lane_value(x[c], pl_addr[o]);
                         const double ru = fma(-rcnt[o], mean, fma(-pre, keep_pre[o], x[c]));
                         if (rend[o]) dst[o][c] = ru;
                     }
