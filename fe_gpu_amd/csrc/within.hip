@@ -304,6 +304,10 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     // records two tiles ahead, row data one tile ahead: loading the next tile's rows never waits for a
     // record requested in the same iteration (gathers a tile ahead as well cost a wave per SIMD of
     // occupancy and more than they hide)
+    // the segment scan serves the means (eliminated primary) and the run sums of the large dims
+    bool need_scan = a.elim != 0;
+    if constexpr (MODE != kFinal)
+        for (int o = 0; o < a.nd; ++o) need_scan |= !a.sd[o].small;
     WaveRows<ND> cur, nxt;
     load_wave_rows(a, load_wave_rec(a, t), lane, cur);
     int64_t rec_n = load_wave_rec(a, t + wstride);
@@ -350,13 +354,10 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         // One segmented scan per column gives the prefix x = sum of v over [segment start, lane]: its value
         // at the segment end is the segment total (for the mean), and differences of it are the run sums
         // of the large dims, so no second scan is needed.
-        bool any_large = false;
-        if constexpr (MODE != kFinal)
-            for (int o = 0; o < a.nd; ++o) any_large |= !a.sd[o].small;
         double x[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) x[c] = v[c];
-        if (a.elim || any_large) seg_scan_cols<CT>(x, ScanFactors(ScanMask(lane, ss)));
+        if (need_scan) seg_scan_cols<CT>(x, ScanFactors(ScanMask(lane, ss)));
         const unsigned long long segmask = mask_le(se) & ~((1ull << ss) - 1ull);
         if constexpr (MODE == kFinal) {
             double u[CT];
