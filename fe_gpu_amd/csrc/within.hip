@@ -457,7 +457,9 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
                 for (int o = 0; o < ND; ++o) {
                     if (o >= a.nd) break;
                     if (a.sd[o].small) {
-                        lds_add(&tq[o][c], u);  // lanes off the slot add zero
+                        // only the lanes in the column's subsample add (a masked-off lane costs no LDS
+                        // atomic bandwidth)
+                        if (word & cb[c]) lds_add(&tq[o][c], u);
                     } else {
                         const double pre = lane_value(x[c], pl_addr[o]);
                         const double ru = fma(-rcnt[o], mean, fma(-pre, keep_pre[o], x[c]));
