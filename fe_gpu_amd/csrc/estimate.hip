@@ -394,6 +394,8 @@ struct WaveScores {
     double* small_part;             // [grid][small_lds]
     double* rss_part;               // [grid]
     const uint8_t* w;               // panel-order weights (nullptr: all rows)
+    const int* spos;                // sorted position of each row in one subset's partition (nullptr: none)
+    double* tsort;                  // the score rows in that order (row-major n x K)
 };
 
 // Row data of one wave tile as loaded: the loads are issued one tile ahead of their use, without a
@@ -403,7 +405,7 @@ struct WaveScores {
 template <int KT, typename TX>
 struct ScoreRows {
     int64_t rec;
-    int ip, k[3], rd[kMaxRunOuts];
+    int ip, k[3], rd[kMaxRunOuts], sp;
     uint8_t w;
     double r;  // residual when read; with FP64 columns also y~ when the residual is formed
     TX y;      // FP32 columns: y~ when the residual is formed
@@ -427,6 +429,7 @@ __device__ __forceinline__ void load_score_rows(const WaveScores& a, const TX* c
 #pragma unroll
     for (int j = 0; j < kMaxRunOuts; ++j) q.rd[j] = (j < a.nrun ? a.rdest[j] : a.idP)[ri];
     q.w = (a.w ? a.w : reinterpret_cast<const uint8_t*>(a.idP))[ri];
+    q.sp = (a.spos ? a.spos : a.idP)[ri];
     if constexpr (std::is_same<TX, double>::value)
         q.r = (a.r ? a.r : yc)[ri];
     else if (a.r)
@@ -477,6 +480,7 @@ __global__ __launch_bounds__(kBlock) void wave_scores_kernel(WaveScores a, const
 #pragma unroll
         for (int j = 0; j < kMaxRunOuts; ++j) rdl[j] = cur.rd[j];
         const bool wrow = valid && (!a.w || cur.w);
+        const int64_t spr = cur.sp;
         double rr;
         if (a.r) {
             rr = cur.r;
@@ -500,6 +504,19 @@ __global__ __launch_bounds__(kBlock) void wave_scores_kernel(WaveScores a, const
         rec_n = rec_nn;
         cur = nxt;
         __builtin_amdgcn_sched_barrier(0);
+        // the score row at its place in the sorted partition of the subset that reads it in that order
+        if (a.spos && valid) {
+            double* dst = a.tsort + spr * a.K;
+            if (a.K == KT) {  // KT is even: 16-byte stores
+#pragma unroll
+                for (int k = 0; k < KT / 2; ++k)
+                    reinterpret_cast<double2*>(dst)[k] = double2{tv[2 * k], tv[2 * k + 1]};
+            } else {
+#pragma unroll
+                for (int k = 0; k < KT; ++k)
+                    if (k < a.K) dst[k] = tv[k];
+            }
+        }
         const int ipp = __shfl_up(ip, 1, kWave);
         int kp[3];
 #pragma unroll
@@ -1234,7 +1251,7 @@ int project_meat(const std::vector<double>& Mz, int m, const std::vector<double>
 // Which subsets of the sandwich the fused pass over the wave tiles forms, and how: the slot layout of
 // the other cluster dims and the pass's output tables.  The rest go to the methods that read the
 // residual or the score matrix.
-enum { kRouteOther = 0, kRouteNest = 1, kRouteRun = 2, kRouteSmall = 3, kRouteAlias = 4 };
+enum { kRouteOther = 0, kRouteNest = 1, kRouteRun = 2, kRouteSmall = 3, kRouteAlias = 4, kRouteSorted = 5 };
 
 struct ScorePlan {
     bool fused = false;   // a fused pass runs
@@ -1246,6 +1263,8 @@ struct ScorePlan {
     int small_cells[kMaxSmallOuts] = {};
     bool needs_r = false;  // a subset left to the other methods reads the residual or the score matrix
     bool reads_T = false;  // ... the score matrix
+    int sorted_mask = 0;   // the subset whose score rows the pass writes in its partition's order (0: none)
+    unsigned sorted_dims = 0;
 };
 
 int plan_scores(Panel* p, const int* cdims, int ncd, int Kt, unsigned long long* dcount, ScorePlan& pl) {
@@ -1327,6 +1346,13 @@ int plan_scores(Panel* p, const int* cdims, int ncd, int Kt, unsigned long long*
             pl.index[mask] = q;
             route = kRouteSmall;
         }
+        if (route == kRouteOther && fused_ok && !has_p && pl.sorted_mask == 0) {
+            // one subset through its sorted partition: the pass writes its score rows in that order, so the
+            // partition kernel reads them in sequence instead of gathering them
+            pl.sorted_mask = mask;
+            pl.sorted_dims = dm;
+            route = kRouteSorted;
+        }
         if (route == kRouteOther) {
             // the other methods: block-tile nested, small tables and run sums read the residual; the
             // block-tile nested, sorted-partition and segment-sum methods read the score matrix
@@ -1346,7 +1372,7 @@ int plan_scores(Panel* p, const int* cdims, int ncd, int Kt, unsigned long long*
     for (int j = 0; j < w.nrun; ++j)
         for (int q = 0; q < w.nsub; ++q)
             if (w.use[q] == (1 << w.rslot[j])) w.rshare[j] = q;
-    pl.fused = w.nsub > 0 || w.nrun > 0 || w.nsmall > 0;
+    pl.fused = w.nsub > 0 || w.nrun > 0 || w.nsmall > 0 || pl.sorted_mask != 0;
     return 0;
 }
 
@@ -1862,6 +1888,7 @@ int finish_request(Panel* p, ReqState& s) {
         // The fused pass over the wave tiles: the meats of the subsets it forms (a subset whose clusters are
         // single rows repeats the meat of the first such subset), and the rss of a residual formed in it.
         std::map<int, std::pair<std::vector<double>, int64_t>> pre;  // subset mask -> (meat, clusters)
+        double* tsort = nullptr;  // score rows of the sorted-route subset, in its partition's order
         if (plan.fused) {
             const int KT = plan.KT;
             const int NP = KT * (KT + 1) / 2;
@@ -1926,6 +1953,11 @@ int finish_request(Panel* p, ReqState& s) {
                 if ((e = db.get(&ws.runbuf[j], (size_t)p->d[plan.run_dim[j]].n_runs * KT + 1))) return e;
             ws.small_part = spart;
             ws.rss_part = rssp;
+            if (plan.sorted_mask) {
+                if ((e = partition_positions(p, plan.sorted_dims, &ws.spos, 0))) return e;
+                if ((e = db.get(&tsort, (size_t)n * K))) return e;
+                ws.tsort = tsort;
+            }
             FEHIP_CHECK(hipMemsetAsync(ucounts, 0, sizeof(unsigned long long) * kMaxNestSubsets, 0), "memset");
             scores_kernel(true);
             FEHIP_LAUNCH_CHECK("wave scores");
@@ -2006,10 +2038,12 @@ int finish_request(Panel* p, ReqState& s) {
                 }
             const int single = len == 1 ? __builtin_ctz(dims_mask) : -1;
             int64_t ncl = 0;
+            // the sorted-route subset reads the score rows the fused pass wrote in its partition's order
+            const bool in_order = plan.route[mask] == kRouteSorted;
             if (pre.count(mask)) {
                 M = pre[mask].first;
                 ncl = pre[mask].second;
-            } else if (has_p && p->n_long == 0 && K <= 12) {
+            } else if (!in_order && has_p && p->n_long == 0 && K <= 12) {
                 if ((e = need_T())) return e;
                 NestKeys nk{};
                 for (int d = 0; d < p->m; ++d)
@@ -2031,7 +2065,7 @@ int finish_request(Panel* p, ReqState& s) {
                 unsigned long long h = 0;
                 FEHIP_CHECK(copy_sync(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
                 ncl = (int64_t)h;
-            } else if (all_small && cells <= kSmallCells && K <= 11) {
+            } else if (!in_order && all_small && cells <= kSmallCells && K <= 11) {
                 SmallKeys sk{};
                 int stride = 1;
                 for (int d = 0; d < p->m; ++d)
@@ -2078,7 +2112,8 @@ int finish_request(Panel* p, ReqState& s) {
                         for (int b2 = 0; b2 < K; ++b2)
                             M[(size_t)b2 * K + a2] += hS[(size_t)q * KT + a2] * hS[(size_t)q * KT + b2];
                 }
-            } else if (single >= 0 && single != p->P && !p->d[single].small && p->n_long == 0 && K <= 12) {
+            } else if (!in_order && single >= 0 && single != p->P && !p->d[single].small && p->n_long == 0 &&
+                       K <= 12) {
                 const DimStruct& D = p->d[single];
                 const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
                 double *rb = nullptr, *Sg = nullptr;
@@ -2114,17 +2149,18 @@ int finish_request(Panel* p, ReqState& s) {
                 if ((e = meat_fused(Sg, K, KT, nullptr, nullptr, D.G, mpart, M))) return e;
                 ncl = Geff[single];
             } else {
-                if ((e = need_T())) return e;
+                if (!in_order && (e = need_T())) return e;
+                const double* Tm = in_order ? tsort : T;
                 const Partition* part_s = nullptr;
                 if ((e = panel_partition(p, dims_mask, &part_s, 0))) return e;
                 ncl = part_s->n_clusters;
                 const int64_t nS = part_s->n_clusters;
                 // a partition of singletons needs no row order: sum_i T_i T_i'
                 const bool singletons = nS == n;
-                const int* ord = singletons ? nullptr : part_s->order;
+                const int* ord = (singletons || in_order) ? nullptr : part_s->order;
                 const int64_t* off = singletons ? nullptr : part_s->off;
                 if (K <= 12 && singletons) {
-                    if ((e = meat_fused(T, K, K, nullptr, nullptr, n, mpart, M))) return e;
+                    if ((e = meat_fused(Tm, K, K, nullptr, nullptr, n, mpart, M))) return e;
                 } else if (K <= 12) {
                     const int KT = K <= 4 ? 4 : (K <= 8 ? 8 : 12);
                     const int64_t nwaves = (n + kWave - 1) / kWave;
@@ -2136,14 +2172,14 @@ int finish_request(Panel* p, ReqState& s) {
                     const unsigned grid = (unsigned)std::max<int64_t>(
                         1, std::min<int64_t>((nwaves + kBlock / kWave - 1) / (kBlock / kWave), (int64_t)kMeatBlocks));
                     if (KT == 4)
-                        hipLaunchKernelGGL(sorted_wave_meat_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, (const double*)T,
-                                           K, (const int*)part_s->order, (const int*)part_s->keys, n, mpart, bkey, bval);
+                        hipLaunchKernelGGL(sorted_wave_meat_kernel<4>, dim3(grid), dim3(kBlock), 0, 0, (const double*)Tm,
+                                           K, ord, (const int*)part_s->keys, n, mpart, bkey, bval);
                     else if (KT == 8)
-                        hipLaunchKernelGGL(sorted_wave_meat_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, (const double*)T,
-                                           K, (const int*)part_s->order, (const int*)part_s->keys, n, mpart, bkey, bval);
+                        hipLaunchKernelGGL(sorted_wave_meat_kernel<8>, dim3(grid), dim3(kBlock), 0, 0, (const double*)Tm,
+                                           K, ord, (const int*)part_s->keys, n, mpart, bkey, bval);
                     else
                         hipLaunchKernelGGL(sorted_wave_meat_kernel<12>, dim3(grid), dim3(kBlock), 0, 0,
-                                           (const double*)T, K, (const int*)part_s->order, (const int*)part_s->keys, n,
+                                           (const double*)Tm, K, ord, (const int*)part_s->keys, n,
                                            mpart, bkey, bval);
                     FEHIP_LAUNCH_CHECK("sorted wave meat");
                     if ((e = reduce_upper(mpart, (int)grid, KT, K, M))) return e;
@@ -2167,7 +2203,7 @@ int finish_request(Panel* p, ReqState& s) {
                         if ((e = db.get(&S, (size_t)S_cap))) return e;
                     }
                     hipLaunchKernelGGL(segsum_kernel, dim3(stream_grid((size_t)nS * K, kBlock)), dim3(kBlock), 0, 0,
-                                       (const double*)T, K, ord, off, nS, S);
+                                       (const double*)Tm, K, ord, off, nS, S);
                     FEHIP_LAUNCH_CHECK("segsum");
                     if ((e = gemm_impl(false, true, K, K, nS, 1.0, S, K, S, K, 0.0, Md, K, true))) return e;
                     FEHIP_CHECK(copy_sync(M.data(), Md, M.size() * sizeof(double), hipMemcpyDeviceToHost), "meat copy");

@@ -211,6 +211,7 @@ void free_panel(Panel* p) {
     for (auto& kv : p->partitions) {
         Partition& q = kv.second;
         dfree(q.order);
+        dfree(q.pos);
         if (q.off != p->seg_off) dfree(q.off);
         if (q.keys != p->d[p->P].ids) dfree(q.keys);
     }
@@ -463,6 +464,26 @@ int panel_partition(Panel* p, unsigned mask, const Partition** out, hipStream_t 
     }
     auto res = p->partitions.emplace(mask, part);
     *out = &res.first->second;
+    return 0;
+}
+
+int partition_positions(Panel* p, unsigned mask, const int** pos, hipStream_t s) {
+    std::lock_guard<std::recursive_mutex> guard(p->mu);
+    const Partition* part = nullptr;
+    int st = panel_partition(p, mask, &part, s);
+    if (st) return st;
+    Partition& q = p->partitions.find(mask)->second;
+    if (!q.order) {
+        *pos = nullptr;
+        return 0;
+    }
+    if (!q.pos) {
+        if ((st = dalloc(&q.pos, (size_t)p->n, &p->bytes))) return st;
+        hipLaunchKernelGGL(invert_order_kernel, dim3(stream_grid((size_t)p->n, kBlock)), dim3(kBlock), 0, s,
+                           (const int*)q.order, p->n, q.pos);
+        FEHIP_LAUNCH_CHECK("partition positions");
+    }
+    *pos = q.pos;
     return 0;
 }
 

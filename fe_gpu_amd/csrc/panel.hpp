@@ -39,6 +39,7 @@ struct Partition {
     int* order = nullptr;      // nullptr => identity (panel order)
     int64_t* off = nullptr;    // [n_clusters + 1]
     int* keys = nullptr;       // cluster of each sorted position (grouped, nondecreasing)
+    int* pos = nullptr;        // sorted position of each row (inverse of order), built on first use
 };
 
 struct Panel {
@@ -77,6 +78,8 @@ struct Panel {
 };
 
 int panel_partition(Panel* p, unsigned dims_mask, const Partition** out, hipStream_t s);
+// The partition's sorted position of every row (nullptr for a partition in panel order), cached.
+int partition_positions(Panel* p, unsigned dims_mask, const int** pos, hipStream_t s);
 
 // gemm on the default stream (linalg.hip); C = alpha op(A) op(B) + beta C
 int gemm_impl(bool tA, bool tB, int m, int n, long long k, double alpha, const double* A, long long ldA,
