@@ -55,6 +55,12 @@ class ServerStats(C.Structure):
                 ("p50_s", C.c_double), ("p99_s", C.c_double), ("max_s", C.c_double)]
 
 
+class BinHeader(C.Structure):
+    _fields_ = [("n_obs", I64), ("n_regressors", I32), ("n_instruments", I32), ("n_fe", I32), ("has_cluster", I32),
+                ("has_weights", I32), ("precision", I32), ("new_format", I32), ("has_precision_flag", I32),
+                ("data_offset", I64), ("metadata_bytes", I64)]
+
+
 COARSE = [
     ("fehip_panel_create", C.c_int, [C.POINTER(VP), C.c_int, I64, C.POINTER(VP)]),
     ("fehip_panel_destroy", C.c_int, [VP]),
@@ -75,6 +81,8 @@ COARSE = [
     ("fehip_kstats_reset", C.c_int, []),
     ("fehip_kstats_get", C.c_int, [C.c_int, C.c_char_p, C.c_int, C.POINTER(I64), DP, DP, DP]),
     ("fehip_device_copy", C.c_int, [VP, VP, I64]),
+    ("fehip_bin_read_header", C.c_int, [C.c_char_p, C.POINTER(BinHeader)]),
+    ("fehip_bin_load", C.c_int, [C.c_char_p, C.POINTER(BinHeader), VP, VP, VP, VP, VP, VP]),
 ]
 
 

@@ -222,3 +222,55 @@ def estimate(y, W, fe, Z=None, weights=None, panel: Panel = None, **kw) -> Fit:
     if panel is None:
         pan.close()
     return fit
+
+
+# ---- `.bin` dataset files (src/io/fe_data_io.f90) straight into device memory ----
+
+def bin_header(path) -> dict:
+    """fehip_bin_read_header: the header of a reference `.bin` file (layout and precision decided from
+    the file size as load_dataset_from_file does, src/io/fe_data_io.f90:78-202)."""
+    h = S.BinHeader()
+    check(load().fehip_bin_read_header(str(path).encode(), C.byref(h)), "fehip_bin_read_header")
+    return {name: getattr(h, name) for name, _ in S.BinHeader._fields_}
+
+
+@dataclass
+class BinData:
+    """The blocks of a `.bin` file in device memory (values in FP64, ids int32; None for absent blocks)."""
+    header: dict
+    y: DeviceBuffer
+    W: DeviceBuffer | None
+    Z: DeviceBuffer | None
+    fe: DeviceBuffer | None
+    cluster: DeviceBuffer | None
+    weights: DeviceBuffer | None
+
+    def fe_ptrs(self) -> list:
+        n = self.header["n_obs"]
+        return [self.fe.ptr + 4 * n * d for d in range(self.header["n_fe"])]
+
+    def panel(self) -> Panel:
+        return Panel(self.fe_ptrs(), self.header["n_obs"])
+
+    def free(self):
+        for b in (self.y, self.W, self.Z, self.fe, self.cluster, self.weights):
+            if b is not None:
+                b.free()
+
+
+def load_bin(path) -> BinData:
+    """fehip_bin_load: stream every block of a `.bin` file into device buffers of its own."""
+    h = S.BinHeader()
+    lib = load()
+    check(lib.fehip_bin_read_header(str(path).encode(), C.byref(h)), "fehip_bin_read_header")
+    n = h.n_obs
+    y = DeviceBuffer(8 * n)
+    W = DeviceBuffer(8 * n * h.n_regressors) if h.n_regressors > 0 else None
+    Z = DeviceBuffer(8 * n * h.n_instruments) if h.n_instruments > 0 else None
+    fe = DeviceBuffer(4 * n * h.n_fe) if h.n_fe > 0 else None
+    cl = DeviceBuffer(4 * n) if h.has_cluster else None
+    w = DeviceBuffer(8 * n) if h.has_weights else None
+    ptr = lambda b: b.ptr if b is not None else None  # noqa: E731
+    check(lib.fehip_bin_load(str(path).encode(), C.byref(h), y.ptr, ptr(W), ptr(Z), ptr(fe), ptr(cl), ptr(w)),
+          "fehip_bin_load")
+    return BinData({name: getattr(h, name) for name, _ in S.BinHeader._fields_}, y, W, Z, fe, cl, w)

@@ -167,6 +167,34 @@ int fehip_server_reset_stats(fehip_server* server);
 /* Finishes every queued request, then stops the worker. */
 int fehip_server_destroy(fehip_server* server);
 
+/* ---- `.bin` dataset files (the reference's format) straight into device memory ----
+ * Replaces load_dataset_from_file (src/io/fe_data_io.f90:23-66, header rules :78-202, size formula
+ * :342-358) followed by the upload of src/gpu/fe_gpu_data.f90:41-120.  The header layout (with or
+ * without L) and the value precision are decided from the file size exactly as the reference does;
+ * metadata_bytes counts the trailing META block (names; not read here). */
+typedef struct {
+    int64_t n_obs;
+    int32_t n_regressors;       /* K */
+    int32_t n_instruments;      /* L (0 in the old layout) */
+    int32_t n_fe;
+    int32_t has_cluster;
+    int32_t has_weights;
+    int32_t precision;          /* 0 = FP64 values, 1 = FP32 values */
+    int32_t new_format;         /* 1: 28-byte header with L; 0: 24-byte header without it */
+    int32_t has_precision_flag; /* an int32 flag word follows the header */
+    int64_t data_offset;        /* byte offset of y */
+    int64_t metadata_bytes;     /* bytes after the data blocks */
+} fehip_bin_header;
+
+int fehip_bin_read_header(const char* path, fehip_bin_header* out);
+/* Streams the blocks into device buffers: d_y (n), d_W (n x K, col-major), d_Z (n x L), d_fe_ids
+ * (n_fe x n int32, dim-major), d_cluster (n int32), d_weights (n).  Values land in FP64 (an FP32 file
+ * is widened on the device, as the reference widens it on the host).  A null destination skips its
+ * block.  The file is read in 64 MiB chunks into two pinned buffers, each chunk copied while the
+ * next one is read. */
+int fehip_bin_load(const char* path, const fehip_bin_header* h, double* d_y, double* d_W, double* d_Z,
+                   int32_t* d_fe_ids, int32_t* d_cluster, double* d_weights);
+
 /* Deterministic synthetic worker-firm-time panel generated on the device (the 150M-row benchmark
  * panel does not fit a host round trip).  Rows are ordered by worker then period.  Writes
  * d_fe (3 x n int32, dim-major: worker, firm, period), d_y (n) and d_W (n x 8, col-major). */

@@ -63,7 +63,7 @@ def test_ctypes_structs_match_the_c_compiler(tmp_path):
     structs = {"fehip_panel_info": S.PanelInfo, "fehip_demean_opts": S.DemeanOpts,
                "fehip_demean_stats": S.DemeanStats, "fehip_opts": S.Opts, "fehip_result": S.Result,
                "fehip_request": S.Request, "fehip_server_opts": S.ServerOpts,
-               "fehip_server_stats": S.ServerStats}
+               "fehip_server_stats": S.ServerStats, "fehip_bin_header": S.BinHeader}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "fehip.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'printf("{cname} sizeof %zu\\n", sizeof({cname}));')
