@@ -134,6 +134,53 @@ def serving_panel_leg(args, lib, pan, d_y, d_W, n):
     }
 
 
+def ingest_leg(args, lib):
+    """`.bin` -> HBM ingest (fehip_bin_load): a worker-firm-time file of args.ingest_rows rows (y and 8
+    regressors in FP64, 3 FE id columns: 84 bytes per row, the reference's format with its precision
+    flag) written to local disk once, then streamed into device buffers from the page cache.  Timed
+    beside it, the reference's shape of the same step: the whole file read into host arrays, then
+    uploaded from pageable memory."""
+    import struct
+    import tempfile
+    from fe_gpu_amd import api
+    from fe_gpu_amd._lib import check
+    from fe_gpu_amd.device import DeviceBuffer
+    T = 30
+    nw = max(1, args.ingest_rows // T)
+    n = nw * T
+    d_fe, d_y, d_W = DeviceBuffer(3 * n * 4), DeviceBuffer(n * 8), DeviceBuffer(8 * n * 8)
+    check(lib.fehip_synth_worker_firm(nw, T, max(1, nw // 5), 0.10, 909, d_fe.ptr, d_y.ptr, d_W.ptr), "synth")
+    fd, path = tempfile.mkstemp(suffix=".bin")
+    try:
+        with os.fdopen(fd, "wb") as f:
+            f.write(struct.pack("<qiiiiii", n, 8, 0, 3, 0, 0, 0))  # header (new layout) + FP64 flag
+            d_y.to_array(np.float64, n).tofile(f)
+            d_W.to_array(np.float64, 8 * n).tofile(f)
+            d_fe.to_array(np.int32, 3 * n).tofile(f)
+        d_fe.free(), d_y.free(), d_W.free()
+        size = os.path.getsize(path)
+        api.load_bin(path).free()  # page cache warm, buffers sized once
+        t = time.perf_counter()
+        d = api.load_bin(path)
+        dt = time.perf_counter() - t
+        d.free()
+        t = time.perf_counter()
+        with open(path, "rb") as f:
+            f.seek(32)
+            host = [np.fromfile(f, np.float64, n), np.fromfile(f, np.float64, 8 * n), np.fromfile(f, np.int32, 3 * n)]
+        bufs = [DeviceBuffer.from_array(a) for a in host]
+        dt_ref = time.perf_counter() - t
+        for b in bufs:
+            b.free()
+    finally:
+        os.unlink(path)
+    log(f"ingest: {size / 1e9:.2f} GB in {dt * 1e3:.0f} ms ({size / dt / 1e9:.1f} GB/s); host arrays + upload "
+        f"{dt_ref * 1e3:.0f} ms")
+    return {"workload": f".bin file of {n} rows (y, 8 FP64 regressors, 3 FE id columns), page cache warm",
+            "bytes": size, "seconds": dt, "GBps": size / dt / 1e9,
+            "host_arrays_then_upload": {"seconds": dt_ref, "GBps": size / dt_ref / 1e9}}
+
+
 def iv_wide_leg(args, lib):
     """Config 3: 2SLS on a wide expanded design -- 1M individuals x 20 periods (20M rows), 50k firms,
     20 periods; K = 252 columns (xend and 7 cat8 levels endogenous, x1..x4, i.ind&&i.msp = 239
@@ -291,6 +338,8 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-iv-wide", action="store_true")
     ap.add_argument("--no-serve-panel", action="store_true")
+    ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--ingest-rows", type=int, default=10_000_000)
     ap.add_argument("--serve-panel-requests", type=int, default=32)
     ap.add_argument("--server-workers", type=int, default=2, help="batches in flight at once")
     ap.add_argument("--server-max-batch", type=int, default=0, help="requests per batch (0: 32)")
@@ -421,6 +470,10 @@ def main():
     if not args.no_serving:
         log("serving leg")
         serving = serving_leg(args, lib)
+    ingest = None
+    if not args.no_ingest:
+        log("ingest leg")
+        ingest = ingest_leg(args, lib)
     lat = np.array(step_s) * 1e3
     out = {
         "metric": "FE-regressions/sec (150M-row 3-FE OLS, 3-way clustered SEs, FP64)",
@@ -457,6 +510,7 @@ def main():
         "serving_150M": serve_panel,
         "iv_wide": iv_wide,
         "serving": serving,
+        "ingest": ingest,
     }
     print(json.dumps(out), flush=True)
 

@@ -8,14 +8,15 @@
 //   blocks: y, W (N x K column-major), Z (N x L), FE ids (n_fe x N, dim-major), cluster, weights (:48-62).
 // The reference reads every block into host arrays, widens FP32 values to FP64 on the host (:243-299) and
 // uploads afterwards (src/gpu/fe_gpu_data.f90:41-120).  Here the file is read in chunks into two pinned
-// host buffers while the previous chunk is copied to the device on its own stream, and FP32 values are
-// widened on the device, so the host never holds the data set.
+// host buffers (by several reader threads) while the previous chunk is copied to the device on its own
+// stream, and FP32 values are widened on the device, so the host never holds the data set.
 #include <fcntl.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 
 #include "../../include/fehip.h"
 #include "common.hpp"
@@ -35,6 +36,28 @@ int64_t data_bytes(int64_t n, int32_t K, int32_t L, int32_t n_fe, bool cluster, 
     t += (int64_t)n_fe * n * 4;
     if (cluster) t += n * 4;
     return t;
+}
+
+bool read_exact(int fd, void* dst, size_t bytes, int64_t off);
+
+// a chunk read by several threads at once (reads from the page cache are bound by one core's copy rate)
+constexpr int kReaders = 4;
+bool read_parallel(int fd, void* dst, size_t bytes, int64_t off) {
+    if (bytes < ((size_t)4 << 20)) return read_exact(fd, dst, bytes, off);
+    const size_t part = (bytes / kReaders + 4095) & ~(size_t)4095;
+    bool ok[kReaders];
+    std::thread th[kReaders];
+    for (int i = 0; i < kReaders; ++i) {
+        const size_t b0 = std::min(bytes, part * (size_t)i), b1 = std::min(bytes, part * (size_t)(i + 1));
+        ok[i] = true;
+        th[i] = std::thread([&, i, b0, b1] { ok[i] = read_exact(fd, static_cast<char*>(dst) + b0, b1 - b0, off + (int64_t)b0); });
+    }
+    bool all = true;
+    for (int i = 0; i < kReaders; ++i) {
+        th[i].join();
+        all = all && ok[i];
+    }
+    return all;
 }
 
 bool read_exact(int fd, void* dst, size_t bytes, int64_t off) {
@@ -192,7 +215,7 @@ extern "C" int fehip_bin_load(const char* path, const fehip_bin_header* h, doubl
             const int64_t len = std::min<int64_t>((int64_t)kChunk, b.bytes - done);
             // the pinned buffer (and its device stage) is free once its previous copy has finished
             FEHIP_CHECK(hipEventSynchronize(r.done[slot]), "ingest wait");
-            if (!read_exact(f.fd, r.host[slot], (size_t)len, off + done))
+            if (!read_parallel(f.fd, r.host[slot], (size_t)len, off + done))
                 return set_error("reading %s at byte %lld", path, (long long)(off + done));
             if (b.real && vb == 4) {
                 FEHIP_CHECK(hipMemcpyAsync(r.stage[slot], r.host[slot], (size_t)len, hipMemcpyHostToDevice, s), "ingest copy");
