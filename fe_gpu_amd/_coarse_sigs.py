@@ -81,6 +81,8 @@ COARSE = [
     ("fehip_kstats_reset", C.c_int, []),
     ("fehip_kstats_get", C.c_int, [C.c_int, C.c_char_p, C.c_int, C.POINTER(I64), DP, DP, DP]),
     ("fehip_device_copy", C.c_int, [VP, VP, I64]),
+    ("fehip_fe_values", C.c_int, [VP, VP, VP, C.c_int, I64, DP, C.POINTER(DemeanOpts), VP, VP,
+                                  C.POINTER(DemeanStats)]),
     ("fehip_bin_read_header", C.c_int, [C.c_char_p, C.POINTER(BinHeader)]),
     ("fehip_bin_load", C.c_int, [C.c_char_p, C.POINTER(BinHeader), VP, VP, VP, VP, VP, VP]),
 ]

@@ -224,6 +224,32 @@ def estimate(y, W, fe, Z=None, weights=None, panel: Panel = None, **kw) -> Fit:
     return fit
 
 
+def fe_values(panel: Panel, y_ptr: int, W_ptr: int, K: int, beta, ld: int = None, **kw):
+    """fehip_fe_values: the fixed effects of the fit y ~ W beta, per absorbed dim (ascending), as the
+    reference's --save-fe reconstruction defines them (src/core/fe_pipeline.f90:1619-1715).  Returns
+    [(values, counts)] per dim (level id - 1 indexes both), and the solve's statistics."""
+    lib = load()
+    info = panel.info()
+    fm = kw.get("fe_mask", 0) or ((1 << info["n_fe"]) - 1)
+    dims = [d for d in range(info["n_fe"]) if fm >> d & 1]
+    Gs = [info["group_sizes"][d] for d in dims]
+    tot = sum(Gs)
+    vals, cnts = DeviceBuffer(8 * max(1, tot)), DeviceBuffer(4 * max(1, tot))
+    o = demean_opts(**kw)
+    st = S.DemeanStats()
+    b = np.ascontiguousarray(beta, dtype=np.float64) if K else np.zeros(1)
+    check(lib.fehip_fe_values(panel.handle, y_ptr, W_ptr or None, K, ld or panel.n_obs, _dp(b), C.byref(o), vals.ptr,
+                              cnts.ptr, C.byref(st)), "fehip_fe_values")
+    v = vals.to_array(np.float64, tot)
+    c = cnts.to_array(np.int32, tot)
+    vals.free(), cnts.free()
+    out, off = [], 0
+    for g in Gs:
+        out.append((v[off:off + g], c[off:off + g]))
+        off += g
+    return out, dict(iterations=st.iterations, converged=st.converged, max_update=st.max_update)
+
+
 # ---- `.bin` dataset files (src/io/fe_data_io.f90) straight into device memory ----
 
 def bin_header(path) -> dict:

@@ -167,6 +167,16 @@ int fehip_server_reset_stats(fehip_server* server);
 /* Finishes every queued request, then stops the worker. */
 int fehip_server_destroy(fehip_server* server);
 
+/* The estimated fixed effects of a fitted regression: replaces reconstruct_and_save_fixed_effects
+ * (src/core/fe_pipeline.f90:1619-1715), whose alternating projection over r = y - W beta converges to
+ * per-dim values that each have row-weighted mean zero; here the FE-space solve of r gives the same
+ * fixed point.  For every dim in opts->fe_mask (0 = all), in ascending dim order, G_d entries are
+ * written at the running offset: d_values (level id - 1 indexes the dim's block; 0 for levels without
+ * rows) and d_counts (rows per level; the reference's output files list the levels with rows).  beta is
+ * host memory (K doubles); opts->tol and opts->max_iter bound the solve. */
+int fehip_fe_values(fehip_panel* panel, const double* d_y, const double* d_W, int K, int64_t ld, const double* beta,
+                    const fehip_demean_opts* opts, double* d_values, int32_t* d_counts, fehip_demean_stats* stats);
+
 /* ---- `.bin` dataset files (the reference's format) straight into device memory ----
  * Replaces load_dataset_from_file (src/io/fe_data_io.f90:23-66, header rules :78-202, size formula
  * :342-358) followed by the upload of src/gpu/fe_gpu_data.f90:41-120.  The header layout (with or

@@ -227,3 +227,38 @@ def estimate(y, W, fe, G=None, Z=None, cluster=(), iv_reg=(), iv_inst=(), tol=1e
     d["cluster_dims"] = list(r.cluster_dims)[: r.n_cluster_dims]
     d["n_clusters"] = list(r.n_clusters)
     return OracleFit(beta, se, np.array(vcov), d, y, Wf, Zf)
+
+
+def reconstruct_fe(y, W, beta, fe, tol=1e-8, max_iter=10000):
+    """reconstruct_and_save_fixed_effects (src/core/fe_pipeline.f90:1619-1715), restated in numpy:
+    r = y - W beta (:1653-1659); level counts over every row (:1666-1673); then forward alternating
+    projection over the dims: group means of the current residual (:1680-1689), centred by their
+    row-weighted mean (:1691-1704), added to the dim's values and subtracted from the residual
+    (:1706-1717), until the largest update is below tol (:1718).  Returns [(values, counts)] per dim
+    (level id - 1 indexes both; levels without rows have value 0) and the iteration count."""
+    fe = np.asarray(fe)
+    r = np.asarray(y, dtype=np.float64).copy()
+    if W is not None and len(beta):
+        r -= np.asarray(W, dtype=np.float64) @ np.asarray(beta, dtype=np.float64)
+    m = fe.shape[0]
+    G = [int(fe[d].max()) for d in range(m)]
+    idx = [fe[d].astype(np.int64) - 1 for d in range(m)]
+    cnt = [np.bincount(idx[d], minlength=G[d]) for d in range(m)]
+    val = [np.zeros(G[d]) for d in range(m)]
+    it = 0
+    for it in range(1, max_iter + 1):
+        mx = 0.0
+        for d in range(m):
+            s = np.bincount(idx[d], weights=r, minlength=G[d])
+            has = cnt[d] > 0
+            mean = np.where(has, s / np.maximum(cnt[d], 1), 0.0)
+            tot = cnt[d].sum()
+            center = (mean * cnt[d]).sum() / tot if tot > 0 else 0.0
+            mean = np.where(has, mean - center, 0.0)
+            val[d] += mean
+            if has.any():
+                mx = max(mx, float(np.abs(mean[has]).max()))
+            r -= mean[idx[d]]
+        if mx < tol:
+            break
+    return [(val[d], cnt[d]) for d in range(m)], it
