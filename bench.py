@@ -416,6 +416,17 @@ def main():
     kernels = {v["name"]: {"launches": v["launches"], "avg_ms": v["ms"] / max(1, v["launches"]),
                            "GBps": (v["bytes"] / max(1, v["launches"])) / (v["ms"] / max(1, v["launches"]) * 1e-3) / 1e9
                            if v["ms"] > 0 else 0.0} for v in ks.values()}
+    # the fixed effects of the fit (the reference's --save-fe reconstruction) on the same panel
+    fe_rec = None
+    if fit is not None:
+        t = time.perf_counter()
+        fev, fest = api.fe_values(pan, d_y.ptr, d_W.ptr, 8, fit.beta, tol=args.tol, max_iter=2000)
+        fe_values_s = time.perf_counter() - t
+        levels = [int((c > 0).sum()) for _, c in fev]
+        log(f"fe values: {fe_values_s:.3f} s, {fest['iterations']} iterations, levels {levels}")
+        fe_rec = {"workload": "fixed effects of the headline fit per dim (worker, firm, period), each centred by its "
+                              "row-weighted mean, tol " + str(args.tol),
+                  "seconds": fe_values_s, "fe_iterations": fest["iterations"], "levels": levels}
     mixed = None
     if not args.no_mixed:
         # config 4: the same panel, columns stored in FP32, cross products (and beta) in FP64
@@ -511,6 +522,7 @@ def main():
         "iv_wide": iv_wide,
         "serving": serving,
         "ingest": ingest,
+        "fe_values": fe_rec,
     }
     print(json.dumps(out), flush=True)
 
