@@ -61,6 +61,10 @@ class BinHeader(C.Structure):
                 ("data_offset", I64), ("metadata_bytes", I64)]
 
 
+class DesignColumn(C.Structure):
+    _fields_ = [("n_factors", I32), ("col", I32 * 4), ("level", I32 * 4), ("categorical", I32 * 4)]
+
+
 COARSE = [
     ("fehip_panel_create", C.c_int, [C.POINTER(VP), C.c_int, I64, C.POINTER(VP)]),
     ("fehip_panel_destroy", C.c_int, [VP]),
@@ -83,6 +87,8 @@ COARSE = [
     ("fehip_device_copy", C.c_int, [VP, VP, I64]),
     ("fehip_fe_values", C.c_int, [VP, VP, VP, C.c_int, I64, DP, C.POINTER(DemeanOpts), VP, VP,
                                   C.POINTER(DemeanStats)]),
+    ("fehip_design_levels", C.c_int, [VP, I64, C.POINTER(I32), C.POINTER(I32), C.POINTER(I32)]),
+    ("fehip_design_expand", C.c_int, [VP, I64, C.c_int, I64, C.POINTER(DesignColumn), C.c_int, VP, I64]),
     ("fehip_bin_read_header", C.c_int, [C.c_char_p, C.POINTER(BinHeader)]),
     ("fehip_bin_load", C.c_int, [C.c_char_p, C.POINTER(BinHeader), VP, VP, VP, VP, VP, VP]),
 ]

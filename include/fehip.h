@@ -177,6 +177,28 @@ int fehip_server_destroy(fehip_server* server);
 int fehip_fe_values(fehip_panel* panel, const double* d_y, const double* d_W, int K, int64_t ld, const double* beta,
                     const fehip_demean_opts* opts, double* d_values, int32_t* d_counts, fehip_demean_stats* stats);
 
+/* ---- formula design expansion on the device (replaces build_formula_design_matrix,
+ * src/src_main.f90:595-736, which builds the expanded design on the host) ----
+ * fehip_design_levels: the levels of a categorical column as collect_levels_for_column (:1389-1432)
+ * finds them: the first 40 distinct values within 1e-6 of an integer, in row order, sorted ascending;
+ * levels receives up to 40 values (host memory), truncated = 1 when the list filled up (the reference
+ * warns then).  fehip_design_expand: output column j = product over its factors of (categorical:
+ * nint(x) == level ? 1 : 0; continuous: x), x read from raw column col of d_raw (n_raw columns,
+ * leading dim ld_raw); the caller lists the columns in the reference's order (a categorical term: its
+ * levels but the base, ascending; an interaction: the factors' blocks multiplied out, the first factor
+ * outermost). */
+#define FEHIP_DESIGN_MAX_FACTORS 4
+typedef struct {
+    int32_t n_factors;
+    int32_t col[FEHIP_DESIGN_MAX_FACTORS];
+    int32_t level[FEHIP_DESIGN_MAX_FACTORS];
+    int32_t categorical[FEHIP_DESIGN_MAX_FACTORS];
+} fehip_design_column;
+
+int fehip_design_levels(const double* d_col, int64_t n, int32_t* levels, int32_t* n_levels, int32_t* truncated);
+int fehip_design_expand(const double* d_raw, int64_t ld_raw, int n_raw, int64_t n, const fehip_design_column* cols,
+                        int n_cols, double* d_out, int64_t ld_out);
+
 /* ---- `.bin` dataset files (the reference's format) straight into device memory ----
  * Replaces load_dataset_from_file (src/io/fe_data_io.f90:23-66, header rules :78-202, size formula
  * :342-358) followed by the upload of src/gpu/fe_gpu_data.f90:41-120.  The header layout (with or
