@@ -181,6 +181,41 @@ def ingest_leg(args, lib):
             "host_arrays_then_upload": {"seconds": dt_ref, "GBps": size / dt_ref / 1e9}}
 
 
+def design_leg(args, lib):
+    """Device design expansion of the wide 2SLS formula's factor part from raw codes: 20M rows, cat8
+    (8 levels), ind (30), msp (8), x1, x2 -> i.cat8 + i.ind&&i.msp + x1&x2 = 7 + 29 x 7 + 1 = 211 FP64
+    columns (the reference builds these on the host before its upload)."""
+    from fe_gpu_amd import design
+    from fe_gpu_amd.device import DeviceBuffer
+    n = args.iv_individuals * 20
+    rng = np.random.default_rng(404)
+    raw = np.empty((n, 5), order="F")
+    raw[:, 0] = rng.integers(1, 9, n)
+    raw[:, 1] = rng.integers(1, 31, n)
+    raw[:, 2] = rng.integers(1, 9, n)
+    raw[:, 3] = rng.normal(size=n)
+    raw[:, 4] = rng.normal(size=n)
+    draw = DeviceBuffer.from_array(raw)
+    del raw
+    t = time.perf_counter()
+    levels = {c: design.column_levels(draw.ptr + 8 * n * c, n)[0] for c in (0, 1, 2)}
+    t_levels = time.perf_counter() - t
+    F = design.Factor
+    cols, names = design.plan([F("cat8", 0, True)], [[F("ind", 1, True), F("msp", 2, True)],
+                                                     [F("x1", 3), F("x2", 4)]], levels)
+    design.expand(draw.ptr, n, 5, n, cols[:1]).free()  # first launch outside the timing
+    t = time.perf_counter()
+    out = design.expand(draw.ptr, n, 5, n, cols)
+    dt = time.perf_counter() - t
+    out.free()
+    draw.free()
+    written = 8.0 * n * len(cols)
+    log(f"design expansion: {len(cols)} columns x {n} rows in {dt * 1e3:.1f} ms ({written / dt / 1e9:.0f} GB/s written), "
+        f"levels {t_levels * 1e3:.1f} ms")
+    return {"workload": f"i.cat8 + i.ind&&i.msp + x1&x2 over {n} rows -> {len(cols)} FP64 columns",
+            "columns": len(cols), "seconds": dt, "GBps_written": written / dt / 1e9, "levels_seconds": t_levels}
+
+
 def iv_wide_leg(args, lib):
     """Config 3: 2SLS on a wide expanded design -- 1M individuals x 20 periods (20M rows), 50k firms,
     20 periods; K = 252 columns (xend and 7 cat8 levels endogenous, x1..x4, i.ind&&i.msp = 239
@@ -339,6 +374,7 @@ def main():
     ap.add_argument("--no-iv-wide", action="store_true")
     ap.add_argument("--no-serve-panel", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
+    ap.add_argument("--no-design", action="store_true")
     ap.add_argument("--ingest-rows", type=int, default=10_000_000)
     ap.add_argument("--serve-panel-requests", type=int, default=32)
     ap.add_argument("--server-workers", type=int, default=2, help="batches in flight at once")
@@ -485,6 +521,10 @@ def main():
     if not args.no_ingest:
         log("ingest leg")
         ingest = ingest_leg(args, lib)
+    design_x = None
+    if not args.no_design:
+        log("design expansion leg")
+        design_x = design_leg(args, lib)
     lat = np.array(step_s) * 1e3
     out = {
         "metric": "FE-regressions/sec (150M-row 3-FE OLS, 3-way clustered SEs, FP64)",
@@ -522,6 +562,7 @@ def main():
         "iv_wide": iv_wide,
         "serving": serving,
         "ingest": ingest,
+        "design_expansion": design_x,
         "fe_values": fe_rec,
     }
     print(json.dumps(out), flush=True)
