@@ -9,7 +9,7 @@
 // :1178-1230); an interaction is the column-wise product of its factors' blocks, the first factor's
 // columns outermost (append_interaction_columns, :1276-1335).  For the wide 2SLS design that is some
 // 40 GB of host memory and PCIe traffic before the solve; here only the raw columns are resident and
-// every output column is written once, by a kernel, from a per-column list of factors.
+// every output value is written once, by a kernel, from a per-column list of factors.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -76,19 +76,23 @@ struct ColSpec {
     int cat[FEHIP_DESIGN_MAX_FACTORS];
 };
 
-// out[row + j * ld_out] = product over the column's factors of (categorical: nint(x) == level; else x)
-__global__ void expand_kernel(const double* __restrict__ raw, int64_t ld_raw, int64_t n, const ColSpec* __restrict__ spec,
-                              int n_cols, double* __restrict__ out, int64_t ld_out) {
-    const int j = blockIdx.y;
-    if (j >= n_cols) return;
-    const ColSpec s = spec[j];
+// out[row + j * ld_out] = product over the column's factors of (categorical: nint(x) == level; else x).
+// One thread per row writes the row of every column of the launch: the few raw values a row needs are
+// read from memory once and then come from the cache, and each column's store of a wave is 64
+// consecutive doubles.
+__global__ __launch_bounds__(256) void expand_kernel(const double* __restrict__ raw, int64_t ld_raw, int64_t n,
+                                                     const ColSpec* __restrict__ spec, int n_cols,
+                                                     double* __restrict__ out, int64_t ld_out) {
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
-        double v = 1.0;
-        for (int f = 0; f < s.n; ++f) {
-            const double x = raw[r + (int64_t)s.col[f] * ld_raw];
-            v *= s.cat[f] ? (nint_ll(x) == (long long)s.level[f] ? 1.0 : 0.0) : x;
+        for (int j = 0; j < n_cols; ++j) {
+            const ColSpec& s = spec[j];  // uniform: scalar loads
+            double v = 1.0;
+            for (int f = 0; f < s.n; ++f) {
+                const double x = raw[r + (int64_t)s.col[f] * ld_raw];
+                v *= s.cat[f] ? (nint_ll(x) == (long long)s.level[f] ? 1.0 : 0.0) : x;
+            }
+            __builtin_nontemporal_store(v, &out[r + (int64_t)j * ld_out]);
         }
-        out[r + (int64_t)j * ld_out] = v;
     }
 }
 
@@ -164,12 +168,9 @@ extern "C" int fehip_design_expand(const double* d_raw, int64_t ld_raw, int n_ra
     ColSpec* ds = static_cast<ColSpec*>(cache_alloc(sizeof(ColSpec) * (size_t)n_cols));
     if (!ds) return set_error("design spec buffer");
     hipError_t e = hipMemcpyAsync(ds, hs.data(), sizeof(ColSpec) * (size_t)n_cols, hipMemcpyHostToDevice, 0);
-    // grid y: output columns, in launches of at most 65535
-    for (int j0 = 0; j0 < n_cols && e == hipSuccess; j0 += 65535) {
-        const int nc = std::min(65535, n_cols - j0);
-        const unsigned gx = std::max(1u, std::min(stream_grid((size_t)n, 256), 4096u / (unsigned)std::min(nc, 64)));
-        hipLaunchKernelGGL(expand_kernel, dim3(gx, (unsigned)nc), dim3(256), 0, 0, d_raw, ld_raw, n,
-                           (const ColSpec*)ds + j0, nc, d_out + (int64_t)j0 * ld_out, ld_out);
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(expand_kernel, dim3(stream_grid((size_t)n, 256)), dim3(256), 0, 0, d_raw, ld_raw, n,
+                           (const ColSpec*)ds, n_cols, d_out, ld_out);
         e = hipGetLastError();
     }
     if (e == hipSuccess) e = hipStreamSynchronize(0);
