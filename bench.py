@@ -375,6 +375,7 @@ def main():
     ap.add_argument("--no-serve-panel", action="store_true")
     ap.add_argument("--no-ingest", action="store_true")
     ap.add_argument("--no-design", action="store_true")
+    ap.add_argument("--no-fe-values", action="store_true")
     ap.add_argument("--ingest-rows", type=int, default=10_000_000)
     ap.add_argument("--serve-panel-requests", type=int, default=32)
     ap.add_argument("--server-workers", type=int, default=2, help="batches in flight at once")
@@ -454,7 +455,7 @@ def main():
                            if v["ms"] > 0 else 0.0} for v in ks.values()}
     # the fixed effects of the fit (the reference's --save-fe reconstruction) on the same panel
     fe_rec = None
-    if fit is not None:
+    if fit is not None and not args.no_fe_values:
         t = time.perf_counter()
         fev, fest = api.fe_values(pan, d_y.ptr, d_W.ptr, 8, fit.beta, tol=args.tol, max_iter=2000)
         fe_values_s = time.perf_counter() - t

@@ -63,11 +63,6 @@ __global__ void copy_table_kernel(const double* __restrict__ src, int64_t G, dou
         dst[g] = src[g];
 }
 
-__global__ void level_counts_kernel(const int* __restrict__ ids, int64_t n, int* __restrict__ cnt) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        atomicAdd(&cnt[ids[i]], 1);
-}
-
 // per block: sum of cnt * val and of cnt (fixed order within the block)
 __global__ __launch_bounds__(kBlock) void weighted_sum_kernel(const double* __restrict__ val, const int* __restrict__ cnt,
                                                               int64_t G, double* __restrict__ part) {
@@ -173,9 +168,9 @@ extern "C" int fehip_fe_values(fehip_panel* panel, const double* d_y, const doub
         } else {
             hipLaunchKernelGGL(copy_table_kernel, dim3(stream_grid((size_t)G, kBlock)), dim3(kBlock), 0, 0,
                                (const double*)alpha.p + toff, G, val);
-            FEHIP_CHECK(hipMemsetAsync(cnt, 0, sizeof(int) * (size_t)G, 0), "counts");
-            hipLaunchKernelGGL(level_counts_kernel, dim3(stream_grid((size_t)n, kBlock)), dim3(kBlock), 0, 0,
-                               (const int*)p->d[d].ids, n, cnt);
+            // rows per level: the panel's own counts
+            FEHIP_CHECK(hipMemcpyAsync(cnt, p->d[d].counts, sizeof(int) * (size_t)G, hipMemcpyDeviceToDevice, 0),
+                        "counts");
             toff += G;
         }
         FEHIP_LAUNCH_CHECK("fe values");
