@@ -396,6 +396,8 @@ struct WaveScores {
     const uint8_t* w;               // panel-order weights (nullptr: all rows)
     const int* spos;                // sorted position of each row in one subset's partition (nullptr: none)
     double* tsort;                  // the score rows in that order (row-major n x K)
+    int* tkey;                      // with spos: 2 * (the row's cell of the small dims) + (row weighted), or nullptr
+    int cstride[3];                 // per slot: the radix of its ids in that cell index (0: not in it)
 };
 
 // Row data of one wave tile as loaded: the loads are issued one tile ahead of their use, without a
@@ -515,6 +517,12 @@ __global__ __launch_bounds__(kBlock) void wave_scores_kernel(WaveScores a, const
 #pragma unroll
                 for (int k = 0; k < KT; ++k)
                     if (k < a.K) dst[k] = tv[k];
+            }
+            if (a.tkey) {
+                int cell = 0;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) cell += kk[j] * a.cstride[j];
+                a.tkey[spr] = 2 * cell + (wrow ? 1 : 0);
             }
         }
         const int ipp = __shfl_up(ip, 1, kWave);
@@ -744,6 +752,79 @@ __global__ __launch_bounds__(kBlock) void sorted_wave_meat_kernel(const double* 
             for (int k = 0; k < KT; ++k) bval[b1 * KT + k] = open_right ? sv[k] : 0.0;
         }
     }
+    gram_store_upper<KT>(gst, gram_stage<KT>(), acc, part + (int64_t)blockIdx.x * (KT * (KT + 1) / 2));
+}
+
+// A subset of one large dim A (not the primary) and small dims B with at most kMaxRunCells cells.  The
+// fused pass writes each row's scores, and its cell of B (key = 2 * cell + weighted), at the row's place
+// in A's partition: the partition sorts stably by A, so the groups of A are contiguous and the rows of a
+// run of A inside a primary segment stay together, and the pass writes whole runs instead of single rows
+// to random places.  One wave per group of A adds the group's rows into an LDS table with a row per cell
+// (plus a count of weighted rows), then adds S S' of the cells that hold a weighted row to its
+// matrix-core accumulator and clears the table.
+constexpr int kMaxRunCells = 64;
+
+template <int KT>
+__global__ __launch_bounds__(kBlock) void run_cells_meat_kernel(const double* __restrict__ T, int K,
+                                                                const int* __restrict__ tkey,
+                                                                const int64_t* __restrict__ off, int64_t G,
+                                                                int ncells, double* part,
+                                                                unsigned long long* n_clusters) {
+    __shared__ double gst[(kBlock / kWave) * gram_stage<KT>()];
+    __shared__ double tab[(kBlock / kWave) * kMaxRunCells * (KT + 1)];
+    gram4_t acc = {0.0, 0.0, 0.0, 0.0};
+    unsigned long long ncl = 0;
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
+    double* mystage = gst + (size_t)wv * gram_stage<KT>();
+    double* mytab = tab + (size_t)wv * kMaxRunCells * (KT + 1);
+    for (int e = lane; e < kMaxRunCells * (KT + 1); e += kWave) mytab[e] = 0.0;
+    const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t g = (int64_t)blockIdx.x * (kBlock / kWave) + wv; g < G; g += wstride) {
+        const int64_t a = off[g], b = off[g + 1];
+        __builtin_amdgcn_wave_barrier();
+        for (int64_t q = a; q < b; q += kWave) {
+            const int64_t i = q + lane;
+            if (i < b) {
+                const int key = tkey[i];
+                double* row = mytab + min(max(key >> 1, 0), ncells - 1) * (KT + 1);
+                double v[KT];
+                if (K == KT) {  // KT is even and rows start 16-byte aligned
+                    const double2* src = reinterpret_cast<const double2*>(T + i * KT);
+#pragma unroll
+                    for (int k = 0; k < KT / 2; ++k) {
+                        const double2 x = src[k];
+                        v[2 * k] = x.x;
+                        v[2 * k + 1] = x.y;
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < KT; ++k) v[k] = k < K ? T[i * K + k] : 0.0;
+                }
+#pragma unroll
+                for (int k = 0; k < KT; ++k)
+                    if (k < K) lds_add(&row[k], v[k]);
+                if (key & 1) lds_add(&row[KT], 1.0);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        double sv[KT];
+        bool on = false;
+        if (lane < ncells) {
+            double* row = mytab + lane * (KT + 1);
+            on = row[KT] > 0.0;
+#pragma unroll
+            for (int k = 0; k < KT; ++k) sv[k] = on ? row[k] : 0.0;
+#pragma unroll
+            for (int k = 0; k <= KT; ++k) row[k] = 0.0;
+        } else {
+#pragma unroll
+            for (int k = 0; k < KT; ++k) sv[k] = 0.0;
+        }
+        const unsigned long long BU = __ballot(on);
+        if (lane == 0) ncl += (unsigned long long)__popcll(BU);
+        gram_wave<KT>(mystage, sv, lane, acc);
+    }
+    if (lane == 0 && ncl) atomicAdd(n_clusters, ncl);
     gram_store_upper<KT>(gst, gram_stage<KT>(), acc, part + (int64_t)blockIdx.x * (KT * (KT + 1) / 2));
 }
 
@@ -1251,7 +1332,15 @@ int project_meat(const std::vector<double>& Mz, int m, const std::vector<double>
 // Which subsets of the sandwich the fused pass over the wave tiles forms, and how: the slot layout of
 // the other cluster dims and the pass's output tables.  The rest go to the methods that read the
 // residual or the score matrix.
-enum { kRouteOther = 0, kRouteNest = 1, kRouteRun = 2, kRouteSmall = 3, kRouteAlias = 4, kRouteSorted = 5 };
+enum {
+    kRouteOther = 0,
+    kRouteNest = 1,
+    kRouteRun = 2,
+    kRouteSmall = 3,
+    kRouteAlias = 4,
+    kRouteSorted = 5,
+    kRouteRunCells = 6
+};
 
 struct ScorePlan {
     bool fused = false;   // a fused pass runs
@@ -1263,8 +1352,9 @@ struct ScorePlan {
     int small_cells[kMaxSmallOuts] = {};
     bool needs_r = false;  // a subset left to the other methods reads the residual or the score matrix
     bool reads_T = false;  // ... the score matrix
-    int sorted_mask = 0;   // the subset whose score rows the pass writes in its partition's order (0: none)
-    unsigned sorted_dims = 0;
+    int sorted_mask = 0;   // the subset whose score rows the pass writes in a partition's order (0: none)
+    unsigned sorted_dims = 0;  // the dims of that partition (the subset's, or its large dim's for RunCells)
+    int cells = 0;             // RunCells: cells of the subset's small dims
 };
 
 int plan_scores(Panel* p, const int* cdims, int ncd, int Kt, unsigned long long* dcount, ScorePlan& pl) {
@@ -1347,11 +1437,36 @@ int plan_scores(Panel* p, const int* cdims, int ncd, int Kt, unsigned long long*
             route = kRouteSmall;
         }
         if (route == kRouteOther && fused_ok && !has_p && pl.sorted_mask == 0) {
-            // one subset through its sorted partition: the pass writes its score rows in that order, so the
-            // partition kernel reads them in sequence instead of gathering them
+            // one subset through a sorted partition: the pass writes its score rows in that order, so the
+            // partition kernel reads them in sequence instead of gathering them.  A large dim crossed with
+            // small dims of few cells takes the large dim's partition, whose runs stay contiguous, and the
+            // rows' cells; any other such subset takes its own partition.
+            int large = -1, nlarge = 0;
+            int64_t bcells = 1;
+            for (int d = 0; d < p->m; ++d)
+                if ((dm >> d) & 1u) {
+                    if (p->d[d].small) {
+                        bcells *= p->d[d].G;
+                    } else {
+                        large = d;
+                        ++nlarge;
+                    }
+                }
             pl.sorted_mask = mask;
-            pl.sorted_dims = dm;
-            route = kRouteSorted;
+            if (nlarge == 1 && len > 1 && bcells <= kMaxRunCells) {
+                pl.sorted_dims = 1u << large;
+                pl.cells = (int)bcells;
+                int stride = 1;
+                for (int d = 0; d < p->m; ++d)
+                    if (((dm >> d) & 1u) && d != large) {
+                        w.cstride[slot_of(d)] = stride;
+                        stride *= p->d[d].G;
+                    }
+                route = kRouteRunCells;
+            } else {
+                pl.sorted_dims = dm;
+                route = kRouteSorted;
+            }
         }
         if (route == kRouteOther) {
             // the other methods: block-tile nested, small tables and run sums read the residual; the
@@ -1953,10 +2068,15 @@ int finish_request(Panel* p, ReqState& s) {
                 if ((e = db.get(&ws.runbuf[j], (size_t)p->d[plan.run_dim[j]].n_runs * KT + 1))) return e;
             ws.small_part = spart;
             ws.rss_part = rssp;
+            int* tkey = nullptr;
             if (plan.sorted_mask) {
                 if ((e = partition_positions(p, plan.sorted_dims, &ws.spos, 0))) return e;
                 if ((e = db.get(&tsort, (size_t)n * K))) return e;
                 ws.tsort = tsort;
+                if (plan.route[plan.sorted_mask] == kRouteRunCells) {
+                    if ((e = db.get(&tkey, (size_t)n))) return e;
+                    ws.tkey = tkey;
+                }
             }
             FEHIP_CHECK(hipMemsetAsync(ucounts, 0, sizeof(unsigned long long) * kMaxNestSubsets, 0), "memset");
             scores_kernel(true);
@@ -2013,6 +2133,30 @@ int finish_request(Panel* p, ReqState& s) {
                             for (int b2 = 0; b2 < K; ++b2) Ms[(size_t)b2 * K + a2] += row[a2] * row[b2];
                     }
                     pre[mask] = std::make_pair(Ms, nc);
+                } else if (rt == kRouteRunCells) {
+                    const Partition* pa = nullptr;
+                    if ((e = panel_partition(p, plan.sorted_dims, &pa, 0))) return e;
+                    FEHIP_CHECK(hipMemsetAsync(ucount, 0, sizeof(unsigned long long), 0), "memset");
+                    const size_t cneed = (size_t)((pa->n_clusters + kBlock / kWave - 1) / (kBlock / kWave));
+                    unsigned cgrid = 1;
+#define FEHIP_RUN_CELLS(KTV)                                                                                      \
+    cgrid = std::min<unsigned>(kMeatBlocks, resident_grid(run_cells_meat_kernel<KTV>, kBlock, 0, cneed));        \
+    hipLaunchKernelGGL(run_cells_meat_kernel<KTV>, dim3(cgrid), dim3(kBlock), 0, 0, (const double*)tsort, K,     \
+                       (const int*)tkey, (const int64_t*)pa->off, pa->n_clusters, plan.cells, mpart, ucount);
+                    if (KT == 4) {
+                        FEHIP_RUN_CELLS(4)
+                    } else if (KT == 8) {
+                        FEHIP_RUN_CELLS(8)
+                    } else {
+                        FEHIP_RUN_CELLS(12)
+                    }
+#undef FEHIP_RUN_CELLS
+                    FEHIP_LAUNCH_CHECK("run cells meat");
+                    std::vector<double> Mc;
+                    if ((e = reduce_upper(mpart, (int)cgrid, KT, K, Mc))) return e;
+                    unsigned long long h = 0;
+                    FEHIP_CHECK(copy_sync(&h, ucount, sizeof(h), hipMemcpyDeviceToHost), "count copy");
+                    pre[mask] = std::make_pair(Mc, (int64_t)h);
                 }
             }
             for (int mask = 1; mask < (1 << ncd); ++mask)

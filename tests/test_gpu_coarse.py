@@ -216,6 +216,30 @@ def test_sorted_subsets_with_clusters_across_waves(K, cluster):
     assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
 
 
+@pytest.mark.parametrize("K,cluster,weighted", [(3, (2, 3), False), (8, (1, 2, 3), False), (8, (2, 3), True),
+                                                 (12, (1, 2, 3), True)])
+def test_large_by_small_cluster_subset(K, cluster, weighted):
+    """A cluster subset of one large dim (firms, more groups than the LDS tables take) and a small one
+    (periods) on a sticky worker-firm panel: the score rows are written in the firm partition, whole runs
+    at a time, with each row's period cell, and the cells of a firm are summed by one wave."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    note(f"large x small subset K={K} cluster={cluster} weighted={weighted}")
+    p = synth.worker_firm_panel(seed=300 + K, n_workers=2000, n_periods=20, n_firms=1500)
+    rng = np.random.default_rng(K)
+    W = p.W[:, :K] if K <= 8 else np.column_stack([p.W, rng.normal(size=(p.W.shape[0], K - 8))])
+    n = W.shape[0]
+    fe = np.vstack([_redense(p.fe[d]) for d in range(3)])  # no unused firm ids (G_d = max id)
+    w = (rng.random(n) < 0.7).astype(np.uint8) if weighted else None
+    fit = api.estimate(p.y, W, fe, weights=w, cluster=cluster, tol=1e-13, max_iter=20000)
+    m = np.ones(n, bool) if w is None else w == 1
+    fe_sub = np.vstack([_redense(fe[d][m]) for d in range(3)])
+    ref = oracle.estimate(p.y[m], W[m], fe_sub, cluster=cluster, tol=1e-13, max_iter=200000)
+    assert fit.res["n_clusters"] == ref.res["n_clusters"]
+    assert fit.res["dof_resid"] == ref.res["dof_resid"]
+    assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
+
+
 def test_wide_iv_design_matches_oracle():
     """A wide 2SLS design (40 regressors, 8 endogenous, 12 instruments): the normal equations, the
     first stage and X^ run through the wide GEMM paths."""
