@@ -1036,7 +1036,9 @@ __global__ __launch_bounds__(kBlock) void cg_start_scalars(const double* part_rz
 
 template <int CT>
 __global__ __launch_bounds__(kBlock) void cg_pq_kernel(int64_t S, const double* __restrict__ p,
-                                                       const double* __restrict__ q, double* part) {
+                                                       const double* __restrict__ q, double* part,
+                                                       const int* __restrict__ active) {
+    if (!any_active<CT>(active)) return;  // every column converged: alpha is 0 whatever the partials
     __shared__ double ws[kBlock / kWave][CT];
     double a[CT];
 #pragma unroll
@@ -1075,6 +1077,8 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t S, const CgSt
                                                            const double* __restrict__ wcnt, double* __restrict__ x,
                                                            double* __restrict__ r, double* __restrict__ z,
                                                            double* part_rz, double* part_max) {
+    // every column converged before this step: x, r and z stay as they are
+    if (!any_active<CT>(st->active)) return;
     double al[CT], rz[CT], mx[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
@@ -1134,6 +1138,7 @@ __global__ __launch_bounds__(kBlock) void cg_beta_scalars(const double* part_rz,
 
 template <int CT>
 __global__ void cg_pdir_kernel(int64_t S, const CgState* __restrict__ st, const double* __restrict__ z, double* p) {
+    if (!any_active<CT>(st->active)) return;
     double be[CT];
     int act[CT];
 #pragma unroll
@@ -1492,7 +1497,7 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             int e2 = apply_and_reduce<CT>(p, L, a, kIter, q, st, n_on);
             if (e2) return e2;
             hipLaunchKernelGGL((cg_pq_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S, (const double*)pp,
-                               (const double*)q, sc.part);
+                               (const double*)q, sc.part, (const int*)sc.st->active);
             hipLaunchKernelGGL((cg_alpha_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part, kCgBlocks,
                                sc.st);
             hipLaunchKernelGGL((cg_update_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S, (const CgState*)sc.st,
