@@ -293,6 +293,10 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     double* sacc = dyn;                     // [Gs][CT]
     double* sp = dyn + (size_t)a.Gs * CT;   // [Gs][CT]
     stage_tables<CT, MODE>(a, sacc, sp);
+    // 1/k for the row counts k = 0..64 of a segment (0 for k = 0), each the correctly rounded quotient,
+    // so a lookup gives the value the division would
+    __shared__ double inv_count[kWave + 1];
+    if (threadIdx.x <= kWave) inv_count[threadIdx.x] = threadIdx.x > 0 ? 1.0 / (double)threadIdx.x : 0.0;
     __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
@@ -304,10 +308,10 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     // records two tiles ahead, row data one tile ahead: loading the next tile's rows never waits for a
     // record requested in the same iteration (gathers a tile ahead as well cost a wave per SIMD of
     // occupancy and more than they hide)
-    // the segment scan serves the means (eliminated primary) and the run sums of the large dims
-    bool need_scan = a.elim != 0;
-    if constexpr (MODE != kFinal)
-        for (int o = 0; o < a.nd; ++o) need_scan |= !a.sd[o].small;
+    // the segment scan serves the means (eliminated primary) and the run sums of the large dims; the
+    // operator and right-hand-side passes take it unconditionally (their solves nearly always need it,
+    // and a branch around it costs a copy of every column at the join)
+    const bool need_scan = MODE != kFinal || a.elim != 0;
     WaveRows<ND> cur, nxt;
     load_wave_rows(a, load_wave_rec(a, t), lane, cur);
     int64_t rec_n = load_wave_rec(a, t + wstride);
@@ -371,7 +375,7 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
                     if (cb[c] != cnt_bit) {     // columns of one slot share the row counts
                         const unsigned long long B = __ballot((word & cb[c]) != 0);
                         const int cnt = __popcll(B & segmask);
-                        inv = cnt > 0 ? 1.0 / (double)cnt : 0.0;
+                        inv = ld_lds(&inv_count[cnt]);
                         inslot = (word & cb[c]) ? 1.0 : 0.0;
                         cnt_bit = cb[c];
                     }
@@ -445,7 +449,7 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
                 if (cb[c] != cnt_bit) {     // columns of one slot share the row counts
                     const unsigned long long B = __ballot((word & cb[c]) != 0);
                     const int cnt = __popcll(B & segmask);
-                    inv = cnt > 0 ? 1.0 / (double)cnt : 0.0;
+                    inv = ld_lds(&inv_count[cnt]);
                     inslot = (word & cb[c]) ? 1.0 : 0.0;
                     cnt_bit = cb[c];
 #pragma unroll
