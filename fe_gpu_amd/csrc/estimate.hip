@@ -446,7 +446,7 @@ __device__ __forceinline__ void load_score_rows(const WaveScores& a, const TX* c
 // ucount[s] (non-empty clusters); run sums into runbuf; small tables into small_part; r^2 into rss_part.
 // NS: compile-time bound on the nested subsets (a.nsub <= NS), which sizes their accumulators
 template <int KT, typename TX, int NS>
-__global__ __launch_bounds__(kBlock) void wave_scores_kernel(WaveScores a, const TX* const* __restrict__ X, double* part,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 ? (NS == 1 ? 4 : 3) : 1))) void wave_scores_kernel(WaveScores a, const TX* const* __restrict__ X, double* part,
                                                              unsigned long long* ucount) {
     __shared__ double gst[(kBlock / kWave) * gram_stage<KT>()];  // per-wave staging of the cluster sums
     extern __shared__ double stab[];                            // small-cell tables [small_lds]
