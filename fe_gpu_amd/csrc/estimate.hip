@@ -507,9 +507,30 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
         cur = nxt;
         __builtin_amdgcn_sched_barrier(0);
         // the score row at its place in the sorted partition of the subset that reads it in that order
+        if constexpr (KT == 4 || KT == 8) {
+            if (a.spos && a.K == KT) {
+                // Full rows go through the wave's staging area, so that one store instruction writes
+                // 64 / (KT / 2) whole rows, one 16-byte piece per lane: the rows of a run land next to each
+                // other in the partition, and such a store covers them with whole cache lines.
+                constexpr int P = KT / 2, RPI = kWave / P;  // pieces per row, rows per instruction
+                double2* st = reinterpret_cast<double2*>(mystage);
+#pragma unroll
+                for (int k = 0; k < P; ++k) st[lane * P + k] = double2{tv[2 * k], tv[2 * k + 1]};
+                __builtin_amdgcn_wave_barrier();
+#pragma unroll
+                for (int i = 0; i < kWave / RPI; ++i) {
+                    const int row = i * RPI + lane / P, piece = lane % P;
+                    const int pos = __shfl((int)spr, row, kWave);
+                    if (row < nr) reinterpret_cast<double2*>(a.tsort + (int64_t)pos * KT)[piece] = st[row * P + piece];
+                }
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
         if (a.spos && valid) {
             double* dst = a.tsort + spr * a.K;
-            if (a.K == KT) {  // KT is even: 16-byte stores
+            if ((KT == 4 || KT == 8) && a.K == KT) {
+                // written above
+            } else if (a.K == KT) {  // KT is even: 16-byte stores
 #pragma unroll
                 for (int k = 0; k < KT / 2; ++k)
                     reinterpret_cast<double2*>(dst)[k] = double2{tv[2 * k], tv[2 * k + 1]};
