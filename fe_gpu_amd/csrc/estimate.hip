@@ -569,10 +569,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
         }
         // run sums of a single large dim: the runs of equal id inside the segment
         auto store_runs = [&](int j, const double (&sv)[KT], bool rend) {
-            if (rend) {
-                double* dst = a.runbuf[j] + (int64_t)rdl[j] * KT;
+            if (rend) {  // KT is even and the rows are KT doubles: 16-byte stores
+                double2* dst = reinterpret_cast<double2*>(a.runbuf[j] + (int64_t)rdl[j] * KT);
 #pragma unroll
-                for (int k = 0; k < KT; ++k) dst[k] = sv[k];
+                for (int k = 0; k < KT / 2; ++k) dst[k] = double2{sv[2 * k], sv[2 * k + 1]};
             }
         };
         for (int j = 0; j < a.nrun; ++j) {
