@@ -76,7 +76,7 @@ def within_transform(ds: GpuDataset, tol: float, max_iter: int):
     n_fe = len(ds.dims)
     K, L, n = ds.n_reg, ds.n_inst, ds.n_obs
     relax = [1.5] * n_fe
-    prev_change = [np.finfo(np.float64).max] * n_fe
+    prev_change = [float(np.finfo(np.float64).max)] * n_fe  # huge(1.0d0); 1.05x of it is +inf, as in Fortran
     order = list(range(n_fe))
     for a in range(n_fe - 1):                       # exchange sort by ascending group count
         for b in range(a + 1, n_fe):
