@@ -216,16 +216,18 @@ def test_sorted_subsets_with_clusters_across_waves(K, cluster):
     assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
 
 
-@pytest.mark.parametrize("K,cluster,weighted", [(3, (2, 3), False), (8, (1, 2, 3), False), (8, (2, 3), True),
-                                                 (12, (1, 2, 3), True)])
-def test_large_by_small_cluster_subset(K, cluster, weighted):
+@pytest.mark.parametrize("K,cluster,weighted,nw,nf", [(3, (2, 3), False, 2000, 1500), (8, (1, 2, 3), False, 2000, 1500),
+                                                       (8, (2, 3), True, 2000, 1500), (12, (1, 2, 3), True, 2000, 1500),
+                                                       (8, (2, 3), False, 6000, 1100)])
+def test_large_by_small_cluster_subset(K, cluster, weighted, nw, nf):
     """A cluster subset of one large dim (firms, more groups than the LDS tables take) and a small one
     (periods) on a sticky worker-firm panel: the score rows are written in the firm partition, whole runs
-    at a time, with each row's period cell, and the cells of a firm are summed by one wave."""
+    at a time, with each row's period cell, and the cells of a firm are summed by one wave (the last case
+    has firms of more than 64 rows, which the wave takes in several passes)."""
     import oracle
     from fe_gpu_amd import api, synth
-    note(f"large x small subset K={K} cluster={cluster} weighted={weighted}")
-    p = synth.worker_firm_panel(seed=300 + K, n_workers=2000, n_periods=20, n_firms=1500)
+    note(f"large x small subset K={K} cluster={cluster} weighted={weighted} workers={nw} firms={nf}")
+    p = synth.worker_firm_panel(seed=300 + K, n_workers=nw, n_periods=20, n_firms=nf)
     rng = np.random.default_rng(K)
     W = p.W[:, :K] if K <= 8 else np.column_stack([p.W, rng.normal(size=(p.W.shape[0], K - 8))])
     n = W.shape[0]
@@ -235,6 +237,8 @@ def test_large_by_small_cluster_subset(K, cluster, weighted):
     m = np.ones(n, bool) if w is None else w == 1
     fe_sub = np.vstack([_redense(fe[d][m]) for d in range(3)])
     ref = oracle.estimate(p.y[m], W[m], fe_sub, cluster=cluster, tol=1e-13, max_iter=200000)
+    if nw > 2000:
+        assert np.bincount(fe[1]).max() > 64  # firms longer than one wave pass
     assert fit.res["n_clusters"] == ref.res["n_clusters"]
     assert fit.res["dof_resid"] == ref.res["dof_resid"]
     assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
