@@ -1537,7 +1537,9 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             hipGraph_t graph = nullptr;
             std::lock_guard<std::mutex> capture_lock(capture_mutex());
             FEHIP_CHECK(hipStreamBeginCapture(gs, hipStreamCaptureModeThreadLocal), "cg capture");
-            const int ec = step(gs);
+            // one graph holds the `every` steps between two convergence checks
+            int ec = 0;
+            for (int k = 0; k < every && !ec; ++k) ec = step(gs);
             const hipError_t he = hipStreamEndCapture(gs, &graph);
             if (ec) {
                 if (graph) (void)hipGraphDestroy(graph);
@@ -1549,7 +1551,7 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             FEHIP_CHECK(hi, "cg graph");
         }
         int it = 0;
-        for (; it < o.max_iter; ++it) {
+        while (it < o.max_iter) {
             if (it % every == 0) {
                 FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs), "cg state");
                 FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
@@ -1565,11 +1567,13 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
                 n_on = 0;
                 for (int c = 0; c < ncols; ++c) n_on += sc.st_host->active[c] != 0;
             }
-            if (gexec) {
+            if (gexec && it % every == 0 && it + every <= o.max_iter) {
                 FEHIP_CHECK(hipGraphLaunch(gexec, gs), "cg graph launch");
-            } else if ((e = step(gs))) {
-                return e;
+                it += every;
+                continue;
             }
+            if ((e = step(gs))) return e;  // per-kernel timing, or the steps past the last whole graph
+            ++it;
         }
         FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs), "cg state");
         FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
