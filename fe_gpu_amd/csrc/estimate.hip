@@ -1889,6 +1889,19 @@ int finish_request(Panel* p, ReqState& s) {
         }
         rss = std::max(0.0, g(0, 0) - 2.0 * by + bq);
         rss_done = true;
+        // y~'y~ - 2 b'W~'y~ + b'W~'W~b loses about eps * y~'y~ to cancellation: when the fit leaves less
+        // than 1e-6 of y~'y~ (within R^2 near 1), FP64 columns take rss from the residual itself, as the
+        // reference does (r = y~ - W~b, then r'r).  FP32 columns keep the FP64 cross products, whose
+        // error is below that of a residual formed from the rounded columns.
+        if (!f32 && !(rss > 1e-6 * g(0, 0))) {
+            FEHIP_CHECK(copy_sync(dbeta, beta, sizeof(double) * K, hipMemcpyHostToDevice), "beta copy");
+            hipLaunchKernelGGL(resid_scores_kernel<double>, dim3(kResidBlocks), dim3(256), 0, 0, dcols, n, K,
+                               (const double*)dbeta, r, (double*)nullptr, K, dcols + 1, part);
+            FEHIP_LAUNCH_CHECK("residual");
+            std::vector<double> ps;
+            if ((e = host_sum_parts(part, kResidBlocks, 1, ps))) return e;
+            rss = ps[0];
+        }
     } else if (!f32 || have_gram) {
         // With FP32 storage and cross products formed in FP64 before the rounding, beta above is already
         // the FP64 solution (no refinement); the residual and scores then read the FP32 columns, whose

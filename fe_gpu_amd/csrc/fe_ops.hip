@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kBlock) void accumulate_lds_kernel(
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
         int g = ids[i] - 1;
-        if (g < 0) continue;
+        if (g < 0 || g >= G) continue;  // ids outside 1..n_groups are skipped, as the reference's small-G kernel does
         atomicAdd(&tab[g], y[i]);
         for (int k = 0; k < K; ++k) atomicAdd(&tab[(1 + k) * G + g], W[i + (size_t)k * ld]);
         for (int k = 0; k < L; ++k) atomicAdd(&tab[(1 + K + k) * G + g], Z[i + (size_t)k * ld]);
@@ -63,7 +63,7 @@ __device__ __forceinline__ double run_sum(double v, int lane, int seg_end) {
 // accumulate, large G: one row per lane, runs of equal ids merged within the wavefront.
 __global__ __launch_bounds__(kBlock) void accumulate_runs_kernel(
     const double* __restrict__ y, const double* __restrict__ W, const double* __restrict__ Z,
-    const int* __restrict__ ids, size_t n, int K, int L, size_t ld, double* __restrict__ sy,
+    const int* __restrict__ ids, size_t n, int G, int K, int L, size_t ld, double* __restrict__ sy,
     double* __restrict__ sW, double* __restrict__ sZ, int* __restrict__ cnt) {
     const int lane = threadIdx.x & (kWave - 1);
     const size_t wave_stride = (size_t)gridDim.x * blockDim.x;
@@ -72,7 +72,9 @@ __global__ __launch_bounds__(kBlock) void accumulate_runs_kernel(
         const size_t i = base + lane;
         const bool in = i < n;
         int g = in ? ids[i] - 1 : -1;
-        if (g < 0) g = -1 - lane;               // invalid rows form singleton runs
+        // ids outside 1..n_groups form singleton runs and are skipped (the reference's large-G kernel
+        // would write past its tables for ids above n_groups; its small-G kernel skips them)
+        if (g < 0 || g >= G) g = -1 - lane;
         int gnext = __shfl_down(g, 1, kWave);
         int gprev = __shfl_up(g, 1, kWave);
         bool tail = (lane == kWave - 1) || (gnext != g);
@@ -201,8 +203,8 @@ int fe_gpu_fe_accumulate(const double* y, const double* W, const double* Z, cons
                            n_groups, K, L, leading_dim, group_sum_y, group_sum_W, group_sum_Z, group_counts);
     } else {
         unsigned grid = stream_grid(n_obs, kBlock, 8);
-        hipLaunchKernelGGL(accumulate_runs_kernel, dim3(grid), dim3(kBlock), 0, 0, y, W, Z, fe_ids, n_obs, K, L,
-                           leading_dim, group_sum_y, group_sum_W, group_sum_Z, group_counts);
+        hipLaunchKernelGGL(accumulate_runs_kernel, dim3(grid), dim3(kBlock), 0, 0, y, W, Z, fe_ids, n_obs, n_groups,
+                           K, L, leading_dim, group_sum_y, group_sum_W, group_sum_Z, group_counts);
     }
     FEHIP_LAUNCH_CHECK("accumulate kernel");
     return set_ok();

@@ -263,7 +263,9 @@ int fe_gpu_runtime_init(int device_id, fe_gpu_device_info* info) {
         info->multiprocessor_count = prop.multiProcessorCount;
         info->major = prop.major;
         info->minor = prop.minor;
-        std::strncpy(info->name, prop.name, FE_GPU_NAME_LEN - 1);
+        // the marketing name can be empty (it is on MI355X): fall back to the architecture name (gfx950...)
+        const char* name = prop.name[0] ? prop.name : prop.gcnArchName;
+        std::strncpy(info->name, name, FE_GPU_NAME_LEN - 1);
         info->name[FE_GPU_NAME_LEN - 1] = '\0';
     }
     return set_ok();

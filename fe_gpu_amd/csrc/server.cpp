@@ -42,6 +42,7 @@ struct Ticket {
     int status = 0;
     std::string msg;
     bool done = false;
+    bool claimed = false;  // a waiter holds this ticket; a second waiter is refused
 };
 
 struct Server {
@@ -264,6 +265,10 @@ extern "C" int fehip_server_wait(fehip_server* srv, int64_t ticket, double* late
     auto it = sv.tickets.find(ticket);
     if (it == sv.tickets.end()) return set_error("fehip_server_wait: unknown ticket %lld", (long long)ticket);
     Ticket* t = it->second.get();
+    // one waiter per ticket: the ticket is freed when its wait returns, so a second concurrent waiter
+    // would read freed memory
+    if (t->claimed) return set_error("fehip_server_wait: ticket %lld already has a waiter", (long long)ticket);
+    t->claimed = true;
     sv.cv_done.wait(lk, [&] { return t->done; });
     const int status = t->status;
     const std::string msg = t->msg;

@@ -296,3 +296,28 @@ def test_pipeline_iv_matches_oracle(lib):
         fit = oracle.estimate(p.y, p.W, p.fe, Z=p.Z, iv_reg=[1], cluster=cluster, tol=1e-12, max_iter=20000)
         assert res["is_iv"] == 1 and fit.res["is_iv"] == 1
         _fits_close(beta, se, fit)
+
+
+@pytest.mark.parametrize("G", [13, 5000])
+def test_accumulate_skips_ids_outside_the_groups(lib, G):
+    """Ids below 1 or above n_groups are skipped by both accumulate paths (LDS tables for small G,
+    merged runs otherwise), as the reference's small-G kernel does."""
+    from fe_gpu_amd.device import DeviceBuffer
+    note(f"accumulate out-of-range ids G={G}")
+    rng = np.random.default_rng(G + 1)
+    n, K = 30000, 2
+    ids = rng.integers(-2, G + 4, n).astype(np.int32)
+    y = rng.normal(size=n)
+    W = np.asfortranarray(rng.normal(size=(n, K)))
+    dy, dW, di = dev(y), dev(W), dev(ids)
+    sy, sW, cnt = DeviceBuffer(G * 8), DeviceBuffer(G * K * 8), DeviceBuffer(G * 4)
+    for b in (sy, sW, cnt):
+        b.zero()
+    assert lib.fe_gpu_fe_accumulate(dy.ptr, dW.ptr, None, di.ptr, n, G, K, 0, n, sy.ptr, sW.ptr, None, cnt.ptr) == 0
+    ok = (ids >= 1) & (ids <= G)
+    ey = np.zeros(G); np.add.at(ey, ids[ok] - 1, y[ok])
+    eW = np.zeros((G, K)); np.add.at(eW, ids[ok] - 1, W[ok])
+    ec = np.bincount(ids[ok] - 1, minlength=G)
+    assert np.array_equal(cnt.to_array(np.int32, G), ec)
+    assert rel(sy.to_array(np.float64, G), ey) < 1e-12
+    assert rel(sW.to_array(np.float64, (G, K)), eW) < 1e-12

@@ -111,7 +111,7 @@ def test_estimate_matches_oracle(case, cluster):
     if cl:
         assert fit.res["n_clusters"] == ref.res["n_clusters"]
         assert fit.res["psd_fix_applied"] == ref.res["psd_fix_applied"]
-    for k in ("rss", "tss_total", "tss_within", "r2", "r2_within", "r2_adj"):
+    for k in ("rss", "tss_total", "tss_within", "r2", "r2_within", "r2_adj", "f_stat"):
         assert abs(fit.res[k] - ref.res[k]) <= 1e-8 * max(1.0, abs(ref.res[k])), k
 
 
@@ -258,3 +258,99 @@ def test_wide_iv_design_matches_oracle():
     assert fit.res["is_iv"] == 1 and fit.res["n_instruments"] == ref.res["n_instruments"]
     assert fit.res["dof_model"] == ref.res["dof_model"]
     assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
+
+
+def test_iv_reverts_to_ols_and_skips_bad_indices():
+    """The reference's selection (src/core/fe_pipeline.f90:664-753): out-of-range endogenous and
+    instrument indices are ignored, no valid instrument index means every instrument column, and fewer
+    instruments than endogenous regressors reverts to OLS (is_iv 0, n_instruments 0)."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    note("IV selection and reversion")
+    p = synth.random_fe_problem(92, n=5000, K=4, L=2, G=(900, 700, 6))
+    cases = [
+        dict(iv_reg=[1, 2, 3], iv_inst=[]),      # 3 endogenous, 2 instruments: OLS
+        dict(iv_reg=[2, 9], iv_inst=[]),         # index 9 ignored: 1 endogenous, IV
+        dict(iv_reg=[1], iv_inst=[5, 0]),        # no valid instrument index: every instrument column
+        dict(iv_reg=[1, 3], iv_inst=[2, 7]),     # one valid instrument for two endogenous: OLS
+    ]
+    for c in cases:
+        fit = api.estimate(p.y, p.W, p.fe, Z=p.Z, cluster=(1,), tol=1e-13, max_iter=20000, **c)
+        ref = oracle.estimate(p.y, p.W, p.fe, Z=p.Z, cluster=(1,), tol=1e-13, max_iter=200000, **c)
+        assert fit.res["is_iv"] == ref.res["is_iv"], c
+        assert fit.res["n_instruments"] == ref.res["n_instruments"], c
+        assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8, c
+    ols = oracle.estimate(p.y, p.W, p.fe, cluster=(1,), tol=1e-13, max_iter=200000)
+    fit = api.estimate(p.y, p.W, p.fe, Z=p.Z, iv_reg=[1, 2, 3], cluster=(1,), tol=1e-13, max_iter=20000)
+    assert fit.res["is_iv"] == 0 and fit.res["n_instruments"] == 0
+    assert rel(fit.beta, ols.beta) < 1e-8 and rel(fit.se, ols.se) < 1e-8
+
+
+def test_homoskedastic_se_of_a_nearly_exact_fit():
+    """Noise of sd 1e-5 leaves rss about 1e-10 of y~'y~: rss is then taken from the residual, not from
+    the cross products (whose cancellation would cost about 1e-6 relative), so the SEs keep 1e-8."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    note("nearly exact fit")
+    p = synth.random_fe_problem(94, n=8000, K=3, G=(1500, 900, 7))
+    rng = np.random.default_rng(5)
+    y = p.W @ np.array([0.7, -1.2, 0.4]) + 3.0 * rng.normal(size=1500)[p.fe[0] - 1] + 1e-5 * rng.normal(size=p.n)
+    fit = api.estimate(y, p.W, p.fe, tol=1e-14, max_iter=20000)
+    ref = oracle.estimate(y, p.W, p.fe, tol=1e-14, max_iter=200000)
+    assert ref.res["rss"] < 1e-8 * ref.res["tss_within"]
+    assert rel(fit.beta, ref.beta) < 1e-8
+    assert rel(fit.se, ref.se) < 1e-8
+    assert abs(fit.res["rss"] - ref.res["rss"]) <= 1e-8 * ref.res["rss"]
+
+
+@pytest.mark.parametrize("max_iter,check_every", [(7, 4), (5, 3), (8, 4)])
+def test_cg_iteration_cap(max_iter, check_every):
+    """A solve stopped by max_iter before converging: the captured graph of check_every steps plus the
+    steps past the last whole graph give the same result as launching every step directly (the
+    per-kernel timing path), and report converged 0 with max_iter iterations."""
+    from fe_gpu_amd import _lib, api, synth
+    from fe_gpu_amd.device import DeviceBuffer
+    note(f"cg cap max_iter={max_iter} check_every={check_every}")
+    p = synth.worker_firm_panel(seed=11, n_workers=3000, n_periods=12, n_firms=600)
+    X = np.asfortranarray(np.column_stack([p.y, p.W[:, :4]]))
+    pan = api.Panel.from_host(p.fe)
+    L = _lib.load()
+    outs = []
+    for timed in (False, True):
+        d = DeviceBuffer.from_array(X)
+        L.fehip_kstats_enable(1 if timed else 0)
+        try:
+            st = api.demean(pan, d.ptr, X.shape[1], X.shape[0], tol=1e-15, max_iter=max_iter,
+                            check_every=check_every)
+        finally:
+            L.fehip_kstats_enable(0)
+        assert st["converged"] == 0 and st["iterations"] == max_iter, st
+        outs.append(d.to_array(np.float64, X.shape, order="F"))
+    assert np.max(np.abs(outs[0] - outs[1])) <= 1e-14 * np.max(np.abs(outs[1]))
+    pan.close()
+
+
+def test_server_ticket_takes_one_waiter():
+    """Two threads waiting on one ticket: exactly one receives the result; the other is refused (the
+    ticket is freed when its wait returns)."""
+    import ctypes as C
+    import threading
+    from fe_gpu_amd import _lib, api, synth
+    from fe_gpu_amd.device import DeviceBuffer
+    note("server ticket waiters")
+    p = synth.random_fe_problem(95, n=20000, K=2, G=(3000, 800, 9))
+    pan = api.Panel.from_host(p.fe)
+    dy, dW = DeviceBuffer.from_array(p.y), DeviceBuffer.from_array(np.asfortranarray(p.W))
+    srv = api.Server(pan, workers=1)
+    t = srv.submit(api.Request(dy.ptr, dW.ptr, 2, options=dict(cluster=(1, 2), tol=1e-10)))
+    L = _lib.load()
+    status = []
+    th = threading.Thread(target=lambda: status.append(L.fehip_server_wait(srv.handle, t, None)))
+    th.start()
+    status.append(L.fehip_server_wait(srv.handle, t, C.byref(C.c_double())))
+    th.join()
+    assert sorted(s == 0 for s in status) == [False, True], status
+    L.fe_gpu_runtime_clear_error()
+    srv._pending.clear()
+    srv.close()
+    pan.close()
