@@ -39,6 +39,7 @@ ABI = [
     ("fe_gpu_gemm", INT, [C.c_char, C.c_char, INT, INT, INT, C.c_double, VP, INT, VP, INT, C.c_double, VP, INT]),
     ("fe_gpu_cluster_scores", INT, [VP, VP, VP, INT, INT, INT, INT, VP]),
     ("fe_gpu_cluster_meat", INT, [INT, INT, VP, INT, VP]),
+    ("fe_gpu_syrk_noalloc", INT, [INT, INT, C.c_double, VP, INT, C.c_double, VP]),
     ("fe_gpu_fe_accumulate", INT, [VP, VP, VP, VP, SZ, INT, INT, INT, SZ, VP, VP, VP, VP]),
     ("fe_gpu_fe_compute_means", INT, [VP, VP, VP, VP, INT, INT, INT]),
     ("fe_gpu_fe_subtract", INT, [VP, VP, VP, VP, SZ, INT, INT, SZ, VP, VP, VP, C.c_double]),

@@ -7,6 +7,8 @@
 //   * the cluster scores merge runs of equal cluster ids inside each wavefront before the
 //     FP64 atomic, so cluster-sorted rows pay one atomic per run and column.
 #include <algorithm>
+#include <mutex>
+#include <vector>
 
 #include "../../include/fe_gpu_runtime.h"
 #include "common.hpp"
@@ -193,9 +195,12 @@ __global__ void gemm_finish_kernel(const double* __restrict__ partial, int nspli
 
 namespace {
 // A^T B on the matrix cores: tile size with the least padding of m and n (64, 96 or 128), split-K
-// over the rows to fill the device.
+// over the rows to fill the device.  The split-K partials come from the library's workspace, or from
+// `fixed` (capacity `fixed_cap` doubles) when the caller must not allocate; the split count is then
+// bounded by that capacity.
 int gemm_tn_mfma(bool rm, int m, int n, long long k, double alpha, const double* A, long long ldA, const double* B,
-                 long long ldB, double beta, double* C, long long ldC, bool upper_only) {
+                 long long ldB, double beta, double* C, long long ldC, bool upper_only, double* fixed,
+                 size_t fixed_cap) {
     int nb = 2;
     double best = 0.0;
     for (int cand : {4, 3, 2}) {
@@ -211,10 +216,11 @@ int gemm_tn_mfma(bool rm, int m, int n, long long k, double alpha, const double*
     long long nsplit = std::max<long long>(1, (4LL * num_cus()) / std::max<long long>(1, tiles));
     nsplit = std::min<long long>(nsplit, std::max<long long>(1, k / 1024));
     nsplit = std::min<long long>(nsplit, std::max<long long>(1, (512LL << 20) / ((long long)m * n * 8)));
+    if (fixed) nsplit = std::min<long long>(nsplit, std::max<long long>(1, (long long)(fixed_cap / ((size_t)m * n))));
     long long kchunk = (k + nsplit - 1) / nsplit;
     kchunk = (kchunk + 31) / 32 * 32;
     nsplit = (k + kchunk - 1) / kchunk;
-    double* partial = static_cast<double*>(workspace((size_t)nsplit * m * n * sizeof(double), 2));
+    double* partial = fixed ? fixed : static_cast<double*>(workspace((size_t)nsplit * m * n * sizeof(double), 2));
     if (!partial) return set_error("gemm: workspace allocation failed");
     const int tx = (m + bt - 1) / bt, ty = (n + bt - 1) / bt;
     const long long per_xcd = (nsplit + kXcds - 1) / kXcds;
@@ -238,14 +244,16 @@ int gemm_tn_mfma(bool rm, int m, int n, long long k, double alpha, const double*
 }
 }  // namespace
 
-int gemm_impl(bool tA, bool tB, int m, int n, long long k, double alpha, const double* A, long long ldA,
-              const double* B, long long ldB, double beta, double* C, long long ldC, bool upper_only) {
+namespace {
+int gemm_dispatch(bool tA, bool tB, int m, int n, long long k, double alpha, const double* A, long long ldA,
+                  const double* B, long long ldB, double beta, double* C, long long ldC, bool upper_only,
+                  double* fixed, size_t fixed_cap) {
     if (m <= 0 || n <= 0) return set_ok();
     // wide cross products over many observations go to the matrix cores
     if (tA != tB && m >= 16 && n >= 16 && k >= 4096) {  // A^T B (column-major) or A B^T (row-major view)
-        const bool timed = kstats_enabled();
+        const bool timed = !fixed && kstats_enabled();  // the timing events are not for the no-allocation path
         if (timed) kstat_begin(kStatGram, 0);
-        const int e = gemm_tn_mfma(!tA, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, upper_only);
+        const int e = gemm_tn_mfma(!tA, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, upper_only, fixed, fixed_cap);
         if (timed) {
             const double pairs = upper_only ? 0.5 * (double)m * (m + 1) : (double)m * n;
             kstat_end(kStatGram, 0, 8.0 * (double)k * (upper_only ? m : m + n), 2.0 * pairs * (double)k);
@@ -261,6 +269,7 @@ int gemm_impl(bool tA, bool tB, int m, int n, long long k, double alpha, const d
         if (nsplit > max_by_k) nsplit = max_by_k;
         long long max_by_mem = (256LL << 20) / ((long long)m * n * 8);
         if (nsplit > max_by_mem) nsplit = max_by_mem;
+        if (fixed && nsplit > (long long)(fixed_cap / ((size_t)m * n))) nsplit = (long long)(fixed_cap / ((size_t)m * n));
         if (nsplit < 1) nsplit = 1;
         if (nsplit > 65535) nsplit = 65535;
     }
@@ -275,7 +284,7 @@ int gemm_impl(bool tA, bool tB, int m, int n, long long k, double alpha, const d
         FEHIP_LAUNCH_CHECK("gemm tile kernel");
         return set_ok();
     }
-    double* partial = static_cast<double*>(workspace((size_t)nsplit * m * n * sizeof(double), 2));
+    double* partial = fixed ? fixed : static_cast<double*>(workspace((size_t)nsplit * m * n * sizeof(double), 2));
     if (!partial) return set_error("gemm: workspace allocation failed");
     hipLaunchKernelGGL(gemm_tile_kernel, grid, dim3(kBlock), 0, 0, tA, tB, m, n, k, A, ldA, B, ldB, kchunk, alpha,
                        beta, C, ldC, partial, upper_only);
@@ -284,6 +293,44 @@ int gemm_impl(bool tA, bool tB, int m, int n, long long k, double alpha, const d
                        (int)nsplit, m, n, alpha, beta, C, ldC, upper_only);
     FEHIP_LAUNCH_CHECK("gemm finish kernel");
     return set_ok();
+}
+}  // namespace
+
+int gemm_impl(bool tA, bool tB, int m, int n, long long k, double alpha, const double* A, long long ldA,
+              const double* B, long long ldB, double beta, double* C, long long ldC, bool upper_only) {
+    return gemm_dispatch(tA, tB, m, n, k, alpha, A, ldA, B, ldB, beta, C, ldC, upper_only, nullptr, 0);
+}
+
+// ------------------------------------------------------------------------------------------------
+// Scratch of the allocation-free SYRK: the split-K partials live in this fixed array, one copy per
+// device for the life of the process.  Host code never passes the array's name to a kernel (on the
+// host it is a stand-in, not a device address): it uses the address hipGetSymbolAddress returns for
+// the current device, looked up once per device (at fe_gpu_linalg_init or the first call).
+constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+std::mutex g_scratch_mutex;
+std::vector<double*> g_scratch_ptr;  // per device: the device address of `scratch`, or null
+
+// the current device's address of `scratch`; nullptr (with the error message set) on failure
+double* noalloc_scratch() {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0) {
+        set_error("syrk_noalloc: no current device");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error("syrk_noalloc: hipGetSymbolAddress(scratch) failed: %s", hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[dev];
 }
 
 namespace {
@@ -371,7 +418,11 @@ using namespace fehip;
 
 extern "C" {
 
-int fe_gpu_linalg_init(void) { return set_ok(); }
+int fe_gpu_linalg_init(void) {
+    // look up the no-allocation scratch of the current device now, so its first call does no set-up
+    if (!noalloc_scratch()) return 1;
+    return set_ok();
+}
 
 int fe_gpu_linalg_shutdown(void) { return 0; }
 
@@ -379,6 +430,18 @@ int fe_gpu_syrk(int n_rows, int n_cols, double alpha, const double* W, int ldW, 
     if (n_cols == 0) return set_ok();
     if (n_cols < 0 || n_rows < 0) return set_error("syrk: negative dimension");
     return gemm_impl(true, false, n_cols, n_cols, n_rows, alpha, W, ldW, W, ldW, beta, Q, n_cols, true);
+}
+
+int fe_gpu_syrk_noalloc(int n_rows, int n_cols, double alpha, const double* W, int ldW, double beta, double* Q) {
+    if (n_cols == 0) return set_ok();
+    if (n_cols < 0 || n_rows < 0) return set_error("syrk_noalloc: negative dimension");
+    const size_t cap = kNoAllocScratchBytes / sizeof(double);
+    if ((size_t)n_cols * (size_t)n_cols > cap)
+        return set_error("syrk_noalloc: n_cols = %d needs %zu bytes of scratch, above the %zu-byte maximum (n_cols <= %d)",
+                         n_cols, (size_t)n_cols * n_cols * sizeof(double), kNoAllocScratchBytes, 2048);
+    double* fixed = noalloc_scratch();
+    if (!fixed) return 1;
+    return gemm_dispatch(true, false, n_cols, n_cols, n_rows, alpha, W, ldW, W, ldW, beta, Q, n_cols, true, fixed, cap);
 }
 
 int fe_gpu_gemv(int n_rows, int n_cols, double alpha, const double* W, int ldW, const double* y, double beta,

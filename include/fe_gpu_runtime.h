@@ -89,6 +89,13 @@ int fe_gpu_cluster_scores(const double* residual, const double* W, const int* cl
                           int ldW, int n_clusters, double* scores);
 /* meat(upper) = scores^T scores. */
 int fe_gpu_cluster_meat(int n_clusters, int n_cols, const double* scores, int ldScores, double* meat);
+/* fe_gpu_syrk without any allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments and result as fe_gpu_syrk; its split-K partials live in a
+ * fixed 32 MiB device array held by each GPU the library runs on until the process ends, so
+ * n_cols * n_cols * 8 must not exceed 32 MiB (n_cols <= 2048; larger is an error).  Every call on
+ * one GPU shares that array: calls that overlap on one GPU (other host threads or streams) race. */
+int fe_gpu_syrk_noalloc(int n_rows, int n_cols, double alpha, const double* W, int ldW, double beta, double* Q);
 
 /* ---- FE projection pieces (src/gpu/fe_gpu_runtime.h:82-114; CUDA impl fe_gpu_runtime.cu:573-641) ---- */
 
