@@ -365,72 +365,49 @@ typedef struct {
     double *y, *W, *Z;
 } fo_cols;
 
-static double fo_absmax(const double* a, int64_t m) {
-    double r = 0.0;
-    for (int64_t i = 0; i < m; ++i)
-        if (fabs(a[i]) > r) r = fabs(a[i]);
-    return r;
-}
-
+/* The tables are kept column by column here ([col][g]; y, then W, then Z): each column's group sums run
+ * over the rows in row order on their own, so the columns are independent work for the threads and the
+ * results do not depend on the thread count.  The table layout is internal to this restatement (the
+ * reference's group-major layout changes neither a sum's order nor any value). */
 static double fo_relaxed_project(const fo_cols* c, const int32_t* ids, int32_t G, double r, double* my, double* mW,
                                  double* mZ, int32_t* cnt, double* py, double* pW, double* pZ, double* max_update) {
     const int64_t n = c->n, K = c->K, L = c->L;
-    memset(my, 0, (size_t)G * sizeof(double));
-    if (K > 0) memset(mW, 0, (size_t)G * (size_t)K * sizeof(double));
-    if (L > 0) memset(mZ, 0, (size_t)G * (size_t)L * sizeof(double));
+    const int64_t ncol = 1 + K + L;
     memset(cnt, 0, (size_t)G * sizeof(int32_t));
     for (int64_t i = 0; i < n; ++i) {
         int32_t g = ids[i] - 1;
         if (g < 0) continue;
-        my[g] += c->y[i];
-        for (int64_t k = 0; k < K; ++k) mW[(int64_t)g * K + k] += c->W[k * n + i];
-        for (int64_t k = 0; k < L; ++k) mZ[(int64_t)g * L + k] += c->Z[k * n + i];
         cnt[g] += 1;
     }
-    for (int32_t g = 0; g < G; ++g) {
-        double inv = cnt[g] > 0 ? 1.0 / (double)cnt[g] : 0.0;
-        my[g] *= inv;
-        for (int64_t k = 0; k < K; ++k) mW[(int64_t)g * K + k] *= inv;
-        for (int64_t k = 0; k < L; ++k) mZ[(int64_t)g * L + k] *= inv;
-    }
-    /* mix: mixed = prev + r (m - prev); prev = mixed; mean = mixed */
-    for (int64_t e = 0; e < G; ++e) {
-        double v = py[e] + r * (my[e] - py[e]);
-        py[e] = v;
-        my[e] = v;
-    }
-    if (K > 0)
-        for (int64_t e = 0; e < (int64_t)G * K; ++e) {
-            double v = pW[e] + r * (mW[e] - pW[e]);
-            pW[e] = v;
-            mW[e] = v;
+    double cd = 0.0;
+#pragma omp parallel for schedule(dynamic) reduction(max : cd) if (n * ncol >= (1 << 18))
+    for (int64_t col = 0; col < ncol; ++col) {
+        double* x = col == 0 ? c->y : (col <= K ? c->W + (col - 1) * n : c->Z + (col - 1 - K) * n);
+        double* m = col == 0 ? my : (col <= K ? mW + (col - 1) * (int64_t)G : mZ + (col - 1 - K) * (int64_t)G);
+        double* pv = col == 0 ? py : (col <= K ? pW + (col - 1) * (int64_t)G : pZ + (col - 1 - K) * (int64_t)G);
+        memset(m, 0, (size_t)G * sizeof(double));
+        for (int64_t i = 0; i < n; ++i) {
+            int32_t g = ids[i] - 1;
+            if (g < 0) continue;
+            m[g] += x[i];
         }
-    if (L > 0)
-        for (int64_t e = 0; e < (int64_t)G * L; ++e) {
-            double v = pZ[e] + r * (mZ[e] - pZ[e]);
-            pZ[e] = v;
-            mZ[e] = v;
+        for (int32_t g = 0; g < G; ++g) {
+            double inv = cnt[g] > 0 ? 1.0 / (double)cnt[g] : 0.0;
+            m[g] *= inv;
+            /* mix: mixed = prev + r (m - prev); prev = mixed; mean = mixed */
+            double v = pv[g] + r * (m[g] - pv[g]);
+            pv[g] = v;
+            m[g] = v;
+            if (fabs(v) > cd) cd = fabs(v);
         }
-    for (int64_t i = 0; i < n; ++i) {
-        int32_t g = ids[i] - 1;
-        if (g < 0) continue;
-        c->y[i] -= my[g];
-        for (int64_t k = 0; k < K; ++k) c->W[k * n + i] -= mW[(int64_t)g * K + k];
-        for (int64_t k = 0; k < L; ++k) c->Z[k * n + i] -= mZ[(int64_t)g * L + k];
+        for (int64_t i = 0; i < n; ++i) {
+            int32_t g = ids[i] - 1;
+            if (g < 0) continue;
+            x[i] -= m[g];
+        }
     }
-    double ch = fo_absmax(my, G);
-    double cd = ch;
-    if (ch > *max_update) *max_update = ch;
-    if (K > 0) {
-        ch = fo_absmax(mW, (int64_t)G * K);
-        if (ch > cd) cd = ch;
-        if (ch > *max_update) *max_update = ch;
-    }
-    if (L > 0) {
-        ch = fo_absmax(mZ, (int64_t)G * L);
-        if (ch > cd) cd = ch;
-        if (ch > *max_update) *max_update = ch;
-    }
+    /* change = max |mixed mean| over the y, W and Z tables */
+    if (cd > *max_update) *max_update = cd;
     return cd;
 }
 
@@ -756,11 +733,15 @@ static void fo_finalize_stats(fo_result* r, int64_t n_obs, int has_fes) {
 static void fo_meat(const int32_t* ids, int64_t nc, const double* r, const double* X, int64_t n, int64_t K,
                     double* meat) {
     double* S = (double*)calloc((size_t)(nc > 0 ? nc : 1) * (size_t)(K > 0 ? K : 1), sizeof(double));
-    for (int64_t i = 0; i < n; ++i) {
-        int64_t c = (int64_t)ids[i] - 1;
-        if (c < 0 || c >= nc) continue;
-        for (int64_t j = 0; j < K; ++j) S[j * nc + c] += r[i] * X[j * n + i];
-    }
+    /* column j of the scores on its own thread, rows in order: values independent of the thread count */
+#pragma omp parallel for schedule(dynamic) if (n * K >= (1 << 18))
+    for (int64_t j = 0; j < K; ++j)
+        for (int64_t i = 0; i < n; ++i) {
+            int64_t c = (int64_t)ids[i] - 1;
+            if (c < 0 || c >= nc) continue;
+            S[j * nc + c] += r[i] * X[j * n + i];
+        }
+#pragma omp parallel for schedule(dynamic) if (nc * K >= (1 << 16))
     for (int64_t a = 0; a < K; ++a)
         for (int64_t b = a; b < K; ++b) {
             double s = 0.0;
@@ -825,6 +806,8 @@ static int fo_estimate_gpu(int64_t n, int64_t K, int64_t L, int64_t n_fe, double
     /* cross products, upper then symmetrised (symmetrize_upper) */
     double* WW = (double*)malloc((size_t)(K * K) * sizeof(double));
     double* Wy = (double*)malloc((size_t)K * sizeof(double));
+    /* threads take whole entries (each dot runs serially), so the values do not depend on their count */
+#pragma omp parallel for schedule(dynamic) if (n * K >= (1 << 18))
     for (int64_t a = 0; a < K; ++a) {
         Wy[a] = fo_dot(W + a * n, y, n);
         for (int64_t b = a; b < K; ++b) {
@@ -877,6 +860,7 @@ static int fo_estimate_gpu(int64_t n, int64_t K, int64_t L, int64_t n_fe, double
         double* Qzz = (double*)malloc((size_t)(n_aug * n_aug) * sizeof(double));
         double* Qzy = (double*)malloc((size_t)n_aug * sizeof(double));
         double* Qzx = (double*)malloc((size_t)(n_aug * K) * sizeof(double));
+#pragma omp parallel for schedule(dynamic) if (n * n_aug >= (1 << 18))
         for (int64_t a = 0; a < n_aug; ++a) {
             Qzy[a] = fo_dot(Za + a * n, y, n);
             for (int64_t b = a; b < n_aug; ++b) {
@@ -907,12 +891,14 @@ static int fo_estimate_gpu(int64_t n, int64_t K, int64_t L, int64_t n_fe, double
             return 0;
         }
         double* proj = (double*)malloc((size_t)(n_aug * K) * sizeof(double));
+#pragma omp parallel for schedule(static)
         for (int64_t k = 0; k < K; ++k)
             for (int64_t a = 0; a < n_aug; ++a) {
                 double s = 0.0;
                 for (int64_t c = 0; c < n_aug; ++c) s += Qzinv[c * n_aug + a] * Qzx[k * n_aug + c];
                 proj[k * n_aug + a] = s;
             }
+#pragma omp parallel for schedule(static)
         for (int64_t p = 0; p < K; ++p)
             for (int64_t q = 0; q < K; ++q) {
                 double s = 0.0;
@@ -931,6 +917,7 @@ static int fo_estimate_gpu(int64_t n, int64_t K, int64_t L, int64_t n_fe, double
             bm[p] = s;
         }
         Xhat = (double*)malloc((size_t)n * (size_t)K * sizeof(double));
+#pragma omp parallel for schedule(dynamic)
         for (int64_t k = 0; k < K; ++k)
             for (int64_t i = 0; i < n; ++i) {
                 double s = 0.0;
@@ -986,6 +973,7 @@ static int fo_estimate_gpu(int64_t n, int64_t K, int64_t L, int64_t n_fe, double
     if (info == 0) {
         /* compute_standard_errors (:1109-1420) */
         double* r = (double*)malloc((size_t)n * sizeof(double));
+#pragma omp parallel for schedule(static) if (n * K >= (1 << 18))
         for (int64_t i = 0; i < n; ++i) {
             double s = y[i];
             for (int64_t k = 0; k < K; ++k) s -= W[k * n + i] * beta[k];
@@ -1040,12 +1028,14 @@ static int fo_estimate_gpu(int64_t n, int64_t K, int64_t L, int64_t n_fe, double
                 res->n_clusters[mask - 1] = nc;
                 fo_meat(idp, nc, r, X, n, K, meat);
                 /* cov += w * Qinv M_kept Qinv */
+#pragma omp parallel for schedule(static) if (kept >= 32)
                 for (int64_t a = 0; a < kept; ++a)
                     for (int64_t b = 0; b < kept; ++b) {
                         double s = 0.0;
                         for (int64_t c = 0; c < kept; ++c) s += meat[keep[c] * K + keep[a]] * Qinv[b * kept + c];
                         tmp[b * kept + a] = s;
                     }
+#pragma omp parallel for schedule(static) if (kept >= 32)
                 for (int64_t a = 0; a < kept; ++a)
                     for (int64_t b = 0; b < kept; ++b) {
                         double s = 0.0;

@@ -354,3 +354,43 @@ def test_server_ticket_takes_one_waiter():
     srv._pending.clear()
     srv.close()
     pan.close()
+
+
+def test_config3_wide_2sls_matches_oracle():
+    """The wide 2SLS workload at a size the oracle finishes in seconds: the device generator's panel
+    (1,500 individuals x 20 periods, 400 firms, 10% switching) with K = 252 design columns (xend and the
+    7 cat8 dummies endogenous, x1..x4, the 29 + 7 + 203 dummies of i.ind&&i.msp, x1&x2), 21 excluded
+    instruments (z1, z2, 19 zcat20 dummies), so L_aug = 244 + 21 = 265; fe(id firm time), cluster(id
+    firm).  It runs the normal equations of both stages on the matrix cores
+    (src/core/fe_pipeline.f90:947-1032, selection :664-753)."""
+    import oracle
+    from fe_gpu_amd import _lib, api
+    from fe_gpu_amd._lib import check
+    from fe_gpu_amd.device import DeviceBuffer
+    ni, T, nf, K, L = 1500, 20, 400, 252, 21
+    n = ni * T
+    note(f"config-3 shape 2SLS n={n} K={K} L={L}")
+    lib = _lib.load()
+    d_fe, d_y = DeviceBuffer(3 * n * 4), DeviceBuffer(n * 8)
+    d_W, d_Z = DeviceBuffer(K * n * 8), DeviceBuffer(L * n * 8)
+    check(lib.fehip_synth_iv_wide(ni, T, nf, 0.10, 303, d_fe.ptr, d_y.ptr, d_W.ptr, d_Z.ptr), "iv wide synth")
+    fe = d_fe.to_array(np.int32, (3, n))
+    y = d_y.to_array(np.float64, n)
+    W = d_W.to_array(np.float64, (n, K), order="F")
+    Z = d_Z.to_array(np.float64, (n, L), order="F")
+    G = fe.max(axis=1)
+    kw = dict(cluster=(1, 2), iv_reg=list(range(1, 9)))
+    pan = api.Panel([d_fe.ptr, d_fe.ptr + n * 4, d_fe.ptr + 2 * n * 4], n)
+    fit = api.estimate_device(pan, d_y.ptr, d_W.ptr, K, d_Z.ptr, L, n, tol=1e-13, max_iter=20000, **kw)
+    pan.close()
+    note("oracle")
+    ref = oracle.estimate(y, W, fe, G=G, Z=Z, tol=1e-13, max_iter=200000, **kw)
+    assert ref.res["is_iv"] == 1 and ref.res["n_instruments"] == 265 and ref.res["dof_model"] == 252
+    for k in ("is_iv", "n_instruments", "dof_model", "dof_fes", "dof_resid", "rank"):
+        assert fit.res[k] == ref.res[k], k
+    assert fit.res["n_clusters"] == ref.res["n_clusters"]
+    assert fit.res["psd_fix_applied"] == ref.res["psd_fix_applied"]
+    assert rel(fit.beta, ref.beta) < 1e-8
+    assert rel(fit.se, ref.se) < 1e-8
+    for k in ("rss", "tss_within", "r2", "f_stat"):
+        assert abs(fit.res[k] - ref.res[k]) <= 1e-8 * max(1.0, abs(ref.res[k])), k
