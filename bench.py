@@ -445,7 +445,7 @@ def main():
     lib.fehip_kstats_enable(1)
     api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kw)
     lib.fehip_kstats_enable(0)
-    ks = {k: kstat(lib, k) for k in range(6)}
+    ks = {k: kstat(lib, k) for k in range(8)}
     dom = ks[0]
     avg_ms = dom["ms"] / max(1, dom["launches"])
     per_launch_bytes = dom["bytes"] / max(1, dom["launches"])
@@ -480,7 +480,7 @@ def main():
         lib.fehip_kstats_enable(1)
         api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kwm)
         lib.fehip_kstats_enable(0)
-        mk = {k: kstat(lib, k) for k in range(6)}
+        mk = {k: kstat(lib, k) for k in range(8)}
         berr = np.abs(mfit.beta - fit.beta) / np.abs(fit.beta)
         serr = np.abs(mfit.se - fit.se) / np.abs(fit.se)
         log(f"mixed precision: {mtotal / args.steps:.3f} s per regression, max beta err {berr.max():.2e}, "

@@ -33,7 +33,7 @@ class Result(C.Structure):
     _fields_ = [("converged", I32), ("fe_iterations", I32), ("solver_info", I32), ("rank", I32),
                 ("dof_model", I32), ("dof_fes", I32), ("is_iv", I32), ("n_instruments", I32),
                 ("dof_resid", I64), ("n_obs", I64), ("n_cluster_dims", I32), ("cluster_dims", I32 * 3),
-                ("n_clusters", I64 * 7), ("min_single_cluster", I64), ("psd_fix_applied", I32), ("reserved", I32),
+                ("n_clusters", I64 * 7), ("min_single_cluster", I64), ("psd_fix_applied", I32), ("fe_refinements", I32),
                 ("rss", C.c_double), ("tss_total", C.c_double), ("tss_within", C.c_double), ("r2", C.c_double),
                 ("r2_adj", C.c_double), ("r2_within", C.c_double), ("f_stat", C.c_double),
                 ("max_update", C.c_double), ("t_demean", C.c_double), ("t_regression", C.c_double),

@@ -129,6 +129,9 @@ __device__ __forceinline__ void lds_addf(float* p, float v) {
 __device__ __forceinline__ double ld_global(const double* p) {
     return *(const __attribute__((address_space(1))) double*)(p);
 }
+__device__ __forceinline__ float ld_global(const float* p) {
+    return *(const __attribute__((address_space(1))) float*)(p);
+}
 __device__ __forceinline__ double ld_lds(const double* p) {
     return *(const __attribute__((address_space(3))) double*)(p);
 }

@@ -2618,6 +2618,7 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
             ws.col_iters[c2] = wsu.col_iters[u];
             ws.col_converged[c2] = wsu.col_converged[u];
             ws.col_max_update[c2] = wsu.col_max_update[u];
+            ws.col_refines[c2] = wsu.col_refines[u];
             if (src == c2 || share) continue;
             if (dst[c2].out)
                 FEHIP_CHECK(hipMemcpyAsync(dst[c2].out, dst[src].out, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, 0),
@@ -2640,6 +2641,7 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
                 s.result->fe_iterations = std::max(s.result->fe_iterations, ws.col_iters[c]);
                 s.result->converged = s.result->converged && ws.col_converged[c];
                 s.result->max_update = std::max(s.result->max_update, ws.col_max_update[c]);
+                s.result->fe_refinements = std::max(s.result->fe_refinements, ws.col_refines[c]);
             }
             s.t1 = t1;
             s.result->t_demean = std::chrono::duration<double>(t1 - s.t0).count();

@@ -8,7 +8,12 @@
 
 namespace fehip {
 
-enum KStatId { kStatApplyIter = 0, kStatApplyData, kStatFinalize, kStatReduceRuns, kStatGram, kStatSegsum, kStatCount };
+// apply_iter32: the FP32 operator product of the mixed-precision solve; apply_resid: its FP64 products
+// for the true residual b - S x
+enum KStatId {
+    kStatApplyIter = 0, kStatApplyData, kStatFinalize, kStatReduceRuns, kStatGram, kStatSegsum, kStatApplyIter32,
+    kStatApplyResid, kStatCount
+};
 
 bool kstats_enabled();
 // Record a start event before a launch and the stop event after it; bytes/flops are the

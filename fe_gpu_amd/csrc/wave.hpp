@@ -91,3 +91,71 @@ __device__ __forceinline__ double seg_scan(double x, const ScanMask& m) {
 __device__ __forceinline__ unsigned long long mask_le(int lane) { return (2ull << lane) - 1ull; }
 
 }  // namespace fehip
+
+namespace fehip {
+
+// ---- FP32 segmented scans: one fused multiply-add with a DPP source per column and step ----
+// x += dpp(x) * f in one v_fmac_f32_dpp (src0 moved across lanes, src1 the lane's 0/1 factor).  The
+// backend does not fold a 32-bit DPP move into the multiply-add, so the step is written out here.  A
+// DPP read of a VGPR needs two wait states after the VALU write of it: each block starts with
+// s_nop 1, and inside a block every column is read once and written once, in order.  Lanes without a
+// source lane keep their value (the factor is 0 there anyway); all 64 lanes must be active.
+#define FEHIP_FMAC_DPP(i, f, ctrl) "v_fmac_f32_dpp %" #i ", %" #i ", %" #f " " ctrl " row_mask:0xf bank_mask:0xf\n\t"
+
+#define FEHIP_SCAN_STEP_FNS(NAME, CTRL)                                                                              \
+    __device__ __forceinline__ void NAME##4(float& a, float& b, float& c, float& d, float f) {                      \
+        asm("s_nop 1\n\t" FEHIP_FMAC_DPP(0, 4, CTRL) FEHIP_FMAC_DPP(1, 4, CTRL) FEHIP_FMAC_DPP(2, 4, CTRL)          \
+                FEHIP_FMAC_DPP(3, 4, CTRL)                                                                           \
+            : "+v"(a), "+v"(b), "+v"(c), "+v"(d)                                                                     \
+            : "v"(f));                                                                                               \
+    }                                                                                                                \
+    __device__ __forceinline__ void NAME##2(float& a, float& b, float f) {                                          \
+        asm("s_nop 1\n\t" FEHIP_FMAC_DPP(0, 2, CTRL) FEHIP_FMAC_DPP(1, 2, CTRL) : "+v"(a), "+v"(b) : "v"(f));        \
+    }                                                                                                                \
+    __device__ __forceinline__ void NAME##1(float& a, float f) {                                                    \
+        asm("s_nop 1\n\t" FEHIP_FMAC_DPP(0, 1, CTRL) : "+v"(a) : "v"(f));                                            \
+    }
+
+FEHIP_SCAN_STEP_FNS(scan_shr1_, "row_shr:1")
+FEHIP_SCAN_STEP_FNS(scan_shr2_, "row_shr:2")
+FEHIP_SCAN_STEP_FNS(scan_shr4_, "row_shr:4")
+FEHIP_SCAN_STEP_FNS(scan_shr8_, "row_shr:8")
+FEHIP_SCAN_STEP_FNS(scan_bc15_, "row_bcast:15")
+FEHIP_SCAN_STEP_FNS(scan_bc31_, "row_bcast:31")
+#undef FEHIP_SCAN_STEP_FNS
+#undef FEHIP_FMAC_DPP
+
+#define FEHIP_SCAN_COLS(NAME, F)                                         \
+    {                                                                    \
+        int c = 0;                                                       \
+        for (; c + 4 <= N; c += 4) NAME##4(x[c], x[c + 1], x[c + 2], x[c + 3], F); \
+        if (c + 2 <= N) {                                                \
+            NAME##2(x[c], x[c + 1], F);                                  \
+            c += 2;                                                      \
+        }                                                                \
+        if (c < N) NAME##1(x[c], F);                                     \
+    }
+
+struct ScanFactorsF {
+    float f1, f2, f4, f8, f15, f31;
+    __device__ __forceinline__ explicit ScanFactorsF(const ScanMask& m)
+        : f1(m.m1 ? 1.0f : 0.0f), f2(m.m2 ? 1.0f : 0.0f), f4(m.m4 ? 1.0f : 0.0f), f8(m.m8 ? 1.0f : 0.0f),
+          f15(m.b15 ? 1.0f : 0.0f), f31(m.b31 ? 1.0f : 0.0f) {}
+};
+
+// the segmented inclusive scan of seg_scan_cols on N FP32 columns
+template <int N>
+__device__ __forceinline__ void seg_scan_cols_f32(float (&x)[N], const ScanFactorsF& f) {
+#pragma unroll
+    for (int step = 0; step < 1; ++step) {
+        FEHIP_SCAN_COLS(scan_shr1_, f.f1)
+        FEHIP_SCAN_COLS(scan_shr2_, f.f2)
+        FEHIP_SCAN_COLS(scan_shr4_, f.f4)
+        FEHIP_SCAN_COLS(scan_shr8_, f.f8)
+        FEHIP_SCAN_COLS(scan_bc15_, f.f15)
+        FEHIP_SCAN_COLS(scan_bc31_, f.f31)
+    }
+}
+#undef FEHIP_SCAN_COLS
+
+}  // namespace fehip

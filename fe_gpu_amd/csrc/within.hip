@@ -52,6 +52,7 @@ struct ScatterDim {
     const int* run_dest;         // large: group-order position of each run
     const int* row_rdest;        // large: per row, run_dest of the run it ends (-1 inside a run)
     double* runbuf;              // large: [n_runs][CT] in group order
+    float* runbuf32;             // large: [n_runs][CT] in group order, the FP32 operator's run partials
 };
 
 struct ApplyArgs {
@@ -86,6 +87,12 @@ struct ApplyArgs {
     const double* alpha;       // kFinal: solved tables [S][CT]
     int to_work;               // kFinal: 1 = panel-order work columns, 0 = caller columns in place
     double* gram;              // kFinal: per-block 16 x 16 cross-product records (nullptr: none)
+    // kIter on a column tile whose columns share one row slot: the lean operator kernel (wave_op_kernel)
+    int lean;                  // 1: wave tiles through wave_op_kernel
+    uint32_t ubit;             // the tile's slot bit (~0u without row masks)
+    int op32;                  // 1: FP32 direction tables (p32) and run partials (runbuf32)
+    const float* p32;          // kIter with op32: [S][CT rounded up to 4] FP32 copy of p
+    double* dump;              // lean kernel: per-wave slots [waves][kMaxCT] that lanes ending no run write
 };
 
 // The final pass holds every row's transformed values of the tile's columns in FP64 registers, so
@@ -480,6 +487,175 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     } else {
         if (a.gram) gram_store_block<CT>(gstage, gram_stage<CT>(), gacc, a.gram + (size_t)(a.part_base + blockIdx.x) * 256);
     }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Lean operator product q = S p over the wave tiles, for a column tile whose columns share one row slot
+// (every solve of a single request).  The slot's row counts, the run geometry of each large dim and the
+// segment bounds are formed once per tile rather than once per column, and every column of the tile is
+// computed (a column whose CG has stopped costs its share of the work; its output is not read).
+// TS is the storage of the direction tables and the run partials:
+//   double -- the FP64 operator;
+//   float  -- the FP32 operator of the mixed-precision solve: p is gathered from an FP32 copy whose rows
+//             are padded to 16 bytes, the row sums, segmented scans (one fused DPP multiply-add per column
+//             and step) and run sums are FP32, the small dims' group sums accumulate in FP64 in LDS, and
+//             the FP32 run partials are summed per group in FP64 (reduce_runs32_kernel).  Its rounding is
+//             corrected by FP64 products of the true residual in the solve around it.
+template <typename TS>
+struct OpStore {
+    static constexpr int pad(int ct) { return sizeof(TS) == 4 ? (ct + 3) & ~3 : ct; }
+};
+
+__device__ __align__(16) float g_zero_row32[kMaxCT];
+
+__device__ __forceinline__ float lane_value(float x, int addr) {
+    return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
+}
+
+template <int CT, int ND, typename TS>
+__global__ __launch_bounds__(kBlock) void wave_op_kernel(ApplyArgs a) {
+    constexpr bool kF32 = sizeof(TS) == 4;
+    constexpr int CTP = OpStore<TS>::pad(CT);  // table row stride
+    if (!any_active<CT>(a.active)) return;
+    const TS* tab = kF32 ? reinterpret_cast<const TS*>(a.p32) : reinterpret_cast<const TS*>(a.p);
+    const TS* zrow = kF32 ? reinterpret_cast<const TS*>(g_zero_row32) : reinterpret_cast<const TS*>(g_zero_row);
+    extern __shared__ double dyn[];
+    double* sacc = dyn;  // [Gs][CT] small-dim group sums
+    for (int e = threadIdx.x; e < a.Gs * CT; e += blockDim.x) sacc[e] = 0.0;
+    __shared__ double inv_count[kWave + 1];
+    if (threadIdx.x <= kWave) inv_count[threadIdx.x] = threadIdx.x > 0 ? 1.0 / (double)threadIdx.x : 0.0;
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    const uint32_t ubit = a.ubit;
+    int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    TS* const wdump = reinterpret_cast<TS*>(a.dump) + t * kMaxCT;
+    // the same software pipeline as wave_tile_kernel: records two tiles ahead, rows one tile ahead
+    WaveRows<ND> cur, nxt;
+    load_wave_rows(a, load_wave_rec(a, t), lane, cur);
+    int64_t rec_n = load_wave_rec(a, t + wstride);
+    for (; t < a.n_wtiles; t += wstride) {
+        const int nr = (int)(cur.rec >> 32);
+        const bool valid = lane < nr;
+        const uint32_t word = valid ? (a.wm ? cur.word : ~0u) : 0u;
+        const bool on = (word & ubit) != 0;
+        const int ip = valid ? cur.ip : -1;
+        int gid[ND], rdl[ND];
+#pragma unroll
+        for (int o = 0; o < ND; ++o) {
+            gid[o] = (o < a.nd && valid) ? cur.g[o] : -1;
+            rdl[o] = cur.rd[o];  // used only on lanes that end a run (valid ones)
+        }
+        // the tile's gathers first (rows past the tile and dims past a.nd read the zero row), then the
+        // next tiles' record and rows
+        TS tv[ND][CT];
+#pragma unroll
+        for (int o = 0; o < ND; ++o) {
+            const int64_t off = (a.sd[o].tab_off + (gid[o] >= 0 ? gid[o] : 0)) * CTP;
+            const TS* src = gid[o] >= 0 ? tab + off : zrow;
+#pragma unroll
+            for (int c = 0; c < CT; ++c) tv[o][c] = ld_global(src + c);
+        }
+        const int64_t rec_nn = load_wave_rec(a, t + 2 * wstride);
+        load_wave_rows(a, rec_n, lane, nxt);
+        rec_n = rec_nn;
+        cur = nxt;
+        __builtin_amdgcn_sched_barrier(0);
+        TS v[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            TS sum = tv[0][c];
+#pragma unroll
+            for (int o = 1; o < ND; ++o) sum += tv[o][c];
+            v[c] = on ? sum : (TS)0;  // rows outside the slot count as zero
+        }
+        // segment of this lane: [ss, se]
+        const int ipp = __shfl_up(ip, 1, kWave);
+        const bool hs = valid && (lane == 0 || ip != ipp);
+        const unsigned long long H = __ballot(hs);
+        const int ss = 63 - __clzll(H & mask_le(lane));
+        const unsigned long long nh = H & ~mask_le(lane);
+        const int se = nh ? __ffsll((long long)nh) - 2 : nr - 1;
+        TS x[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) x[c] = v[c];
+        const ScanMask smask(lane, ss);
+        if constexpr (kF32)
+            seg_scan_cols_f32<CT>(x, ScanFactorsF(smask));
+        else
+            seg_scan_cols<CT>(x, ScanFactors(smask));
+        const unsigned long long segmask = mask_le(se) & ~((1ull << ss) - 1ull);
+        const unsigned long long B = __ballot(on);
+        const TS inv = (TS)ld_lds(&inv_count[__popcll(B & segmask)]);
+        const int se_addr = se << 2;
+        // per scattered dim: the small dim's LDS row, or the large dim's run geometry
+        double* tq[ND];
+        TS* dst[ND];
+        int pl_addr[ND];
+        TS keep_pre[ND], rcnt[ND];
+        bool rend[ND];
+#pragma unroll
+        for (int o = 0; o < ND; ++o) {
+            tq[o] = sacc;
+            dst[o] = nullptr;
+            pl_addr[o] = 0;
+            keep_pre[o] = rcnt[o] = (TS)0;
+            rend[o] = false;
+            if (o >= a.nd) continue;
+            const ScatterDim& d = a.sd[o];
+            const int g = gid[o];
+            if (d.small) {
+                tq[o] = sacc + (size_t)(d.lds_off + (on ? g : 0)) * CT;
+                continue;
+            }
+            const int gp = __shfl_up(g, 1, kWave);
+            const bool rh = valid && (hs || g != gp);
+            const unsigned long long HR = __ballot(rh);
+            const int rs = 63 - __clzll(HR & mask_le(lane));
+            rend[o] = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
+            pl_addr[o] = (rs > 0 ? rs - 1 : 0) << 2;
+            const unsigned long long runmask = mask_le(lane) & ~((1ull << rs) - 1ull);
+            keep_pre[o] = rs == ss ? (TS)0 : (TS)1;
+            rcnt[o] = (TS)__popcll(B & runmask);
+            TS* rb = kF32 ? reinterpret_cast<TS*>(d.runbuf32) : reinterpret_cast<TS*>(d.runbuf);
+            // lanes ending no run write the wave's dump slot: every lane stores, so every path issues the
+            // same memory instructions and the wait for the next tile's prefetched rows never waits for
+            // this tile's stores
+            dst[o] = rend[o] ? rb + (int64_t)rdl[o] * CT : wdump;
+        }
+        // segment means of every column (the lane permutes issued back to back), the rows' values u, then
+        // per dim the small-dim adds or the run sums; the branches are uniform and outside the column loops
+        TS mean[CT];
+        if (a.elim) {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) mean[c] = lane_value(x[c], se_addr);
+#pragma unroll
+            for (int c = 0; c < CT; ++c) mean[c] *= inv;
+        } else {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) mean[c] = (TS)0;
+        }
+#pragma unroll
+        for (int o = 0; o < ND; ++o) {
+            if (o >= a.nd) break;
+            if (a.sd[o].small) {
+                if (on) {
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) lds_add(&tq[o][c], (double)(v[c] - mean[c]));
+                }
+            } else {
+                // run sum: x(run end) - x(run start - 1) - (rows of the slot in the run) * mean
+                TS pre[CT];
+#pragma unroll
+                for (int c = 0; c < CT; ++c) pre[c] = lane_value(x[c], pl_addr[o]);
+#pragma unroll
+                for (int c = 0; c < CT; ++c) dst[o][c] = fma(-rcnt[o], mean[c], fma(-pre[c], keep_pre[o], x[c]));
+            }
+        }
+    }
+    __syncthreads();
+    double* part = a.small_part + (size_t)(a.part_base + blockIdx.x) * a.Gs * CT;
+    for (int e = threadIdx.x; e < a.Gs * CT; e += blockDim.x) part[e] = sacc[e];
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -905,6 +1081,29 @@ __global__ void reduce_runs_kernel(const double* __restrict__ runbuf, const int6
     }
 }
 
+// q[g][c] = FP64 sum of the FP32 run partials of group g (runs in group order)
+template <int CT>
+__global__ void reduce_runs32_kernel(const float* __restrict__ runbuf, const int64_t* __restrict__ goff, int G,
+                                     double* q, const int* __restrict__ active) {
+    if (!any_active<CT>(active)) return;
+    const int64_t total = (int64_t)G * CT;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const int g = (int)(e / CT), c = (int)(e % CT);
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        const int64_t k0 = goff[g], k1 = goff[g + 1];
+        int64_t k = k0;
+        for (; k + 4 <= k1; k += 4) {
+            s0 += (double)runbuf[k * CT + c];
+            s1 += (double)runbuf[(k + 1) * CT + c];
+            s2 += (double)runbuf[(k + 2) * CT + c];
+            s3 += (double)runbuf[(k + 3) * CT + c];
+        }
+        for (; k < k1; ++k) s0 += (double)runbuf[k * CT + c];
+        q[e] = (s0 + s1) + (s2 + s3);
+    }
+}
+
 // one workgroup per table entry: the threads stride over the per-block partials in a fixed order and
 // combine them with a fixed-shape tree
 template <int CT>
@@ -927,10 +1126,20 @@ __global__ __launch_bounds__(kBlock) void reduce_small_kernel(const double* __re
 }
 
 // ---- CG in table space ([S][CT], one CG per column) ---------------------------------------------
+// tol_in: the level at which a round of a column's CG is done -- tol, or in a round of the mixed-precision
+// solve max(tol, delta * the criterion of the round's true residual).  A column stops (active = 0) once
+// its criterion is below tol; below tol_in only `reached` is set and the column goes on iterating until
+// every column of the tile has reached its level, so no column waits idle for the slowest.
 struct CgState {
-    double rz[kMaxCT], alpha[kMaxCT], beta[kMaxCT], maxn[kMaxCT];
-    int active[kMaxCT], iters[kMaxCT];
+    double rz[kMaxCT], alpha[kMaxCT], beta[kMaxCT], maxn[kMaxCT], tol_in[kMaxCT];
+    int active[kMaxCT], iters[kMaxCT], reached[kMaxCT];
 };
+
+// FP32 copy of the direction p for the FP32 operator (rows padded to 16 bytes)
+template <int CT>
+__device__ __forceinline__ void store_p32(float* p32, int64_t g, int c, double v) {
+    if (p32) p32[g * ((CT + 3) & ~3) + c] = (float)v;
+}
 
 template <int CT>
 __device__ __forceinline__ void block_rz_max(const double (&rz)[CT], const double (&mx)[CT], double* part_rz,
@@ -957,12 +1166,16 @@ __device__ __forceinline__ void block_rz_max(const double (&rz)[CT], const doubl
     }
 }
 
+// CG start: z = M^-1 r, p = z (and its FP32 copy), x = 0.  With b and q given (the reliable update of
+// the mixed-precision solve), r = b - q, the true residual of the current x, and z = M^-1 r only: x and
+// the direction p are kept (cg_reliable_scalars and cg_pdir_kernel continue the same CG).
 template <int CT>
-__global__ __launch_bounds__(kBlock) void cg_init_kernel(int64_t S, const double* __restrict__ r,
+__global__ __launch_bounds__(kBlock) void cg_init_kernel(int64_t S, double* __restrict__ r,
                                                          const double* __restrict__ minv,
                                                          const double* __restrict__ wcnt, double* __restrict__ z,
                                                          double* __restrict__ p, double* __restrict__ x,
-                                                         double* part_rz, double* part_max) {
+                                                         double* part_rz, double* part_max, float* p32,
+                                                         const double* __restrict__ b, const double* __restrict__ q) {
     double rz[CT], mx[CT];
 #pragma unroll
     for (int c = 0; c < CT; ++c) rz[c] = mx[c] = 0.0;
@@ -970,11 +1183,20 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(int64_t S, const double
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
             const int64_t e = g * CT + c;
-            const double rv = r[e];
+            double rv;
+            if (b) {
+                rv = b[e] - q[e];
+                r[e] = rv;
+            } else {
+                rv = r[e];
+                x[e] = 0.0;
+            }
             const double zv = minv[e] * rv;
             z[e] = zv;
-            p[e] = zv;
-            x[e] = 0.0;
+            if (!b) {
+                p[e] = zv;
+                store_p32<CT>(p32, g, c, zv);
+            }
             rz[c] += rv * zv;
             const double n = wcnt[e];
             if (n > 0.0) mx[c] = fmax(mx[c], fabs(rv) / n);
@@ -1022,20 +1244,52 @@ __device__ __forceinline__ void block_reduce_cols(const double* __restrict__ par
 
 template <int CT>
 __global__ __launch_bounds__(kBlock) void cg_start_scalars(const double* part_rz, const double* part_max, int nblk,
-                                                           double tol, int ncols, CgState* st) {
+                                                           double tol, double delta, int ncols, int first,
+                                                           CgState* st) {
     __shared__ double sum[CT], mx[CT];
     block_reduce_cols<CT>(part_rz, part_max, nblk, sum, mx);
     const int c = threadIdx.x;
     if (c >= kMaxCT) return;
     if (c >= CT) {
         st->active[c] = 0;
+        st->reached[c] = 1;
         return;
     }
     st->rz[c] = sum[c];
     st->maxn[c] = mx[c];
-    st->iters[c] = 0;
+    st->tol_in[c] = fmax(tol, delta * mx[c]);
+    if (first) st->iters[c] = 0;  // a restart keeps counting the steps
     st->active[c] = (c < ncols) && (mx[c] >= tol) && sum[c] > 0.0;
+    st->reached[c] = !st->active[c];
     st->alpha[c] = st->beta[c] = 0.0;
+}
+
+// Reliable update: the recursive residual of each column is replaced by its true residual (cg_init_kernel
+// with b, q), and the CG goes on with the same direction: beta = r'z' / r_k z_k, where r_k z_k is the
+// product before the column's last step (beta and rz of that step are kept in the state), so in exact
+// arithmetic the iteration is that of a CG with no update at all.  A column continues while the
+// criterion of its true residual is at least tol; its next round stops at max(tol, delta x criterion).
+template <int CT>
+__global__ __launch_bounds__(kBlock) void cg_reliable_scalars(const double* part_rz, const double* part_max, int nblk,
+                                                              double tol, double delta, int ncols, CgState* st) {
+    __shared__ double sum[CT], mx[CT];
+    block_reduce_cols<CT>(part_rz, part_max, nblk, sum, mx);
+    const int c = threadIdx.x;
+    if (c >= CT) return;
+    const double rz = sum[c];
+    st->beta[c] = st->rz[c] > 0.0 ? st->beta[c] * (rz / st->rz[c]) : 0.0;
+    st->rz[c] = rz;
+    st->maxn[c] = mx[c];
+    st->tol_in[c] = fmax(tol, delta * mx[c]);
+    st->active[c] = (c < ncols) && (mx[c] >= tol) && rz > 0.0;
+    st->reached[c] = !st->active[c];
+}
+
+// the FP64 operator takes over: each column's rounds stop at tol itself
+template <int CT>
+__global__ void cg_reliable_tol(double tol, CgState* st) {
+    const int c = threadIdx.x;
+    if (c < CT) st->tol_in[c] = tol;
 }
 
 template <int CT>
@@ -1137,11 +1391,16 @@ __global__ __launch_bounds__(kBlock) void cg_beta_scalars(const double* part_rz,
     st->maxn[c] = mx;
     st->beta[c] = st->rz[c] > 0.0 ? rz / st->rz[c] : 0.0;
     st->rz[c] = rz;
-    if (mx < tol || !(rz > 0.0)) st->active[c] = 0;
+    if (mx < st->tol_in[c]) st->reached[c] = 1;
+    if (mx < tol || !(rz > 0.0)) {
+        st->active[c] = 0;
+        st->reached[c] = 1;
+    }
 }
 
 template <int CT>
-__global__ void cg_pdir_kernel(int64_t S, const CgState* __restrict__ st, const double* __restrict__ z, double* p) {
+__global__ void cg_pdir_kernel(int64_t S, const CgState* __restrict__ st, const double* __restrict__ z, double* p,
+                               float* p32) {
     if (!any_active<CT>(st->active)) return;
     double be[CT];
     int act[CT];
@@ -1153,7 +1412,11 @@ __global__ void cg_pdir_kernel(int64_t S, const CgState* __restrict__ st, const 
     for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x)
 #pragma unroll
         for (int c = 0; c < CT; ++c)
-            if (act[c]) p[g * CT + c] = z[g * CT + c] + be[c] * p[g * CT + c];
+            if (act[c]) {
+                const double pv = z[g * CT + c] + be[c] * p[g * CT + c];
+                p[g * CT + c] = pv;
+                store_p32<CT>(p32, g, c, pv);
+            }
 }
 
 // Per-column preconditioner and weighted counts from the per-slot Schur diagonals ([slot][S][2]).
@@ -1282,11 +1545,19 @@ void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, 
         } else {
             const DimStruct& D = p->d[L.dims[o]];
             if (time_runs) kstat_begin(kStatReduceRuns, s);
-            hipLaunchKernelGGL((reduce_runs_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock, 32)), dim3(kBlock), 0,
-                               s, (const double*)a.sd[o].runbuf, (const int64_t*)D.grp_run_off, d.G, qo, active);
+            if (a.op32)
+                hipLaunchKernelGGL((reduce_runs32_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock, 32)),
+                                   dim3(kBlock), 0, s, (const float*)a.sd[o].runbuf32, (const int64_t*)D.grp_run_off,
+                                   d.G, qo, active);
+            else
+                hipLaunchKernelGGL((reduce_runs_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock, 32)),
+                                   dim3(kBlock), 0, s, (const double*)a.sd[o].runbuf, (const int64_t*)D.grp_run_off,
+                                   d.G, qo, active);
             if (time_runs)
                 kstat_end(kStatReduceRuns, s,
-                          on > 0.0 ? 8.0 * on * (double)(D.n_runs + d.G) + 8.0 * (double)(d.G + 1) : 0.0);
+                          on > 0.0 ? (a.op32 ? 4.0 : 8.0) * on * (double)D.n_runs + 8.0 * on * (double)d.G +
+                                         8.0 * (double)(d.G + 1)
+                                   : 0.0);
         }
     }
 }
@@ -1299,7 +1570,25 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
     size_t lt = tables_lds<CT>(L);
     if (MODE == kFinal && a.gram) lt += sizeof(double) * (size_t)(kBlock / kWave) * gram_stage<CT>();
     a.part_base = 0;
-    if (gw > 0) {
+    if (MODE == kIter && a.lean && gw > 0) {
+        // the lean operator kernel: FP64 or FP32 storage, by the scattered dims at compile time
+        const size_t lo = sizeof(double) * (size_t)L.Gs * CT;
+#define FEHIP_OP_LAUNCH(NDV, TSV)                                                                                 \
+    do {                                                                                                         \
+        gw = std::min(gw, 2u * resident_grid(wave_op_kernel<CT, NDV, TSV>, kBlock, lo, gw));                     \
+        hipLaunchKernelGGL((wave_op_kernel<CT, NDV, TSV>), dim3(gw), dim3(kBlock), lo, s, a);                    \
+    } while (0)
+        if (a.op32) {
+            if (a.nd <= 2) FEHIP_OP_LAUNCH(2, float);
+            else if (a.nd == 3) FEHIP_OP_LAUNCH(3, float);
+            else FEHIP_OP_LAUNCH(kMaxScatter, float);
+        } else {
+            if (a.nd <= 2) FEHIP_OP_LAUNCH(2, double);
+            else if (a.nd == 3) FEHIP_OP_LAUNCH(3, double);
+            else FEHIP_OP_LAUNCH(kMaxScatter, double);
+        }
+#undef FEHIP_OP_LAUNCH
+    } else if (gw > 0) {
         // two rounds of resident workgroups, so that workgroups of the second round fill the places of
         // the first round's early finishers (at most the panel's grid, which sizes the partial buffers)
         if (a.nd <= 2) {
@@ -1333,8 +1622,9 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
 // whose CG is still running, which the caller knows only when it has polled the state just before;
 // -1 = all), used for the algorithmic byte count of per-kernel timing.
 template <int CT>
-int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* q, hipStream_t s, int n_on = -1) {
-    const int sid = mode == kIter ? kStatApplyIter : kStatApplyData;
+int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* q, hipStream_t s, int n_on = -1,
+                     int stat = -1) {
+    const int sid = stat >= 0 ? stat : mode == kIter ? (a.op32 ? kStatApplyIter32 : kStatApplyIter) : kStatApplyData;
     const bool timed = kstats_enabled();
     if (timed) kstat_begin(sid, s);
     const unsigned grid = mode == kIter ? launch_pass<CT, kIter>(p, L, a, s) : launch_pass<CT, kData>(p, L, a, s);
@@ -1346,10 +1636,11 @@ int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* 
         // data columns and the permutation for the right-hand side); a launch after every column has
         // converged returns at once and moves nothing
         const double n = (double)p->n;
+        const double es = a.op32 ? 4.0 : 8.0;  // bytes of a table entry and of a run partial
         double bytes = n * 4.0 * (L.nd + 1) + (a.wm ? 4.0 * n : 0.0) + 8.0 * (double)grid * L.Gs * on;
         for (int o = 0; o < L.nd; ++o) {
-            bytes += 8.0 * on * (double)L.sd[o].G;
-            if (!L.sd[o].small) bytes += (8.0 * on + 4.0) * (double)p->d[L.dims[o]].n_runs;
+            bytes += es * on * (double)L.sd[o].G;
+            if (!L.sd[o].small) bytes += (es * on + 4.0) * (double)p->d[L.dims[o]].n_runs;
         }
         if (mode == kData) bytes += n * 8.0 * a.ncols + (p->perm ? 4.0 * n : 0.0);
         kstat_end(sid, s, on > 0.0 ? bytes : 0.0);
@@ -1364,6 +1655,10 @@ struct Scratch {
     double* tab = nullptr;      // CG arrays
     double* dg = nullptr;       // per-slot Schur diagonals and weighted counts [slot][S][2]
     double* runbuf[kMaxLarge] = {nullptr, nullptr, nullptr};
+    float* runbuf32[kMaxLarge] = {nullptr, nullptr, nullptr};  // FP32 operator: run partials
+    float* p32 = nullptr;       // FP32 operator: the direction's FP32 copy [S][16]
+    double* bvec = nullptr;     // FP32 operator: the right-hand side [S][CT], kept for true residuals
+    double* dump = nullptr;     // lean operator kernel: per-wave slots of the lanes that end no run
     double* small_part = nullptr;
     double* part = nullptr;     // reduction partials
     double* gram_rec = nullptr; // per-block cross-product records of the final pass
@@ -1381,6 +1676,10 @@ struct Scratch {
 };
 
 constexpr int kCgBlocks = 2048;  // CG vector kernels: eight waves per SIMD keep the table loads in flight
+// A round of the FP32-operator PCG stops once its residual criterion is this fraction of the round's true
+// (FP64) residual: well above the FP32 operator's relative rounding (~1e-6), so each round's FP64
+// residual falls by about this factor.
+constexpr double kInnerRel = 1e-4;
 
 ApplyArgs base_args(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm) {
     ApplyArgs a{};
@@ -1391,7 +1690,10 @@ ApplyArgs base_args(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm) 
     a.nd = L.nd;
     for (int i = 0; i < L.nd; ++i) {
         a.sd[i] = L.sd[i];
-        if (!L.sd[i].small) a.sd[i].runbuf = sc.runbuf[L.sd[i].large_slot];
+        if (!L.sd[i].small) {
+            a.sd[i].runbuf = sc.runbuf[L.sd[i].large_slot];
+            a.sd[i].runbuf32 = sc.runbuf32[L.sd[i].large_slot];
+        }
     }
     a.Gs = L.Gs;
     a.perm = p->perm;
@@ -1404,6 +1706,7 @@ ApplyArgs base_args(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm) 
     a.n_wtiles = p->n_wtiles;
     a.long_segs = p->long_segs;
     a.n_long = p->n_long;
+    a.dump = sc.dump;
     return a;
 }
 
@@ -1481,19 +1784,35 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
     auto t0 = std::chrono::steady_clock::now();
     ts = TileStats();
     for (int c = 0; c < ncols; ++c) ts.col_converged[c] = 1;
+    // one slot for every column of the tile: the lean operator kernel
+    bool uniform = true;
+    for (int c = 1; c < ncols; ++c) uniform = uniform && wc.slot[c0 + c] == wc.slot[c0];
+    a.lean = uniform ? 1 : 0;
+    a.ubit = wm ? (1u << wc.slot[c0]) : ~0u;
+    // the FP32 operator (mixed precision asked, one slot, every primary segment in wave tiles): rounds of
+    // FP32-operator PCG, each stopped once its own residual is delta times the round's true residual,
+    // with the true residual b - S x of the FP64 operator between rounds; the stopping rule is checked
+    // on that FP64 residual only, so the result meets the same criterion as the FP64 solve
+    const bool op32 = o.precision == 1 && uniform && sc.p32 != nullptr;
+    const double delta = op32 ? kInnerRel : 0.0;
     if (S > 0) {
         hipLaunchKernelGGL((make_minv_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, s, S,
                            (const double*)sc.dg, cs, minv, wcnt);
         FEHIP_LAUNCH_CHECK("minv");
         int e = apply_and_reduce<CT>(p, L, a, kData, r, s);  // b = D_o' W M_P x
         if (e) return e;
-        hipLaunchKernelGGL((cg_init_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, (const double*)r,
-                           (const double*)minv, (const double*)wcnt, z, pp, xa, sc.part, sc.part + kCgBlocks * CT);
+        if (op32) FEHIP_CHECK(hipMemcpyAsync(sc.bvec, r, sizeof(double) * (size_t)(S * CT), hipMemcpyDeviceToDevice, s),
+                              "right-hand side copy");
+        hipLaunchKernelGGL((cg_init_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, r, (const double*)minv,
+                           (const double*)wcnt, z, pp, xa, sc.part, sc.part + kCgBlocks * CT, op32 ? sc.p32 : nullptr,
+                           (const double*)nullptr, (const double*)nullptr);
         hipLaunchKernelGGL((cg_start_scalars<CT>), dim3(1), dim3(kBlock), 0, s, (const double*)sc.part,
-                           (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, ncols, sc.st);
+                           (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, delta, ncols, 1, sc.st);
         FEHIP_LAUNCH_CHECK("cg init");
         const int every = o.check_every > 0 ? o.check_every : 4;
         a.p = pp;
+        a.op32 = op32 ? 1 : 0;
+        a.p32 = op32 ? sc.p32 : nullptr;
         a.active = sc.st->active;
         // one CG step: operator product, p.q, alpha, update, r.z, beta, new direction
         int n_on = -1;  // columns still running (known under per-kernel timing, which polls every step)
@@ -1510,7 +1829,7 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             hipLaunchKernelGGL((cg_beta_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part,
                                (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, sc.st);
             hipLaunchKernelGGL((cg_pdir_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, st, S,
-                               (const CgState*)sc.st, (const double*)z, pp);
+                               (const CgState*)sc.st, (const double*)z, pp, a.op32 ? sc.p32 : nullptr);
             FEHIP_LAUNCH_CHECK("cg step");
             return 0;
         };
@@ -1550,30 +1869,81 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             (void)hipGraphDestroy(graph);
             FEHIP_CHECK(hi, "cg graph");
         }
-        int it = 0;
-        while (it < o.max_iter) {
-            if (it % every == 0) {
-                FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs), "cg state");
-                FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
-                bool any = false;
-                for (int c = 0; c < CT; ++c) any |= sc.st_host->active[c] != 0;
-                if (!any) break;
+        int it = 0;       // CG steps of every round
+        int refines = 0;  // FP64 true-residual products (FP32 operator)
+        double prev_worst = HUGE_VAL;
+        for (;;) {
+            int j = 0;  // steps of this round (the graph holds `every` of them)
+            while (it < o.max_iter) {
+                if (j % every == 0) {
+                    FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs),
+                                "cg state");
+                    FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
+                    bool pending = false;  // a column still short of this round's level
+                    for (int c = 0; c < CT; ++c) pending |= sc.st_host->active[c] != 0 && !sc.st_host->reached[c];
+                    if (!pending) break;
+                }
+                if (!gexec && kstats_enabled()) {
+                    // per-kernel timing: the state is read before every step so the step's byte count covers
+                    // the columns it works on; the launches are those of the untimed path
+                    FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs),
+                                "cg state");
+                    FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
+                    n_on = 0;
+                    for (int c = 0; c < ncols; ++c) n_on += sc.st_host->active[c] != 0;
+                }
+                if (gexec && j % every == 0 && it + every <= o.max_iter) {
+                    FEHIP_CHECK(hipGraphLaunch(gexec, gs), "cg graph launch");
+                    it += every;
+                    j += every;
+                    continue;
+                }
+                if ((e = step(gs))) return e;  // per-kernel timing, or the steps past the last whole graph
+                ++it;
+                ++j;
             }
-            if (!gexec && kstats_enabled()) {
-                // per-kernel timing: the state is read before every step so the step's byte count covers
-                // the columns it works on; the launches are those of the untimed path
-                FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs), "cg state");
-                FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
-                n_on = 0;
-                for (int c = 0; c < ncols; ++c) n_on += sc.st_host->active[c] != 0;
+            if (!op32) break;
+            // true residual r = b - S x with the FP64 operator, then a restart of the CG from it
+            ApplyArgs ar = a;
+            ar.p = xa;
+            ar.op32 = 0;
+            ar.p32 = nullptr;
+            ar.active = nullptr;
+            if ((e = apply_and_reduce<CT>(p, L, ar, kIter, q, gs, -1, kStatApplyResid))) return e;
+            ++refines;
+            const bool fp32_round = a.op32 != 0;
+            hipLaunchKernelGGL((cg_init_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, gs, S, r, (const double*)minv,
+                               (const double*)wcnt, z, pp, xa, sc.part, sc.part + kCgBlocks * CT, (float*)nullptr,
+                               (const double*)sc.bvec, (const double*)q);
+            FEHIP_LAUNCH_CHECK("cg reliable update");
+            // the new direction waits for the decision below: a round that did not halve the true residual
+            // goes on with the FP64 operator (the FP32 operator's rounding is in the way), launched directly
+            hipLaunchKernelGGL((cg_reliable_scalars<CT>), dim3(1), dim3(kBlock), 0, gs, (const double*)sc.part,
+                               (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, fp32_round ? delta : 0.0,
+                               ncols, sc.st);
+            FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs), "cg state");
+            FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
+            bool any = false;
+            double worst = 0.0;
+            for (int c = 0; c < ncols; ++c)
+                if (sc.st_host->active[c]) {
+                    any = true;
+                    worst = std::max(worst, sc.st_host->maxn[c]);
+                }
+            if (!any || it >= o.max_iter) break;
+            if (fp32_round && !(worst < 0.5 * prev_worst)) {
+                a.op32 = 0;
+                a.p32 = nullptr;
+                if (gexec) {
+                    (void)hipGraphExecDestroy(gexec);
+                    gexec = nullptr;
+                }
+                hipLaunchKernelGGL((cg_reliable_tol<CT>), dim3(1), dim3(kMaxCT), 0, gs, o.tol, sc.st);
             }
-            if (gexec && it % every == 0 && it + every <= o.max_iter) {
-                FEHIP_CHECK(hipGraphLaunch(gexec, gs), "cg graph launch");
-                it += every;
-                continue;
-            }
-            if ((e = step(gs))) return e;  // per-kernel timing, or the steps past the last whole graph
-            ++it;
+            hipLaunchKernelGGL((cg_pdir_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, gs, S,
+                               (const CgState*)sc.st, (const double*)z, pp, a.op32 ? sc.p32 : nullptr);
+            FEHIP_LAUNCH_CHECK("cg reliable update");
+            prev_worst = worst;
         }
         FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs), "cg state");
         FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
@@ -1586,6 +1956,7 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             ts.col_max_update[c] = sc.st_host->maxn[c];
         }
         ts.apply_launches = it;
+        ts.refine_launches = refines;
         for (int c = 0; c < ncols; ++c)
             if (wc.alpha[c0 + c])
                 hipLaunchKernelGGL(take_column_kernel, dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, s,
@@ -1639,6 +2010,8 @@ int time_width(Panel* p, const Layout& L, Scratch& sc, hipStream_t s, hipEvent_t
     a.ncols = CT;
     a.p = sc.tab;
     a.active = nullptr;
+    a.lean = 1;  // the kernel of an unweighted single-request solve
+    a.ubit = ~0u;
     double* q = sc.tab + L.S * kMaxCT;
     unsigned grid = launch_pass<CT, kIter>(p, L, a, s);  // warm-up
     reduce_tables<CT>(p, L, a, -1, q, grid, nullptr, s, false);
@@ -1811,8 +2184,22 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
             const int64_t R = p->d[L.dims[i]].n_runs;
             if ((e = sc.alloc(&sc.runbuf[L.sd[i].large_slot], (size_t)(R * kMaxCT + 1)))) return e;
         }
+    // the FP32 operator runs on panels whose primary segments all lie in wave tiles (the lean kernel's
+    // domain); elsewhere a mixed-precision solve uses the FP64 operator
+    if (o.precision == 1 && S > 0 && p->n_mtiles == 0 && p->n_long == 0) {
+        for (int i = 0; i < L.nd; ++i)
+            if (!L.sd[i].small) {
+                const int64_t R = p->d[L.dims[i]].n_runs;
+                if ((e = sc.alloc(&sc.runbuf32[L.sd[i].large_slot], (size_t)(R * kMaxCT + 4)))) return e;
+            }
+        if ((e = sc.alloc(&sc.p32, (size_t)(S * kMaxCT + 4)))) return e;
+        if ((e = sc.alloc(&sc.bvec, (size_t)(S * kMaxCT + 4)))) return e;
+        // padding entries of the FP32 rows are loaded with the row (never used): keep them finite
+        FEHIP_CHECK(hipMemsetAsync(sc.p32, 0, sizeof(float) * (size_t)(S * kMaxCT + 4), s), "p32 clear");
+    }
     const unsigned grid = apply_grid(p);
     if ((e = sc.alloc(&sc.small_part, (size_t)grid * (L.Gs > 0 ? L.Gs : 1) * kMaxCT))) return e;
+    if ((e = sc.alloc(&sc.dump, (size_t)wave_grid(p) * (kBlock / kWave) * kMaxCT + kMaxCT))) return e;
     if ((e = sc.alloc(&sc.part, (size_t)kCgBlocks * kMaxCT * 2))) return e;
     if (wc.gram) {
         if (wc.gram_ld < ncols) return set_error("within transform: cross-product leading dim %lld < %d columns",
@@ -1884,6 +2271,7 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
             ws.col_converged[c0 + c] = ts.col_converged[c];
             ws.col_max_update[c0 + c] = ts.col_max_update[c];
             ws.col_tile[c0 + c] = tile_id;
+            ws.col_refines[c0 + c] = ts.refine_launches;
         }
         ++tile_id;
         const int64_t id_bytes = n * 4 * L.nd + (int64_t)(p->d[p->P].G + 1) * 8 + (wm ? 4 * n : 0);

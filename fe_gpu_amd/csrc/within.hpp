@@ -13,6 +13,7 @@ struct TileStats {
     double max_update = 0.0;
     double seconds = 0.0;
     int apply_launches = 0;
+    int refine_launches = 0;  // FP64 true-residual products of the mixed-precision solve
     int col_iters[16] = {0};
     int col_converged[16] = {0};
     double col_max_update[16] = {0};
@@ -20,7 +21,8 @@ struct TileStats {
 
 struct WithinStats {
     explicit WithinStats(int ncols = 0)
-        : col_iters(ncols, 0), col_converged(ncols, 1), col_max_update(ncols, 0.0), col_tile(ncols, 0) {}
+        : col_iters(ncols, 0), col_converged(ncols, 1), col_max_update(ncols, 0.0), col_tile(ncols, 0),
+          col_refines(ncols, 0) {}
     int iterations = 0;
     int converged = 1;
     double max_update = 0.0;
@@ -31,6 +33,7 @@ struct WithinStats {
     std::vector<int> col_converged;  // per column: criterion met
     std::vector<double> col_max_update;  // per column: last max |r_g / n_g|
     std::vector<int> col_tile;       // per column: column tile it was solved in
+    std::vector<int> col_refines;    // per column: FP64 true-residual products (FP32-operator solve)
 };
 
 // Columns of one solve: input and output device pointers and the request slot of each column.

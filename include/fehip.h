@@ -93,7 +93,8 @@ typedef struct {
     int32_t n_cluster_dims, cluster_dims[3];
     int64_t n_clusters[7];
     int64_t min_single_cluster;
-    int32_t psd_fix_applied, reserved;
+    int32_t psd_fix_applied;
+    int32_t fe_refinements;  /* FP64 true-residual products of a solve with the FP32 operator (0 in FP64) */
     double rss, tss_total, tss_within, r2, r2_adj, r2_within, f_stat, max_update;
     double t_demean, t_regression, t_se;  /* seconds */
 } fehip_result;

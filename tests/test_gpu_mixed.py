@@ -87,3 +87,31 @@ def test_mixed_precision_in_a_batch_equals_alone():
         fa = api.estimate_device(pan, q.y_ptr, q.W_ptr, q.K, **q.options)
         assert rel(fb.beta, fa.beta) < 1e-10 and rel(fb.se, fa.se) < 1e-10
     pan.close()
+
+
+@pytest.mark.parametrize("tol", [1e-8, 1e-12])
+def test_fp32_operator_solve_meets_the_fp64_criterion(tol):
+    """On a worker-firm-period panel (every primary segment in wave tiles) the mixed-precision solve runs
+    its CG on the FP32 operator, with reliable updates from FP64 products of the true residual between
+    rounds.  Its stopping rule is checked on that FP64 residual, so it ends where the FP64 solve ends:
+    converged, the FE-space criterion below tol, and beta / SE of the FP32-storage fit within the stated
+    bars of the FP64 fit (and of the oracle at the tight tolerance)."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    note(f"FP32 operator tol={tol}")
+    p = synth.worker_firm_panel(seed=41, n_workers=6000, n_periods=15, n_firms=900)
+    kw = dict(cluster=(1, 2, 3), tol=tol, max_iter=20000)
+    f64 = api.estimate(p.y, p.W, p.fe, precision=0, **kw)
+    f32 = api.estimate(p.y, p.W, p.fe, precision=1, **kw)
+    assert f64.res["fe_refinements"] == 0
+    assert f32.res["fe_refinements"] >= 1          # the FP32-operator path ran
+    assert f32.res["converged"] == 1 and f32.res["max_update"] < tol
+    for k in INT_KEYS:
+        assert f32.res[k] == f64.res[k], k
+    eb, es = rel(f32.beta, f64.beta), rel(f32.se, f64.se)
+    note(f"  refinements {f32.res['fe_refinements']}, steps {f32.res['fe_iterations']} vs {f64.res['fe_iterations']}, "
+         f"beta rel err {eb:.2e}, se rel err {es:.2e}")
+    assert eb <= 1e-6 and es <= 1e-5
+    if tol < 1e-10:
+        ref = oracle.estimate(p.y, p.W, p.fe, cluster=(1, 2, 3), tol=1e-13, max_iter=200000)
+        assert rel(f32.beta, ref.beta) <= 1e-8 and rel(f32.se, ref.se) <= 1e-5
