@@ -81,6 +81,7 @@ COARSE = [
     ("fehip_server_destroy", C.c_int, [VP]),
     ("fehip_synth_worker_firm", C.c_int, [I64, I32, I32, C.c_double, C.c_uint64, VP, VP, VP]),
     ("fehip_synth_iv_wide", C.c_int, [I64, I32, I32, C.c_double, C.c_uint64, VP, VP, VP, VP]),
+    ("fehip_synth_mask", C.c_int, [I64, C.c_double, C.c_uint64, VP]),
     ("fehip_kstats_enable", C.c_int, [C.c_int]),
     ("fehip_kstats_reset", C.c_int, []),
     ("fehip_kstats_get", C.c_int, [C.c_int, C.c_char_p, C.c_int, C.POINTER(I64), DP, DP, DP]),

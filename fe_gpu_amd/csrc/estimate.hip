@@ -442,6 +442,25 @@ __device__ __forceinline__ void load_score_rows(const WaveScores& a, const TX* c
     for (int k = 0; k < KT; ++k) q.x[k] = xk[k][ri];
 }
 
+// Segmented sums of the score values over [run start rs, lane]: with FP64 columns an FP64 scan; with FP32
+// columns (the mixed-precision form, whose scores carry the storage rounding anyway) an FP32 scan of one
+// fused DPP multiply-add per column and step, widened back to FP64 for the cross products.
+template <int KT, typename TX>
+__device__ __forceinline__ void scan_scores(double (&sv)[KT], const double (&tv)[KT], int lane, int rs) {
+    if constexpr (std::is_same<TX, float>::value) {
+        float f[KT];
+#pragma unroll
+        for (int k = 0; k < KT; ++k) f[k] = (float)tv[k];
+        seg_scan_cols_f32<KT>(f, ScanFactorsF(ScanMask(lane, rs)));
+#pragma unroll
+        for (int k = 0; k < KT; ++k) sv[k] = (double)f[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < KT; ++k) sv[k] = tv[k];
+        seg_scan_cols<KT>(sv, ScanFactors(ScanMask(lane, rs)));
+    }
+}
+
 // Per subset s: part[(s * gridDim.x + block) * NP + e] (upper triangle of the block's sum of S S') and
 // ucount[s] (non-empty clusters); run sums into runbuf; small tables into small_part; r^2 into rss_part.
 // NS: compile-time bound on the nested subsets (a.nsub <= NS), which sizes their accumulators
@@ -585,9 +604,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
             const int rs = 63 - __clzll(HR & mask_le(lane));
             const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
             double sv[KT];
-#pragma unroll
-            for (int k = 0; k < KT; ++k) sv[k] = tv[k];
-            seg_scan_cols<KT>(sv, ScanFactors(ScanMask(lane, rs)));
+            scan_scores<KT, TX>(sv, tv, lane, rs);
             store_runs(j, sv, rend);
         }
 #pragma unroll
@@ -607,9 +624,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
             const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
             bool ne = (BW & mask_le(lane) & ~((1ull << rs) - 1ull)) != 0;  // run holds a weighted row
             double sv[KT];
+            if (mode != 2) {
+                scan_scores<KT, TX>(sv, tv, lane, rs);
+            } else {
 #pragma unroll
-            for (int k = 0; k < KT; ++k) sv[k] = tv[k];
-            if (mode != 2) seg_scan_cols<KT>(sv, ScanFactors(ScanMask(lane, rs)));
+                for (int k = 0; k < KT; ++k) sv[k] = tv[k];
+            }
             for (int j = 0; j < a.nrun; ++j)
                 if (a.rshare[j] == q) store_runs(j, sv, rend);
             bool leader = rend;

@@ -179,3 +179,23 @@ extern "C" int fehip_synth_iv_wide(int64_t n_ind, int32_t n_periods, int32_t n_f
     FEHIP_CHECK(hipDeviceSynchronize(), "iv wide generator");
     return fehip::set_ok();
 }
+
+namespace fehip {
+namespace {
+__global__ void synth_mask_kernel(int64_t n, double p, uint64_t seed, uint8_t* m) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m[i] = uni(seed, 50, (uint64_t)i) < p ? 1 : 0;
+}
+}  // namespace
+}  // namespace fehip
+
+extern "C" int fehip_synth_mask(int64_t n, double p, uint64_t seed, uint8_t* d_mask) {
+    (void)hipGetLastError();
+    if (n < 0 || (n > 0 && !d_mask) || !(p >= 0.0 && p <= 1.0)) return fehip::set_error("fehip_synth_mask: bad arguments");
+    if (n == 0) return fehip::set_ok();
+    hipLaunchKernelGGL(fehip::synth_mask_kernel, dim3(fehip::stream_grid((size_t)n, 256)), dim3(256), 0, 0, n, p, seed,
+                       d_mask);
+    FEHIP_LAUNCH_CHECK("mask generator");
+    FEHIP_CHECK(hipStreamSynchronize(0), "mask generator");  // complete before other threads' streams read it
+    return fehip::set_ok();
+}

@@ -1680,6 +1680,7 @@ constexpr int kCgBlocks = 2048;  // CG vector kernels: eight waves per SIMD keep
 // (FP64) residual: well above the FP32 operator's relative rounding (~1e-6), so each round's FP64
 // residual falls by about this factor.
 constexpr double kInnerRel = 1e-4;
+constexpr int64_t kOp32MinTable = 4096;  // table entries below which a solve keeps the FP64 operator
 
 ApplyArgs base_args(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm) {
     ApplyArgs a{};
@@ -1793,7 +1794,11 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
     // FP32-operator PCG, each stopped once its own residual is delta times the round's true residual,
     // with the true residual b - S x of the FP64 operator between rounds; the stopping rule is checked
     // on that FP64 residual only, so the result meets the same criterion as the FP64 solve
-    const bool op32 = o.precision == 1 && uniform && sc.p32 != nullptr;
+    // A system that is diagonal (one scattered dim, primary not absorbed: Jacobi CG ends in one step) or
+    // small (a few thousand table entries: steps are cheap) gains nothing from FP32 steps, and would pay
+    // their FP64 true-residual products, so it stays on the FP64 operator.
+    const bool cheap = (L.nd == 1 && !L.elim) || S <= kOp32MinTable;
+    const bool op32 = o.precision == 1 && uniform && sc.p32 != nullptr && !cheap;
     const double delta = op32 ? kInnerRel : 0.0;
     if (S > 0) {
         hipLaunchKernelGGL((make_minv_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, s, S,

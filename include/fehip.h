@@ -242,10 +242,15 @@ int fehip_synth_worker_firm(int64_t n_workers, int32_t n_periods, int32_t n_firm
 int fehip_synth_iv_wide(int64_t n_ind, int32_t n_periods, int32_t n_firms, double switch_p, uint64_t seed,
                         int32_t* d_fe, double* d_y, double* d_W, double* d_Z);
 
+/* Deterministic row subsample on the device: d_mask[i] = 1 with probability p (a counter-based hash of
+ * seed and i), else 0; n uint8 values (a request's row mask of the serving workload). */
+int fehip_synth_mask(int64_t n, double p, uint64_t seed, uint8_t* d_mask);
+
 /* Opt-in per-kernel timing with HIP events on the stream each kernel is launched on.  Kernel ids:
  * 0 apply_iter (the CG operator product, the dominant kernel), 1 apply_data (right-hand side),
- * 2 finalize (M_F columns), 3 reduce_runs, 4 gram, 5 segsum (cluster score sums).  bytes/flops
- * accumulate the algorithmic work of every timed launch. */
+ * 2 finalize (M_F columns), 3 reduce_runs, 4 gram, 5 segsum (cluster score sums), 6 apply_iter32 (the
+ * FP32 operator product of a mixed-precision solve), 7 apply_resid (its FP64 true-residual products).
+ * bytes/flops accumulate the algorithmic work of every timed launch. */
 int fehip_kstats_enable(int on);
 int fehip_kstats_reset(void);
 int fehip_kstats_get(int id, char* name, int name_len, int64_t* launches, double* total_ms, double* bytes,

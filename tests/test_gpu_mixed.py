@@ -99,7 +99,8 @@ def test_fp32_operator_solve_meets_the_fp64_criterion(tol):
     import oracle
     from fe_gpu_amd import api, synth
     note(f"FP32 operator tol={tol}")
-    p = synth.worker_firm_panel(seed=41, n_workers=6000, n_periods=15, n_firms=900)
+    # 5,000 firms: a table space large enough for the FP32 operator (small ones keep the FP64 operator)
+    p = synth.worker_firm_panel(seed=41, n_workers=6000, n_periods=15, n_firms=5000)
     kw = dict(cluster=(1, 2, 3), tol=tol, max_iter=20000)
     f64 = api.estimate(p.y, p.W, p.fe, precision=0, **kw)
     f32 = api.estimate(p.y, p.W, p.fe, precision=1, **kw)
@@ -115,3 +116,18 @@ def test_fp32_operator_solve_meets_the_fp64_criterion(tol):
     if tol < 1e-10:
         ref = oracle.estimate(p.y, p.W, p.fe, cluster=(1, 2, 3), tol=1e-13, max_iter=200000)
         assert rel(f32.beta, ref.beta) <= 1e-8 and rel(f32.se, ref.se) <= 1e-5
+
+
+def test_small_or_diagonal_systems_keep_the_fp64_operator():
+    """A mixed-precision request whose FE-space system is diagonal (one absorbed dim besides the
+    unabsorbed primary) or small takes no FP32 rounds (no FP64 refinement products), and equals the
+    FP64 path."""
+    from fe_gpu_amd import api, synth
+    note("small / diagonal systems")
+    p = synth.worker_firm_panel(seed=42, n_workers=4000, n_periods=10, n_firms=3000)
+    for fe_mask in (0b010, 0b101):   # {firm} without the primary; {worker, period}: 10 table entries
+        kw = dict(cluster=(2,) if fe_mask == 0b010 else (1,), fe_mask=fe_mask, tol=1e-10, max_iter=20000)
+        f64 = api.estimate(p.y, p.W, p.fe, precision=0, **kw)
+        f32 = api.estimate(p.y, p.W, p.fe, precision=1, **kw)
+        assert f32.res["fe_refinements"] == 0
+        assert rel(f32.beta, f64.beta) <= 1e-6 and rel(f32.se, f64.se) <= 1e-5
