@@ -93,7 +93,34 @@ void sym_eig(const std::vector<double>& A, int n, std::vector<double>& w, std::v
     for (int i = 0; i < n; ++i) w[i] = a[(size_t)i * n + i];
 }
 
+// True when A (symmetric) factors as U^T U with every pivot above 1e-12 times the largest diagonal
+// entry: A is then positive definite with a clear margin, so clipping its negative eigenvalues would
+// return A itself (its eigen-reconstruction V diag(w) V^T equals A up to rounding).
+static bool clearly_positive_definite(const std::vector<double>& A, int n) {
+    double dmax = 0.0;
+    for (int i = 0; i < n; ++i) dmax = std::fmax(dmax, A[(size_t)i * n + i]);
+    if (!(dmax > 0.0)) return false;
+    const double floor_pivot = 1e-12 * dmax;
+    std::vector<double> U(A);
+    for (int j = 0; j < n; ++j) {
+        double s = U[(size_t)j * n + j];
+        for (int k = 0; k < j; ++k) s -= U[(size_t)j * n + k] * U[(size_t)j * n + k];
+        if (!(s > floor_pivot)) return false;
+        const double ujj = std::sqrt(s);
+        U[(size_t)j * n + j] = ujj;
+        for (int i = j + 1; i < n; ++i) {
+            double t = U[(size_t)i * n + j];
+            for (int k = 0; k < j; ++k) t -= U[(size_t)j * n + k] * U[(size_t)i * n + k];
+            U[(size_t)i * n + j] = t / ujj;
+        }
+    }
+    return true;
+}
+
 int psd_clip(std::vector<double>& A, int n) {
+    // the reference's dsyev + clip of negative eigenvalues + rebuild (src/core/fe_pipeline.f90:1797-1894),
+    // taken only when A is not clearly positive definite (otherwise it would give back A)
+    if (clearly_positive_definite(A, n)) return 0;
     std::vector<double> w, V;
     sym_eig(A, n, w, V);
     int clipped = 0;

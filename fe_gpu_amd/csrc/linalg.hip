@@ -114,13 +114,14 @@ constexpr int kXcds = 8;
 
 // RM: the reduction index runs along the leading dimension's complement, i.e. element (p, i) of the
 // k x m operand is A[p ldA + i] (the A B^T shape of S'S for row-major cluster sums); otherwise A[p + i ldA].
-template <int NB, bool RM>
+template <int NB, bool RM, bool VEC>
 __global__ __launch_bounds__(256) void gemm_tn_mfma_kernel(int m, int n, long long k, const double* __restrict__ A,
                                                            long long ldA, const double* __restrict__ B, long long ldB,
                                                            long long kchunk, int nsplit, int tiles_x, int tiles_y,
                                                            double* partial, bool upper_only) {
-    constexpr int BT = 32 * NB;  // block tile
-    constexpr int KS = 32;       // slab rows
+    constexpr int BT = 32 * NB;            // block tile
+    constexpr int KS = 32;                 // slab rows
+    constexpr int PER = KS * BT / 256;     // elements of a slab each thread loads, per operand
     __shared__ double As[KS][BT + 1];
     __shared__ double Bs[KS][BT + 1];
     const int ntiles = tiles_x * tiles_y;
@@ -129,9 +130,12 @@ __global__ __launch_bounds__(256) void gemm_tn_mfma_kernel(int m, int n, long lo
     if (split >= nsplit) return;
     const int i0 = (tile % tiles_x) * BT, j0 = (tile / tiles_x) * BT;
     if (upper_only && i0 > j0 + BT - 1) return;
-    const long long k0 = (long long)split * kchunk;
-    long long k1 = k0 + kchunk;
-    if (k1 > k) k1 = k;
+    // Split z takes the slabs z, z + nsplit, z + 2 nsplit, ... of the reduction: the workgroups in flight
+    // read neighbouring rows, so a tall column-major operand's columns are read through a narrow window
+    // of pages at a time (contiguous chunks per split would spread them over every page at once)
+    (void)kchunk;
+    const long long kstride = (long long)nsplit * KS;
+    const long long k0 = (long long)split * KS;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wi = (wave & 1) * 16 * NB, wj = (wave >> 1) * 16 * NB;
     const int lr = lane & 15, lk = lane >> 4;
@@ -140,15 +144,72 @@ __global__ __launch_bounds__(256) void gemm_tn_mfma_kernel(int m, int n, long lo
     for (int a = 0; a < NB; ++a)
 #pragma unroll
         for (int b = 0; b < NB; ++b) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    for (long long p0 = k0; p0 < k1; p0 += KS) {
-        for (int e = t; e < KS * BT; e += 256) {
-            const int pp = RM ? e / BT : e % KS, ii = RM ? e % BT : e / KS;  // walk the contiguous direction
+    // Software pipeline: the next slab is loaded into registers while the matrix cores work on the
+    // current one from LDS, so the wait for global memory falls behind a slab of MFMAs instead of in
+    // front of it.  A thread's elements walk the operand's contiguous direction (coalesced); with VEC
+    // (even leading dims, 16-byte aligned operands) each load takes two adjacent elements.
+    constexpr int W = VEC ? 2 : 1;  // elements per load
+    double ra[PER], rb[PER];
+    auto load_slab = [&](long long p0) {
+        const long long k1 = p0 + KS < k ? p0 + KS : k;
+#pragma unroll
+        for (int q = 0; q < PER / W; ++q) {
+            const int e = (t + 256 * q) * W;  // first of the W adjacent elements (same row / column)
+            const int pp = RM ? e / BT : e % KS, ii = RM ? e % BT : e / KS;
             const long long p = p0 + pp;
             const int ia = i0 + ii, jb = j0 + ii;
-            As[pp][ii] = (p < k1 && ia < m) ? (RM ? A[p * ldA + ia] : A[p + (long long)ia * ldA]) : 0.0;
-            Bs[pp][ii] = (p < k1 && jb < n) ? (RM ? B[p * ldB + jb] : B[p + (long long)jb * ldB]) : 0.0;
+            if constexpr (VEC) {
+                // the pair lies in one column (RM: one row); both halves in range or both out, since the
+                // slab rows and the tile columns come in even counts and k1, m, n bound them
+                const bool oka = RM ? (p < k1 && ia + 1 < m) : (p + 1 < k1 && ia < m);
+                const bool okb = RM ? (p < k1 && jb + 1 < n) : (p + 1 < k1 && jb < n);
+                const double* pa = RM ? A + p * ldA + ia : A + p + (long long)ia * ldA;
+                const double* pb = RM ? B + p * ldB + jb : B + p + (long long)jb * ldB;
+                double2 va = {0.0, 0.0}, vb = {0.0, 0.0};
+                if (oka) {
+                    va = *reinterpret_cast<const double2*>(pa);
+                } else {
+                    const bool ok0 = p < k1 && ia < m;  // a partial pair at the edge: elementwise
+                    const bool ok1 = RM ? (p < k1 && ia + 1 < m) : (p + 1 < k1 && ia < m);
+                    va.x = ok0 ? pa[0] : 0.0;
+                    va.y = ok1 ? pa[RM ? 1 : 1] : 0.0;
+                }
+                if (okb) {
+                    vb = *reinterpret_cast<const double2*>(pb);
+                } else {
+                    const bool ok0 = p < k1 && jb < n;
+                    const bool ok1 = RM ? (p < k1 && jb + 1 < n) : (p + 1 < k1 && jb < n);
+                    vb.x = ok0 ? pb[0] : 0.0;
+                    vb.y = ok1 ? pb[1] : 0.0;
+                }
+                ra[2 * q] = va.x;
+                ra[2 * q + 1] = va.y;
+                rb[2 * q] = vb.x;
+                rb[2 * q + 1] = vb.y;
+            } else {
+                ra[q] = (p < k1 && ia < m) ? (RM ? A[p * ldA + ia] : A[p + (long long)ia * ldA]) : 0.0;
+                rb[q] = (p < k1 && jb < n) ? (RM ? B[p * ldB + jb] : B[p + (long long)jb * ldB]) : 0.0;
+            }
         }
+    };
+    auto store_slab = [&]() {
+#pragma unroll
+        for (int q = 0; q < PER / W; ++q) {
+            const int e = (t + 256 * q) * W;
+            const int pp = RM ? e / BT : e % KS, ii = RM ? e % BT : e / KS;
+#pragma unroll
+            for (int h = 0; h < W; ++h) {
+                const int ph = RM ? pp : pp + h, ih = RM ? ii + h : ii;
+                As[ph][ih] = ra[W * q + h];
+                Bs[ph][ih] = rb[W * q + h];
+            }
+        }
+    };
+    if (k0 < k) load_slab(k0);
+    for (long long p0 = k0; p0 < k; p0 += kstride) {
+        store_slab();
         __syncthreads();
+        if (p0 + kstride < k) load_slab(p0 + kstride);  // in flight during this slab's MFMAs
 #pragma unroll
         for (int kk = 0; kk < KS / 4; ++kk) {
             double av[NB], bv[NB];
@@ -194,6 +255,19 @@ __global__ void gemm_finish_kernel(const double* __restrict__ partial, int nspli
 }  // namespace
 
 namespace {
+// resident workgroups of one GEMM kernel instance on the device (its occupancy times the CUs)
+unsigned resident_slots(int nb, bool rm, bool vec) {
+    const size_t big = (size_t)1 << 40;
+#define FEHIP_SLOTS(NBV, RMV, VECV) resident_grid(gemm_tn_mfma_kernel<NBV, RMV, VECV>, 256, 0, big)
+    if (nb == 4) return rm ? (vec ? FEHIP_SLOTS(4, true, true) : FEHIP_SLOTS(4, true, false))
+                           : (vec ? FEHIP_SLOTS(4, false, true) : FEHIP_SLOTS(4, false, false));
+    if (nb == 3) return rm ? (vec ? FEHIP_SLOTS(3, true, true) : FEHIP_SLOTS(3, true, false))
+                           : (vec ? FEHIP_SLOTS(3, false, true) : FEHIP_SLOTS(3, false, false));
+    return rm ? (vec ? FEHIP_SLOTS(2, true, true) : FEHIP_SLOTS(2, true, false))
+              : (vec ? FEHIP_SLOTS(2, false, true) : FEHIP_SLOTS(2, false, false));
+#undef FEHIP_SLOTS
+}
+
 // A^T B on the matrix cores: tile size with the least padding of m and n (64, 96 or 128), split-K
 // over the rows to fill the device.  The split-K partials come from the library's workspace, or from
 // `fixed` (capacity `fixed_cap` doubles) when the caller must not allocate; the split count is then
@@ -212,29 +286,47 @@ int gemm_tn_mfma(bool rm, int m, int n, long long k, double alpha, const double*
         }
     }
     const int bt = 32 * nb;
-    const long long tiles = (long long)((m + bt - 1) / bt) * ((n + bt - 1) / bt);
-    long long nsplit = std::max<long long>(1, (4LL * num_cus()) / std::max<long long>(1, tiles));
+    const int tx0 = (m + bt - 1) / bt, ty0 = (n + bt - 1) / bt;
+    const long long tiles = (long long)tx0 * ty0;
+    // tiles that do work (the upper triangle of a symmetric product) times the splits fill the device's
+    // resident workgroup slots in whole rounds, so no round runs partly empty
+    long long active = tiles;
+    if (upper_only)
+        for (int a = 0; a < tx0; ++a)
+            for (int b = 0; b < ty0; ++b) active -= (a * bt > b * bt + bt - 1) ? 1 : 0;
+    const bool vec = (ldA % 2 == 0) && (ldB % 2 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0);
+    // splits are dealt to the XCDs round-robin with all their tiles, so a multiple of the XCD count whose
+    // active workgroups fill each XCD's slots twice over (two rounds: the second hides the first's tail)
+    const long long slots = (long long)resident_slots(nb, rm, vec);
+    long long nsplit = kXcds * std::max<long long>(1, (2 * slots) / (kXcds * std::max<long long>(1, active)));
     nsplit = std::min<long long>(nsplit, std::max<long long>(1, k / 1024));
     nsplit = std::min<long long>(nsplit, std::max<long long>(1, (512LL << 20) / ((long long)m * n * 8)));
     if (fixed) nsplit = std::min<long long>(nsplit, std::max<long long>(1, (long long)(fixed_cap / ((size_t)m * n))));
-    long long kchunk = (k + nsplit - 1) / nsplit;
-    kchunk = (kchunk + 31) / 32 * 32;
-    nsplit = (k + kchunk - 1) / kchunk;
+    // every split must own at least one 32-row slab (the slabs are interleaved across the splits)
+    nsplit = std::min<long long>(nsplit, (k + 31) / 32);
+    const long long kchunk = 0;  // unused by the interleaved kernel
     double* partial = fixed ? fixed : static_cast<double*>(workspace((size_t)nsplit * m * n * sizeof(double), 2));
     if (!partial) return set_error("gemm: workspace allocation failed");
     const int tx = (m + bt - 1) / bt, ty = (n + bt - 1) / bt;
     const long long per_xcd = (nsplit + kXcds - 1) / kXcds;
     const unsigned grid = (unsigned)(kXcds * per_xcd * tx * ty);
-#define FEHIP_MFMA_LAUNCH(NBV, RMV)                                                                           \
-    hipLaunchKernelGGL((gemm_tn_mfma_kernel<NBV, RMV>), dim3(grid), dim3(256), 0, 0, m, n, k, A, ldA, B, ldB, kchunk, \
-                       (int)nsplit, tx, ty, partial, upper_only)
-    if (nb == 4) {
-        if (rm) FEHIP_MFMA_LAUNCH(4, true); else FEHIP_MFMA_LAUNCH(4, false);
-    } else if (nb == 3) {
-        if (rm) FEHIP_MFMA_LAUNCH(3, true); else FEHIP_MFMA_LAUNCH(3, false);
-    } else {
-        if (rm) FEHIP_MFMA_LAUNCH(2, true); else FEHIP_MFMA_LAUNCH(2, false);
-    }
+#define FEHIP_MFMA_LAUNCH(NBV, RMV, VECV)                                                                        \
+    hipLaunchKernelGGL((gemm_tn_mfma_kernel<NBV, RMV, VECV>), dim3(grid), dim3(256), 0, 0, m, n, k, A, ldA, B, ldB, \
+                       kchunk, (int)nsplit, tx, ty, partial, upper_only)
+#define FEHIP_MFMA_RM(NBV)                                              \
+    do {                                                                \
+        if (rm) {                                                       \
+            if (vec) FEHIP_MFMA_LAUNCH(NBV, true, true);                 \
+            else FEHIP_MFMA_LAUNCH(NBV, true, false);                    \
+        } else {                                                        \
+            if (vec) FEHIP_MFMA_LAUNCH(NBV, false, true);                \
+            else FEHIP_MFMA_LAUNCH(NBV, false, false);                   \
+        }                                                               \
+    } while (0)
+    if (nb == 4) FEHIP_MFMA_RM(4);
+    else if (nb == 3) FEHIP_MFMA_RM(3);
+    else FEHIP_MFMA_RM(2);
+#undef FEHIP_MFMA_RM
 #undef FEHIP_MFMA_LAUNCH
     FEHIP_LAUNCH_CHECK("gemm mfma kernel");
     hipLaunchKernelGGL(gemm_finish_kernel, dim3(stream_grid((size_t)m * n, kBlock)), dim3(kBlock), 0, 0, partial,
