@@ -1091,6 +1091,173 @@ __global__ void group_runs_kernel(const double* __restrict__ runbuf, const int* 
     }
 }
 
+constexpr int kWideCols = 16;  // score columns per step of the wide pass: one 128-byte line of a sum row
+
+// Scores wider than the register-tiled kernels hold (2SLS instrument sets, wide designs): one pass over
+// the columns of x, in tiles of whole primary segments, forms the sums of r x over each primary segment
+// (rows of Sseg, in segment order) and over each run of the large dim D (rows of Rg, at the run's
+// position in D's group order), both row-major with leading dimension ld (a multiple of kWideCols;
+// the columns past K are written as zeros).  A tile's rows stay in LDS for kWideCols columns at a
+// time, so every column is read once and coalesced and every sum row leaves in whole 128-byte lines.
+// cont[position] = 1 flags a run whose (segment, group) cell an earlier run of the same segment
+// opened; that earlier run directly precedes it in group order, since group order sorts the runs of
+// a group by their position in the panel.  counts[0] += cells holding a weighted row (every cell when
+// w is null), counts[1] += flagged runs.
+template <typename TX>
+__global__ __launch_bounds__(kBlock) void wide_scores_kernel(
+    const double* __restrict__ r, const TX* const* __restrict__ X, int K, int ld, const int64_t* __restrict__ seg_off,
+    const int64_t* __restrict__ tile_seg, int64_t n_tiles, const int* __restrict__ ids,
+    const int64_t* __restrict__ seg_run_off, const int* __restrict__ run_dest, const uint8_t* __restrict__ w,
+    double* Sseg, double* Rg, uint8_t* cont, unsigned long long* counts) {
+    __shared__ double ts[kWideCols][kTileRows + 1];
+    __shared__ int soff[kTileRows + 1];
+    __shared__ int rstart[kTileRows + 1];
+    __shared__ int rdst[kTileRows];
+    __shared__ int rseg[kTileRows];
+    __shared__ int rgid[kTileRows];
+    __shared__ unsigned char rw[kTileRows];
+    __shared__ int wcnt[kBlock / kWave];
+    const int j = threadIdx.x, lane = j & (kWave - 1), wv = j / kWave;
+    unsigned long long ncell = 0, ncont = 0;
+    for (int64_t t = blockIdx.x; t < n_tiles; t += gridDim.x) {
+        const int64_t s0 = tile_seg[2 * t], s1 = tile_seg[2 * t + 1];
+        const int nseg = (int)(s1 - s0);
+        const int64_t R0 = seg_off[s0];
+        __syncthreads();  // the previous tile's reads of the shared arrays are done
+        for (int k = j; k <= nseg; k += blockDim.x) soff[k] = (int)(seg_off[s0 + k] - R0);
+        const int nr = (int)(seg_off[s1] - R0);
+        __syncthreads();
+        const bool valid = j < nr;
+        const double rj = valid ? r[R0 + j] : 0.0;
+        int nrun = 0;
+        if (Rg) {
+            // runs: a row opens one at its segment's start or where D's id changes
+            const int g = valid ? ids[R0 + j] : -1;
+            int seg = 0;
+            if (valid) {  // the last segment starting at or before row j (empty segments start there too)
+                int lo = 0, hi = nseg - 1;
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if (soff[mid] <= j)
+                        lo = mid;
+                    else
+                        hi = mid - 1;
+                }
+                seg = lo;
+            }
+            int gp = __shfl_up(g, 1, kWave);
+            if (lane == 0) gp = (valid && j > 0) ? ids[R0 + j - 1] : -1;
+            const bool head = valid && (soff[seg] == j || g != gp);
+            const unsigned long long bal = __ballot(head);
+            if (lane == 0) wcnt[wv] = __popcll(bal);
+            __syncthreads();
+            int base = 0;
+            for (int q = 0; q < kBlock / kWave; ++q) {
+                base += q < wv ? wcnt[q] : 0;
+                nrun += wcnt[q];
+            }
+            if (head) {
+                const int q = base + __popcll(bal & ((1ull << lane) - 1ull));
+                rstart[q] = j;
+                rseg[q] = seg;
+                rgid[q] = g;
+                rdst[q] = run_dest[seg_run_off[s0] + q];
+            }
+            if (j == 0) rstart[nrun] = nr;
+            __syncthreads();
+            if (j < nrun) {
+                bool wq = w == nullptr;
+                for (int i = rstart[j]; i < rstart[j + 1] && !wq; ++i) wq = w[R0 + i] != 0;
+                rw[j] = wq ? 1 : 0;
+            }
+            __syncthreads();
+            if (j < nrun) {
+                bool c = false;
+                for (int q2 = j - 1; q2 >= 0 && rseg[q2] == rseg[j] && !c; --q2) c = rgid[q2] == rgid[j];
+                cont[rdst[j]] = c ? 1 : 0;
+                if (c) {
+                    ++ncont;
+                } else {
+                    bool cw = rw[j] != 0;
+                    for (int q2 = j + 1; q2 < nrun && rseg[q2] == rseg[j]; ++q2)
+                        if (rgid[q2] == rgid[j]) cw |= rw[q2] != 0;
+                    ncell += cw ? 1 : 0;
+                }
+            }
+        }
+        // the next block of columns is loaded into registers while the current one is summed
+        double nv[kWideCols];
+        auto load_cols = [&](int k0) {
+            const int nc = min(kWideCols, K - k0);
+#pragma unroll
+            for (int c = 0; c < kWideCols; ++c) nv[c] = (valid && c < nc) ? rj * (double)X[k0 + c][R0 + j] : 0.0;
+        };
+        load_cols(0);
+        for (int k0 = 0; k0 < ld; k0 += kWideCols) {
+            const int nc = min(kWideCols, K - k0);
+#pragma unroll
+            for (int c = 0; c < kWideCols; ++c) ts[c][j] = nv[c];
+            __syncthreads();
+            if (k0 + kWideCols < ld) load_cols(k0 + kWideCols);
+            if (Sseg)
+                for (int e = j; e < nseg * kWideCols; e += blockDim.x) {
+                    const int s = e / kWideCols, c = e % kWideCols;
+                    double sm = 0.0;
+                    if (c < nc)
+                        for (int i = soff[s]; i < soff[s + 1]; ++i) sm += ts[c][i];
+                    Sseg[(s0 + s) * ld + k0 + c] = sm;
+                }
+            if (Rg)
+                for (int e = j; e < nrun * kWideCols; e += blockDim.x) {
+                    const int q = e / kWideCols, c = e % kWideCols;
+                    double sm = 0.0;
+                    if (c < nc)
+                        for (int i = rstart[q]; i < rstart[q + 1]; ++i) sm += ts[c][i];
+                    Rg[(int64_t)rdst[q] * ld + k0 + c] = sm;
+                }
+            __syncthreads();
+        }
+    }
+    for (int o = kWave / 2; o > 0; o >>= 1) {
+        ncell += __shfl_down(ncell, o, kWave);
+        ncont += __shfl_down(ncont, o, kWave);
+    }
+    if (lane == 0 && Rg) {
+        if (ncell) atomicAdd(counts, ncell);
+        if (ncont) atomicAdd(counts + 1, ncont);
+    }
+}
+
+// S[g][c] = sum of rows off[g] .. off[g+1) of R (row-major, leading dimension ld), c < ld
+__global__ void row_group_sums_kernel(const double* __restrict__ R, int ld, const int64_t* __restrict__ off, int64_t G,
+                                      double* S) {
+    const int64_t total = G * ld;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t g = e / ld;
+        const int c = (int)(e - g * ld);
+        double sm = 0.0;
+        for (int64_t k = off[g]; k < off[g + 1]; ++k) sm += R[k * ld + c];
+        S[e] = sm;
+    }
+}
+
+// Rows of R flagged in cont are added into the nearest unflagged row before them (their cell's first
+// run) and cleared, so R then holds one sum row per cell and zero rows.
+__global__ void merge_cells_kernel(double* R, int ld, const uint8_t* __restrict__ cont, int64_t nR) {
+    const int64_t total = nR * ld;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t q = e / ld;
+        if (cont[q] || q + 1 >= nR || !cont[q + 1]) continue;
+        const int c = (int)(e - q * ld);
+        double sm = R[e];
+        for (int64_t q2 = q + 1; q2 < nR && cont[q2]; ++q2) {
+            sm += R[q2 * ld + c];
+            R[q2 * ld + c] = 0.0;
+        }
+        R[e] = sm;
+    }
+}
+
 constexpr int kMeatBlocks = 1024;
 
 // meat of one partition (K <= 12 fused path)
@@ -1380,7 +1547,8 @@ enum {
     kRouteSmall = 3,
     kRouteAlias = 4,
     kRouteSorted = 5,
-    kRouteRunCells = 6
+    kRouteRunCells = 6,
+    kRouteWide = 7
 };
 
 struct ScorePlan {
@@ -1396,6 +1564,10 @@ struct ScorePlan {
     int sorted_mask = 0;   // the subset whose score rows the pass writes in a partition's order (0: none)
     unsigned sorted_dims = 0;  // the dims of that partition (the subset's, or its large dim's for RunCells)
     int cells = 0;             // RunCells: cells of the subset's small dims
+    // Wide: subsets of the primary and one large dim D read the segment sums and D's run sums of the
+    // wide pass
+    int wide_dim = -1;
+    bool wide_seg = false, wide_run = false;
 };
 
 int plan_scores(Panel* p, const int* cdims, int ncd, int Kt, unsigned long long* dcount, ScorePlan& pl) {
@@ -1414,6 +1586,17 @@ int plan_scores(Panel* p, const int* cdims, int ncd, int Kt, unsigned long long*
         return -1;  // not reached: at most three cluster dims
     };
     int single_q = -1;
+    // scores too wide for the fused pass: the subsets of the primary and the first large cluster dim
+    // go through the wide pass, which needs the tiles of whole segments to cover the panel
+    unsigned wide_ok = 0;
+    int wide_d = -1;
+    if (Kt > 12 && p->n_long == 0 && p->n_tiles > 0) {
+        for (int j = 0; j < ncd && wide_d < 0; ++j) {
+            const int d = cdims[j] - 1;
+            if (d != p->P && !p->d[d].small && p->d[d].n_runs > 0) wide_d = d;
+        }
+        wide_ok = (1u << p->P) | (wide_d >= 0 ? 1u << wide_d : 0u);
+    }
     for (int mask = 1; mask < (1 << ncd); ++mask) {
         unsigned dm = 0;
         int len = 0;
@@ -1508,6 +1691,13 @@ int plan_scores(Panel* p, const int* cdims, int ncd, int Kt, unsigned long long*
                 pl.sorted_dims = dm;
                 route = kRouteSorted;
             }
+        }
+        if (route == kRouteOther && wide_ok && (dm & ~wide_ok) == 0) {
+            route = kRouteWide;
+            pl.wide_dim = wide_d;
+            pl.wide_seg |= has_p;
+            pl.wide_run |= wide_d >= 0 && ((dm >> wide_d) & 1u);
+            pl.needs_r = true;
         }
         if (route == kRouteOther) {
             // the other methods: block-tile nested, small tables and run sums read the residual; the
@@ -2052,7 +2242,9 @@ int finish_request(Panel* p, ReqState& s) {
         if ((e = db.get(&dcount, 1)) || (e = db.get(&ucount, 1))) return e;
         double* Md = nullptr;
         if ((e = db.get(&Md, (size_t)K * K))) return e;
-        std::vector<double> M((size_t)K * K), tmp((size_t)kept * kept);
+        // the signed sum of the subsets' meats (the sandwich is linear in the meat, so the 2SLS
+        // projection and Q^-1 M Q^-1 are applied once, to the sum)
+        std::vector<double> M((size_t)K * K), Macc((size_t)K * K, 0.0);
         int64_t min_single = 0x7fffffff;
         // The fused pass over the wave tiles: the meats of the subsets it forms (a subset whose clusters are
         // single rows repeats the meat of the first such subset), and the rss of a residual formed in it.
@@ -2215,6 +2407,82 @@ int finish_request(Panel* p, ReqState& s) {
             }
             for (int mask = 1; mask < (1 << ncd); ++mask)
                 if (plan.route[mask] == kRouteAlias) pre[mask] = pre[plan.index[mask]];
+        }
+        if (plan.wide_seg || plan.wide_run) {
+            // the wide pass: segment sums and D's run sums (group order) of r x, then per subset the
+            // cluster sums and their meat S'S on the matrix cores
+            const int ld = (K + kWideCols - 1) / kWideCols * kWideCols;
+            const int64_t GP = p->d[p->P].G;
+            const DimStruct* D = plan.wide_run ? &p->d[plan.wide_dim] : nullptr;
+            double *Sseg = nullptr, *Rg = nullptr;
+            uint8_t* cont = nullptr;
+            unsigned long long* wcounts = nullptr;
+            if (plan.wide_seg && (e = db.get(&Sseg, (size_t)GP * ld))) return e;
+            if (D && ((e = db.get(&Rg, (size_t)D->n_runs * ld)) || (e = db.get(&cont, (size_t)D->n_runs)))) return e;
+            if ((e = db.get(&wcounts, 2))) return e;
+            FEHIP_CHECK(hipMemsetAsync(wcounts, 0, 2 * sizeof(unsigned long long), 0), "memset");
+            const int* Dids = D ? D->ids : nullptr;
+            const int64_t* Droff = D ? D->seg_run_off : nullptr;
+            const int* Ddest = D ? D->run_dest : nullptr;
+            if (X32) {
+                const unsigned grid = resident_grid(wide_scores_kernel<float>, kBlock, 0, (size_t)p->n_tiles);
+                hipLaunchKernelGGL(wide_scores_kernel<float>, dim3(grid), dim3(kBlock), 0, 0, (const double*)r, X32, K,
+                                   ld, p->seg_off, p->tile_seg, p->n_tiles, Dids, Droff, Ddest, wpanel, Sseg, Rg,
+                                   cont, wcounts);
+            } else {
+                const unsigned grid = resident_grid(wide_scores_kernel<double>, kBlock, 0, (size_t)p->n_tiles);
+                hipLaunchKernelGGL(wide_scores_kernel<double>, dim3(grid), dim3(kBlock), 0, 0, (const double*)r, X, K,
+                                   ld, p->seg_off, p->tile_seg, p->n_tiles, Dids, Droff, Ddest, wpanel, Sseg, Rg,
+                                   cont, wcounts);
+            }
+            FEHIP_LAUNCH_CHECK("wide scores");
+            unsigned long long hw[2] = {0, 0};
+            if (D) FEHIP_CHECK(copy_sync(hw, wcounts, sizeof(hw), hipMemcpyDeviceToHost), "count copy");
+            auto meat_rows = [&](const double* Sr, int64_t nS, std::vector<double>& Mo) -> int {
+                int e2 = gemm_impl(false, true, K, K, nS, 1.0, Sr, ld, Sr, ld, 0.0, Md, K, true);
+                if (e2) return e2;
+                Mo.assign((size_t)K * K, 0.0);
+                FEHIP_CHECK(copy_sync(Mo.data(), Md, Mo.size() * sizeof(double), hipMemcpyDeviceToHost), "meat copy");
+                for (int j = 0; j < K; ++j)
+                    for (int i = j + 1; i < K; ++i) Mo[(size_t)j * K + i] = Mo[(size_t)i * K + j];
+                return 0;
+            };
+            bool merged = false;
+            for (int mask = 1; mask < (1 << ncd); ++mask) {
+                if (plan.route[mask] != kRouteWide) continue;
+                bool hp = false, hd = false;
+                for (int j = 0; j < ncd; ++j)
+                    if (mask & (1 << j)) {
+                        hp |= cdims[j] - 1 == p->P;
+                        hd |= cdims[j] - 1 == plan.wide_dim;
+                    }
+                std::vector<double> Mw;
+                int64_t nw = 0;
+                if (hp && !hd) {
+                    if ((e = meat_rows(Sseg, GP, Mw))) return e;
+                    nw = Geff[p->P];
+                } else if (hd && !hp) {  // group sums: the runs of a group are contiguous in group order
+                    double* Sd = nullptr;
+                    if ((e = db.get(&Sd, (size_t)D->G * ld))) return e;
+                    hipLaunchKernelGGL(row_group_sums_kernel, dim3(stream_grid((size_t)D->G * ld, kBlock)),
+                                       dim3(kBlock), 0, 0, (const double*)Rg, ld, (const int64_t*)D->grp_run_off,
+                                       (int64_t)D->G, Sd);
+                    FEHIP_LAUNCH_CHECK("group sums");
+                    if ((e = meat_rows(Sd, D->G, Mw))) return e;
+                    db.release(Sd);
+                    nw = Geff[plan.wide_dim];
+                } else {  // the cells (segment, group): runs, merged where a segment returns to a group
+                    if (hw[1] > 0 && !merged) {
+                        hipLaunchKernelGGL(merge_cells_kernel, dim3(stream_grid((size_t)D->n_runs * ld, kBlock)),
+                                           dim3(kBlock), 0, 0, Rg, ld, (const uint8_t*)cont, D->n_runs);
+                        FEHIP_LAUNCH_CHECK("merge cells");
+                        merged = true;
+                    }
+                    if ((e = meat_rows(Rg, D->n_runs, Mw))) return e;
+                    nw = (int64_t)hw[0];
+                }
+                pre[mask] = std::make_pair(Mw, nw);
+            }
         }
         for (int mask = 1; mask < (1 << ncd); ++mask) {
             unsigned dims_mask = 0;
@@ -2427,24 +2695,31 @@ int finish_request(Panel* p, ReqState& s) {
                 min_single = std::min<int64_t>(min_single, ncl);
             }
             result->n_clusters[mask - 1] = ncl;
-            // 2SLS: the meat of r X^ is proj' M_z proj
-            const std::vector<double>* Mp = &M;
-            std::vector<double> Mx;
-            if (use_iv && (e = project_meat(M, n_aug, proj, Kx, Mx))) return e;
-            if (use_iv) Mp = &Mx;
-            // cov += w Qinv M_kept Qinv
-            for (int a = 0; a < kept; ++a)
-                for (int c = 0; c < kept; ++c) {
-                    double s = 0.0;
-                    for (int k = 0; k < kept; ++k)
-                        s += (*Mp)[(size_t)keep[k] * Kx + keep[a]] * Qinv[(size_t)c * kept + k];
-                    tmp[(size_t)c * kept + a] = s;
+            for (size_t i2 = 0; i2 < M.size(); ++i2) Macc[i2] += wgt * M[i2];
+        }
+        // 2SLS: the meat of r X^ is proj' M_z proj
+        const std::vector<double>* Mp = &Macc;
+        std::vector<double> Mx;
+        if (use_iv && (e = project_meat(Macc, n_aug, proj, Kx, Mx))) return e;
+        if (use_iv) Mp = &Mx;
+        // cov = Qinv M_kept Qinv (column-major; the inner loops run down contiguous columns)
+        {
+            std::vector<double> Mk((size_t)kept * kept), tmp((size_t)kept * kept, 0.0);
+            for (int c = 0; c < kept; ++c)
+                for (int a = 0; a < kept; ++a) Mk[(size_t)c * kept + a] = (*Mp)[(size_t)keep[c] * Kx + keep[a]];
+            for (int c = 0; c < kept; ++c)
+                for (int k = 0; k < kept; ++k) {
+                    const double q = Qinv[(size_t)c * kept + k];
+                    const double* mk = Mk.data() + (size_t)k * kept;
+                    double* tc = tmp.data() + (size_t)c * kept;
+                    for (int a = 0; a < kept; ++a) tc[a] += mk[a] * q;
                 }
-            for (int a = 0; a < kept; ++a)
-                for (int c = 0; c < kept; ++c) {
-                    double s = 0.0;
-                    for (int k = 0; k < kept; ++k) s += Qinv[(size_t)k * kept + a] * tmp[(size_t)c * kept + k];
-                    cov[(size_t)c * kept + a] += wgt * s;
+            for (int c = 0; c < kept; ++c)
+                for (int k = 0; k < kept; ++k) {
+                    const double t = tmp[(size_t)c * kept + k];
+                    const double* qk = Qinv.data() + (size_t)k * kept;
+                    double* cc = cov.data() + (size_t)c * kept;
+                    for (int a = 0; a < kept; ++a) cc[a] += qk[a] * t;
                 }
         }
         const int nested_adj = dof_nested > 0 ? 1 : 0;

@@ -394,3 +394,54 @@ def test_config3_wide_2sls_matches_oracle():
     assert rel(fit.se, ref.se) < 1e-8
     for k in ("rss", "tss_within", "r2", "f_stat"):
         assert abs(fit.res[k] - ref.res[k]) <= 1e-8 * max(1.0, abs(ref.res[k])), k
+
+
+def _returning_panel(seed, n_workers, n_periods, n_firms, K):
+    """Workers moving between a home firm and a second one, so a worker's rows of one firm form several
+    runs (a return), with the rows shuffled (the panel sorts them)."""
+    rng = np.random.default_rng(seed)
+    home = rng.integers(1, n_firms + 1, n_workers)
+    other = rng.integers(1, n_firms + 1, n_workers)
+    at_home = rng.random((n_workers, n_periods)) < 0.6
+    firm = np.where(at_home, home[:, None], other[:, None]).astype(np.int32)
+    worker = np.repeat(np.arange(1, n_workers + 1, dtype=np.int32), n_periods)
+    period = np.tile(np.arange(1, n_periods + 1, dtype=np.int32), n_workers)
+    fe = np.vstack([worker, firm.reshape(-1), period])
+    n = fe.shape[1]
+    W = rng.normal(size=(n, K)) + 0.3 * rng.normal(size=n_workers)[worker - 1, None]
+    y = W @ rng.normal(size=K) + rng.normal(size=n_workers)[worker - 1] + rng.normal(size=n_firms + 1)[fe[1]] \
+        + rng.normal(size=n)
+    perm = rng.permutation(n)
+    fe = np.vstack([_redense(f) for f in fe[:, perm]])
+    return y[perm], np.asfortranarray(W[perm]), fe
+
+
+@pytest.mark.parametrize("cluster,weighted,precision", [((1, 2), False, 0), ((2,), False, 0), ((1,), False, 0),
+                                                        ((1, 2, 3), False, 0), ((1, 2), True, 0), ((1, 2), False, 1)])
+def test_wide_scores_cluster_sums(cluster, weighted, precision):
+    """Scores wider than the fused pass (K = 20): the clusters of the primary dim (workers), of a large
+    dim (firms, 1,500 > the LDS tables) and of their cells come from one pass of segment and run sums.
+    Workers return to an earlier firm, so a cell holds several runs (merged in group order); the 3-way
+    case mixes these subsets with the period subsets, which take the other methods; the weighted case
+    counts only the cells holding a weighted row.  FP32 storage (precision 1) is held to the FP32
+    bounds: beta 1e-6, SE 1e-5 relative."""
+    import oracle
+    from fe_gpu_amd import api
+    note(f"wide scores cluster={cluster} weighted={weighted} precision={precision}")
+    y, W, fe = _returning_panel(41, 2500, 12, 1500, 20)
+    n = len(y)
+    order = np.lexsort((fe[2], fe[0]))
+    f0, f1 = fe[0][order], fe[1][order]
+    runs = 1 + int(np.sum((f0[1:] != f0[:-1]) | (f1[1:] != f1[:-1])))
+    cells = len(np.unique(f0.astype(np.int64) * 4096 + f1))
+    assert runs > cells + 1000  # many returns
+    rng = np.random.default_rng(9)
+    w = (rng.random(n) < 0.7).astype(np.uint8) if weighted else None
+    fit = api.estimate(y, W, fe, weights=w, cluster=cluster, tol=1e-13, max_iter=20000, precision=precision)
+    m = np.ones(n, bool) if w is None else w == 1
+    fe_sub = np.vstack([_redense(fe[d][m]) for d in range(3)])
+    ref = oracle.estimate(y[m], W[m], fe_sub, cluster=cluster, tol=1e-13, max_iter=200000)
+    assert fit.res["n_clusters"] == ref.res["n_clusters"]
+    assert fit.res["dof_resid"] == ref.res["dof_resid"]
+    bt, st = (1e-8, 1e-8) if precision == 0 else (1e-6, 1e-5)
+    assert rel(fit.beta, ref.beta) < bt and rel(fit.se, ref.se) < st
