@@ -2438,13 +2438,23 @@ int finish_request(Panel* p, ReqState& s) {
             FEHIP_LAUNCH_CHECK("wide scores");
             unsigned long long hw[2] = {0, 0};
             if (D) FEHIP_CHECK(copy_sync(hw, wcounts, sizeof(hw), hipMemcpyDeviceToHost), "count copy");
+            // S'S at the padded width ld (zero columns past K, an even width for the pair loads), then
+            // its leading K x K block
+            double* Mdw = nullptr;
+            if ((e = db.get(&Mdw, (size_t)ld * ld))) return e;
+            std::vector<double> hw_meat((size_t)ld * ld);
             auto meat_rows = [&](const double* Sr, int64_t nS, std::vector<double>& Mo) -> int {
-                int e2 = gemm_impl(false, true, K, K, nS, 1.0, Sr, ld, Sr, ld, 0.0, Md, K, true);
+                int e2 = gemm_impl(false, true, ld, ld, nS, 1.0, Sr, ld, Sr, ld, 0.0, Mdw, ld, true);
                 if (e2) return e2;
+                FEHIP_CHECK(copy_sync(hw_meat.data(), Mdw, hw_meat.size() * sizeof(double), hipMemcpyDeviceToHost),
+                            "meat copy");
                 Mo.assign((size_t)K * K, 0.0);
-                FEHIP_CHECK(copy_sync(Mo.data(), Md, Mo.size() * sizeof(double), hipMemcpyDeviceToHost), "meat copy");
                 for (int j = 0; j < K; ++j)
-                    for (int i = j + 1; i < K; ++i) Mo[(size_t)j * K + i] = Mo[(size_t)i * K + j];
+                    for (int i = 0; i <= j; ++i) {
+                        const double v = hw_meat[(size_t)j * ld + i];
+                        Mo[(size_t)j * K + i] = v;
+                        Mo[(size_t)i * K + j] = v;
+                    }
                 return 0;
             };
             bool merged = false;

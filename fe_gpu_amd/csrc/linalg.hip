@@ -8,6 +8,7 @@
 //     FP64 atomic, so cluster-sorted rows pay one atomic per run and column.
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/fe_gpu_runtime.h"
@@ -102,26 +103,35 @@ __global__ __launch_bounds__(kBlock) void gemm_tile_kernel(bool tA, bool tB, int
 // C = A^T B for tall A (k x m) and B (k x n), column-major: the normal-equation products X'X, Z'Z, Z'X
 // of wide designs, on the FP64 matrix cores (v_mfma_f64_16x16x4f64).  A workgroup of four waves owns a
 // (32 NB) x (32 NB) output tile, each wave a (16 NB) x (16 NB) block of NB x NB MFMA tiles; slabs of
-// 32 rows of A and B are staged in LDS (loaded along the contiguous row direction) and each lane feeds
+// kSlab rows of A and B are staged in LDS (loaded along the contiguous direction) and each lane feeds
 // the matrix core its element (row = lane % 16 of the 16-block, k = lane / 16 of the 4-step).  Each
-// split of the reduction writes its own partial tile; the splits are summed in a fixed order.
+// split of the reduction writes its own partial tile; the splits are summed in a fixed order.  On the
+// diagonal tiles of a symmetric product (A = B) one operand is loaded, and with only the upper
+// triangle wanted the 16 x 16 blocks below the diagonal are skipped.
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
 // Workgroups are dealt to the 8 XCDs round-robin by linear id; the grid is laid out so that all
 // output tiles of one reduction split land on the same XCD one after another, and the row slabs they
 // all read are shared through that XCD's L2.
 constexpr int kXcds = 8;
+constexpr int kSlab = 16;  // reduction rows per LDS slab
 
 // RM: the reduction index runs along the leading dimension's complement, i.e. element (p, i) of the
 // k x m operand is A[p ldA + i] (the A B^T shape of S'S for row-major cluster sums); otherwise A[p + i ldA].
+// VEC: each load takes two adjacent elements along the contiguous direction (RM: m and n even;
+// otherwise k even), so a pair is either wholly in range or wholly out.  Out-of-range positions read
+// the operand's first element and are zeroed, without branches, so the loads of a slab stay in flight
+// together.  Three (NB = 3) or four (NB = 2) workgroups per CU keep the matrix cores fed while others
+// wait on their slabs.
 template <int NB, bool RM, bool VEC>
-__global__ __launch_bounds__(256) void gemm_tn_mfma_kernel(int m, int n, long long k, const double* __restrict__ A,
-                                                           long long ldA, const double* __restrict__ B, long long ldB,
-                                                           long long kchunk, int nsplit, int tiles_x, int tiles_y,
-                                                           double* partial, bool upper_only) {
+__global__ __launch_bounds__(256, NB == 2 ? 4 : (NB == 3 ? 3 : 2)) void gemm_tn_mfma_kernel(
+    int m, int n, long long k, const double* __restrict__ A, long long ldA, const double* __restrict__ B,
+    long long ldB, int nsplit, int tiles_x, int tiles_y, double* partial, bool upper_only) {
     constexpr int BT = 32 * NB;            // block tile
-    constexpr int KS = 32;                 // slab rows
+    constexpr int KS = kSlab;
+    constexpr int W = VEC ? 2 : 1;         // elements per load
     constexpr int PER = KS * BT / 256;     // elements of a slab each thread loads, per operand
+    static_assert(PER % W == 0, "slab elements per thread");
     __shared__ double As[KS][BT + 1];
     __shared__ double Bs[KS][BT + 1];
     const int ntiles = tiles_x * tiles_y;
@@ -133,113 +143,108 @@ __global__ __launch_bounds__(256) void gemm_tn_mfma_kernel(int m, int n, long lo
     // Split z takes the slabs z, z + nsplit, z + 2 nsplit, ... of the reduction: the workgroups in flight
     // read neighbouring rows, so a tall column-major operand's columns are read through a narrow window
     // of pages at a time (contiguous chunks per split would spread them over every page at once)
-    (void)kchunk;
     const long long kstride = (long long)nsplit * KS;
     const long long k0 = (long long)split * KS;
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int wi = (wave & 1) * 16 * NB, wj = (wave >> 1) * 16 * NB;
     const int lr = lane & 15, lk = lane >> 4;
-    double4_t acc[NB][NB];
-#pragma unroll
-    for (int a = 0; a < NB; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    // Software pipeline: the next slab is loaded into registers while the matrix cores work on the
-    // current one from LDS, so the wait for global memory falls behind a slab of MFMAs instead of in
-    // front of it.  A thread's elements walk the operand's contiguous direction (coalesced); with VEC
-    // (even leading dims, 16-byte aligned operands) each load takes two adjacent elements.
-    constexpr int W = VEC ? 2 : 1;  // elements per load
-    double ra[PER], rb[PER];
-    auto load_slab = [&](long long p0) {
-        const long long k1 = p0 + KS < k ? p0 + KS : k;
-#pragma unroll
-        for (int q = 0; q < PER / W; ++q) {
-            const int e = (t + 256 * q) * W;  // first of the W adjacent elements (same row / column)
-            const int pp = RM ? e / BT : e % KS, ii = RM ? e % BT : e / KS;
-            const long long p = p0 + pp;
-            const int ia = i0 + ii, jb = j0 + ii;
-            if constexpr (VEC) {
-                // the pair lies in one column (RM: one row); both halves in range or both out, since the
-                // slab rows and the tile columns come in even counts and k1, m, n bound them
-                const bool oka = RM ? (p < k1 && ia + 1 < m) : (p + 1 < k1 && ia < m);
-                const bool okb = RM ? (p < k1 && jb + 1 < n) : (p + 1 < k1 && jb < n);
-                const double* pa = RM ? A + p * ldA + ia : A + p + (long long)ia * ldA;
-                const double* pb = RM ? B + p * ldB + jb : B + p + (long long)jb * ldB;
-                double2 va = {0.0, 0.0}, vb = {0.0, 0.0};
-                if (oka) {
-                    va = *reinterpret_cast<const double2*>(pa);
-                } else {
-                    const bool ok0 = p < k1 && ia < m;  // a partial pair at the edge: elementwise
-                    const bool ok1 = RM ? (p < k1 && ia + 1 < m) : (p + 1 < k1 && ia < m);
-                    va.x = ok0 ? pa[0] : 0.0;
-                    va.y = ok1 ? pa[RM ? 1 : 1] : 0.0;
-                }
-                if (okb) {
-                    vb = *reinterpret_cast<const double2*>(pb);
-                } else {
-                    const bool ok0 = p < k1 && jb < n;
-                    const bool ok1 = RM ? (p < k1 && jb + 1 < n) : (p + 1 < k1 && jb < n);
-                    vb.x = ok0 ? pb[0] : 0.0;
-                    vb.y = ok1 ? pb[1] : 0.0;
-                }
-                ra[2 * q] = va.x;
-                ra[2 * q + 1] = va.y;
-                rb[2 * q] = vb.x;
-                rb[2 * q + 1] = vb.y;
-            } else {
-                ra[q] = (p < k1 && ia < m) ? (RM ? A[p * ldA + ia] : A[p + (long long)ia * ldA]) : 0.0;
-                rb[q] = (p < k1 && jb < n) ? (RM ? B[p * ldB + jb] : B[p + (long long)jb * ldB]) : 0.0;
-            }
+    // the element(s) of operand X at reduction row p, column c (bounds lim, k1), zero outside
+    auto fetch = [&](const double* X, long long ldX, int lim, long long p, long long k1, int c, double* dst) {
+        const bool ok = p < k1 && c < lim;
+        const long long off = ok ? (RM ? p * ldX + c : p + (long long)c * ldX) : 0;
+        if constexpr (VEC) {
+            const double2 v = *reinterpret_cast<const double2*>(X + off);
+            dst[0] = ok ? v.x : 0.0;
+            dst[1] = ok ? v.y : 0.0;
+        } else {
+            const double v = X[off];
+            dst[0] = ok ? v : 0.0;
         }
     };
-    auto store_slab = [&]() {
+    auto body = [&](auto same_tag) {
+        constexpr bool SAME = decltype(same_tag)::value;  // diagonal tile of A = B: Bs is As
+        double4_t acc[NB][NB];
 #pragma unroll
-        for (int q = 0; q < PER / W; ++q) {
-            const int e = (t + 256 * q) * W;
-            const int pp = RM ? e / BT : e % KS, ii = RM ? e % BT : e / KS;
+        for (int a = 0; a < NB; ++a)
 #pragma unroll
-            for (int h = 0; h < W; ++h) {
-                const int ph = RM ? pp : pp + h, ih = RM ? ii + h : ii;
-                As[ph][ih] = ra[W * q + h];
-                Bs[ph][ih] = rb[W * q + h];
+            for (int b = 0; b < NB; ++b) acc[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
+        // Software pipeline: the next slab is loaded into registers while the matrix cores work on the
+        // current one from LDS, so the wait for global memory falls behind a slab of MFMAs.
+        double ra[PER], rb[SAME ? 1 : PER];
+        auto load_slab = [&](long long p0) {
+            const long long k1 = p0 + KS < k ? p0 + KS : k;
+#pragma unroll
+            for (int q = 0; q < PER / W; ++q) {
+                const int e = (t + 256 * q) * W;  // first of the W adjacent elements (same row / column)
+                const int pp = RM ? e / BT : e % KS, ii = RM ? e % BT : e / KS;
+                fetch(A, ldA, m, p0 + pp, k1, i0 + ii, ra + W * q);
+                if constexpr (!SAME) fetch(B, ldB, n, p0 + pp, k1, j0 + ii, rb + W * q);
             }
+        };
+        auto store_slab = [&]() {
+#pragma unroll
+            for (int q = 0; q < PER / W; ++q) {
+                const int e = (t + 256 * q) * W;
+                const int pp = RM ? e / BT : e % KS, ii = RM ? e % BT : e / KS;
+#pragma unroll
+                for (int h = 0; h < W; ++h) {
+                    const int ph = RM ? pp : pp + h, ih = RM ? ii + h : ii;
+                    As[ph][ih] = ra[W * q + h];
+                    if constexpr (!SAME) Bs[ph][ih] = rb[W * q + h];
+                }
+            }
+        };
+        if (k0 < k) load_slab(k0);
+        for (long long p0 = k0; p0 < k; p0 += kstride) {
+            store_slab();
+            __syncthreads();
+            if (p0 + kstride < k) load_slab(p0 + kstride);  // in flight during this slab's MFMAs
+#pragma unroll
+            for (int kk = 0; kk < KS / 4; ++kk) {
+                double av[NB], bv[NB];
+#pragma unroll
+                for (int a = 0; a < NB; ++a) av[a] = As[kk * 4 + lk][wi + a * 16 + lr];
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    if constexpr (SAME)
+                        bv[b] = As[kk * 4 + lk][wj + b * 16 + lr];
+                    else
+                        bv[b] = Bs[kk * 4 + lk][wj + b * 16 + lr];
+                }
+#pragma unroll
+                for (int a = 0; a < NB; ++a)
+#pragma unroll
+                    for (int b = 0; b < NB; ++b) {
+                        if (SAME && upper_only && wi + a * 16 > wj + b * 16) continue;  // below the diagonal
+                        acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+                    }
+            }
+            __syncthreads();
         }
+        // D[i][j] of a 16 x 16 tile: j = lane % 16, i = lane / 16 + 4 v (measured lane layout of the
+        // f64 16x16x4 instruction on gfx950)
+        double* out = partial + (size_t)split * m * n;
+#pragma unroll
+        for (int a = 0; a < NB; ++a)
+#pragma unroll
+            for (int b = 0; b < NB; ++b)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int i = i0 + wi + a * 16 + lk + 4 * v, j = j0 + wj + b * 16 + lr;
+                    if (i < m && j < n && !(upper_only && i > j)) out[(size_t)j * m + i] = acc[a][b][v];
+                }
     };
-    if (k0 < k) load_slab(k0);
-    for (long long p0 = k0; p0 < k; p0 += kstride) {
-        store_slab();
-        __syncthreads();
-        if (p0 + kstride < k) load_slab(p0 + kstride);  // in flight during this slab's MFMAs
-#pragma unroll
-        for (int kk = 0; kk < KS / 4; ++kk) {
-            double av[NB], bv[NB];
-#pragma unroll
-            for (int a = 0; a < NB; ++a) av[a] = As[kk * 4 + lk][wi + a * 16 + lr];
-#pragma unroll
-            for (int b = 0; b < NB; ++b) bv[b] = Bs[kk * 4 + lk][wj + b * 16 + lr];
-#pragma unroll
-            for (int a = 0; a < NB; ++a)
-#pragma unroll
-                for (int b = 0; b < NB; ++b) acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
-        }
-        __syncthreads();
-    }
-    // D[i][j] of a 16 x 16 tile: j = lane % 16, i = lane / 16 + 4 v (measured lane layout of the
-    // f64 16x16x4 instruction on gfx950)
-    double* out = partial + (size_t)split * m * n;
-#pragma unroll
-    for (int a = 0; a < NB; ++a)
-#pragma unroll
-        for (int b = 0; b < NB; ++b)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int i = i0 + wi + a * 16 + lk + 4 * v, j = j0 + wj + b * 16 + lr;
-                if (i < m && j < n && !(upper_only && i > j)) out[(size_t)j * m + i] = acc[a][b][v];
-            }
+    if (i0 == j0 && A == B && ldA == ldB)
+        body(std::true_type{});
+    else
+        body(std::false_type{});
 }
 
+// C = alpha (sum of the split partials + ta tb') + beta C, where ta/tb (strides sa, sb; null when
+// absent) are the reduction row a pair-loading kernel left out (an odd row count)
 __global__ void gemm_finish_kernel(const double* __restrict__ partial, int nsplit, int m, int n, double alpha,
-                                   double beta, double* C, long long ldC, bool upper_only) {
+                                   double beta, double* C, long long ldC, bool upper_only, const double* ta,
+                                   long long sa, const double* tb, long long sb) {
     const size_t total = (size_t)m * n;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
@@ -247,6 +252,7 @@ __global__ void gemm_finish_kernel(const double* __restrict__ partial, int nspli
         if (upper_only && i > j) continue;
         double s = 0.0;
         for (int z = 0; z < nsplit; ++z) s += partial[(size_t)z * total + e];
+        if (ta) s += ta[(long long)i * sa] * tb[(long long)j * sb];
         double* cp = C + i + (long long)j * ldC;
         *cp = beta == 0.0 ? alpha * s : alpha * s + beta * (*cp);
     }
@@ -294,25 +300,30 @@ int gemm_tn_mfma(bool rm, int m, int n, long long k, double alpha, const double*
     if (upper_only)
         for (int a = 0; a < tx0; ++a)
             for (int b = 0; b < ty0; ++b) active -= (a * bt > b * bt + bt - 1) ? 1 : 0;
-    const bool vec = (ldA % 2 == 0) && (ldB % 2 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0);
-    // splits are dealt to the XCDs round-robin with all their tiles, so a multiple of the XCD count whose
-    // active workgroups fill each XCD's slots twice over (two rounds: the second hides the first's tail)
+    // pair loads: even leading dims, 16-byte aligned operands, and pairs that never straddle an edge
+    // (RM: even m and n; otherwise an even row count, an odd last row being added by the finish pass)
+    const bool vec = (ldA % 2 == 0) && (ldB % 2 == 0) && ((uintptr_t)A % 16 == 0) && ((uintptr_t)B % 16 == 0) &&
+                     (!rm || (m % 2 == 0 && n % 2 == 0)) && (rm || k >= 2);
+    const bool tail = vec && !rm && (k % 2 == 1);
+    const long long kk = tail ? k - 1 : k;  // rows the kernel reduces
+    // splits are dealt to the XCDs round-robin with all their tiles, so a multiple of the XCD count; the
+    // active workgroups fill each XCD's slots about eight times over: diagonal tiles of a symmetric
+    // product finish sooner than the others, and the many rounds even out the slots' finishing times
     const long long slots = (long long)resident_slots(nb, rm, vec);
-    long long nsplit = kXcds * std::max<long long>(1, (2 * slots) / (kXcds * std::max<long long>(1, active)));
-    nsplit = std::min<long long>(nsplit, std::max<long long>(1, k / 1024));
+    long long nsplit = kXcds * std::max<long long>(1, (8 * slots) / (kXcds * std::max<long long>(1, active)));
+    nsplit = std::min<long long>(nsplit, std::max<long long>(1, kk / 1024));
     nsplit = std::min<long long>(nsplit, std::max<long long>(1, (512LL << 20) / ((long long)m * n * 8)));
     if (fixed) nsplit = std::min<long long>(nsplit, std::max<long long>(1, (long long)(fixed_cap / ((size_t)m * n))));
-    // every split must own at least one 32-row slab (the slabs are interleaved across the splits)
-    nsplit = std::min<long long>(nsplit, (k + 31) / 32);
-    const long long kchunk = 0;  // unused by the interleaved kernel
+    // every split must own at least one slab (the slabs are interleaved across the splits)
+    nsplit = std::max<long long>(1, std::min<long long>(nsplit, (kk + kSlab - 1) / kSlab));
     double* partial = fixed ? fixed : static_cast<double*>(workspace((size_t)nsplit * m * n * sizeof(double), 2));
     if (!partial) return set_error("gemm: workspace allocation failed");
     const int tx = (m + bt - 1) / bt, ty = (n + bt - 1) / bt;
     const long long per_xcd = (nsplit + kXcds - 1) / kXcds;
     const unsigned grid = (unsigned)(kXcds * per_xcd * tx * ty);
 #define FEHIP_MFMA_LAUNCH(NBV, RMV, VECV)                                                                        \
-    hipLaunchKernelGGL((gemm_tn_mfma_kernel<NBV, RMV, VECV>), dim3(grid), dim3(256), 0, 0, m, n, k, A, ldA, B, ldB, \
-                       kchunk, (int)nsplit, tx, ty, partial, upper_only)
+    hipLaunchKernelGGL((gemm_tn_mfma_kernel<NBV, RMV, VECV>), dim3(grid), dim3(256), 0, 0, m, n, kk, A, ldA, B, ldB, \
+                       (int)nsplit, tx, ty, partial, upper_only)
 #define FEHIP_MFMA_RM(NBV)                                              \
     do {                                                                \
         if (rm) {                                                       \
@@ -330,7 +341,8 @@ int gemm_tn_mfma(bool rm, int m, int n, long long k, double alpha, const double*
 #undef FEHIP_MFMA_LAUNCH
     FEHIP_LAUNCH_CHECK("gemm mfma kernel");
     hipLaunchKernelGGL(gemm_finish_kernel, dim3(stream_grid((size_t)m * n, kBlock)), dim3(kBlock), 0, 0, partial,
-                       (int)nsplit, m, n, alpha, beta, C, ldC, upper_only);
+                       (int)nsplit, m, n, alpha, beta, C, ldC, upper_only, tail ? A + (k - 1) : nullptr, ldA,
+                       tail ? B + (k - 1) : nullptr, ldB);
     FEHIP_LAUNCH_CHECK("gemm finish kernel");
     return set_ok();
 }
@@ -382,7 +394,8 @@ int gemm_dispatch(bool tA, bool tB, int m, int n, long long k, double alpha, con
                        beta, C, ldC, partial, upper_only);
     FEHIP_LAUNCH_CHECK("gemm tile kernel");
     hipLaunchKernelGGL(gemm_finish_kernel, dim3(stream_grid((size_t)m * n, kBlock)), dim3(kBlock), 0, 0, partial,
-                       (int)nsplit, m, n, alpha, beta, C, ldC, upper_only);
+                       (int)nsplit, m, n, alpha, beta, C, ldC, upper_only, (const double*)nullptr, 0LL,
+                       (const double*)nullptr, 0LL);
     FEHIP_LAUNCH_CHECK("gemm finish kernel");
     return set_ok();
 }

@@ -142,6 +142,53 @@ def test_gemm_wide_cross_products_on_matrix_cores(lib, m, n, k, shape):
     assert rel(dC.to_array(np.float64, (m, n), order="F"), exp) < 1e-12
 
 
+@pytest.mark.parametrize("case", ["TN_odd_rows", "TN_same", "NT_even", "NT_same", "NT_odd"])
+def test_gemm_pair_loads_and_symmetric_tiles(lib, case):
+    """The matrix-core GEMM's pair loads (even leading dims): an odd row count on the A^T B shape leaves its
+    last row to the finish pass; A = B shares one operand on the diagonal tiles; the A B^T shape takes
+    pairs along even m and n, and single loads for odd ones."""
+    note(f"gemm pair loads {case}")
+    rng = np.random.default_rng(len(case))
+    k = 20011
+    if case.startswith("TN"):
+        m, n = (130, 130) if case == "TN_same" else (130, 120)
+        ld = k + 1                                   # even leading dim, odd row count
+        Ab = np.asfortranarray(rng.normal(size=(ld, m)))
+        Bb = Ab if case == "TN_same" else np.asfortranarray(rng.normal(size=(ld, n)))
+        exp = Ab[:k].T @ Bb[:k]
+        dA = dev(Ab)
+        dB = dA if case == "TN_same" else dev(Bb)
+        args = (b"T", b"N", m, n, k, 1.0, dA.ptr, ld, dB.ptr, ld)
+    else:
+        m, n = {"NT_even": (264, 120), "NT_same": (264, 264), "NT_odd": (265, 121)}[case]
+        ld = 272                                     # row-major view: element (p, i) at p * ld + i
+        Ab = np.asfortranarray(rng.normal(size=(ld, k)))
+        Bb = Ab if case == "NT_same" else np.asfortranarray(rng.normal(size=(ld, k)))
+        exp = Ab[:m] @ Bb[:n].T
+        dA = dev(Ab)
+        dB = dA if case == "NT_same" else dev(Bb)
+        args = (b"N", b"T", m, n, k, 1.0, dA.ptr, ld, dB.ptr, ld)
+    dC = dev(np.zeros((m, n), order="F"))
+    assert lib.fe_gpu_gemm(*args, 0.0, dC.ptr, m) == 0
+    assert rel(dC.to_array(np.float64, (m, n), order="F"), exp) < 1e-12
+
+
+def test_syrk_pair_loads_odd_rows_upper(lib):
+    """SYRK on the matrix cores with an even leading dim and an odd row count: pair loads, the last row
+    from the finish pass, one operand and no below-diagonal blocks on the diagonal tiles."""
+    note("syrk pair loads odd rows")
+    rng = np.random.default_rng(12)
+    n, K, ld = 30011, 140, 30012
+    Wb = np.asfortranarray(rng.normal(size=(ld, K)))
+    dW = dev(Wb)
+    dQ = dev(np.full((K, K), 7.0, order="F"))
+    assert lib.fe_gpu_syrk(n, K, 1.0, dW.ptr, ld, 0.0, dQ.ptr) == 0
+    Q = dQ.to_array(np.float64, (K, K), order="F")
+    iu = np.triu_indices(K)
+    assert rel(Q[iu], (Wb[:n].T @ Wb[:n])[iu]) < 1e-12
+    assert np.all(Q[np.tril_indices(K, -1)] == 7.0)
+
+
 def test_syrk_wide_upper_on_matrix_cores(lib):
     note("syrk mfma 140 cols")
     rng = np.random.default_rng(11)
