@@ -307,12 +307,13 @@ int gemm_tn_mfma(bool rm, int m, int n, long long k, double alpha, const double*
     const bool tail = vec && !rm && (k % 2 == 1);
     const long long kk = tail ? k - 1 : k;  // rows the kernel reduces
     // splits are dealt to the XCDs round-robin with all their tiles, so a multiple of the XCD count; the
-    // active workgroups fill each XCD's slots about eight times over: diagonal tiles of a symmetric
-    // product finish sooner than the others, and the many rounds even out the slots' finishing times
+    // active workgroups fill each XCD's slots about sixteen times over: diagonal tiles of a symmetric
+    // product finish sooner than the others, and the many short rounds even out the slots' finishing
+    // times (the partials stay within 1 GiB; the finish pass reads them once)
     const long long slots = (long long)resident_slots(nb, rm, vec);
-    long long nsplit = kXcds * std::max<long long>(1, (8 * slots) / (kXcds * std::max<long long>(1, active)));
+    long long nsplit = kXcds * std::max<long long>(1, (16 * slots) / (kXcds * std::max<long long>(1, active)));
     nsplit = std::min<long long>(nsplit, std::max<long long>(1, kk / 1024));
-    nsplit = std::min<long long>(nsplit, std::max<long long>(1, (512LL << 20) / ((long long)m * n * 8)));
+    nsplit = std::min<long long>(nsplit, std::max<long long>(1, (1024LL << 20) / ((long long)m * n * 8)));
     if (fixed) nsplit = std::min<long long>(nsplit, std::max<long long>(1, (long long)(fixed_cap / ((size_t)m * n))));
     // every split must own at least one slab (the slabs are interleaved across the splits)
     nsplit = std::max<long long>(1, std::min<long long>(nsplit, (kk + kSlab - 1) / kSlab));
