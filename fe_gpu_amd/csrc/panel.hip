@@ -2,6 +2,7 @@
 // src/core/fe_grouping.f90:11-37 requires), orders rows by the primary dim with a stable
 // radix sort, and builds the per-segment run structures and cached cluster partitions.
 #include <cstring>
+#include <cstdlib>
 #include <vector>
 
 #include <rocprim/rocprim.hpp>
@@ -166,6 +167,22 @@ __global__ void subset_keys_kernel(KeyDims kd, int nd, int64_t n, unsigned long 
         keys[i] = k;
         rows[i] = (int)i;
     }
+}
+
+// key = hi * glo + lo per row (hi < 2^31, glo <= 2^31: below 2^62), with the row index
+__global__ void pair_keys_kernel(const int* __restrict__ hi, unsigned long long glo, const int* __restrict__ lo,
+                                 int64_t n, unsigned long long* keys, int* rows) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        keys[i] = (unsigned long long)hi[i] * glo + (unsigned long long)lo[i];
+        rows[i] = (int)i;
+    }
+}
+
+// dense[rows[k]] = cid[k] - 1: the 0-based rank of each row's key
+__global__ void scatter_rank_kernel(const int* __restrict__ rows, const int* __restrict__ cid, int64_t n, int* dense) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) dense[rows[k]] = cid[k] - 1;
 }
 
 __global__ void heads_kernel(const unsigned long long* __restrict__ k, int64_t n, int* flags) {
@@ -402,49 +419,92 @@ int panel_partition(Panel* p, unsigned mask, const Partition** out, hipStream_t 
         part.off = p->seg_off;
         part.keys = p->d[p->P].ids;
     } else {
-        KeyDims kd{};
-        int nd = 0;
+        // Rows sorted by the subset's mixed-radix key (first dim least significant), clusters = runs of
+        // equal keys.  When the key needs more than 64 bits the dims are combined pairwise instead,
+        // most significant first: each stage sorts (dense id of the dims so far, next dim's id), a key
+        // below 2^62, and replaces the pair by its dense rank; the last stage's order and clusters are
+        // those of the single key.
+        std::vector<int> dims;
+        bool fits = true;
         unsigned long long span = 1;
         for (int d = 0; d < p->m; ++d) {
             if (!(mask & (1u << d))) continue;
-            if (nd == 4) return set_error("cluster subsets of more than 4 dims are not supported");
-            kd.ids[nd] = p->d[d].ids;
-            kd.stride[nd] = span;
-            unsigned long long g = (unsigned long long)p->d[d].G;
-            if (g != 0 && span > ~0ull / g) return set_error("cluster subset key exceeds 64 bits");
-            span *= g;
-            ++nd;
+            if ((int)dims.size() == 4) return set_error("cluster subsets of more than 4 dims are not supported");
+            dims.push_back(d);
+            const unsigned long long g = (unsigned long long)std::max(1, p->d[d].G);
+            if (span > ~0ull / g) fits = false;
+            else span *= g;
         }
+        const char* force = getenv("FEHIP_PARTITION_PAIRWISE");  // test hook: the pairwise path for any span
+        if (force && force[0] == '1' && dims.size() > 1) fits = false;
         const int64_t n = p->n;
         unsigned long long *k0, *k1;
-        int *r0, *r1, *flags, *cid;
+        int *r0, *r1, *flags, *cid, *dense = nullptr;
         int st;
         if ((st = dalloc(&k0, n))) return st;
         if ((st = dalloc(&k1, n))) return st;
         if ((st = dalloc(&r0, n))) return st;
         if ((st = dalloc(&r1, n))) return st;
-        hipLaunchKernelGGL(subset_keys_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, s, kd, nd, n, k0, r0);
-        FEHIP_LAUNCH_CHECK("subset keys");
         rocprim::double_buffer<unsigned long long> kb(k0, k1);
         rocprim::double_buffer<int> vb(r0, r1);
-        size_t tmp = 0;
-        int bits = bits_for(span - 1);
-        FEHIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, kb, vb, (size_t)n, 0, bits, s), "sort size");
-        void* t = workspace(tmp, 7);
-        if (!t) return set_error("sort workspace");
-        FEHIP_CHECK(rocprim::radix_sort_pairs(t, tmp, kb, vb, (size_t)n, 0, bits, s), "subset sort");
-        flags = reinterpret_cast<int*>(kb.alternate());
-        cid = flags + n;  // alternate key buffer holds 2n ints
-        hipLaunchKernelGGL(heads_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, s, kb.current(), n, flags);
-        FEHIP_LAUNCH_CHECK("heads");
-        size_t tmp2 = 0;
-        FEHIP_CHECK(rocprim::inclusive_scan(nullptr, tmp2, flags, cid, (size_t)n, rocprim::plus<int>(), s), "scan");
-        void* t2 = workspace(tmp2, 6);
-        if (!t2) return set_error("scan workspace");
-        FEHIP_CHECK(rocprim::inclusive_scan(t2, tmp2, flags, cid, (size_t)n, rocprim::plus<int>(), s), "cid scan");
+        // sort the (key, row) pairs over `bits`, flag the heads and number them (cid, 1-based)
+        auto rank_keys = [&](int bits, int* nc_out) -> int {
+            kb = rocprim::double_buffer<unsigned long long>(k0, k1);
+            vb = rocprim::double_buffer<int>(r0, r1);
+            size_t tmp = 0;
+            FEHIP_CHECK(rocprim::radix_sort_pairs(nullptr, tmp, kb, vb, (size_t)n, 0, bits, s), "sort size");
+            void* t = workspace(tmp, 7);
+            if (!t) return set_error("sort workspace");
+            FEHIP_CHECK(rocprim::radix_sort_pairs(t, tmp, kb, vb, (size_t)n, 0, bits, s), "subset sort");
+            flags = reinterpret_cast<int*>(kb.alternate());
+            cid = flags + n;  // alternate key buffer holds 2n ints
+            hipLaunchKernelGGL(heads_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, s, kb.current(), n,
+                               flags);
+            FEHIP_LAUNCH_CHECK("heads");
+            size_t tmp2 = 0;
+            FEHIP_CHECK(rocprim::inclusive_scan(nullptr, tmp2, flags, cid, (size_t)n, rocprim::plus<int>(), s),
+                        "scan");
+            void* t2 = workspace(tmp2, 6);
+            if (!t2) return set_error("scan workspace");
+            FEHIP_CHECK(rocprim::inclusive_scan(t2, tmp2, flags, cid, (size_t)n, rocprim::plus<int>(), s),
+                        "cid scan");
+            FEHIP_CHECK(hipMemcpyAsync(nc_out, cid + n - 1, sizeof(int), hipMemcpyDeviceToHost, s), "count");
+            FEHIP_CHECK(hipStreamSynchronize(s), "partition sync");
+            return 0;
+        };
         int nc = 0;
-        FEHIP_CHECK(hipMemcpyAsync(&nc, cid + n - 1, sizeof(int), hipMemcpyDeviceToHost, s), "count");
-        FEHIP_CHECK(hipStreamSynchronize(s), "partition sync");
+        if (fits) {
+            KeyDims kd{};
+            unsigned long long stride = 1;
+            for (size_t j = 0; j < dims.size(); ++j) {
+                kd.ids[j] = p->d[dims[j]].ids;
+                kd.stride[j] = stride;
+                stride *= (unsigned long long)std::max(1, p->d[dims[j]].G);
+            }
+            hipLaunchKernelGGL(subset_keys_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, s, kd,
+                               (int)dims.size(), n, k0, r0);
+            FEHIP_LAUNCH_CHECK("subset keys");
+            if ((st = rank_keys(bits_for(span - 1), &nc))) return st;
+        } else {
+            if ((st = dalloc(&dense, n))) return st;
+            const int* hi = p->d[dims.back()].ids;  // dense id of the dims combined so far (0-based)
+            unsigned long long ghi = (unsigned long long)std::max(1, p->d[dims.back()].G);
+            for (int j = (int)dims.size() - 2; j >= 0; --j) {
+                const int d = dims[j];
+                const unsigned long long glo = (unsigned long long)std::max(1, p->d[d].G);
+                hipLaunchKernelGGL(pair_keys_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, s, hi, glo,
+                                   (const int*)p->d[d].ids, n, k0, r0);
+                FEHIP_LAUNCH_CHECK("pair keys");
+                if ((st = rank_keys(bits_for(ghi * glo - 1), &nc))) return st;
+                if (j > 0) {  // the pair's dense rank per row feeds the next stage
+                    hipLaunchKernelGGL(scatter_rank_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, s,
+                                       (const int*)vb.current(), (const int*)cid, n, dense);
+                    FEHIP_LAUNCH_CHECK("pair ranks");
+                    hi = dense;
+                    ghi = (unsigned long long)std::max(1, nc);
+                }
+            }
+        }
         part.n_clusters = nc;
         if ((st = dalloc(&part.off, (size_t)nc + 1, &p->bytes))) return st;
         hipLaunchKernelGGL(head_offsets_kernel, dim3(stream_grid(n, kBlock)), dim3(kBlock), 0, s, flags, cid, n,
@@ -461,6 +521,7 @@ int panel_partition(Panel* p, unsigned mask, const Partition** out, hipStream_t 
         dfree(k1);
         dfree(r0);
         dfree(r1);
+        if (dense) dfree(dense);
     }
     auto res = p->partitions.emplace(mask, part);
     *out = &res.first->second;

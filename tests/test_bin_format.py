@@ -91,3 +91,37 @@ def test_library_header_errors(tmp_path):
     assert lib.fehip_bin_read_header(bad.encode(), C.byref(hh)) != 0
     with pytest.raises(ValueError):
         bf.read_header(bad)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+@pytest.mark.parametrize("name", ["loader_case_fp64_noflag.bin", "loader_case_fp32_flag.bin"])
+def test_reference_loader_fixtures(name):
+    """The two files of the reference's loader test (tests/test_data_loader.f90:17-132, written by
+    tests/golden/make_loader_fixtures.py): the numpy reader and the library's header parser decide the
+    layout and precision the reference test asserts, and the blocks equal its expected values to its
+    tolerances (1e-12 FP64, 1e-6 FP32)."""
+    import json
+    exp = json.load(open(os.path.join(GOLDEN, "loader_cases.json")))[name]
+    path = os.path.join(GOLDEN, name)
+    h, b = bf.read(path)
+    for k, v in exp["header"].items():
+        assert h[k] == v, k
+    tol = exp["tol"]
+    assert np.max(np.abs(b["y"] - exp["y"])) <= tol
+    assert np.max(np.abs(b["W"] - np.array(exp["W"]).T)) <= tol
+    assert np.array_equal(b["fe"], np.array(exp["fe"], dtype=np.int32))
+    if exp["cluster"] is None:
+        assert b["cluster"] is None
+    else:
+        assert np.array_equal(b["cluster"], exp["cluster"])
+    if exp["weights"] is None:
+        assert b["weights"] is None
+    else:
+        assert np.max(np.abs(b["weights"] - exp["weights"])) <= tol
+    from fe_gpu_amd import _coarse_sigs as S
+    from fe_gpu_amd._lib import load
+    hh = S.BinHeader()
+    assert load().fehip_bin_read_header(path.encode(), C.byref(hh)) == 0
+    assert {name_: getattr(hh, name_) for name_, _ in S.BinHeader._fields_} == h

@@ -74,3 +74,20 @@ def test_multi_chunk_file_and_regression_on_loaded_panel(tmp_path):
     assert np.max(np.abs(fit.se - ref.se) / np.abs(ref.se)) < 1e-12
     pan.close()
     d.free()
+
+
+@pytest.mark.parametrize("name", ["loader_case_fp64_noflag.bin", "loader_case_fp32_flag.bin"])
+def test_reference_loader_fixtures_on_device(name):
+    """The reference loader test's two files (tests/test_data_loader.f90:17-132): the device ingest equals
+    the reader bit for bit and the values that test expects (FP32 widened on the device)."""
+    import json
+    note(f"bin ingest fixture {name}")
+    golden = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+    exp = json.load(open(os.path.join(golden, "loader_cases.json")))[name]
+    d = _check_blocks(os.path.join(golden, name))
+    n = exp["header"]["n_obs"]
+    for k, v in exp["header"].items():
+        assert d.header[k] == v, k
+    assert np.max(np.abs(d.y.to_array(np.float64, n) - exp["y"])) <= exp["tol"]
+    assert np.array_equal(d.fe.to_array(np.int32, (len(exp["fe"]), n)), np.array(exp["fe"], dtype=np.int32))
+    d.free()

@@ -445,3 +445,24 @@ def test_wide_scores_cluster_sums(cluster, weighted, precision):
     assert fit.res["dof_resid"] == ref.res["dof_resid"]
     bt, st = (1e-8, 1e-8) if precision == 0 else (1e-6, 1e-5)
     assert rel(fit.beta, ref.beta) < bt and rel(fit.se, ref.se) < st
+
+
+@pytest.mark.parametrize("K", [3, 20])
+def test_pairwise_subset_keys_match_oracle(K, monkeypatch):
+    """Cluster subsets whose mixed-radix key would pass 64 bits are partitioned pairwise (dense rank of the
+    dims so far, then the next dim).  The test hook FEHIP_PARTITION_PAIRWISE=1 sends every multi-dim subset
+    there; 3-way clustered fits (the sorted-partition route at K = 3, the partition sums at K = 20) then
+    match the oracle, whose builder takes its hash path only past 2^62 (cluster_id_builder.c:259-269).
+    At K = 3 the two large non-primary dims put {2,3} on a two-dim sorted partition; at K = 20 the
+    subsets with the 9-group dim ({1,3}, {2,3}, {1,2,3}) take two- and three-dim partitions."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    note(f"pairwise subset keys K={K}")
+    monkeypatch.setenv("FEHIP_PARTITION_PAIRWISE", "1")
+    G = (3000, 1500, 1100) if K == 3 else (2000, 1200, 9)
+    p = synth.random_fe_problem(91 + K, n=12000, K=K, G=G)
+    fit = api.estimate(p.y, p.W, p.fe, cluster=(1, 2, 3), tol=1e-13, max_iter=20000)
+    monkeypatch.delenv("FEHIP_PARTITION_PAIRWISE")
+    ref = oracle.estimate(p.y, p.W, p.fe, cluster=(1, 2, 3), tol=1e-13, max_iter=200000)
+    assert fit.res["n_clusters"] == ref.res["n_clusters"]
+    assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
