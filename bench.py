@@ -713,7 +713,7 @@ def main():
     lib.fehip_kstats_enable(1)
     api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kw)
     lib.fehip_kstats_enable(0)
-    ks = {k: kstat(lib, k) for k in range(8)}
+    ks = {k: kstat(lib, k) for k in range(9)}
     # accuracy of the timed fit: the same request converged to tol 1e-12 on the same path is the reference
     tight = api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, cluster=(1, 2, 3), tol=1e-12, max_iter=5000)
     accuracy = {"reference": "same request, FP64, tol 1e-12", "reference_fe_iterations": tight.res["fe_iterations"],
@@ -755,7 +755,7 @@ def main():
         lib.fehip_kstats_enable(1)
         api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kwm)
         lib.fehip_kstats_enable(0)
-        mk = {k: kstat(lib, k) for k in range(8)}
+        mk = {k: kstat(lib, k) for k in range(9)}
         berr = np.abs(mfit.beta - fit.beta) / np.abs(fit.beta)
         serr = np.abs(mfit.se - fit.se) / np.abs(fit.se)
         berr_t = float(np.max(np.abs(mfit.beta - tight.beta) / np.abs(tight.beta)))

@@ -26,8 +26,8 @@ bool g_on = false;
 std::vector<Pending> g_pending;
 hipEvent_t g_open[kStatCount];
 Acc g_acc[kStatCount];
-const char* g_names[kStatCount] = {"apply_iter", "apply_data", "finalize", "reduce_runs",
-                                   "gram",       "segsum",     "apply_iter32", "apply_resid"};
+const char* g_names[kStatCount] = {"apply_iter", "apply_data", "finalize",    "reduce_runs", "gram",
+                                   "segsum",     "apply_iter32", "apply_resid", "couplings"};
 
 void resolve_locked() {
     for (auto& p : g_pending) {

@@ -9,10 +9,11 @@
 namespace fehip {
 
 // apply_iter32: the FP32 operator product of the mixed-precision solve; apply_resid: its FP64 products
-// for the true residual b - S x
+// for the true residual b - S x; couplings: the one-time operator products that build the run-level
+// operator's couplings with the small dims
 enum KStatId {
     kStatApplyIter = 0, kStatApplyData, kStatFinalize, kStatReduceRuns, kStatGram, kStatSegsum, kStatApplyIter32,
-    kStatApplyResid, kStatCount
+    kStatApplyResid, kStatCouplings, kStatCount
 };
 
 bool kstats_enabled();

@@ -108,21 +108,28 @@ __global__ void count_runs_kernel(const int64_t* __restrict__ seg_off, int64_t G
 }
 
 __global__ void write_runs_kernel(const int64_t* __restrict__ seg_off, int64_t G, const int* __restrict__ ids,
-                                  const int64_t* __restrict__ run_off, unsigned* run_grp, int* run_idx) {
+                                  const int64_t* __restrict__ run_off, unsigned* run_grp, int* run_idx, int* run_gid,
+                                  int* run_len) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < G; s += stride) {
         int64_t r0 = seg_off[s], r1 = seg_off[s + 1];
         int64_t k = run_off[s];
         int prev = -1;
+        int64_t start = r0;
         for (int64_t r = r0; r < r1; ++r) {
             int v = ids[r];
             if (r == r0 || v != prev) {
+                if (r > r0) run_len[k - 1] |= (int)(r - start);
                 run_grp[k] = (unsigned)v;
                 run_idx[k] = (int)k;
+                run_gid[k] = v;
+                run_len[k] = r == r0 ? (int)0x80000000u : 0;  // bit 31: the segment's first run
+                start = r;
                 ++k;
             }
             prev = v;
         }
+        if (r1 > r0) run_len[k - 1] |= (int)(r1 - start);
     }
 }
 
@@ -239,6 +246,12 @@ void free_panel(Panel* p) {
     dfree(p->long_segs);
     for (auto& kv : p->diag_cache) dfree(kv.second);
     p->diag_cache.clear();
+    for (auto& kv : p->run_couplings) {
+        dfree(kv.second.small_rows);
+        dfree(kv.second.B);
+        dfree(kv.second.C);
+    }
+    p->run_couplings.clear();
     dfree(p->wtile_seg);
     dfree(p->wtile_rec);
     dfree(p->mtile_seg);
@@ -250,6 +263,10 @@ void free_panel(Panel* p) {
         dfree(p->d[d].run_dest);
         dfree(p->d[d].row_rdest);
         dfree(p->d[d].grp_run_off);
+        dfree(p->d[d].run_gid);
+        dfree(p->d[d].run_len);
+        dfree(p->d[d].rtile_rec);
+        dfree(p->d[d].rlong_segs);
     }
     delete p;
 }
@@ -290,8 +307,10 @@ int build_runs(Panel* p, int d, hipStream_t s) {
     if ((st = dalloc(&grp2, R))) return st;
     if ((st = dalloc(&D.run_order, R, &p->bytes))) return st;
     if ((st = dalloc(&idx2, R))) return st;
+    if ((st = dalloc(&D.run_gid, R, &p->bytes))) return st;
+    if ((st = dalloc(&D.run_len, R, &p->bytes))) return st;
     hipLaunchKernelGGL(write_runs_kernel, dim3(stream_grid(GP, kBlock)), dim3(kBlock), 0, s, p->seg_off, GP, D.ids,
-                       D.seg_run_off, grp, D.run_order);
+                       D.seg_run_off, grp, D.run_order, D.run_gid, D.run_len);
     FEHIP_LAUNCH_CHECK("write runs");
     int* cnt = nullptr;
     if ((st = dalloc(&cnt, (size_t)D.G))) return st;

@@ -31,6 +31,26 @@ struct DimStruct {
     int64_t* grp_run_off = nullptr;  // [G + 1] offsets into run_order
     int* run_dest = nullptr;         // position of each run in group order (inverse of run_order)
     int* row_rdest = nullptr;        // [n] per row: run_dest of the run it ends, -1 inside a run
+    int* run_gid = nullptr;          // [n_runs] group of each run (runs in panel order)
+    int* run_len = nullptr;          // [n_runs] rows of each run; bit 31 set on a segment's first run
+    // run tiles (built on first use): whole primary segments packed into at most 64 runs each
+    // (first run | runs << 32), and the segments of more runs, listed apart
+    int64_t n_rtiles = -1;
+    int64_t* rtile_rec = nullptr;
+    int64_t n_rlong = 0;
+    int64_t* rlong_segs = nullptr;
+};
+
+// The couplings of one large non-primary dim D with the small absorbed dims (all of them, Gs groups)
+// in the FE-space operator S = D_o' M_P D_o of an unweighted solve: B = D_D' M_P D_s ([G_D][Gs]) and
+// C = D_s' M_P D_s ([Gs][Gs]), so that S p = (D_D' M_P D_D p_D + B p_s, B' p_D + C p_s) with the first
+// term formed from D's runs alone.  Built once per absorbed-dims mask.
+struct RunCoupling {
+    int o_large = -1;               // layout index of D
+    int Gs = 0;
+    int* small_rows = nullptr;      // [Gs] table rows of the small groups
+    double* B = nullptr;            // [G_D][Gs]
+    double* C = nullptr;            // [Gs][Gs]
 };
 
 // Row partition of a cluster subset: rows of cluster c are order[off[c] .. off[c+1]).
@@ -71,6 +91,8 @@ struct Panel {
     std::map<unsigned, int> nest_modes;
     // Schur diagonal and group counts of unweighted solves, per absorbed-dims mask ([S][2])
     std::map<unsigned, double*> diag_cache;
+    // run-level operator couplings of unweighted solves, per absorbed-dims mask
+    std::map<unsigned, RunCoupling> run_couplings;
     // measured time of one operator product per column-tile width (index into the width list), per
     // absorbed-dims mask; filled by the within transform's tuner on first use
     std::map<unsigned, std::vector<float>> width_ms;

@@ -20,6 +20,7 @@
 // Stopping rule: max over groups of |r_g / n_g|, the size of the group-mean update the next
 // alternating projection would make -- the reference's criterion (fe_gpu_demean.f90:196-211,294).
 #include <algorithm>
+#include <cstdlib>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -38,6 +39,7 @@ constexpr int kBlock = 256;
 constexpr int kMaxScatter = 6;
 constexpr int kMaxLarge = 3;
 constexpr int kMaxCT = 16;
+constexpr int kMaxCouplingGroups = 64;  // small groups a run-level operator couples (LDS staging)
 
 enum Mode { kIter = 0, kData = 1, kDiag = 2, kFinal = 3 };
 
@@ -93,6 +95,7 @@ struct ApplyArgs {
     int op32;                  // 1: FP32 direction tables (p32) and run partials (runbuf32)
     const float* p32;          // kIter with op32: [S][CT rounded up to 4] FP32 copy of p
     double* dump;              // lean kernel: per-wave slots [waves][kMaxCT] that lanes ending no run write
+    const RunCoupling* rc;     // kIter: the run-level operator's couplings (host side; nullptr: row kernels)
 };
 
 // The final pass holds every row's transformed values of the tile's columns in FP64 registers, so
@@ -1125,6 +1128,204 @@ __global__ __launch_bounds__(kBlock) void reduce_small_kernel(const double* __re
     }
 }
 
+// ---- run-level operator ------------------------------------------------------------------------
+// An unweighted solve that absorbs the primary dim P, one large dim D and small dims s (Gs groups in
+// all) has S p = D_o' M_P D_o p = (A p_D + B p_s, B' p_D + C p_s) with A = D_D' M_P D_D and the
+// couplings B = D_D' M_P D_s, C = D_s' M_P D_s built once (RunCoupling).  A needs no row: in a primary
+// segment of n rows whose runs of D (group g_r, length l_r) give m = sum_r l_r p_D[g_r] / n, run r
+// contributes l_r (p_D[g_r] - m) to group g_r.
+
+// One wave per run tile (whole primary segments, at most 64 runs): a lane per run gathers its group's
+// row of p_D, a segmented scan of l p_D and of l over the tile's segments gives each segment's mean,
+// and every lane writes its run's partial l (p_D[g] - mean) at the run's group-order position (summed
+// per group by reduce_runs).  Table rows have stride ctp; bit 31 of run_len marks a segment's first run.
+template <int CT, typename TS>
+__global__ __launch_bounds__(kBlock) void run_wave_kernel(const TS* __restrict__ tab, int ctp, int64_t tab_off,
+                                                          const int64_t* __restrict__ rtile_rec, int64_t n_rtiles,
+                                                          const int* __restrict__ run_gid,
+                                                          const int* __restrict__ run_len,
+                                                          const int* __restrict__ run_dest, TS* __restrict__ out,
+                                                          const int* __restrict__ active) {
+    if (!any_active<CT>(active)) return;
+    constexpr bool kF32 = sizeof(TS) == 4;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; t < n_rtiles; t += wstride) {
+        const int64_t rec = rtile_rec[t];
+        const int64_t r0 = (int64_t)(uint32_t)rec;
+        const int nr = (int)(rec >> 32);
+        const bool valid = lane < nr;
+        const int64_t r = r0 + (valid ? lane : 0);
+        const int lw = run_len[r];
+        const int g = run_gid[r];
+        const int dest = run_dest[r];
+        const bool head = valid && (lane == 0 || lw < 0);
+        const TS l = valid ? (TS)(lw & 0x7fffffff) : (TS)0;
+        const TS* row = tab + (tab_off + g) * ctp;
+        TS v[CT], x[CT + 1];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) v[c] = ld_global(row + c);
+#pragma unroll
+        for (int c = 0; c < CT; ++c) x[c] = l * v[c];
+        x[CT] = l;
+        const unsigned long long H = __ballot(head);
+        const int ss = 63 - __clzll(H & mask_le(lane));
+        const unsigned long long nh = H & ~mask_le(lane);
+        const int se = nh ? __ffsll((long long)nh) - 2 : nr - 1;
+        const ScanMask smask(lane, ss);
+        if constexpr (kF32)
+            seg_scan_cols_f32<CT + 1>(x, ScanFactorsF(smask));
+        else
+            seg_scan_cols<CT + 1>(x, ScanFactors(smask));
+        const int se_addr = (se < 0 ? 0 : se) << 2;
+        const TS inv = (TS)1 / lane_value(x[CT], se_addr);
+        TS mean[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) mean[c] = lane_value(x[c], se_addr);
+        if (valid) {
+            TS* dst = out + (int64_t)dest * CT;
+#pragma unroll
+            for (int c = 0; c < CT; ++c) dst[c] = l * (v[c] - mean[c] * inv);
+        }
+    }
+}
+
+// Segments of more than 64 runs: one thread per listed segment, its mean over the runs, then each
+// run's partial.
+template <int CT, typename TS>
+__global__ __launch_bounds__(kBlock) void run_long_kernel(const TS* __restrict__ tab, int ctp, int64_t tab_off,
+                                                          const int64_t* __restrict__ segs, int64_t nseg,
+                                                          const int64_t* __restrict__ seg_run_off,
+                                                          const int* __restrict__ run_gid,
+                                                          const int* __restrict__ run_len,
+                                                          const int* __restrict__ run_dest, TS* __restrict__ out,
+                                                          const int* __restrict__ active) {
+    if (!any_active<CT>(active)) return;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += stride) {
+        const int64_t w = segs[i];
+        const int64_t r0 = seg_run_off[w], r1 = seg_run_off[w + 1];
+        double m[CT], nw = 0.0;
+#pragma unroll
+        for (int c = 0; c < CT; ++c) m[c] = 0.0;
+        for (int64_t r = r0; r < r1; ++r) {
+            const TS* row = tab + (tab_off + run_gid[r]) * ctp;
+            const double l = (double)(run_len[r] & 0x7fffffff);
+            nw += l;
+#pragma unroll
+            for (int c = 0; c < CT; ++c) m[c] += l * (double)row[c];
+        }
+        const double inv = 1.0 / nw;
+#pragma unroll
+        for (int c = 0; c < CT; ++c) m[c] *= inv;
+        for (int64_t r = r0; r < r1; ++r) {
+            const TS* row = tab + (tab_off + run_gid[r]) * ctp;
+            const double l = (double)(run_len[r] & 0x7fffffff);
+            TS* dst = out + (int64_t)run_dest[r] * CT;
+#pragma unroll
+            for (int c = 0; c < CT; ++c) dst[c] = (TS)(l * ((double)row[c] - m[c]));
+        }
+    }
+}
+
+constexpr int kCouplingTile = 64;  // groups of D per LDS tile of the coupling pass
+
+// One pass over B in tiles of kCouplingTile groups of D (B's rows and p_D's rows staged in LDS): the
+// large rows get q_D[g][c] += sum_t B[g][t] p_s[t][c], and each block leaves
+// part[blk][t][c] = sum over its groups of B[g][t] p_D[g][c].
+template <int CT, typename TS>
+__global__ __launch_bounds__(kBlock) void coupling_pass_kernel(const double* __restrict__ B, int Gs, int G,
+                                                               int64_t tab_off, const TS* __restrict__ tab, int ctp,
+                                                               const int* __restrict__ small_rows, double* q,
+                                                               double* part, const int* __restrict__ active) {
+    if (!any_active<CT>(active)) return;
+    __shared__ double sb[kCouplingTile][kMaxCouplingGroups + 1];
+    __shared__ double sp[kCouplingTile][kMaxCT + 1];
+    __shared__ double ps[kMaxCouplingGroups * kMaxCT];
+    for (int e = threadIdx.x; e < Gs * CT; e += blockDim.x)
+        ps[e] = (double)tab[(int64_t)small_rows[e / CT] * ctp + e % CT];
+    constexpr int kPairs = (kMaxCouplingGroups * kMaxCT + kBlock - 1) / kBlock;
+    double acc[kPairs];
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) acc[k] = 0.0;
+    for (int64_t g0 = (int64_t)blockIdx.x * kCouplingTile; g0 < G; g0 += (int64_t)gridDim.x * kCouplingTile) {
+        const int ng = (int)(G - g0 < kCouplingTile ? G - g0 : kCouplingTile);
+        __syncthreads();  // the previous tile's reads are done (and ps is in place)
+        for (int e = threadIdx.x; e < ng * Gs; e += blockDim.x) sb[e / Gs][e % Gs] = B[g0 * Gs + e];
+        for (int e = threadIdx.x; e < ng * CT; e += blockDim.x)
+            sp[e / CT][e % CT] = (double)tab[(tab_off + g0 + e / CT) * ctp + e % CT];
+        __syncthreads();
+        for (int e = threadIdx.x; e < ng * CT; e += blockDim.x) {
+            const int gi = e / CT, c = e % CT;
+            double s2 = 0.0;
+            for (int t = 0; t < Gs; ++t) s2 += sb[gi][t] * ps[t * CT + c];
+            q[(tab_off + g0 + gi) * CT + c] += s2;
+        }
+#pragma unroll
+        for (int k = 0; k < kPairs; ++k) {
+            const int pi = threadIdx.x + k * kBlock;
+            if (pi < Gs * CT) {
+                const int t = pi / CT, c = pi % CT;
+                double s2 = 0.0;
+                for (int gi = 0; gi < ng; ++gi) s2 += sb[gi][t] * sp[gi][c];
+                acc[k] += s2;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < kPairs; ++k) {
+        const int pi = threadIdx.x + k * kBlock;
+        if (pi < Gs * CT) part[(int64_t)blockIdx.x * Gs * CT + pi] = acc[k];
+    }
+}
+
+// q_s[t][c] = the blocks' parts summed in a fixed order + sum_t2 C[t][t2] p_s[t2][c]: one workgroup per
+// entry (t, c), written to the small group's table row
+template <int CT, typename TS>
+__global__ __launch_bounds__(kBlock) void coupling_final_kernel(const double* __restrict__ part, int nblk,
+                                                                const double* __restrict__ C, int Gs,
+                                                                const TS* __restrict__ tab, int ctp,
+                                                                const int* __restrict__ small_rows, double* q,
+                                                                const int* __restrict__ active) {
+    if (!any_active<CT>(active)) return;
+    __shared__ double ws[kBlock / kWave];
+    const int pi = blockIdx.x;
+    const int t = pi / CT, c = pi % CT;
+    double s2 = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += blockDim.x) s2 += part[(int64_t)b * Gs * CT + pi];
+    for (int t2 = threadIdx.x; t2 < Gs; t2 += blockDim.x)
+        s2 += C[t * Gs + t2] * (double)tab[(int64_t)small_rows[t2] * ctp + c];
+    s2 = wave_sum(s2);
+    if ((threadIdx.x & (kWave - 1)) == 0) ws[threadIdx.x / kWave] = s2;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double tt = 0.0;
+        for (int w = 0; w < kBlock / kWave; ++w) tt += ws[w];
+        q[(int64_t)small_rows[t] * CT + c] = tt;
+    }
+}
+
+// unit directions of small groups t0 .. t0 + nj - 1: P[small_rows[t0 + j]][j] = 1 (P cleared before)
+__global__ void coupling_units_kernel(double* P, int ct, const int* __restrict__ small_rows, int t0, int nj) {
+    const int j = threadIdx.x;
+    if (j < nj) P[(int64_t)small_rows[t0 + j] * ct + j] = 1.0;
+}
+
+// B[g][t0 + j] and C[t][t0 + j] from the operator's products Q ([S][ct]) of those unit directions
+__global__ void coupling_extract_kernel(const double* __restrict__ Q, int ct, int t0, int nj, int64_t tab_off, int G,
+                                        int Gs, const int* __restrict__ small_rows, double* B, double* C) {
+    const int64_t total = (int64_t)(G + Gs) * nj;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const int64_t i = e / nj;
+        const int j = (int)(e - i * nj);
+        if (i < G)
+            B[i * Gs + t0 + j] = Q[(tab_off + i) * ct + j];
+        else
+            C[(i - G) * Gs + t0 + j] = Q[(int64_t)small_rows[i - G] * ct + j];
+    }
+}
+
 // ---- CG in table space ([S][CT], one CG per column) ---------------------------------------------
 // tol_in: the level at which a round of a column's CG is done -- tol, or in a round of the mixed-precision
 // solve max(tol, delta * the criterion of the round's true residual).  A column stops (active = 0) once
@@ -1617,6 +1818,71 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
     return gw + gm + gl;
 }
 
+// The run-level operator product q = S p (kIter with couplings): D's block from its runs, reduced per
+// group, plus the couplings with the small groups.
+template <int CT>
+int run_op_apply(Panel* p, const Layout& L, const ApplyArgs& a, double* q, hipStream_t s, double on, bool timed,
+                 int sid) {
+    const RunCoupling& rc = *a.rc;
+    const int o = rc.o_large;
+    const ScatterDim& d = L.sd[o];
+    const DimStruct& D = p->d[L.dims[o]];
+    const bool f32 = a.op32 != 0;
+    const int ctp = f32 ? OpStore<float>::pad(CT) : CT;
+    if (timed) kstat_begin(sid, s);
+#define FEHIP_RUN_OP(TSV, TAB, OUT)                                                                                 \
+    do {                                                                                                           \
+        if (D.n_rtiles > 0) {                                                                                      \
+            const unsigned gw = std::min<unsigned>(                                                                \
+                (unsigned)((D.n_rtiles + kBlock / kWave - 1) / (kBlock / kWave)),                                  \
+                2u * resident_grid(run_wave_kernel<CT, TSV>, kBlock, 0, (size_t)1 << 30));                         \
+            hipLaunchKernelGGL((run_wave_kernel<CT, TSV>), dim3(gw), dim3(kBlock), 0, s, TAB, ctp, d.tab_off,      \
+                               (const int64_t*)D.rtile_rec, D.n_rtiles, (const int*)D.run_gid,                     \
+                               (const int*)D.run_len, (const int*)D.run_dest, OUT, a.active);                      \
+        }                                                                                                          \
+        if (D.n_rlong > 0)                                                                                         \
+            hipLaunchKernelGGL((run_long_kernel<CT, TSV>), dim3(stream_grid((size_t)D.n_rlong, kBlock)),            \
+                               dim3(kBlock), 0, s, TAB, ctp, d.tab_off, (const int64_t*)D.rlong_segs, D.n_rlong,   \
+                               (const int64_t*)D.seg_run_off, (const int*)D.run_gid, (const int*)D.run_len,        \
+                               (const int*)D.run_dest, OUT, a.active);                                             \
+    } while (0)
+    if (f32)
+        FEHIP_RUN_OP(float, a.p32, a.sd[o].runbuf32);
+    else
+        FEHIP_RUN_OP(double, a.p, a.sd[o].runbuf);
+#undef FEHIP_RUN_OP
+    FEHIP_LAUNCH_CHECK("run operator");
+    if (timed) {
+        // algorithmic bytes: the runs' group, length and destination, D's table rows once, the run
+        // partials written
+        const double es = f32 ? 4.0 : 8.0;
+        const double R = (double)D.n_runs;
+        kstat_end(sid, s, on > 0.0 ? 12.0 * R + es * on * ((double)d.G + R) : 0.0);
+    }
+    reduce_tables<CT>(p, L, a, o, q, 0, a.active, s, timed, on);
+    if (rc.Gs > 0) {
+        const int G = d.G;
+        // partial blocks within the per-block partial buffer ([apply_grid][Gs][kMaxCT])
+        const unsigned nblk = std::max(
+            1u, std::min<unsigned>(std::min(1024u, apply_grid(p)), (unsigned)((G + kCouplingTile - 1) / kCouplingTile)));
+#define FEHIP_COUPLING(TSV, TAB)                                                                                   \
+    do {                                                                                                           \
+        hipLaunchKernelGGL((coupling_pass_kernel<CT, TSV>), dim3(nblk), dim3(kBlock), 0, s, (const double*)rc.B,   \
+                           rc.Gs, G, d.tab_off, TAB, ctp, (const int*)rc.small_rows, q, a.small_part, a.active);   \
+        hipLaunchKernelGGL((coupling_final_kernel<CT, TSV>), dim3((unsigned)(rc.Gs * CT)), dim3(kBlock), 0, s,     \
+                           (const double*)a.small_part, (int)nblk, (const double*)rc.C, rc.Gs, TAB, ctp,           \
+                           (const int*)rc.small_rows, q, a.active);                                                \
+    } while (0)
+        if (f32)
+            FEHIP_COUPLING(float, a.p32);
+        else
+            FEHIP_COUPLING(double, a.p);
+#undef FEHIP_COUPLING
+        FEHIP_LAUNCH_CHECK("couplings");
+    }
+    return 0;
+}
+
 // One operator product (kIter) or right-hand side (kData) into q (table space [S][CT]).  n_on: the
 // columns this launch works on (the right-hand side: every column; an operator product: the columns
 // whose CG is still running, which the caller knows only when it has polled the state just before;
@@ -1626,6 +1892,12 @@ int apply_and_reduce(Panel* p, const Layout& L, ApplyArgs& a, int mode, double* 
                      int stat = -1) {
     const int sid = stat >= 0 ? stat : mode == kIter ? (a.op32 ? kStatApplyIter32 : kStatApplyIter) : kStatApplyData;
     const bool timed = kstats_enabled();
+    if (mode == kIter && a.rc) {
+        const int e = run_op_apply<CT>(p, L, a, q, s, n_on < 0 ? (double)a.ncols : (double)n_on, timed, sid);
+        if (e) return e;
+        FEHIP_LAUNCH_CHECK("reduce kernels");
+        return 0;
+    }
     if (timed) kstat_begin(sid, s);
     const unsigned grid = mode == kIter ? launch_pass<CT, kIter>(p, L, a, s) : launch_pass<CT, kData>(p, L, a, s);
     FEHIP_LAUNCH_CHECK("operator kernels");
@@ -1664,6 +1936,7 @@ struct Scratch {
     double* gram_rec = nullptr; // per-block cross-product records of the final pass
     CgState* st = nullptr;
     CgState* st_host = nullptr;  // per-thread pinned scratch
+    const RunCoupling* rc = nullptr;  // the run-level operator's couplings (nullptr: row kernels)
     ~Scratch() {
         for (void* v : owned) cache_free(v);
     }
@@ -1708,6 +1981,7 @@ ApplyArgs base_args(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm) 
     a.long_segs = p->long_segs;
     a.n_long = p->n_long;
     a.dump = sc.dump;
+    a.rc = sc.rc;
     return a;
 }
 
@@ -1734,6 +2008,130 @@ int slot_diagonal(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm, ui
         reduce_tables<2>(p, L, a, od, dg_slot, gt + gl, nullptr, s, false);
         FEHIP_LAUNCH_CHECK("diag reduce");
     }
+    return 0;
+}
+
+// The run-level operator applies to unweighted solves that absorb the primary dim and exactly one large
+// dim with runs, besides at most kMaxCouplingGroups small groups.  FEHIP_RUN_OPERATOR=0 keeps the row
+// kernels (for comparisons).
+bool run_operator_applies(Panel* p, const Layout& L, const uint32_t* wm) {
+    if (wm || !L.elim || L.S <= 0 || L.Gs > kMaxCouplingGroups) return false;
+    const char* env = getenv("FEHIP_RUN_OPERATOR");
+    if (env && env[0] == '0') return false;
+    int nl = 0, o_large = -1;
+    for (int o = 0; o < L.nd; ++o)
+        if (!L.sd[o].small) {
+            ++nl;
+            o_large = o;
+        }
+    return nl == 1 && p->d[L.dims[o_large]].run_gid != nullptr && p->d[L.dims[o_large]].n_runs > 0;
+}
+
+// Run tiles of dim d (once per panel): whole primary segments packed greedily into at most 64 runs,
+// segments of more runs listed apart.  Caller holds the panel's lock.
+int build_run_tiles(Panel* p, int dd, hipStream_t s) {
+    DimStruct& D = p->d[dd];
+    if (D.n_rtiles >= 0) return 0;
+    const int64_t GP = p->d[p->P].G;
+    std::vector<int64_t> off((size_t)GP + 1);
+    FEHIP_CHECK(hipMemcpyAsync(off.data(), D.seg_run_off, sizeof(int64_t) * (size_t)(GP + 1), hipMemcpyDeviceToHost, s),
+                "run offsets");
+    FEHIP_CHECK(hipStreamSynchronize(s), "run offsets");
+    std::vector<int64_t> recs, longs;
+    recs.reserve((size_t)(off[GP] / 48 + 16));
+    int64_t w = 0;
+    while (w < GP) {
+        const int64_t nr = off[w + 1] - off[w];
+        if (nr == 0) {
+            ++w;
+            continue;
+        }
+        if (nr > kWave) {
+            longs.push_back(w);
+            ++w;
+            continue;
+        }
+        int64_t e = w + 1;
+        while (e < GP && off[e + 1] - off[w] <= kWave) ++e;
+        recs.push_back((off[w] & 0xffffffffLL) | ((off[e] - off[w]) << 32));
+        w = e;
+    }
+    int e;
+    if ((e = dalloc(&D.rtile_rec, std::max<size_t>(1, recs.size()), &p->bytes))) return e;
+    if ((e = dalloc(&D.rlong_segs, std::max<size_t>(1, longs.size()), &p->bytes))) return e;
+    if (!recs.empty())
+        FEHIP_CHECK(hipMemcpyAsync(D.rtile_rec, recs.data(), sizeof(int64_t) * recs.size(), hipMemcpyHostToDevice, s),
+                    "run tiles");
+    if (!longs.empty())
+        FEHIP_CHECK(hipMemcpyAsync(D.rlong_segs, longs.data(), sizeof(int64_t) * longs.size(), hipMemcpyHostToDevice, s),
+                    "long run segments");
+    FEHIP_CHECK(hipStreamSynchronize(s), "run tiles");
+    D.n_rlong = (int64_t)longs.size();
+    D.n_rtiles = (int64_t)recs.size();
+    return 0;
+}
+
+// The couplings B, C of the absorbed-dims mask (built on first use, kept with the panel): the row
+// operator applied to unit directions of the small groups, sixteen at a time, gives B's and C's
+// columns.  Runs on the calling thread's stream before any capture.
+int run_couplings(Panel* p, const Layout& L, unsigned mask, Scratch& sc, hipStream_t s) {
+    std::lock_guard<std::recursive_mutex> guard(p->mu);
+    auto it = p->run_couplings.find(mask);
+    if (it != p->run_couplings.end()) {
+        sc.rc = &it->second;
+        return 0;
+    }
+    RunCoupling rc;
+    for (int o = 0; o < L.nd; ++o)
+        if (!L.sd[o].small) {
+            int e0 = build_run_tiles(p, L.dims[o], s);
+            if (e0) return e0;
+        }
+    std::vector<int> rows;
+    for (int o = 0; o < L.nd; ++o) {
+        if (!L.sd[o].small) {
+            rc.o_large = o;
+            continue;
+        }
+        for (int g = 0; g < L.sd[o].G; ++g) rows.push_back((int)(L.sd[o].tab_off + g));
+    }
+    rc.Gs = (int)rows.size();
+    const int G = L.sd[rc.o_large].G;
+    const int64_t S = L.S;
+    int e;
+    if ((e = dalloc(&rc.small_rows, (size_t)std::max(1, rc.Gs), &p->bytes)) ||
+        (e = dalloc(&rc.B, (size_t)std::max<int64_t>(1, (int64_t)G * rc.Gs), &p->bytes)) ||
+        (e = dalloc(&rc.C, (size_t)std::max(1, rc.Gs * rc.Gs), &p->bytes)))
+        return e;
+    if (rc.Gs > 0) {
+        FEHIP_CHECK(hipMemcpyAsync(rc.small_rows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, s),
+                    "coupling rows");
+        double* P = sc.tab;            // the CG arrays are not in use yet: [S][16] directions
+        double* Q = sc.tab + S * kMaxCT;  // and their products
+        for (int t0 = 0; t0 < rc.Gs; t0 += kMaxCT) {
+            const int nj = std::min(kMaxCT, rc.Gs - t0);
+            FEHIP_CHECK(hipMemsetAsync(P, 0, sizeof(double) * (size_t)(S * kMaxCT), s), "unit directions");
+            hipLaunchKernelGGL(coupling_units_kernel, dim3(1), dim3(kMaxCT), 0, s, P, kMaxCT,
+                               (const int*)rc.small_rows, t0, nj);
+            ApplyArgs a = base_args(p, L, sc, nullptr);
+            a.rc = nullptr;
+            a.p = P;
+            a.lean = 1;
+            a.ubit = ~0u;
+            a.anybit = ~0u;
+            for (int c = 0; c < kMaxCT; ++c) a.cbit[c] = ~0u;
+            a.ncols = kMaxCT;
+            a.active = nullptr;
+            if ((e = apply_and_reduce<kMaxCT>(p, L, a, kIter, Q, s, nj, kStatCouplings))) return e;
+            hipLaunchKernelGGL(coupling_extract_kernel, dim3(stream_grid((size_t)(G + rc.Gs) * nj, kBlock)),
+                               dim3(kBlock), 0, s, (const double*)Q, kMaxCT, t0, nj, L.sd[rc.o_large].tab_off, G,
+                               rc.Gs, (const int*)rc.small_rows, rc.B, rc.C);
+            FEHIP_LAUNCH_CHECK("coupling columns");
+        }
+        FEHIP_CHECK(hipStreamSynchronize(s), "couplings");
+    }
+    auto res = p->run_couplings.emplace(mask, rc);
+    sc.rc = &res.first->second;
     return 0;
 }
 
@@ -2018,11 +2416,16 @@ int time_width(Panel* p, const Layout& L, Scratch& sc, hipStream_t s, hipEvent_t
     a.lean = 1;  // the kernel of an unweighted single-request solve
     a.ubit = ~0u;
     double* q = sc.tab + L.S * kMaxCT;
-    unsigned grid = launch_pass<CT, kIter>(p, L, a, s);  // warm-up
-    reduce_tables<CT>(p, L, a, -1, q, grid, nullptr, s, false);
+    auto product = [&]() -> int {
+        if (a.rc) return run_op_apply<CT>(p, L, a, q, s, (double)CT, false, kStatApplyIter);
+        const unsigned grid = launch_pass<CT, kIter>(p, L, a, s);
+        reduce_tables<CT>(p, L, a, -1, q, grid, nullptr, s, false);
+        return 0;
+    };
+    int e = product();  // warm-up
+    if (e) return e;
     FEHIP_CHECK(hipEventRecord(ev0, s), "tuner event");
-    grid = launch_pass<CT, kIter>(p, L, a, s);
-    reduce_tables<CT>(p, L, a, -1, q, grid, nullptr, s, false);
+    if ((e = product())) return e;
     FEHIP_CHECK(hipEventRecord(ev1, s), "tuner event");
     FEHIP_CHECK(hipEventSynchronize(ev1), "tuner sync");
     FEHIP_CHECK(hipEventElapsedTime(ms, ev0, ev1), "tuner time");
@@ -2237,6 +2640,8 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
         for (int b = 0; b < nslots; ++b)
             if (used[b] && (e = slot_diagonal(p, L, sc, wm, 1u << b, sc.dg + (int64_t)b * S * 2, s))) return e;
     }
+    // the run-level operator for the CG products where it applies (the tuner below times it)
+    if (run_operator_applies(p, L, wm) && (e = run_couplings(p, L, mask, sc, s))) return e;
     // column tiles: the tuner's measured cost per width picks the split of ncols columns; with cross
     // products asked and at most ct_max columns, one tile holds them all
     std::vector<int> tiles;

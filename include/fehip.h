@@ -249,7 +249,8 @@ int fehip_synth_mask(int64_t n, double p, uint64_t seed, uint8_t* d_mask);
 /* Opt-in per-kernel timing with HIP events on the stream each kernel is launched on.  Kernel ids:
  * 0 apply_iter (the CG operator product, the dominant kernel), 1 apply_data (right-hand side),
  * 2 finalize (M_F columns), 3 reduce_runs, 4 gram, 5 segsum (cluster score sums), 6 apply_iter32 (the
- * FP32 operator product of a mixed-precision solve), 7 apply_resid (its FP64 true-residual products).
+ * FP32 operator product of a mixed-precision solve), 7 apply_resid (its FP64 true-residual products),
+ * 8 couplings (the one-time products that build the run-level operator's small-dim couplings).
  * bytes/flops accumulate the algorithmic work of every timed launch. */
 int fehip_kstats_enable(int on);
 int fehip_kstats_reset(void);

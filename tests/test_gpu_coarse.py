@@ -466,3 +466,48 @@ def test_pairwise_subset_keys_match_oracle(K, monkeypatch):
     ref = oracle.estimate(p.y, p.W, p.fe, cluster=(1, 2, 3), tol=1e-13, max_iter=200000)
     assert fit.res["n_clusters"] == ref.res["n_clusters"]
     assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
+
+
+@pytest.mark.parametrize("case", ["worker_firm_time", "with_region", "worker_firm", "mixed_precision", "returns"])
+def test_run_operator_matches_row_operator_and_oracle(case, monkeypatch):
+    """The run-level operator (an unweighted solve absorbing the primary dim, one large dim and small dims:
+    the large dim's block from its runs, the small dims through the couplings B = D_D'M_P D_s and
+    C = D_s'M_P D_s built once) gives the fit of the row kernels (FEHIP_RUN_OPERATOR=0) to 1e-10 and the
+    oracle's to 1e-8 (FP64) or to the FP32 bounds (beta 1e-6, SE 1e-5) for the mixed-precision solve.
+    Cases: worker x firm x period; plus a 50-group region dim (two small dims); worker x firm only (no
+    couplings); the FP32 operator rounds; workers returning to an earlier firm (several runs of one group
+    in a segment)."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    note(f"run operator {case}")
+    if case == "returns":
+        y, W, fe = _returning_panel(43, 3000, 12, 1500, 6)
+        cl = (1, 2)
+    else:
+        # the FP32 operator rounds need a table of more than 4,096 entries: 6,000 firms there
+        p = synth.worker_firm_panel(seed=61, n_workers=6000, n_periods=15,
+                                    n_firms=6000 if case == "mixed_precision" else 2500)
+        y, W = p.y, p.W
+        fe = np.vstack([_redense(p.fe[d]) for d in range(3)])
+        cl = (1, 2, 3)
+        if case == "with_region":
+            region = (np.arange(fe[1].max() + 1) % 50 + 1).astype(np.int32)[fe[1]]
+            fe = np.vstack([fe, region])
+            cl = (1, 2)
+        if case == "worker_firm":
+            fe = fe[:2]
+            cl = (1, 2)
+    prec = 1 if case == "mixed_precision" else 0
+    kw = dict(cluster=cl, tol=1e-13 if prec == 0 else 1e-12, max_iter=20000, precision=prec)
+    fit = api.estimate(y, W, fe, **kw)
+    monkeypatch.setenv("FEHIP_RUN_OPERATOR", "0")
+    row = api.estimate(y, W, fe, **kw)
+    monkeypatch.delenv("FEHIP_RUN_OPERATOR")
+    ref = oracle.estimate(y, W, fe, cluster=cl, tol=1e-13, max_iter=200000)
+    assert fit.res["n_clusters"] == ref.res["n_clusters"] and fit.res["dof_resid"] == ref.res["dof_resid"]
+    if prec == 0:
+        assert rel(fit.beta, row.beta) < 1e-10 and rel(fit.se, row.se) < 1e-10
+        assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
+    else:
+        assert rel(fit.beta, ref.beta) < 1e-6 and rel(fit.se, ref.se) < 1e-5
+        assert fit.res["fe_refinements"] >= 1
