@@ -656,7 +656,7 @@ def main():
     ap.add_argument("--serve-seq", type=int, default=2000)
     ap.add_argument("--serve-open", type=int, default=2000)
     ap.add_argument("--serve-cpu-requests", type=int, default=4)
-    ap.add_argument("--serve-cpu-rows", type=int, default=1_000_000)
+    ap.add_argument("--serve-cpu-rows", type=int, default=300_000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; sharding across GPUs is a design, "

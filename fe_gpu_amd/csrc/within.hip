@@ -1251,7 +1251,9 @@ __global__ __launch_bounds__(kBlock) void coupling_pass_kernel(const double* __r
     for (int64_t g0 = (int64_t)blockIdx.x * kCouplingTile; g0 < G; g0 += (int64_t)gridDim.x * kCouplingTile) {
         const int ng = (int)(G - g0 < kCouplingTile ? G - g0 : kCouplingTile);
         __syncthreads();  // the previous tile's reads are done (and ps is in place)
-        for (int e = threadIdx.x; e < ng * Gs; e += blockDim.x) sb[e / Gs][e % Gs] = B[g0 * Gs + e];
+        // a wave per row of B: the row's Gs entries along the lanes (no division by the runtime Gs)
+        for (int gi = threadIdx.x / kWave; gi < ng; gi += kBlock / kWave)
+            for (int t = threadIdx.x & (kWave - 1); t < Gs; t += kWave) sb[gi][t] = B[(g0 + gi) * Gs + t];
         for (int e = threadIdx.x; e < ng * CT; e += blockDim.x)
             sp[e / CT][e % CT] = (double)tab[(tab_off + g0 + e / CT) * ctp + e % CT];
         __syncthreads();
