@@ -1229,55 +1229,108 @@ __global__ __launch_bounds__(kBlock) void run_long_kernel(const TS* __restrict__
 }
 
 constexpr int kCouplingTile = 64;  // groups of D per LDS tile of the coupling pass
+typedef double double4_t __attribute__((ext_vector_type(4)));
 
-// One pass over B in tiles of kCouplingTile groups of D (B's rows and p_D's rows staged in LDS): the
-// large rows get q_D[g][c] += sum_t B[g][t] p_s[t][c], and each block leaves
-// part[blk][t][c] = sum over its groups of B[g][t] p_D[g][c].
+// One pass over B in tiles of kCouplingTile groups of D (B's rows and p_D's rows staged in LDS, zero past
+// Gs and CT) on the FP64 matrix cores (v_mfma_f64_16x16x4f64: A element (i, k) in lane i + 16 k, B
+// element (k, j) in lane j + 16 k, D[i][j] in lane j + 16 (i % 4), element i / 4): wave w adds
+// (B p_s) of the tile's rows 16 w .. 16 w + 15 to q_D, and accumulates its quarter of the tile's
+// reduction in B' p_D, kept across tiles; the block's four quarters are summed at the end into
+// part[blk][t][c].
 template <int CT, typename TS>
 __global__ __launch_bounds__(kBlock) void coupling_pass_kernel(const double* __restrict__ B, int Gs, int G,
                                                                int64_t tab_off, const TS* __restrict__ tab, int ctp,
                                                                const int* __restrict__ small_rows, double* q,
                                                                double* part, const int* __restrict__ active) {
+    static_assert(kCouplingTile == 4 * 16 && kMaxCouplingGroups == 64 && kMaxCT <= 16, "coupling tile shape");
     if (!any_active<CT>(active)) return;
-    __shared__ double sb[kCouplingTile][kMaxCouplingGroups + 1];
-    __shared__ double sp[kCouplingTile][kMaxCT + 1];
-    __shared__ double ps[kMaxCouplingGroups * kMaxCT];
-    for (int e = threadIdx.x; e < Gs * CT; e += blockDim.x)
-        ps[e] = (double)tab[(int64_t)small_rows[e / CT] * ctp + e % CT];
-    constexpr int kPairs = (kMaxCouplingGroups * kMaxCT + kBlock - 1) / kBlock;
-    double acc[kPairs];
+    __shared__ double sb[kCouplingTile * kMaxCouplingGroups];  // the tile's rows of B, row stride Gs
+    __shared__ double sp[kCouplingTile][16 + 1];                // [g][c]
+    __shared__ double ps[kMaxCouplingGroups][16 + 1];           // p_s: [t][c], zero past Gs and CT
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int KS1 = (Gs + 3) / 4;   // k-steps over t (B p_s)
+    const int MB = (Gs + 15) / 16;  // 16-row blocks of t (B' p_D)
+    for (int e = threadIdx.x; e < kMaxCouplingGroups * 16; e += blockDim.x) {
+        const int t = e / 16, c = e % 16;
+        ps[t][c] = (t < Gs && c < CT) ? (double)tab[(int64_t)small_rows[t] * ctp + c] : 0.0;
+    }
+    double4_t acc2[4];
 #pragma unroll
-    for (int k = 0; k < kPairs; ++k) acc[k] = 0.0;
+    for (int mb = 0; mb < 4; ++mb) acc2[mb] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    constexpr int kBPer = kCouplingTile * kMaxCouplingGroups / kBlock, kPPer = kCouplingTile * 16 / kBlock;
     for (int64_t g0 = (int64_t)blockIdx.x * kCouplingTile; g0 < G; g0 += (int64_t)gridDim.x * kCouplingTile) {
         const int ng = (int)(G - g0 < kCouplingTile ? G - g0 : kCouplingTile);
-        __syncthreads();  // the previous tile's reads are done (and ps is in place)
-        // a wave per row of B: the row's Gs entries along the lanes (no division by the runtime Gs)
-        for (int gi = threadIdx.x / kWave; gi < ng; gi += kBlock / kWave)
-            for (int t = threadIdx.x & (kWave - 1); t < Gs; t += kWave) sb[gi][t] = B[(g0 + gi) * Gs + t];
-        for (int e = threadIdx.x; e < ng * CT; e += blockDim.x)
-            sp[e / CT][e % CT] = (double)tab[(tab_off + g0 + e / CT) * ctp + e % CT];
-        __syncthreads();
-        for (int e = threadIdx.x; e < ng * CT; e += blockDim.x) {
-            const int gi = e / CT, c = e % CT;
-            double s2 = 0.0;
-            for (int t = 0; t < Gs; ++t) s2 += sb[gi][t] * ps[t * CT + c];
-            q[(tab_off + g0 + gi) * CT + c] += s2;
+        const int nb = ng * Gs;  // the tile's entries of B, contiguous
+        // every load of the tile is issued (clamped addresses, no branch) before the LDS stores wait on them
+        double rb[kBPer], rp[kPPer];
+#pragma unroll
+        for (int k = 0; k < kBPer; ++k) {
+            const int e = threadIdx.x + kBlock * k;
+            const double v = B[g0 * Gs + (e < nb ? e : 0)];
+            rb[k] = e < nb ? v : 0.0;
         }
 #pragma unroll
-        for (int k = 0; k < kPairs; ++k) {
-            const int pi = threadIdx.x + k * kBlock;
-            if (pi < Gs * CT) {
-                const int t = pi / CT, c = pi % CT;
-                double s2 = 0.0;
-                for (int gi = 0; gi < ng; ++gi) s2 += sb[gi][t] * sp[gi][c];
-                acc[k] += s2;
+        for (int k = 0; k < kPPer; ++k) {
+            const int e = threadIdx.x + kBlock * k;
+            const int gi = e / 16, c = e % 16;
+            const bool ok = gi < ng && c < CT;
+            const double v = (double)tab[ok ? (tab_off + g0 + gi) * ctp + c : 0];
+            rp[k] = ok ? v : 0.0;
+        }
+        __syncthreads();  // the previous tile's reads are done
+#pragma unroll
+        for (int k = 0; k < kBPer; ++k) sb[threadIdx.x + kBlock * k] = rb[k];
+#pragma unroll
+        for (int k = 0; k < kPPer; ++k) {
+            const int e = threadIdx.x + kBlock * k;
+            sp[e / 16][e % 16] = rp[k];
+        }
+        __syncthreads();
+        // q_D rows 16 w + i: (B tile)(p_s); B entries past Gs or past the tile's rows read as zero
+        double4_t acc = {0.0, 0.0, 0.0, 0.0};
+        const int gr = w * 16 + lr;
+#pragma unroll
+        for (int ks = 0; ks < 16; ++ks)
+            if (ks < KS1) {
+                const int t = ks * 4 + lk;
+                const double a = (gr < ng && t < Gs) ? sb[gr * Gs + t] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, ps[t][lr], acc, 0, 0, 0);
+            }
+        if (lr < CT) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int gi = w * 16 + lk + 4 * v;
+                if (gi < ng) q[(tab_off + g0 + gi) * CT + lr] += acc[v];
             }
         }
-    }
+        // (B tile)' (p_D tile) over this wave's 16 groups of the tile
 #pragma unroll
-    for (int k = 0; k < kPairs; ++k) {
-        const int pi = threadIdx.x + k * kBlock;
-        if (pi < Gs * CT) part[(int64_t)blockIdx.x * Gs * CT + pi] = acc[k];
+        for (int mb = 0; mb < 4; ++mb)
+            if (mb < MB) {
+#pragma unroll
+                for (int ks = 0; ks < 4; ++ks) {
+                    const int gi = w * 16 + ks * 4 + lk, t = mb * 16 + lr;
+                    const double a = (gi < ng && t < Gs) ? sb[gi * Gs + t] : 0.0;
+                    acc2[mb] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, sp[gi][lr], acc2[mb], 0, 0, 0);
+                }
+            }
+    }
+    // the four waves' quarters, summed in wave order
+    __syncthreads();
+    double* red = sb;  // [4 waves][4 mb][64 lanes][4]: 4096 doubles
+#pragma unroll
+    for (int mb = 0; mb < 4; ++mb)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) red[((w * 4 + mb) * kWave + lane) * 4 + v] = acc2[mb][v];
+    __syncthreads();
+    for (int pi = threadIdx.x; pi < Gs * CT; pi += blockDim.x) {
+        const int t = pi / CT, c = pi % CT;
+        const int mb = t / 16, i = t % 16;
+        const int ln = c + 16 * (i % 4), v = i / 4;
+        double s2 = 0.0;
+        for (int ww = 0; ww < kBlock / kWave; ++ww) s2 += red[((ww * 4 + mb) * kWave + ln) * 4 + v];
+        part[(int64_t)blockIdx.x * Gs * CT + pi] = s2;
     }
 }
 
