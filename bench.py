@@ -76,17 +76,19 @@ def cpu_baseline(n_workers_full, n_periods, n_firms_full, sample_workers, seed):
 
 
 TRAFFIC_PROFILE = "profiles/apply_iter_150M_c2_hbm_bytes.txt"
+# the kernel of the CG operator product (apply_iter) at the headline workload: the run-level operator
+DOMINANT_KERNEL = "run_wave_kernel<9, double>"
 
 
 def measured_traffic(n_rows):
-    """HBM bytes per launch of the CG operator product on the 150M-row panel, from the committed
+    """HBM bytes per working launch of the CG operator product on the 150M-row panel, from the committed
     FETCH_SIZE / WRITE_SIZE profile (calibrated against a copy kernel); None for other sizes or when the
     profile is absent."""
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), TRAFFIC_PROFILE)
     if n_rows != 150_000_000 or not os.path.exists(path):
         return None
     for line in open(path):
-        if line.strip().endswith("wave_tile_kernel<9, 0, 2>"):
+        if line.strip().endswith(DOMINANT_KERNEL):
             return float(line.split()[5]) * 1e9
     return None
 
