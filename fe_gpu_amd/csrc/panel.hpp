@@ -46,7 +46,11 @@ struct DimStruct {
 // C = D_s' M_P D_s ([Gs][Gs]), so that S p = (D_D' M_P D_D p_D + B p_s, B' p_D + C p_s) with the first
 // term formed from D's runs alone.  Built once per absorbed-dims mask.
 struct RunCoupling {
-    int o_large = -1;               // layout index of D
+    // 0: S p = (A p_D + B p_s, B' p_D + C p_s) with A from D's runs in the primary segments (primary absorbed);
+    // 1: the primary not absorbed, so A = D_D' D_D is the diagonal of D's group counts;
+    // 2: no large dim, S = C
+    int kind = 0;
+    int o_large = -1;               // layout index of D (-1: none)
     int Gs = 0;
     int* small_rows = nullptr;      // [Gs] table rows of the small groups
     double* B = nullptr;            // [G_D][Gs]

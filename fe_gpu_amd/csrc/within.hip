@@ -96,6 +96,7 @@ struct ApplyArgs {
     const float* p32;          // kIter with op32: [S][CT rounded up to 4] FP32 copy of p
     double* dump;              // lean kernel: per-wave slots [waves][kMaxCT] that lanes ending no run write
     const RunCoupling* rc;     // kIter: the run-level operator's couplings (host side; nullptr: row kernels)
+    const double* dg;          // unweighted solves: Schur diagonal and group counts [S][2]
 };
 
 // The final pass holds every row's transformed values of the tile's columns in FP64 registers, so
@@ -1381,6 +1382,21 @@ __global__ void coupling_extract_kernel(const double* __restrict__ Q, int ct, in
     }
 }
 
+// q_D[g][c] = n_g p_D[g][c]: with the primary dim not absorbed every row lies in one group of D, so
+// A = D_D' D_D is the diagonal of D's group counts (the first entry of the Schur diagonal's rows)
+template <int CT>
+__global__ __launch_bounds__(kBlock) void diag_scale_kernel(const double* __restrict__ dg,
+                                                            const double* __restrict__ p, int64_t tab_off, int G,
+                                                            double* __restrict__ q, const int* __restrict__ active) {
+    if (!any_active<CT>(active)) return;
+    const int64_t total = (int64_t)G * CT;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const int64_t g = e / CT;
+        q[tab_off * CT + e] = dg[(tab_off + g) * 2] * p[tab_off * CT + e];
+    }
+}
+
 // ---- CG in table space ([S][CT], one CG per column) ---------------------------------------------
 // tol_in: the level at which a round of a column's CG is done -- tol, or in a round of the mixed-precision
 // solve max(tol, delta * the criterion of the round's true residual).  A column stops (active = 0) once
@@ -1880,10 +1896,41 @@ int run_op_apply(Panel* p, const Layout& L, const ApplyArgs& a, double* q, hipSt
                  int sid) {
     const RunCoupling& rc = *a.rc;
     const int o = rc.o_large;
-    const ScatterDim& d = L.sd[o];
-    const DimStruct& D = p->d[L.dims[o]];
     const bool f32 = a.op32 != 0;
     const int ctp = f32 ? OpStore<float>::pad(CT) : CT;
+    if (rc.kind != 0) {
+        // table-level products only: A p_D as the group counts times p_D (no primary absorbed), then the
+        // couplings; the FP64 direction is read (these solves keep the FP64 operator)
+        const int G = o >= 0 ? L.sd[o].G : 0;
+        const int64_t off = o >= 0 ? L.sd[o].tab_off : 0;
+        if (timed) kstat_begin(sid, s);
+        if (G > 0) {
+            hipLaunchKernelGGL((diag_scale_kernel<CT>), dim3(stream_grid((size_t)G * CT, kBlock)), dim3(kBlock), 0, s,
+                               a.dg, a.p, off, G, q, a.active);
+            FEHIP_LAUNCH_CHECK("diagonal product");
+        }
+        if (rc.Gs > 0) {
+            const unsigned nblk = G > 0 ? std::max(1u, std::min<unsigned>(std::min(1024u, apply_grid(p)),
+                                                                          (unsigned)((G + kCouplingTile - 1) / kCouplingTile)))
+                                        : 0u;
+            if (nblk > 0)
+                hipLaunchKernelGGL((coupling_pass_kernel<CT, double>), dim3(nblk), dim3(kBlock), 0, s,
+                                   (const double*)rc.B, rc.Gs, G, off, a.p, CT, (const int*)rc.small_rows, q,
+                                   a.small_part, a.active);
+            hipLaunchKernelGGL((coupling_final_kernel<CT, double>), dim3((unsigned)(rc.Gs * CT)), dim3(kBlock), 0, s,
+                               (const double*)a.small_part, (int)nblk, (const double*)rc.C, rc.Gs, a.p, CT,
+                               (const int*)rc.small_rows, q, a.active);
+            FEHIP_LAUNCH_CHECK("couplings");
+        }
+        // algorithmic bytes: p and q of D's rows, B read once, the small rows
+        if (timed)
+            kstat_end(sid, s, on > 0.0 ? 8.0 * (double)G * (2.0 * on + 1.0) + 8.0 * (double)G * rc.Gs +
+                                             8.0 * (double)rc.Gs * (rc.Gs + 2.0 * on)
+                                       : 0.0);
+        return 0;
+    }
+    const ScatterDim& d = L.sd[o];
+    const DimStruct& D = p->d[L.dims[o]];
     if (timed) kstat_begin(sid, s);
 #define FEHIP_RUN_OP(TSV, TAB, OUT)                                                                                 \
     do {                                                                                                           \
@@ -2037,6 +2084,7 @@ ApplyArgs base_args(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm) 
     a.n_long = p->n_long;
     a.dump = sc.dump;
     a.rc = sc.rc;
+    a.dg = sc.dg;
     return a;
 }
 
@@ -2066,20 +2114,26 @@ int slot_diagonal(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm, ui
     return 0;
 }
 
-// The run-level operator applies to unweighted solves that absorb the primary dim and exactly one large
-// dim with runs, besides at most kMaxCouplingGroups small groups.  FEHIP_RUN_OPERATOR=0 keeps the row
-// kernels (for comparisons).
-bool run_operator_applies(Panel* p, const Layout& L, const uint32_t* wm) {
-    if (wm || !L.elim || L.S <= 0 || L.Gs > kMaxCouplingGroups) return false;
+// The table-level operators apply to unweighted solves with at most kMaxCouplingGroups small groups and
+// at most one large scattered dim D:
+//  - the primary absorbed and D present with runs: A from the runs (kind 0);
+//  - the primary not absorbed and D present: A is diagonal (kind 1);
+//  - no large scattered dim: S = C (kind 2).
+// FEHIP_RUN_OPERATOR=0 keeps the row kernels (for comparisons).
+int table_operator_kind(Panel* p, const Layout& L, const uint32_t* wm) {
+    if (wm || L.S <= 0 || L.Gs > kMaxCouplingGroups) return -1;
     const char* env = getenv("FEHIP_RUN_OPERATOR");
-    if (env && env[0] == '0') return false;
+    if (env && env[0] == '0') return -1;
     int nl = 0, o_large = -1;
     for (int o = 0; o < L.nd; ++o)
         if (!L.sd[o].small) {
             ++nl;
             o_large = o;
         }
-    return nl == 1 && p->d[L.dims[o_large]].run_gid != nullptr && p->d[L.dims[o_large]].n_runs > 0;
+    if (nl == 0) return L.Gs > 0 ? 2 : -1;
+    if (nl > 1) return -1;
+    if (!L.elim) return 1;
+    return p->d[L.dims[o_large]].run_gid != nullptr && p->d[L.dims[o_large]].n_runs > 0 ? 0 : -1;
 }
 
 // Run tiles of dim d (once per panel): whole primary segments packed greedily into at most 64 runs,
@@ -2129,7 +2183,7 @@ int build_run_tiles(Panel* p, int dd, hipStream_t s) {
 // The couplings B, C of the absorbed-dims mask (built on first use, kept with the panel): the row
 // operator applied to unit directions of the small groups, sixteen at a time, gives B's and C's
 // columns.  Runs on the calling thread's stream before any capture.
-int run_couplings(Panel* p, const Layout& L, unsigned mask, Scratch& sc, hipStream_t s) {
+int run_couplings(Panel* p, const Layout& L, unsigned mask, int kind, Scratch& sc, hipStream_t s) {
     std::lock_guard<std::recursive_mutex> guard(p->mu);
     auto it = p->run_couplings.find(mask);
     if (it != p->run_couplings.end()) {
@@ -2137,8 +2191,9 @@ int run_couplings(Panel* p, const Layout& L, unsigned mask, Scratch& sc, hipStre
         return 0;
     }
     RunCoupling rc;
+    rc.kind = kind;
     for (int o = 0; o < L.nd; ++o)
-        if (!L.sd[o].small) {
+        if (!L.sd[o].small && kind == 0) {
             int e0 = build_run_tiles(p, L.dims[o], s);
             if (e0) return e0;
         }
@@ -2151,7 +2206,8 @@ int run_couplings(Panel* p, const Layout& L, unsigned mask, Scratch& sc, hipStre
         for (int g = 0; g < L.sd[o].G; ++g) rows.push_back((int)(L.sd[o].tab_off + g));
     }
     rc.Gs = (int)rows.size();
-    const int G = L.sd[rc.o_large].G;
+    const int G = rc.o_large >= 0 ? L.sd[rc.o_large].G : 0;
+    const int64_t off_large = rc.o_large >= 0 ? L.sd[rc.o_large].tab_off : 0;
     const int64_t S = L.S;
     int e;
     if ((e = dalloc(&rc.small_rows, (size_t)std::max(1, rc.Gs), &p->bytes)) ||
@@ -2179,7 +2235,7 @@ int run_couplings(Panel* p, const Layout& L, unsigned mask, Scratch& sc, hipStre
             a.active = nullptr;
             if ((e = apply_and_reduce<kMaxCT>(p, L, a, kIter, Q, s, nj, kStatCouplings))) return e;
             hipLaunchKernelGGL(coupling_extract_kernel, dim3(stream_grid((size_t)(G + rc.Gs) * nj, kBlock)),
-                               dim3(kBlock), 0, s, (const double*)Q, kMaxCT, t0, nj, L.sd[rc.o_large].tab_off, G,
+                               dim3(kBlock), 0, s, (const double*)Q, kMaxCT, t0, nj, off_large, G,
                                rc.Gs, (const int*)rc.small_rows, rc.B, rc.C);
             FEHIP_LAUNCH_CHECK("coupling columns");
         }
@@ -2250,7 +2306,7 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
     // A system that is diagonal (one scattered dim, primary not absorbed: Jacobi CG ends in one step) or
     // small (a few thousand table entries: steps are cheap) gains nothing from FP32 steps, and would pay
     // their FP64 true-residual products, so it stays on the FP64 operator.
-    const bool cheap = (L.nd == 1 && !L.elim) || S <= kOp32MinTable;
+    const bool cheap = (L.nd == 1 && !L.elim) || S <= kOp32MinTable || (sc.rc && sc.rc->kind != 0);
     const bool op32 = o.precision == 1 && uniform && sc.p32 != nullptr && !cheap;
     const double delta = op32 ? kInnerRel : 0.0;
     if (S > 0) {
@@ -2696,7 +2752,8 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
             if (used[b] && (e = slot_diagonal(p, L, sc, wm, 1u << b, sc.dg + (int64_t)b * S * 2, s))) return e;
     }
     // the run-level operator for the CG products where it applies (the tuner below times it)
-    if (run_operator_applies(p, L, wm) && (e = run_couplings(p, L, mask, sc, s))) return e;
+    const int op_kind = table_operator_kind(p, L, wm);
+    if (op_kind >= 0 && (e = run_couplings(p, L, mask, op_kind, sc, s))) return e;
     // column tiles: the tuner's measured cost per width picks the split of ncols columns; with cross
     // products asked and at most ct_max columns, one tile holds them all
     std::vector<int> tiles;

@@ -511,3 +511,33 @@ def test_run_operator_matches_row_operator_and_oracle(case, monkeypatch):
     else:
         assert rel(fit.beta, ref.beta) < 1e-6 and rel(fit.se, ref.se) < 1e-5
         assert fit.res["fe_refinements"] >= 1
+
+
+@pytest.mark.parametrize("fe_mask", [0b0110, 0b0010, 0b0101, 0b1100, 0b1110, 0b1101])
+def test_table_operator_matches_row_operator_and_oracle(fe_mask, monkeypatch):
+    """Unweighted solves without the run structure take table-level products: with the primary dim not
+    absorbed, one large dim D and small dims s give S p = (n_D p_D + B p_s, B' p_D + C p_s) with the counts
+    B = D_D'D_s, C = D_s'D_s ({firm, period}, {firm}, {firm, period, region}); with no large dim S = C
+    ({worker, period}: C = D_s'M_P D_s; {period, region}; {worker, period, region}).  Each fit equals the
+    row kernels' (FEHIP_RUN_OPERATOR=0) to 1e-10 and the oracle's on the same FE dims to 1e-8, with the
+    cluster dims inside the FE set."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    note(f"table operator fe_mask={fe_mask:04b}")
+    p = synth.worker_firm_panel(seed=73, n_workers=3000, n_periods=12, n_firms=1500)
+    fe3 = np.vstack([_redense(p.fe[d]) for d in range(3)])
+    region = (np.arange(fe3[1].max() + 1) % 40 + 1).astype(np.int32)[fe3[1] - 1]
+    fe = np.vstack([fe3, region])
+    dims = [d for d in range(4) if fe_mask >> d & 1]
+    cl = tuple(d + 1 for d in dims[:2])
+    kw = dict(cluster=cl, fe_mask=fe_mask, tol=1e-13, max_iter=20000)
+    fit = api.estimate(p.y, p.W, fe, **kw)
+    monkeypatch.setenv("FEHIP_RUN_OPERATOR", "0")
+    row = api.estimate(p.y, p.W, fe, **kw)
+    monkeypatch.delenv("FEHIP_RUN_OPERATOR")
+    ref = oracle.estimate(p.y, p.W, fe[dims], cluster=tuple(dims.index(c - 1) + 1 for c in cl), tol=1e-13,
+                          max_iter=200000)
+    assert fit.res["dof_fes"] == ref.res["dof_fes"] and fit.res["dof_resid"] == ref.res["dof_resid"]
+    assert fit.res["n_clusters"][:len(cl)] == ref.res["n_clusters"][:len(cl)]
+    assert rel(fit.beta, row.beta) < 1e-10 and rel(fit.se, row.se) < 1e-10
+    assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
