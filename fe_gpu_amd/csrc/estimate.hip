@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <memory>
 #include <set>
 #include <tuple>
 #include <type_traits>
@@ -2758,13 +2759,211 @@ int finish_request(Panel* p, ReqState& s) {
 
 constexpr int kMaxSlots = 32;
 
-// Solve a batch: requests that absorb the same FE dims share the within-transform sweeps (up to
-// 32 requests per solve, one mask bit each); every request then takes its own normal-equation
-// and sandwich step.
 // a chunk whose distinct columns number at most this many takes its cross products from the final
 // pass of the within-transform (one column tile)
 constexpr int kGramCols = 16;
 
+// One chunk of a batch: requests that absorb the same FE dims and share the within-transform sweeps (up
+// to 32 requests per solve, one mask bit each), and what the solve leaves for the requests' normal
+// equations and sandwiches.
+struct ColDst {
+    double* out;
+    float* out32;
+    double* alpha;
+};
+
+struct ChunkRun {
+    std::vector<int> idx;   // request indices
+    double bytes = 0.0;     // work storage of its requests
+    bool share = false;     // one copy per distinct column, shared by the requests
+    bool weighted = false;  // a request has a row subsample
+    unsigned mask = 0;
+    int group = -1;         // group of chunks whose right-hand sides and final pass are shared
+    DevBuf cb;              // the chunk's device buffers
+    uint32_t* wm = nullptr;
+    int ncols = 0, nu = 0;
+    std::vector<ColDst> dst;
+    std::vector<int> uidx, upos;
+    WithinCols wc{0};
+    double* dgram = nullptr;
+    std::vector<double> gh;  // cross products of the solved columns (nu x nu, upper triangle at [hi * nu + lo])
+    WithinStats wsu;
+};
+
+// Chunks of the same primary treatment (all absorb the primary dim, or none does), unweighted and with
+// shared column storage: their right-hand sides come from one pass over the union of their FE dims and
+// columns (each FE dim's block of D_o' M_P x, or D_o' x, does not depend on the other dims of the set).
+struct ChunkGroup {
+    unsigned mask = 0;
+    std::vector<int> members;
+    std::vector<const double*> xs;  // distinct input columns
+    SharedRhs rhs{};
+    bool has_rhs = false;
+    DevBuf buf;
+};
+
+const double* req_col(const ReqState& q, int j) {
+    return j == 0 ? q.d_y : (j <= q.K ? q.d_W + (int64_t)(j - 1) * q.ld : q.d_Z + (int64_t)(j - 1 - q.K) * q.ld);
+}
+
+// Preparation of the chunk's requests, its column list (a column that several requests share -- same
+// input, same row subsample, same storage precision -- is solved once) and its solve.
+int solve_chunk(Panel* p, std::vector<ReqState>& rs, ChunkRun& ch, const ChunkGroup* grp, BatchShared& shared) {
+    int e;
+    const int64_t n = p->n;
+    for (int i : ch.idx) {
+        rs[i].shared = ch.share;
+        if ((e = prepare_request(p, rs[i], shared))) return e;
+    }
+    ch.ncols = 0;
+    for (int i : ch.idx) ch.ncols += rs[i].C;
+    if (ch.weighted) {
+        std::vector<const uint8_t*> w(ch.idx.size());
+        for (size_t b = 0; b < ch.idx.size(); ++b) w[b] = rs[ch.idx[b]].d_weights;
+        if ((e = ch.cb.get(&ch.wm, (size_t)n))) return e;
+        if ((e = build_mask_words(p, w.data(), (int)ch.idx.size(), ch.wm, 0))) return e;
+    }
+    const int ncols = ch.ncols;
+    ch.dst.assign(ncols, ColDst{nullptr, nullptr, nullptr});
+    ch.uidx.assign(ncols, 0);
+    std::map<std::tuple<const double*, const uint8_t*, bool>, int> seen;
+    ch.wc = WithinCols(0);
+    WithinCols& wc = ch.wc;
+    int c = 0;
+    for (size_t b = 0; b < ch.idx.size(); ++b) {
+        ReqState& s = rs[ch.idx[b]];
+        if (ch.share) s.shared_cols.assign(s.C, nullptr);
+        for (int j = 0; j < s.C; ++j, ++c) {
+            const double* x = req_col(s, j);
+            const auto key = std::make_tuple(x, s.d_weights, s.f32);
+            auto it = seen.find(key);
+            if (ch.share) {
+                if (it == seen.end()) {  // first use: the column's one copy
+                    if (s.f32 ? (e = ch.cb.get(&ch.dst[c].out32, (size_t)n)) : (e = ch.cb.get(&ch.dst[c].out, (size_t)n)))
+                        return e;
+                }
+                const ColDst& d = ch.dst[it == seen.end() ? c : it->second];
+                s.shared_cols[j] = s.f32 ? (const void*)d.out32 : (const void*)d.out;
+            } else {
+                ch.dst[c] = {s.work ? s.work + (int64_t)j * n : nullptr, s.work32 ? s.work32 + (int64_t)j * n : nullptr,
+                             s.alpha ? s.alpha + (int64_t)j * s.S : nullptr};
+            }
+            if (it != seen.end() && (ch.share || (ch.dst[it->second].alpha != nullptr) == (ch.dst[c].alpha != nullptr))) {
+                ch.uidx[c] = it->second;
+                continue;
+            }
+            seen[key] = c;
+            ch.uidx[c] = c;
+            wc.x.push_back(x);
+            wc.out.push_back(ch.dst[c].out);
+            wc.out32.push_back(ch.dst[c].out32);
+            wc.alpha.push_back(ch.dst[c].alpha);
+            wc.slot.push_back(ch.wm ? (int)b : 0);
+        }
+    }
+    ch.nu = (int)wc.x.size();
+    ch.upos.assign(ncols, -1);  // position of each solved column in wc
+    for (int c2 = 0, u = 0; c2 < ncols; ++c2)
+        if (ch.uidx[c2] == c2) ch.upos[c2] = u++;
+    const ReqState& lead = rs[ch.idx[0]];
+    fehip_demean_opts dopt = lead.opts->demean;
+    dopt.fe_mask = lead.fe_mask;
+    if (grp && grp->has_rhs) {  // right-hand sides from the group's shared pass
+        wc.rhs = &grp->rhs;
+        wc.rhs_col.resize(ch.nu);
+        for (int u = 0; u < ch.nu; ++u) {
+            const auto f = std::find(grp->xs.begin(), grp->xs.end(), wc.x[u]);
+            if (f == grp->xs.end()) return set_error("batch: a column is missing from the shared right-hand sides");
+            wc.rhs_col[u] = (int)(f - grp->xs.begin());
+        }
+    }
+    if (ch.nu <= kGramCols) {
+        // cross products of the solved columns from the final pass (one column tile when they fit)
+        if ((e = ch.cb.get(&ch.dgram, (size_t)ch.nu * ch.nu))) return e;
+        wc.gram = ch.dgram;
+        wc.gram_ld = ch.nu;
+    }
+    ch.wsu = WithinStats(ch.nu);
+    if ((e = within_transform(p, wc, ch.nu, ch.wm, ch.wm ? (int)ch.idx.size() : 1, dopt, 1, ch.wsu, 0))) return e;
+    if (ch.dgram) {
+        ch.gh.assign((size_t)ch.nu * ch.nu, 0.0);
+        FEHIP_CHECK(copy_sync(ch.gh.data(), ch.dgram, ch.gh.size() * sizeof(double), hipMemcpyDeviceToHost), "gram copy");
+    }
+    return 0;
+}
+
+// The requests' cross products, the copies of columns shared by requests with their own storage, the
+// solve statistics, then every request's normal equations and sandwich.
+int finish_chunk(Panel* p, std::vector<ReqState>& rs, ChunkRun& ch) {
+    int e;
+    const int64_t n = p->n;
+    if (!ch.gh.empty()) {
+        int cc = 0;
+        for (int i : ch.idx) {
+            ReqState& s = rs[i];
+            std::vector<int> u(s.C);
+            bool one_tile = true;
+            for (int j = 0; j < s.C; ++j) {
+                u[j] = ch.upos[ch.uidx[cc + j]];
+                one_tile = one_tile && ch.wsu.col_tile[u[j]] == ch.wsu.col_tile[u[0]];
+            }
+            cc += s.C;
+            s.gram.clear();
+            if (!one_tile) continue;
+            s.gram.resize((size_t)s.C * s.C);
+            for (int b2 = 0; b2 < s.C; ++b2)
+                for (int a2 = 0; a2 < s.C; ++a2) {  // upper triangle of the solve's matrix, mirrored
+                    const int lo = std::min(u[a2], u[b2]), hi = std::max(u[a2], u[b2]);
+                    s.gram[(size_t)b2 * s.C + a2] = ch.gh[(size_t)hi * ch.nu + lo];
+                }
+        }
+    }
+    const int ncols = ch.ncols;
+    WithinStats ws(ncols);
+    for (int c2 = 0; c2 < ncols; ++c2) {
+        const int src = ch.uidx[c2], u = ch.upos[src];
+        ws.col_iters[c2] = ch.wsu.col_iters[u];
+        ws.col_converged[c2] = ch.wsu.col_converged[u];
+        ws.col_max_update[c2] = ch.wsu.col_max_update[u];
+        ws.col_refines[c2] = ch.wsu.col_refines[u];
+        if (src == c2 || ch.share) continue;
+        if (ch.dst[c2].out)
+            FEHIP_CHECK(hipMemcpyAsync(ch.dst[c2].out, ch.dst[src].out, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, 0),
+                        "shared column");
+        if (ch.dst[c2].out32)
+            FEHIP_CHECK(hipMemcpyAsync(ch.dst[c2].out32, ch.dst[src].out32, sizeof(float) * (size_t)n,
+                                       hipMemcpyDeviceToDevice, 0),
+                        "shared column");
+        if (ch.dst[c2].alpha && ch.dst[src].alpha)
+            FEHIP_CHECK(hipMemcpyAsync(ch.dst[c2].alpha, ch.dst[src].alpha, sizeof(double) * (size_t)within_table_size(p, ch.mask),
+                                       hipMemcpyDeviceToDevice, 0),
+                        "shared tables");
+    }
+    const auto t1 = clk::now();
+    int c = 0;
+    for (int i : ch.idx) {
+        ReqState& s = rs[i];
+        s.result->converged = 1;
+        for (int j = 0; j < s.C; ++j, ++c) {
+            s.result->fe_iterations = std::max(s.result->fe_iterations, ws.col_iters[c]);
+            s.result->converged = s.result->converged && ws.col_converged[c];
+            s.result->max_update = std::max(s.result->max_update, ws.col_max_update[c]);
+            s.result->fe_refinements = std::max(s.result->fe_refinements, ws.col_refines[c]);
+        }
+        s.t1 = t1;
+        s.result->t_demean = std::chrono::duration<double>(t1 - s.t0).count();
+    }
+    for (int i : ch.idx) {
+        if ((e = finish_request(p, rs[i]))) return e;
+        rs[i].db.clear();  // release the request's device buffers before the next chunk
+    }
+    ch.cb.clear();
+    return 0;
+}
+
+// Solve a batch: requests that absorb the same FE dims share the within-transform sweeps (chunks), chunks
+// of the same primary treatment share their right-hand-side pass, and every request then takes its own
+// normal-equation and sandwich step.
 int run_batch(Panel* p, std::vector<ReqState>& rs) {
     int e;
     const int64_t n = p->n;
@@ -2777,185 +2976,86 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
     size_t free_b = 0, total_b = 0;
     FEHIP_CHECK(hipMemGetInfo(&free_b, &total_b), "memory info");
     const double budget = 0.7 * (double)(free_b + cache_idle_bytes());
-    std::vector<int> done(rs.size(), 0);
     BatchShared shared;
-    for (size_t i0 = 0; i0 < rs.size(); ++i0) {
-        if (done[i0]) continue;
-        // next chunk: unsolved requests with the same FE set, in submission order, within the slot
-        // count and the memory budget (the first request always goes)
-        std::vector<int> chunk;
-        double bytes = 0.0;
-        const fehip_demean_opts& lead = rs[i0].opts->demean;
-        for (size_t i = i0; i < rs.size() && (int)chunk.size() < kMaxSlots; ++i) {
-            const fehip_demean_opts& o = rs[i].opts->demean;
-            if (done[i] || rs[i].fe_mask != rs[i0].fe_mask || o.tol != lead.tol || o.max_iter != lead.max_iter ||
-                o.check_every != lead.check_every)
-                continue;
-            const bool f32 = o.precision == 1 && !(rs[i].opts->n_iv_reg > 0 && rs[i].L > 0);
-            const double b = (f32 ? 4.0 : 8.0) * (double)n * (1 + std::max(0, rs[i].K) + std::max(0, rs[i].L)) +
-                             2.0 * (double)n;
-            if (!chunk.empty() && bytes + b > budget) break;
-            bytes += b;
-            chunk.push_back((int)i);
-        }
-        // Shared storage: when the chunk's distinct columns fit one column tile and no request is a 2SLS
-        // one, every request reads its transformed columns (and takes its cross products) from one copy
-        // per distinct column, and no request holds a work buffer of its own.
-        auto col_ptr = [](const ReqState& q, int j) {
-            return j == 0 ? q.d_y : (j <= q.K ? q.d_W + (int64_t)(j - 1) * q.ld : q.d_Z + (int64_t)(j - 1 - q.K) * q.ld);
-        };
-        bool share = true;
-        {
+    // chunks: unsolved requests with the same FE set, in submission order, within the slot count and the
+    // memory budget (the first request always goes)
+    std::vector<std::unique_ptr<ChunkRun>> runs;
+    {
+        std::vector<int> taken(rs.size(), 0);
+        for (size_t i0 = 0; i0 < rs.size(); ++i0) {
+            if (taken[i0]) continue;
+            auto ch = std::make_unique<ChunkRun>();
+            const fehip_demean_opts& lead = rs[i0].opts->demean;
+            for (size_t i = i0; i < rs.size() && (int)ch->idx.size() < kMaxSlots; ++i) {
+                const fehip_demean_opts& o = rs[i].opts->demean;
+                if (taken[i] || rs[i].fe_mask != rs[i0].fe_mask || o.tol != lead.tol || o.max_iter != lead.max_iter ||
+                    o.check_every != lead.check_every)
+                    continue;
+                const bool f32 = o.precision == 1 && !(rs[i].opts->n_iv_reg > 0 && rs[i].L > 0);
+                const double b = (f32 ? 4.0 : 8.0) * (double)n * (1 + std::max(0, rs[i].K) + std::max(0, rs[i].L)) +
+                                 2.0 * (double)n;
+                if (!ch->idx.empty() && ch->bytes + b > budget) break;
+                ch->bytes += b;
+                ch->idx.push_back((int)i);
+            }
+            for (int i : ch->idx) taken[i] = 1;
+            ch->mask = rs[i0].fe_mask;
+            // Shared storage: when the chunk's distinct columns fit one column tile and no request is a 2SLS
+            // one, every request reads its transformed columns (and takes its cross products) from one copy
+            // per distinct column, and no request holds a work buffer of its own.
             std::set<std::tuple<const double*, const uint8_t*, bool>> keys;
-            for (int i : chunk) {
+            bool share = true;
+            for (int i : ch->idx) {
                 const ReqState& q = rs[i];
                 const bool iv = q.opts->n_iv_reg > 0 && q.L > 0;
                 share = share && !iv;
                 const bool f32 = q.opts->demean.precision == 1 && !iv;
-                for (int j = 0; j < 1 + q.K + q.L; ++j) keys.insert(std::make_tuple(col_ptr(q, j), q.d_weights, f32));
+                ch->weighted |= q.d_weights != nullptr;
+                for (int j = 0; j < 1 + q.K + q.L; ++j) keys.insert(std::make_tuple(req_col(q, j), q.d_weights, f32));
             }
-            share = share && (int)keys.size() <= std::min(kGramCols, within_tile_cap(p, rs[i0].fe_mask));
+            ch->share = share && (int)keys.size() <= std::min(kGramCols, within_tile_cap(p, ch->mask));
+            runs.push_back(std::move(ch));
         }
-        for (int i : chunk) {
-            rs[i].shared = share;
-            if ((e = prepare_request(p, rs[i], shared))) return e;
+    }
+    // groups of chunks that share their right-hand-side and final passes
+    std::vector<std::unique_ptr<ChunkGroup>> groups;
+    for (unsigned elim = 0; elim < 2; ++elim) {
+        auto g = std::make_unique<ChunkGroup>();
+        for (size_t ci = 0; ci < runs.size(); ++ci) {
+            const ChunkRun& ch = *runs[ci];
+            if (!ch.share || ch.weighted || ((ch.mask >> p->P) & 1u) != elim) continue;
+            std::vector<const double*> xs = g->xs;
+            for (int i : ch.idx)  // (the requests are not prepared yet: C = 1 + K + L)
+                for (int j = 0; j < 1 + rs[i].K + rs[i].L; ++j)
+                    if (std::find(xs.begin(), xs.end(), req_col(rs[i], j)) == xs.end()) xs.push_back(req_col(rs[i], j));
+            const unsigned um = g->mask | ch.mask;
+            const int cap = std::min(kGramCols, within_tile_cap(p, um));
+            if ((int)xs.size() > cap) continue;
+            g->xs = xs;
+            g->mask = um;
+            g->members.push_back((int)ci);
         }
-        int ncols = 0;
-        bool weighted = false;
-        for (int i : chunk) {
-            ncols += rs[i].C;
-            weighted |= rs[i].d_weights != nullptr;
-        }
-        DevBuf cb;
-        uint32_t* wm = nullptr;
-        if (weighted) {
-            std::vector<const uint8_t*> w(chunk.size());
-            for (size_t b = 0; b < chunk.size(); ++b) w[b] = rs[chunk[b]].d_weights;
-            if ((e = cb.get(&wm, (size_t)n))) return e;
-            if ((e = build_mask_words(p, w.data(), (int)chunk.size(), wm, 0))) return e;
-        }
-        // the chunk's columns; a column that several requests share (same input, same row subsample, same
-        // storage precision) is solved once and copied to the others
-        struct ColDst {
-            double* out;
-            float* out32;
-            double* alpha;
-        };
-        std::vector<ColDst> dst(ncols);
-        std::vector<int> uidx(ncols);
-        std::map<std::tuple<const double*, const uint8_t*, bool>, int> seen;
-        WithinCols wc(0);
-        int c = 0;
-        for (size_t b = 0; b < chunk.size(); ++b) {
-            ReqState& s = rs[chunk[b]];
-            if (share) s.shared_cols.assign(s.C, nullptr);
-            for (int j = 0; j < s.C; ++j, ++c) {
-                const double* x = col_ptr(s, j);
-                const auto key = std::make_tuple(x, s.d_weights, s.f32);
-                auto it = seen.find(key);
-                if (share) {
-                    if (it == seen.end()) {  // first use: the column's one copy
-                        dst[c] = {nullptr, nullptr, nullptr};
-                        if (s.f32 ? (e = cb.get(&dst[c].out32, (size_t)n)) : (e = cb.get(&dst[c].out, (size_t)n)))
-                            return e;
-                    }
-                    const ColDst& d = dst[it == seen.end() ? c : it->second];
-                    s.shared_cols[j] = s.f32 ? (const void*)d.out32 : (const void*)d.out;
-                } else {
-                    dst[c] = {s.work ? s.work + (int64_t)j * n : nullptr,
-                              s.work32 ? s.work32 + (int64_t)j * n : nullptr,
-                              s.alpha ? s.alpha + (int64_t)j * s.S : nullptr};
-                }
-                if (it != seen.end() && (share || (dst[it->second].alpha != nullptr) == (dst[c].alpha != nullptr))) {
-                    uidx[c] = it->second;
-                    continue;
-                }
-                seen[key] = c;
-                uidx[c] = c;
-                wc.x.push_back(x);
-                wc.out.push_back(dst[c].out);
-                wc.out32.push_back(dst[c].out32);
-                wc.alpha.push_back(dst[c].alpha);
-                wc.slot.push_back(wm ? (int)b : 0);
-            }
-        }
-        const int nu = (int)wc.x.size();
-        std::vector<int> upos(ncols, -1);  // position of each solved column in wc
-        for (int c2 = 0, u = 0; c2 < ncols; ++c2)
-            if (uidx[c2] == c2) upos[c2] = u++;
-        fehip_demean_opts dopt = rs[i0].opts->demean;
-        dopt.fe_mask = rs[i0].fe_mask;
-        // cross products of the solved columns from the final pass (one column tile when they fit)
-        double* dgram = nullptr;
-        if (nu <= kGramCols) {
-            if ((e = cb.get(&dgram, (size_t)nu * nu))) return e;
-            wc.gram = dgram;
-            wc.gram_ld = nu;
-        }
-        WithinStats wsu(nu);
-        if ((e = within_transform(p, wc, nu, wm, wm ? (int)chunk.size() : 1, dopt, 1, wsu, 0))) return e;
-        if (dgram) {
-            std::vector<double> gh((size_t)nu * nu);
-            FEHIP_CHECK(copy_sync(gh.data(), dgram, gh.size() * sizeof(double), hipMemcpyDeviceToHost), "gram copy");
-            int cc = 0;
-            for (int i : chunk) {
-                ReqState& s = rs[i];
-                std::vector<int> u(s.C);
-                bool one_tile = true;
-                for (int j = 0; j < s.C; ++j) {
-                    u[j] = upos[uidx[cc + j]];
-                    one_tile = one_tile && wsu.col_tile[u[j]] == wsu.col_tile[u[0]];
-                }
-                cc += s.C;
-                s.gram.clear();
-                if (!one_tile) continue;
-                s.gram.resize((size_t)s.C * s.C);
-                for (int b2 = 0; b2 < s.C; ++b2)
-                    for (int a2 = 0; a2 < s.C; ++a2) {  // upper triangle of the solve's matrix, mirrored
-                        const int lo = std::min(u[a2], u[b2]), hi = std::max(u[a2], u[b2]);
-                        s.gram[(size_t)b2 * s.C + a2] = gh[(size_t)hi * nu + lo];
-                    }
-            }
-        }
-        WithinStats ws(ncols);
-        for (int c2 = 0; c2 < ncols; ++c2) {
-            const int src = uidx[c2], u = upos[src];
-            ws.col_iters[c2] = wsu.col_iters[u];
-            ws.col_converged[c2] = wsu.col_converged[u];
-            ws.col_max_update[c2] = wsu.col_max_update[u];
-            ws.col_refines[c2] = wsu.col_refines[u];
-            if (src == c2 || share) continue;
-            if (dst[c2].out)
-                FEHIP_CHECK(hipMemcpyAsync(dst[c2].out, dst[src].out, sizeof(double) * (size_t)n, hipMemcpyDeviceToDevice, 0),
-                            "shared column");
-            if (dst[c2].out32)
-                FEHIP_CHECK(hipMemcpyAsync(dst[c2].out32, dst[src].out32, sizeof(float) * (size_t)n,
-                                           hipMemcpyDeviceToDevice, 0),
-                            "shared column");
-            if (dst[c2].alpha && dst[src].alpha)
-                FEHIP_CHECK(hipMemcpyAsync(dst[c2].alpha, dst[src].alpha, sizeof(double) * (size_t)within_table_size(p, rs[i0].fe_mask),
-                                           hipMemcpyDeviceToDevice, 0),
-                            "shared tables");
-        }
-        const auto t1 = clk::now();
-        c = 0;
-        for (int i : chunk) {
-            ReqState& s = rs[i];
-            s.result->converged = 1;
-            for (int j = 0; j < s.C; ++j, ++c) {
-                s.result->fe_iterations = std::max(s.result->fe_iterations, ws.col_iters[c]);
-                s.result->converged = s.result->converged && ws.col_converged[c];
-                s.result->max_update = std::max(s.result->max_update, ws.col_max_update[c]);
-                s.result->fe_refinements = std::max(s.result->fe_refinements, ws.col_refines[c]);
-            }
-            s.t1 = t1;
-            s.result->t_demean = std::chrono::duration<double>(t1 - s.t0).count();
-            done[i] = 1;
-        }
-        for (int i : chunk) {
-            if ((e = finish_request(p, rs[i]))) return e;
-            rs[i].db.clear();  // release the request's device buffers before the next chunk
-        }
+        if (g->members.size() < 2) continue;
+        const int gi = (int)groups.size();
+        for (int ci : g->members) runs[ci]->group = gi;
+        groups.push_back(std::move(g));
+    }
+    for (auto& g : groups) {
+        if (within_table_size(p, g->mask) <= 0) continue;
+        double* b = nullptr;
+        if ((e = g->buf.get(&b, (size_t)within_table_size(p, g->mask) * kGramCols + 16))) return e;
+        int ld = 0;
+        if ((e = within_rhs(p, g->mask, g->xs.data(), (int)g->xs.size(), b, &ld, 0))) return e;
+        g->rhs.b = b;
+        g->rhs.ld = ld;
+        if ((e = within_dim_offsets(p, g->mask, g->rhs.dim_off))) return e;
+        g->has_rhs = true;
+    }
+    // solve and finish every chunk in turn (its buffers are released before the next chunk)
+    for (auto& chp : runs) {
+        ChunkRun& ch = *chp;
+        if ((e = solve_chunk(p, rs, ch, ch.group >= 0 ? groups[ch.group].get() : nullptr, shared))) return e;
+        if ((e = finish_chunk(p, rs, ch))) return e;
     }
     return 0;
 }

@@ -1715,6 +1715,39 @@ __global__ void take_column_kernel(const double* __restrict__ tab, int64_t S, in
         out[g] = tab[g * CT + c];
 }
 
+// Table blocks moved between two layouts of table space: entry g of dim block o goes from src_off[o] + g
+// (zero where src_off[o] < 0 or there is no source) to dst_off[o] + g.
+struct TableMap {
+    int nd;
+    int64_t dst_off[kMaxScatter];
+    int64_t src_off[kMaxScatter];
+    int G[kMaxScatter];
+};
+
+__device__ __forceinline__ int64_t map_entry(const TableMap& m, int64_t e) {
+#pragma unroll
+    for (int o = 0; o < kMaxScatter; ++o)
+        if (o < m.nd && e >= m.dst_off[o] && e < m.dst_off[o] + m.G[o])
+            return m.src_off[o] < 0 ? -1 : m.src_off[o] + (e - m.dst_off[o]);
+    return -1;
+}
+
+// r[e][c] = b[map(e)][col[c]] for the tile's columns (zero past ncols): a solve's right-hand sides taken
+// from a computation shared with other FE sets
+struct RhsCols {
+    int col[kMaxCT];
+};
+__global__ void rhs_gather_kernel(double* __restrict__ r, int CT, int64_t S, int ncols, const double* __restrict__ b,
+                                  int ldb, TableMap m, RhsCols rc) {
+    const int64_t total = S * CT;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = i / CT;
+        const int c = (int)(i - e * CT);
+        const int64_t se = map_entry(m, e);
+        r[i] = (c < ncols && se >= 0) ? b[se * ldb + rc.col[c]] : 0.0;
+    }
+}
+
 __global__ void mask_words_kernel(const uint8_t* __restrict__ w, const int* __restrict__ perm, int64_t n, uint32_t bit,
                                   int first, uint32_t* wm) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
@@ -2313,8 +2346,26 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
         hipLaunchKernelGGL((make_minv_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, s, S,
                            (const double*)sc.dg, cs, minv, wcnt);
         FEHIP_LAUNCH_CHECK("minv");
-        int e = apply_and_reduce<CT>(p, L, a, kData, r, s);  // b = D_o' W M_P x
-        if (e) return e;
+        int e = 0;
+        if (wc.rhs) {
+            // b from the right-hand sides shared with other FE sets: this layout's dim blocks, the tile's columns
+            TableMap m{};
+            m.nd = L.nd;
+            for (int o = 0; o < L.nd; ++o) {
+                m.dst_off[o] = L.sd[o].tab_off;
+                m.src_off[o] = wc.rhs->dim_off[L.dims[o]];
+                m.G[o] = L.sd[o].G;
+                if (m.src_off[o] < 0) return set_error("shared right-hand sides lack FE dim %d", L.dims[o] + 1);
+            }
+            RhsCols rcol{};
+            for (int c = 0; c < kMaxCT; ++c) rcol.col[c] = c < ncols ? wc.rhs_col[c0 + c] : 0;
+            hipLaunchKernelGGL(rhs_gather_kernel, dim3(stream_grid((size_t)(S * CT), kBlock)), dim3(kBlock), 0, s, r, CT,
+                               S, ncols, wc.rhs->b, wc.rhs->ld, m, rcol);
+            FEHIP_LAUNCH_CHECK("shared right-hand sides");
+        } else {
+            e = apply_and_reduce<CT>(p, L, a, kData, r, s);  // b = D_o' W M_P x
+            if (e) return e;
+        }
         if (op32) FEHIP_CHECK(hipMemcpyAsync(sc.bvec, r, sizeof(double) * (size_t)(S * CT), hipMemcpyDeviceToDevice, s),
                               "right-hand side copy");
         hipLaunchKernelGGL((cg_init_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, r, (const double*)minv,
@@ -2657,6 +2708,87 @@ int within_sweep_with_tables(Panel* p, unsigned fe_mask, const double* x, const 
     a.alpha = alpha;
     launch_pass<1, kFinal>(p, L, a, s);
     FEHIP_LAUNCH_CHECK("sweep with tables");
+    return 0;
+}
+
+// ---- right-hand sides shared by the solves of several unweighted FE sets (a batch's requests) ----
+
+static int tile_cap(const Layout& L);
+
+int within_dim_offsets(Panel* p, unsigned fe_mask, int64_t* off) {
+    unsigned mask = fe_mask ? fe_mask : ((1u << p->m) - 1u);
+    mask &= (1u << p->m) - 1u;
+    Layout L;
+    const int e = make_layout(p, mask, L);
+    if (e) return e;
+    for (int d = 0; d < FEHIP_MAX_FE; ++d) off[d] = -1;
+    for (int o = 0; o < L.nd; ++o) off[L.dims[o]] = L.sd[o].tab_off;
+    return 0;
+}
+
+// scratch of a right-hand-side pass: run partials of the large dims and per-block small-dim partials
+static int pass_scratch(Panel* p, const Layout& L, Scratch& sc) {
+    int e;
+    for (int i = 0; i < L.nd; ++i)
+        if (!L.sd[i].small) {
+            const int64_t R = p->d[L.dims[i]].n_runs;
+            if ((e = sc.alloc(&sc.runbuf[L.sd[i].large_slot], (size_t)(R * kMaxCT + 1)))) return e;
+        }
+    const unsigned grid = apply_grid(p);
+    if ((e = sc.alloc(&sc.small_part, (size_t)grid * (L.Gs > 0 ? L.Gs : 1) * kMaxCT))) return e;
+    if ((e = sc.alloc(&sc.dump, (size_t)wave_grid(p) * (kBlock / kWave) * kMaxCT + kMaxCT))) return e;
+    return 0;
+}
+
+static int width_for(int n) {
+    for (int w : kWidths)
+        if (w >= n) return w;
+    return 16;
+}
+
+template <int CT>
+static int rhs_cols(Panel* p, const Layout& L, Scratch& sc, const double* const* x, int nx, double* b, hipStream_t s) {
+    ApplyArgs a = base_args(p, L, sc, nullptr);
+    a.rc = nullptr;
+    for (int c = 0; c < kMaxCT; ++c) {
+        const bool in = c < nx;
+        a.cbit[c] = in ? ~0u : 0u;
+        a.xc[c] = in ? x[c] : nullptr;
+        a.oc[c] = nullptr;
+        a.oc32[c] = nullptr;
+    }
+    a.anybit = ~0u;
+    a.ubit = ~0u;
+    a.ncols = nx;
+    return apply_and_reduce<CT>(p, L, a, kData, b, s);
+}
+
+int within_rhs(Panel* p, unsigned fe_mask, const double* const* x, int nx, double* b, int* ld, hipStream_t s) {
+    unsigned mask = fe_mask ? fe_mask : ((1u << p->m) - 1u);
+    mask &= (1u << p->m) - 1u;
+    Layout L;
+    int e = make_layout(p, mask, L);
+    if (e) return e;
+    if (nx < 1 || nx > kMaxCT) return set_error("within_rhs: 1..%d columns (%d)", kMaxCT, nx);
+    const int ct = width_for(nx);
+    if (ct > tile_cap(L)) return set_error("within_rhs: small FE dims exceed the LDS budget at %d columns", ct);
+    *ld = ct;
+    if (L.S == 0) return 0;
+    Scratch sc;
+    if ((e = pass_scratch(p, L, sc))) return e;
+    switch (ct) {
+        case 1: e = rhs_cols<1>(p, L, sc, x, nx, b, s); break;
+        case 2: e = rhs_cols<2>(p, L, sc, x, nx, b, s); break;
+        case 3: e = rhs_cols<3>(p, L, sc, x, nx, b, s); break;
+        case 4: e = rhs_cols<4>(p, L, sc, x, nx, b, s); break;
+        case 6: e = rhs_cols<6>(p, L, sc, x, nx, b, s); break;
+        case 8: e = rhs_cols<8>(p, L, sc, x, nx, b, s); break;
+        case 9: e = rhs_cols<9>(p, L, sc, x, nx, b, s); break;
+        case 12: e = rhs_cols<12>(p, L, sc, x, nx, b, s); break;
+        default: e = rhs_cols<16>(p, L, sc, x, nx, b, s); break;
+    }
+    if (e) return e;
+    FEHIP_CHECK(hipStreamSynchronize(s), "shared right-hand sides");  // the scratch is released on return
     return 0;
 }
 

@@ -36,6 +36,15 @@ struct WithinStats {
     std::vector<int> col_refines;    // per column: FP64 true-residual products (FP32-operator solve)
 };
 
+// Right-hand sides formed once for the solves of several unweighted FE sets: b ([S_u][ld]) holds
+// D_o' M_P x (primary absorbed) or D_o' x over the union of those sets' dims (within_rhs); a solve takes
+// the blocks of its own dims.  dim_off: table offset of each panel dim in b's layout (-1: not in it).
+struct SharedRhs {
+    const double* b = nullptr;
+    int ld = 0;
+    int64_t dim_off[FEHIP_MAX_FE];
+};
+
 // Columns of one solve: input and output device pointers and the request slot of each column.
 // out32 (optional, per column) takes the result in FP32 instead of out; alpha (optional, per
 // column) receives the solved FE tables ([S] doubles, S = within_table_size).
@@ -52,7 +61,18 @@ struct WithinCols {
     // untouched (WithinStats::col_tile); with at most 16 columns the solve uses one tile.
     double* gram = nullptr;
     int64_t gram_ld = 0;
+    // optional: the right-hand sides come from a shared computation (column rhs_col[c] of rhs->b)
+    const SharedRhs* rhs = nullptr;
+    std::vector<int> rhs_col;
 };
+
+// Table offset of each panel dim in the layout of fe_mask (-1: not scattered there).
+int within_dim_offsets(Panel* p, unsigned fe_mask, int64_t* off);
+
+// The right-hand sides of nx <= 16 caller-order columns over fe_mask (unweighted), into b ([S][*ld],
+// *ld the column-tile width used, at most 16; b holds S * 16 doubles).
+int within_rhs(Panel* p, unsigned fe_mask, const double* const* x, int nx, double* b, int* ld, hipStream_t s);
+
 
 // Widest column tile of a solve over fe_mask (columns that one tile, hence one set of cross products,
 // can hold); 0 if the small dims do not fit the LDS budget.

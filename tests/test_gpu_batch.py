@@ -202,3 +202,53 @@ def test_server_rejects_bad_requests_and_drains_on_close():
     assert all(np.array_equal(f.beta, fits[0].beta) or rel(f.beta, fits[0].beta) < 1e-12 for f in fits)
     srv.close()
     pan.close()
+
+
+def test_chunks_sharing_right_hand_sides():
+    """Unweighted requests over every FE subset of a 3-dim panel: the chunks that absorb the primary dim
+    share one right-hand-side pass over the union of their dims and columns, and so do the chunks that do
+    not.  Each request, FP64 or FP32 storage, equals the same request solved alone (1e-10; FP32 storage
+    1e-6 / 1e-5) and the oracle on its FE dims (1e-8; FP32 storage 1e-6 / 1e-5); a weighted request in the
+    same batch keeps its own chunk."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    from fe_gpu_amd.device import DeviceBuffer
+    note("batch of chunks sharing right-hand sides")
+    p = synth.random_fe_problem(406, n=8000, K=5, G=(1500, 800, 9))
+    n = p.n
+    rng = np.random.default_rng(11)
+    pan = api.Panel.from_host(p.fe)
+    dy = DeviceBuffer.from_array(p.y)
+    dW = DeviceBuffer.from_array(np.asfortranarray(p.W))
+    w = (rng.random(n) < 0.5).astype(np.uint8)
+    dw = DeviceBuffer.from_array(w)
+    specs = []
+    for fm in range(1, 8):
+        dims = [d + 1 for d in range(3) if fm >> d & 1]
+        for prec in (0, 1):
+            k = int(rng.integers(1, 6))
+            j0 = int(rng.integers(0, 5 - k + 1))
+            ncl = int(rng.integers(0, len(dims) + 1))
+            cl = tuple(sorted(rng.choice(dims, size=ncl, replace=False).tolist())) if ncl else ()
+            specs.append((fm, j0, k, cl, prec, False))
+    specs.append((0b011, 0, 3, (1,), 0, True))
+    reqs = [api.Request(dy.ptr, dW.ptr + 8 * n * j0, k, 0, 0, n, dw.ptr if wt else 0,
+                        dict(cluster=cl, fe_mask=fm, tol=1e-13, max_iter=20000, precision=prec))
+            for (fm, j0, k, cl, prec, wt) in specs]
+    fits = api.estimate_batch(pan, reqs)
+    for i, ((fm, j0, k, cl, prec, wt), q, fb) in enumerate(zip(specs, reqs, fits)):
+        note(f"request {i}: fe_mask={fm:03b} K={k} cluster={cl} precision={prec} weighted={wt}")
+        fa = api.estimate_device(pan, q.y_ptr, q.W_ptr, q.K, 0, 0, n, q.weights_ptr, **q.options)
+        for key in INT_KEYS:
+            assert fb.res[key] == fa.res[key], (i, key)
+        bt, st = (1e-10, 1e-10) if prec == 0 else (1e-6, 1e-5)
+        assert rel(fb.beta, fa.beta) < bt and rel(fb.se, fa.se) < st, i
+        m = np.ones(n, bool) if not wt else w == 1
+        dims = [d for d in range(3) if fm >> d & 1]
+        fe_sub = np.vstack([p.fe[d] if not wt else _redense(p.fe[d][m]) for d in dims])
+        cl_sub = tuple(dims.index(c - 1) + 1 for c in cl)
+        ref = oracle.estimate(p.y[m], p.W[m][:, j0:j0 + k], fe_sub, cluster=cl_sub, tol=1e-13, max_iter=200000)
+        assert fb.res["dof_fes"] == ref.res["dof_fes"] and fb.res["dof_resid"] == ref.res["dof_resid"], i
+        bt, st = (1e-8, 1e-8) if prec == 0 else (1e-6, 1e-5)
+        assert rel(fb.beta, ref.beta) < bt and rel(fb.se, ref.se) < st, i
+    pan.close()
