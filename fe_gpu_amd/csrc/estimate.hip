@@ -375,6 +375,8 @@ struct WaveScores {
     const double* r;            // residual (panel order), or nullptr: formed from y and beta
     const void* y;              // y~ column (the columns' type) when r is null
     const double* beta;         // [K] when r is null
+    double betah[12];           // the same coefficients by value: kernel arguments, read once per wave (loads
+                                // through beta are repeated every tile, as the pass's stores may alias them)
     int K;
     const int* idP;
     const int64_t* wtile_rec;   // first row | rows << 32 of each wave tile
@@ -443,23 +445,17 @@ __device__ __forceinline__ void load_score_rows(const WaveScores& a, const TX* c
     for (int k = 0; k < KT; ++k) q.x[k] = xk[k][ri];
 }
 
-// Segmented sums of the score values over [run start rs, lane]: with FP64 columns an FP64 scan; with FP32
-// columns (the mixed-precision form, whose scores carry the storage rounding anyway) an FP32 scan of one
-// fused DPP multiply-add per column and step, widened back to FP64 for the cross products.
-template <int KT, typename TX>
-__device__ __forceinline__ void scan_scores(double (&sv)[KT], const double (&tv)[KT], int lane, int rs) {
-    if constexpr (std::is_same<TX, float>::value) {
-        float f[KT];
+// Segmented sums of the score values over [run start rs, lane], in the scores' type: with FP64 columns an
+// FP64 scan; with FP32 columns (the mixed-precision form, whose scores carry the storage rounding anyway) an
+// FP32 scan of one fused DPP multiply-add per column and step.
+template <int KT, typename TV>
+__device__ __forceinline__ void scan_scores(TV (&sv)[KT], const TV (&tv)[KT], int lane, int rs) {
 #pragma unroll
-        for (int k = 0; k < KT; ++k) f[k] = (float)tv[k];
-        seg_scan_cols_f32<KT>(f, ScanFactorsF(ScanMask(lane, rs)));
-#pragma unroll
-        for (int k = 0; k < KT; ++k) sv[k] = (double)f[k];
-    } else {
-#pragma unroll
-        for (int k = 0; k < KT; ++k) sv[k] = tv[k];
+    for (int k = 0; k < KT; ++k) sv[k] = tv[k];
+    if constexpr (std::is_same<TV, float>::value)
+        seg_scan_cols_f32<KT>(sv, ScanFactorsF(ScanMask(lane, rs)));
+    else
         seg_scan_cols<KT>(sv, ScanFactors(ScanMask(lane, rs)));
-    }
 }
 
 // Per subset s: part[(s * gridDim.x + block) * NP + e] (upper triangle of the block's sum of S S') and
@@ -487,6 +483,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
 #pragma unroll
     for (int k = 0; k < KT; ++k) xk[k] = X[k < a.K ? k : 0];
     const TX* yc = a.r ? xk[0] : static_cast<const TX*>(a.y);
+    using TV = TX;  // type of the scores and their segmented sums
     int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
     ScoreRows<KT, TX> cur, nxt;
     load_score_rows<KT, TX>(a, xk, yc, load_score_rec(a, t), lane, cur);
@@ -503,6 +500,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
         for (int j = 0; j < kMaxRunOuts; ++j) rdl[j] = cur.rd[j];
         const bool wrow = valid && (!a.w || cur.w);
         const int64_t spr = cur.sp;
+        // the residual in FP64 (it is the small difference of y~ and W~ beta); the scores in the columns'
+        // type (FP32 storage: one rounding of r x, as the storage rounded x; the sums over clusters and the
+        // cross products stay FP64)
         double rr;
         if (a.r) {
             rr = cur.r;
@@ -513,13 +513,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
                 rr = (double)cur.y;
 #pragma unroll
             for (int k = 0; k < KT; ++k)
-                if (k < a.K) rr = fma(-a.beta[k], (double)cur.x[k], rr);
+                if (k < a.K) rr = fma(-a.betah[k], (double)cur.x[k], rr);
         }
         rr = valid ? rr : 0.0;
         rss = fma(rr, rr, rss);
-        double tv[KT];
+        const TV rv = (TV)rr;
+        TV tv[KT];
 #pragma unroll
-        for (int k = 0; k < KT; ++k) tv[k] = k < a.K ? rr * (double)cur.x[k] : 0.0;
+        for (int k = 0; k < KT; ++k) tv[k] = k < a.K ? rv * cur.x[k] : (TV)0;
         // the next tiles' record and rows
         const int64_t rec_nn = load_score_rec(a, t + 2 * wstride);
         load_score_rows<KT, TX>(a, xk, yc, rec_n, lane, nxt);
@@ -535,7 +536,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
                 constexpr int P = KT / 2, RPI = kWave / P;  // pieces per row, rows per instruction
                 double2* st = reinterpret_cast<double2*>(mystage);
 #pragma unroll
-                for (int k = 0; k < P; ++k) st[lane * P + k] = double2{tv[2 * k], tv[2 * k + 1]};
+                for (int k = 0; k < P; ++k) st[lane * P + k] = double2{(double)tv[2 * k], (double)tv[2 * k + 1]};
                 __builtin_amdgcn_wave_barrier();
 #pragma unroll
                 for (int i = 0; i < kWave / RPI; ++i) {
@@ -553,11 +554,11 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
             } else if (a.K == KT) {  // KT is even: 16-byte stores
 #pragma unroll
                 for (int k = 0; k < KT / 2; ++k)
-                    reinterpret_cast<double2*>(dst)[k] = double2{tv[2 * k], tv[2 * k + 1]};
+                    reinterpret_cast<double2*>(dst)[k] = double2{(double)tv[2 * k], (double)tv[2 * k + 1]};
             } else {
 #pragma unroll
                 for (int k = 0; k < KT; ++k)
-                    if (k < a.K) dst[k] = tv[k];
+                    if (k < a.K) dst[k] = (double)tv[k];
             }
             if (a.tkey) {
                 int cell = 0;
@@ -583,16 +584,16 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
                 double* row = stab + a.soff[q] + cell * (KT + 1);
 #pragma unroll
                 for (int k = 0; k < KT; ++k)
-                    if (k < a.K) lds_add(&row[k], tv[k]);
+                    if (k < a.K) lds_add(&row[k], (double)tv[k]);
                 if (wrow) lds_add(&row[KT], 1.0);
             }
         }
         // run sums of a single large dim: the runs of equal id inside the segment
-        auto store_runs = [&](int j, const double (&sv)[KT], bool rend) {
+        auto store_runs = [&](int j, const TV (&sv)[KT], bool rend) {
             if (rend) {  // KT is even and the rows are KT doubles: 16-byte stores
                 double2* dst = reinterpret_cast<double2*>(a.runbuf[j] + (int64_t)rdl[j] * KT);
 #pragma unroll
-                for (int k = 0; k < KT / 2; ++k) dst[k] = double2{sv[2 * k], sv[2 * k + 1]};
+                for (int k = 0; k < KT / 2; ++k) dst[k] = double2{(double)sv[2 * k], (double)sv[2 * k + 1]};
             }
         };
         for (int j = 0; j < a.nrun; ++j) {
@@ -604,8 +605,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
             const unsigned long long HR = __ballot(rh);
             const int rs = 63 - __clzll(HR & mask_le(lane));
             const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
-            double sv[KT];
-            scan_scores<KT, TX>(sv, tv, lane, rs);
+            TV sv[KT];
+            scan_scores<KT, TV>(sv, tv, lane, rs);
             store_runs(j, sv, rend);
         }
 #pragma unroll
@@ -624,9 +625,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
             const int rs = 63 - __clzll(le);
             const bool rend = valid && (lane == nr - 1 || (lane < kWave - 1 && ((HR >> (lane + 1)) & 1ull)));
             bool ne = (BW & mask_le(lane) & ~((1ull << rs) - 1ull)) != 0;  // run holds a weighted row
-            double sv[KT];
+            TV sv[KT];
             if (mode != 2) {
-                scan_scores<KT, TX>(sv, tv, lane, rs);
+                scan_scores<KT, TV>(sv, tv, lane, rs);
             } else {
 #pragma unroll
                 for (int k = 0; k < KT; ++k) sv[k] = tv[k];
@@ -649,7 +650,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
                     if (__ballot(take)) {
 #pragma unroll
                         for (int k = 0; k < KT; ++k) {
-                            const double vj = __shfl(sv[k], j, kWave);
+                            const TV vj = __shfl(sv[k], j, kWave);
                             if (take) sv[k] += vj;
                         }
                         const bool nej = __shfl((int)ne, j, kWave) != 0;
@@ -662,7 +663,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
             if (lane == 0) nclusters[q] += (unsigned long long)__popcll(BU);
             // sum of S S' over the clusters that end here: the cross products of the masked sums
 #pragma unroll
-            for (int k = 0; k < KT; ++k) sv[k] = on ? sv[k] : 0.0;
+            for (int k = 0; k < KT; ++k) sv[k] = on ? sv[k] : (TV)0;
             gram_wave<KT>(mystage, sv, lane, acc[q]);
         }
     }
@@ -2258,6 +2259,7 @@ int finish_request(Panel* p, ReqState& s) {
             ws.r = resid_in_pass ? nullptr : r;
             ws.y = resid_in_pass ? ycol : nullptr;
             ws.beta = dbeta;
+            for (int k = 0; k < 12; ++k) ws.betah[k] = k < K ? beta[k] : 0.0;
             ws.K = K;
             ws.idP = p->d[p->P].ids;
             ws.wtile_rec = p->wtile_rec;

@@ -44,11 +44,11 @@ __device__ __forceinline__ void gram_rows64(const double* st, int lane, gram4_t&
     }
 }
 
-// one wave: stage this lane's row (CT values) and add the wave's V'V to acc
-template <int CT>
-__device__ __forceinline__ void gram_wave(double* st, const double (&v)[CT], int lane, gram4_t& acc) {
+// one wave: stage this lane's row (CT values, widened to FP64) and add the wave's V'V to acc
+template <int CT, typename T>
+__device__ __forceinline__ void gram_wave(double* st, const T (&v)[CT], int lane, gram4_t& acc) {
 #pragma unroll
-    for (int c = 0; c < CT; ++c) st[lane * CT + c] = v[c];
+    for (int c = 0; c < CT; ++c) st[lane * CT + c] = (double)v[c];
     __builtin_amdgcn_wave_barrier();
     gram_rows64<CT>(st, lane, acc);
     __builtin_amdgcn_wave_barrier();
