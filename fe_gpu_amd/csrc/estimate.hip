@@ -2963,9 +2963,30 @@ int finish_chunk(Panel* p, std::vector<ReqState>& rs, ChunkRun& ch) {
     return 0;
 }
 
+// Two requests of a batch that ask for the same regression (same columns, row subsample and options) get
+// the same answer: the later one is served from the earlier one's computation.
+bool same_request(const ReqState& a, const ReqState& b) {
+    if (a.d_y != b.d_y || a.d_W != b.d_W || a.d_Z != b.d_Z || a.K != b.K || a.L != b.L || a.ld != b.ld ||
+        a.d_weights != b.d_weights || a.fe_mask != b.fe_mask)
+        return false;
+    const fehip_opts &x = *a.opts, &y = *b.opts;
+    const fehip_demean_opts &dx = x.demean, &dy = y.demean;
+    if (dx.tol != dy.tol || dx.max_iter != dy.max_iter || dx.method != dy.method || dx.precision != dy.precision ||
+        dx.check_every != dy.check_every || x.n_cluster != y.n_cluster || x.n_iv_reg != y.n_iv_reg ||
+        x.n_iv_inst != y.n_iv_inst)
+        return false;
+    for (int i = 0; i < x.n_cluster && i < 4; ++i)
+        if (x.cluster_dims[i] != y.cluster_dims[i]) return false;
+    for (int i = 0; i < x.n_iv_reg; ++i)
+        if (x.iv_reg[i] != y.iv_reg[i]) return false;
+    for (int i = 0; i < x.n_iv_inst; ++i)
+        if (x.iv_inst[i] != y.iv_inst[i]) return false;
+    return true;
+}
+
 // Solve a batch: requests that absorb the same FE dims share the within-transform sweeps (chunks), chunks
 // of the same primary treatment share their right-hand-side pass, and every request then takes its own
-// normal-equation and sandwich step.
+// normal-equation and sandwich step (a request equal to an earlier one takes that one's results).
 int run_batch(Panel* p, std::vector<ReqState>& rs) {
     int e;
     const int64_t n = p->n;
@@ -2979,11 +3000,16 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
     FEHIP_CHECK(hipMemGetInfo(&free_b, &total_b), "memory info");
     const double budget = 0.7 * (double)(free_b + cache_idle_bytes());
     BatchShared shared;
+    std::vector<int> copy_of(rs.size(), -1);
+    for (size_t i = 1; i < rs.size(); ++i)
+        for (size_t j = 0; j < i && copy_of[i] < 0; ++j)
+            if (copy_of[j] < 0 && (!rs[i].vcov || rs[j].vcov) && same_request(rs[i], rs[j])) copy_of[i] = (int)j;
     // chunks: unsolved requests with the same FE set, in submission order, within the slot count and the
     // memory budget (the first request always goes)
     std::vector<std::unique_ptr<ChunkRun>> runs;
     {
         std::vector<int> taken(rs.size(), 0);
+        for (size_t i = 0; i < rs.size(); ++i) taken[i] = copy_of[i] >= 0;
         for (size_t i0 = 0; i0 < rs.size(); ++i0) {
             if (taken[i0]) continue;
             auto ch = std::make_unique<ChunkRun>();
@@ -3058,6 +3084,16 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
         ChunkRun& ch = *chp;
         if ((e = solve_chunk(p, rs, ch, ch.group >= 0 ? groups[ch.group].get() : nullptr, shared))) return e;
         if ((e = finish_chunk(p, rs, ch))) return e;
+    }
+    for (size_t i = 0; i < rs.size(); ++i) {
+        if (copy_of[i] < 0) continue;
+        const ReqState& src = rs[copy_of[i]];
+        ReqState& dst = rs[i];
+        *dst.result = *src.result;
+        const int K = std::max(0, dst.K);
+        std::copy(src.beta, src.beta + K, dst.beta);
+        std::copy(src.se, src.se + K, dst.se);
+        if (dst.vcov) std::copy(src.vcov, src.vcov + (size_t)K * K, dst.vcov);
     }
     return 0;
 }

@@ -209,7 +209,7 @@ def test_chunks_sharing_right_hand_sides():
     share one right-hand-side pass over the union of their dims and columns, and so do the chunks that do
     not.  Each request, FP64 or FP32 storage, equals the same request solved alone (1e-10; FP32 storage
     1e-6 / 1e-5) and the oracle on its FE dims (1e-8; FP32 storage 1e-6 / 1e-5); a weighted request in the
-    same batch keeps its own chunk."""
+    same batch keeps its own chunk; two requests repeated at the end get their originals' results."""
     import oracle
     from fe_gpu_amd import api, synth
     from fe_gpu_amd.device import DeviceBuffer
@@ -232,6 +232,7 @@ def test_chunks_sharing_right_hand_sides():
             cl = tuple(sorted(rng.choice(dims, size=ncl, replace=False).tolist())) if ncl else ()
             specs.append((fm, j0, k, cl, prec, False))
     specs.append((0b011, 0, 3, (1,), 0, True))
+    specs += [specs[3], specs[10]]  # requests equal to earlier ones are served from their computation
     reqs = [api.Request(dy.ptr, dW.ptr + 8 * n * j0, k, 0, 0, n, dw.ptr if wt else 0,
                         dict(cluster=cl, fe_mask=fm, tol=1e-13, max_iter=20000, precision=prec))
             for (fm, j0, k, cl, prec, wt) in specs]
@@ -251,4 +252,7 @@ def test_chunks_sharing_right_hand_sides():
         assert fb.res["dof_fes"] == ref.res["dof_fes"] and fb.res["dof_resid"] == ref.res["dof_resid"], i
         bt, st = (1e-8, 1e-8) if prec == 0 else (1e-6, 1e-5)
         assert rel(fb.beta, ref.beta) < bt and rel(fb.se, ref.se) < st, i
+    for dup, orig in ((len(specs) - 2, 3), (len(specs) - 1, 10)):
+        assert np.array_equal(fits[dup].beta, fits[orig].beta) and np.array_equal(fits[dup].se, fits[orig].se)
+        assert np.array_equal(fits[dup].vcov, fits[orig].vcov)
     pan.close()
