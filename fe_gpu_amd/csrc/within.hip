@@ -93,7 +93,7 @@ struct ApplyArgs {
     int lean;                  // 1: wave tiles through wave_op_kernel
     uint32_t ubit;             // the tile's slot bit (~0u without row masks)
     int op32;                  // 1: FP32 direction tables (p32) and run partials (runbuf32)
-    const float* p32;          // kIter with op32: [S][CT rounded up to 4] FP32 copy of p
+    const float* p32;          // kIter with op32: [S][OpStore<float>::pad(CT)] FP32 copy of p
     double* dump;              // lean kernel: per-wave slots [waves][kMaxCT] that lanes ending no run write
     const RunCoupling* rc;     // kIter: the run-level operator's couplings (host side; nullptr: row kernels)
     const double* dg;          // unweighted solves: Schur diagonal and group counts [S][2]
@@ -507,6 +507,8 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
 //             corrected by FP64 products of the true residual in the solve around it.
 template <typename TS>
 struct OpStore {
+    // FP32 rows padded to a multiple of 16 bytes (padding them to whole powers of two, so no row straddles
+    // a cache line, measured slower: the larger table)
     static constexpr int pad(int ct) { return sizeof(TS) == 4 ? (ct + 3) & ~3 : ct; }
 };
 
@@ -1410,7 +1412,7 @@ struct CgState {
 // FP32 copy of the direction p for the FP32 operator (rows padded to 16 bytes)
 template <int CT>
 __device__ __forceinline__ void store_p32(float* p32, int64_t g, int c, double v) {
-    if (p32) p32[g * ((CT + 3) & ~3) + c] = (float)v;
+    if (p32) p32[g * OpStore<float>::pad(CT) + c] = (float)v;
 }
 
 template <int CT>
