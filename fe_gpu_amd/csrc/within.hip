@@ -1065,7 +1065,7 @@ __global__ void gram_final_kernel(const double* __restrict__ part, int nblk, int
 // ------------------------------------------------------------------------------------------------
 // q[g][c] = sum of the run partials of group g (runs in group order)
 template <int CT>
-__global__ void reduce_runs_kernel(const double* __restrict__ runbuf, const int64_t* __restrict__ goff, int G,
+__global__ void reduce_runs_kernel(const double* __restrict__ runbuf, int sp, const int64_t* __restrict__ goff, int G,
                                    double* q, const int* __restrict__ active) {
     if (!any_active<CT>(active)) return;
     const int64_t total = (int64_t)G * CT;
@@ -1077,19 +1077,19 @@ __global__ void reduce_runs_kernel(const double* __restrict__ runbuf, const int6
         const int64_t k0 = goff[g], k1 = goff[g + 1];
         int64_t k = k0;
         for (; k + 4 <= k1; k += 4) {
-            s0 += runbuf[k * CT + c];
-            s1 += runbuf[(k + 1) * CT + c];
-            s2 += runbuf[(k + 2) * CT + c];
-            s3 += runbuf[(k + 3) * CT + c];
+            s0 += runbuf[k * sp + c];
+            s1 += runbuf[(k + 1) * sp + c];
+            s2 += runbuf[(k + 2) * sp + c];
+            s3 += runbuf[(k + 3) * sp + c];
         }
-        for (; k < k1; ++k) s0 += runbuf[k * CT + c];
+        for (; k < k1; ++k) s0 += runbuf[k * sp + c];
         q[e] = (s0 + s1) + (s2 + s3);
     }
 }
 
 // q[g][c] = FP64 sum of the FP32 run partials of group g (runs in group order)
 template <int CT>
-__global__ void reduce_runs32_kernel(const float* __restrict__ runbuf, const int64_t* __restrict__ goff, int G,
+__global__ void reduce_runs32_kernel(const float* __restrict__ runbuf, int sp, const int64_t* __restrict__ goff, int G,
                                      double* q, const int* __restrict__ active) {
     if (!any_active<CT>(active)) return;
     const int64_t total = (int64_t)G * CT;
@@ -1100,12 +1100,12 @@ __global__ void reduce_runs32_kernel(const float* __restrict__ runbuf, const int
         const int64_t k0 = goff[g], k1 = goff[g + 1];
         int64_t k = k0;
         for (; k + 4 <= k1; k += 4) {
-            s0 += (double)runbuf[k * CT + c];
-            s1 += (double)runbuf[(k + 1) * CT + c];
-            s2 += (double)runbuf[(k + 2) * CT + c];
-            s3 += (double)runbuf[(k + 3) * CT + c];
+            s0 += (double)runbuf[k * sp + c];
+            s1 += (double)runbuf[(k + 1) * sp + c];
+            s2 += (double)runbuf[(k + 2) * sp + c];
+            s3 += (double)runbuf[(k + 3) * sp + c];
         }
-        for (; k < k1; ++k) s0 += (double)runbuf[k * CT + c];
+        for (; k < k1; ++k) s0 += (double)runbuf[k * sp + c];
         q[e] = (s0 + s1) + (s2 + s3);
     }
 }
@@ -1137,6 +1137,24 @@ __global__ __launch_bounds__(kBlock) void reduce_small_kernel(const double* __re
 // couplings B = D_D' M_P D_s, C = D_s' M_P D_s built once (RunCoupling).  A needs no row: in a primary
 // segment of n rows whose runs of D (group g_r, length l_r) give m = sum_r l_r p_D[g_r] / n, run r
 // contributes l_r (p_D[g_r] - m) to group g_r.
+
+// A run partial of the run-level operator.  FP64: CT values padded to a 16-byte multiple and written with
+// 16-byte stores (the partials land at scattered places, so every store instruction touches a line per
+// lane: fewer, wider stores touch fewer; c2: operator 1.36 -> 1.30 ms).  FP32: CT values unpadded (the
+// padded 16-byte form measured slower there, 0.97 -> 1.05 ms, its larger rows outweighing the wider stores).
+template <int CT, typename TS>
+struct RunRow {
+    static constexpr int kStride = sizeof(TS) == 4 ? CT : (CT + 1) & ~1;
+    __device__ __forceinline__ static void store(TS* dst, const TS (&w)[kStride]) {
+        if constexpr (sizeof(TS) == 4) {
+#pragma unroll
+            for (int c = 0; c < kStride; ++c) dst[c] = w[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < kStride; c += 2) *reinterpret_cast<double2*>(dst + c) = double2{w[c], w[c + 1]};
+        }
+    }
+};
 
 // One wave per run tile (whole primary segments, at most 64 runs): a lane per run gathers its group's
 // row of p_D, a segmented scan of l p_D and of l over the tile's segments gives each segment's mean,
@@ -1186,9 +1204,10 @@ __global__ __launch_bounds__(kBlock) void run_wave_kernel(const TS* __restrict__
 #pragma unroll
         for (int c = 0; c < CT; ++c) mean[c] = lane_value(x[c], se_addr);
         if (valid) {
-            TS* dst = out + (int64_t)dest * CT;
+            TS w[RunRow<CT, TS>::kStride];
 #pragma unroll
-            for (int c = 0; c < CT; ++c) dst[c] = l * (v[c] - mean[c] * inv);
+            for (int c = 0; c < RunRow<CT, TS>::kStride; ++c) w[c] = c < CT ? l * (v[c] - mean[c] * inv) : (TS)0;
+            RunRow<CT, TS>::store(out + (int64_t)dest * RunRow<CT, TS>::kStride, w);
         }
     }
 }
@@ -1224,9 +1243,10 @@ __global__ __launch_bounds__(kBlock) void run_long_kernel(const TS* __restrict__
         for (int64_t r = r0; r < r1; ++r) {
             const TS* row = tab + (tab_off + run_gid[r]) * ctp;
             const double l = (double)(run_len[r] & 0x7fffffff);
-            TS* dst = out + (int64_t)run_dest[r] * CT;
+            TS w[RunRow<CT, TS>::kStride];
 #pragma unroll
-            for (int c = 0; c < CT; ++c) dst[c] = (TS)(l * ((double)row[c] - m[c]));
+            for (int c = 0; c < RunRow<CT, TS>::kStride; ++c) w[c] = c < CT ? (TS)(l * ((double)row[c] - m[c])) : (TS)0;
+            RunRow<CT, TS>::store(out + (int64_t)run_dest[r] * RunRow<CT, TS>::kStride, w);
         }
     }
 }
@@ -1841,7 +1861,7 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s);
 // the run partials (only_dim >= 0 restricts to one scatter dim)
 template <int CT>
 void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, double* q, unsigned grid,
-                   const int* active, hipStream_t s, bool time_runs, double on = 0.0) {
+                   const int* active, hipStream_t s, bool time_runs, double on = 0.0, int run_stride = CT) {
     for (int o = 0; o < L.nd; ++o) {
         if (only_dim >= 0 && o != only_dim) continue;
         const ScatterDim& d = L.sd[o];
@@ -1854,12 +1874,12 @@ void reduce_tables(Panel* p, const Layout& L, const ApplyArgs& a, int only_dim, 
             if (time_runs) kstat_begin(kStatReduceRuns, s);
             if (a.op32)
                 hipLaunchKernelGGL((reduce_runs32_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock, 32)),
-                                   dim3(kBlock), 0, s, (const float*)a.sd[o].runbuf32, (const int64_t*)D.grp_run_off,
-                                   d.G, qo, active);
+                                   dim3(kBlock), 0, s, (const float*)a.sd[o].runbuf32, run_stride,
+                                   (const int64_t*)D.grp_run_off, d.G, qo, active);
             else
                 hipLaunchKernelGGL((reduce_runs_kernel<CT>), dim3(stream_grid((size_t)d.G * CT, kBlock, 32)),
-                                   dim3(kBlock), 0, s, (const double*)a.sd[o].runbuf, (const int64_t*)D.grp_run_off,
-                                   d.G, qo, active);
+                                   dim3(kBlock), 0, s, (const double*)a.sd[o].runbuf, run_stride,
+                                   (const int64_t*)D.grp_run_off, d.G, qo, active);
             if (time_runs)
                 kstat_end(kStatReduceRuns, s,
                           on > 0.0 ? (a.op32 ? 4.0 : 8.0) * on * (double)D.n_runs + 8.0 * on * (double)d.G +
@@ -1996,7 +2016,7 @@ int run_op_apply(Panel* p, const Layout& L, const ApplyArgs& a, double* q, hipSt
         const double R = (double)D.n_runs;
         kstat_end(sid, s, on > 0.0 ? 12.0 * R + es * on * ((double)d.G + R) : 0.0);
     }
-    reduce_tables<CT>(p, L, a, o, q, 0, a.active, s, timed, on);
+    reduce_tables<CT>(p, L, a, o, q, 0, a.active, s, timed, on, f32 ? RunRow<CT, float>::kStride : RunRow<CT, double>::kStride);
     if (rc.Gs > 0) {
         const int G = d.G;
         // partial blocks within the per-block partial buffer ([apply_grid][Gs][kMaxCT])
