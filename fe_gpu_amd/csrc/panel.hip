@@ -266,7 +266,10 @@ void free_panel(Panel* p) {
         dfree(p->d[d].run_gid);
         dfree(p->d[d].run_len);
         dfree(p->d[d].rtile_rec);
+        dfree(p->d[d].rtile_seg);
         dfree(p->d[d].rlong_segs);
+        dfree(p->d[d].grp_seg);
+        dfree(p->d[d].grp_len);
     }
     delete p;
 }

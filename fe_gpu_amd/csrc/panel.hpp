@@ -37,8 +37,12 @@ struct DimStruct {
     // (first run | runs << 32), and the segments of more runs, listed apart
     int64_t n_rtiles = -1;
     int64_t* rtile_rec = nullptr;
+    int64_t* rtile_seg = nullptr;    // first primary segment of each run tile
     int64_t n_rlong = 0;
     int64_t* rlong_segs = nullptr;
+    // the runs in group order (run_order): each one's primary segment and length (built with the run tiles)
+    int* grp_seg = nullptr;
+    int* grp_len = nullptr;
 };
 
 // The couplings of one large non-primary dim D with the small absorbed dims (all of them, Gs groups)

@@ -1110,32 +1110,6 @@ __global__ void reduce_runs32_kernel(const float* __restrict__ runbuf, int sp, c
     }
 }
 
-// q[g][c] = sum of the run partials of group g, the partials in run (segment) order and gathered through
-// run_order (the runs of each group in group order): FP64 sums whatever the partials' type
-template <int CT, typename TS>
-__global__ void reduce_runs_gather_kernel(const TS* __restrict__ runbuf, int sp, const int* __restrict__ run_order,
-                                          const int64_t* __restrict__ goff, int G, double* q,
-                                          const int* __restrict__ active) {
-    if (!any_active<CT>(active)) return;
-    const int64_t total = (int64_t)G * CT;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
-        const int g = (int)(e / CT), c = (int)(e % CT);
-        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        const int64_t k0 = goff[g], k1 = goff[g + 1];
-        int64_t k = k0;
-        for (; k + 4 <= k1; k += 4) {
-            const int64_t r0 = run_order[k], r1 = run_order[k + 1], r2 = run_order[k + 2], r3 = run_order[k + 3];
-            s0 += (double)runbuf[r0 * sp + c];
-            s1 += (double)runbuf[r1 * sp + c];
-            s2 += (double)runbuf[r2 * sp + c];
-            s3 += (double)runbuf[r3 * sp + c];
-        }
-        for (; k < k1; ++k) s0 += (double)runbuf[(int64_t)run_order[k] * sp + c];
-        q[e] = (s0 + s1) + (s2 + s3);
-    }
-}
-
 // one workgroup per table entry: the threads stride over the per-block partials in a fixed order and
 // combine them with a fixed-shape tree
 template <int CT>
@@ -1183,24 +1157,28 @@ struct RunRow {
 };
 
 // One wave per run tile (whole primary segments, at most 64 runs): a lane per run gathers its group's
-// row of p_D, a segmented scan of l p_D and of l over the tile's segments gives each segment's mean,
-// and every lane writes its run's partial l (p_D[g] - mean) at the run's own place (run order, so a wave's
-// partials are one contiguous piece; reduce_runs_gather_kernel gathers them per group).  Written at the
-// group-order place instead, each lane's store lands on its own line: c2 operator 1.30 ms + 0.36 ms of
-// per-group sums, against 0.76 + 0.68 ms.  Table rows have stride ctp; bit 31 of run_len marks a
-// segment's first run.
+// row of p_D, and a segmented scan of l p_D and of l over the tile's segments gives each segment's mean m,
+// which the segment's last lane writes at the segment's place ([G_P] rows, one contiguous piece per wave).
+// The group sums A p_D = sum_r l_r (p_D[g_r] - m_seg(r)) are then formed per group from the group-order list
+// of runs (reduce_segmeans_kernel), gathering m by segment: 5M segment rows at c2 instead of 19.5M run
+// partials (c2 operator product 0.76 + 0.68 ms with the run partials written in run order and gathered per
+// group; 1.30 + 0.36 ms with them written at their group-order places).  Table rows have stride ctp; bit 31
+// of run_len marks a segment's first run.
 template <int CT, typename TS>
 __global__ __launch_bounds__(kBlock) void run_wave_kernel(const TS* __restrict__ tab, int ctp, int64_t tab_off,
-                                                          const int64_t* __restrict__ rtile_rec, int64_t n_rtiles,
+                                                          const int64_t* __restrict__ rtile_rec,
+                                                          const int64_t* __restrict__ rtile_seg, int64_t n_rtiles,
                                                           const int* __restrict__ run_gid,
-                                                          const int* __restrict__ run_len, TS* __restrict__ out,
+                                                          const int* __restrict__ run_len, TS* __restrict__ mseg,
                                                           const int* __restrict__ active) {
     if (!any_active<CT>(active)) return;
     constexpr bool kF32 = sizeof(TS) == 4;
+    constexpr int SP = RunRow<CT, TS>::kStride;
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
     for (int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; t < n_rtiles; t += wstride) {
         const int64_t rec = rtile_rec[t];
+        const int64_t seg0 = rtile_seg[t];
         const int64_t r0 = (int64_t)(uint32_t)rec;
         const int nr = (int)(rec >> 32);
         const bool valid = lane < nr;
@@ -1210,45 +1188,40 @@ __global__ __launch_bounds__(kBlock) void run_wave_kernel(const TS* __restrict__
         const bool head = valid && (lane == 0 || lw < 0);
         const TS l = valid ? (TS)(lw & 0x7fffffff) : (TS)0;
         const TS* row = tab + (tab_off + g) * ctp;
-        TS v[CT], x[CT + 1];
+        TS x[CT + 1];
 #pragma unroll
-        for (int c = 0; c < CT; ++c) v[c] = ld_global(row + c);
-#pragma unroll
-        for (int c = 0; c < CT; ++c) x[c] = l * v[c];
+        for (int c = 0; c < CT; ++c) x[c] = l * ld_global(row + c);
         x[CT] = l;
         const unsigned long long H = __ballot(head);
-        const int ss = 63 - __clzll(H & mask_le(lane));
         const unsigned long long nh = H & ~mask_le(lane);
+        const int ss = 63 - __clzll(H & mask_le(lane));
         const int se = nh ? __ffsll((long long)nh) - 2 : nr - 1;
         const ScanMask smask(lane, ss);
         if constexpr (kF32)
             seg_scan_cols_f32<CT + 1>(x, ScanFactorsF(smask));
         else
             seg_scan_cols<CT + 1>(x, ScanFactors(smask));
-        const int se_addr = (se < 0 ? 0 : se) << 2;
-        const TS inv = (TS)1 / lane_value(x[CT], se_addr);
-        TS mean[CT];
+        if (valid && lane == se) {  // the segment's last run holds its totals
+            const TS inv = (TS)1 / x[CT];
+            TS w[SP];
 #pragma unroll
-        for (int c = 0; c < CT; ++c) mean[c] = lane_value(x[c], se_addr);
-        if (valid) {
-            TS w[RunRow<CT, TS>::kStride];
-#pragma unroll
-            for (int c = 0; c < RunRow<CT, TS>::kStride; ++c) w[c] = c < CT ? l * (v[c] - mean[c] * inv) : (TS)0;
-            RunRow<CT, TS>::store(out + r * RunRow<CT, TS>::kStride, w);
+            for (int c = 0; c < SP; ++c) w[c] = c < CT ? x[c] * inv : (TS)0;
+            const int64_t seg = seg0 + __popcll(H & mask_le(lane)) - 1;
+            RunRow<CT, TS>::store(mseg + seg * SP, w);
         }
     }
 }
 
-// Segments of more than 64 runs: one thread per listed segment, its mean over the runs, then each
-// run's partial.
+// Segments of more than 64 runs: one thread per listed segment writes its mean.
 template <int CT, typename TS>
 __global__ __launch_bounds__(kBlock) void run_long_kernel(const TS* __restrict__ tab, int ctp, int64_t tab_off,
                                                           const int64_t* __restrict__ segs, int64_t nseg,
                                                           const int64_t* __restrict__ seg_run_off,
                                                           const int* __restrict__ run_gid,
-                                                          const int* __restrict__ run_len, TS* __restrict__ out,
+                                                          const int* __restrict__ run_len, TS* __restrict__ mseg,
                                                           const int* __restrict__ active) {
     if (!any_active<CT>(active)) return;
+    constexpr int SP = RunRow<CT, TS>::kStride;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nseg; i += stride) {
         const int64_t w = segs[i];
@@ -1264,16 +1237,57 @@ __global__ __launch_bounds__(kBlock) void run_long_kernel(const TS* __restrict__
             for (int c = 0; c < CT; ++c) m[c] += l * (double)row[c];
         }
         const double inv = 1.0 / nw;
+        TS v[SP];
 #pragma unroll
-        for (int c = 0; c < CT; ++c) m[c] *= inv;
-        for (int64_t r = r0; r < r1; ++r) {
-            const TS* row = tab + (tab_off + run_gid[r]) * ctp;
-            const double l = (double)(run_len[r] & 0x7fffffff);
-            TS w[RunRow<CT, TS>::kStride];
-#pragma unroll
-            for (int c = 0; c < RunRow<CT, TS>::kStride; ++c) w[c] = c < CT ? (TS)(l * ((double)row[c] - m[c])) : (TS)0;
-            RunRow<CT, TS>::store(out + r * RunRow<CT, TS>::kStride, w);
+        for (int c = 0; c < SP; ++c) v[c] = c < CT ? (TS)(m[c] * inv) : (TS)0;
+        RunRow<CT, TS>::store(mseg + w * SP, v);
+    }
+}
+
+// q[g][c] = sum over the runs r of group g (group order) of l_r (p_D[g][c] - m[seg(r)][c]): the run-level
+// operator's A p_D from the segment means, gathered by segment; FP64 arithmetic whatever the tables' type
+template <int CT, typename TS>
+__global__ void reduce_segmeans_kernel(const TS* __restrict__ tab, int ctp, int64_t tab_off,
+                                       const TS* __restrict__ mseg, const int* __restrict__ grp_seg,
+                                       const int* __restrict__ grp_len, const int64_t* __restrict__ goff, int G,
+                                       double* q, const int* __restrict__ active) {
+    if (!any_active<CT>(active)) return;
+    constexpr int SP = RunRow<CT, TS>::kStride;
+    const int64_t total = (int64_t)G * CT;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += stride) {
+        const int g = (int)(e / CT), c = (int)(e % CT);
+        const double pg = (double)tab[(tab_off + g) * ctp + c];
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        const int64_t k0 = goff[g], k1 = goff[g + 1];
+        int64_t k = k0;
+        for (; k + 4 <= k1; k += 4) {
+            const int64_t a0 = grp_seg[k], a1 = grp_seg[k + 1], a2 = grp_seg[k + 2], a3 = grp_seg[k + 3];
+            const double l0 = grp_len[k], l1 = grp_len[k + 1], l2 = grp_len[k + 2], l3 = grp_len[k + 3];
+            s0 += l0 * (pg - (double)mseg[a0 * SP + c]);
+            s1 += l1 * (pg - (double)mseg[a1 * SP + c]);
+            s2 += l2 * (pg - (double)mseg[a2 * SP + c]);
+            s3 += l3 * (pg - (double)mseg[a3 * SP + c]);
         }
+        for (; k < k1; ++k) s0 += (double)grp_len[k] * (pg - (double)mseg[(int64_t)grp_seg[k] * SP + c]);
+        q[e] = (s0 + s1) + (s2 + s3);
+    }
+}
+
+// segment of every run (runs in panel order)
+__global__ void run_segments_kernel(const int64_t* __restrict__ seg_run_off, int64_t GP, int* __restrict__ run_seg) {
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < GP; w += (int64_t)gridDim.x * blockDim.x)
+        for (int64_t r = seg_run_off[w]; r < seg_run_off[w + 1]; ++r) run_seg[r] = (int)w;
+}
+
+// the runs in group order: segment and length of each
+__global__ void group_order_runs_kernel(const int* __restrict__ run_order, const int* __restrict__ run_seg,
+                                        const int* __restrict__ run_len, int64_t R, int* __restrict__ grp_seg,
+                                        int* __restrict__ grp_len) {
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < R; k += (int64_t)gridDim.x * blockDim.x) {
+        const int r = run_order[k];
+        grp_seg[k] = run_seg[r];
+        grp_len[k] = run_len[r] & 0x7fffffff;
     }
 }
 
@@ -2020,8 +2034,8 @@ int run_op_apply(Panel* p, const Layout& L, const ApplyArgs& a, double* q, hipSt
                 (unsigned)((D.n_rtiles + kBlock / kWave - 1) / (kBlock / kWave)),                                  \
                 2u * resident_grid(run_wave_kernel<CT, TSV>, kBlock, 0, (size_t)1 << 30));                         \
             hipLaunchKernelGGL((run_wave_kernel<CT, TSV>), dim3(gw), dim3(kBlock), 0, s, TAB, ctp, d.tab_off,      \
-                               (const int64_t*)D.rtile_rec, D.n_rtiles, (const int*)D.run_gid,                     \
-                               (const int*)D.run_len, OUT, a.active);                                              \
+                               (const int64_t*)D.rtile_rec, (const int64_t*)D.rtile_seg, D.n_rtiles,               \
+                               (const int*)D.run_gid, (const int*)D.run_len, OUT, a.active);                       \
         }                                                                                                          \
         if (D.n_rlong > 0)                                                                                         \
             hipLaunchKernelGGL((run_long_kernel<CT, TSV>), dim3(stream_grid((size_t)D.n_rlong, kBlock)),            \
@@ -2036,28 +2050,27 @@ int run_op_apply(Panel* p, const Layout& L, const ApplyArgs& a, double* q, hipSt
 #undef FEHIP_RUN_OP
     FEHIP_LAUNCH_CHECK("run operator");
     if (timed) {
-        // algorithmic bytes: the runs' group, length and destination, D's table rows once, the run
-        // partials written
+        // algorithmic bytes: the runs' group and length, D's table rows once, the segment means written
         const double es = f32 ? 4.0 : 8.0;
-        const double R = (double)D.n_runs;
-        kstat_end(sid, s, on > 0.0 ? 12.0 * R + es * on * ((double)d.G + R) : 0.0);
+        const double R = (double)D.n_runs, GP = (double)p->d[p->P].G;
+        kstat_end(sid, s, on > 0.0 ? 8.0 * R + es * on * ((double)d.G + GP) : 0.0);
     }
     {
-        // per-group sums of the partials, gathered in group order from their run-order places
+        // A p_D per group from the segment means, the runs of each group in group order
         double* qo = q + d.tab_off * CT;
         const unsigned gr = stream_grid((size_t)d.G * CT, kBlock, 32);
         if (timed) kstat_begin(kStatReduceRuns, s);
         if (f32)
-            hipLaunchKernelGGL((reduce_runs_gather_kernel<CT, float>), dim3(gr), dim3(kBlock), 0, s,
-                               (const float*)a.sd[o].runbuf32, RunRow<CT, float>::kStride, (const int*)D.run_order,
-                               (const int64_t*)D.grp_run_off, d.G, qo, a.active);
+            hipLaunchKernelGGL((reduce_segmeans_kernel<CT, float>), dim3(gr), dim3(kBlock), 0, s, a.p32, ctp,
+                               d.tab_off, (const float*)a.sd[o].runbuf32, (const int*)D.grp_seg,
+                               (const int*)D.grp_len, (const int64_t*)D.grp_run_off, d.G, qo, a.active);
         else
-            hipLaunchKernelGGL((reduce_runs_gather_kernel<CT, double>), dim3(gr), dim3(kBlock), 0, s,
-                               (const double*)a.sd[o].runbuf, RunRow<CT, double>::kStride, (const int*)D.run_order,
-                               (const int64_t*)D.grp_run_off, d.G, qo, a.active);
+            hipLaunchKernelGGL((reduce_segmeans_kernel<CT, double>), dim3(gr), dim3(kBlock), 0, s, a.p, ctp,
+                               d.tab_off, (const double*)a.sd[o].runbuf, (const int*)D.grp_seg,
+                               (const int*)D.grp_len, (const int64_t*)D.grp_run_off, d.G, qo, a.active);
         if (timed)
             kstat_end(kStatReduceRuns, s,
-                      on > 0.0 ? (f32 ? 4.0 : 8.0) * on * (double)D.n_runs + 4.0 * (double)D.n_runs +
+                      on > 0.0 ? 8.0 * (double)D.n_runs + (f32 ? 4.0 : 8.0) * on * ((double)D.n_runs + (double)d.G) +
                                      8.0 * on * (double)d.G + 8.0 * (double)(d.G + 1)
                                : 0.0);
     }
@@ -2245,8 +2258,9 @@ int build_run_tiles(Panel* p, int dd, hipStream_t s) {
     FEHIP_CHECK(hipMemcpyAsync(off.data(), D.seg_run_off, sizeof(int64_t) * (size_t)(GP + 1), hipMemcpyDeviceToHost, s),
                 "run offsets");
     FEHIP_CHECK(hipStreamSynchronize(s), "run offsets");
-    std::vector<int64_t> recs, longs;
+    std::vector<int64_t> recs, segs, longs;
     recs.reserve((size_t)(off[GP] / 48 + 16));
+    segs.reserve((size_t)(off[GP] / 48 + 16));
     int64_t w = 0;
     while (w < GP) {
         const int64_t nr = off[w + 1] - off[w];
@@ -2259,21 +2273,41 @@ int build_run_tiles(Panel* p, int dd, hipStream_t s) {
             ++w;
             continue;
         }
+        // a tile stops before an empty segment, so its segments are consecutive non-empty ones (the kernel
+        // numbers them from the tile's first by their head runs)
         int64_t e = w + 1;
-        while (e < GP && off[e + 1] - off[w] <= kWave) ++e;
+        while (e < GP && off[e + 1] > off[e] && off[e + 1] - off[w] <= kWave) ++e;
         recs.push_back((off[w] & 0xffffffffLL) | ((off[e] - off[w]) << 32));
+        segs.push_back(w);
         w = e;
     }
     int e;
     if ((e = dalloc(&D.rtile_rec, std::max<size_t>(1, recs.size()), &p->bytes))) return e;
+    if ((e = dalloc(&D.rtile_seg, std::max<size_t>(1, segs.size()), &p->bytes))) return e;
     if ((e = dalloc(&D.rlong_segs, std::max<size_t>(1, longs.size()), &p->bytes))) return e;
-    if (!recs.empty())
+    if (!recs.empty()) {
         FEHIP_CHECK(hipMemcpyAsync(D.rtile_rec, recs.data(), sizeof(int64_t) * recs.size(), hipMemcpyHostToDevice, s),
                     "run tiles");
+        FEHIP_CHECK(hipMemcpyAsync(D.rtile_seg, segs.data(), sizeof(int64_t) * segs.size(), hipMemcpyHostToDevice, s),
+                    "run tiles");
+    }
     if (!longs.empty())
         FEHIP_CHECK(hipMemcpyAsync(D.rlong_segs, longs.data(), sizeof(int64_t) * longs.size(), hipMemcpyHostToDevice, s),
                     "long run segments");
+    // the runs in group order: segment and length of each
+    const int64_t R = D.n_runs;
+    int* run_seg = nullptr;
+    if ((e = dalloc(&D.grp_seg, (size_t)std::max<int64_t>(1, R), &p->bytes)) ||
+        (e = dalloc(&D.grp_len, (size_t)std::max<int64_t>(1, R), &p->bytes)) ||
+        (e = dalloc(&run_seg, (size_t)std::max<int64_t>(1, R), nullptr)))
+        return e;
+    hipLaunchKernelGGL(run_segments_kernel, dim3(stream_grid((size_t)GP, kBlock)), dim3(kBlock), 0, s,
+                       (const int64_t*)D.seg_run_off, GP, run_seg);
+    hipLaunchKernelGGL(group_order_runs_kernel, dim3(stream_grid((size_t)R, kBlock)), dim3(kBlock), 0, s,
+                       (const int*)D.run_order, (const int*)run_seg, (const int*)D.run_len, R, D.grp_seg, D.grp_len);
+    FEHIP_LAUNCH_CHECK("group-order runs");
     FEHIP_CHECK(hipStreamSynchronize(s), "run tiles");
+    dfree(run_seg);
     D.n_rlong = (int64_t)longs.size();
     D.n_rtiles = (int64_t)recs.size();
     return 0;
