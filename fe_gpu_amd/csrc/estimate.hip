@@ -686,16 +686,25 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
         a.small_part[(int64_t)blockIdx.x * a.small_lds + e] = stab[e];
 }
 
-// sums of the run partials of each group (contiguous in group order): S[g][c] for c < KT
+// sums of the run partials of each group (contiguous in group order): S[g][c] for c < KT, four loads in
+// flight per thread
 template <int KT>
 __global__ void group_sum_kernel(const double* __restrict__ runbuf, const int64_t* __restrict__ goff, int G,
                                  double* S) {
     const int64_t total = (int64_t)G * KT;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
         const int g = (int)(e / KT), c = (int)(e % KT);
-        double sm = 0.0;
-        for (int64_t k = goff[g]; k < goff[g + 1]; ++k) sm += runbuf[k * KT + c];
-        S[e] = sm;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        const int64_t k0 = goff[g], k1 = goff[g + 1];
+        int64_t k = k0;
+        for (; k + 4 <= k1; k += 4) {
+            s0 += runbuf[k * KT + c];
+            s1 += runbuf[(k + 1) * KT + c];
+            s2 += runbuf[(k + 2) * KT + c];
+            s3 += runbuf[(k + 3) * KT + c];
+        }
+        for (; k < k1; ++k) s0 += runbuf[k * KT + c];
+        S[e] = (s0 + s1) + (s2 + s3);
     }
 }
 
