@@ -741,9 +741,13 @@ def main():
         fe_rec = {"workload": "fixed effects of the headline fit per dim (worker, firm, period), each centred by its "
                               "row-weighted mean, tol " + str(args.tol),
                   "seconds": fe_values_s, "fe_iterations": fest["iterations"], "levels": levels}
+    # FP32 copies of the panel's columns for the FP32-storage forms below (the mixed leg and the server): their
+    # right-hand-side and final passes read half the bytes; the FP64 legs above read the FP64 columns
+    pan.register_columns(d_y.ptr, 1)
+    pan.register_columns(d_W.ptr, 8)
     mixed = None
     if not args.no_mixed:
-        # config 4: the same panel, columns stored in FP32, cross products (and beta) in FP64
+        # config 4: the same panel, columns stored in FP32, cross products in FP64
         kwm = dict(kw, precision=1)
         api.estimate_device(pan, d_y.ptr, d_W.ptr, 8, **kwm)
         ms, mfit = [], None
@@ -765,11 +769,12 @@ def main():
         log(f"mixed precision: {mtotal / args.steps:.3f} s per regression, max beta err {berr.max():.2e}, "
             f"max se err {serr.max():.2e}")
         mixed = {
-            "workload": "same panel and model; FE-space CG on the FP32 operator (FP32 direction tables, run "
-                        "partials and segmented scans) with reliable updates from FP64 true residuals, stopping "
-                        "rule on the FP64 residual; within-transformed columns stored in FP32, their cross "
-                        "products accumulated in FP64 before the rounding; residual and scores from the FP32 "
-                        "columns, sandwich in FP64",
+            "workload": "same panel and model; columns registered with the panel in FP32 (right-hand side and "
+                        "final pass read them); FE-space CG on the FP32 operator (FP32 direction tables, segment "
+                        "means and segmented scans) with reliable updates from FP64 true residuals, stopping rule "
+                        "on the FP64 residual; within-transformed columns stored in FP32, their cross products "
+                        "accumulated in FP64; residual in FP64 and scores in FP32 from the FP32 columns, sandwich "
+                        "in FP64",
             "value": args.steps / mtotal, "unit": "regressions/s", "ms_per_step": mtotal / args.steps * 1e3,
             "vs_fp64": (args.steps / mtotal) / (args.steps / total),
             "beta_rel_err_vs_fp64": [float(v) for v in berr], "se_rel_err_vs_fp64": [float(v) for v in serr],

@@ -69,6 +69,7 @@ COARSE = [
     ("fehip_panel_create", C.c_int, [C.POINTER(VP), C.c_int, I64, C.POINTER(VP)]),
     ("fehip_panel_destroy", C.c_int, [VP]),
     ("fehip_panel_get_info", C.c_int, [VP, C.POINTER(PanelInfo)]),
+    ("fehip_panel_register_columns", C.c_int, [VP, VP, C.c_int, I64]),
     ("fehip_demean", C.c_int, [VP, VP, C.c_int, I64, VP, C.POINTER(DemeanOpts), C.POINTER(DemeanStats)]),
     ("fehip_estimate", C.c_int, [VP, VP, VP, C.c_int, VP, C.c_int, I64, VP, C.POINTER(Opts), C.POINTER(Result), DP,
                                  DP, DP]),

@@ -40,6 +40,12 @@ class Panel:
                     max_segment=inf.max_segment, group_sizes=list(inf.group_sizes)[:m],
                     n_runs=list(inf.n_runs)[:m], device_bytes=inf.device_bytes)
 
+    def register_columns(self, cols_ptr: int, n_cols: int, ld: int = None):
+        """fehip_panel_register_columns: keep FP32 copies of n_cols FP64 device columns (caller row order) for
+        the FP32-storage requests that read them; the columns must not change while registered."""
+        ld = self.n_obs if ld is None else ld
+        check(self.lib.fehip_panel_register_columns(self.handle, cols_ptr, n_cols, ld), "fehip_panel_register_columns")
+
     def close(self):
         if self.handle:
             check(self.lib.fehip_panel_destroy(self.handle), "fehip_panel_destroy")

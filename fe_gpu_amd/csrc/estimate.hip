@@ -2808,6 +2808,7 @@ struct ChunkGroup {
     unsigned mask = 0;
     std::vector<int> members;
     std::vector<const double*> xs;  // distinct input columns
+    bool f32 = true;                // every request of the group stores FP32 (the pass may read FP32 copies)
     SharedRhs rhs{};
     bool has_rhs = false;
     DevBuf buf;
@@ -2866,6 +2867,7 @@ int solve_chunk(Panel* p, std::vector<ReqState>& rs, ChunkRun& ch, const ChunkGr
             seen[key] = c;
             ch.uidx[c] = c;
             wc.x.push_back(x);
+            wc.x32.push_back(s.f32 ? panel_col32(p, x) : nullptr);  // FP32 storage reads a registered FP32 copy
             wc.out.push_back(ch.dst[c].out);
             wc.out32.push_back(ch.dst[c].out32);
             wc.alpha.push_back(ch.dst[c].alpha);
@@ -3071,6 +3073,8 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
             g->xs = xs;
             g->mask = um;
             g->members.push_back((int)ci);
+            for (int i : ch.idx)
+                g->f32 = g->f32 && rs[i].opts->demean.precision == 1 && !(rs[i].opts->n_iv_reg > 0 && rs[i].L > 0);
         }
         if (g->members.size() < 2) continue;
         const int gi = (int)groups.size();
@@ -3082,7 +3086,12 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
         double* b = nullptr;
         if ((e = g->buf.get(&b, (size_t)within_table_size(p, g->mask) * kGramCols + 16))) return e;
         int ld = 0;
-        if ((e = within_rhs(p, g->mask, g->xs.data(), (int)g->xs.size(), b, &ld, 0))) return e;
+        std::vector<const float*> xs32;
+        if (g->f32)
+            for (const double* x : g->xs) xs32.push_back(panel_col32(p, x));
+        if ((e = within_rhs(p, g->mask, g->xs.data(), xs32.empty() ? nullptr : xs32.data(), (int)g->xs.size(), b, &ld,
+                            0)))
+            return e;
         g->rhs.b = b;
         g->rhs.ld = ld;
         if ((e = within_dim_offsets(p, g->mask, g->rhs.dim_off))) return e;

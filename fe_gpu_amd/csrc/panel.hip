@@ -240,6 +240,8 @@ void free_panel(Panel* p) {
         if (q.keys != p->d[p->P].ids) dfree(q.keys);
     }
     p->partitions.clear();
+    for (auto& r : p->reg_cols) dfree(r.copy);
+    p->reg_cols.clear();
     dfree(p->perm);
     dfree(p->seg_off);
     dfree(p->tile_seg);
@@ -570,6 +572,29 @@ int partition_positions(Panel* p, unsigned mask, const int** pos, hipStream_t s)
     return 0;
 }
 
+namespace {
+__global__ void cols_to_f32_kernel(const double* __restrict__ src, int64_t ld, int ncols, int64_t n,
+                                   float* __restrict__ dst) {
+    const int64_t total = (int64_t)ncols * n;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t c = e / n, i = e - c * n;
+        dst[e] = (float)src[c * ld + i];
+    }
+}
+
+}  // namespace
+
+const float* panel_col32(Panel* p, const double* x) {
+    std::lock_guard<std::recursive_mutex> guard(p->mu);
+    for (const auto& r : p->reg_cols) {
+        if (x < r.base) continue;
+        const int64_t off = x - r.base;
+        if (off % r.ld != 0 || off / r.ld >= r.ncols) continue;
+        return r.copy + (off / r.ld) * p->n;
+    }
+    return nullptr;
+}
+
 }  // namespace fehip
 
 using namespace fehip;
@@ -686,6 +711,36 @@ int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehi
     }
     FEHIP_CHECK(locked_device_sync(), "panel build");
     *out = reinterpret_cast<fehip_panel*>(p);
+    return set_ok();
+}
+
+int fehip_panel_register_columns(fehip_panel* panel, const double* d_cols, int n_cols, int64_t ld) {
+    (void)hipGetLastError();
+    if (!panel || !d_cols || n_cols < 1 || ld < 1) return set_error("fehip_panel_register_columns: bad argument");
+    Panel* p = reinterpret_cast<Panel*>(panel);
+    if (ld < p->n) return set_error("fehip_panel_register_columns: leading dim %lld < %lld rows", (long long)ld,
+                                    (long long)p->n);
+    std::lock_guard<std::recursive_mutex> guard(p->mu);
+    // a block registered again (same base and leading dim) is converted again, in place when it fits
+    Panel::RegCols* slot = nullptr;
+    for (auto& r : p->reg_cols)
+        if (r.base == d_cols && r.ld == ld) slot = &r;
+    if (slot && slot->ncols < n_cols) {
+        dfree(slot->copy);
+        p->bytes -= (int64_t)sizeof(float) * slot->ncols * p->n;
+        slot->copy = nullptr;
+    }
+    if (!slot) {
+        p->reg_cols.push_back(Panel::RegCols{d_cols, ld, 0, nullptr});
+        slot = &p->reg_cols.back();
+    }
+    int e;
+    if (!slot->copy && (e = dalloc(&slot->copy, (size_t)n_cols * (size_t)p->n, &p->bytes))) return e;
+    slot->ncols = n_cols;
+    hipLaunchKernelGGL(cols_to_f32_kernel, dim3(stream_grid((size_t)n_cols * (size_t)p->n, 256)), dim3(256), 0, 0,
+                       d_cols, ld, n_cols, p->n, slot->copy);
+    FEHIP_LAUNCH_CHECK("register columns");
+    FEHIP_CHECK(hipStreamSynchronize(0), "register columns");
     return set_ok();
 }
 

@@ -104,8 +104,19 @@ struct Panel {
     // measured time of one operator product per column-tile width (index into the width list), per
     // absorbed-dims mask; filled by the within transform's tuner on first use
     std::map<unsigned, std::vector<float>> width_ms;
+    // FP32 copies of registered FP64 columns (caller row order), used by FP32-storage requests
+    struct RegCols {
+        const double* base;
+        int64_t ld;
+        int ncols;
+        float* copy;  // [ncols][n]
+    };
+    std::vector<RegCols> reg_cols;
     int64_t bytes = 0;
 };
+
+// The FP32 copy of FP64 column x if x lies in a registered block, else nullptr.
+const float* panel_col32(Panel* p, const double* x);
 
 int panel_partition(Panel* p, unsigned dims_mask, const Partition** out, hipStream_t s);
 // The partition's sorted position of every row (nullptr for a partition in panel order), cached.

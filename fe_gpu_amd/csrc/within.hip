@@ -72,6 +72,8 @@ struct ApplyArgs {
     const double* p;           // kIter: [S][CT]
     const int* active;         // kIter: per-column CG activity (nullptr = all active)
     const double* xc[kMaxCT];  // kData/kFinal: input columns (caller row order)
+    const float* xc32[kMaxCT]; // kData/kFinal: their FP32 copies, read instead when in32 (wave tiles only)
+    int in32;
     double* oc[kMaxCT];        // kFinal: output columns (FP64)
     float* oc32[kMaxCT];       // kFinal: FP32 output columns (panel order), used where non-null
     int ncols;                 // valid columns in this tile
@@ -190,18 +192,25 @@ __device__ __forceinline__ double lane_value(double x, int addr) {
 // the L1 cache) and, for kData/kFinal, the input columns, from clamped addresses.  row_combine adds them
 // up.  Between the two the caller issues the next tile's prefetch, so the wait for these loads does not
 // wait for the prefetch.
-template <int CT, int MODE, int ND>
+template <int CT, int MODE, int ND, typename XT = double>
 struct RowLoads {
     double t[(MODE == kIter || MODE == kFinal) ? ND : 1][CT];
-    double x[(MODE == kData || MODE == kFinal) ? CT : 1];
+    XT x[(MODE == kData || MODE == kFinal) ? CT : 1];
 };
 
-template <int CT, int MODE, int ND>
-__device__ __forceinline__ void row_loads(const ApplyArgs& a, int64_t r, const int (&g)[ND], RowLoads<CT, MODE, ND>& L) {
+// XT: the input columns' type (FP32 copies of registered columns, or the FP64 columns)
+template <int CT, int MODE, int ND, typename XT = double>
+__device__ __forceinline__ void row_loads(const ApplyArgs& a, int64_t r, const int (&g)[ND],
+                                          RowLoads<CT, MODE, ND, XT>& L) {
     if constexpr (MODE == kData || MODE == kFinal) {
         const int64_t src = a.perm ? a.perm[r] : r;
+        if constexpr (std::is_same<XT, float>::value) {
 #pragma unroll
-        for (int c = 0; c < CT; ++c) L.x[c] = ld_global((c < a.ncols ? a.xc[c] : a.xc[0]) + src);
+            for (int c = 0; c < CT; ++c) L.x[c] = ld_global((c < a.ncols ? a.xc32[c] : a.xc32[0]) + src);
+        } else {
+#pragma unroll
+            for (int c = 0; c < CT; ++c) L.x[c] = ld_global((c < a.ncols ? a.xc[c] : a.xc[0]) + src);
+        }
     }
     if constexpr (MODE == kIter || MODE == kFinal) {
         const double* tab = MODE == kIter ? a.p : a.alpha;
@@ -215,12 +224,12 @@ __device__ __forceinline__ void row_loads(const ApplyArgs& a, int64_t r, const i
     }
 }
 
-template <int CT, int MODE, int ND>
-__device__ __forceinline__ void row_combine(const ApplyArgs& a, const RowLoads<CT, MODE, ND>& L, double (&v)[CT]) {
+template <int CT, int MODE, int ND, typename XT = double>
+__device__ __forceinline__ void row_combine(const ApplyArgs& a, const RowLoads<CT, MODE, ND, XT>& L, double (&v)[CT]) {
 #pragma unroll
     for (int c = 0; c < CT; ++c) {
         double s = 0.0;
-        if constexpr (MODE == kData || MODE == kFinal) s = c < a.ncols ? L.x[c] : 0.0;
+        if constexpr (MODE == kData || MODE == kFinal) s = c < a.ncols ? (double)L.x[c] : 0.0;
         if constexpr (MODE == kIter) {
             s = L.t[0][c];
 #pragma unroll
@@ -283,7 +292,7 @@ __device__ __forceinline__ int tile_runs(bool valid, bool rhead, int j, int nr, 
 // every lane of its segment by one lane permute, and the weighted row counts come from ballots.
 // Nothing is staged in LDS except the small-dim tables; the waves of a block run independently.
 
-template <int CT, int MODE, int ND>
+template <int CT, int MODE, int ND, typename XT = double>
 __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
     // ND: compile-time bound on the scattered dims (a.nd <= ND), so their loops unroll to what is used
     if constexpr (MODE == kIter) {
@@ -342,8 +351,8 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         }
         // the tile's loads are requested first; the next tiles' record and rows follow, so waiting for the
         // tile's loads does not wait for the prefetch (one in-order memory counter covers both)
-        RowLoads<CT, MODE, ND> ld;
-        row_loads<CT, MODE, ND>(a, r, gid, ld);
+        RowLoads<CT, MODE, ND, XT> ld;
+        row_loads<CT, MODE, ND, XT>(a, r, gid, ld);
         const int64_t rec_nn = load_wave_rec(a, t + 2 * wstride);
         load_wave_rows(a, rec_n, lane, nxt);
         rec_n = rec_nn;
@@ -352,7 +361,7 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         // every tile holds at least one row (the panel drops tiles of empty segments), so there is no
         // early exit here: a path without this tile's stores would make the wait for the prefetch drain
         double v[CT];
-        row_combine<CT, MODE, ND>(a, ld, v);
+        row_combine<CT, MODE, ND, XT>(a, ld, v);
         // rows outside a column's subsample (and, for the data columns, past the tile) count as zero; an
         // operator product without row masks has nothing to clear (its rows past the tile gathered zeros)
         if (MODE != kIter || a.wm) {
@@ -1958,16 +1967,27 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
     } else if (gw > 0) {
         // two rounds of resident workgroups, so that workgroups of the second round fill the places of
         // the first round's early finishers (at most the panel's grid, which sizes the partial buffers)
-        if (a.nd <= 2) {
-            gw = std::min(gw, 2u * resident_grid(wave_tile_kernel<CT, MODE, 2>, kBlock, lt, gw));
-            hipLaunchKernelGGL((wave_tile_kernel<CT, MODE, 2>), dim3(gw), dim3(kBlock), lt, s, a);
-        } else if (a.nd == 3) {
-            gw = std::min(gw, 2u * resident_grid(wave_tile_kernel<CT, MODE, 3>, kBlock, lt, gw));
-            hipLaunchKernelGGL((wave_tile_kernel<CT, MODE, 3>), dim3(gw), dim3(kBlock), lt, s, a);
-        } else {
-            gw = std::min(gw, 2u * resident_grid(wave_tile_kernel<CT, MODE, kMaxScatter>, kBlock, lt, gw));
-            hipLaunchKernelGGL((wave_tile_kernel<CT, MODE, kMaxScatter>), dim3(gw), dim3(kBlock), lt, s, a);
+#define FEHIP_TILE_LAUNCH(NDV, XTV)                                                                               \
+    do {                                                                                                         \
+        gw = std::min(gw, 2u * resident_grid(wave_tile_kernel<CT, MODE, NDV, XTV>, kBlock, lt, gw));             \
+        hipLaunchKernelGGL((wave_tile_kernel<CT, MODE, NDV, XTV>), dim3(gw), dim3(kBlock), lt, s, a);            \
+    } while (0)
+        // the FP32 copies of registered input columns (right-hand side and final pass)
+        bool done = false;
+        if constexpr (MODE == kData || MODE == kFinal) {
+            if (a.in32) {
+                if (a.nd <= 2) FEHIP_TILE_LAUNCH(2, float);
+                else if (a.nd == 3) FEHIP_TILE_LAUNCH(3, float);
+                else FEHIP_TILE_LAUNCH(kMaxScatter, float);
+                done = true;
+            }
         }
+        if (!done) {
+            if (a.nd <= 2) FEHIP_TILE_LAUNCH(2, double);
+            else if (a.nd == 3) FEHIP_TILE_LAUNCH(3, double);
+            else FEHIP_TILE_LAUNCH(kMaxScatter, double);
+        }
+#undef FEHIP_TILE_LAUNCH
     }
     if (gm > 0) {
         ApplyArgs b = a;
@@ -2411,10 +2431,15 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
         a.cbit[c] = in ? (wm ? (1u << slot) : ~0u) : 0u;
         a.anybit |= a.cbit[c];
         a.xc[c] = in ? wc.x[c0 + c] : nullptr;
+        a.xc32[c] = (in && !wc.x32.empty()) ? wc.x32[c0 + c] : nullptr;
         a.oc[c] = in ? wc.out[c0 + c] : nullptr;
         a.oc32[c] = in ? wc.out32[c0 + c] : nullptr;
         cs.slot[c] = slot;
     }
+    // the right-hand side and the final pass read the FP32 copies of registered columns when every column
+    // of the tile has one and every primary segment lies in a wave tile (the block and long kernels read FP64)
+    a.in32 = p->n_mtiles == 0 && p->n_long == 0 && ncols > 0 ? 1 : 0;
+    for (int c = 0; c < ncols; ++c) a.in32 = a.in32 && a.xc32[c] != nullptr;
     a.ncols = ncols;
     a.to_work = to_work;
     double* xa = sc.tab;           // alpha
@@ -2847,23 +2872,28 @@ static int width_for(int n) {
 }
 
 template <int CT>
-static int rhs_cols(Panel* p, const Layout& L, Scratch& sc, const double* const* x, int nx, double* b, hipStream_t s) {
+static int rhs_cols(Panel* p, const Layout& L, Scratch& sc, const double* const* x, const float* const* x32, int nx,
+                    double* b, hipStream_t s) {
     ApplyArgs a = base_args(p, L, sc, nullptr);
     a.rc = nullptr;
     for (int c = 0; c < kMaxCT; ++c) {
         const bool in = c < nx;
         a.cbit[c] = in ? ~0u : 0u;
         a.xc[c] = in ? x[c] : nullptr;
+        a.xc32[c] = (in && x32) ? x32[c] : nullptr;
         a.oc[c] = nullptr;
         a.oc32[c] = nullptr;
     }
     a.anybit = ~0u;
     a.ubit = ~0u;
     a.ncols = nx;
+    a.in32 = x32 && p->n_mtiles == 0 && p->n_long == 0 ? 1 : 0;
+    for (int c = 0; c < nx; ++c) a.in32 = a.in32 && a.xc32[c] != nullptr;
     return apply_and_reduce<CT>(p, L, a, kData, b, s);
 }
 
-int within_rhs(Panel* p, unsigned fe_mask, const double* const* x, int nx, double* b, int* ld, hipStream_t s) {
+int within_rhs(Panel* p, unsigned fe_mask, const double* const* x, const float* const* x32, int nx, double* b, int* ld,
+               hipStream_t s) {
     unsigned mask = fe_mask ? fe_mask : ((1u << p->m) - 1u);
     mask &= (1u << p->m) - 1u;
     Layout L;
@@ -2877,15 +2907,15 @@ int within_rhs(Panel* p, unsigned fe_mask, const double* const* x, int nx, doubl
     Scratch sc;
     if ((e = pass_scratch(p, L, sc))) return e;
     switch (ct) {
-        case 1: e = rhs_cols<1>(p, L, sc, x, nx, b, s); break;
-        case 2: e = rhs_cols<2>(p, L, sc, x, nx, b, s); break;
-        case 3: e = rhs_cols<3>(p, L, sc, x, nx, b, s); break;
-        case 4: e = rhs_cols<4>(p, L, sc, x, nx, b, s); break;
-        case 6: e = rhs_cols<6>(p, L, sc, x, nx, b, s); break;
-        case 8: e = rhs_cols<8>(p, L, sc, x, nx, b, s); break;
-        case 9: e = rhs_cols<9>(p, L, sc, x, nx, b, s); break;
-        case 12: e = rhs_cols<12>(p, L, sc, x, nx, b, s); break;
-        default: e = rhs_cols<16>(p, L, sc, x, nx, b, s); break;
+        case 1: e = rhs_cols<1>(p, L, sc, x, x32, nx, b, s); break;
+        case 2: e = rhs_cols<2>(p, L, sc, x, x32, nx, b, s); break;
+        case 3: e = rhs_cols<3>(p, L, sc, x, x32, nx, b, s); break;
+        case 4: e = rhs_cols<4>(p, L, sc, x, x32, nx, b, s); break;
+        case 6: e = rhs_cols<6>(p, L, sc, x, x32, nx, b, s); break;
+        case 8: e = rhs_cols<8>(p, L, sc, x, x32, nx, b, s); break;
+        case 9: e = rhs_cols<9>(p, L, sc, x, x32, nx, b, s); break;
+        case 12: e = rhs_cols<12>(p, L, sc, x, x32, nx, b, s); break;
+        default: e = rhs_cols<16>(p, L, sc, x, x32, nx, b, s); break;
     }
     if (e) return e;
     FEHIP_CHECK(hipStreamSynchronize(s), "shared right-hand sides");  // the scratch is released on return

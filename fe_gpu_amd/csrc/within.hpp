@@ -51,6 +51,7 @@ struct SharedRhs {
 struct WithinCols {
     explicit WithinCols(int n) : x(n), out(n), out32(n, nullptr), alpha(n, nullptr), slot(n, 0) {}
     std::vector<const double*> x;
+    std::vector<const float*> x32;  // optional (empty, or one per column): FP32 copies of x, or nullptr
     std::vector<double*> out;
     std::vector<float*> out32;
     std::vector<double*> alpha;
@@ -70,8 +71,10 @@ struct WithinCols {
 int within_dim_offsets(Panel* p, unsigned fe_mask, int64_t* off);
 
 // The right-hand sides of nx <= 16 caller-order columns over fe_mask (unweighted), into b ([S][*ld],
-// *ld the column-tile width used, at most 16; b holds S * 16 doubles).
-int within_rhs(Panel* p, unsigned fe_mask, const double* const* x, int nx, double* b, int* ld, hipStream_t s);
+// *ld the column-tile width used, at most 16; b holds S * 16 doubles).  x32 (optional): FP32 copies of the
+// columns, read instead when every column has one.
+int within_rhs(Panel* p, unsigned fe_mask, const double* const* x, const float* const* x32, int nx, double* b, int* ld,
+               hipStream_t s);
 
 
 // Widest column tile of a solve over fe_mask (columns that one tile, hence one set of cross products,

@@ -33,6 +33,14 @@ class DeviceBuffer:
             check(self.lib.fe_gpu_runtime_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), "memcpy_dtoh")
         return out
 
+    def copy_from(self, a: np.ndarray):
+        """Overwrite the buffer's first a.nbytes bytes with a host array (its memory order)."""
+        a = np.ascontiguousarray(a) if a.flags.c_contiguous else np.asfortranarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the buffer")
+        if a.nbytes:
+            check(self.lib.fe_gpu_runtime_memcpy_htod(self.ptr, a.ctypes.data, a.nbytes), "memcpy_htod")
+
     def zero(self):
         check(self.lib.fe_gpu_runtime_memset(self.ptr, 0, self.nbytes), "memset")
 

@@ -45,6 +45,14 @@ int fehip_panel_create(const int* const* d_fe_ids, int n_fe, int64_t n_obs, fehi
 int fehip_panel_destroy(fehip_panel* panel);
 int fehip_panel_get_info(const fehip_panel* panel, fehip_panel_info* info);
 
+/* Keep an FP32 copy of n_cols FP64 device columns (caller row order, leading dim ld >= n_obs) with the
+ * panel.  A request with opts.demean.precision = 1 (FP32 storage) whose columns lie in a registered block
+ * reads the copy in its right-hand-side and final passes, half the bytes of the FP64 columns; its results
+ * then carry the columns' FP32 rounding (about 1e-7 relative).  The caller must not change registered
+ * columns while the panel lives, or must register them again after a change.  No counterpart in the
+ * reference, whose columns are FP64 throughout (src/io/fe_data_io.f90:243-299 widens FP32 files). */
+int fehip_panel_register_columns(fehip_panel* panel, const double* d_cols, int n_cols, int64_t ld);
+
 /* Within-transform options. */
 typedef struct {
     double tol;           /* stop when max |group-mean update| < tol (the reference's criterion) */

@@ -131,3 +131,52 @@ def test_small_or_diagonal_systems_keep_the_fp64_operator():
         f32 = api.estimate(p.y, p.W, p.fe, precision=1, **kw)
         assert f32.res["fe_refinements"] == 0
         assert rel(f32.beta, f64.beta) <= 1e-6 and rel(f32.se, f64.se) <= 1e-5
+
+
+def test_registered_fp32_columns():
+    """FP32-storage requests read the FP32 copies of registered columns in their right-hand-side and final
+    passes (half the bytes): alone and in a batch they stay within the FP32 bars of the FP64 fits (beta 1e-6,
+    SE 1e-5) and of the oracle, differ from the unregistered FP32 fits only by the inputs' FP32 rounding, and a
+    column changed and registered again gives the fit of the new data."""
+    import oracle
+    from fe_gpu_amd import api, synth
+    from fe_gpu_amd.device import DeviceBuffer
+    note("registered FP32 columns")
+    p = synth.worker_firm_panel(seed=43, n_workers=5000, n_periods=12, n_firms=4000)
+    n = p.n
+    K = p.W.shape[1]
+    dy = DeviceBuffer.from_array(p.y)
+    dW = DeviceBuffer.from_array(np.asfortranarray(p.W))
+    plain = api.Panel.from_host(p.fe)
+    pan = api.Panel.from_host(p.fe)
+    pan.register_columns(dy.ptr, 1)
+    pan.register_columns(dW.ptr, K)
+    specs = [(0b111, 0, K, (1, 2, 3)), (0b011, 1, 4, (2,)), (0b110, 0, 3, (2, 3)), (0b101, 2, 5, ()), (0b010, 0, 2, (2,))]
+    reqs = [api.Request(dy.ptr, dW.ptr + 8 * n * j0, k, options=dict(cluster=cl, fe_mask=fm, tol=1e-12, max_iter=20000,
+                                                                     precision=1))
+            for (fm, j0, k, cl) in specs]
+    batch = api.estimate_batch(pan, reqs)
+    moved = 0.0
+    for (fm, j0, k, cl), q, fb in zip(specs, reqs, batch):
+        fa = api.estimate_device(pan, q.y_ptr, q.W_ptr, q.K, **q.options)
+        f64 = api.estimate_device(plain, q.y_ptr, q.W_ptr, q.K, **dict(q.options, precision=0))
+        fu = api.estimate_device(plain, q.y_ptr, q.W_ptr, q.K, **q.options)
+        assert rel(fb.beta, fa.beta) < 1e-10 and rel(fb.se, fa.se) < 1e-10
+        assert rel(fa.beta, f64.beta) <= 1e-6 and rel(fa.se, f64.se) <= 1e-5
+        assert rel(fa.beta, fu.beta) <= 1e-6 and rel(fa.se, fu.se) <= 1e-5
+        moved = max(moved, rel(fa.beta, fu.beta))
+        dims = [d for d in range(3) if fm >> d & 1]
+        ref = oracle.estimate(p.y, p.W[:, j0:j0 + k], p.fe[dims], cluster=tuple(dims.index(c - 1) + 1 for c in cl),
+                              tol=1e-12, max_iter=200000)
+        assert rel(fa.beta, ref.beta) <= 1e-6 and rel(fa.se, ref.se) <= 1e-5
+    assert moved > 1e-12  # the registered copies were read (their rounding moves beta)
+    # change the first regressor on the device, register again: the fit follows the new data
+    W2 = p.W.copy()
+    W2[:, 0] *= 2.0
+    dW.copy_from(np.asfortranarray(W2))
+    pan.register_columns(dW.ptr, K)
+    fa = api.estimate_device(pan, dy.ptr, dW.ptr, K, cluster=(1,), tol=1e-12, max_iter=20000, precision=1)
+    f64 = api.estimate_device(plain, dy.ptr, dW.ptr, K, cluster=(1,), tol=1e-12, max_iter=20000, precision=0)
+    assert rel(fa.beta, f64.beta) <= 1e-6 and rel(fa.se, f64.se) <= 1e-5
+    pan.close()
+    plain.close()
