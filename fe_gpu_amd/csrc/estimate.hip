@@ -3018,6 +3018,8 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
     // chunks: unsolved requests with the same FE set, in submission order, within the slot count and the
     // memory budget (the first request always goes)
     std::vector<std::unique_ptr<ChunkRun>> runs;
+    const char* ms_env = getenv("FEHIP_MULTI_SLOT");
+    const bool multi_slot = ms_env && ms_env[0] == '1';
     {
         std::vector<int> taken(rs.size(), 0);
         for (size_t i = 0; i < rs.size(); ++i) taken[i] = copy_of[i] >= 0;
@@ -3030,6 +3032,11 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
                 if (taken[i] || rs[i].fe_mask != rs[i0].fe_mask || o.tol != lead.tol || o.max_iter != lead.max_iter ||
                     o.check_every != lead.check_every)
                     continue;
+                // requests with different row subsamples share no column, right-hand side or table: one
+                // multi-slot solve would iterate as long as its slowest subsample and run the multi-slot row
+                // kernels, so each subsample takes its own chunk (64 requests of the 10M-row stream: 344 ms
+                // one subsample per chunk, 375 ms shared); FEHIP_MULTI_SLOT=1 shares the sweeps (comparisons)
+                if (rs[i].d_weights != rs[i0].d_weights && !multi_slot) continue;
                 const bool f32 = o.precision == 1 && !(rs[i].opts->n_iv_reg > 0 && rs[i].L > 0);
                 const double b = (f32 ? 4.0 : 8.0) * (double)n * (1 + std::max(0, rs[i].K) + std::max(0, rs[i].L)) +
                                  2.0 * (double)n;

@@ -55,8 +55,14 @@ def _specs(rng, n):
     return out
 
 
-def test_batch_equals_alone_and_oracle():
+@pytest.mark.parametrize("multi_slot", [False, True])
+def test_batch_equals_alone_and_oracle(multi_slot, monkeypatch):
+    """Requests with row subsamples take one chunk per subsample, or with FEHIP_MULTI_SLOT=1 share
+    multi-slot sweeps (per-row mask words, per-column slot bits); either way each equals the same request
+    alone and the oracle."""
     import oracle
+    if multi_slot:
+        monkeypatch.setenv("FEHIP_MULTI_SLOT", "1")
     from fe_gpu_amd import api, synth
     from fe_gpu_amd.device import DeviceBuffer
     note("batch of mixed requests")
@@ -103,9 +109,11 @@ def test_batch_equals_alone_and_oracle():
     pan.close()
 
 
-def test_batch_beyond_32_slots_and_duplicates():
-    """40 weighted requests with one FE set need two sweeps; duplicates give identical outputs."""
+def test_batch_beyond_32_slots_and_duplicates(monkeypatch):
+    """40 weighted requests with one FE set, 20 subsamples each used twice: shared multi-slot sweeps
+    (FEHIP_MULTI_SLOT=1); duplicates give identical outputs."""
     from fe_gpu_amd import api, synth
+    monkeypatch.setenv("FEHIP_MULTI_SLOT", "1")
     from fe_gpu_amd.device import DeviceBuffer
     note("batch of 40 weighted requests")
     p = synth.random_fe_problem(405, n=5000, K=3, G=(1200, 700, 5))
