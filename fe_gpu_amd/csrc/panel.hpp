@@ -59,6 +59,10 @@ struct RunCoupling {
     int* small_rows = nullptr;      // [Gs] table rows of the small groups
     double* B = nullptr;            // [G_D][Gs]
     double* C = nullptr;            // [Gs][Gs]
+    // a solve with a row subsample: its runs' lengths counted over the subsample (run order, with the
+    // segment-head bit, and group order); nullptr: the panel's run lengths
+    const int* run_len = nullptr;
+    const int* grp_len = nullptr;
 };
 
 // Row partition of a cluster subset: rows of cluster c are order[off[c] .. off[c+1]).

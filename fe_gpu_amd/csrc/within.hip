@@ -1210,8 +1210,8 @@ __global__ __launch_bounds__(kBlock) void run_wave_kernel(const TS* __restrict__
             seg_scan_cols_f32<CT + 1>(x, ScanFactorsF(smask));
         else
             seg_scan_cols<CT + 1>(x, ScanFactors(smask));
-        if (valid && lane == se) {  // the segment's last run holds its totals
-            const TS inv = (TS)1 / x[CT];
+        if (valid && lane == se) {  // the segment's last run holds its totals (no rows in a subsample: m = 0)
+            const TS inv = x[CT] > (TS)0 ? (TS)1 / x[CT] : (TS)0;
             TS w[SP];
 #pragma unroll
             for (int c = 0; c < SP; ++c) w[c] = c < CT ? x[c] * inv : (TS)0;
@@ -1245,7 +1245,7 @@ __global__ __launch_bounds__(kBlock) void run_long_kernel(const TS* __restrict__
 #pragma unroll
             for (int c = 0; c < CT; ++c) m[c] += l * (double)row[c];
         }
-        const double inv = 1.0 / nw;
+        const double inv = nw > 0.0 ? 1.0 / nw : 0.0;
         TS v[SP];
 #pragma unroll
         for (int c = 0; c < SP; ++c) v[c] = c < CT ? (TS)(m[c] * inv) : (TS)0;
@@ -1429,6 +1429,25 @@ __global__ __launch_bounds__(kBlock) void coupling_final_kernel(const double* __
         double tt = 0.0;
         for (int w = 0; w < kBlock / kWave; ++w) tt += ws[w];
         q[(int64_t)small_rows[t] * CT + c] = tt;
+    }
+}
+
+// Run lengths over one row slot's subsample: the row that ends each run (row_rdest >= 0) counts the run's rows
+// in the slot, walking back over the run (rows of equal primary and D ids); written in group order (grp_lw)
+// and at the run's place with the segment-head bit of run_len (lw).
+__global__ void weighted_runs_kernel(const int* __restrict__ idP, const int* __restrict__ idD,
+                                     const int* __restrict__ row_rdest, const uint32_t* __restrict__ wm, uint32_t bit,
+                                     const int* __restrict__ run_order, const int* __restrict__ run_len, int64_t n,
+                                     int* __restrict__ grp_lw, int* __restrict__ lw) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int k = row_rdest[i];
+        if (k < 0) continue;
+        const int p0 = idP[i], d0 = idD[i];
+        int c = 0;
+        for (int64_t j = i; j >= 0 && idP[j] == p0 && idD[j] == d0; --j) c += (wm[j] & bit) ? 1 : 0;
+        grp_lw[k] = c;
+        const int r = run_order[k];
+        lw[r] = c | (run_len[r] & (int)0x80000000);
     }
 }
 
@@ -2046,6 +2065,8 @@ int run_op_apply(Panel* p, const Layout& L, const ApplyArgs& a, double* q, hipSt
     }
     const ScatterDim& d = L.sd[o];
     const DimStruct& D = p->d[L.dims[o]];
+    const int* rlen = rc.run_len ? rc.run_len : (const int*)D.run_len;  // a subsample's own run lengths
+    const int* glen = rc.grp_len ? rc.grp_len : (const int*)D.grp_len;
     if (timed) kstat_begin(sid, s);
 #define FEHIP_RUN_OP(TSV, TAB, OUT)                                                                                 \
     do {                                                                                                           \
@@ -2055,13 +2076,12 @@ int run_op_apply(Panel* p, const Layout& L, const ApplyArgs& a, double* q, hipSt
                 2u * resident_grid(run_wave_kernel<CT, TSV>, kBlock, 0, (size_t)1 << 30));                         \
             hipLaunchKernelGGL((run_wave_kernel<CT, TSV>), dim3(gw), dim3(kBlock), 0, s, TAB, ctp, d.tab_off,      \
                                (const int64_t*)D.rtile_rec, (const int64_t*)D.rtile_seg, D.n_rtiles,               \
-                               (const int*)D.run_gid, (const int*)D.run_len, OUT, a.active);                       \
+                               (const int*)D.run_gid, rlen, OUT, a.active);                                        \
         }                                                                                                          \
         if (D.n_rlong > 0)                                                                                         \
             hipLaunchKernelGGL((run_long_kernel<CT, TSV>), dim3(stream_grid((size_t)D.n_rlong, kBlock)),            \
                                dim3(kBlock), 0, s, TAB, ctp, d.tab_off, (const int64_t*)D.rlong_segs, D.n_rlong,   \
-                               (const int64_t*)D.seg_run_off, (const int*)D.run_gid, (const int*)D.run_len,        \
-                               OUT, a.active);                                                                     \
+                               (const int64_t*)D.seg_run_off, (const int*)D.run_gid, rlen, OUT, a.active);         \
     } while (0)
     if (f32)
         FEHIP_RUN_OP(float, a.p32, a.sd[o].runbuf32);
@@ -2082,12 +2102,12 @@ int run_op_apply(Panel* p, const Layout& L, const ApplyArgs& a, double* q, hipSt
         if (timed) kstat_begin(kStatReduceRuns, s);
         if (f32)
             hipLaunchKernelGGL((reduce_segmeans_kernel<CT, float>), dim3(gr), dim3(kBlock), 0, s, a.p32, ctp,
-                               d.tab_off, (const float*)a.sd[o].runbuf32, (const int*)D.grp_seg,
-                               (const int*)D.grp_len, (const int64_t*)D.grp_run_off, d.G, qo, a.active);
+                               d.tab_off, (const float*)a.sd[o].runbuf32, (const int*)D.grp_seg, glen,
+                               (const int64_t*)D.grp_run_off, d.G, qo, a.active);
         else
             hipLaunchKernelGGL((reduce_segmeans_kernel<CT, double>), dim3(gr), dim3(kBlock), 0, s, a.p, ctp,
-                               d.tab_off, (const double*)a.sd[o].runbuf, (const int*)D.grp_seg,
-                               (const int*)D.grp_len, (const int64_t*)D.grp_run_off, d.G, qo, a.active);
+                               d.tab_off, (const double*)a.sd[o].runbuf, (const int*)D.grp_seg, glen,
+                               (const int64_t*)D.grp_run_off, d.G, qo, a.active);
         if (timed)
             kstat_end(kStatReduceRuns, s,
                       on > 0.0 ? 8.0 * (double)D.n_runs + (f32 ? 4.0 : 8.0) * on * ((double)D.n_runs + (double)d.G) +
@@ -2171,6 +2191,7 @@ struct Scratch {
     CgState* st = nullptr;
     CgState* st_host = nullptr;  // per-thread pinned scratch
     const RunCoupling* rc = nullptr;  // the run-level operator's couplings (nullptr: row kernels)
+    RunCoupling own_rc;               // a subsample's couplings and run lengths (built for this solve)
     ~Scratch() {
         for (void* v : owned) cache_free(v);
     }
@@ -2246,14 +2267,19 @@ int slot_diagonal(Panel* p, const Layout& L, Scratch& sc, const uint32_t* wm, ui
     return 0;
 }
 
-// The table-level operators apply to unweighted solves with at most kMaxCouplingGroups small groups and
-// at most one large scattered dim D:
+// The table-level operators apply to solves with at most kMaxCouplingGroups small groups and at most one
+// large scattered dim D:
 //  - the primary absorbed and D present with runs: A from the runs (kind 0);
 //  - the primary not absorbed and D present: A is diagonal (kind 1);
 //  - no large scattered dim: S = C (kind 2).
-// FEHIP_RUN_OPERATOR=0 keeps the row kernels (for comparisons).
-int table_operator_kind(Panel* p, const Layout& L, const uint32_t* wm) {
-    if (wm || L.S <= 0 || L.Gs > kMaxCouplingGroups) return -1;
+// A solve with one row slot (one subsample) takes kind 0, with its run lengths and couplings counted over the
+// subsample and built for the solve: a subsample's couplings cost ceil(Gs / 16) row-operator products at 16
+// columns and are not kept, which kinds 1 and 2 (diagonal or small systems, solved in a few steps) do not
+// repay (c5 panel: 1.9 -> 3.2 ms per region-only request, against 8.3 -> 7.7 ms per {worker, firm, time}
+// request of 18 steps).  Several slots keep the row kernels.  FEHIP_RUN_OPERATOR=0 keeps the row kernels.
+int table_operator_kind(Panel* p, const Layout& L, const uint32_t* wm, int nslots) {
+    if ((wm && nslots != 1) || L.S <= 0 || L.Gs > kMaxCouplingGroups) return -1;
+    if (wm && !L.elim) return -1;
     const char* env = getenv("FEHIP_RUN_OPERATOR");
     if (env && env[0] == '0') return -1;
     int nl = 0, o_large = -1;
@@ -2262,7 +2288,7 @@ int table_operator_kind(Panel* p, const Layout& L, const uint32_t* wm) {
             ++nl;
             o_large = o;
         }
-    if (nl == 0) return L.Gs > 0 ? 2 : -1;
+    if (nl == 0) return L.Gs > 0 && !wm ? 2 : -1;
     if (nl > 1) return -1;
     if (!L.elim) return 1;
     return p->d[L.dims[o_large]].run_gid != nullptr && p->d[L.dims[o_large]].n_runs > 0 ? 0 : -1;
@@ -2333,23 +2359,12 @@ int build_run_tiles(Panel* p, int dd, hipStream_t s) {
     return 0;
 }
 
-// The couplings B, C of the absorbed-dims mask (built on first use, kept with the panel): the row
-// operator applied to unit directions of the small groups, sixteen at a time, gives B's and C's
-// columns.  Runs on the calling thread's stream before any capture.
-int run_couplings(Panel* p, const Layout& L, unsigned mask, int kind, Scratch& sc, hipStream_t s) {
-    std::lock_guard<std::recursive_mutex> guard(p->mu);
-    auto it = p->run_couplings.find(mask);
-    if (it != p->run_couplings.end()) {
-        sc.rc = &it->second;
-        return 0;
-    }
-    RunCoupling rc;
-    rc.kind = kind;
-    for (int o = 0; o < L.nd; ++o)
-        if (!L.sd[o].small && kind == 0) {
-            int e0 = build_run_tiles(p, L.dims[o], s);
-            if (e0) return e0;
-        }
+// The couplings B, C of a table-level operator: the row operator applied to unit directions of the small
+// groups, sixteen at a time, gives B's and C's columns (over slot 0's rows when wm is given).  alloc(ptr,
+// count) provides the device arrays.
+template <typename Alloc>
+int fill_couplings(Panel* p, const Layout& L, RunCoupling& rc, Scratch& sc, const uint32_t* wm, Alloc alloc,
+                   hipStream_t s) {
     std::vector<int> rows;
     for (int o = 0; o < L.nd; ++o) {
         if (!L.sd[o].small) {
@@ -2363,37 +2378,83 @@ int run_couplings(Panel* p, const Layout& L, unsigned mask, int kind, Scratch& s
     const int64_t off_large = rc.o_large >= 0 ? L.sd[rc.o_large].tab_off : 0;
     const int64_t S = L.S;
     int e;
-    if ((e = dalloc(&rc.small_rows, (size_t)std::max(1, rc.Gs), &p->bytes)) ||
-        (e = dalloc(&rc.B, (size_t)std::max<int64_t>(1, (int64_t)G * rc.Gs), &p->bytes)) ||
-        (e = dalloc(&rc.C, (size_t)std::max(1, rc.Gs * rc.Gs), &p->bytes)))
+    if ((e = alloc(&rc.small_rows, (size_t)std::max(1, rc.Gs))) ||
+        (e = alloc(&rc.B, (size_t)std::max<int64_t>(1, (int64_t)G * rc.Gs))) ||
+        (e = alloc(&rc.C, (size_t)std::max(1, rc.Gs * rc.Gs))))
         return e;
-    if (rc.Gs > 0) {
-        FEHIP_CHECK(hipMemcpyAsync(rc.small_rows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, s),
-                    "coupling rows");
-        double* P = sc.tab;            // the CG arrays are not in use yet: [S][16] directions
-        double* Q = sc.tab + S * kMaxCT;  // and their products
-        for (int t0 = 0; t0 < rc.Gs; t0 += kMaxCT) {
-            const int nj = std::min(kMaxCT, rc.Gs - t0);
-            FEHIP_CHECK(hipMemsetAsync(P, 0, sizeof(double) * (size_t)(S * kMaxCT), s), "unit directions");
-            hipLaunchKernelGGL(coupling_units_kernel, dim3(1), dim3(kMaxCT), 0, s, P, kMaxCT,
-                               (const int*)rc.small_rows, t0, nj);
-            ApplyArgs a = base_args(p, L, sc, nullptr);
-            a.rc = nullptr;
-            a.p = P;
-            a.lean = 1;
-            a.ubit = ~0u;
-            a.anybit = ~0u;
-            for (int c = 0; c < kMaxCT; ++c) a.cbit[c] = ~0u;
-            a.ncols = kMaxCT;
-            a.active = nullptr;
-            if ((e = apply_and_reduce<kMaxCT>(p, L, a, kIter, Q, s, nj, kStatCouplings))) return e;
-            hipLaunchKernelGGL(coupling_extract_kernel, dim3(stream_grid((size_t)(G + rc.Gs) * nj, kBlock)),
-                               dim3(kBlock), 0, s, (const double*)Q, kMaxCT, t0, nj, off_large, G,
-                               rc.Gs, (const int*)rc.small_rows, rc.B, rc.C);
-            FEHIP_LAUNCH_CHECK("coupling columns");
-        }
-        FEHIP_CHECK(hipStreamSynchronize(s), "couplings");
+    if (rc.Gs == 0) return 0;
+    FEHIP_CHECK(hipMemcpyAsync(rc.small_rows, rows.data(), sizeof(int) * rows.size(), hipMemcpyHostToDevice, s),
+                "coupling rows");
+    const uint32_t bit = wm ? 1u : ~0u;
+    double* P = sc.tab;               // the CG arrays are not in use yet: [S][16] directions
+    double* Q = sc.tab + S * kMaxCT;  // and their products
+    for (int t0 = 0; t0 < rc.Gs; t0 += kMaxCT) {
+        const int nj = std::min(kMaxCT, rc.Gs - t0);
+        FEHIP_CHECK(hipMemsetAsync(P, 0, sizeof(double) * (size_t)(S * kMaxCT), s), "unit directions");
+        hipLaunchKernelGGL(coupling_units_kernel, dim3(1), dim3(kMaxCT), 0, s, P, kMaxCT, (const int*)rc.small_rows,
+                           t0, nj);
+        ApplyArgs a = base_args(p, L, sc, wm);
+        a.rc = nullptr;
+        a.p = P;
+        a.lean = 1;
+        a.ubit = bit;
+        a.anybit = bit;
+        for (int c = 0; c < kMaxCT; ++c) a.cbit[c] = bit;
+        a.ncols = kMaxCT;
+        a.active = nullptr;
+        if ((e = apply_and_reduce<kMaxCT>(p, L, a, kIter, Q, s, nj, kStatCouplings))) return e;
+        hipLaunchKernelGGL(coupling_extract_kernel, dim3(stream_grid((size_t)(G + rc.Gs) * nj, kBlock)),
+                           dim3(kBlock), 0, s, (const double*)Q, kMaxCT, t0, nj, off_large, G, rc.Gs,
+                           (const int*)rc.small_rows, rc.B, rc.C);
+        FEHIP_LAUNCH_CHECK("coupling columns");
     }
+    return 0;
+}
+
+// The table-level operator's couplings for this solve.  Unweighted: built on first use per absorbed-dims
+// mask and kept with the panel.  One row slot (a subsample): built for the solve, with the large dim's run
+// lengths counted over the subsample (kind 0).  Runs on the calling thread's stream before any capture.
+int run_couplings(Panel* p, const Layout& L, unsigned mask, int kind, Scratch& sc, const uint32_t* wm,
+                  hipStream_t s) {
+    std::lock_guard<std::recursive_mutex> guard(p->mu);
+    int e;
+    if (kind == 0)
+        for (int o = 0; o < L.nd; ++o)
+            if (!L.sd[o].small && (e = build_run_tiles(p, L.dims[o], s))) return e;
+    if (wm) {
+        RunCoupling& rc = sc.own_rc;
+        rc = RunCoupling();
+        rc.kind = kind;
+        if ((e = fill_couplings(p, L, rc, sc, wm, [&](auto** ptr, size_t n) { return sc.alloc(ptr, n); }, s)))
+            return e;
+        if (kind == 0) {
+            const DimStruct& D = p->d[L.dims[rc.o_large]];
+            const int64_t R = D.n_runs;
+            int* lw = nullptr;
+            int* glw = nullptr;
+            if ((e = sc.alloc(&lw, (size_t)std::max<int64_t>(1, R))) || (e = sc.alloc(&glw, (size_t)std::max<int64_t>(1, R))))
+                return e;
+            hipLaunchKernelGGL(weighted_runs_kernel, dim3(stream_grid((size_t)p->n, kBlock)), dim3(kBlock), 0, s,
+                               (const int*)p->d[p->P].ids, (const int*)D.ids, (const int*)D.row_rdest, wm, 1u,
+                               (const int*)D.run_order, (const int*)D.run_len, p->n, glw, lw);
+            FEHIP_LAUNCH_CHECK("subsample run lengths");
+            rc.run_len = lw;
+            rc.grp_len = glw;
+        }
+        sc.rc = &rc;
+        return 0;
+    }
+    auto it = p->run_couplings.find(mask);
+    if (it != p->run_couplings.end()) {
+        sc.rc = &it->second;
+        return 0;
+    }
+    RunCoupling rc;
+    rc.kind = kind;
+    if ((e = fill_couplings(p, L, rc, sc, nullptr, [&](auto** ptr, size_t n) { return dalloc(ptr, n, &p->bytes); },
+                            s)))
+        return e;
+    FEHIP_CHECK(hipStreamSynchronize(s), "couplings");
     auto res = p->run_couplings.emplace(mask, rc);
     sc.rc = &res.first->second;
     return 0;
@@ -3014,8 +3075,8 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
             if (used[b] && (e = slot_diagonal(p, L, sc, wm, 1u << b, sc.dg + (int64_t)b * S * 2, s))) return e;
     }
     // the run-level operator for the CG products where it applies (the tuner below times it)
-    const int op_kind = table_operator_kind(p, L, wm);
-    if (op_kind >= 0 && (e = run_couplings(p, L, mask, op_kind, sc, s))) return e;
+    const int op_kind = table_operator_kind(p, L, wm, nslots);
+    if (op_kind >= 0 && (e = run_couplings(p, L, mask, op_kind, sc, wm, s))) return e;
     // column tiles: the tuner's measured cost per width picks the split of ncols columns; with cross
     // products asked and at most ct_max columns, one tile holds them all
     std::vector<int> tiles;

@@ -5,6 +5,7 @@ Tolerances: integer outputs (group sizes, dof, cluster counts, drop decisions) e
 within 1e-8 relative of the oracle with both sides converged to a tight tolerance (the two AP
 schemes stop at different points at the production tolerance 1e-8).
 """
+import ctypes as C
 import sys
 
 import numpy as np
@@ -541,3 +542,56 @@ def test_table_operator_matches_row_operator_and_oracle(fe_mask, monkeypatch):
     assert fit.res["n_clusters"][:len(cl)] == ref.res["n_clusters"][:len(cl)]
     assert rel(fit.beta, row.beta) < 1e-10 and rel(fit.se, row.se) < 1e-10
     assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
+
+
+@pytest.mark.parametrize("fe_mask", [0b0111, 0b0011, 0b1011, 0b0110, 0b0101, 0b1100])
+def test_subsample_table_operator_matches_row_operator_and_oracle(fe_mask, monkeypatch):
+    """A solve over a row subsample (one weights mask) with the primary dim absorbed takes the run-level
+    products with the runs' lengths in the primary segments counted over the subsample's rows and the
+    couplings B, C built from the masked row operator ({worker, firm}, {worker, firm, period},
+    {worker, firm, region}); the other FE sets keep the row kernels.  The mask drops every row of each 7th
+    worker (primary segments without rows) and of each 11th firm.  Each fit equals the row kernels'
+    (FEHIP_RUN_OPERATOR=0) to 1e-10 and the oracle's on the masked rows to 1e-8; a mixed-precision fit (FP32
+    operator rounds) is within 1e-6 (beta) and 1e-5 (SE) of the FP64 fit."""
+    import oracle
+    from fe_gpu_amd import _lib, api, synth
+    note(f"subsample table operator fe_mask={fe_mask:04b}")
+    lib = _lib.load()
+    p = synth.worker_firm_panel(seed=79, n_workers=3000, n_periods=12, n_firms=1500)
+    fe3 = np.vstack([_redense(p.fe[d]) for d in range(3)])
+    region = (np.arange(fe3[1].max() + 1) % 40 + 1).astype(np.int32)[fe3[1] - 1]
+    fe = np.vstack([fe3, region])
+    rng = np.random.default_rng(7)
+    w = (rng.random(p.n) < 0.6) & (fe[0] % 7 != 0) & (fe[1] % 11 != 0)
+    dims = [d for d in range(4) if fe_mask >> d & 1]
+    cl = tuple(d + 1 for d in dims[:2])
+    kw = dict(cluster=cl, fe_mask=fe_mask, tol=1e-13, max_iter=20000)
+    pan = api.Panel.from_host(fe)
+    lib.fehip_kstats_reset()
+    lib.fehip_kstats_enable(1)
+    try:
+        fit = api.estimate(p.y, p.W, fe, weights=w.astype(np.uint8), panel=pan, **kw)
+    finally:
+        lib.fehip_kstats_enable(0)
+    name = C.create_string_buffer(32)
+    launches = C.c_int64()
+    dd = C.c_double()
+    lib.fehip_kstats_get(8, name, 32, C.byref(launches), C.byref(dd), C.byref(dd), C.byref(dd))
+    if fe_mask in (0b0111, 0b1011):
+        assert launches.value > 0, "the couplings of the run-level operator were not built"
+    else:
+        assert launches.value == 0
+    fit32 = api.estimate(p.y, p.W, fe, weights=w.astype(np.uint8), panel=pan, precision=1, **kw)
+    monkeypatch.setenv("FEHIP_RUN_OPERATOR", "0")
+    row = api.estimate(p.y, p.W, fe, weights=w.astype(np.uint8), panel=pan, **kw)
+    monkeypatch.delenv("FEHIP_RUN_OPERATOR")
+    m = w
+    fe_sub = np.vstack([_redense(fe[d][m]) for d in dims])
+    ref = oracle.estimate(p.y[m], p.W[m], fe_sub, cluster=tuple(dims.index(c - 1) + 1 for c in cl), tol=1e-13,
+                          max_iter=200000)
+    assert fit.res["n_obs"] == int(m.sum())
+    assert fit.res["dof_fes"] == ref.res["dof_fes"] and fit.res["dof_resid"] == ref.res["dof_resid"]
+    assert rel(fit.beta, row.beta) < 1e-10 and rel(fit.se, row.se) < 1e-10
+    assert rel(fit.beta, ref.beta) < 1e-8 and rel(fit.se, ref.se) < 1e-8
+    assert rel(fit32.beta, fit.beta) < 1e-6 and rel(fit32.se, fit.se) < 1e-5
+    pan.close()
