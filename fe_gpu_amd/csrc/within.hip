@@ -76,6 +76,7 @@ struct ApplyArgs {
     int in32;
     double* oc[kMaxCT];        // kFinal: output columns (FP64)
     float* oc32[kMaxCT];       // kFinal: FP32 output columns (panel order), used where non-null
+    int out32;                 // kFinal: FP32 output columns for 1 = every column of the tile, 0 = none, 2 = some
     int ncols;                 // valid columns in this tile
     const int* perm;
     double* small_part;        // [blocks][Gs][CT]
@@ -385,7 +386,19 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
         const unsigned long long segmask = mask_le(se) & ~((1ull << ss) - 1ull);
         if constexpr (MODE == kFinal) {
             double u[CT];
-            {
+            if (a.lean) {
+                // every column in the slot a.ubit (uniform): one row count, and the segment totals of all
+                // columns fetched back to back; without an absorbed primary the mean is 0 (inv = 0)
+                const bool ins = (word & a.ubit) != 0;
+                const int cnt = __popcll(__ballot(ins) & segmask);
+                const double inv = a.elim ? ld_lds(&inv_count[cnt]) : 0.0;
+                const double inslot = ins ? 1.0 : 0.0;
+                double tot[CT];
+#pragma unroll
+                for (int c = 0; c < CT; ++c) tot[c] = lane_value(x[c], se << 2);
+#pragma unroll
+                for (int c = 0; c < CT; ++c) u[c] = fma(-(tot[c] * inv), inslot, v[c]);  // v is 0 outside the slot
+            } else {
                 double inv = 0.0, inslot = 0.0;
                 uint32_t cnt_bit = 0u;
 #pragma unroll
@@ -404,16 +417,30 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
                 }
             }
             if (valid) {
-                const int64_t src = a.perm ? a.perm[r] : r;
+                // unrolled with a static index into u (an early exit from this loop would index u at run time
+                // and serialize the columns' stores behind a scalar load of each column's pointer); the kind of
+                // output is uniform over the tile's columns in the common cases, tested once
+                if (a.out32 == 1) {
 #pragma unroll
-                for (int c = 0; c < CT; ++c) {
-                    if (c >= a.ncols) break;
-                    if (a.oc32[c])
-                        a.oc32[c][r] = (float)u[c];
-                    else if (a.to_work)
-                        a.oc[c][r] = u[c];
-                    else if (word & cb[c])
-                        a.oc[c][src] = u[c];
+                    for (int c = 0; c < CT; ++c)
+                        if (c < a.ncols) a.oc32[c][r] = (float)u[c];
+                } else if (a.out32 == 0 && a.to_work) {
+#pragma unroll
+                    for (int c = 0; c < CT; ++c)
+                        if (c < a.ncols) a.oc[c][r] = u[c];
+                } else {
+                    const int64_t src = a.perm ? a.perm[r] : r;
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) {
+                        if (c < a.ncols) {
+                            if (a.oc32[c])
+                                a.oc32[c][r] = (float)u[c];
+                            else if (a.to_work)
+                                a.oc[c][r] = u[c];
+                            else if (word & cb[c])
+                                a.oc[c][src] = u[c];
+                        }
+                    }
                 }
             }
             if (a.gram)  // uniform; u is 0 on rows past the tile and outside each column's slot
@@ -503,6 +530,134 @@ __global__ __launch_bounds__(kBlock) void wave_tile_kernel(ApplyArgs a) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Lean final pass over the wave tiles, for a column tile whose columns share one row slot and write
+// panel-order outputs of one type (the FP32 storage of a mixed-precision solve, or the FP64 work
+// columns): u = M_P (x - sum_o alpha_o[g_o]) with the cross products u'u, as the final pass of
+// wave_tile_kernel computes them, from a compact argument set.  Column and row addresses are 32-bit
+// byte offsets from per-column bases (the host takes this kernel only where every offset fits), so the
+// loop keeps its bases in scalar registers and shares one offset register between the columns.
+struct FinalArgs {
+    const int64_t* wtile_rec;
+    int64_t n_wtiles;
+    const int* idP;
+    const int* perm;            // caller row of each panel row (nullptr: the identity)
+    const uint32_t* wm;         // panel-order row mask words (nullptr: every row is in the slot)
+    uint32_t ubit;              // the tile's slot bit
+    int elim;                   // 1: the primary dim is absorbed (segment means subtracted)
+    int ncols;
+    const int* ids[3];          // scattered dims: ids (panel order)
+    const double* tab[3];       // and their solved tables, alpha + tab_off * CT
+    const void* x[kMaxCT];      // input columns (caller row order)
+    void* out[kMaxCT];          // output columns (panel order)
+    double* gram;               // per-block 16 x 16 cross-product records (nullptr: none)
+};
+
+template <int ND>
+struct FinalRows {
+    int64_t rec;
+    int ip, src;
+    uint32_t word;
+    int g[ND];
+};
+
+__device__ __forceinline__ int64_t final_rec(const FinalArgs& a, int64_t t) {
+    const int64_t v = a.wtile_rec[t < a.n_wtiles ? t : a.n_wtiles - 1];
+    return t < a.n_wtiles ? v : 0;
+}
+
+// rows of a tile as loaded, from clamped addresses (lanes past the tile read the first row)
+template <int ND>
+__device__ __forceinline__ void final_rows(const FinalArgs& a, int64_t rec, int lane, FinalRows<ND>& w) {
+    w.rec = rec;
+    const uint32_t nr = (uint32_t)(rec >> 32);
+    const uint32_t r = (uint32_t)rec + ((uint32_t)lane < nr ? (uint32_t)lane : 0u);
+    w.ip = a.idP[r];
+    w.src = (a.perm ? a.perm : a.idP)[r];
+    w.word = (a.wm ? a.wm : reinterpret_cast<const uint32_t*>(a.idP))[r];
+#pragma unroll
+    for (int o = 0; o < ND; ++o) w.g[o] = a.ids[o][r];
+}
+
+template <typename T>
+__device__ __forceinline__ const T* at_offset(const void* base, uint32_t bytes) {
+    return reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + bytes);
+}
+
+template <int CT, int ND, typename XT, typename OT>
+__global__ __launch_bounds__(kBlock) void wave_final_kernel(FinalArgs a) {
+    extern __shared__ double gst[];  // per-wave staging of the cross products
+    __shared__ double inv_count[kWave + 1];
+    if (threadIdx.x <= kWave) inv_count[threadIdx.x] = threadIdx.x > 0 ? 1.0 / (double)threadIdx.x : 0.0;
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    gram4_t gacc = {0.0, 0.0, 0.0, 0.0};
+    double* mystage = gst + (size_t)(threadIdx.x / kWave) * gram_stage<CT>();
+    int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    FinalRows<ND> cur, nxt;
+    final_rows<ND>(a, final_rec(a, t), lane, cur);
+    int64_t rec_n = final_rec(a, t + wstride);
+    for (; t < a.n_wtiles; t += wstride) {
+        const int nr = (int)(cur.rec >> 32);
+        const bool valid = lane < nr;
+        const uint32_t r = (uint32_t)cur.rec + (valid ? (uint32_t)lane : 0u);
+        const uint32_t src = a.perm ? (uint32_t)cur.src : r;
+        const bool ins = valid && (!a.wm || (cur.word & a.ubit) != 0);
+        const int ip = valid ? cur.ip : -1;
+        // this tile's loads first, then the next tiles' record and rows (the wait for the first does not
+        // wait for the prefetch)
+        XT xv[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) xv[c] = ld_global(at_offset<XT>(a.x[c < a.ncols ? c : 0], src * (uint32_t)sizeof(XT)));
+        double tv[ND][CT];
+#pragma unroll
+        for (int o = 0; o < ND; ++o) {
+            const double* row = at_offset<double>(a.tab[o], (uint32_t)cur.g[o] * (uint32_t)(CT * sizeof(double)));
+#pragma unroll
+            for (int c = 0; c < CT; ++c) tv[o][c] = ld_global(row + c);
+        }
+        const int64_t rec_nn = final_rec(a, t + 2 * wstride);
+        final_rows<ND>(a, rec_n, lane, nxt);
+        rec_n = rec_nn;
+        cur = nxt;
+        __builtin_amdgcn_sched_barrier(0);
+        double v[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) {
+            double s = (double)xv[c];
+#pragma unroll
+            for (int o = 0; o < ND; ++o) s -= tv[o][c];
+            v[c] = (ins && c < a.ncols) ? s : 0.0;
+        }
+        // the segment [ss, se] of this lane and its mean (the absorbed primary)
+        const int ipp = __shfl_up(ip, 1, kWave);
+        const bool hs = valid && (lane == 0 || ip != ipp);
+        const unsigned long long H = __ballot(hs);
+        const int ss = 63 - __clzll(H & mask_le(lane));
+        const unsigned long long nh = H & ~mask_le(lane);
+        const int se = nh ? __ffsll((long long)nh) - 2 : nr - 1;
+        const unsigned long long segmask = mask_le(se) & ~((1ull << ss) - 1ull);
+        double x[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) x[c] = v[c];
+        if (a.elim) seg_scan_cols<CT>(x, ScanFactors(ScanMask(lane, ss)));
+        const int cnt = __popcll(__ballot(ins) & segmask);
+        const double inv = a.elim ? ld_lds(&inv_count[cnt]) : 0.0;
+        const double inslot = ins ? 1.0 : 0.0;
+        double u[CT];
+#pragma unroll
+        for (int c = 0; c < CT; ++c) u[c] = fma(-(lane_value(x[c], se << 2) * inv), inslot, v[c]);
+        if (valid) {
+#pragma unroll
+            for (int c = 0; c < CT; ++c)
+                if (c < a.ncols)
+                    *const_cast<OT*>(at_offset<OT>(a.out[c], r * (uint32_t)sizeof(OT))) = (OT)u[c];
+        }
+        if (a.gram) gram_wave<CT>(mystage, u, lane, gacc);  // u is 0 past the tile and outside the slot
+    }
+    if (a.gram) gram_store_block<CT>(gst, gram_stage<CT>(), gacc, a.gram + (size_t)blockIdx.x * 256);
+}
+
 // Lean operator product q = S p over the wave tiles, for a column tile whose columns share one row slot
 // (every solve of a single request).  The slot's row counts, the run geometry of each large dim and the
 // segment bounds are formed once per tile rather than once per column, and every column of the tile is
@@ -1965,7 +2120,57 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
     size_t lt = tables_lds<CT>(L);
     if (MODE == kFinal && a.gram) lt += sizeof(double) * (size_t)(kBlock / kWave) * gram_stage<CT>();
     a.part_base = 0;
-    if (MODE == kIter && a.lean && gw > 0) {
+    bool lean_final = false;
+    if constexpr (MODE == kFinal) {
+        // the lean final pass: one slot, panel-order outputs of one type, 1..3 scattered dims, and every
+        // row and table offset within 32 bits
+        const bool in32 = a.in32 != 0;
+        const bool out32 = a.out32 == 1;
+        lean_final = gw > 0 && a.lean && a.nd >= 1 && a.nd <= 3 && (out32 || (a.out32 == 0 && a.to_work)) &&
+                     (out32 || !in32) && (uint64_t)p->n * 8u < (1ull << 32) &&
+                     (uint64_t)L.S * CT * 8u < (1ull << 32);
+        if (lean_final) {
+            FinalArgs f{};
+            f.wtile_rec = a.wtile_rec;
+            f.n_wtiles = a.n_wtiles;
+            f.idP = a.idP;
+            f.perm = a.perm;
+            f.wm = a.wm;
+            f.ubit = a.ubit;
+            f.elim = a.elim;
+            f.ncols = a.ncols;
+            for (int o = 0; o < a.nd; ++o) {
+                f.ids[o] = a.sd[o].ids;
+                f.tab[o] = a.alpha + a.sd[o].tab_off * CT;
+            }
+            for (int c = 0; c < kMaxCT; ++c) {
+                const bool in = c < a.ncols;
+                f.x[c] = in ? (in32 ? (const void*)a.xc32[c] : (const void*)a.xc[c]) : nullptr;
+                f.out[c] = in ? (out32 ? (void*)a.oc32[c] : (void*)a.oc[c]) : nullptr;
+            }
+            f.gram = a.gram;
+            const size_t lf = a.gram ? sizeof(double) * (size_t)(kBlock / kWave) * gram_stage<CT>() : 0;
+#define FEHIP_FINAL_LAUNCH(NDV, XTV, OTV)                                                                         \
+    do {                                                                                                         \
+        gw = std::min(gw, 2u * resident_grid(wave_final_kernel<CT, NDV, XTV, OTV>, kBlock, lf, gw));             \
+        hipLaunchKernelGGL((wave_final_kernel<CT, NDV, XTV, OTV>), dim3(gw), dim3(kBlock), lf, s, f);            \
+    } while (0)
+#define FEHIP_FINAL_TYPES(NDV)                                                                                    \
+    do {                                                                                                         \
+        if (in32) FEHIP_FINAL_LAUNCH(NDV, float, float);                                                         \
+        else if (out32) FEHIP_FINAL_LAUNCH(NDV, double, float);                                                  \
+        else FEHIP_FINAL_LAUNCH(NDV, double, double);                                                            \
+    } while (0)
+            if (a.nd == 1) FEHIP_FINAL_TYPES(1);
+            else if (a.nd == 2) FEHIP_FINAL_TYPES(2);
+            else FEHIP_FINAL_TYPES(3);
+#undef FEHIP_FINAL_TYPES
+#undef FEHIP_FINAL_LAUNCH
+        }
+    }
+    if (lean_final) {
+        // wave tiles done above
+    } else if (MODE == kIter && a.lean && gw > 0) {
         // the lean operator kernel: FP64 or FP32 storage, by the scattered dims at compile time
         const size_t lo = sizeof(double) * (size_t)L.Gs * CT;
 #define FEHIP_OP_LAUNCH(NDV, TSV)                                                                                 \
@@ -2503,6 +2708,9 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
     for (int c = 0; c < ncols; ++c) a.in32 = a.in32 && a.xc32[c] != nullptr;
     a.ncols = ncols;
     a.to_work = to_work;
+    int n32 = 0;
+    for (int c = 0; c < ncols; ++c) n32 += a.oc32[c] != nullptr ? 1 : 0;
+    a.out32 = n32 == 0 ? 0 : (n32 == ncols ? 1 : 2);
     double* xa = sc.tab;           // alpha
     double* r = xa + S * CT;
     double* z = r + S * CT;
