@@ -557,7 +557,7 @@ struct FinalRows {
     int64_t rec;
     int ip, src;
     uint32_t word;
-    int g[ND];
+    int g[ND > 0 ? ND : 1];  // ND = 0: only the primary dim is absorbed
 };
 
 __device__ __forceinline__ int64_t final_rec(const FinalArgs& a, int64_t t) {
@@ -609,7 +609,7 @@ __global__ __launch_bounds__(kBlock) void wave_final_kernel(FinalArgs a) {
         XT xv[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) xv[c] = ld_global(at_offset<XT>(a.x[c < a.ncols ? c : 0], src * (uint32_t)sizeof(XT)));
-        double tv[ND][CT];
+        double tv[ND > 0 ? ND : 1][CT];
 #pragma unroll
         for (int o = 0; o < ND; ++o) {
             const double* row = at_offset<double>(a.tab[o], (uint32_t)cur.g[o] * (uint32_t)(CT * sizeof(double)));
@@ -2122,11 +2122,11 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
     a.part_base = 0;
     bool lean_final = false;
     if constexpr (MODE == kFinal) {
-        // the lean final pass: one slot, panel-order outputs of one type, 1..3 scattered dims, and every
+        // the lean final pass: one slot, panel-order outputs of one type, at most 3 scattered dims, and every
         // row and table offset within 32 bits
         const bool in32 = a.in32 != 0;
         const bool out32 = a.out32 == 1;
-        lean_final = gw > 0 && a.lean && a.nd >= 1 && a.nd <= 3 && (out32 || (a.out32 == 0 && a.to_work)) &&
+        lean_final = gw > 0 && a.lean && a.nd <= 3 && (out32 || (a.out32 == 0 && a.to_work)) &&
                      (out32 || !in32) && (uint64_t)p->n * 8u < (1ull << 32) &&
                      (uint64_t)L.S * CT * 8u < (1ull << 32);
         if (lean_final) {
@@ -2161,7 +2161,8 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
         else if (out32) FEHIP_FINAL_LAUNCH(NDV, double, float);                                                  \
         else FEHIP_FINAL_LAUNCH(NDV, double, double);                                                            \
     } while (0)
-            if (a.nd == 1) FEHIP_FINAL_TYPES(1);
+            if (a.nd == 0) FEHIP_FINAL_TYPES(0);
+            else if (a.nd == 1) FEHIP_FINAL_TYPES(1);
             else if (a.nd == 2) FEHIP_FINAL_TYPES(2);
             else FEHIP_FINAL_TYPES(3);
 #undef FEHIP_FINAL_TYPES
