@@ -1652,34 +1652,43 @@ struct CgState {
     int active[kMaxCT], iters[kMaxCT], reached[kMaxCT];
 };
 
-// FP32 copy of the direction p for the FP32 operator (rows padded to 16 bytes)
+// Flat element loops over a [S][CT] table for the CG vector kernels: thread t of the grid takes elements
+// t, t + T, t + 2T, ... with T the largest multiple of CT not above the grid's thread count, so each thread
+// stays on one column (t % CT) and a wave's loads are consecutive elements (one 512-byte piece per FP64 load
+// instruction, where a thread per table row reads rows CT elements apart).
 template <int CT>
-__device__ __forceinline__ void store_p32(float* p32, int64_t g, int c, double v) {
-    if (p32) p32[g * OpStore<float>::pad(CT) + c] = (float)v;
-}
-
-template <int CT>
-__device__ __forceinline__ void block_rz_max(const double (&rz)[CT], const double (&mx)[CT], double* part_rz,
-                                             double* part_max) {
-    __shared__ double ws[2][kBlock / kWave][CT];
-    const int wv = threadIdx.x / kWave, ln = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-        double a = wave_sum(rz[c]), b = wave_max(mx[c]);
-        if (ln == 0) {
-            ws[0][wv][c] = a;
-            ws[1][wv][c] = b;
-        }
+struct FlatCols {
+    int64_t t, T, n;
+    int c;
+    __device__ __forceinline__ explicit FlatCols(int64_t S) {
+        const int64_t all = (int64_t)gridDim.x * blockDim.x;
+        T = all - all % CT;
+        t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+        c = (int)(t % CT);
+        n = S * CT;
     }
+    __device__ __forceinline__ bool on() const { return t < T; }
+};
+
+// Column partials of a block (sum of a, max of m) at part_a[block][c] and part_m[block][c]: thread i of the
+// block holds column (c0 + i) % CT, with c0 the column of its first thread; threads past T hold zeros.  Every
+// thread of the block calls it; the order of the sums is fixed.
+template <int CT>
+__device__ __forceinline__ void col_partials(double a, double m, double* part_a, double* part_m) {
+    __shared__ double sa[kBlock], sm[kBlock];
+    sa[threadIdx.x] = a;
+    sm[threadIdx.x] = m;
     __syncthreads();
     if (threadIdx.x < CT) {
-        double a = 0.0, b = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) {
-            a += ws[0][w][threadIdx.x];
-            b = fmax(b, ws[1][w][threadIdx.x]);
+        const int c0 = (int)(((int64_t)blockIdx.x * blockDim.x) % CT);
+        const int c = threadIdx.x;
+        double ta = 0.0, tm = 0.0;
+        for (int i = (c - c0 + CT) % CT; i < (int)blockDim.x; i += CT) {
+            ta += sa[i];
+            tm = fmax(tm, sm[i]);
         }
-        part_rz[blockIdx.x * CT + threadIdx.x] = a;
-        part_max[blockIdx.x * CT + threadIdx.x] = b;
+        part_a[blockIdx.x * CT + c] = ta;
+        if (part_m) part_m[blockIdx.x * CT + c] = tm;
     }
 }
 
@@ -1691,35 +1700,35 @@ __global__ __launch_bounds__(kBlock) void cg_init_kernel(int64_t S, double* __re
                                                          const double* __restrict__ minv,
                                                          const double* __restrict__ wcnt, double* __restrict__ z,
                                                          double* __restrict__ p, double* __restrict__ x,
-                                                         double* part_rz, double* part_max, float* p32,
+                                                         double* part_rz, double* part_max, float* __restrict__ p32,
                                                          const double* __restrict__ b, const double* __restrict__ q) {
-    double rz[CT], mx[CT];
-#pragma unroll
-    for (int c = 0; c < CT; ++c) rz[c] = mx[c] = 0.0;
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
-#pragma unroll
-        for (int c = 0; c < CT; ++c) {
-            const int64_t e = g * CT + c;
-            double rv;
-            if (b) {
-                rv = b[e] - q[e];
+    constexpr int CTP = OpStore<float>::pad(CT);
+    const FlatCols<CT> f(S);
+    double rz = 0.0, mx = 0.0;
+    if (f.on()) {
+        if (b) {
+            for (int64_t e = f.t; e < f.n; e += f.T) {
+                const double rv = b[e] - q[e], mv = minv[e], n = wcnt[e];
+                const double zv = mv * rv;
                 r[e] = rv;
-            } else {
-                rv = r[e];
+                z[e] = zv;
+                rz += rv * zv;
+                if (n > 0.0) mx = fmax(mx, fabs(rv) / n);
+            }
+        } else {
+            for (int64_t e = f.t; e < f.n; e += f.T) {
+                const double rv = r[e], mv = minv[e], n = wcnt[e];
+                const double zv = mv * rv;
                 x[e] = 0.0;
-            }
-            const double zv = minv[e] * rv;
-            z[e] = zv;
-            if (!b) {
+                z[e] = zv;
                 p[e] = zv;
-                store_p32<CT>(p32, g, c, zv);
+                if (p32) p32[(e / CT) * CTP + f.c] = (float)zv;
+                rz += rv * zv;
+                if (n > 0.0) mx = fmax(mx, fabs(rv) / n);
             }
-            rz[c] += rv * zv;
-            const double n = wcnt[e];
-            if (n > 0.0) mx[c] = fmax(mx[c], fabs(rv) / n);
         }
     }
-    block_rz_max<CT>(rz, mx, part_rz, part_max);
+    col_partials<CT>(rz, mx, part_rz, part_max);
 }
 
 // Column sums (and maxima) of per-block partials [nblk][CT], by the whole block in a fixed order;
@@ -1814,25 +1823,11 @@ __global__ __launch_bounds__(kBlock) void cg_pq_kernel(int64_t S, const double* 
                                                        const double* __restrict__ q, double* part,
                                                        const int* __restrict__ active) {
     if (!any_active<CT>(active)) return;  // every column converged: alpha is 0 whatever the partials
-    __shared__ double ws[kBlock / kWave][CT];
-    double a[CT];
-#pragma unroll
-    for (int c = 0; c < CT; ++c) a[c] = 0.0;
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x)
-#pragma unroll
-        for (int c = 0; c < CT; ++c) a[c] += p[g * CT + c] * q[g * CT + c];
-    const int wv = threadIdx.x / kWave, ln = threadIdx.x & (kWave - 1);
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-        double t = wave_sum(a[c]);
-        if (ln == 0) ws[wv][c] = t;
-    }
-    __syncthreads();
-    if (threadIdx.x < CT) {
-        double t = 0.0;
-        for (int w = 0; w < kBlock / kWave; ++w) t += ws[w][threadIdx.x];
-        part[blockIdx.x * CT + threadIdx.x] = t;
-    }
+    const FlatCols<CT> f(S);
+    double a = 0.0;
+    if (f.on())
+        for (int64_t e = f.t; e < f.n; e += f.T) a += p[e] * q[e];
+    col_partials<CT>(a, 0.0, part, nullptr);
 }
 
 template <int CT>
@@ -1854,42 +1849,26 @@ __global__ __launch_bounds__(kBlock) void cg_update_kernel(int64_t S, const CgSt
                                                            double* part_rz, double* part_max) {
     // every column converged before this step: x, r and z stay as they are
     if (!any_active<CT>(st->active)) return;
-    double al[CT], rz[CT], mx[CT];
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-        al[c] = st->alpha[c];
-        rz[c] = mx[c] = 0.0;
-    }
-    // the arrays do not overlap, so every load of a table row is issued before its stores
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x) {
-        double rv[CT], xv[CT], pv[CT], qv[CT], mv[CT], nv[CT];
-#pragma unroll
-        for (int c = 0; c < CT; ++c) {
-            const int64_t e = g * CT + c;
-            rv[c] = r[e];
-            mv[c] = minv[e];
-            nv[c] = wcnt[e];
-            if (al[c] != 0.0) {
-                xv[c] = x[e];
-                pv[c] = p[e];
-                qv[c] = q[e];
+    const FlatCols<CT> f(S);
+    double rz = 0.0, mx = 0.0;
+    if (f.on()) {
+        const double al = st->alpha[f.c];
+        for (int64_t e = f.t; e < f.n; e += f.T) {
+            double rv = r[e];
+            const double mv = minv[e], n = wcnt[e];
+            if (al != 0.0) {
+                const double xv = x[e], pv = p[e], qv = q[e];
+                x[e] = fma(al, pv, xv);
+                rv = fma(-al, qv, rv);
+                r[e] = rv;
             }
-        }
-#pragma unroll
-        for (int c = 0; c < CT; ++c) {
-            const int64_t e = g * CT + c;
-            if (al[c] != 0.0) {
-                x[e] = fma(al[c], pv[c], xv[c]);
-                rv[c] = fma(-al[c], qv[c], rv[c]);
-                r[e] = rv[c];
-            }
-            const double zv = mv[c] * rv[c];
+            const double zv = mv * rv;
             z[e] = zv;
-            rz[c] += rv[c] * zv;
-            if (nv[c] > 0.0) mx[c] = fmax(mx[c], fabs(rv[c]) / nv[c]);
+            rz += rv * zv;
+            if (n > 0.0) mx = fmax(mx, fabs(rv) / n);
         }
     }
-    block_rz_max<CT>(rz, mx, part_rz, part_max);
+    col_partials<CT>(rz, mx, part_rz, part_max);
 }
 
 template <int CT>
@@ -1916,24 +1895,122 @@ __global__ __launch_bounds__(kBlock) void cg_beta_scalars(const double* part_rz,
 }
 
 template <int CT>
-__global__ void cg_pdir_kernel(int64_t S, const CgState* __restrict__ st, const double* __restrict__ z, double* p,
-                               float* p32) {
+__global__ void cg_pdir_kernel(int64_t S, const CgState* __restrict__ st, const double* __restrict__ z,
+                               double* __restrict__ p, float* __restrict__ p32) {
     if (!any_active<CT>(st->active)) return;
-    double be[CT];
-    int act[CT];
-#pragma unroll
-    for (int c = 0; c < CT; ++c) {
-        be[c] = st->beta[c];
-        act[c] = st->active[c];
+    constexpr int CTP = OpStore<float>::pad(CT);
+    const FlatCols<CT> f(S);
+    if (!f.on() || !st->active[f.c]) return;
+    const double be = st->beta[f.c];
+    for (int64_t e = f.t; e < f.n; e += f.T) {
+        const double pv = z[e] + be * p[e];
+        p[e] = pv;
+        if (p32) p32[(e / CT) * CTP + f.c] = (float)pv;
     }
-    for (int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; g < S; g += (int64_t)gridDim.x * blockDim.x)
-#pragma unroll
-        for (int c = 0; c < CT; ++c)
-            if (act[c]) {
-                const double pv = z[g * CT + c] + be[c] * p[g * CT + c];
-                p[g * CT + c] = pv;
-                store_p32<CT>(p32, g, c, pv);
+}
+
+// ---- FP32 rounds of the mixed-precision solve ----
+// Within a round of the FP32 operator the direction lives only in its FP32 copy p32 (the operand the operator
+// reads; rows padded as OpStore<float>), the recursive residual in FP32 (r32, [S][CT]) and x in FP64, and
+// z = M^-1 r is formed where it is used instead of being stored.  The tile's columns share one row slot, so
+// the Jacobi diagonal and the group counts are read once per table row from the slot's Schur diagonal
+// (dgs, [S][2]).  A round needs its residual to fall by 1e-4 only (its rounding is below the FP32 operator's),
+// and each round starts from the FP64 true residual.  Bytes per table entry and step: 64 against 116 for the
+// FP64 vectors.
+__device__ __forceinline__ void jacobi_entry(const double* __restrict__ dgs, int64_t g, double& minv, double& n) {
+    const double d = dgs[2 * g];
+    n = dgs[2 * g + 1];
+    minv = (n > 0.0 && d > 1e-9 * n) ? 1.0 / d : 0.0;  // as make_minv_kernel
+}
+
+template <int CT>
+__global__ __launch_bounds__(kBlock) void cg32_pq_kernel(int64_t S, const float* __restrict__ p32,
+                                                         const double* __restrict__ q, double* part,
+                                                         const int* __restrict__ active) {
+    if (!any_active<CT>(active)) return;
+    constexpr int CTP = OpStore<float>::pad(CT);
+    const FlatCols<CT> f(S);
+    double a = 0.0;
+    if (f.on())
+        for (int64_t e = f.t; e < f.n; e += f.T) a += (double)p32[(e / CT) * CTP + f.c] * q[e];
+    col_partials<CT>(a, 0.0, part, nullptr);
+}
+
+// x += alpha p, r -= alpha q (r in FP32, the update formed in FP64), and the partials of r'z and of
+// max |r| / n with z = M^-1 r
+template <int CT>
+__global__ __launch_bounds__(kBlock) void cg32_update_kernel(int64_t S, const CgState* __restrict__ st,
+                                                             const float* __restrict__ p32,
+                                                             const double* __restrict__ q,
+                                                             const double* __restrict__ dgs, double* __restrict__ x,
+                                                             float* __restrict__ r32, double* part_rz,
+                                                             double* part_max) {
+    if (!any_active<CT>(st->active)) return;
+    constexpr int CTP = OpStore<float>::pad(CT);
+    const FlatCols<CT> f(S);
+    double rz = 0.0, mx = 0.0;
+    if (f.on()) {
+        const double al = st->alpha[f.c];
+        for (int64_t e = f.t; e < f.n; e += f.T) {
+            const int64_t g = e / CT;
+            double minv, n;
+            jacobi_entry(dgs, g, minv, n);
+            double rv = (double)r32[e];
+            if (al != 0.0) {
+                const double xv = x[e], pv = (double)p32[g * CTP + f.c], qv = q[e];
+                x[e] = fma(al, pv, xv);
+                const float rn = (float)fma(-al, qv, rv);
+                r32[e] = rn;
+                rv = (double)rn;
             }
+            rz += rv * (minv * rv);
+            if (n > 0.0) mx = fmax(mx, fabs(rv) / n);
+        }
+    }
+    col_partials<CT>(rz, mx, part_rz, part_max);
+}
+
+// p = M^-1 r + beta p on the FP32 copy
+template <int CT>
+__global__ void cg32_pdir_kernel(int64_t S, const CgState* __restrict__ st, const double* __restrict__ dgs,
+                                 const float* __restrict__ r32, float* __restrict__ p32) {
+    if (!any_active<CT>(st->active)) return;
+    constexpr int CTP = OpStore<float>::pad(CT);
+    const FlatCols<CT> f(S);
+    if (!f.on() || !st->active[f.c]) return;
+    const double be = st->beta[f.c];
+    for (int64_t e = f.t; e < f.n; e += f.T) {
+        const int64_t g = e / CT;
+        double minv, n;
+        jacobi_entry(dgs, g, minv, n);
+        float* pe = p32 + g * CTP + f.c;
+        *pe = (float)fma(be, (double)*pe, minv * (double)r32[e]);
+    }
+}
+
+// Start of an FP32 round from the FP64 residual r and z = M^-1 r (the first start, or a reliable update):
+// r32 = r, and the direction p = z + beta p on the FP32 copy (beta 0 at the first start).  to64: the solve
+// leaves the FP32 operator instead, and the direction is written to the FP64 array p.
+template <int CT>
+__global__ void cg32_restart_kernel(int64_t S, const CgState* __restrict__ st, const double* __restrict__ r,
+                                    const double* __restrict__ z, float* __restrict__ r32, float* __restrict__ p32,
+                                    double* __restrict__ p, int to64) {
+    constexpr int CTP = OpStore<float>::pad(CT);
+    const FlatCols<CT> f(S);
+    if (!f.on()) return;
+    const double be = st->beta[f.c];
+    const int act = st->active[f.c];
+    for (int64_t e = f.t; e < f.n; e += f.T) {
+        if (!to64) r32[e] = (float)r[e];
+        if (act) {
+            float* pe = p32 + (e / CT) * CTP + f.c;
+            const double pv = fma(be, (double)*pe, z[e]);
+            if (to64)
+                p[e] = pv;
+            else
+                *pe = (float)pv;
+        }
+    }
 }
 
 // Per-column preconditioner and weighted counts from the per-slot Schur diagonals ([slot][S][2]).
@@ -2389,6 +2466,7 @@ struct Scratch {
     double* runbuf[kMaxLarge] = {nullptr, nullptr, nullptr};
     float* runbuf32[kMaxLarge] = {nullptr, nullptr, nullptr};  // FP32 operator: run partials
     float* p32 = nullptr;       // FP32 operator: the direction's FP32 copy [S][16]
+    float* r32 = nullptr;       // FP32 operator: the recursive residual of a round [S][CT]
     double* bvec = nullptr;     // FP32 operator: the right-hand side [S][CT], kept for true residuals
     double* dump = nullptr;     // lean operator kernel: per-wave slots of the lanes that end no run
     double* small_part = nullptr;
@@ -2768,6 +2846,11 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
                            (const double*)nullptr, (const double*)nullptr);
         hipLaunchKernelGGL((cg_start_scalars<CT>), dim3(1), dim3(kBlock), 0, s, (const double*)sc.part,
                            (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, delta, ncols, 1, sc.st);
+        // the FP32 rounds' Jacobi diagonal and counts: the tile's slot's Schur diagonal
+        const double* dgs = sc.dg + (int64_t)wc.slot[c0] * S * 2;
+        if (op32)
+            hipLaunchKernelGGL((cg32_restart_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, s, S,
+                               (const CgState*)sc.st, (const double*)r, (const double*)z, sc.r32, sc.p32, pp, 0);
         FEHIP_LAUNCH_CHECK("cg init");
         const int every = o.check_every > 0 ? o.check_every : 4;
         a.p = pp;
@@ -2779,6 +2862,22 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
         auto step = [&](hipStream_t st) -> int {
             int e2 = apply_and_reduce<CT>(p, L, a, kIter, q, st, n_on);
             if (e2) return e2;
+            if (a.op32) {
+                // FP32 round: the direction's FP32 copy, the FP32 residual, x in FP64
+                hipLaunchKernelGGL((cg32_pq_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S,
+                                   (const float*)sc.p32, (const double*)q, sc.part, (const int*)sc.st->active);
+                hipLaunchKernelGGL((cg_alpha_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part,
+                                   kCgBlocks, sc.st);
+                hipLaunchKernelGGL((cg32_update_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S,
+                                   (const CgState*)sc.st, (const float*)sc.p32, (const double*)q, dgs, xa, sc.r32,
+                                   sc.part, sc.part + kCgBlocks * CT);
+                hipLaunchKernelGGL((cg_beta_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part,
+                                   (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, sc.st);
+                hipLaunchKernelGGL((cg32_pdir_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, st, S,
+                                   (const CgState*)sc.st, dgs, (const float*)sc.r32, sc.p32);
+                FEHIP_LAUNCH_CHECK("cg step");
+                return 0;
+            }
             hipLaunchKernelGGL((cg_pq_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S, (const double*)pp,
                                (const double*)q, sc.part, (const int*)sc.st->active);
             hipLaunchKernelGGL((cg_alpha_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part, kCgBlocks,
@@ -2900,8 +2999,15 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
                 }
                 hipLaunchKernelGGL((cg_reliable_tol<CT>), dim3(1), dim3(kMaxCT), 0, gs, o.tol, sc.st);
             }
-            hipLaunchKernelGGL((cg_pdir_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, gs, S,
-                               (const CgState*)sc.st, (const double*)z, pp, a.op32 ? sc.p32 : nullptr);
+            // the direction after the update: the FP32 copy for another FP32 round, the FP64 direction from the
+            // FP32 copy when the solve leaves the FP32 operator, and from the FP64 one after an FP64 round
+            if (fp32_round)
+                hipLaunchKernelGGL((cg32_restart_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, gs, S,
+                                   (const CgState*)sc.st, (const double*)r, (const double*)z, sc.r32, sc.p32, pp,
+                                   a.op32 ? 0 : 1);
+            else
+                hipLaunchKernelGGL((cg_pdir_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, gs, S,
+                                   (const CgState*)sc.st, (const double*)z, pp, (float*)nullptr);
             FEHIP_LAUNCH_CHECK("cg reliable update");
             prev_worst = worst;
         }
@@ -3244,6 +3350,7 @@ int within_transform(Panel* p, const WithinCols& wc, int ncols, const uint32_t* 
                 if ((e = sc.alloc(&sc.runbuf32[L.sd[i].large_slot], (size_t)(R * kMaxCT + 4)))) return e;
             }
         if ((e = sc.alloc(&sc.p32, (size_t)(S * kMaxCT + 4)))) return e;
+        if ((e = sc.alloc(&sc.r32, (size_t)(S * kMaxCT + 4)))) return e;
         if ((e = sc.alloc(&sc.bvec, (size_t)(S * kMaxCT + 4)))) return e;
         // padding entries of the FP32 rows are loaded with the row (never used): keep them finite
         FEHIP_CHECK(hipMemsetAsync(sc.p32, 0, sizeof(float) * (size_t)(S * kMaxCT + 4), s), "p32 clear");
