@@ -682,11 +682,16 @@ __device__ __forceinline__ float lane_value(float x, int addr) {
     return __int_as_float(__builtin_amdgcn_ds_bpermute(addr, __float_as_int(x)));
 }
 
-template <int CT, int ND, typename TS>
-__global__ __launch_bounds__(kBlock) void wave_op_kernel(ApplyArgs a) {
+// DATA: the right-hand side b = D_o' W M_P x instead, the rows' values read from the input columns (XT: the
+// FP64 columns or the FP32 copies of registered ones; rows in the caller's order, 32-bit byte offsets) and
+// every other step as for the operator product, in FP64
+template <int CT, int ND, typename TS, bool DATA = false, typename XT = double>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(CT <= 9 && ND <= 2 ? 4 : 1))) void wave_op_kernel(
+    ApplyArgs a) {  // four waves per SIMD where that fits without spills (up to 9 columns, 2 scattered dims)
     constexpr bool kF32 = sizeof(TS) == 4;
+    static_assert(!DATA || !kF32, "the right-hand side is formed in FP64");
     constexpr int CTP = OpStore<TS>::pad(CT);  // table row stride
-    if (!any_active<CT>(a.active)) return;
+    if (!DATA && !any_active<CT>(a.active)) return;
     const TS* tab = kF32 ? reinterpret_cast<const TS*>(a.p32) : reinterpret_cast<const TS*>(a.p);
     const TS* zrow = kF32 ? reinterpret_cast<const TS*>(g_zero_row32) : reinterpret_cast<const TS*>(g_zero_row);
     extern __shared__ double dyn[];
@@ -716,15 +721,29 @@ __global__ __launch_bounds__(kBlock) void wave_op_kernel(ApplyArgs a) {
             gid[o] = (o < a.nd && valid) ? cur.g[o] : -1;
             rdl[o] = cur.rd[o];  // used only on lanes that end a run (valid ones)
         }
-        // the tile's gathers first (rows past the tile and dims past a.nd read the zero row), then the
-        // next tiles' record and rows
-        TS tv[ND][CT];
+        // the tile's gathers (rows past the tile and dims past a.nd read the zero row) or its input columns
+        // (rows past the tile read the first row) first, then the next tiles' record and rows
+        TS tv[DATA ? 1 : ND][CT];
+        XT xv[DATA ? CT : 1];
+        if constexpr (DATA) {
+            const uint32_t r = (uint32_t)cur.rec + (valid ? (uint32_t)lane : 0u);
 #pragma unroll
-        for (int o = 0; o < ND; ++o) {
-            const int64_t off = (a.sd[o].tab_off + (gid[o] >= 0 ? gid[o] : 0)) * CTP;
-            const TS* src = gid[o] >= 0 ? tab + off : zrow;
+            for (int c = 0; c < CT; ++c) {
+                const void* col;
+                if constexpr (sizeof(XT) == 4)
+                    col = a.xc32[c < a.ncols ? c : 0];
+                else
+                    col = a.xc[c < a.ncols ? c : 0];
+                xv[c] = ld_global(at_offset<XT>(col, r * (uint32_t)sizeof(XT)));
+            }
+        } else {
 #pragma unroll
-            for (int c = 0; c < CT; ++c) tv[o][c] = ld_global(src + c);
+            for (int o = 0; o < ND; ++o) {
+                const int64_t off = (a.sd[o].tab_off + (gid[o] >= 0 ? gid[o] : 0)) * CTP;
+                const TS* src = gid[o] >= 0 ? tab + off : zrow;
+#pragma unroll
+                for (int c = 0; c < CT; ++c) tv[o][c] = ld_global(src + c);
+            }
         }
         const int64_t rec_nn = load_wave_rec(a, t + 2 * wstride);
         load_wave_rows(a, rec_n, lane, nxt);
@@ -734,10 +753,14 @@ __global__ __launch_bounds__(kBlock) void wave_op_kernel(ApplyArgs a) {
         TS v[CT];
 #pragma unroll
         for (int c = 0; c < CT; ++c) {
-            TS sum = tv[0][c];
+            if constexpr (DATA) {
+                v[c] = (on && c < a.ncols) ? (TS)xv[c] : (TS)0;  // rows outside the slot count as zero
+            } else {
+                TS sum = tv[0][c];
 #pragma unroll
-            for (int o = 1; o < ND; ++o) sum += tv[o][c];
-            v[c] = on ? sum : (TS)0;  // rows outside the slot count as zero
+                for (int o = 1; o < ND; ++o) sum += tv[o][c];
+                v[c] = on ? sum : (TS)0;  // rows outside the slot count as zero
+            }
         }
         // segment of this lane: [ss, se]
         const int ipp = __shfl_up(ip, 1, kWave);
@@ -2248,6 +2271,24 @@ unsigned launch_pass(Panel* p, const Layout& L, ApplyArgs& a, hipStream_t s) {
     }
     if (lean_final) {
         // wave tiles done above
+    } else if (MODE == kData && a.lean && gw > 0 && !a.perm && (uint64_t)p->n * 8u < (1ull << 32)) {
+        // the right-hand side through the lean kernel: one slot, rows in the caller's order
+        const size_t lo = sizeof(double) * (size_t)L.Gs * CT;
+#define FEHIP_DATA_LAUNCH(NDV, XTV)                                                                               \
+    do {                                                                                                         \
+        gw = std::min(gw, 2u * resident_grid(wave_op_kernel<CT, NDV, double, true, XTV>, kBlock, lo, gw));       \
+        hipLaunchKernelGGL((wave_op_kernel<CT, NDV, double, true, XTV>), dim3(gw), dim3(kBlock), lo, s, a);      \
+    } while (0)
+        if (a.in32) {
+            if (a.nd <= 2) FEHIP_DATA_LAUNCH(2, float);
+            else if (a.nd == 3) FEHIP_DATA_LAUNCH(3, float);
+            else FEHIP_DATA_LAUNCH(kMaxScatter, float);
+        } else {
+            if (a.nd <= 2) FEHIP_DATA_LAUNCH(2, double);
+            else if (a.nd == 3) FEHIP_DATA_LAUNCH(3, double);
+            else FEHIP_DATA_LAUNCH(kMaxScatter, double);
+        }
+#undef FEHIP_DATA_LAUNCH
     } else if (MODE == kIter && a.lean && gw > 0) {
         // the lean operator kernel: FP64 or FP32 storage, by the scattered dims at compile time
         const size_t lo = sizeof(double) * (size_t)L.Gs * CT;
@@ -3263,6 +3304,7 @@ static int rhs_cols(Panel* p, const Layout& L, Scratch& sc, const double* const*
     }
     a.anybit = ~0u;
     a.ubit = ~0u;
+    a.lean = 1;  // one slot: every row of every column
     a.ncols = nx;
     a.in32 = x32 && p->n_mtiles == 0 && p->n_long == 0 ? 1 : 0;
     for (int c = 0; c < nx; ++c) a.in32 = a.in32 && a.xc32[c] != nullptr;
