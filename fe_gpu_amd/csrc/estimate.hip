@@ -398,7 +398,8 @@ struct WaveScores {
     double* rss_part;               // [grid]
     const uint8_t* w;               // panel-order weights (nullptr: all rows)
     const int* spos;                // sorted position of each row in one subset's partition (nullptr: none)
-    double* tsort;                  // the score rows in that order (row-major n x K)
+    void* tsort;                    // the score rows in that order (row-major n x K)
+    int tsort32;                    // 1: those rows in FP32 (the scores of FP32 columns, for the run-cells route)
     int* tkey;                      // with spos: 2 * (the row's cell of the small dims) + (row weighted), or nullptr
     int cstride[3];                 // per slot: the radix of its ids in that cell index (0: not in it)
 };
@@ -528,34 +529,61 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
         cur = nxt;
         __builtin_amdgcn_sched_barrier(0);
         // the score row at its place in the sorted partition of the subset that reads it in that order
+        bool ts32 = false;  // FP32 score rows (uniform)
+        if constexpr (std::is_same<TV, float>::value) ts32 = a.tsort32 != 0;
         if constexpr (KT == 4 || KT == 8) {
             if (a.spos && a.K == KT) {
                 // Full rows go through the wave's staging area, so that one store instruction writes
-                // 64 / (KT / 2) whole rows, one 16-byte piece per lane: the rows of a run land next to each
-                // other in the partition, and such a store covers them with whole cache lines.
-                constexpr int P = KT / 2, RPI = kWave / P;  // pieces per row, rows per instruction
-                double2* st = reinterpret_cast<double2*>(mystage);
+                // 64 / (pieces per row) whole rows, one 16-byte piece per lane: the rows of a run land next to
+                // each other in the partition, and such a store covers them with whole cache lines.
+                if (ts32) {
+                    constexpr int P = KT / 4, RPI = kWave / P;  // pieces per row, rows per instruction
+                    float4* st = reinterpret_cast<float4*>(mystage);
 #pragma unroll
-                for (int k = 0; k < P; ++k) st[lane * P + k] = double2{(double)tv[2 * k], (double)tv[2 * k + 1]};
-                __builtin_amdgcn_wave_barrier();
+                    for (int k = 0; k < P; ++k)
+                        st[lane * P + k] = float4{(float)tv[4 * k], (float)tv[4 * k + 1], (float)tv[4 * k + 2],
+                                                  (float)tv[4 * k + 3]};
+                    __builtin_amdgcn_wave_barrier();
+                    float* ts = static_cast<float*>(a.tsort);
 #pragma unroll
-                for (int i = 0; i < kWave / RPI; ++i) {
-                    const int row = i * RPI + lane / P, piece = lane % P;
-                    const int pos = __shfl((int)spr, row, kWave);
-                    if (row < nr) reinterpret_cast<double2*>(a.tsort + (int64_t)pos * KT)[piece] = st[row * P + piece];
+                    for (int i = 0; i < kWave / RPI; ++i) {
+                        const int row = i * RPI + lane / P, piece = lane % P;
+                        const int pos = __shfl((int)spr, row, kWave);
+                        if (row < nr) reinterpret_cast<float4*>(ts + (int64_t)pos * KT)[piece] = st[row * P + piece];
+                    }
+                    __builtin_amdgcn_wave_barrier();
+                } else {
+                    constexpr int P = KT / 2, RPI = kWave / P;
+                    double2* st = reinterpret_cast<double2*>(mystage);
+#pragma unroll
+                    for (int k = 0; k < P; ++k) st[lane * P + k] = double2{(double)tv[2 * k], (double)tv[2 * k + 1]};
+                    __builtin_amdgcn_wave_barrier();
+                    double* ts = static_cast<double*>(a.tsort);
+#pragma unroll
+                    for (int i = 0; i < kWave / RPI; ++i) {
+                        const int row = i * RPI + lane / P, piece = lane % P;
+                        const int pos = __shfl((int)spr, row, kWave);
+                        if (row < nr) reinterpret_cast<double2*>(ts + (int64_t)pos * KT)[piece] = st[row * P + piece];
+                    }
+                    __builtin_amdgcn_wave_barrier();
                 }
-                __builtin_amdgcn_wave_barrier();
             }
         }
         if (a.spos && valid) {
-            double* dst = a.tsort + spr * a.K;
             if ((KT == 4 || KT == 8) && a.K == KT) {
                 // written above
+            } else if (ts32) {
+                float* dst = static_cast<float*>(a.tsort) + spr * a.K;
+#pragma unroll
+                for (int k = 0; k < KT; ++k)
+                    if (k < a.K) dst[k] = (float)tv[k];
             } else if (a.K == KT) {  // KT is even: 16-byte stores
+                double* dst = static_cast<double*>(a.tsort) + spr * a.K;
 #pragma unroll
                 for (int k = 0; k < KT / 2; ++k)
                     reinterpret_cast<double2*>(dst)[k] = double2{(double)tv[2 * k], (double)tv[2 * k + 1]};
             } else {
+                double* dst = static_cast<double*>(a.tsort) + spr * a.K;
 #pragma unroll
                 for (int k = 0; k < KT; ++k)
                     if (k < a.K) dst[k] = (double)tv[k];
@@ -816,8 +844,8 @@ __global__ __launch_bounds__(kBlock) void sorted_wave_meat_kernel(const double* 
 // matrix-core accumulator and clears the table.
 constexpr int kMaxRunCells = 64;
 
-template <int KT>
-__global__ __launch_bounds__(kBlock) void run_cells_meat_kernel(const double* __restrict__ T, int K,
+template <int KT, typename TT>
+__global__ __launch_bounds__(kBlock) void run_cells_meat_kernel(const TT* __restrict__ T, int K,
                                                                 const int* __restrict__ tkey,
                                                                 const int64_t* __restrict__ off, int64_t G,
                                                                 int ncells, double* part,
@@ -840,17 +868,29 @@ __global__ __launch_bounds__(kBlock) void run_cells_meat_kernel(const double* __
                 const int key = tkey[i];
                 double* row = mytab + min(max(key >> 1, 0), ncells - 1) * (KT + 1);
                 double v[KT];
-                if (K == KT) {  // KT is even and rows start 16-byte aligned
-                    const double2* src = reinterpret_cast<const double2*>(T + i * KT);
+                if (K == KT) {  // rows start 16-byte aligned: FP64 pairs, or FP32 quadruples (KT is 4, 8 or 12)
+                    if constexpr (sizeof(TT) == 4) {
+                        const float4* src = reinterpret_cast<const float4*>(T + i * KT);
 #pragma unroll
-                    for (int k = 0; k < KT / 2; ++k) {
-                        const double2 x = src[k];
-                        v[2 * k] = x.x;
-                        v[2 * k + 1] = x.y;
+                        for (int k = 0; k < KT / 4; ++k) {
+                            const float4 x = src[k];
+                            v[4 * k] = x.x;
+                            v[4 * k + 1] = x.y;
+                            v[4 * k + 2] = x.z;
+                            v[4 * k + 3] = x.w;
+                        }
+                    } else {
+                        const double2* src = reinterpret_cast<const double2*>(T + i * KT);
+#pragma unroll
+                        for (int k = 0; k < KT / 2; ++k) {
+                            const double2 x = src[k];
+                            v[2 * k] = x.x;
+                            v[2 * k + 1] = x.y;
+                        }
                     }
                 } else {
 #pragma unroll
-                    for (int k = 0; k < KT; ++k) v[k] = k < K ? T[i * K + k] : 0.0;
+                    for (int k = 0; k < KT; ++k) v[k] = k < K ? (double)T[i * K + k] : 0.0;
                 }
 #pragma unroll
                 for (int k = 0; k < KT; ++k)
@@ -2334,6 +2374,8 @@ int finish_request(Panel* p, ReqState& s) {
                 if (plan.route[plan.sorted_mask] == kRouteRunCells) {
                     if ((e = db.get(&tkey, (size_t)n))) return e;
                     ws.tkey = tkey;
+                    // the scores of FP32 columns are FP32 values: the run-cells route takes them as such
+                    ws.tsort32 = X32 ? 1 : 0;
                 }
             }
             FEHIP_CHECK(hipMemsetAsync(ucounts, 0, sizeof(unsigned long long) * kMaxNestSubsets, 0), "memset");
@@ -2397,10 +2439,16 @@ int finish_request(Panel* p, ReqState& s) {
                     FEHIP_CHECK(hipMemsetAsync(ucount, 0, sizeof(unsigned long long), 0), "memset");
                     const size_t cneed = (size_t)((pa->n_clusters + kBlock / kWave - 1) / (kBlock / kWave));
                     unsigned cgrid = 1;
-#define FEHIP_RUN_CELLS(KTV)                                                                                      \
-    cgrid = std::min<unsigned>(kMeatBlocks, resident_grid(run_cells_meat_kernel<KTV>, kBlock, 0, cneed));        \
-    hipLaunchKernelGGL(run_cells_meat_kernel<KTV>, dim3(cgrid), dim3(kBlock), 0, 0, (const double*)tsort, K,     \
+#define FEHIP_RUN_CELLS_T(KTV, TTV)                                                                               \
+    cgrid = std::min<unsigned>(kMeatBlocks, resident_grid(run_cells_meat_kernel<KTV, TTV>, kBlock, 0, cneed));   \
+    hipLaunchKernelGGL((run_cells_meat_kernel<KTV, TTV>), dim3(cgrid), dim3(kBlock), 0, 0, (const TTV*)tsort, K, \
                        (const int*)tkey, (const int64_t*)pa->off, pa->n_clusters, plan.cells, mpart, ucount);
+#define FEHIP_RUN_CELLS(KTV)                  \
+    if (ws.tsort32) {                         \
+        FEHIP_RUN_CELLS_T(KTV, float)         \
+    } else {                                  \
+        FEHIP_RUN_CELLS_T(KTV, double)        \
+    }
                     if (KT == 4) {
                         FEHIP_RUN_CELLS(4)
                     } else if (KT == 8) {
@@ -2409,6 +2457,7 @@ int finish_request(Panel* p, ReqState& s) {
                         FEHIP_RUN_CELLS(12)
                     }
 #undef FEHIP_RUN_CELLS
+#undef FEHIP_RUN_CELLS_T
                     FEHIP_LAUNCH_CHECK("run cells meat");
                     std::vector<double> Mc;
                     if ((e = reduce_upper(mpart, (int)cgrid, KT, K, Mc))) return e;
