@@ -1629,6 +1629,37 @@ __global__ void weighted_runs_kernel(const int* __restrict__ idP, const int* __r
     }
 }
 
+// The same counts over the wave tiles (every primary segment in one): a wave per tile, a lane per row; the rows
+// of the slot in a run are a popcount of the slot's ballot over the run's lanes, formed at the run's last lane.
+__global__ __launch_bounds__(kBlock) void weighted_runs_wave_kernel(const int64_t* __restrict__ wtile_rec,
+                                                                    int64_t n_wtiles, const int* __restrict__ row_rdest,
+                                                                    const uint32_t* __restrict__ wm, uint32_t bit,
+                                                                    const int* __restrict__ run_order,
+                                                                    const int* __restrict__ run_len,
+                                                                    int* __restrict__ grp_lw, int* __restrict__ lw) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t wstride = (int64_t)gridDim.x * (kBlock / kWave);
+    for (int64_t t = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; t < n_wtiles; t += wstride) {
+        const int64_t rec = wtile_rec[t];
+        const int nr = (int)(rec >> 32);
+        const bool valid = lane < nr;
+        const int64_t r = (rec & 0xffffffffll) + (valid ? lane : 0);
+        const int k = valid ? row_rdest[r] : -1;  // >= 0 at the last row of a run
+        const bool in = valid && (wm[r] & bit) != 0;
+        const unsigned long long E = __ballot(k >= 0);  // run ends
+        const unsigned long long B = __ballot(in);
+        // the run of this lane starts after the previous run end
+        const unsigned long long before = E & ((1ull << lane) - 1ull);
+        const int rs = before ? 64 - __clzll(before) : 0;
+        if (k >= 0) {
+            const int c = __popcll(B & mask_le(lane) & ~((1ull << rs) - 1ull));
+            grp_lw[k] = c;
+            const int ro = run_order[k];
+            lw[ro] = c | (run_len[ro] & (int)0x80000000);
+        }
+    }
+}
+
 // unit directions of small groups t0 .. t0 + nj - 1: P[small_rows[t0 + j]][j] = 1 (P cleared before)
 __global__ void coupling_units_kernel(double* P, int ct, const int* __restrict__ small_rows, int t0, int nj) {
     const int j = threadIdx.x;
@@ -2530,6 +2561,12 @@ struct Scratch {
 };
 
 constexpr int kCgBlocks = 2048;  // CG vector kernels: eight waves per SIMD keep the table loads in flight
+// blocks of the CG vector kernels for a table of `elems` entries: about eight entries per thread, at most
+// kCgBlocks (a small system's scalar kernels then sum fewer per-block partials)
+inline int cg_blocks(int64_t elems) {
+    const int64_t b = (elems + (int64_t)kBlock * 8 - 1) / ((int64_t)kBlock * 8);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(kCgBlocks, b));
+}
 // A round of the FP32-operator PCG stops once its residual criterion is this fraction of the round's true
 // (FP64) residual: well above the FP32 operator's relative rounding (~1e-6), so each round's FP64
 // residual falls by about this factor.
@@ -2760,9 +2797,15 @@ int run_couplings(Panel* p, const Layout& L, unsigned mask, int kind, Scratch& s
             int* glw = nullptr;
             if ((e = sc.alloc(&lw, (size_t)std::max<int64_t>(1, R))) || (e = sc.alloc(&glw, (size_t)std::max<int64_t>(1, R))))
                 return e;
-            hipLaunchKernelGGL(weighted_runs_kernel, dim3(stream_grid((size_t)p->n, kBlock)), dim3(kBlock), 0, s,
-                               (const int*)p->d[p->P].ids, (const int*)D.ids, (const int*)D.row_rdest, wm, 1u,
-                               (const int*)D.run_order, (const int*)D.run_len, p->n, glw, lw);
+            if (p->n_mtiles == 0 && p->n_long == 0 && p->n_wtiles > 0)  // every segment in a wave tile
+                hipLaunchKernelGGL(weighted_runs_wave_kernel, dim3(stream_grid((size_t)p->n_wtiles, kBlock / kWave)),
+                                   dim3(kBlock), 0, s, (const int64_t*)p->wtile_rec, p->n_wtiles,
+                                   (const int*)D.row_rdest, wm, 1u, (const int*)D.run_order, (const int*)D.run_len,
+                                   glw, lw);
+            else
+                hipLaunchKernelGGL(weighted_runs_kernel, dim3(stream_grid((size_t)p->n, kBlock)), dim3(kBlock), 0, s,
+                                   (const int*)p->d[p->P].ids, (const int*)D.ids, (const int*)D.row_rdest, wm, 1u,
+                                   (const int*)D.run_order, (const int*)D.run_len, p->n, glw, lw);
             FEHIP_LAUNCH_CHECK("subsample run lengths");
             rc.run_len = lw;
             rc.grp_len = glw;
@@ -2809,6 +2852,7 @@ template <int CT>
 int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int c0, int ncols, const uint32_t* wm,
                const fehip_demean_opts& o, int to_work, TileStats& ts, hipStream_t s) {
     const int64_t S = L.S;
+    const int nbcg = cg_blocks(S * CT);  // blocks of the CG vector kernels
     ApplyArgs a = base_args(p, L, sc, wm);
     ColSlots cs{};
     a.anybit = 0;
@@ -2883,11 +2927,11 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
         }
         if (op32) FEHIP_CHECK(hipMemcpyAsync(sc.bvec, r, sizeof(double) * (size_t)(S * CT), hipMemcpyDeviceToDevice, s),
                               "right-hand side copy");
-        hipLaunchKernelGGL((cg_init_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, s, S, r, (const double*)minv,
+        hipLaunchKernelGGL((cg_init_kernel<CT>), dim3(nbcg), dim3(kBlock), 0, s, S, r, (const double*)minv,
                            (const double*)wcnt, z, pp, xa, sc.part, sc.part + kCgBlocks * CT, op32 ? sc.p32 : nullptr,
                            (const double*)nullptr, (const double*)nullptr);
         hipLaunchKernelGGL((cg_start_scalars<CT>), dim3(1), dim3(kBlock), 0, s, (const double*)sc.part,
-                           (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, delta, ncols, 1, sc.st);
+                           (const double*)sc.part + kCgBlocks * CT, nbcg, o.tol, delta, ncols, 1, sc.st);
         // the FP32 rounds' Jacobi diagonal and counts: the tile's slot's Schur diagonal
         const double* dgs = sc.dg + (int64_t)wc.slot[c0] * S * 2;
         if (op32)
@@ -2906,29 +2950,29 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             if (e2) return e2;
             if (a.op32) {
                 // FP32 round: the direction's FP32 copy, the FP32 residual, x in FP64
-                hipLaunchKernelGGL((cg32_pq_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S,
+                hipLaunchKernelGGL((cg32_pq_kernel<CT>), dim3(nbcg), dim3(kBlock), 0, st, S,
                                    (const float*)sc.p32, (const double*)q, sc.part, (const int*)sc.st->active);
                 hipLaunchKernelGGL((cg_alpha_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part,
-                                   kCgBlocks, sc.st);
-                hipLaunchKernelGGL((cg32_update_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S,
+                                   nbcg, sc.st);
+                hipLaunchKernelGGL((cg32_update_kernel<CT>), dim3(nbcg), dim3(kBlock), 0, st, S,
                                    (const CgState*)sc.st, (const float*)sc.p32, (const double*)q, dgs, xa, sc.r32,
                                    sc.part, sc.part + kCgBlocks * CT);
                 hipLaunchKernelGGL((cg_beta_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part,
-                                   (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, sc.st);
+                                   (const double*)sc.part + kCgBlocks * CT, nbcg, o.tol, sc.st);
                 hipLaunchKernelGGL((cg32_pdir_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, st, S,
                                    (const CgState*)sc.st, dgs, (const float*)sc.r32, sc.p32);
                 FEHIP_LAUNCH_CHECK("cg step");
                 return 0;
             }
-            hipLaunchKernelGGL((cg_pq_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S, (const double*)pp,
+            hipLaunchKernelGGL((cg_pq_kernel<CT>), dim3(nbcg), dim3(kBlock), 0, st, S, (const double*)pp,
                                (const double*)q, sc.part, (const int*)sc.st->active);
-            hipLaunchKernelGGL((cg_alpha_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part, kCgBlocks,
+            hipLaunchKernelGGL((cg_alpha_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part, nbcg,
                                sc.st);
-            hipLaunchKernelGGL((cg_update_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, st, S, (const CgState*)sc.st,
+            hipLaunchKernelGGL((cg_update_kernel<CT>), dim3(nbcg), dim3(kBlock), 0, st, S, (const CgState*)sc.st,
                                (const double*)pp, (const double*)q, (const double*)minv, (const double*)wcnt, xa, r, z,
                                sc.part, sc.part + kCgBlocks * CT);
             hipLaunchKernelGGL((cg_beta_scalars<CT>), dim3(1), dim3(kBlock), 0, st, (const double*)sc.part,
-                               (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, sc.st);
+                               (const double*)sc.part + kCgBlocks * CT, nbcg, o.tol, sc.st);
             hipLaunchKernelGGL((cg_pdir_kernel<CT>), dim3(stream_grid(S, kBlock)), dim3(kBlock), 0, st, S,
                                (const CgState*)sc.st, (const double*)z, pp, a.op32 ? sc.p32 : nullptr);
             FEHIP_LAUNCH_CHECK("cg step");
@@ -3013,14 +3057,14 @@ int solve_tile(Panel* p, const Layout& L, Scratch& sc, const WithinCols& wc, int
             if ((e = apply_and_reduce<CT>(p, L, ar, kIter, q, gs, -1, kStatApplyResid))) return e;
             ++refines;
             const bool fp32_round = a.op32 != 0;
-            hipLaunchKernelGGL((cg_init_kernel<CT>), dim3(kCgBlocks), dim3(kBlock), 0, gs, S, r, (const double*)minv,
+            hipLaunchKernelGGL((cg_init_kernel<CT>), dim3(nbcg), dim3(kBlock), 0, gs, S, r, (const double*)minv,
                                (const double*)wcnt, z, pp, xa, sc.part, sc.part + kCgBlocks * CT, (float*)nullptr,
                                (const double*)sc.bvec, (const double*)q);
             FEHIP_LAUNCH_CHECK("cg reliable update");
             // the new direction waits for the decision below: a round that did not halve the true residual
             // goes on with the FP64 operator (the FP32 operator's rounding is in the way), launched directly
             hipLaunchKernelGGL((cg_reliable_scalars<CT>), dim3(1), dim3(kBlock), 0, gs, (const double*)sc.part,
-                               (const double*)sc.part + kCgBlocks * CT, kCgBlocks, o.tol, fp32_round ? delta : 0.0,
+                               (const double*)sc.part + kCgBlocks * CT, nbcg, o.tol, fp32_round ? delta : 0.0,
                                ncols, sc.st);
             FEHIP_CHECK(hipMemcpyAsync(sc.st_host, sc.st, sizeof(CgState), hipMemcpyDeviceToHost, gs), "cg state");
             FEHIP_CHECK(hipStreamSynchronize(gs), "cg sync");
