@@ -251,6 +251,7 @@ void free_panel(Panel* p) {
     for (auto& kv : p->run_couplings) {
         dfree(kv.second.small_rows);
         dfree(kv.second.B);
+        dfree(kv.second.B32);
         dfree(kv.second.C);
     }
     p->run_couplings.clear();

@@ -58,6 +58,7 @@ struct RunCoupling {
     int Gs = 0;
     int* small_rows = nullptr;      // [Gs] table rows of the small groups
     double* B = nullptr;            // [G_D][Gs]
+    float* B32 = nullptr;           // B in FP32, for the FP32 operator's products
     double* C = nullptr;            // [Gs][Gs]
     // a solve with a row subsample: its runs' lengths counted over the subsample (run order, with the
     // segment-head bit, and group order); nullptr: the panel's run lengths

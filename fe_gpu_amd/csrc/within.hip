@@ -1488,7 +1488,7 @@ typedef double double4_t __attribute__((ext_vector_type(4)));
 // reduction in B' p_D, kept across tiles; the block's four quarters are summed at the end into
 // part[blk][t][c].
 template <int CT, typename TS>
-__global__ __launch_bounds__(kBlock) void coupling_pass_kernel(const double* __restrict__ B, int Gs, int G,
+__global__ __launch_bounds__(kBlock) void coupling_pass_kernel(const TS* __restrict__ B, int Gs, int G,
                                                                int64_t tab_off, const TS* __restrict__ tab, int ctp,
                                                                const int* __restrict__ small_rows, double* q,
                                                                double* part, const int* __restrict__ active) {
@@ -1517,7 +1517,7 @@ __global__ __launch_bounds__(kBlock) void coupling_pass_kernel(const double* __r
 #pragma unroll
         for (int k = 0; k < kBPer; ++k) {
             const int e = threadIdx.x + kBlock * k;
-            const double v = B[g0 * Gs + (e < nb ? e : 0)];
+            const double v = (double)B[g0 * Gs + (e < nb ? e : 0)];
             rb[k] = e < nb ? v : 0.0;
         }
 #pragma unroll
@@ -1658,6 +1658,12 @@ __global__ __launch_bounds__(kBlock) void weighted_runs_wave_kernel(const int64_
             lw[ro] = c | (run_len[ro] & (int)0x80000000);
         }
     }
+}
+
+// out = (float) in, n entries
+__global__ void to_f32_kernel(const double* __restrict__ in, int64_t n, float* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (float)in[i];
 }
 
 // unit directions of small groups t0 .. t0 + nj - 1: P[small_rows[t0 + j]][j] = 1 (P cleared before)
@@ -2475,18 +2481,18 @@ int run_op_apply(Panel* p, const Layout& L, const ApplyArgs& a, double* q, hipSt
         // 2048 blocks (eight waves per SIMD) keep more of B's tiles in flight (c2: 1024 blocks 166 us, 2048 141 us)
         const unsigned nblk = std::max(
             1u, std::min<unsigned>(std::min(2048u, apply_grid(p)), (unsigned)((G + kCouplingTile - 1) / kCouplingTile)));
-#define FEHIP_COUPLING(TSV, TAB)                                                                                   \
+#define FEHIP_COUPLING(TSV, TAB, BV)                                                                               \
     do {                                                                                                           \
-        hipLaunchKernelGGL((coupling_pass_kernel<CT, TSV>), dim3(nblk), dim3(kBlock), 0, s, (const double*)rc.B,   \
+        hipLaunchKernelGGL((coupling_pass_kernel<CT, TSV>), dim3(nblk), dim3(kBlock), 0, s, BV,                    \
                            rc.Gs, G, d.tab_off, TAB, ctp, (const int*)rc.small_rows, q, a.small_part, a.active);   \
         hipLaunchKernelGGL((coupling_final_kernel<CT, TSV>), dim3((unsigned)(rc.Gs * CT)), dim3(kBlock), 0, s,     \
                            (const double*)a.small_part, (int)nblk, (const double*)rc.C, rc.Gs, TAB, ctp,           \
                            (const int*)rc.small_rows, q, a.active);                                                \
     } while (0)
         if (f32)
-            FEHIP_COUPLING(float, a.p32);
+            FEHIP_COUPLING(float, a.p32, (const float*)rc.B32);  // B in FP32 for the FP32 operator
         else
-            FEHIP_COUPLING(double, a.p);
+            FEHIP_COUPLING(double, a.p, (const double*)rc.B);
 #undef FEHIP_COUPLING
         FEHIP_LAUNCH_CHECK("couplings");
     }
@@ -2743,6 +2749,7 @@ int fill_couplings(Panel* p, const Layout& L, RunCoupling& rc, Scratch& sc, cons
     int e;
     if ((e = alloc(&rc.small_rows, (size_t)std::max(1, rc.Gs))) ||
         (e = alloc(&rc.B, (size_t)std::max<int64_t>(1, (int64_t)G * rc.Gs))) ||
+        (e = alloc(&rc.B32, (size_t)std::max<int64_t>(1, (int64_t)G * rc.Gs))) ||
         (e = alloc(&rc.C, (size_t)std::max(1, rc.Gs * rc.Gs))))
         return e;
     if (rc.Gs == 0) return 0;
@@ -2770,6 +2777,11 @@ int fill_couplings(Panel* p, const Layout& L, RunCoupling& rc, Scratch& sc, cons
                            dim3(kBlock), 0, s, (const double*)Q, kMaxCT, t0, nj, off_large, G, rc.Gs,
                            (const int*)rc.small_rows, rc.B, rc.C);
         FEHIP_LAUNCH_CHECK("coupling columns");
+    }
+    if (G > 0) {
+        hipLaunchKernelGGL(to_f32_kernel, dim3(stream_grid((size_t)G * rc.Gs, kBlock)), dim3(kBlock), 0, s,
+                           (const double*)rc.B, (int64_t)G * rc.Gs, rc.B32);
+        FEHIP_LAUNCH_CHECK("coupling copy");
     }
     return 0;
 }
