@@ -689,10 +689,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(KT <= 8 
             const bool on = leader && ne;
             const unsigned long long BU = __ballot(on);
             if (lane == 0) nclusters[q] += (unsigned long long)__popcll(BU);
-            // sum of S S' over the clusters that end here: the cross products of the masked sums
-#pragma unroll
-            for (int k = 0; k < KT; ++k) sv[k] = on ? sv[k] : (TV)0;
-            gram_wave<KT>(mystage, sv, lane, acc[q]);
+            // sum of S S' over the clusters that end here: the cross products of their sums, packed (a subset
+            // of singletons keeps every row)
+            gram_wave_sparse<KT>(mystage, sv, on, lane, acc[q]);
         }
     }
     constexpr int NP = KT * (KT + 1) / 2;

@@ -54,6 +54,37 @@ __device__ __forceinline__ void gram_wave(double* st, const T (&v)[CT], int lane
     __builtin_amdgcn_wave_barrier();
 }
 
+// gram_wave over the lanes where `on` holds only (the others' rows are zero): those rows are packed to the front
+// of the staging area, the rows after them up to a whole instruction are cleared, and the products take
+// ceil(rows / (4 * row groups)) instructions instead of 16 / row groups.  A subset whose clusters end at a few
+// lanes of the tile (one per segment or run) then costs one or two instructions.
+template <int CT, typename T>
+__device__ __forceinline__ void gram_wave_sparse(double* st, const T (&v)[CT], bool on, int lane, gram4_t& acc) {
+    constexpr int B = gram_block<CT>(), G = 16 / B, R = 4 * G;  // block width, row groups, rows per instruction
+    const unsigned long long m = __ballot(on);
+    const int n = __popcll(m);
+    if (n == 0) return;  // uniform across the wave
+    const int nr = (n + R - 1) / R * R;
+    if (on) {
+        const int pos = __popcll(m & ((1ull << lane) - 1ull));
+#pragma unroll
+        for (int c = 0; c < CT; ++c) st[pos * CT + c] = (double)v[c];
+    }
+    if (lane < nr - n) {
+#pragma unroll
+        for (int c = 0; c < CT; ++c) st[(n + lane) * CT + c] = 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int col = lane & 15, rr = lane >> 4;
+    const int g = col / B, c = col - g * B;
+    for (int k = 0; k < nr / R; ++k) {
+        const int row = R * k + 4 * g + rr;
+        const double x = c < CT ? st[row * CT + c] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(x, x, acc, 0, 0, 0);
+    }
+    __builtin_amdgcn_wave_barrier();
+}
+
 // The block's waves add their accumulators in a fixed order and the diagonal blocks are folded; the
 // CT x CT result is left in stage_all[i * 16 + j] (i, j < CT) and visible to every thread.  stage_all
 // holds blockDim.x / 64 records of `stage` doubles.
