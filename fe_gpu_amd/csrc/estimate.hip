@@ -811,10 +811,7 @@ __global__ __launch_bounds__(kBlock) void sorted_wave_meat_kernel(const double* 
         const bool nexthead = lane < last ? ((H >> (lane + 1)) & 1ull) != 0 : !cont;
         const bool cend = valid && (lane <= last) && nexthead;
         const bool complete = cend && le != 0;  // started at a head of this wave
-        double cs[KT];
-#pragma unroll
-        for (int k = 0; k < KT; ++k) cs[k] = complete ? sv[k] : 0.0;
-        gram_wave<KT>(mystage, cs, lane, acc);
+        gram_wave_sparse<KT>(mystage, sv, complete, lane, acc);  // the clusters that end here, packed
         // boundary pieces
         const bool open_left = (H & 1ull) == 0ull;
         const int e0 = H ? __ffsll((long long)H) - 2 : last;  // last lane of the leading piece
@@ -913,7 +910,7 @@ __global__ __launch_bounds__(kBlock) void run_cells_meat_kernel(const TT* __rest
         }
         const unsigned long long BU = __ballot(on);
         if (lane == 0) ncl += (unsigned long long)__popcll(BU);
-        gram_wave<KT>(mystage, sv, lane, acc);
+        gram_wave_sparse<KT>(mystage, sv, on, lane, acc);  // the cells with a weighted row, packed
     }
     if (lane == 0 && ncl) atomicAdd(n_clusters, ncl);
     gram_store_upper<KT>(gst, gram_stage<KT>(), acc, part + (int64_t)blockIdx.x * (KT * (KT + 1) / 2));
