@@ -80,6 +80,7 @@ std::mutex& capture_mutex();
 // Whether the calling thread replays the CG step from a captured HIP graph (default) or launches it
 // directly; threads that run concurrently with other device work switch capture off.
 void set_thread_capture(bool on);
+bool thread_capture();  // the calling thread's setting
 hipError_t locked_free(void* p);        // hipFree under capture_mutex
 hipError_t locked_host_free(void* p);   // hipHostFree under capture_mutex
 hipError_t locked_device_sync();        // hipDeviceSynchronize under capture_mutex

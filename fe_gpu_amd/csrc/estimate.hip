@@ -17,6 +17,8 @@
 #include <map>
 #include <memory>
 #include <set>
+#include <string>
+#include <thread>
 #include <tuple>
 #include <type_traits>
 #include <vector>
@@ -3013,6 +3015,7 @@ int finish_chunk(Panel* p, std::vector<ReqState>& rs, ChunkRun& ch) {
     }
     for (int i : ch.idx) {
         if ((e = finish_request(p, rs[i]))) return e;
+        FEHIP_CHECK(hipStreamSynchronize(0), "request done");  // its kernels have read its buffers
         rs[i].db.clear();  // release the request's device buffers before the next chunk
     }
     ch.cb.clear();
@@ -3149,12 +3152,49 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
         if ((e = within_dim_offsets(p, g->mask, g->rhs.dim_off))) return e;
         g->has_rhs = true;
     }
-    // solve and finish every chunk in turn (its buffers are released before the next chunk)
+    // Solve the chunks in turn; each chunk's finish (normal equations, residuals, sandwiches) runs on a helper
+    // thread, on that thread's default stream, while the next chunk is solved, so the host work between the
+    // sandwiches' kernels and the solves' small kernels overlap on the device.  A chunk's solve has completed on
+    // the device when it returns (its cross products are read back), and a finish releases its requests' buffers
+    // only after its own stream has drained.  While the helper runs, the solves launch their CG steps directly:
+    // the runtime refuses blocking calls from any thread while a graph is being captured.
+    const bool capture = thread_capture();
+    if (runs.size() > 1) set_thread_capture(false);
+    int device = 0;
+    (void)hipGetDevice(&device);
+    std::thread finisher;
+    int fin_e = 0;
+    std::string fin_msg;
+    auto join_finisher = [&]() {
+        if (finisher.joinable()) finisher.join();
+        if (fin_e) set_error("%s", fin_msg.c_str());
+        return fin_e;
+    };
     for (auto& chp : runs) {
         ChunkRun& ch = *chp;
-        if ((e = solve_chunk(p, rs, ch, ch.group >= 0 ? groups[ch.group].get() : nullptr, shared))) return e;
-        if ((e = finish_chunk(p, rs, ch))) return e;
+        e = solve_chunk(p, rs, ch, ch.group >= 0 ? groups[ch.group].get() : nullptr, shared);
+        if (join_finisher() || e) {
+            set_thread_capture(capture);
+            return e ? e : fin_e;
+        }
+        if (runs.size() == 1) {
+            if ((e = finish_chunk(p, rs, ch))) return e;
+            continue;
+        }
+        ChunkRun* chq = chp.get();
+        finisher = std::thread([&, chq]() {
+            (void)hipSetDevice(device);
+            set_thread_capture(false);
+            fin_e = finish_chunk(p, rs, *chq);
+            if (fin_e) fin_msg = last_error();
+            release_workspaces();
+        });
     }
+    if (join_finisher()) {
+        set_thread_capture(capture);
+        return fin_e;
+    }
+    set_thread_capture(capture);
     for (size_t i = 0; i < rs.size(); ++i) {
         if (copy_of[i] < 0) continue;
         const ReqState& src = rs[copy_of[i]];

@@ -3263,6 +3263,7 @@ int plan_tiles(Panel* p, const Layout& L, Scratch& sc, unsigned mask, int ncols,
 }  // namespace
 
 void set_thread_capture(bool on) { t_capture = on; }
+bool thread_capture() { return t_capture; }
 
 int build_mask_words(Panel* p, const uint8_t* const* w_caller, int nslots, uint32_t* wm, hipStream_t s) {
     if (nslots < 1 || nslots > 32) return set_error("1..32 request slots per batch (%d)", nslots);
