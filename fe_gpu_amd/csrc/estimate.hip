@@ -18,6 +18,9 @@
 #include <memory>
 #include <set>
 #include <string>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
 #include <thread>
 #include <tuple>
 #include <type_traits>
@@ -3152,49 +3155,78 @@ int run_batch(Panel* p, std::vector<ReqState>& rs) {
         if ((e = within_dim_offsets(p, g->mask, g->rhs.dim_off))) return e;
         g->has_rhs = true;
     }
-    // Solve the chunks in turn; each chunk's finish (normal equations, residuals, sandwiches) runs on a helper
+    // Solve the chunks in turn; each chunk's finish (normal equations, residuals, sandwiches) runs on one helper
     // thread, on that thread's default stream, while the next chunk is solved, so the host work between the
     // sandwiches' kernels and the solves' small kernels overlap on the device.  A chunk's solve has completed on
     // the device when it returns (its cross products are read back), and a finish releases its requests' buffers
-    // only after its own stream has drained.  While the helper runs, the solves launch their CG steps directly:
-    // the runtime refuses blocking calls from any thread while a graph is being captured.
-    const bool capture = thread_capture();
-    if (runs.size() > 1) set_thread_capture(false);
-    int device = 0;
-    (void)hipGetDevice(&device);
-    std::thread finisher;
-    int fin_e = 0;
-    std::string fin_msg;
-    auto join_finisher = [&]() {
-        if (finisher.joinable()) finisher.join();
-        if (fin_e) set_error("%s", fin_msg.c_str());
-        return fin_e;
-    };
-    for (auto& chp : runs) {
-        ChunkRun& ch = *chp;
-        e = solve_chunk(p, rs, ch, ch.group >= 0 ? groups[ch.group].get() : nullptr, shared);
-        if (join_finisher() || e) {
-            set_thread_capture(capture);
-            return e ? e : fin_e;
-        }
-        if (runs.size() == 1) {
-            if ((e = finish_chunk(p, rs, ch))) return e;
-            continue;
-        }
-        ChunkRun* chq = chp.get();
-        finisher = std::thread([&, chq]() {
+    // only after its own stream has drained; the solver waits when the helper is a chunk behind, which bounds
+    // the buffers held.  While the helper runs, the solves launch their CG steps directly: the runtime refuses
+    // blocking calls from any thread while a graph is being captured.
+    if (runs.size() == 1) {
+        ChunkRun& ch = *runs[0];
+        if ((e = solve_chunk(p, rs, ch, ch.group >= 0 ? groups[ch.group].get() : nullptr, shared))) return e;
+        if ((e = finish_chunk(p, rs, ch))) return e;
+    } else {
+        const bool capture = thread_capture();
+        set_thread_capture(false);
+        int device = 0;
+        (void)hipGetDevice(&device);
+        std::mutex qm;
+        std::condition_variable qcv;
+        std::deque<ChunkRun*> queue;
+        int pending = 0;  // chunks handed over and not finished
+        bool closed = false;
+        int fin_e = 0;
+        std::string fin_msg;
+        std::thread helper([&]() {
             (void)hipSetDevice(device);
             set_thread_capture(false);
-            fin_e = finish_chunk(p, rs, *chq);
-            if (fin_e) fin_msg = last_error();
+            for (;;) {
+                ChunkRun* c = nullptr;
+                {
+                    std::unique_lock<std::mutex> lk(qm);
+                    qcv.wait(lk, [&] { return !queue.empty() || closed; });
+                    if (queue.empty()) break;
+                    c = queue.front();
+                    queue.pop_front();
+                }
+                int fe = 0;
+                std::string msg;
+                if (!fin_e) {
+                    fe = finish_chunk(p, rs, *c);
+                    if (fe) msg = last_error();
+                }
+                std::lock_guard<std::mutex> lk(qm);
+                if (fe && !fin_e) {
+                    fin_e = fe;
+                    fin_msg = msg;
+                }
+                --pending;
+                qcv.notify_all();
+            }
             release_workspaces();
         });
-    }
-    if (join_finisher()) {
+        for (auto& chp : runs) {
+            ChunkRun& ch = *chp;
+            e = solve_chunk(p, rs, ch, ch.group >= 0 ? groups[ch.group].get() : nullptr, shared);
+            std::unique_lock<std::mutex> lk(qm);
+            if (e || fin_e) break;
+            qcv.wait(lk, [&] { return pending == 0 || fin_e != 0; });  // at most one chunk being finished
+            if (fin_e) break;
+            queue.push_back(chp.get());
+            ++pending;
+            qcv.notify_all();
+        }
+        {
+            std::lock_guard<std::mutex> lk(qm);
+            closed = true;
+            qcv.notify_all();
+        }
+        helper.join();
         set_thread_capture(capture);
-        return fin_e;
+        if (e) return e;
+        if (fin_e) return set_error("%s", fin_msg.c_str());
     }
-    set_thread_capture(capture);
     for (size_t i = 0; i < rs.size(); ++i) {
         if (copy_of[i] < 0) continue;
         const ReqState& src = rs[copy_of[i]];
